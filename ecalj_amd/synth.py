@@ -1,0 +1,320 @@
+"""Seeded synthetic inputs for the QSGW chi0 -> W -> Sigma_c path.
+
+ecalj obtains these inputs from its DFT/GW front end (eigenvectors cphi,
+radial integrals ppb, Coulomb matrix v(q), tetrahedron weights).  Here they
+are drawn from fixed distributions so that every run, on any machine, sees
+the same numbers:
+
+* energies: occupied U[-12, 0] eV, unoccupied U[0.5, 60] eV, sorted, E_F = 0;
+* eigenvectors: complex Gaussian nb x nbt, QR-orthonormalised;
+* ppb: real Gaussian, shape nb x nb x nI (real, as S/m_zmel.f90:38), with
+  variance 1/(nI*Omega), Omega = alat^3/4 the fcc cell volume.  This makes
+  sum_I |zmel(I,n,m)|^2 ~ 1/Omega, the value for cell-normalised product
+  basis and Bloch functions; the literal variance 1/nb gives nI/nb instead
+  (600x larger at nI=600), which puts epsilon at ~1e4 and Sigma_c at ~1e4 eV;
+* v(q) = Q diag(4 pi/|q+G|^2) Q^H with Q unitary (QR of a complex Gaussian),
+  G the first nI integer-lattice vectors sorted by shell, q = (0.1,0.2,0.3) 2pi/a;
+* histogram weights: each (occ n at k, unocc m at k+q) pair spreads pi*w_k
+  uniformly over the window [0.98 D, 1.02 D], D = e_m - e_n in Hartree,
+  split over the histogram bins it overlaps.  This stands in for the
+  tetrahedron weights whw (S/tetwt5.f90:335, which carry the same pi/vol
+  normalisation); the bins are ecalj's own mesh (S/m_freq.f90:95-108).
+
+Draw order is fixed: energies -> eigenvectors -> ppb -> v(q), from
+numpy PCG64(seed).  Arrays are Fortran-ordered complex128 / float64.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RYDBERG_EV = 13.605693122994
+HARTREE_EV = 2.0 * RYDBERG_EV
+
+
+@dataclass(frozen=True)
+class Config:
+    name: str
+    nk: int          # internal k-points (nqbz)
+    nb: int          # eigenvector basis size (ndima)
+    no: int          # occupied bands
+    nu: int          # unoccupied bands
+    ni: int          # mixed product basis size (ngb)
+    nf: int          # imaginary frequencies (niw), Gauss-Legendre supported sizes only
+    seed: int
+    alat: float = 10.68
+    ntq: int = 0     # bands whose Sigma is computed; 0 -> 2*no
+    esmr: float = 0.003   # Ry
+    ua: float = 1.0
+    nspin: int = 1
+    hist_ratio: float = 1.03
+    hist_dw: float = 1e-5
+    window: float = 0.02
+    note: str = ""
+
+    @property
+    def nbt(self) -> int:
+        return self.no + self.nu
+
+    @property
+    def nq(self) -> int:
+        return self.ntq if self.ntq else 2 * self.no
+
+    def scaled(self, **kw) -> "Config":
+        return dataclasses.replace(self, **kw)
+
+
+# The workloads named by the metric, plus small shapes for parity tests.
+CONFIGS = {
+    # Si-like, CPU-sized
+    "C1": Config("C1", nk=8, nb=100, no=4, nu=60, ni=200, nf=12, seed=20261203, alat=10.26),
+    # GaAs-like, FP64 GPU baseline form
+    "C2": Config("C2", nk=64, nb=180, no=9, nu=150, ni=600, nf=16, seed=20261204),
+    # Mixed precision; nb raised from 300 to 320 so that QR can give nbt = 318
+    # orthonormal eigenvectors (300 < 318 is inconsistent).
+    "C3": Config("C3", nk=64, nb=320, no=18, nu=300, ni=1200, nf=20, seed=20261205,
+                 note="nb=320 (spec says 300 < nbt=318)"),
+    # parity-test shapes
+    "T1": Config("T1", nk=2, nb=12, no=2, nu=5, ni=16, nf=6, seed=11),
+    "T2": Config("T2", nk=3, nb=24, no=3, nu=13, ni=40, nf=10, seed=12),
+    "T3": Config("T3", nk=4, nb=40, no=4, nu=30, ni=72, nf=12, seed=13),
+}
+
+
+def _qr_unitary(rng, n, m):
+    a = rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+    q, r = np.linalg.qr(a)
+    # fix the phase so the draw is deterministic across LAPACK builds
+    d = np.diag(r)
+    q = q * (d / np.abs(d))[None, :].conj()
+    return np.asfortranarray(q)
+
+
+def _energies(rng, no, nu):
+    eo = np.sort(rng.uniform(-12.0, 0.0, no))
+    eu = np.sort(rng.uniform(0.5, 60.0, nu))
+    return np.concatenate([eo, eu])
+
+
+def gvectors(ni: int) -> np.ndarray:
+    """First ni integer vectors of a cubic lattice, sorted by (|n|^2, n)."""
+    r = 1
+    while (2 * r + 1) ** 3 < 2 * ni + 64:
+        r += 1
+    rng = np.arange(-r - 1, r + 2)
+    g = np.array(np.meshgrid(rng, rng, rng, indexing="ij")).reshape(3, -1).T
+    key = (g ** 2).sum(1)
+    order = np.lexsort((g[:, 2], g[:, 1], g[:, 0], key))
+    return g[order[:ni]]
+
+
+@dataclass
+class QPoint:
+    """All inputs of one q-point of the chi0 -> W -> Sigma_c path."""
+    cfg: Config
+    # chi0 side: bands at k (occupied used) and k+q (unoccupied used)
+    e_k: np.ndarray        # nbt x nk  (eV)
+    e_kq: np.ndarray       # nbt x nk
+    c_k: np.ndarray        # nb x nbt x nk
+    c_kq: np.ndarray       # nb x nbt x nk
+    # Sigma side: end states at q, middle states at q-k
+    e_q: np.ndarray        # nbt
+    e_qmk: np.ndarray      # nbt x nk
+    c_q: np.ndarray        # nb x nbt
+    c_qmk: np.ndarray      # nb x nbt x nk
+    ppb: np.ndarray        # nb x nb x ni (real)
+    vcoud: np.ndarray      # ni
+    zcousq: np.ndarray     # ni x ni
+    wk: np.ndarray         # nk BZ weights
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def vcousq(self):
+        return np.sqrt(self.vcoud)
+
+
+def make_qpoint(cfg: Config) -> QPoint:
+    rng = np.random.Generator(np.random.PCG64(cfg.seed))
+    nk, nb, nbt, ni = cfg.nk, cfg.nb, cfg.nbt, cfg.ni
+    if nb < nbt:
+        raise ValueError(f"{cfg.name}: nb={nb} < nbt={nbt}; QR cannot give nbt orthonormal vectors")
+    # energies
+    e_k = np.stack([_energies(rng, cfg.no, cfg.nu) for _ in range(nk)], axis=1)
+    e_kq = np.stack([_energies(rng, cfg.no, cfg.nu) for _ in range(nk)], axis=1)
+    e_q = _energies(rng, cfg.no, cfg.nu)
+    e_qmk = np.stack([_energies(rng, cfg.no, cfg.nu) for _ in range(nk)], axis=1)
+    # eigenvectors
+    c_k = np.stack([_qr_unitary(rng, nb, nbt) for _ in range(nk)], axis=2)
+    c_kq = np.stack([_qr_unitary(rng, nb, nbt) for _ in range(nk)], axis=2)
+    c_q = _qr_unitary(rng, nb, nbt)
+    c_qmk = np.stack([_qr_unitary(rng, nb, nbt) for _ in range(nk)], axis=2)
+    # ppb: real Gaussian with variance 1/(ni*Omega)
+    omega_cell = cfg.alat ** 3 / 4.0
+    ppb = np.asfortranarray(rng.standard_normal((nb, nb, ni)) / math.sqrt(ni * omega_cell))
+    # v(q)
+    q = np.array([0.1, 0.2, 0.3])
+    g = gvectors(ni)
+    tpioa = 2.0 * math.pi / cfg.alat
+    qg2 = (((q[None, :] + g) * tpioa) ** 2).sum(1)
+    vcoud = 4.0 * math.pi / qg2
+    zcousq = _qr_unitary(rng, ni, ni)
+    wk = np.full(nk, 1.0 / nk)
+    return QPoint(cfg, np.asfortranarray(e_k), np.asfortranarray(e_kq), np.asfortranarray(c_k),
+                  np.asfortranarray(c_kq), e_q, np.asfortranarray(e_qmk), c_q, np.asfortranarray(c_qmk),
+                  ppb, vcoud, zcousq, wk)
+
+
+# --------------------------------------------------------------------------
+def omg2max_hartree(qp: QPoint) -> float:
+    """omg2max of getfreq2 (S/m_freq.f90:63-64) from the band range, in Hartree."""
+    allE = np.concatenate([qp.e_k.ravel(), qp.e_kq.ravel()]) / RYDBERG_EV
+    return (allE.max() - allE.min()) * 0.5 + 0.2
+
+
+def frhis(qp: QPoint) -> np.ndarray:
+    """Histogram bin edges frhis(1:nwhis+1) in Hartree (S/m_freq.f90:95-108)."""
+    cfg = qp.cfg
+    aa = cfg.hist_ratio - 1.0
+    bb = cfg.hist_dw / aa
+    omg = omg2max_hartree(qp)
+    iw = 0
+    while True:
+        iw += 1
+        if bb * (math.exp(aa * (iw - 1)) - 1.0) > omg + 1e-6:
+            break
+    nwhis = iw + 2
+    return np.array([bb * (math.exp(aa * (i - 1)) - 1.0) for i in range(1, nwhis + 2)])
+
+
+@dataclass
+class PairWeights:
+    """Per-k band pairs with histogram weights, the tetwt5 output layout
+    (n1b/n2b, ihw/nhw/jhw, whw; S/m_tetwt.f90:7-25)."""
+    n1b: list       # per k: int array of occupied band indices (1-based)
+    n2b: list       # per k: unoccupied band indices (1-based, in 1..nbt)
+    ihw: list       # per k: first bin (1-based)
+    nhw: list       # per k: number of bins
+    jhw: list       # per k: pointer into whw (1-based)
+    whw: np.ndarray # all weights
+
+
+def pair_weights(qp: QPoint, fr: np.ndarray) -> PairWeights:
+    cfg = qp.cfg
+    nwhis = len(fr) - 1
+    n1b, n2b, ihw, nhw, jhw = [], [], [], [], []
+    whw_parts = []
+    ptr = 1
+    occ = np.arange(1, cfg.no + 1)
+    unocc = np.arange(cfg.no + 1, cfg.nbt + 1)
+    for k in range(cfg.nk):
+        nn, mm = np.meshgrid(occ, unocc, indexing="ij")  # pair order: n outer, m inner
+        nn = nn.ravel()
+        mm = mm.ravel()
+        d = (qp.e_kq[mm - 1, k] - qp.e_k[nn - 1, k]) / RYDBERG_EV * 0.5  # Hartree
+        lo = d * (1.0 - cfg.window)
+        hi = d * (1.0 + cfg.window)
+        b0 = np.searchsorted(fr, lo, side="right")  # bin index (1-based) containing lo
+        b1 = np.searchsorted(fr, hi, side="right")
+        if b0.min() < 1 or b1.max() > nwhis:
+            raise ValueError("histogram window outside the bin mesh")
+        cnt = b1 - b0 + 1
+        span = int(cnt.max())
+        w = np.zeros((len(d), span))
+        tot = math.pi * qp.wk[k]
+        for s in range(span):
+            b = b0 + s
+            valid = b <= b1
+            bl = fr[np.minimum(b, nwhis) - 1]
+            br = fr[np.minimum(b, nwhis)]
+            ov = np.clip(np.minimum(hi, br) - np.maximum(lo, bl), 0.0, None)
+            w[:, s] = np.where(valid, tot * ov / (hi - lo), 0.0)
+        n1b.append(nn.astype(np.int32))
+        n2b.append(mm.astype(np.int32))
+        ihw.append(b0.astype(np.int32))
+        nhw.append(cnt.astype(np.int32))
+        j = np.empty(len(d), np.int32)
+        j[0] = ptr
+        j[1:] = ptr + np.cumsum(cnt)[:-1]
+        jhw.append(j)
+        ptr += int(cnt.sum())
+        mask = np.arange(span)[None, :] < cnt[:, None]
+        whw_parts.append(w[mask])
+    return PairWeights(n1b, n2b, ihw, nhw, jhw, np.concatenate(whw_parts))
+
+
+@dataclass
+class X0Index:
+    """The flattened index lists of x0kf_v4hz_init (S/x0kf_v4h.f90:48-142)."""
+    kc: np.ndarray
+    itc: np.ndarray
+    itpc: np.ndarray
+    jpmc: np.ndarray
+    iwini: np.ndarray
+    iwend: np.ndarray
+    icouini: np.ndarray
+    whwc: np.ndarray
+    icounkmin: np.ndarray
+    icounkmax: np.ndarray
+    nkmin: np.ndarray
+    nkmax: np.ndarray
+    nkqmin: np.ndarray
+    nkqmax: np.ndarray
+
+
+def x0kf_index(pw: PairWeights, nctot: int = 0, npm: int = 1) -> X0Index:
+    """Restates x0kf_v4hz_init job=0 then job=1 (S/x0kf_v4h.f90:96-138) for
+    valence pairs and npm=1: it = nctot + n1b, itp = n2b - nkqmin(k) + 1."""
+    nk = len(pw.n1b)
+    nkmin = np.array([p.min() for p in pw.n1b], np.int32)
+    nkmax = np.array([p.max() for p in pw.n1b], np.int32)
+    nkqmin = np.array([p.min() for p in pw.n2b], np.int32)
+    nkqmax = np.array([p.max() for p in pw.n2b], np.int32)
+    kc, itc, itpc, jpmc, iwini, iwend, icouini, whwc = ([] for _ in range(8))
+    icounkmin = np.zeros(nk, np.int32)
+    icounkmax = np.zeros(nk, np.int32)
+    icoun = 0
+    icount = 0
+    for k in range(nk):
+        npair = len(pw.n1b[k])
+        icounkmin[k] = icoun + 1
+        kc.append(np.full(npair, k + 1, np.int32))
+        itc.append(nctot + pw.n1b[k])
+        itpc.append(pw.n2b[k] - nkqmin[k] + 1)
+        jpmc.append(np.ones(npair, np.int32))
+        iwini.append(pw.ihw[k])
+        iwend.append(pw.ihw[k] + pw.nhw[k] - 1)
+        ici = np.empty(npair, np.int32)
+        ici[0] = icount + 1
+        ici[1:] = icount + 1 + np.cumsum(pw.nhw[k])[:-1]
+        icouini.append(ici)
+        tot = int(pw.nhw[k].sum())
+        whwc.append(pw.whw[pw.jhw[k][0] - 1: pw.jhw[k][0] - 1 + tot])
+        icoun += npair
+        icount += tot
+        icounkmax[k] = icoun
+    cat = lambda xs, dt: np.ascontiguousarray(np.concatenate(xs).astype(dt))
+    return X0Index(cat(kc, np.int32), cat(itc, np.int32), cat(itpc, np.int32), cat(jpmc, np.int32),
+                   cat(iwini, np.int32), cat(iwend, np.int32), cat(icouini, np.int32),
+                   cat(whwc, np.float64), icounkmin, icounkmax, nkmin, nkmax, nkqmin, nkqmax)
+
+
+def sigma_inputs(qp: QPoint, seed_offset: int = 1000):
+    """Synthetic Sigma_x, Sigma_x(core) (Hartree) and vxc (eV) for the QP update."""
+    cfg = qp.cfg
+    rng = np.random.Generator(np.random.PCG64(cfg.seed + seed_offset))
+    sex = rng.uniform(-0.8, -0.1, cfg.nq)
+    sexcore = rng.uniform(-0.05, 0.0, cfg.nq)
+    vxc = rng.uniform(-20.0, -5.0, cfg.nq)
+    return sex, sexcore, vxc
+
+
+def wc_phases(cfg: Config, seed_offset: int = 2000) -> np.ndarray:
+    """Unit-modulus phases d_k(I): W_c at internal k is taken as
+    D_k W_c(q) D_k^H (same spectrum, Hermitian, negative semidefinite)."""
+    rng = np.random.Generator(np.random.PCG64(cfg.seed + seed_offset))
+    ph = rng.uniform(0.0, 2.0 * math.pi, (cfg.ni, cfg.nk))
+    ph[:, 0] = 0.0
+    return np.asfortranarray(np.exp(1j * ph))

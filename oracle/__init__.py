@@ -1,0 +1,286 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is the checker for the HIP library in
+``ecalj_amd``: only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package.  Every wrapper
+takes and returns numpy arrays in Fortran (column-major) order; complex arrays
+are complex128.  Each C function cites the ecalj routine it restates
+(see oracle/oracle.h).
+
+``ref()`` loads ``oracle/_ref/libecaljref.so``: ecalj's own
+S/hilbertmat.f90 and S/wfacx.f90 compiled unmodified with flang by
+oracle/Makefile, used to pin the restatement.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so (and oracle/_ref when the ecalj tree is present)."""
+    so = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        _LIB = C.CDLL(so)
+        _LIB.orc_gauss.restype = C.c_int
+        _LIB.orc_freq01.restype = C.c_int
+        _LIB.orc_frhis.restype = C.c_int
+        _LIB.orc_zmel.restype = C.c_int
+        _LIB.orc_wc_build.restype = C.c_int
+        _LIB.orc_zinv.restype = C.c_int
+        _LIB.orc_wfacx2.restype = C.c_double
+        _LIB.orc_weavx2.restype = C.c_double
+        for name in ("orc_wfacx2", "orc_weavx2"):
+            getattr(_LIB, name).argtypes = [C.c_double] * 4
+    return _LIB
+
+
+def ref():
+    """The reference-compiled checker, or None if it was not built here."""
+    global _REF
+    if _REF is None:
+        so = os.path.join(_HERE, "_ref", "libecaljref.so")
+        if not os.path.exists(so):
+            return None
+        _REF = C.CDLL(so)
+        _REF._QMm_wfacPwfacx2.restype = C.c_double
+        _REF._QMm_wfacPweavx2.restype = C.c_double
+    return _REF
+
+
+def _p(a: np.ndarray):
+    assert a.flags.f_contiguous or a.ndim <= 1, "oracle arrays must be Fortran-ordered"
+    return a.ctypes.data_as(_dp)
+
+
+def _ip_(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+def _cz(x) -> np.ndarray:
+    return np.asfortranarray(np.asarray(x, dtype=np.complex128))
+
+
+def _cd(x) -> np.ndarray:
+    return np.asfortranarray(np.asarray(x, dtype=np.float64))
+
+
+def _ci(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+# --------------------------------------------------------------------------
+def zgemm(opa, opb, a, b, alpha=1.0, beta=0.0, c=None):
+    a = _cz(a)
+    b = _cz(b)
+    m = a.shape[0] if opa == "N" else a.shape[1]
+    k = a.shape[1] if opa == "N" else a.shape[0]
+    n = b.shape[1] if opb == "N" else b.shape[0]
+    c = np.zeros((m, n), np.complex128, order="F") if c is None else _cz(c).copy(order="F")
+    _ensure_cplx_sigs()
+    lib().orc_zgemm(opa.encode(), opb.encode(), m, n, k, _cplx(alpha), _p(a), a.shape[0], _p(b),
+                    b.shape[0], _cplx(beta), _p(c), m)
+    return c
+
+
+class _Cplx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def _cplx(z):
+    z = complex(z)
+    return _Cplx(z.real, z.imag)
+
+
+lib_cplx_set = False
+
+
+def _ensure_cplx_sigs():
+    global lib_cplx_set
+    if lib_cplx_set:
+        return
+    L = lib()
+    L.orc_zgemm.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, _Cplx, _dp, C.c_long, _dp,
+                            C.c_long, _Cplx, _dp, C.c_long]
+    L.orc_hilbertmat.argtypes = [_Cplx, C.c_int, _dp, _dp, _dp, _dp]
+    lib_cplx_set = True
+
+
+def gauss(n, a=0.0, b=1.0):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    if lib().orc_gauss(n, C.c_double(a), C.c_double(b), _p(x), _p(w)) != 0:
+        raise ValueError(f"gauss: unsupported n={n}")
+    return x, w
+
+
+def freq01(nx, ua=1.0):
+    fx, fw, wx, ex = (np.zeros(nx) for _ in range(4))
+    if lib().orc_freq01(nx, C.c_double(ua), _p(fx), _p(fw), _p(wx), _p(ex)) != 0:
+        raise ValueError(f"freq01: unsupported nx={nx}")
+    return fx, fw, wx, ex
+
+
+def frhis(omg2max, oratio=1.03, dw=1e-5):
+    n = lib().orc_frhis(C.c_double(oratio), C.c_double(dw), C.c_double(omg2max), None)
+    f = np.zeros(n + 1)
+    lib().orc_frhis(C.c_double(oratio), C.c_double(dw), C.c_double(omg2max), _p(f))
+    return n, f
+
+
+def hilbertmat(zz, his_L, his_C, his_R):
+    _ensure_cplx_sigs()
+    nwhis = (len(his_L) - 1) // 2
+    r = np.zeros(2 * nwhis + 1, np.complex128)
+    lib().orc_hilbertmat(_cplx(zz), nwhis, _p(_cd(his_L)), _p(_cd(his_C)), _p(_cd(his_R)), _p(r))
+    return r
+
+
+def imatt(freqi, frhis_):
+    freqi = _cd(freqi)
+    frhis_ = _cd(frhis_)
+    niw, nwhis = len(freqi), len(frhis_) - 1
+    out = np.zeros((niw, nwhis), order="F")
+    lib().orc_imatt(niw, _p(freqi), nwhis, _p(frhis_), _p(out))
+    return out
+
+
+def zmel(nv, mdim, phase, ppb, cphim, cphiq, m2e=None, conjg=False):
+    nv = _ci(nv)
+    mdim = _ci(mdim)
+    phase = np.ascontiguousarray(np.asarray(phase, np.complex128))
+    ppb = np.ascontiguousarray(np.asarray(ppb, np.float64).ravel(order="F"))
+    cphim = _cz(cphim)
+    cphiq = _cz(cphiq)
+    ndima = cphim.shape[0]
+    nm, nq = cphim.shape[1], cphiq.shape[1]
+    ngb = int(mdim.sum())
+    if m2e is not None:
+        m2e = _cz(m2e)
+        nbb = m2e.shape[1]
+    else:
+        nbb = ngb
+    out = np.zeros((nbb, nm, nq), np.complex128, order="F")
+    rc = lib().orc_zmel(len(nv), _ip_(nv), _ip_(mdim), _p(phase), _p(ppb), ndima, nm, _p(cphim), nq,
+                        _p(cphiq), _p(m2e) if m2e is not None else None, nbb, int(bool(conjg)), _p(out))
+    if rc != 0:
+        raise ValueError(f"orc_zmel rc={rc}")
+    return out
+
+
+def x0gemm(rcxq, npr, ipr_col, npr_col, nwhis, npm, ns1, ns2, zmel_, icoun_lo, icoun_hi, iwini, iwend,
+           itc, itpc, jpmc, icouini, whwc):
+    """Accumulates into rcxq (Fortran array npr x npr_col x nslice) in place."""
+    assert rcxq.dtype == np.complex128 and rcxq.flags.f_contiguous
+    zmel_ = _cz(zmel_)
+    nbb, _, nqtot = zmel_.shape
+    lib().orc_x0gemm(_p(rcxq), npr, ipr_col, npr_col, nwhis, npm, ns1, ns2, _p(zmel_), nbb, nqtot,
+                     icoun_lo, icoun_hi, _ip_(_ci(iwini)), _ip_(_ci(iwend)), _ip_(_ci(itc)),
+                     _ip_(_ci(itpc)), _ip_(_ci(jpmc)), _ip_(_ci(icouini)), _p(_cd(whwc)))
+    return rcxq
+
+
+def dpsion_imag(rcxq, frhis_, imatt_, ecut=1e10):
+    """Returns zxqi; rcxq (npr x npr_col x nwhis+1) is scaled in place as the reference does."""
+    assert rcxq.dtype == np.complex128 and rcxq.flags.f_contiguous
+    npr, npr_col, ns = rcxq.shape
+    nwhis = ns - 1
+    niw = imatt_.shape[0]
+    zxqi = np.zeros((npr, npr_col, niw), np.complex128, order="F")
+    lib().orc_dpsion_imag(npr, npr_col, nwhis, niw, _p(_cd(frhis_)), _p(_cd(imatt_)), C.c_double(ecut),
+                          _p(rcxq), _p(zxqi))
+    return zxqi
+
+
+def chi0_freq(z, w):
+    z = _cz(z)
+    w = _cd(w)
+    ni, npairs = z.shape
+    nf = w.shape[0]
+    chi = np.zeros((ni, ni, nf), np.complex128, order="F")
+    lib().orc_chi0_freq(ni, npairs, nf, _p(z), _p(w), _p(chi))
+    return chi
+
+
+def wc_build(chi, vcousq, m2e, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True):
+    chi = _cz(chi)
+    ngb, _, nw = chi.shape
+    wc = np.zeros((ngb, ngb, nw), np.complex128, order="F")
+    rc = lib().orc_wc_build(ngb, nw, nspin, int(is_x0_m_basis), int(is_wc_m_basis), _p(chi),
+                            _p(_cd(vcousq)), _p(_cz(m2e)), _p(wc), ngb)
+    if rc != 0:
+        raise np.linalg.LinAlgError("orc_wc_build: singular epsilon")
+    return wc
+
+
+def zinv(a):
+    a = _cz(a).copy(order="F")
+    if lib().orc_zinv(a.shape[0], _p(a), C.c_long(a.shape[0])) != 0:
+        raise np.linalg.LinAlgError("singular")
+    return a
+
+
+def wgtim(omega, ekc, esmr, wkkr, x, wt, expa, ua=1.0, ns1=1, nctot=0):
+    omega = _cd(omega)
+    ekc = _cd(ekc)
+    niw = len(x)
+    nm = len(ekc)
+    ntq = len(omega)
+    out = np.zeros((niw + 1, nm, ntq), order="F")
+    lib().orc_wgtim(niw, ns1, ns1 + nm - 1, ntq, nctot, _p(omega), _p(ekc), C.c_double(esmr),
+                    C.c_double(ua), C.c_double(wkkr), _p(_cd(x)), _p(_cd(wt)), _p(_cd(expa)), _p(out))
+    return out
+
+
+def sigc_imag(zmel_, wc, wgt, zsec=None):
+    zmel_ = _cz(zmel_)
+    wc = _cz(wc)
+    ngb, nm, ntq = zmel_.shape
+    niw = wc.shape[2] - 1
+    zsec = np.zeros((ntq, ntq), np.complex128, order="F") if zsec is None else _cz(zsec).copy(order="F")
+    lib().orc_sigc_imag(ngb, nm, ntq, niw, _p(zmel_), _p(wc), wc.shape[0], _p(_cd(wgt)), _p(zsec),
+                        zsec.shape[0])
+    return zsec
+
+
+def wfacx2(e1, e2, ek, esmr):
+    return lib().orc_wfacx2(e1, e2, ek, esmr)
+
+
+def weavx2(e1, e2, ek, esmr):
+    return lib().orc_weavx2(e1, e2, ek, esmr)
+
+
+def qp_energy(elda, sex, sexcore, sec, vxc):
+    sec = _cz(sec)
+    ntq = len(elda)
+    eqp = np.zeros(ntq)
+    lib().orc_qp_energy(ntq, _p(_cd(elda)), _p(_cd(sex)), _p(_cd(sexcore)), _p(sec), sec.shape[0],
+                        _p(_cd(vxc)), _p(eqp))
+    return eqp
+
+
+def sigma_herm(sex2, sexcore2, sec, vxcmat):
+    ntq = sec.shape[0]
+    se = np.zeros((ntq, ntq), np.complex128, order="F")
+    lib().orc_sigma_herm(ntq, _p(_cz(sex2)), _p(_cz(sexcore2)), _p(_cz(sec)), _p(_cz(vxcmat)), _p(se))
+    return se

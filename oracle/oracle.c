@@ -1,0 +1,624 @@
+/*
+ * oracle.c -- FP64 CPU restatement of ecalj's QSGW chi0 -> W -> Sigma_c path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never part of the
+ * product path.  Each function cites the reference routine it follows
+ * (S/ = SRC/subroutines/ of ecalj).  OpenMP is used only to parallelise over
+ * independent output columns; summation order inside one output element is
+ * the reference's order.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.14159265358979323846
+
+/* ---------------------------------------------------------------------- */
+/* zmm_h (S/m_blas.f90:205-232): C = alpha op(A) op(B) + beta C, column-major.
+ * The reference calls zgemm3m; this restates plain zgemm semantics.       */
+static inline double complex opget(char op, const double complex* a, long lda, long i, long l) {
+  /* element (i,l) of op(A) */
+  if (op == 'N') return a[i + lda * l];
+  if (op == 'T') return a[l + lda * i];
+  return conj(a[l + lda * i]);
+}
+
+void orc_zgemm(char opa, char opb, int m, int n, int k, double complex alpha,
+               const double complex* a, long lda, const double complex* b, long ldb,
+               double complex beta, double complex* c, long ldc) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int j = 0; j < n; ++j) {
+    double complex* cj = c + ldc * (long)j;
+    if (beta == 0.0) {
+      for (int i = 0; i < m; ++i) cj[i] = 0.0;
+    } else if (beta != 1.0) {
+      for (int i = 0; i < m; ++i) cj[i] *= beta;
+    }
+    if (opa == 'N') {
+      /* axpy form: contiguous in i */
+      for (int l = 0; l < k; ++l) {
+        double complex blj = alpha * opget(opb, b, ldb, l, j);
+        if (blj == 0.0) continue;
+        const double complex* al = a + lda * (long)l;
+        for (int i = 0; i < m; ++i) cj[i] += al[i] * blj;
+      }
+    } else {
+      /* dot form: op(A)(i,l) = A(l,i) or conj(A(l,i)), contiguous in l */
+      double complex* bcol = NULL;
+      double complex bbuf[512];
+      if (k <= 512) bcol = bbuf; else bcol = (double complex*)malloc(sizeof(double complex) * k);
+      for (int l = 0; l < k; ++l) bcol[l] = opget(opb, b, ldb, l, j);
+      for (int i = 0; i < m; ++i) {
+        const double complex* ai = a + lda * (long)i;
+        double sr = 0.0, si = 0.0;
+        if (opa == 'C') {
+          for (int l = 0; l < k; ++l) {
+            double ar = creal(ai[l]), aim = -cimag(ai[l]);
+            double br = creal(bcol[l]), bi = cimag(bcol[l]);
+            sr += ar * br - aim * bi;
+            si += ar * bi + aim * br;
+          }
+        } else {
+          for (int l = 0; l < k; ++l) {
+            double ar = creal(ai[l]), aim = cimag(ai[l]);
+            double br = creal(bcol[l]), bi = cimag(bcol[l]);
+            sr += ar * br - aim * bi;
+            si += ar * bi + aim * br;
+          }
+        }
+        cj[i] += alpha * (sr + I * si);
+      }
+      if (bcol != bbuf) free(bcol);
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* gauss (S/mate.f90:1-206).  The reference tabulates 15-digit nodes for
+ * n in {6,10,12,16,20,24,32,40,48} and stops otherwise (:113-124).  Here the
+ * nodes are the roots of P_n found by Newton iteration (textbook method),
+ * weights 2/((1-x^2) P_n'(x)^2), mapped to [a,b] as x = e2 +- e1*t
+ * (S/mate.f90:110-111).  Node order follows the reference: ascending.   */
+int orc_gauss(int n, double a, double b, double* x, double* w) {
+  static const int ok[] = {6, 10, 12, 16, 20, 24, 32, 40, 48};
+  int supported = 0;
+  for (int i = 0; i < 9; ++i) supported |= (ok[i] == n);
+  if (!supported) return -1;
+  double e1 = 0.5 * (b - a), e2 = e1 + a;
+  int half = n / 2;
+  for (int i = 1; i <= half; ++i) {
+    /* i-th positive root counted from the largest */
+    double t = cos(PI * (i - 0.25) / (n + 0.5));
+    double dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = t;
+      for (int j = 2; j <= n; ++j) {
+        double p2 = ((2.0 * j - 1.0) * t * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+      double dt = p1 / dp;
+      t -= dt;
+      if (fabs(dt) < 1e-16) break;
+    }
+    {
+      double p0 = 1.0, p1 = t;
+      for (int j = 2; j <= n; ++j) {
+        double p2 = ((2.0 * j - 1.0) * t * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+    }
+    double wt = 2.0 / ((1.0 - t * t) * dp * dp);
+    /* roots t_i descending in i; reference stores x(half+i')=e2+e1*t ascending */
+    int ip = half - i + 1; /* 1-based index of this root among positive roots, ascending */
+    x[half + ip - 1] = e2 + e1 * t;
+    x[half - ip] = e2 - e1 * t;
+    w[half + ip - 1] = wt * e1;
+    w[half - ip] = wt * e1;
+  }
+  return 0;
+}
+
+/* freq01 (S/m_freq.f90:164-186); expa as in readfreq_r (S/m_readfreq_r.f90:44-49). */
+int orc_freq01(int nx, double ua, double* freqx, double* freqw, double* wx, double* expa) {
+  if (orc_gauss(nx, 0.0, 1.0, freqx, wx) != 0) return -1;
+  for (int i = 0; i < nx; ++i) {
+    freqw[i] = (1.0 - freqx[i]) / freqx[i];
+    if (expa) expa[i] = exp(-ua * ua * freqw[i] * freqw[i]);
+  }
+  return 0;
+}
+
+/* Histogram bins (S/m_freq.f90:95-108). */
+int orc_frhis(double oratio, double dw, double omg2max, double* frhis) {
+  double aa = oratio - 1.0, bb = dw / aa;
+  int iw = 0;
+  for (;;) {
+    iw += 1;
+    if (bb * (exp(aa * (iw - 1)) - 1.0) > omg2max + 1e-6) break;
+  }
+  int nwhis = iw + 2;
+  if (frhis)
+    for (int i = 1; i <= nwhis + 1; ++i) frhis[i - 1] = bb * (exp(aa * (i - 1)) - 1.0);
+  return nwhis;
+}
+
+/* ---------------------------------------------------------------------- */
+/* hilbertmat (S/hilbertmat.f90:14-79); inner hilbertmat2 (:56-70).        */
+static void hilbertmat2(double complex zz, double a, double b, double complex* facm,
+                        double complex* facn) {
+  const double eps = 1e-8;
+  double complex db = zz - b, da = zz - a;
+  *facm = 0.0;
+  *facn = 0.0;
+  if (cabs(da) > eps && cabs(db) > eps) {
+    *facn = clog(db / da);
+    *facm = zz * (*facn) + (b - a);
+  }
+}
+
+void orc_hilbertmat(double complex zz, int nwhis, const double* his_L, const double* his_C,
+                    const double* his_R, double complex* rmat) {
+  const double epsz = 1e-13;
+  const double complex zze = zz + I * epsz;
+  /* all arrays indexed -nwhis..nwhis -> offset nwhis */
+  const double* L = his_L + nwhis;
+  const double* C = his_C + nwhis;
+  const double* R = his_R + nwhis;
+  double complex* rm = rmat + nwhis;
+  for (int i = -nwhis; i <= nwhis; ++i) rm[i] = 0.0;
+  for (int iw = -nwhis; iw <= nwhis; ++iw) {
+    if (iw == 0) continue;
+    double ci = C[iw], cim, cip;
+    double complex facm, facn;
+    if (iw == -nwhis) cim = L[iw];
+    else if (iw == 1) cim = 0.0;
+    else cim = C[iw - 1];
+    double cicim = ci - cim;
+    hilbertmat2(zze, L[iw], C[iw], &facm, &facn);
+    rm[iw] += (facm - facn * cim) / cicim;
+    if (iw != -nwhis && iw != 1) rm[iw - 1] += (-facm + facn * ci) / cicim;
+    if (iw == nwhis) cip = R[iw];
+    else if (iw == -1) cip = 0.0;
+    else cip = C[iw + 1];
+    double cipci = cip - ci;
+    hilbertmat2(zze, C[iw], R[iw], &facm, &facn);
+    rm[iw] += (-facm + facn * cip) / cipci;
+    if (iw != nwhis && iw != -1) rm[iw + 1] += (facm - facn * ci) / cipci;
+  }
+}
+
+/* his_L/his_C/his_R from frhis (S/dpsion5.f90:84-86). */
+static void his_edges(int nwhis, const double* frhis, double* L, double* C, double* R) {
+  /* index j = iw + nwhis, iw in -nwhis..nwhis; frhis is 1-based frhis(1:nwhis+1) */
+  for (int iw = -nwhis; iw <= nwhis; ++iw) {
+    int j = iw + nwhis;
+    if (iw < 0) {
+      L[j] = -frhis[-iw];     /* -frhis(-iw+1) */
+      R[j] = -frhis[-iw - 1]; /* -frhis(-iw)   */
+    } else if (iw == 0) {
+      L[j] = 0.0; /* middle element of both source lists is 0 */
+      R[j] = 0.0;
+    } else {
+      L[j] = frhis[iw - 1]; /* frhis(iw)   */
+      R[j] = frhis[iw];     /* frhis(iw+1) */
+    }
+    C[j] = 0.5 * (L[j] + R[j]);
+  }
+}
+
+/* dpsion_init imaginary branch, npm=1 (S/dpsion5.f90:121-129). */
+void orc_imatt(int niw, const double* freqi, int nwhis, const double* frhis, double* imatt) {
+  int n2 = 2 * nwhis + 1;
+  double* L = (double*)malloc(sizeof(double) * n2 * 3);
+  double* C = L + n2;
+  double* R = C + n2;
+  his_edges(nwhis, frhis, L, C, R);
+#pragma omp parallel for
+  for (int it = 0; it < niw; ++it) {
+    double complex* rrr = (double complex*)malloc(sizeof(double complex) * n2);
+    orc_hilbertmat(I * freqi[it], nwhis, L, C, R, rrr);
+    for (int iw = 1; iw <= nwhis; ++iw)
+      imatt[it + (long)niw * (iw - 1)] = creal(rrr[nwhis + iw] - rrr[nwhis - iw]) / PI;
+    free(rrr);
+  }
+  free(L);
+}
+
+/* ---------------------------------------------------------------------- */
+/* zmel (S/m_zmel.f90:146-597).  MT valence-valence block per atom
+ * (:323-371):  ppbvphiq(:,:,i) = ppb(:,:,i)^T cphiq  (:361-362),
+ *              zmelt_d(:,:,i)  = phase * cphim^H ppbvphiq(:,:,i) (:365-366),
+ *              zmelt(ims+i-1,:,:) = zmelt_d(:,:,i) (:367-371);
+ * then M->E (:559-560) and conj (:590-594).                              */
+int orc_zmel(int natom, const int* nv, const int* mdim, const double complex* phase,
+             const double* ppb, int ndima, int nm, const double complex* cphim, int nq,
+             const double complex* cphiq, const double complex* m2e, int nbb, int conjg,
+             double complex* zmel) {
+  int ngb = 0, nvsum = 0;
+  for (int a = 0; a < natom; ++a) {
+    ngb += mdim[a];
+    nvsum += nv[a];
+  }
+  if (nvsum > ndima) return 1;
+  double complex* zmelt = (double complex*)calloc((size_t)ngb * nm * nq, sizeof(double complex));
+  long ias = 0, ims = 0, poff = 0;
+  for (int a = 0; a < natom; ++a) {
+    int v = nv[a], md = mdim[a];
+    const double* pa = ppb + poff;
+    /* ppbvphiq(b, m, i) = sum_c ppb(c, b, i) cphiq(ias+c, m) */
+    double complex* pv = (double complex*)malloc(sizeof(double complex) * (size_t)v * nq * md);
+#pragma omp parallel for
+    for (int i = 0; i < md; ++i)
+      for (int m = 0; m < nq; ++m)
+        for (int b = 0; b < v; ++b) {
+          double complex s = 0.0;
+          for (int c = 0; c < v; ++c)
+            s += pa[c + (long)v * b + (long)v * v * i] * cphiq[ias + c + (long)ndima * m];
+          pv[b + (long)v * m + (long)v * nq * i] = s;
+        }
+    /* zmelt(ims+i, n, m) = phase * sum_b conj(cphim(ias+b, n)) ppbvphiq(b, m, i) */
+#pragma omp parallel for
+    for (int i = 0; i < md; ++i)
+      for (int m = 0; m < nq; ++m)
+        for (int n = 0; n < nm; ++n) {
+          double complex s = 0.0;
+          for (int b = 0; b < v; ++b)
+            s += conj(cphim[ias + b + (long)ndima * n]) * pv[b + (long)v * m + (long)v * nq * i];
+          zmelt[ims + i + (long)ngb * (n + (long)nm * m)] = phase[a] * s;
+        }
+    free(pv);
+    ias += v;
+    ims += md;
+    poff += (long)v * v * md;
+  }
+  long nrow = m2e ? nbb : ngb;
+  if (m2e) {
+    /* zmel = m2e^H zmelt  (nbb x ngb) (ngb x nm*nq) */
+    orc_zgemm('C', 'N', nbb, nm * nq, ngb, 1.0, m2e, ngb, zmelt, ngb, 0.0, zmel, nbb);
+  } else {
+    memcpy(zmel, zmelt, sizeof(double complex) * (size_t)ngb * nm * nq);
+  }
+  if (conjg)
+    for (long i = 0; i < nrow * nm * nq; ++i) zmel[i] = conj(zmel[i]);
+  free(zmelt);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* x0gemm (S/x0gemm.f90:2-97): bin lists (:36-64) then per (jpm, iw)
+ * rcxq(:,:,iw*(3-2jpm)) += zw^H wzw with zw(t,I)=zmel(I,it,itp),
+ * wzw(t,J)=zmel(J+ipr_col-1,it,itp)*whw(t) (:72-90).                    */
+void orc_x0gemm(double complex* rcxq, int npr, int ipr_col, int npr_col, int nwhis, int npm,
+                int ns1, int ns2, const double complex* zmel, int nbb, int nqtot,
+                int icoun_lo, int icoun_hi, const int* iwini, const int* iwend, const int* itc,
+                const int* itpc, const int* jpmc, const int* icouini, const double* whwc) {
+  int nm = ns2 - ns1 + 1;
+  long slice = (long)npr * npr_col;
+  int s0 = (1 - npm) * nwhis; /* Fortran index of slice 0 */
+  (void)nqtot;
+  /* count pairs per (iw, jpm) and build lists in icoun order */
+  int nb = nwhis * npm;
+  int* cnt = (int*)calloc(nb, sizeof(int));
+  for (int ic = icoun_lo; ic <= icoun_hi; ++ic) {
+    int it = itc[ic - 1];
+    if (it < ns1 || it > ns2) continue;
+    for (int iw = iwini[ic - 1]; iw <= iwend[ic - 1]; ++iw) cnt[(iw - 1) + nwhis * (jpmc[ic - 1] - 1)]++;
+  }
+  long* start = (long*)malloc(sizeof(long) * (nb + 1));
+  start[0] = 0;
+  for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + cnt[b];
+  long tot = start[nb];
+  long* col = (long*)malloc(sizeof(long) * (tot + 1));
+  double* wt = (double*)malloc(sizeof(double) * (tot + 1));
+  memset(cnt, 0, sizeof(int) * nb);
+  for (int ic = icoun_lo; ic <= icoun_hi; ++ic) {
+    int it = itc[ic - 1], itp = itpc[ic - 1];
+    if (it < ns1 || it > ns2) continue;
+    for (int iw = iwini[ic - 1]; iw <= iwend[ic - 1]; ++iw) {
+      int b = (iw - 1) + nwhis * (jpmc[ic - 1] - 1);
+      long p = start[b] + cnt[b]++;
+      col[p] = (long)nbb * ((it - ns1) + (long)nm * (itp - 1));
+      wt[p] = whwc[iw - iwini[ic - 1] + icouini[ic - 1] - 1];
+    }
+  }
+  for (int jpm = 1; jpm <= npm; ++jpm)
+    for (int iw = 1; iw <= nwhis; ++iw) {
+      int b = (iw - 1) + nwhis * (jpm - 1);
+      if (start[b + 1] == start[b]) continue;
+      int fidx = iw * (3 - 2 * jpm);
+      double complex* r = rcxq + slice * (fidx - s0);
+#pragma omp parallel for
+      for (int j = 0; j < npr_col; ++j)
+        for (long p = start[b]; p < start[b + 1]; ++p) {
+          const double complex* z = zmel + col[p];
+          double complex wz = z[j + ipr_col - 1] * wt[p];
+          for (int i = 0; i < npr; ++i) r[i + (long)npr * j] += conj(z[i]) * wz;
+        }
+    }
+  free(cnt);
+  free(start);
+  free(col);
+  free(wt);
+}
+
+/* dpsion_chiq imaginary axis, npm=1 (S/dpsion5.f90:211-232). */
+void orc_dpsion_imag(int npr, int npr_col, int nwhis, int niw, const double* frhis,
+                     const double* imatt, double ecut, double complex* rcxq, double complex* zxqi) {
+  long slice = (long)npr * npr_col;
+  for (int iw = 1; iw <= nwhis; ++iw) {
+    double hl = frhis[iw - 1], hr = frhis[iw], hc = 0.5 * (hl + hr);
+    double wfac = (ecut < 1e9) ? exp(-(hc / ecut) * (hc / ecut)) : 1.0;
+    double s = -wfac / (hr - hl);
+    double complex* r = rcxq + slice * iw;
+    for (long e = 0; e < slice; ++e) r[e] *= s;
+  }
+#pragma omp parallel for
+  for (long e = 0; e < slice; ++e)
+    for (int it = 0; it < niw; ++it) {
+      double complex s = 0.0;
+      for (int iw = 1; iw <= nwhis; ++iw) s += rcxq[e + slice * iw] * imatt[it + (long)niw * (iw - 1)];
+      zxqi[e + slice * it] = s;
+    }
+}
+
+/* Composed direct-frequency chi0 (associativity of the x0gemm bin sum and
+ * the dpsion_chiq GEMM, S/x0gemm.f90:88-89 with S/dpsion5.f90:229).      */
+void orc_chi0_freq(int ni, int npairs, int nf, const double complex* z, const double* w,
+                   double complex* chi) {
+  long slice = (long)ni * ni;
+#pragma omp parallel for
+  for (int j = 0; j < ni; ++j)
+    for (int v = 0; v < nf; ++v) {
+      double complex* c = chi + slice * v + (long)ni * j;
+      for (int i = 0; i < ni; ++i) c[i] = 0.0;
+      for (int p = 0; p < npairs; ++p) {
+        const double complex* zp = z + (long)ni * p;
+        double complex wz = w[v + (long)nf * p] * zp[j];
+        for (int i = 0; i < ni; ++i) c[i] += conj(zp[i]) * wz;
+      }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* zminv_h (S/m_lapack.f90:21-47): zgetrf + zgetri.  Restated as LU with
+ * partial pivoting followed by inversion; same inverse up to rounding.   */
+int orc_zinv(int n, double complex* a, long lda) {
+  int* piv = (int*)malloc(sizeof(int) * n);
+  /* LU factorisation, right-looking, partial pivoting */
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = cabs(a[k + lda * k]);
+    for (int i = k + 1; i < n; ++i) {
+      double v = cabs(a[i + lda * k]);
+      if (v > best) { best = v; p = i; }
+    }
+    piv[k] = p;
+    if (best == 0.0) { free(piv); return 2; }
+    if (p != k)
+      for (int j = 0; j < n; ++j) {
+        double complex t = a[k + lda * j];
+        a[k + lda * j] = a[p + lda * j];
+        a[p + lda * j] = t;
+      }
+    double complex inv = 1.0 / a[k + lda * k];
+    for (int i = k + 1; i < n; ++i) a[i + lda * k] *= inv;
+#pragma omp parallel for
+    for (int j = k + 1; j < n; ++j) {
+      double complex akj = a[k + lda * j];
+      if (akj == 0.0) continue;
+      for (int i = k + 1; i < n; ++i) a[i + lda * j] -= a[i + lda * k] * akj;
+    }
+  }
+  /* inv(U) in place (upper triangle) */
+  for (int j = 0; j < n; ++j) {
+    a[j + lda * j] = 1.0 / a[j + lda * j];
+    double complex ajj = -a[j + lda * j];
+    for (int i = 0; i < j; ++i) {
+      double complex s = 0.0;
+      for (int l = i; l < j; ++l) s += a[i + lda * l] * a[l + lda * j];
+      a[i + lda * j] = s;
+    }
+    for (int i = 0; i < j; ++i) a[i + lda * j] *= ajj;
+  }
+  /* solve inv(A) * L = inv(U) for inv(A) (getri), column by column from the right */
+  double complex* work = (double complex*)malloc(sizeof(double complex) * n);
+  for (int j = n - 1; j >= 0; --j) {
+    for (int i = j + 1; i < n; ++i) {
+      work[i] = a[i + lda * j];
+      a[i + lda * j] = 0.0;
+    }
+    for (int l = j + 1; l < n; ++l) {
+      double complex wl = work[l];
+      if (wl == 0.0) continue;
+      for (int i = 0; i < n; ++i) a[i + lda * j] -= a[i + lda * l] * wl;
+    }
+  }
+  free(work);
+  /* undo the row interchanges as column interchanges */
+  for (int k = n - 1; k >= 0; --k) {
+    int p = piv[k];
+    if (p != k)
+      for (int i = 0; i < n; ++i) {
+        double complex t = a[i + lda * k];
+        a[i + lda * k] = a[i + lda * p];
+        a[i + lda * p] = t;
+      }
+  }
+  free(piv);
+  return 0;
+}
+
+/* WVIllwI body, q /= 0 (S/m_llw.f90:299-373). */
+int orc_wc_build(int ngb, int nw, int nspin, int is_x0_m_basis, int is_wc_m_basis,
+                 const double complex* chi, const double* vcousq, const double complex* m2e,
+                 double complex* wc, int ldw) {
+  long n2 = (long)ngb * ngb;
+  double complex* x = (double complex*)malloc(sizeof(double complex) * n2);
+  double complex* t = (double complex*)malloc(sizeof(double complex) * n2);
+  double complex* eps = (double complex*)malloc(sizeof(double complex) * n2);
+  int rc = 0;
+  for (int iw = 0; iw < nw; ++iw) {
+    const double complex* c = chi + n2 * iw;
+    double fac = (nspin == 1) ? 2.0 : 1.0; /* :299-303 */
+    for (long e = 0; e < n2; ++e) x[e] = fac * c[e];
+    if (is_x0_m_basis) { /* :336-337 */
+      orc_zgemm('N', 'N', ngb, ngb, ngb, 1.0, x, ngb, m2e, ngb, 0.0, t, ngb);
+      orc_zgemm('C', 'N', ngb, ngb, ngb, 1.0, m2e, ngb, t, ngb, 0.0, x, ngb);
+    }
+    for (int j = 0; j < ngb; ++j) /* :341-348 */
+      for (int i = 0; i < ngb; ++i) {
+        double complex e = -vcousq[i] * x[i + (long)ngb * j] * vcousq[j];
+        if (i == j) e = 1.0 + e;
+        eps[i + (long)ngb * j] = e;
+      }
+    if (orc_zinv(ngb, eps, ngb) != 0) rc = 2; /* :350 */
+    double complex* w = wc + (long)ldw * ngb * iw;
+    for (int j = 0; j < ngb; ++j) /* :353-360 */
+      for (int i = 0; i < ngb; ++i) {
+        double complex e = vcousq[i] * eps[i + (long)ngb * j] * vcousq[j];
+        if (i == j) e -= vcousq[i] * vcousq[j];
+        t[i + (long)ngb * j] = e;
+      }
+    if (is_wc_m_basis) { /* :366-367 */
+      orc_zgemm('N', 'C', ngb, ngb, ngb, 1.0, t, ngb, m2e, ngb, 0.0, x, ngb);
+      orc_zgemm('N', 'N', ngb, ngb, ngb, 1.0, m2e, ngb, x, ngb, 0.0, t, ngb);
+    }
+    for (int j = 0; j < ngb; ++j)
+      for (int i = 0; i < ngb; ++i) w[i + (long)ldw * j] = t[i + (long)ngb * j];
+  }
+  free(x);
+  free(t);
+  free(eps);
+  return rc;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Imaginary-axis weights (S/m_sxcf_gemm.f90:432-458). */
+void orc_wgtim(int niw, int ns1, int ns2, int ntq, int nctot, const double* omega,
+               const double* ekc, double esmr, double ua, double wkkr, const double* x,
+               const double* wt, const double* expa, double* wgtim) {
+  const double rmax = 2.0;
+  int nm = ns2 - ns1 + 1;
+  double sig = 0.5 * esmr;
+  double sig2 = 2.0 * sig * sig;
+  double* w_ = (double*)malloc(sizeof(double) * (niw + 1));
+  for (int itp = 0; itp < ntq; ++itp)
+    for (int it = ns1; it <= ns2; ++it) {
+      double we = 0.5 * (omega[itp] - ekc[it - ns1]);
+      double aw = fabs(ua * we);
+      double aw2 = aw * aw;
+      double sgn = signbit(we) ? -1.0 : 1.0;
+      if (it <= nctot) {
+        double s = 0.0;
+        for (int i = 0; i < niw; ++i) {
+          double cons = 1.0 / (we * we * x[i] * x[i] + (1.0 - x[i]) * (1.0 - x[i]));
+          w_[1 + i] = we * cons * wt[i] * (-1.0 / PI);
+          s += w_[1 + i] * expa[i];
+        }
+        w_[0] = (fabs(we) < rmax / ua) ? (-s - 0.5 * sgn * exp(we * we * ua * ua) * erfc(ua * fabs(we))) : 0.0;
+      } else {
+        double s = 0.0;
+        for (int i = 0; i < niw; ++i) {
+          double omd = 1.0 / x[i] - 1.0;
+          double omd2w = omd * omd + we * we;
+          double r = omd2w / sig2, cons;
+          if (r > 5e-3)
+            cons = (1.0 - exp(-r)) / omd2w;
+          else
+            cons = (1.0 / sig2 - omd2w / (sig2 * sig2) / 2.0 + omd2w * omd2w / (sig2 * sig2 * sig2) / 6.0 -
+                    omd2w * omd2w * omd2w / pow(sig2, 4) / 24.0 + pow(omd2w, 4) / pow(sig2, 5) / 120.0 -
+                    pow(omd2w, 5) / pow(sig2, 6) / 720.0);
+          w_[1 + i] = -we * cons * wt[i] / (x[i] * x[i]) / PI;
+          s += cons * expa[i] * wt[i] / (x[i] * x[i]);
+        }
+        w_[0] = we * s / PI + sgn * 0.5 * exp(aw2) * (erfc(sqrt(aw2 + we * we / sig2)) - erfc(aw));
+      }
+      double* out = wgtim + (long)(niw + 1) * ((it - ns1) + (long)nm * itp);
+      for (int i = 0; i <= niw; ++i) out[i] = wkkr * w_[i];
+    }
+  free(w_);
+}
+
+/* Imaginary-axis Sigma_c + final GEMM + clip (S/m_sxcf_gemm.f90:463-496, 599-608). */
+void orc_sigc_imag(int ngb, int nm, int ntq, int niw, const double complex* zmel,
+                   const double complex* wc, int ldw, const double* wgtim,
+                   double complex* zsec, int ldz) {
+  long nc = (long)nm * ntq;
+  long n2 = (long)ngb * ngb;
+  double complex* w = (double complex*)malloc(sizeof(double complex) * n2);
+  double complex* wz = (double complex*)malloc(sizeof(double complex) * ngb * nc);
+  double complex* cz = (double complex*)malloc(sizeof(double complex) * ngb * nc);
+  for (int iw = 0; iw <= niw; ++iw) {
+    const double complex* src = wc + (long)ldw * ngb * iw;
+    /* keepwv unpack: wc(i,j)=upper(i,j), wc(j,i)=conj(upper(i,j)) for i<=j (:468-475) */
+    for (int j = 0; j < ngb; ++j)
+      for (int i = 0; i <= j; ++i) {
+        double complex u = src[i + (long)ldw * j];
+        w[i + (long)ngb * j] = u;
+        w[j + (long)ngb * i] = conj(u);
+      }
+    /* wzmel = wgtim(iw,it,itp) * zmel (:482-488) */
+    for (long c = 0; c < nc; ++c) {
+      double g = wgtim[iw + (long)(niw + 1) * c];
+      for (int i = 0; i < ngb; ++i) wz[i + (long)ngb * c] = g * zmel[i + (long)ngb * c];
+    }
+    /* czmelwc (+)= wc^H wzmel (:492) */
+    orc_zgemm('C', 'N', ngb, (int)nc, ngb, 1.0, w, ngb, wz, ngb, iw == 0 ? 0.0 : 1.0, cz, ngb);
+  }
+  /* zsec += czmelwc^H zmel, K = ngb*nm (:599) */
+  orc_zgemm('C', 'N', ntq, ntq, ngb * nm, 1.0, cz, (long)ngb * nm, zmel, (long)ngb * nm, 1.0, zsec, ldz);
+  /* Im(zsec_nn) <= 0 (:601-608) */
+  for (int i = 0; i < ntq; ++i) {
+    double complex z = zsec[i + (long)ldz * i];
+    double zi = cimag(z);
+    if (zi > 0.0) zi = 0.0;
+    zsec[i + (long)ldz * i] = creal(z) + I * zi;
+  }
+  free(w);
+  free(wz);
+  free(cz);
+}
+
+/* wfacx2 / weavx2 (S/wfacx.f90:19-49). */
+double orc_wfacx2(double e1, double e2, double ek, double esmr) {
+  const double ewidthcut = 1e-6;
+  double el = fmin(e1, e2), eh = fmax(e1, e2);
+  if (eh - el < ewidthcut) return 0.0;
+  if (esmr == 0.0) return (el <= ek && ek < eh) ? 1.0 : 0.0;
+  return 0.5 * erfc(-(eh - ek) / sqrt(2.0) / esmr) - 0.5 * erfc(-(el - ek) / sqrt(2.0) / esmr);
+}
+
+double orc_weavx2(double e1, double e2, double ek, double esmr) {
+  const double pi = 3.1415926535897932;
+  double el = fmin(e1, e2), eh = fmax(e1, e2);
+  double wtt = 0.5 * erfc(-(eh - ek) / sqrt(2.0) / esmr) - 0.5 * erfc(-(el - ek) / sqrt(2.0) / esmr);
+  double sig2 = 2.0 * esmr * esmr;
+  return ek + esmr / sqrt(2.0 * pi) * (-exp(-(eh - ek) * (eh - ek) / sig2) + exp(-(el - ek) * (el - ek) / sig2)) / wtt;
+}
+
+/* ---------------------------------------------------------------------- */
+/* QP energy (S/main_hqpe.sc.f90:152-161); hartree = 2*rydberg(). */
+void orc_qp_energy(int ntq, const double* elda, const double* sex, const double* sexcore,
+                   const double complex* sec, int ldsec, const double* vxc, double* eqp) {
+  const double hartree = 2.0 * 13.605693122994;
+  for (int i = 0; i < ntq; ++i) {
+    double rsec = creal(sec[i + (long)ldsec * i]) * hartree;
+    double eshift = sex[i] * hartree + sexcore[i] * hartree + rsec - vxc[i];
+    eqp[i] = elda[i] + eshift;
+  }
+}
+
+/* Hermitised Sigma - Vxc (S/main_hqpe.sc.f90:175-181). */
+void orc_sigma_herm(int ntq, const double complex* sex2, const double complex* sexcore2,
+                    const double complex* sec, const double complex* vxcmat, double complex* se) {
+  for (int j = 0; j < ntq; ++j)
+    for (int i = 0; i < ntq; ++i) {
+      long ij = i + (long)ntq * j, ji = j + (long)ntq * i;
+      se[ij] = sex2[ij] + sexcore2[ij] + 0.5 * (sec[ij] + conj(sec[ji])) - vxcmat[ij];
+    }
+}

@@ -1,0 +1,142 @@
+/*
+ * oracle.h -- FP64 CPU restatement of ecalj's QSGW chi0 -> W -> Sigma_c path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in ecalj_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it.  The product library never links or
+ * calls it.
+ *
+ * Every function restates one reference routine (file:line under the ecalj
+ * source tree, SRC/subroutines = S/).  Arrays are Fortran column-major;
+ * complex arrays are interleaved (re, im) doubles (C99 double complex).
+ * Band and bin indices stored inside index arrays are 1-based, as in Fortran.
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   orc_gauss          numpy.polynomial.legendre.leggauss, exact polynomial
+ *                      integration; the reference's 15-digit table values
+ *   orc_frhis          closed form bb*(exp(aa*(i-1))-1)
+ *   orc_hilbertmat     oracle/_ref (S/hilbertmat.f90 compiled with flang),
+ *                      analytic Hilbert transform of piecewise-linear f
+ *   orc_wfacx2/weavx2  oracle/_ref (S/wfacx.f90 compiled with flang)
+ *   orc_zmel           numpy einsum on tiny inputs; unitary M->E invariance
+ *   orc_x0gemm         brute-force numpy sum over pairs; Hermiticity
+ *   orc_dpsion_imag    numpy; delta-pair limit -2D/(D^2+nu^2)
+ *   orc_chi0_freq      equals x0gemm+dpsion with composed weights
+ *   orc_wc_build       numpy.linalg.inv textbook W; Dyson residual
+ *   orc_wgtim          scipy.integrate.quad of the smeared-Lorentzian integral
+ *   orc_sigc_imag      numpy GEMM chain; Im(Sigma_nn) clip
+ *   orc_qp_energy      closed form of S/main_hqpe.sc.f90:160-161
+ */
+#ifndef ECALJ_ORACLE_H
+#define ECALJ_ORACLE_H
+#include <complex.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* BLAS-semantics complex GEMM, column-major, op in {'N','T','C'};
+ * stands for zmm_h (S/m_blas.f90:205-232). */
+void orc_zgemm(char opa, char opb, int m, int n, int k, double complex alpha,
+               const double complex* a, long lda, const double complex* b, long ldb,
+               double complex beta, double complex* c, long ldc);
+
+/* gauss (S/mate.f90:1-206): n-point Gauss-Legendre on [a,b]. Returns 0 or -1
+ * when n is not one of the reference's supported sizes. */
+int orc_gauss(int n, double a, double b, double* x, double* w);
+
+/* freq01 (S/m_freq.f90:164-186) + readfreq_r (S/m_readfreq_r.f90:41-50). */
+int orc_freq01(int nx, double ua, double* freqx, double* freqw, double* wx, double* expa);
+
+/* Histogram bins (S/m_freq.f90:95-108): returns nwhis; fills frhis[0..nwhis]
+ * when frhis != NULL (frhis(1:nwhis+1) in Fortran). */
+int orc_frhis(double oratio, double dw, double omg2max, double* frhis);
+
+/* hilbertmat (S/hilbertmat.f90:14-79). rmat has 2*nwhis+1 entries for
+ * indices -nwhis..nwhis; his_* likewise. */
+void orc_hilbertmat(double complex zz, int nwhis, const double* his_L, const double* his_C,
+                    const double* his_R, double complex* rmat);
+
+/* dpsion_init, imaginary branch with npm=1 (S/dpsion5.f90:84-89,121-129).
+ * imatt is niw x nwhis column-major. */
+void orc_imatt(int niw, const double* freqi, int nwhis, const double* frhis, double* imatt);
+
+/* zmel, MT valence-valence part with identity rotation
+ * (S/m_zmel.f90:146-597: :323-371 MT contraction, :546-563 M->E,
+ * :590-594 conj).  Atom ia owns basis rows [ias, ias+nv[ia]) and product
+ * basis rows [ims, ims+mdim[ia]) (running sums).  ppb holds the per-atom real
+ * blocks ppb(i1,i2,I) of size nv x nv x mdim, concatenated.  When m2e is
+ * non-NULL (is_m_basis = false) zmel = m2e^H zmelt with m2e ngb x nbb.
+ * zmel out: (m2e ? nbb : ngb) x nm x nq. */
+int orc_zmel(int natom, const int* nv, const int* mdim, const double complex* phase,
+             const double* ppb, int ndima, int nm, const double complex* cphim, int nq,
+             const double complex* cphiq, const double complex* m2e, int nbb, int conjg,
+             double complex* zmel);
+
+/* x0gemm (S/x0gemm.f90:2-97) with its module inputs explicit.
+ * rcxq: npr x npr_col x nslice, slice s <-> Fortran index s + (1-npm)*nwhis
+ *   (npm=1: slices 0..nwhis; npm=2: -nwhis..nwhis).
+ * zmel: nbb x (ns2-ns1+1) x nqtot (Fortran zmel(nbb,ns1:ns2,nqtot)).
+ * icoun range [icoun_lo, icoun_hi] is 1-based into itc/itpc/jpmc/iwini/iwend/icouini. */
+void orc_x0gemm(double complex* rcxq, int npr, int ipr_col, int npr_col, int nwhis, int npm,
+                int ns1, int ns2, const double complex* zmel, int nbb, int nqtot,
+                int icoun_lo, int icoun_hi, const int* iwini, const int* iwend, const int* itc,
+                const int* itpc, const int* jpmc, const int* icouini, const double* whwc);
+
+/* dpsion_chiq, imaginary axis with npm=1 (S/dpsion5.f90:211-232): scales rcxq
+ * bins 1..nwhis in place by -wfac/(R-L) and sets
+ * zxqi(:,:,iv) = sum_iw imatt(iv,iw) rcxq(:,:,iw).  rcxq has nwhis+1 slices
+ * (slice 0 unused); zxqi npr x npr_col x niw. */
+void orc_dpsion_imag(int npr, int npr_col, int nwhis, int niw, const double* frhis,
+                     const double* imatt, double ecut, double complex* rcxq, double complex* zxqi);
+
+/* Composed direct-frequency chi0: chi(I,J,v) = sum_p w(v,p) conj(Z(I,p)) Z(J,p),
+ * i.e. x0gemm followed by dpsion_chiq with w = sum_bins whw*scale*imatt. */
+void orc_chi0_freq(int ni, int npairs, int nf, const double complex* z, const double* w,
+                   double complex* chi);
+
+/* WVIllwI body for q /= 0 (S/m_llw.f90:299-373) with zminv_h
+ * (S/m_lapack.f90:21-47).  chi: ngb x ngb x nw (M basis if is_x0_m_basis),
+ * vcousq = sqrt(vcoud) (S/m_readVcoud.f90:40), m2e = zcousq ngb x ngb.
+ * wc out: ldw x ngb x nw.  Returns 0, or 2 if an epsilon matrix is singular. */
+int orc_wc_build(int ngb, int nw, int nspin, int is_x0_m_basis, int is_wc_m_basis,
+                 const double complex* chi, const double* vcousq, const double complex* m2e,
+                 double complex* wc, int ldw);
+
+/* In-place general complex inverse (getrf + getri semantics). Returns 0/2. */
+int orc_zinv(int n, double complex* a, long lda);
+
+/* Imaginary-axis Sigma_c weights (S/m_sxcf_gemm.f90:432-458).
+ * omega[ntq] and ekc[ns1-1 .. ns2-1] (0-based storage of Fortran ekc(ns1:ns2))
+ * in Ry; esmr in Ry; x,wt: Gauss points/weights (niw); expa: exp(-ua^2 w^2).
+ * wgtim out: (niw+1) x (ns2-ns1+1) x ntq, already multiplied by wkkr. */
+void orc_wgtim(int niw, int ns1, int ns2, int ntq, int nctot, const double* omega,
+               const double* ekc, double esmr, double ua, double wkkr, const double* x,
+               const double* wt, const double* expa, double* wgtim);
+
+/* Imaginary-axis Sigma_c and final contraction (S/m_sxcf_gemm.f90:463-496,
+ * 599-608).  zmel: ngb x nm x ntq (M basis, not conjugated).
+ * wc: ldw x ngb x (niw+1); only its upper triangle is used, as the keepwv
+ * packing does (S/m_sxcf_gemm.f90:337-344,468-475).
+ * zsec: ldz x ntq, accumulated (beta = 1), then Im(zsec_nn) clipped to <= 0. */
+void orc_sigc_imag(int ngb, int nm, int ntq, int niw, const double complex* zmel,
+                   const double complex* wc, int ldw, const double* wgtim,
+                   double complex* zsec, int ldz);
+
+/* wfacx2 / weavx2 (S/wfacx.f90:19-49). */
+double orc_wfacx2(double e1, double e2, double ek, double esmr);
+double orc_weavx2(double e1, double e2, double ek, double esmr);
+
+/* QP energy (S/main_hqpe.sc.f90:152-161): eqp = elda + (sex+sexcore)*hartree
+ * + Re(sec_nn)*hartree - vxc.  Energies in eV, self-energies in Hartree. */
+void orc_qp_energy(int ntq, const double* elda, const double* sex, const double* sexcore,
+                   const double complex* sec, int ldsec, const double* vxc, double* eqp);
+
+/* Hermitised Sigma (S/main_hqpe.sc.f90:175-181, Hartree):
+ * se = sex2 + sexcore2 + (sec + sec^H)/2 - vxcmat, all ntq x ntq. */
+void orc_sigma_herm(int ntq, const double complex* sex2, const double complex* sexcore2,
+                    const double complex* sec, const double complex* vxcmat, double complex* se);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,89 @@
+"""One q-point of the chi0 -> W -> Sigma_c path, composed from the oracle.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).  The composition mirrors
+the reference drivers:
+  hrcxq (S/main_hrcxq.f90:101-120): x0kf_zxq -> WVIllwI for q /= 0;
+  x0kf_zxq (S/x0kf_v4h.f90:208-319): per k zmel (conj) -> x0gemm, then
+      dpsion_chiq on the imaginary axis;
+  hsfp0_sc --job=2 (S/m_sxcf_gemm.f90:305-608): per internal k zmel (M basis,
+      no conj) -> imaginary-axis weights -> Sigma_c contraction;
+  hqpe_sc (S/main_hqpe.sc.f90:160-161): QP energies.
+The frequency list for W and Sigma_c is {0, nu_1..nu_nf}: W_c(0) is the
+real-axis omega=0 record that Sigma_c reads as iw=0 (S/m_sxcf_gemm.f90:477),
+and at omega=0 the real-axis Hilbert weights equal the imaginary-axis ones
+(S/dpsion5.f90:101 vs :125, both hilbertmat at zz = 0 + i*epsz).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+import oracle as orc
+from ecalj_amd import synth
+
+
+@dataclass
+class QPResult:
+    chi: np.ndarray      # ni x ni x (nf+1), M basis, before the nspin factor
+    wc: np.ndarray       # ni x ni x (nf+1), W_c = W - v, M basis
+    zsec: np.ndarray     # ntq x ntq, Hartree
+    eqp: np.ndarray      # ntq, eV
+    nwhis: int
+
+
+def frequencies(cfg):
+    freqx, freqw, wx, expa = orc.freq01(cfg.nf, cfg.ua)
+    nu = np.concatenate([[0.0], freqw])
+    return freqx, freqw, wx, expa, nu
+
+
+def chi0(qp: synth.QPoint, k_subset=None):
+    cfg = qp.cfg
+    fr = synth.frhis(qp)
+    nwhis = len(fr) - 1
+    pw = synth.pair_weights(qp, fr)
+    ix = synth.x0kf_index(pw)
+    _, _, _, _, nu = frequencies(cfg)
+    imatt = orc.imatt(nu, fr)
+    ni, no = cfg.ni, cfg.no
+    rcxq = np.zeros((ni, ni, nwhis + 1), np.complex128, order="F")
+    ks = range(cfg.nk) if k_subset is None else k_subset
+    for k in ks:
+        z = orc.zmel([cfg.nb], [ni], [1.0], qp.ppb, qp.c_k[:, :no, k], qp.c_kq[:, no:, k], conjg=True)
+        orc.x0gemm(rcxq, ni, 1, ni, nwhis, 1, int(ix.nkmin[k]), int(ix.nkmax[k]), z,
+                   int(ix.icounkmin[k]), int(ix.icounkmax[k]), ix.iwini, ix.iwend, ix.itc, ix.itpc,
+                   ix.jpmc, ix.icouini, ix.whwc)
+    chi = orc.dpsion_imag(rcxq, fr, imatt)
+    return chi, nwhis
+
+
+def wc_k_list(qp: synth.QPoint, wc: np.ndarray):
+    """W_c at each internal k: D_k W_c(q) D_k^H (synth.wc_phases)."""
+    ph = synth.wc_phases(qp.cfg)
+    return [np.asfortranarray(ph[:, k][:, None, None] * wc * ph[:, k].conj()[None, :, None])
+            for k in range(qp.cfg.nk)]
+
+
+def sigc(qp: synth.QPoint, wck, k_subset=None):
+    cfg = qp.cfg
+    freqx, _, wx, expa, _ = frequencies(cfg)
+    ntq = cfg.nq
+    zsec = np.zeros((ntq, ntq), np.complex128, order="F")
+    omega = qp.e_q[:ntq] / synth.RYDBERG_EV
+    ks = range(cfg.nk) if k_subset is None else k_subset
+    for k in ks:
+        z = orc.zmel([cfg.nb], [cfg.ni], [1.0], qp.ppb, qp.c_qmk[:, :, k], qp.c_q[:, :ntq], conjg=False)
+        g = orc.wgtim(omega, qp.e_qmk[:, k] / synth.RYDBERG_EV, cfg.esmr, qp.wk[k], freqx, wx, expa, ua=cfg.ua)
+        zsec = orc.sigc_imag(z, wck[k], g, zsec)
+    return zsec
+
+
+def run(qp: synth.QPoint) -> QPResult:
+    cfg = qp.cfg
+    chi, nwhis = chi0(qp)
+    wc = orc.wc_build(chi, qp.vcousq, qp.zcousq, nspin=cfg.nspin)
+    zsec = sigc(qp, wc_k_list(qp, wc))
+    sex, sexcore, vxc = synth.sigma_inputs(qp)
+    eqp = orc.qp_energy(qp.e_q[:cfg.nq], sex, sexcore, zsec, vxc)
+    return QPResult(chi, wc, zsec, eqp, nwhis)
