@@ -1,0 +1,374 @@
+"""ecalj_amd -- MI355X (gfx950) implementation of ecalj's QSGW chi0 -> W -> Sigma_c path.
+
+The compute lives in ``libecalj_gpu.so`` (HIP kernels behind the C ABI of
+``include/ecalj_gpu.h``).  This module is the host-side mirror of the
+reference's interface for the path: a handle holding the module state that
+ecalj keeps in Fortran modules (ppb of m_zmel, the Coulomb eigenbasis of
+m_readVcoud) and one method per entry point, with the reference's argument
+meaning.  Host methods take numpy arrays (Fortran order); ``*_dev`` methods
+take ``DevArray`` buffers allocated through the library, so that the process
+uses the library's HIP runtime only.  There is no CPU fallback: if the
+library is missing or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libecalj_gpu.so")
+_LIB = None
+
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+KCLASS = {"sigc": 0, "chi0": 1, "zmel": 2, "wbuild": 3, "hilbert": 4}
+
+
+class EcaljError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        msg = lib().ecalj_status_string(status).decode()
+        super().__init__(f"{where}: {msg} (status {status})")
+        self.status = status
+
+
+# (name, argtypes); every entry point returns int status
+_i, _l, _d, _p = C.c_int, C.c_int64, C.c_double, C.c_void_p
+_SIGS = {
+    "ecalj_create": [C.POINTER(C.c_void_p), _i],
+    "ecalj_destroy": [_p],
+    "ecalj_set_precision": [_p, _i],
+    "ecalj_set_stream": [_p, _p],
+    "ecalj_sync": [_p],
+    "ecalj_kernel_timing": [_p, _i],
+    "ecalj_kernel_time": [_p, _i, C.POINTER(_d), C.POINTER(_l)],
+    "ecalj_set_ppb": [_p, _i, _p, _p, _p],
+    "ecalj_set_coulomb": [_p, _i, _p, _p],
+    "ecalj_zmel": [_p, _p, _i, _i, _p, _i, _p, _i, _i, _i, _p],
+    "ecalj_zmel_dev": [_p, _p, _i, _i, _p, _i, _p, _i, _i, _i, _p],
+    "ecalj_zmel_strided_dev": [_p, _p, _i, _i, _p, _l, _i, _p, _l, _i, _i, _i, _p],
+    "ecalj_x0gemm": [_p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p],
+    "ecalj_x0gemm_batched_dev": [_p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p, _l, _i, _i, _p, _p, _i, _p, _p,
+                                 _p, _p, _p, _p, _i, _p],
+    "ecalj_dpsion_imag": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
+    "ecalj_dpsion_imag_dev": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
+    "ecalj_chi0_freq": [_p, _p, _p, _i, _i, _i, _p],
+    "ecalj_chi0_freq_dev": [_p, _p, _p, _i, _i, _i, _p],
+    "ecalj_wc_build": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "ecalj_wc_build_dev": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "ecalj_sigc_weights": [_p, _i, _i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p, _p, _p],
+    "ecalj_sigc_weights_dev": [_p, _i, _i, _i, _i, _i, _i, _p, _p, _d, _d, _p, _p, _p, _p, _p],
+    "ecalj_sigc": [_p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
+    "ecalj_sigc_dev": [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
+    "ecalj_qp_energy": [_i, _p, _p, _p, _p, _i, _p, _p],
+    "ecalj_dev_alloc": [_p, C.c_size_t, C.POINTER(C.c_void_p)],
+    "ecalj_dev_free": [_p, _p],
+    "ecalj_memcpy_h2d": [_p, _p, _p, C.c_size_t],
+    "ecalj_memcpy_d2h": [_p, _p, _p, C.c_size_t],
+    "ecalj_memcpy_d2d": [_p, _p, _p, C.c_size_t],
+    "ecalj_dev_zero": [_p, _p, C.c_size_t],
+    "ecalj_dev_phase_similarity": [_p, _p, _i, _i, _p, _i, _p],
+    "ecalj_sigma_herm": [_i, _p, _p, _p, _p, _p],
+}
+
+
+def lib():
+    """Load libecalj_gpu.so; raises if it was not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run python -m ecalj_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.ecalj_status_string.argtypes = [C.c_int]
+        L.ecalj_status_string.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGS) + ["ecalj_status_string"]
+
+
+def _chk(status, where):
+    if status != 0:
+        raise EcaljError(status, where)
+
+
+def _np_ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _t_ptr(t):
+    """Device pointer of a DevArray (or a raw integer address)."""
+    if t is None:
+        return None
+    if isinstance(t, DevArray):
+        return C.c_void_p(t.ptr)
+    return C.c_void_p(int(t))
+
+
+class DevArray:
+    """A device buffer owned by a Handle; shape is Fortran-ordered."""
+
+    def __init__(self, h: "Handle", shape, dtype):
+        self.h = h
+        self.shape = tuple(int(x) for x in (shape if np.iterable(shape) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _chk(lib().ecalj_dev_alloc(h.ptr, self.nbytes, C.byref(p)), "dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, h, a, dtype=None):
+        a = np.asfortranarray(a if dtype is None else np.asarray(a, dtype))
+        d = cls(h, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.asfortranarray(np.asarray(a, self.dtype))
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        _chk(lib().ecalj_memcpy_h2d(self.h.ptr, C.c_void_p(self.ptr), _np_ptr(a), self.nbytes), "memcpy_h2d")
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype, order="F")
+        _chk(lib().ecalj_memcpy_d2h(self.h.ptr, _np_ptr(out), C.c_void_p(self.ptr), self.nbytes), "memcpy_d2h")
+        return out
+
+    def zero(self):
+        _chk(lib().ecalj_dev_zero(self.h.ptr, C.c_void_p(self.ptr), self.nbytes), "dev_zero")
+
+    def free(self):
+        if self.ptr:
+            lib().ecalj_dev_free(self.h.ptr, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.h.ptr:
+                self.free()
+        except Exception:
+            pass
+
+
+class Handle:
+    """One library handle (device binding, stream, module state, scratch)."""
+
+    def __init__(self, device: int = 0, precision: str = "f64"):
+        self._h = C.c_void_p()
+        _chk(lib().ecalj_create(C.byref(self._h), device), "ecalj_create")
+        self.set_precision(precision)
+
+    def close(self):
+        if self._h:
+            lib().ecalj_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def array(self, shape, dtype=np.complex128):
+        return DevArray(self, shape, dtype)
+
+    def to_dev(self, a, dtype=None):
+        return DevArray.from_host(self, a, dtype)
+
+    def phase_similarity(self, a, ngb, nmat, d, nk, out):
+        _chk(lib().ecalj_dev_phase_similarity(self._h, _t_ptr(a), ngb, nmat, _t_ptr(d), nk, _t_ptr(out)),
+             "dev_phase_similarity")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ptr(self):
+        return self._h
+
+    # ---- state
+    def set_precision(self, precision: str):
+        self.precision = precision
+        _chk(lib().ecalj_set_precision(self._h, PREC[precision]), "set_precision")
+
+    def set_stream(self, stream_ptr: int | None):
+        _chk(lib().ecalj_set_stream(self._h, C.c_void_p(stream_ptr) if stream_ptr else None), "set_stream")
+
+    def sync(self):
+        _chk(lib().ecalj_sync(self._h), "sync")
+
+    def kernel_timing(self, on: bool):
+        _chk(lib().ecalj_kernel_timing(self._h, int(on)), "kernel_timing")
+
+    def kernel_time(self, kclass: str):
+        ms = C.c_double()
+        n = C.c_int64()
+        _chk(lib().ecalj_kernel_time(self._h, KCLASS[kclass], C.byref(ms), C.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+    def set_ppb(self, nv, mdim, ppb):
+        """set_ppb (S/m_zmel.f90:110-145): per-atom real blocks nv x nv x mdim."""
+        nv, mdim = _i32(nv), _i32(mdim)
+        flat = np.ascontiguousarray(np.asarray(ppb, np.float64).ravel(order="F"))
+        assert flat.size == int((nv.astype(np.int64) ** 2 * mdim).sum())
+        _chk(lib().ecalj_set_ppb(self._h, len(nv), _np_ptr(nv), _np_ptr(mdim), _np_ptr(flat)), "set_ppb")
+        self.ngb = int(mdim.sum())
+        self.natom = len(nv)
+
+    def set_coulomb(self, vcoud, zcousq):
+        """Readvcoud + set_m2e_prod_basis (S/m_readVcoud.f90:11-43)."""
+        v = np.ascontiguousarray(np.asarray(vcoud, np.float64))
+        z = _f(zcousq, np.complex128)
+        _chk(lib().ecalj_set_coulomb(self._h, len(v), _np_ptr(v), _np_ptr(z)), "set_coulomb")
+
+    # ---- zmel
+    def _phase(self, phase):
+        if phase is None:
+            phase = np.ones(self.natom, np.complex128)
+        return np.ascontiguousarray(np.asarray(phase, np.complex128))
+
+    def zmel(self, cphim, cphiq, phase=None, is_m_basis=True, zmelconjg=False):
+        """get_zmel_init_gemm (S/m_zmel.f90:146-597); cphim ndima x nm (x nbatch)."""
+        cm, cq = _f(cphim, np.complex128), _f(cphiq, np.complex128)
+        nb = 1 if cm.ndim == 2 else cm.shape[2]
+        ndima, nm, nq = cm.shape[0], cm.shape[1], cq.shape[1]
+        out = np.zeros((self.ngb, nm, nq) + ((nb,) if cm.ndim == 3 else ()), np.complex128, order="F")
+        ph = self._phase(phase)
+        _chk(lib().ecalj_zmel(self._h, _np_ptr(ph), ndima, nm, _np_ptr(cm), nq, _np_ptr(cq), int(is_m_basis),
+                              int(zmelconjg), nb, _np_ptr(out)), "zmel")
+        return out
+
+    def zmel_dev(self, cphim, cphiq, out, nm, nq, ndima, nbatch=1, phase=None, is_m_basis=True, zmelconjg=False,
+                 cphim_stride=None, cphiq_stride=None):
+        ph = self._phase(phase)
+        sm = ndima * nm if cphim_stride is None else cphim_stride
+        sq = ndima * nq if cphiq_stride is None else cphiq_stride
+        _chk(lib().ecalj_zmel_strided_dev(self._h, _np_ptr(ph), ndima, nm, _t_ptr(cphim), sm, nq, _t_ptr(cphiq),
+                                          sq, int(is_m_basis), int(zmelconjg), nbatch, _t_ptr(out)), "zmel_dev")
+        self._keep = ph
+
+    # ---- chi0
+    def x0gemm(self, rcxq, npr, ipr_col, npr_col, nwhis, npm, ns1, ns2, zmel, icoun_lo, icoun_hi, iwini, iwend,
+               itc, itpc, jpmc, icouini, whwc):
+        """x0gemm (S/x0gemm.f90:2-97); accumulates into rcxq (Fortran complex array) in place."""
+        assert rcxq.dtype == np.complex128 and rcxq.flags.f_contiguous
+        z = _f(zmel, np.complex128)
+        nbb, _, nqtot = z.shape
+        arrs = [_i32(a) for a in (iwini, iwend, itc, itpc, jpmc, icouini)]
+        w = np.ascontiguousarray(np.asarray(whwc, np.float64))
+        _chk(lib().ecalj_x0gemm(self._h, _np_ptr(rcxq), npr, ipr_col, npr_col, nwhis, npm, ns1, ns2, _np_ptr(z),
+                                nbb, nqtot, icoun_lo, icoun_hi, *[_np_ptr(a) for a in arrs], _np_ptr(w)), "x0gemm")
+        return rcxq
+
+    def x0gemm_batched_dev(self, rcxq, npr, nwhis, nk, ns1, ns2, zmel, zmel_stride, nbb, nqtot, ix,
+                           ipr_col=1, npr_col=None, npm=1):
+        npr_col = npr if npr_col is None else npr_col
+        ns1, ns2 = _i32(ns1), _i32(ns2)
+        arrs = [_i32(a) for a in (ix.icounkmin, ix.icounkmax)]
+        lists = [_i32(a) for a in (ix.iwini, ix.iwend, ix.itc, ix.itpc, ix.jpmc, ix.icouini)]
+        w = np.ascontiguousarray(ix.whwc, np.float64)
+        _chk(lib().ecalj_x0gemm_batched_dev(self._h, _t_ptr(rcxq), npr, ipr_col, npr_col, nwhis, npm, nk,
+                                            _np_ptr(ns1), _np_ptr(ns2), _t_ptr(zmel), zmel_stride, nbb, nqtot,
+                                            _np_ptr(arrs[0]), _np_ptr(arrs[1]), len(ix.itc),
+                                            *[_np_ptr(a) for a in lists], len(w), _np_ptr(w)), "x0gemm_batched_dev")
+
+    def dpsion_imag(self, rcxq, frhis, freqi, ecut=1e10, scale_rcxq=True):
+        """dpsion_chiq, imaginary axis (S/dpsion5.f90:211-232); returns zxqi, scales rcxq in place."""
+        assert rcxq.dtype == np.complex128 and rcxq.flags.f_contiguous
+        npr, npr_col, ns = rcxq.shape
+        fr = np.ascontiguousarray(frhis, np.float64)
+        fi = np.ascontiguousarray(freqi, np.float64)
+        zx = np.zeros((npr, npr_col, len(fi)), np.complex128, order="F")
+        _chk(lib().ecalj_dpsion_imag(self._h, _np_ptr(rcxq), npr, npr_col, ns - 1, len(fi), _np_ptr(fr), _np_ptr(fi),
+                                     ecut, int(scale_rcxq), _np_ptr(zx)), "dpsion_imag")
+        return zx
+
+    def dpsion_imag_dev(self, rcxq, npr, npr_col, nwhis, frhis, freqi, zxqi, ecut=1e10, scale_rcxq=False):
+        fr = np.ascontiguousarray(frhis, np.float64)
+        fi = np.ascontiguousarray(freqi, np.float64)
+        _chk(lib().ecalj_dpsion_imag_dev(self._h, _t_ptr(rcxq), npr, npr_col, nwhis, len(fi), _np_ptr(fr),
+                                         _np_ptr(fi), ecut, int(scale_rcxq), _t_ptr(zxqi)), "dpsion_imag_dev")
+
+    def chi0_freq(self, z, w):
+        z, w = _f(z, np.complex128), _f(w, np.float64)
+        ni, npairs = z.shape
+        nf = w.shape[0]
+        chi = np.zeros((ni, ni, nf), np.complex128, order="F")
+        _chk(lib().ecalj_chi0_freq(self._h, _np_ptr(z), _np_ptr(w), ni, npairs, nf, _np_ptr(chi)), "chi0_freq")
+        return chi
+
+    # ---- W
+    def wc_build(self, chi, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True):
+        """WVIllwI body for q /= 0 (S/m_llw.f90:299-373)."""
+        c = _f(chi, np.complex128)
+        n, _, nw = c.shape
+        wc = np.zeros((n, n, nw), np.complex128, order="F")
+        _chk(lib().ecalj_wc_build(self._h, _np_ptr(c), n, nw, nspin, int(is_x0_m_basis), int(is_wc_m_basis),
+                                  _np_ptr(wc), n), "wc_build")
+        return wc
+
+    def wc_build_dev(self, chi, wc, ngb, nw, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True, ldw=None):
+        _chk(lib().ecalj_wc_build_dev(self._h, _t_ptr(chi), ngb, nw, nspin, int(is_x0_m_basis), int(is_wc_m_basis),
+                                      _t_ptr(wc), ngb if ldw is None else ldw), "wc_build_dev")
+
+    # ---- Sigma_c
+    def sigc_weights(self, omega, ekc, esmr, wkkr, freqx, wwx, expa, ua=1.0, ns1=1, nctot=0):
+        """Imaginary-axis weights (S/m_sxcf_gemm.f90:432-458)."""
+        om = np.ascontiguousarray(omega, np.float64)
+        ek = np.ascontiguousarray(ekc, np.float64)
+        x, wt, ex = (np.ascontiguousarray(a, np.float64) for a in (freqx, wwx, expa))
+        niw, nm, ntq = len(x), len(ek), len(om)
+        g = np.zeros((niw + 1, nm, ntq), order="F")
+        _chk(lib().ecalj_sigc_weights(self._h, niw, ns1, ns1 + nm - 1, ntq, nctot, _np_ptr(om), _np_ptr(ek), esmr,
+                                      ua, wkkr, _np_ptr(x), _np_ptr(wt), _np_ptr(ex), _np_ptr(g)), "sigc_weights")
+        return g
+
+    def sigc_weights_dev(self, nbatch, niw, nm, ntq, omega, ekc, esmr, ua, wkkr, freqx, wwx, expa, out, ns1=1,
+                         nctot=0):
+        _chk(lib().ecalj_sigc_weights_dev(self._h, nbatch, niw, ns1, ns1 + nm - 1, ntq, nctot, _t_ptr(omega),
+                                          _t_ptr(ekc), esmr, ua, _t_ptr(wkkr), _t_ptr(freqx), _t_ptr(wwx),
+                                          _t_ptr(expa), _t_ptr(out)), "sigc_weights_dev")
+
+    def sigc(self, zmel, wc, wgtim, zsec=None):
+        """Sigma_c imaginary axis + final product + clip (S/m_sxcf_gemm.f90:463-496,599-608)."""
+        z, w, g = _f(zmel, np.complex128), _f(wc, np.complex128), _f(wgtim, np.float64)
+        ngb, nm, ntq = z.shape
+        niw = w.shape[2] - 1
+        zs = np.zeros((ntq, ntq), np.complex128, order="F") if zsec is None else _f(zsec, np.complex128).copy(order="F")
+        _chk(lib().ecalj_sigc(self._h, _np_ptr(z), ngb, nm, ntq, niw, _np_ptr(w), w.shape[0], _np_ptr(g),
+                              _np_ptr(zs), zs.shape[0]), "sigc")
+        return zs
+
+    def sigc_dev(self, nbatch, zmel, ngb, nm, ntq, niw, wc, wgtim, zsec, ldw=None, ldz=None):
+        _chk(lib().ecalj_sigc_dev(self._h, nbatch, _t_ptr(zmel), ngb, nm, ntq, niw, _t_ptr(wc),
+                                  ngb if ldw is None else ldw, _t_ptr(wgtim), _t_ptr(zsec),
+                                  ntq if ldz is None else ldz), "sigc_dev")
+
+
+def qp_energy(elda, sex, sexcore, sec, vxc):
+    """QP energies (S/main_hqpe.sc.f90:152-161), eV."""
+    s = _f(sec, np.complex128)
+    e = np.zeros(len(elda))
+    arr = [np.ascontiguousarray(a, np.float64) for a in (elda, sex, sexcore)]
+    v = np.ascontiguousarray(vxc, np.float64)
+    _chk(lib().ecalj_qp_energy(len(e), *[_np_ptr(a) for a in arr], _np_ptr(s), s.shape[0], _np_ptr(v),
+                               _np_ptr(e)), "qp_energy")
+    return e
+
+
+def sigma_herm(sex2, sexcore2, sec, vxcmat):
+    arr = [_f(a, np.complex128) for a in (sex2, sexcore2, sec, vxcmat)]
+    se = np.zeros_like(arr[2], order="F")
+    _chk(lib().ecalj_sigma_herm(arr[2].shape[0], *[_np_ptr(a) for a in arr], _np_ptr(se)), "sigma_herm")
+    return se

@@ -1,0 +1,859 @@
+// api.hip -- the extern "C" entry points of include/ecalj_gpu.h.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "ctx.hpp"
+#include "kernels.hpp"
+
+using ecalj::cd;
+
+namespace ecalj {
+
+ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 256;
+  if (h->slot_bytes[slot] < bytes) {
+    if (h->slot[slot]) {
+      ECALJ_HIP(hipStreamSynchronize(h->stream));
+      ECALJ_HIP(hipFree(h->slot[slot]));
+      h->slot[slot] = nullptr;
+      h->slot_bytes[slot] = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    if (hipMalloc(&h->slot[slot], want) != hipSuccess) {
+      h->slot[slot] = nullptr;
+      return ECALJ_ERR_NOMEM;
+    }
+    h->slot_bytes[slot] = want;
+  }
+  *out = h->slot[slot];
+  return ECALJ_OK;
+}
+
+ecalj_status timing_begin(ecalj_ctx* h, int kclass, int* idx) {
+  *idx = -1;
+  if (!h->timing) return ECALJ_OK;
+  auto& pool = h->ev_pool[kclass];
+  int i = h->ev_used[kclass];
+  if (i >= (int)pool.size()) {
+    ecalj_ctx::EvPair p;
+    ECALJ_HIP(hipEventCreate(&p.a));
+    ECALJ_HIP(hipEventCreate(&p.b));
+    pool.push_back(p);
+  }
+  ECALJ_HIP(hipEventRecord(pool[i].a, h->stream));
+  h->ev_used[kclass] = i + 1;
+  *idx = i;
+  return ECALJ_OK;
+}
+
+ecalj_status timing_end(ecalj_ctx* h, int kclass, int idx) {
+  if (idx < 0) return ECALJ_OK;
+  ECALJ_HIP(hipEventRecord(h->ev_pool[kclass][idx].b, h->stream));
+  return ECALJ_OK;
+}
+
+// Host restatement of hilbertmat (S/hilbertmat.f90:14-79), FP64.
+static void hilbertmat_host(std::complex<double> zz, int nwhis, const double* L, const double* C,
+                            const double* R, std::complex<double>* rmat) {
+  using Z = std::complex<double>;
+  const double eps = 1e-8;
+  const Z zze = zz + Z(0.0, 1e-13);
+  auto h2 = [&](double a, double b, Z& facm, Z& facn) {
+    Z db = zze - b, da = zze - a;
+    facm = 0.0;
+    facn = 0.0;
+    if (std::abs(da) > eps && std::abs(db) > eps) {
+      facn = std::log(db / da);
+      facm = zze * facn + (b - a);
+    }
+  };
+  const double* l = L + nwhis;
+  const double* c = C + nwhis;
+  const double* r = R + nwhis;
+  Z* rm = rmat + nwhis;
+  for (int i = -nwhis; i <= nwhis; ++i) rm[i] = 0.0;
+  for (int iw = -nwhis; iw <= nwhis; ++iw) {
+    if (iw == 0) continue;
+    const double ci = c[iw];
+    const double cim = (iw == -nwhis) ? l[iw] : (iw == 1 ? 0.0 : c[iw - 1]);
+    Z facm, facn;
+    h2(l[iw], c[iw], facm, facn);
+    rm[iw] += (facm - facn * cim) / (ci - cim);
+    if (iw != -nwhis && iw != 1) rm[iw - 1] += (-facm + facn * ci) / (ci - cim);
+    const double cip = (iw == nwhis) ? r[iw] : (iw == -1 ? 0.0 : c[iw + 1]);
+    h2(c[iw], r[iw], facm, facn);
+    rm[iw] += (-facm + facn * cip) / (cip - ci);
+    if (iw != nwhis && iw != -1) rm[iw + 1] += (facm - facn * ci) / (cip - ci);
+  }
+}
+
+// imatt(iv, iw) * (-wfac(iw)/(R-L)) for npm = 1 (S/dpsion5.f90:84-89,121-129,215-220).
+static void hilbert_coef(int nwhis, int niw, const double* frhis, const double* freqi, double ecut,
+                         std::vector<double>& coef, std::vector<double>& scale) {
+  const int n2 = 2 * nwhis + 1;
+  std::vector<double> L(n2), C(n2), R(n2);
+  for (int iw = -nwhis; iw <= nwhis; ++iw) {
+    const int j = iw + nwhis;
+    if (iw < 0) {
+      L[j] = -frhis[-iw];
+      R[j] = -frhis[-iw - 1];
+    } else if (iw == 0) {
+      L[j] = 0.0;
+      R[j] = 0.0;
+    } else {
+      L[j] = frhis[iw - 1];
+      R[j] = frhis[iw];
+    }
+    C[j] = 0.5 * (L[j] + R[j]);
+  }
+  scale.assign(nwhis, 0.0);
+  for (int iw = 1; iw <= nwhis; ++iw) {
+    const double hl = frhis[iw - 1], hr = frhis[iw], hc = 0.5 * (hl + hr);
+    const double wfac = (ecut < 1e9) ? std::exp(-(hc / ecut) * (hc / ecut)) : 1.0;
+    scale[iw - 1] = -wfac / (hr - hl);
+  }
+  coef.assign((size_t)niw * nwhis, 0.0);
+  std::vector<std::complex<double>> rr(n2);
+  const double pi = 3.14159265358979323846;
+  for (int it = 0; it < niw; ++it) {
+    hilbertmat_host(std::complex<double>(0.0, freqi[it]), nwhis, L.data(), C.data(), R.data(), rr.data());
+    for (int iw = 1; iw <= nwhis; ++iw)
+      coef[it + (size_t)niw * (iw - 1)] =
+          std::real(rr[nwhis + iw] - rr[nwhis - iw]) / pi * scale[iw - 1];
+  }
+}
+
+template <class T>
+static ecalj_status upload(ecalj_ctx* h, int slot, const std::vector<T>& v, T** out) {
+  ECALJ_TRY(slot_ensure(h, slot, sizeof(T) * v.size(), (void**)out));
+  if (!v.empty()) {
+    ECALJ_HIP(hipMemcpyAsync(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, h->stream));
+    ECALJ_HIP(hipStreamSynchronize(h->stream));
+  }
+  return ECALJ_OK;
+}
+
+// RAII device buffer for the host-pointer entry points.
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  ecalj_status alloc(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    return hipMalloc(&p, bytes) == hipSuccess ? ECALJ_OK : ECALJ_ERR_NOMEM;
+  }
+  ecalj_status put(ecalj_ctx* h, const void* src, size_t bytes) {
+    ECALJ_TRY(alloc(bytes));
+    if (src && bytes) ECALJ_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, h->stream));
+    return ECALJ_OK;
+  }
+};
+
+static ecalj_status check_flag(ecalj_ctx* h) {
+  int f = 0;
+  ECALJ_HIP(hipMemcpyAsync(&f, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  if (f) {
+    int z = 0;
+    ECALJ_HIP(hipMemcpyAsync(h->d_flag, &z, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    ECALJ_HIP(hipStreamSynchronize(h->stream));
+    return ECALJ_ERR_SINGULAR;
+  }
+  return ECALJ_OK;
+}
+
+}  // namespace ecalj
+
+using namespace ecalj;
+
+extern "C" {
+
+const char* ecalj_status_string(int s) {
+  switch (s) {
+    case ECALJ_OK: return "ok";
+    case ECALJ_ERR_DIM: return "inconsistent or unsupported dimensions";
+    case ECALJ_ERR_SINGULAR: return "epsilon matrix singular";
+    case ECALJ_ERR_HIP: return "HIP runtime error";
+    case ECALJ_ERR_ARG: return "bad argument";
+    case ECALJ_ERR_NOMEM: return "device allocation failed";
+    case ECALJ_ERR_STATE: return "required set-up call missing";
+    default: return "unknown status";
+  }
+}
+
+ecalj_status ecalj_create(ecalj_handle* out, int device) {
+  if (!out) return ECALJ_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  ECALJ_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return ECALJ_ERR_ARG;
+  ECALJ_HIP(hipSetDevice(device));
+  ecalj_ctx* h = new ecalj_ctx;
+  h->device = device;
+  if (hipStreamCreate(&h->own_stream) != hipSuccess) {
+    delete h;
+    return ECALJ_ERR_HIP;
+  }
+  h->stream = h->own_stream;
+  if (hipMalloc(&h->d_flag, sizeof(int)) != hipSuccess) {
+    (void)hipStreamDestroy(h->own_stream);
+    delete h;
+    return ECALJ_ERR_NOMEM;
+  }
+  int z = 0;
+  (void)hipMemcpy(h->d_flag, &z, sizeof(int), hipMemcpyHostToDevice);
+  *out = h;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_destroy(ecalj_handle h) {
+  if (!h) return ECALJ_ERR_ARG;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  for (int s = 0; s < ecalj_ctx::S_NSLOT; ++s)
+    if (h->slot[s]) (void)hipFree(h->slot[s]);
+  if (h->d_ppb) (void)hipFree(h->d_ppb);
+  if (h->d_ppb2) (void)hipFree(h->d_ppb2);
+  if (h->d_vcousq) (void)hipFree(h->d_vcousq);
+  if (h->d_zcousq) (void)hipFree(h->d_zcousq);
+  if (h->d_flag) (void)hipFree(h->d_flag);
+  for (auto& pool : h->ev_pool)
+    for (auto& p : pool) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  delete h;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_set_precision(ecalj_handle h, int p) {
+  if (!h) return ECALJ_ERR_ARG;
+  if (p != ECALJ_PREC_F64 && p != ECALJ_PREC_F32 && p != ECALJ_PREC_BF16X3) return ECALJ_ERR_ARG;
+  h->precision = p;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_set_stream(ecalj_handle h, void* s) {
+  if (!h) return ECALJ_ERR_ARG;
+  h->stream = s ? static_cast<hipStream_t>(s) : h->own_stream;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_sync(ecalj_handle h) {
+  if (!h) return ECALJ_ERR_ARG;
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return check_flag(h);
+}
+
+ecalj_status ecalj_kernel_timing(ecalj_handle h, int enable) {
+  if (!h) return ECALJ_ERR_ARG;
+  h->timing = enable != 0;
+  for (int c = 0; c < 5; ++c) h->ev_used[c] = 0;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_kernel_time(ecalj_handle h, int kclass, double* ms, int64_t* launches) {
+  if (!h || kclass < 0 || kclass >= 5 || !ms || !launches) return ECALJ_ERR_ARG;
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  double tot = 0.0;
+  for (int i = 0; i < h->ev_used[kclass]; ++i) {
+    float t = 0.f;
+    ECALJ_HIP(hipEventElapsedTime(&t, h->ev_pool[kclass][i].a, h->ev_pool[kclass][i].b));
+    tot += t;
+  }
+  *ms = tot;
+  *launches = h->ev_used[kclass];
+  h->ev_used[kclass] = 0;
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ device memory
+ecalj_status ecalj_dev_alloc(ecalj_handle h, size_t bytes, void** ptr) {
+  if (!h || !ptr) return ECALJ_ERR_ARG;
+  *ptr = nullptr;
+  ECALJ_HIP(hipSetDevice(h->device));
+  if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) {
+    *ptr = nullptr;
+    return ECALJ_ERR_NOMEM;
+  }
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_dev_free(ecalj_handle h, void* ptr) {
+  if (!h) return ECALJ_ERR_ARG;
+  if (!ptr) return ECALJ_OK;
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  ECALJ_HIP(hipFree(ptr));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_memcpy_h2d(ecalj_handle h, void* dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!dst || !src))) return ECALJ_ERR_ARG;
+  if (!bytes) return ECALJ_OK;
+  ECALJ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_memcpy_d2h(ecalj_handle h, void* dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!dst || !src))) return ECALJ_ERR_ARG;
+  if (!bytes) return ECALJ_OK;
+  ECALJ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_memcpy_d2d(ecalj_handle h, void* dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!dst || !src))) return ECALJ_ERR_ARG;
+  if (!bytes) return ECALJ_OK;
+  ECALJ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_dev_zero(ecalj_handle h, void* ptr, size_t bytes) {
+  if (!h || (bytes && !ptr)) return ECALJ_ERR_ARG;
+  return dev_zero(h, ptr, bytes);
+}
+
+ecalj_status ecalj_dev_phase_similarity(ecalj_handle h, const void* a, int ngb, int nmat, const void* d, int nk,
+                                        void* out) {
+  if (!h || !a || !d || !out) return ECALJ_ERR_ARG;
+  if (ngb < 0 || nmat < 0 || nk < 0) return ECALJ_ERR_DIM;
+  return phase_similarity(h, (const cd*)a, ngb, nmat, (const cd*)d, nk, (cd*)out);
+}
+
+// ------------------------------------------------------------------ state
+ecalj_status ecalj_set_ppb(ecalj_handle h, int natom, const int* nv, const int* mdim, const double* ppb) {
+  if (!h || natom < 1 || !nv || !mdim || !ppb) return ECALJ_ERR_ARG;
+  int64_t tot = 0;
+  h->nv.assign(nv, nv + natom);
+  h->mdim.assign(mdim, mdim + natom);
+  h->ppb_off.resize(natom);
+  h->ndima_used = 0;
+  h->ngb_ppb = 0;
+  for (int a = 0; a < natom; ++a) {
+    if (nv[a] < 0 || mdim[a] < 0) return ECALJ_ERR_DIM;
+    h->ppb_off[a] = tot;
+    tot += (int64_t)nv[a] * nv[a] * mdim[a];
+    h->ndima_used += nv[a];
+    h->ngb_ppb += mdim[a];
+  }
+  h->natom = natom;
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  if (h->d_ppb) (void)hipFree(h->d_ppb);
+  if (h->d_ppb2) (void)hipFree(h->d_ppb2);
+  h->d_ppb = h->d_ppb2 = nullptr;
+  const size_t bytes = sizeof(double) * std::max<int64_t>(tot, 1);
+  if (hipMalloc(&h->d_ppb, bytes) != hipSuccess) return ECALJ_ERR_NOMEM;
+  if (hipMalloc(&h->d_ppb2, bytes) != hipSuccess) return ECALJ_ERR_NOMEM;
+  ECALJ_HIP(hipMemcpyAsync(h->d_ppb, ppb, sizeof(double) * tot, hipMemcpyHostToDevice, h->stream));
+  for (int a = 0; a < natom; ++a)
+    ECALJ_TRY(ppb_relayout(h, h->d_ppb + h->ppb_off[a], h->d_ppb2 + h->ppb_off[a], nv[a], mdim[a]));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_set_coulomb(ecalj_handle h, int ngb, const double* vcoud, const double* zcousq) {
+  if (!h || ngb < 1 || !vcoud || !zcousq) return ECALJ_ERR_ARG;
+  std::vector<double> vs(ngb);
+  for (int i = 0; i < ngb; ++i) {
+    if (!(vcoud[i] >= 0.0)) return ECALJ_ERR_ARG;
+    vs[i] = std::sqrt(vcoud[i]);  // S/m_readVcoud.f90:40
+  }
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  if (h->d_vcousq) (void)hipFree(h->d_vcousq);
+  if (h->d_zcousq) (void)hipFree(h->d_zcousq);
+  h->d_vcousq = nullptr;
+  h->d_zcousq = nullptr;
+  if (hipMalloc(&h->d_vcousq, sizeof(double) * ngb) != hipSuccess) return ECALJ_ERR_NOMEM;
+  if (hipMalloc(&h->d_zcousq, sizeof(cd) * (size_t)ngb * ngb) != hipSuccess) return ECALJ_ERR_NOMEM;
+  ECALJ_HIP(hipMemcpyAsync(h->d_vcousq, vs.data(), sizeof(double) * ngb, hipMemcpyHostToDevice, h->stream));
+  ECALJ_HIP(hipMemcpyAsync(h->d_zcousq, zcousq, sizeof(cd) * (size_t)ngb * ngb, hipMemcpyHostToDevice, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  h->ngb = ngb;
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ zmel
+// get_zmel_init_gemm (S/m_zmel.f90:146-597), MT valence part.  For each atom
+// block the smaller side is contracted with ppb first:
+//   m-first: T(b,I,n) = sum_a ppb(b,a,I) conj(cphim(a,n));  zmel(I,n,m) = phase sum_b T(b,I,n) cphiq(b,m)
+//   q-first: T(a,I,m) = sum_b ppb(b,a,I) cphiq(b,m);        zmel(I,n,m) = phase sum_a conj(cphim(a,n)) T(a,I,m)
+static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int nm, const cd* cphim,
+                              int64_t sm, int nq, const cd* cphiq, int64_t sq, int is_m_basis, int conjg,
+                              int nbatch, cd* zmel) {
+  if (!h->d_ppb) return ECALJ_ERR_STATE;
+  if (nm < 0 || nq < 0 || nbatch < 1 || ndima < h->ndima_used) return ECALJ_ERR_DIM;
+  const int ngb = h->ngb_ppb;
+  if (!is_m_basis && (!h->d_zcousq || h->ngb != ngb)) return ECALJ_ERR_STATE;
+  if (nm == 0 || nq == 0) return ECALJ_OK;
+  const int64_t zsz = (int64_t)ngb * nm * nq;  // per batch
+  cd* zM = zmel;
+  if (!is_m_basis) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP2, sizeof(cd) * zsz * nbatch, (void**)&zM));
+  const bool conj_now = conjg && is_m_basis;
+  const int bm = (sm == 0) ? 1 : nbatch;
+  const int bq = (sq == 0) ? 1 : nbatch;
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 2, &tev));
+  int64_t ias = 0, ims = 0;
+  for (int a = 0; a < h->natom; ++a) {
+    const int nv = h->nv[a], md = h->mdim[a];
+    const cd ph{phase ? phase[2 * a] : 1.0, phase ? phase[2 * a + 1] : 0.0};
+    const int64_t nvmd = (int64_t)nv * md;
+    cd* T;
+    if ((int64_t)nm * bm <= (int64_t)nq * bq) {
+      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, sizeof(cd) * nvmd * nm * bm, (void**)&T));
+      GemmCall g;  // T[(b+nv*I), (n,k)] = P2 * conj(cphim)
+      g.areal = true; g.cb = true;
+      g.M = (int)nvmd; g.K = nv;
+      g.A = h->d_ppb2 + h->ppb_off[a]; g.lda = nvmd;
+      g.B = cphim + ias; g.ldb = ndima;
+      g.C = T; g.ldc = nvmd;
+      if (bm == 1 || sm == (int64_t)ndima * nm) {
+        g.N = nm * bm;
+      } else {
+        g.N = nm;
+        g.bB = {sm, 0}; g.bC = {nvmd * nm, 0};
+        g.batch = bm; g.div = bm;
+      }
+      ECALJ_TRY(gemm_f64(h, g));
+      GemmCall z;  // per (n,k): zmel(ims+I, n, m) = phase * sum_b T[b + nv*I + nvmd*(n + nm*k)] cphiq_k(ias+b, m)
+      z.ta = true;
+      z.M = md; z.N = nq; z.K = nv;
+      z.A = T; z.lda = nv; z.bA = {nvmd, (bm == 1) ? 0 : nvmd * nm};
+      z.B = cphiq + ias; z.ldb = ndima; z.bB = {0, sq};
+      z.C = zM + ims; z.ldc = (int64_t)ngb * nm; z.bC = {ngb, zsz};
+      z.batch = nm * nbatch; z.div = nm;
+      z.alpha = ph; z.conjC = conj_now;
+      ECALJ_TRY(gemm_f64(h, z));
+    } else {
+      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, sizeof(cd) * nvmd * nq * bq, (void**)&T));
+      GemmCall g;  // T[(a+nv*I), (m,k)] = ppb^T * cphiq, A(r,b) = ppb[b + nv*r]
+      g.areal = true; g.ta = true;
+      g.M = (int)nvmd; g.K = nv;
+      g.A = h->d_ppb + h->ppb_off[a]; g.lda = nv;
+      g.B = cphiq + ias; g.ldb = ndima;
+      g.C = T; g.ldc = nvmd;
+      if (bq == 1 || sq == (int64_t)ndima * nq) {
+        g.N = nq * bq;
+      } else {
+        g.N = nq;
+        g.bB = {sq, 0}; g.bC = {nvmd * nq, 0};
+        g.batch = bq; g.div = bq;
+      }
+      ECALJ_TRY(gemm_f64(h, g));
+      GemmCall z;  // per (m,k): zmel(ims+I, n, m) = phase * sum_a T[a + nv*I + nvmd*(m + nq*k)] conj(cphim_k(ias+a, n))
+      z.ta = true; z.cb = true;
+      z.M = md; z.N = nm; z.K = nv;
+      z.A = T; z.lda = nv; z.bA = {nvmd, (bq == 1) ? 0 : nvmd * nq};
+      z.B = cphim + ias; z.ldb = ndima; z.bB = {0, sm};
+      z.C = zM + ims; z.ldc = ngb; z.bC = {(int64_t)ngb * nm, zsz};
+      z.batch = nq * nbatch; z.div = nq;
+      z.alpha = ph; z.conjC = conj_now;
+      ECALJ_TRY(gemm_f64(h, z));
+    }
+    ias += nv;
+    ims += md;
+  }
+  if (!is_m_basis) {  // zmel = zcousq^H zmel_M  (S/m_zmel.f90:559-560), then conj
+    GemmCall e;
+    e.ta = true; e.ca = true;
+    e.M = ngb; e.N = (int)((int64_t)nm * nq * nbatch); e.K = ngb;
+    e.A = h->d_zcousq; e.lda = ngb;
+    e.B = zM; e.ldb = ngb;
+    e.C = zmel; e.ldc = ngb;
+    e.conjC = conjg;
+    ECALJ_TRY(gemm_f64(h, e));
+  }
+  ECALJ_TRY(timing_end(h, 2, tev));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_zmel_dev(ecalj_handle h, const double* phase, int ndima, int nm, const void* cphim, int nq,
+                            const void* cphiq, int is_m_basis, int zmelconjg, int nbatch, void* zmel) {
+  if (!h || !cphim || !cphiq || !zmel) return ECALJ_ERR_ARG;
+  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  return zmel_impl(h, phase, ndima, nm, (const cd*)cphim, (int64_t)ndima * nm, nq, (const cd*)cphiq,
+                   (int64_t)ndima * nq, is_m_basis, zmelconjg, nbatch, (cd*)zmel);
+}
+
+ecalj_status ecalj_zmel_strided_dev(ecalj_handle h, const double* phase, int ndima, int nm, const void* cphim,
+                                    int64_t cphim_stride, int nq, const void* cphiq, int64_t cphiq_stride,
+                                    int is_m_basis, int zmelconjg, int nbatch, void* zmel) {
+  if (!h || !cphim || !cphiq || !zmel) return ECALJ_ERR_ARG;
+  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  return zmel_impl(h, phase, ndima, nm, (const cd*)cphim, cphim_stride, nq, (const cd*)cphiq, cphiq_stride,
+                   is_m_basis, zmelconjg, nbatch, (cd*)zmel);
+}
+
+ecalj_status ecalj_zmel(ecalj_handle h, const double* phase, int ndima, int nm, const double* cphim, int nq,
+                        const double* cphiq, int is_m_basis, int zmelconjg, int nbatch, double* zmel) {
+  if (!h || !cphim || !cphiq || !zmel || nbatch < 1) return ECALJ_ERR_ARG;
+  const size_t bm = sizeof(cd) * (size_t)ndima * nm * nbatch, bq = sizeof(cd) * (size_t)ndima * nq * nbatch;
+  const size_t bz = sizeof(cd) * (size_t)h->ngb_ppb * nm * nq * nbatch;
+  DevBuf dm, dq, dz;
+  ECALJ_TRY(dm.put(h, cphim, bm));
+  ECALJ_TRY(dq.put(h, cphiq, bq));
+  ECALJ_TRY(dz.alloc(bz));
+  ECALJ_TRY(ecalj_zmel_dev(h, phase, ndima, nm, dm.p, nq, dq.p, is_m_basis, zmelconjg, nbatch, dz.p));
+  ECALJ_HIP(hipMemcpyAsync(zmel, dz.p, bz, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ x0gemm
+// Bin lists of x0gemm (S/x0gemm.f90:36-64), built over all k of the batch,
+// then the per-bin gather-Herk.
+static ecalj_status x0_impl(ecalj_ctx* h, cd* rcxq, int npr, int ipr_col, int npr_col, int nwhis, int npm, int nk,
+                            const int* ns1, const int* ns2, const cd* zmel, int64_t zstride, int nbb, int nqtot,
+                            const int* kmin, const int* kmax, int ncoun, const int* iwini, const int* iwend,
+                            const int* itc, const int* itpc, const int* jpmc, const int* icouini, int ncount,
+                            const double* whwc) {
+  if (npr < 0 || npr_col < 0 || ipr_col < 1 || ipr_col - 1 + npr_col > nbb || npr > nbb || nwhis < 1 ||
+      (npm != 1 && npm != 2) || nk < 1)
+    return ECALJ_ERR_DIM;
+  const int nbin = nwhis * npm;
+  std::vector<int64_t> cnt(nbin + 1, 0);
+  auto visit = [&](auto&& f) -> ecalj_status {
+    for (int k = 0; k < nk; ++k) {
+      const int nm = ns2[k] - ns1[k] + 1;
+      if (kmin[k] < 1 || kmax[k] > ncoun) return ECALJ_ERR_DIM;
+      for (int ic = kmin[k]; ic <= kmax[k]; ++ic) {
+        const int it = itc[ic - 1], itp = itpc[ic - 1], jpm = jpmc[ic - 1];
+        if (it < ns1[k] || it > ns2[k]) continue;
+        if (itp < 1 || itp > nqtot || jpm < 1 || jpm > npm) return ECALJ_ERR_DIM;
+        const int64_t col = zstride * k + (int64_t)nbb * ((it - ns1[k]) + (int64_t)nm * (itp - 1));
+        for (int iw = iwini[ic - 1]; iw <= iwend[ic - 1]; ++iw) {
+          if (iw < 1 || iw > nwhis) return ECALJ_ERR_DIM;
+          const int64_t wi = iw - iwini[ic - 1] + icouini[ic - 1] - 1;
+          if (wi < 0 || wi >= ncount) return ECALJ_ERR_DIM;
+          f((iw - 1) + nwhis * (jpm - 1), col, whwc[wi]);
+        }
+      }
+    }
+    return ECALJ_OK;
+  };
+  ECALJ_TRY(visit([&](int b, int64_t, double) { cnt[b + 1]++; }));
+  for (int b = 0; b < nbin; ++b) cnt[b + 1] += cnt[b];
+  const int64_t tot = cnt[nbin];
+  std::vector<int64_t> col(std::max<int64_t>(tot, 1));
+  std::vector<double> wt(std::max<int64_t>(tot, 1));
+  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+  ECALJ_TRY(visit([&](int b, int64_t c, double w) {
+    const int64_t p = fill[b]++;
+    col[p] = c;
+    wt[p] = w;
+  }));
+  // work items; square output -> upper tiles + mirror
+  const int mirror = (ipr_col == 1 && npr_col == npr) ? 1 : 0;
+  const int tm = (npr + 63) / 64, tn = (npr_col + 63) / 64;
+  struct Item { int list, slice, ti, tj; };
+  std::vector<Item> items;
+  std::vector<int64_t> kk;
+  for (int b = 0; b < nbin; ++b) {
+    const int64_t K = cnt[b + 1] - cnt[b];
+    if (K == 0) continue;
+    const int iw = b % nwhis + 1, jpm = b / nwhis + 1;
+    const int slice = iw * (3 - 2 * jpm) - (1 - npm) * nwhis;
+    for (int ti = 0; ti < tm; ++ti)
+      for (int tj = mirror ? ti : 0; tj < tn; ++tj) {
+        items.push_back({b, slice, ti, tj});
+        kk.push_back(K);
+      }
+  }
+  std::vector<int> order(items.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kk[x] > kk[y]; });
+  std::vector<Item> sorted(items.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  // one upload: [bstart | col | wt | items]
+  const size_t nb1 = nbin + 1;
+  std::vector<char> blob(sizeof(int64_t) * (nb1 + col.size()) + sizeof(double) * wt.size() +
+                         sizeof(Item) * std::max<size_t>(sorted.size(), 1));
+  char* q = blob.data();
+  memcpy(q, cnt.data(), sizeof(int64_t) * nb1);
+  memcpy(q + sizeof(int64_t) * nb1, col.data(), sizeof(int64_t) * col.size());
+  memcpy(q + sizeof(int64_t) * (nb1 + col.size()), wt.data(), sizeof(double) * wt.size());
+  memcpy(q + sizeof(int64_t) * (nb1 + col.size()) + sizeof(double) * wt.size(), sorted.data(),
+         sizeof(Item) * sorted.size());
+  char* d;
+  ECALJ_TRY(upload(h, ecalj_ctx::S_LIST, blob, &d));
+  const int64_t* d_b = (const int64_t*)d;
+  const int64_t* d_col = d_b + nb1;
+  const double* d_w = (const double*)(d_col + col.size());
+  const void* d_items = (const char*)d_w + sizeof(double) * wt.size();
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 1, &tev));
+  ECALJ_TRY(x0_herk(h, zmel, d_col, d_w, d_b, d_items, (int)sorted.size(), rcxq, (int64_t)npr * npr_col, npr,
+                    npr_col, ipr_col, mirror));
+  ECALJ_TRY(timing_end(h, 1, tev));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_x0gemm_batched_dev(ecalj_handle h, void* rcxq, int npr, int ipr_col, int npr_col, int nwhis,
+                                      int npm, int nk, const int* ns1, const int* ns2, const void* zmel,
+                                      int64_t zmel_stride, int nbb, int nqtot, const int* icounkmin,
+                                      const int* icounkmax, int ncoun, const int* iwini, const int* iwend,
+                                      const int* itc, const int* itpc, const int* jpmc, const int* icouini,
+                                      int ncount, const double* whwc) {
+  if (!h || !rcxq || !zmel || !ns1 || !ns2 || !icounkmin || !icounkmax || !iwini || !iwend || !itc || !itpc ||
+      !jpmc || !icouini || !whwc)
+    return ECALJ_ERR_ARG;
+  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  return x0_impl(h, (cd*)rcxq, npr, ipr_col, npr_col, nwhis, npm, nk, ns1, ns2, (const cd*)zmel, zmel_stride, nbb,
+                 nqtot, icounkmin, icounkmax, ncoun, iwini, iwend, itc, itpc, jpmc, icouini, ncount, whwc);
+}
+
+ecalj_status ecalj_x0gemm(ecalj_handle h, double* rcxq, int npr, int ipr_col, int npr_col, int nwhis, int npm,
+                          int ns1, int ns2, const double* zmel, int nbb, int nqtot, int icoun_lo, int icoun_hi,
+                          const int* iwini, const int* iwend, const int* itc, const int* itpc, const int* jpmc,
+                          const int* icouini, const double* whwc) {
+  if (!h || !rcxq || !zmel) return ECALJ_ERR_ARG;
+  if (icoun_hi < icoun_lo) return ECALJ_OK;
+  const int nm = ns2 - ns1 + 1;
+  if (nm < 1) return ECALJ_ERR_DIM;
+  // ncoun / ncount bounds: the caller's arrays cover indices up to icoun_hi
+  int ncount = 0;
+  for (int ic = icoun_lo; ic <= icoun_hi; ++ic)
+    ncount = std::max(ncount, icouini[ic - 1] + iwend[ic - 1] - iwini[ic - 1]);
+  const size_t nslice = (npm == 1) ? (size_t)nwhis + 1 : 2 * (size_t)nwhis + 1;
+  const size_t br = sizeof(cd) * (size_t)npr * npr_col * nslice;
+  const size_t bz = sizeof(cd) * (size_t)nbb * nm * nqtot;
+  DevBuf dr, dz;
+  ECALJ_TRY(dr.put(h, rcxq, br));
+  ECALJ_TRY(dz.put(h, zmel, bz));
+  ECALJ_TRY(ecalj_x0gemm_batched_dev(h, dr.p, npr, ipr_col, npr_col, nwhis, npm, 1, &ns1, &ns2, dz.p, 0, nbb, nqtot,
+                                     &icoun_lo, &icoun_hi, icoun_hi, iwini, iwend, itc, itpc, jpmc, icouini, ncount,
+                                     whwc));
+  ECALJ_HIP(hipMemcpyAsync(rcxq, dr.p, br, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ dpsion
+ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_col, int nwhis, int niw,
+                                   const double* frhis, const double* freqi, double ecut, int scale_rcxq,
+                                   void* zxqi) {
+  if (!h || !rcxq || !frhis || !freqi || !zxqi) return ECALJ_ERR_ARG;
+  if (npr < 0 || npr_col < 0 || nwhis < 1 || niw < 1) return ECALJ_ERR_DIM;
+  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  std::vector<double> coef, scale;
+  hilbert_coef(nwhis, niw, frhis, freqi, ecut, coef, scale);
+  std::vector<double> blob(coef);
+  blob.insert(blob.end(), scale.begin(), scale.end());
+  double* d;
+  ECALJ_TRY(upload(h, ecalj_ctx::S_COEF, blob, &d));
+  const int64_t n = (int64_t)npr * npr_col;
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 4, &tev));
+  ECALJ_TRY(hilbert(h, (cd*)rcxq, n, nwhis, niw, d, scale_rcxq ? d + coef.size() : nullptr, (cd*)zxqi, n));
+  ECALJ_TRY(timing_end(h, 4, tev));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_dpsion_imag(ecalj_handle h, double* rcxq, int npr, int npr_col, int nwhis, int niw,
+                               const double* frhis, const double* freqi, double ecut, int scale_rcxq, double* zxqi) {
+  if (!h || !rcxq || !zxqi) return ECALJ_ERR_ARG;
+  const size_t br = sizeof(cd) * (size_t)npr * npr_col * (nwhis + 1);
+  const size_t bx = sizeof(cd) * (size_t)npr * npr_col * niw;
+  DevBuf dr, dx;
+  ECALJ_TRY(dr.put(h, rcxq, br));
+  ECALJ_TRY(dx.alloc(bx));
+  ECALJ_TRY(ecalj_dpsion_imag_dev(h, dr.p, npr, npr_col, nwhis, niw, frhis, freqi, ecut, scale_rcxq, dx.p));
+  if (scale_rcxq) ECALJ_HIP(hipMemcpyAsync(rcxq, dr.p, br, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipMemcpyAsync(zxqi, dx.p, bx, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ composed chi0
+ecalj_status ecalj_chi0_freq_dev(ecalj_handle h, const void* z, const void* w, int ni, int npairs, int nf,
+                                 void* chi) {
+  if (!h || !z || !w || !chi) return ECALJ_ERR_ARG;
+  if (ni < 0 || npairs < 0 || nf < 0) return ECALJ_ERR_DIM;
+  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 1, &tev));
+  ECALJ_TRY(chi0_freq(h, (const cd*)z, (const double*)w, ni, npairs, nf, (cd*)chi));
+  ECALJ_TRY(timing_end(h, 1, tev));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_chi0_freq(ecalj_handle h, const double* z, const double* w, int ni, int npairs, int nf,
+                             double* chi) {
+  if (!h || !z || !w || !chi) return ECALJ_ERR_ARG;
+  DevBuf dz, dw, dc;
+  ECALJ_TRY(dz.put(h, z, sizeof(cd) * (size_t)ni * npairs));
+  ECALJ_TRY(dw.put(h, w, sizeof(double) * (size_t)nf * npairs));
+  const size_t bc = sizeof(cd) * (size_t)ni * ni * nf;
+  ECALJ_TRY(dc.alloc(bc));
+  ECALJ_TRY(ecalj_chi0_freq_dev(h, dz.p, dw.p, ni, npairs, nf, dc.p));
+  ECALJ_HIP(hipMemcpyAsync(chi, dc.p, bc, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ W build
+ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
+                                int is_wc_m_basis, void* wc, int ldw) {
+  if (!h || !chi || !wc) return ECALJ_ERR_ARG;
+  if (ngb < 1 || nw < 1 || ldw < ngb || (nspin != 1 && nspin != 2)) return ECALJ_ERR_DIM;
+  if (!h->d_vcousq || h->ngb != ngb) return ECALJ_ERR_STATE;
+  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  const int n = ngb;
+  const int64_t nn = (int64_t)n * n;
+  const double fac = (nspin == 1) ? 2.0 : 1.0;  // S/m_llw.f90:299-303
+  cd *eps, *tmp;
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_EPS, sizeof(cd) * nn * nw, (void**)&eps));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP, sizeof(cd) * nn * nw, (void**)&tmp));
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 3, &tev));
+  if (is_x0_m_basis) {  // chi_E = zcousq^H (chi zcousq) (S/m_llw.f90:336-337), eps fused
+    GemmCall g1;
+    g1.M = n; g1.N = n; g1.K = n;
+    g1.A = chi; g1.lda = n; g1.bA = {nn, 0};
+    g1.B = h->d_zcousq; g1.ldb = n;
+    g1.C = tmp; g1.ldc = n; g1.bC = {nn, 0};
+    g1.batch = nw; g1.div = nw;
+    ECALJ_TRY(gemm_f64(h, g1));
+    GemmCall g2;
+    g2.ta = true; g2.ca = true;
+    g2.M = n; g2.N = n; g2.K = n;
+    g2.A = h->d_zcousq; g2.lda = n;
+    g2.B = tmp; g2.ldb = n; g2.bB = {nn, 0};
+    g2.C = eps; g2.ldc = n; g2.bC = {nn, 0};
+    g2.batch = nw; g2.div = nw;
+    g2.alpha = cd{fac, 0.0};
+    g2.epi = EPI_EPS; g2.svec = h->d_vcousq;
+    ECALJ_TRY(gemm_f64(h, g2));
+  } else {
+    ECALJ_TRY(eps_direct(h, (const cd*)chi, h->d_vcousq, fac, n, nw, eps));
+  }
+  ECALJ_TRY(gj_inverse(h, eps, n, n, nn, nw));  // zminv (S/m_llw.f90:350)
+  if (is_wc_m_basis) {  // W_M = zcousq W_E zcousq^H (S/m_llw.f90:366-367)
+    ECALJ_TRY(wc_form(h, eps, h->d_vcousq, n, nw, tmp, n, nn));
+    GemmCall g3;  // eps <- W_E zcousq^H
+    g3.tb = true; g3.cb = true;
+    g3.M = n; g3.N = n; g3.K = n;
+    g3.A = tmp; g3.lda = n; g3.bA = {nn, 0};
+    g3.B = h->d_zcousq; g3.ldb = n;
+    g3.C = eps; g3.ldc = n; g3.bC = {nn, 0};
+    g3.batch = nw; g3.div = nw;
+    ECALJ_TRY(gemm_f64(h, g3));
+    GemmCall g4;  // wc <- zcousq * (W_E zcousq^H)
+    g4.M = n; g4.N = n; g4.K = n;
+    g4.A = h->d_zcousq; g4.lda = n;
+    g4.B = eps; g4.ldb = n; g4.bB = {nn, 0};
+    g4.C = (cd*)wc; g4.ldc = ldw; g4.bC = {(int64_t)ldw * n, 0};
+    g4.batch = nw; g4.div = nw;
+    ECALJ_TRY(gemm_f64(h, g4));
+  } else {
+    ECALJ_TRY(wc_form(h, eps, h->d_vcousq, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
+  }
+  ECALJ_TRY(timing_end(h, 3, tev));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_wc_build(ecalj_handle h, const double* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
+                            int is_wc_m_basis, double* wc, int ldw) {
+  if (!h || !chi || !wc) return ECALJ_ERR_ARG;
+  DevBuf dc, dw;
+  ECALJ_TRY(dc.put(h, chi, sizeof(cd) * (size_t)ngb * ngb * nw));
+  const size_t bw = sizeof(cd) * (size_t)ldw * ngb * nw;
+  ECALJ_TRY(dw.put(h, wc, bw));
+  ECALJ_TRY(ecalj_wc_build_dev(h, dc.p, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, dw.p, ldw));
+  ECALJ_HIP(hipMemcpyAsync(wc, dw.p, bw, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return check_flag(h);
+}
+
+// ------------------------------------------------------------------ Sigma_c
+ecalj_status ecalj_sigc_weights_dev(ecalj_handle h, int nbatch, int niw, int ns1, int ns2, int ntq, int nctot,
+                                    const double* omega, const double* ekc, double esmr, double ua,
+                                    const double* wkkr, const double* freqx, const double* wwx,
+                                    const double* expa, void* wgtim) {
+  if (!h || !omega || !ekc || !wkkr || !freqx || !wwx || !expa || !wgtim) return ECALJ_ERR_ARG;
+  if (niw < 1 || ns2 < ns1 || ntq < 0 || nbatch < 1) return ECALJ_ERR_DIM;
+  return sig_weights(h, nbatch, niw, ns2 - ns1 + 1, ntq, ns1, nctot, omega, ekc, esmr, ua, wkkr, freqx, wwx, expa,
+                     (double*)wgtim);
+}
+
+ecalj_status ecalj_sigc_weights(ecalj_handle h, int niw, int ns1, int ns2, int ntq, int nctot, const double* omega,
+                                const double* ekc, double esmr, double ua, double wkkr, const double* freqx,
+                                const double* wwx, const double* expa, double* wgtim) {
+  if (!h || !omega || !ekc || !freqx || !wwx || !expa || !wgtim) return ECALJ_ERR_ARG;
+  const int nm = ns2 - ns1 + 1;
+  if (nm < 1) return ECALJ_ERR_DIM;
+  DevBuf d_om, d_ek, d_wk, d_x, d_w, d_e, d_g;
+  ECALJ_TRY(d_om.put(h, omega, sizeof(double) * ntq));
+  ECALJ_TRY(d_ek.put(h, ekc, sizeof(double) * nm));
+  ECALJ_TRY(d_wk.put(h, &wkkr, sizeof(double)));
+  ECALJ_TRY(d_x.put(h, freqx, sizeof(double) * niw));
+  ECALJ_TRY(d_w.put(h, wwx, sizeof(double) * niw));
+  ECALJ_TRY(d_e.put(h, expa, sizeof(double) * niw));
+  const size_t bg = sizeof(double) * (size_t)(niw + 1) * nm * ntq;
+  ECALJ_TRY(d_g.alloc(bg));
+  ECALJ_TRY(ecalj_sigc_weights_dev(h, 1, niw, ns1, ns2, ntq, nctot, (double*)d_om.p, (double*)d_ek.p, esmr, ua,
+                                   (double*)d_wk.p, (double*)d_x.p, (double*)d_w.p, (double*)d_e.p, d_g.p));
+  ECALJ_HIP(hipMemcpyAsync(wgtim, d_g.p, bg, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_sigc_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm, int ntq, int niw,
+                            const void* wc, int ldw, const void* wgtim, void* zsec, int ldz) {
+  if (!h || !zmel || !wc || !wgtim || !zsec) return ECALJ_ERR_ARG;
+  if (ngb < 1 || nm < 0 || ntq < 0 || niw < 0 || ldw < ngb || ldz < ntq || nbatch < 1) return ECALJ_ERR_DIM;
+  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  return sig_contract(h, nbatch, (const cd*)zmel, ngb, nm, ntq, niw, (const cd*)wc, ldw, (const double*)wgtim,
+                      (cd*)zsec, ldz);
+}
+
+ecalj_status ecalj_sigc(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, int niw, const double* wc,
+                        int ldw, const double* wgtim, double* zsec, int ldz) {
+  if (!h || !zmel || !wc || !wgtim || !zsec) return ECALJ_ERR_ARG;
+  DevBuf dz, dw, dg, ds;
+  ECALJ_TRY(dz.put(h, zmel, sizeof(cd) * (size_t)ngb * nm * ntq));
+  ECALJ_TRY(dw.put(h, wc, sizeof(cd) * (size_t)ldw * ngb * (niw + 1)));
+  ECALJ_TRY(dg.put(h, wgtim, sizeof(double) * (size_t)(niw + 1) * nm * ntq));
+  const size_t bs = sizeof(cd) * (size_t)ldz * ntq;
+  ECALJ_TRY(ds.put(h, zsec, bs));
+  ECALJ_TRY(ecalj_sigc_dev(h, 1, dz.p, ngb, nm, ntq, niw, dw.p, ldw, dg.p, ds.p, ldz));
+  ECALJ_HIP(hipMemcpyAsync(zsec, ds.p, bs, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ QP update
+ecalj_status ecalj_qp_energy(int ntq, const double* elda, const double* sex, const double* sexcore,
+                             const double* sec, int ldsec, const double* vxc, double* eqp) {
+  if (!elda || !sex || !sexcore || !sec || !vxc || !eqp) return ECALJ_ERR_ARG;
+  if (ntq < 0 || ldsec < ntq) return ECALJ_ERR_DIM;
+  const double hartree = 2.0 * 13.605693122994;  // hartree = 2*rydberg()
+  for (int i = 0; i < ntq; ++i) {
+    const double rsec = sec[2 * (i + (int64_t)ldsec * i)] * hartree;
+    eqp[i] = elda[i] + (sex[i] * hartree + sexcore[i] * hartree + rsec - vxc[i]);
+  }
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_sigma_herm(int ntq, const double* sex2, const double* sexcore2, const double* sec,
+                              const double* vxcmat, double* se) {
+  if (!sex2 || !sexcore2 || !sec || !vxcmat || !se) return ECALJ_ERR_ARG;
+  if (ntq < 0) return ECALJ_ERR_DIM;
+  for (int j = 0; j < ntq; ++j)
+    for (int i = 0; i < ntq; ++i) {
+      const int64_t ij = 2 * (i + (int64_t)ntq * j), ji = 2 * (j + (int64_t)ntq * i);
+      se[ij] = sex2[ij] + sexcore2[ij] + 0.5 * (sec[ij] + sec[ji]) - vxcmat[ij];
+      se[ij + 1] = sex2[ij + 1] + sexcore2[ij + 1] + 0.5 * (sec[ij + 1] - sec[ji + 1]) - vxcmat[ij + 1];
+    }
+  return ECALJ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// ctx.hpp -- handle state and host-side launch helpers.
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+struct ecalj_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int precision = ECALJ_PREC_F64;
+
+  // set_ppb state: per-atom blocks
+  int natom = 0;
+  std::vector<int> nv, mdim;
+  std::vector<int64_t> ppb_off;   // element offset of each atom block
+  int ndima_used = 0, ngb_ppb = 0;
+  double* d_ppb = nullptr;        // original layout ppb(i1,i2,I) per atom
+  double* d_ppb2 = nullptr;       // relaid: P2[(i1 + nv*I) + nv*mdim*i2]
+  // Coulomb state
+  int ngb = 0;
+  double* d_vcousq = nullptr;     // sqrt(vcoud)
+  ecalj::cd* d_zcousq = nullptr;  // ngb x ngb
+
+  // grow-only scratch slots
+  enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
+         S_NSLOT };
+  void* slot[S_NSLOT] = {};
+  size_t slot_bytes[S_NSLOT] = {};
+  int* d_flag = nullptr;          // singular-pivot flag
+
+  // kernel timing
+  bool timing = false;
+  struct EvPair { hipEvent_t a, b; };
+  std::vector<EvPair> ev_pool[5];
+  int ev_used[5] = {0, 0, 0, 0, 0};
+};
+
+namespace ecalj {
+
+ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out);
+// Event bracketing for the kernel classes of ecalj_kernel_timing.
+ecalj_status timing_begin(ecalj_ctx* h, int kclass, int* idx);
+ecalj_status timing_end(ecalj_ctx* h, int kclass, int idx);
+
+// Generic complex GEMM (FP64 MFMA).  ta/tb: operand stored transposed
+// (op = T); ca/cb: operand conjugated.  op 'C' of BLAS = ta + ca.  A may be real.
+struct GemmCall {
+  bool ta = false, tb = false, ca = false, cb = false;
+  bool areal = false;
+  int M = 0, N = 0, K = 0;
+  const void* A = nullptr; int64_t lda = 0; Batch2 bA{0, 0};
+  const cd* B = nullptr; int64_t ldb = 0; Batch2 bB{0, 0};
+  cd* C = nullptr; int64_t ldc = 0; Batch2 bC{0, 0};
+  int batch = 1, div = 1;
+  cd alpha{1.0, 0.0}, beta{0.0, 0.0};
+  int conjC = 0;
+  int epi = 0;
+  const double* svec = nullptr;
+  int gj_k0 = 0, gj_k1 = 0;
+};
+ecalj_status gemm_f64(ecalj_ctx* h, const GemmCall& c);
+
+}  // namespace ecalj

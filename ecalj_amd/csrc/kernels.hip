@@ -1,0 +1,789 @@
+// kernels.hip -- the FP64 kernels of the chi0 -> W -> Sigma_c path and their
+// launchers.  Every launch checks on the host that its grid and operand
+// extents match the buffers it was given before it is issued.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+
+namespace ecalj {
+
+// ==========================================================================
+// Generic GEMM dispatcher
+template <int BM, int BN>
+static ecalj_status launch_gemm(ecalj_ctx* h, const GemmArgs& g, bool at, bool areal, bool bt) {
+  dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, g.batch);
+  if (grid.y > 65535 || grid.z > 65535) return ECALJ_ERR_DIM;
+  const int key = (at ? 4 : 0) | (areal ? 2 : 0) | (bt ? 1 : 0);
+  switch (key) {
+    case 0: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, false, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 1: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, false, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 2: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, true, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 3: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, true, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 4: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, false, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 5: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, false, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 6: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, true, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 7: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, true, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+  }
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+ecalj_status gemm_f64(ecalj_ctx* h, const GemmCall& c) {
+  if (c.M < 0 || c.N < 0 || c.K < 0 || c.batch < 1 || c.div < 1) return ECALJ_ERR_DIM;
+  if (c.M == 0 || c.N == 0) return ECALJ_OK;
+  GemmArgs g;
+  g.M = c.M; g.N = c.N; g.K = c.K; g.batch = c.batch; g.div = c.div;
+  g.A = c.A; g.lda = c.lda; g.bA = c.bA; g.csA = c.ca ? -1.0 : 1.0;
+  g.B = c.B; g.ldb = c.ldb; g.bB = c.bB; g.csB = c.cb ? -1.0 : 1.0;
+  g.C = c.C; g.ldc = c.ldc; g.bC = c.bC;
+  g.alr = c.alpha.re; g.ali = c.alpha.im; g.ber = c.beta.re; g.bei = c.beta.im;
+  g.conjC = c.conjC; g.epi = c.epi; g.svec = c.svec; g.gj_k0 = c.gj_k0; g.gj_k1 = c.gj_k1;
+  const bool at = c.ta, bt = c.tb;
+  if (c.N <= 32) return launch_gemm<64, 32>(h, g, at, c.areal, bt);
+  return launch_gemm<64, 64>(h, g, at, c.areal, bt);
+}
+
+// ==========================================================================
+// ppb relayout: P2[(i1 + nv*I) + nv*md*i2] = ppb[i1 + nv*i2 + nv*nv*I]
+__global__ void ppb_relayout_kernel(const double* __restrict__ src, double* __restrict__ dst, int nv,
+                                    int md) {
+  const int64_t n = (int64_t)nv * nv * md;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i1 = e % nv;
+    const int64_t t = e / nv;
+    const int64_t i2 = t % nv;
+    const int64_t I = t / nv;
+    dst[(i1 + (int64_t)nv * I) + (int64_t)nv * md * i2] = src[e];
+  }
+}
+
+ecalj_status ppb_relayout(ecalj_ctx* h, const double* src, double* dst, int nv, int md) {
+  const int64_t n = (int64_t)nv * nv * md;
+  if (n == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(ppb_relayout_kernel, dim3(blocks), dim3(256), 0, h->stream, src, dst, nv, md);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// ==========================================================================
+// x0gemm: gather-Herk per histogram bin.
+//   rcxq_slice(i,j) += sum_t conj(z_t(i)) * w_t * z_t(ipr_col-1+j)
+// Columns z_t live anywhere in `z` (offset col[t]); the bin's pair list is
+// [bstart[l], bstart[l+1]).  Work item = (list l, output slice, tile ti, tj).
+
+template <int BM, int BK>
+struct LdGatherA {  // A(i,t) = conj(z[col[t] + i])
+  static constexpr int R = BM * BK / kThreads;
+  static constexpr int KSTEP = kThreads / BM;
+  const cd* z; const int64_t* col; int M, K;
+  struct Regs { double re[R]; double im[R]; };
+  __device__ void fetch(int m0, int k0, Regs& r) const {
+    const int i = m0 + (threadIdx.x % BM);
+    const int kb = k0 + threadIdx.x / BM;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = kb + q * KSTEP;
+      cd v = (i < M && k < K) ? z[col[k] + i] : cd{0.0, 0.0};
+      r.re[q] = v.re;
+      r.im[q] = -v.im;
+    }
+  }
+  template <class S>
+  __device__ void store(S& s, const Regs& r) const {
+    const int i = threadIdx.x % BM;
+    const int kb = threadIdx.x / BM;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      s.ar[(kb + q * KSTEP) * S::LDA + i] = r.re[q];
+      s.ai[(kb + q * KSTEP) * S::LDA + i] = r.im[q];
+    }
+  }
+};
+
+template <int BN, int BK>
+struct LdGatherB {  // B(t,j) = w[t] * z[col[t] + off + j]
+  static constexpr int R = BN * BK / kThreads;
+  static constexpr int KSTEP = kThreads / BN;
+  const cd* z; const int64_t* col; const double* w; int64_t off; int N, K;
+  struct Regs { double re[R]; double im[R]; };
+  __device__ void fetch(int n0, int k0, Regs& r) const {
+    const int j = n0 + (threadIdx.x % BN);
+    const int kb = k0 + threadIdx.x / BN;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = kb + q * KSTEP;
+      if (j < N && k < K) {
+        const double wt = w[k];
+        cd v = z[col[k] + off + j];
+        r.re[q] = wt * v.re;
+        r.im[q] = wt * v.im;
+      } else {
+        r.re[q] = 0.0;
+        r.im[q] = 0.0;
+      }
+    }
+  }
+  template <class S>
+  __device__ void store(S& s, const Regs& r) const {
+    const int j = threadIdx.x % BN;
+    const int kb = threadIdx.x / BN;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      s.br[(kb + q * KSTEP) * S::LDB + j] = r.re[q];
+      s.bi[(kb + q * KSTEP) * S::LDB + j] = r.im[q];
+    }
+  }
+};
+
+struct X0Item { int list, slice, ti, tj; };
+
+template <int BT>
+__global__ __launch_bounds__(kThreads) void x0_herk_kernel(const cd* __restrict__ z,
+                                                           const int64_t* __restrict__ col,
+                                                           const double* __restrict__ w,
+                                                           const int64_t* __restrict__ bstart,
+                                                           const X0Item* __restrict__ items,
+                                                           cd* __restrict__ rcxq, int64_t slice_elems,
+                                                           int npr, int npr_col, int ipr_col,
+                                                           int mirror) {
+  constexpr int BK = 16;
+  __shared__ StageF64<BT, BT, BK> s;
+  const X0Item it = items[blockIdx.x];
+  const int64_t b0 = bstart[it.list];
+  const int K = (int)(bstart[it.list + 1] - b0);
+  const int m0 = it.ti * BT, n0 = it.tj * BT;
+  LdGatherA<BT, BK> la{z, col + b0, npr, K};
+  LdGatherB<BT, BK> lb{z, col + b0, w + b0, (int64_t)(ipr_col - 1), npr_col, K};
+  CoreF64<BT, BT, BK, false> core;
+  gemm_mainloop<BT, BT, BK, false>(la, lb, m0, n0, K, s, core);
+  cd* r = rcxq + slice_elems * it.slice;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool mir = mirror && (it.ti != it.tj);
+  core.for_each(wave >> 1, wave & 1, lane, [&](int rr, int cc, double xr, double xi) {
+    const int i = m0 + rr, j = n0 + cc;
+    if (i >= npr || j >= npr_col) return;
+    cd* p = r + i + (int64_t)npr * j;
+    cd o = *p;
+    *p = cd{o.re + xr, o.im + xi};
+    if (mir) {
+      cd* q = r + j + (int64_t)npr * i;
+      cd oq = *q;
+      *q = cd{oq.re + xr, oq.im - xi};
+    }
+  });
+}
+
+ecalj_status x0_herk(ecalj_ctx* h, const cd* z, const int64_t* d_col, const double* d_w,
+                     const int64_t* d_bstart, const void* d_items, int nitems, cd* rcxq,
+                     int64_t slice_elems, int npr, int npr_col, int ipr_col, int mirror) {
+  if (nitems == 0) return ECALJ_OK;
+  hipLaunchKernelGGL(x0_herk_kernel<64>, dim3(nitems), dim3(kThreads), 0, h->stream, z, d_col, d_w,
+                     d_bstart, static_cast<const X0Item*>(d_items), rcxq, slice_elems, npr, npr_col,
+                     ipr_col, mirror);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// ==========================================================================
+// Hilbert transform to the imaginary axis: zxqi(e,v) = sum_iw coef(v,iw) rcxq(e,iw)
+template <int NV>
+__global__ __launch_bounds__(256) void hilbert_kernel(cd* __restrict__ rcxq, int64_t slice, int nwhis,
+                                                      int nv, const double* __restrict__ coef,
+                                                      const double* __restrict__ scale,
+                                                      cd* __restrict__ zxqi, int64_t n) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double ar[NV], ai[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) { ar[v] = 0.0; ai[v] = 0.0; }
+  for (int iw = 1; iw <= nwhis; ++iw) {
+    cd x = rcxq[e + slice * iw];
+    if (scale) {
+      const double s = scale[iw - 1];
+      rcxq[e + slice * iw] = cd{s * x.re, s * x.im};
+    }
+    const double* c = coef + (int64_t)nv * (iw - 1);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (v < nv) {
+        ar[v] += c[v] * x.re;
+        ai[v] += c[v] * x.im;
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+    if (v < nv) zxqi[e + n * v] = cd{ar[v], ai[v]};
+}
+
+ecalj_status hilbert(ecalj_ctx* h, cd* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
+                     const double* d_scale, cd* zxqi, int64_t n) {
+  if (n == 0) return ECALJ_OK;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (nv <= 8)
+    hipLaunchKernelGGL(hilbert_kernel<8>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+  else if (nv <= 16)
+    hipLaunchKernelGGL(hilbert_kernel<16>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+  else if (nv <= 32)
+    hipLaunchKernelGGL(hilbert_kernel<32>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+  else if (nv <= 64)
+    hipLaunchKernelGGL(hilbert_kernel<64>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+  else
+    return ECALJ_ERR_DIM;
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// ==========================================================================
+// Composed-weight chi0: B(p,j) = w(v,p) * z(j,p), A(i,p) = conj(z(i,p)).
+template <int BN, int BK>
+struct LdWeightedZT {  // B(k=p, j) = w[v + nf*p] * z[j + ni*p]
+  static constexpr int R = BN * BK / kThreads;
+  static constexpr int KSTEP = kThreads / BN;
+  const cd* z; const double* w; int64_t ldz; int nf, v, N, K;
+  struct Regs { double re[R]; double im[R]; };
+  __device__ void fetch(int n0, int k0, Regs& r) const {
+    const int j = n0 + (threadIdx.x % BN);
+    const int kb = k0 + threadIdx.x / BN;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = kb + q * KSTEP;
+      if (j < N && k < K) {
+        const double wt = w[v + (int64_t)nf * k];
+        cd x = z[j + ldz * k];
+        r.re[q] = wt * x.re;
+        r.im[q] = wt * x.im;
+      } else {
+        r.re[q] = 0.0;
+        r.im[q] = 0.0;
+      }
+    }
+  }
+  template <class S>
+  __device__ void store(S& s, const Regs& r) const {
+    const int j = threadIdx.x % BN;
+    const int kb = threadIdx.x / BN;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      s.br[(kb + q * KSTEP) * S::LDB + j] = r.re[q];
+      s.bi[(kb + q * KSTEP) * S::LDB + j] = r.im[q];
+    }
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void chi0_freq_kernel(const cd* __restrict__ z,
+                                                             const double* __restrict__ w, int ni,
+                                                             int npairs, int nf, cd* __restrict__ chi) {
+  constexpr int BT = 64, BK = 16;
+  __shared__ StageF64<BT, BT, BK> s;
+  const int v = blockIdx.z;
+  const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
+  LdA_N<BT, BK, false> la{z, (int64_t)ni, ni, npairs, -1.0};   // A(i,p) = conj(z(i,p)): column-major, rows contiguous
+  LdWeightedZT<BT, BK> lb{z, w, (int64_t)ni, nf, v, ni, npairs};
+  CoreF64<BT, BT, BK, false> core;
+  gemm_mainloop<BT, BT, BK, false>(la, lb, m0, n0, npairs, s, core);
+  cd* c = chi + (int64_t)ni * ni * v;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  core.for_each(wave >> 1, wave & 1, lane, [&](int rr, int cc, double xr, double xi) {
+    const int i = m0 + rr, j = n0 + cc;
+    if (i < ni && j < ni) c[i + (int64_t)ni * j] = cd{xr, xi};
+  });
+}
+
+ecalj_status chi0_freq(ecalj_ctx* h, const cd* z, const double* w, int ni, int npairs, int nf, cd* chi) {
+  if (ni == 0 || nf == 0) return ECALJ_OK;
+  if (nf > 65535) return ECALJ_ERR_DIM;
+  dim3 grid((ni + 63) / 64, (ni + 63) / 64, nf);
+  hipLaunchKernelGGL(chi0_freq_kernel, grid, dim3(kThreads), 0, h->stream, z, w, ni, npairs, nf, chi);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// ==========================================================================
+// W build helpers
+// eps = delta - fac * s_i chi_ij s_j (no M->E transform)
+__global__ void eps_direct_kernel(const cd* __restrict__ chi, const double* __restrict__ s, double fac,
+                                  int n, int nw, cd* __restrict__ eps) {
+  const int64_t tot = (int64_t)n * n * nw;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ij = e % ((int64_t)n * n);
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    const double sc = fac * s[i] * s[j];
+    cd x = chi[e];
+    eps[e] = cd{(i == j ? 1.0 : 0.0) - sc * x.re, -sc * x.im};
+  }
+}
+
+// w = s_i inv_ij s_j - delta_ij s_i s_j, written with leading dimension ldw
+__global__ void wc_form_kernel(const cd* __restrict__ inv, const double* __restrict__ s, int n, int nw,
+                               cd* __restrict__ w, int64_t ldw, int64_t wstride) {
+  const int64_t tot = (int64_t)n * n * nw;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t nn = (int64_t)n * n;
+    const int iw = (int)(e / nn);
+    const int64_t ij = e % nn;
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    const double sc = s[i] * s[j];
+    cd x = inv[e];
+    w[(int64_t)iw * wstride + i + ldw * j] = cd{sc * x.re - (i == j ? sc : 0.0), sc * x.im};
+  }
+}
+
+ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac, int n, int nw, cd* eps) {
+  const int64_t tot = (int64_t)n * n * nw;
+  if (tot == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(eps_direct_kernel, dim3(blocks), dim3(256), 0, h->stream, chi, s, fac, n, nw, eps);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const double* s, int n, int nw, cd* w, int64_t ldw,
+                     int64_t wstride) {
+  const int64_t tot = (int64_t)n * n * nw;
+  if (tot == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(wc_form_kernel, dim3(blocks), dim3(256), 0, h->stream, inv, s, n, nw, w, ldw, wstride);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// --------------------------------------------------------------------------
+// Blocked Gauss-Jordan inversion without pivoting, batched.
+// Step for pivot block K = [k0, k0+b):
+//   (1) gj_diag: P = A[K,K] -> Pinv (LDS elimination); Cp = A[:,K]; A[K,K] = I
+//   (2) row panel:  A[K,:] = Pinv * A[K,:]          (A[K,K] becomes Pinv)
+//   (3) trailing:   A[i,:] = (j in K ? 0 : A[i,:]) - Cp[i,:] * A[K,:],  i not in K
+constexpr int GJB = 32;
+
+__global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n, int64_t lda,
+                                                      int64_t astride, int k0, int b,
+                                                      cd* __restrict__ pinv, cd* __restrict__ cp,
+                                                      int* __restrict__ flag) {
+  __shared__ cd p[GJB][GJB + 1];
+  const int mat = blockIdx.x;
+  cd* A = a + astride * mat;
+  cd* P = pinv + (int64_t)GJB * GJB * mat;
+  cd* Cp = cp + (int64_t)n * GJB * mat;
+  const int tid = threadIdx.x;
+  // copy the column panel and the pivot block
+  for (int64_t e = tid; e < (int64_t)n * b; e += blockDim.x) {
+    const int i = (int)(e % n), l = (int)(e / n);
+    Cp[i + (int64_t)n * l] = A[i + lda * (k0 + l)];
+  }
+  for (int e = tid; e < b * b; e += blockDim.x) {
+    const int i = e % b, l = e / b;
+    p[i][l] = A[(k0 + i) + lda * (k0 + l)];
+  }
+  __syncthreads();
+  // in-place Gauss-Jordan on the b x b block
+  for (int kk = 0; kk < b; ++kk) {
+    const cd piv = p[kk][kk];
+    if (!(hypot(piv.re, piv.im) > 1e-300)) {
+      if (tid == 0) *flag = 1;
+    }
+    const double mag = piv.re * piv.re + piv.im * piv.im;
+    const double dr = piv.re / mag, di = -piv.im / mag;  // 1/piv
+    __syncthreads();
+    // row kk scaled by 1/piv (except the pivot column)
+    for (int l = tid; l < b; l += blockDim.x) {
+      if (l == kk) continue;
+      cd x = p[kk][l];
+      p[kk][l] = cd{x.re * dr - x.im * di, x.re * di + x.im * dr};
+    }
+    __syncthreads();
+    // other rows: p[i][l] -= p[i][kk] * p[kk][l]
+    for (int e = tid; e < b * b; e += blockDim.x) {
+      const int i = e % b, l = e / b;
+      if (i == kk || l == kk) continue;
+      cd f = p[i][kk], x = p[kk][l];
+      p[i][l] = cd{p[i][l].re - (f.re * x.re - f.im * x.im), p[i][l].im - (f.re * x.im + f.im * x.re)};
+    }
+    __syncthreads();
+    // pivot column: p[i][kk] = -p[i][kk]/piv ; pivot = 1/piv
+    for (int i = tid; i < b; i += blockDim.x) {
+      cd f = p[i][kk];
+      if (i == kk)
+        p[i][kk] = cd{dr, di};
+      else
+        p[i][kk] = cd{-(f.re * dr - f.im * di), -(f.re * di + f.im * dr)};
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < GJB * GJB; e += blockDim.x) {
+    const int i = e % GJB, l = e / GJB;
+    P[i + GJB * l] = (i < b && l < b) ? p[i][l] : cd{0.0, 0.0};
+  }
+  for (int e = tid; e < b * b; e += blockDim.x) {
+    const int i = e % b, l = e / b;
+    A[(k0 + i) + lda * (k0 + l)] = cd{i == l ? 1.0 : 0.0, 0.0};
+  }
+}
+
+ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat) {
+  if (n == 0 || nmat == 0) return ECALJ_OK;
+  cd *pinv, *cp;
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJP, sizeof(cd) * GJB * GJB * nmat, (void**)&pinv));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJC, sizeof(cd) * (size_t)n * GJB * nmat, (void**)&cp));
+  for (int k0 = 0; k0 < n; k0 += GJB) {
+    const int b = std::min(GJB, n - k0);
+    hipLaunchKernelGGL(gj_diag_kernel, dim3(nmat), dim3(256), 0, h->stream, a, n, lda, astride, k0, b, pinv, cp,
+                       h->d_flag);
+    ECALJ_HIP(hipGetLastError());
+    // (2) row panel: A[K,:] = Pinv (b x b) * A[K,:] (b x n), in place per column tile
+    GemmCall r;
+    r.M = b; r.N = n; r.K = b;
+    r.A = pinv; r.lda = GJB; r.bA = {(int64_t)GJB * GJB, 0};
+    r.B = a + k0; r.ldb = lda; r.bB = {astride, 0};
+    r.C = a + k0; r.ldc = lda; r.bC = {astride, 0};
+    r.batch = nmat; r.div = nmat;
+    ECALJ_TRY(gemm_f64(h, r));
+    // (3) trailing update over all rows not in K
+    GemmCall t;
+    t.M = n; t.N = n; t.K = b;
+    t.A = cp; t.lda = n; t.bA = {(int64_t)n * GJB, 0};
+    t.B = a + k0; t.ldb = lda; t.bB = {astride, 0};
+    t.C = a; t.ldc = lda; t.bC = {astride, 0};
+    t.batch = nmat; t.div = nmat;
+    t.epi = EPI_GJ; t.gj_k0 = k0; t.gj_k1 = k0 + b;
+    ECALJ_TRY(gemm_f64(h, t));
+  }
+  return ECALJ_OK;
+}
+
+// ==========================================================================
+// Sigma_c
+// Unpack the keepwv upper-triangle form and take the conjugate transpose in
+// one pass:  A_v(I,J) = conj(wc_v(J,I)) = (I <= J) ? W_v(I,J) : conj(W_v(J,I)).
+__global__ void sig_unpack_kernel(const cd* __restrict__ w, int64_t ldw, int64_t wstride, int n, int nw,
+                                  cd* __restrict__ out) {
+  const int64_t nn = (int64_t)n * n, tot = nn * nw;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int iw = (int)(e / nn);
+    const int64_t ij = e % nn;
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    const cd* W = w + wstride * iw;
+    cd x;
+    if (i <= j) {
+      x = W[i + ldw * j];
+    } else {
+      cd y = W[j + ldw * i];
+      x = cd{y.re, -y.im};
+    }
+    out[e] = x;
+  }
+}
+
+// B(kk, c) = wgt[v + (niw+1)*c] * zmel[J + n*c],  v = kk / n, J = kk % n
+template <int BN, int BK>
+struct LdSigB {
+  static constexpr int R = BN * BK / kThreads;
+  static constexpr int JSTEP = kThreads / BK;
+  const cd* zmel; const double* wgt; int n, nw1, N, K;
+  struct Regs { double re[R]; double im[R]; };
+  __device__ void fetch(int n0, int k0, Regs& r) const {
+    const int kk = k0 + (threadIdx.x % BK);
+    const int v = kk / n;
+    const int J = kk - v * n;
+    const int cb = n0 + threadIdx.x / BK;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int c = cb + q * JSTEP;
+      if (c < N && kk < K) {
+        const double g = wgt[v + (int64_t)nw1 * c];
+        cd x = zmel[J + (int64_t)n * c];
+        r.re[q] = g * x.re;
+        r.im[q] = g * x.im;
+      } else {
+        r.re[q] = 0.0;
+        r.im[q] = 0.0;
+      }
+    }
+  }
+  template <class S>
+  __device__ void store(S& s, const Regs& r) const {
+    const int k = threadIdx.x % BK;
+    const int jb = threadIdx.x / BK;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      s.br[k * S::LDB + jb + q * JSTEP] = r.re[q];
+      s.bi[k * S::LDB + jb + q * JSTEP] = r.im[q];
+    }
+  }
+};
+
+// czmelwc_b = Wh_b (n x nw1*n) * B_b ; one z-slice per batch (internal k)
+template <int BM, int BN>
+__global__ __launch_bounds__(kThreads) void sig_freq_gemm_kernel(const cd* __restrict__ wh, int64_t wh_stride,
+                                                                 const cd* __restrict__ zmel, int64_t z_stride,
+                                                                 const double* __restrict__ wgt, int64_t g_stride,
+                                                                 int n, int nw1, int ncol, cd* __restrict__ cz,
+                                                                 int64_t cz_stride) {
+  constexpr int BK = 16;
+  __shared__ StageF64<BM, BN, BK> s;
+  const int b = blockIdx.z;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int K = n * nw1;
+  LdA_N<BM, BK, false> la{wh + wh_stride * b, (int64_t)n, n, K, 1.0};
+  LdSigB<BN, BK> lb{zmel + z_stride * b, wgt + g_stride * b, n, nw1, ncol, K};
+  CoreF64<BM, BN, BK, false> core;
+  gemm_mainloop<BM, BN, BK, false>(la, lb, m0, n0, K, s, core);
+  cd* c = cz + cz_stride * b;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  core.for_each(wave >> 1, wave & 1, lane, [&](int rr, int cc, double xr, double xi) {
+    const int i = m0 + rr, j = n0 + cc;
+    if (i < n && j < ncol) c[i + (int64_t)n * j] = cd{xr, xi};
+  });
+}
+
+// zsec partials: part[blk][m + ntq*m'] = sum_{r in chunk} conj(cz[r + R*m]) zmel[r + R*m']
+constexpr int SIG_RC = 32;
+__global__ __launch_bounds__(256) void sig_final_partial_kernel(const cd* __restrict__ cz, const cd* __restrict__ zmel,
+                                                                int64_t R, int ntq, int64_t cz_stride,
+                                                                int64_t z_stride, int chunks_per_batch,
+                                                                cd* __restrict__ part) {
+  extern __shared__ cd sh[];
+  cd* sa = sh;                        // SIG_RC x ntq
+  cd* sb = sh + SIG_RC * ntq;         // SIG_RC x ntq
+  const int b = blockIdx.x / chunks_per_batch;
+  const int ch = blockIdx.x % chunks_per_batch;
+  const cd* A = cz + cz_stride * b;
+  const cd* B = zmel + z_stride * b;
+  const int np = ntq * ntq;
+  double accr[8], acci[8];
+  for (int q = 0; q < 8; ++q) { accr[q] = 0.0; acci[q] = 0.0; }
+  const int64_t rows_per_chunk = (R + chunks_per_batch - 1) / chunks_per_batch;
+  const int64_t r_begin = ch * rows_per_chunk;
+  const int64_t r_end = std::min<int64_t>(R, r_begin + rows_per_chunk);
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += SIG_RC) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < SIG_RC * ntq; e += blockDim.x) {
+      const int rr = e % SIG_RC, m = e / SIG_RC;
+      const int64_t r = r0 + rr;
+      const bool ok = r < r_end;
+      sa[e] = ok ? A[r + R * m] : cd{0.0, 0.0};
+      sb[e] = ok ? B[r + R * m] : cd{0.0, 0.0};
+    }
+    __syncthreads();
+    for (int q = 0; q < 8; ++q) {
+      const int pidx = threadIdx.x + 256 * q;
+      if (pidx >= np) break;
+      const int m = pidx % ntq, mp = pidx / ntq;
+      const cd* pa = sa + SIG_RC * m;
+      const cd* pb = sb + SIG_RC * mp;
+      double sr = 0.0, si = 0.0;
+#pragma unroll 8
+      for (int rr = 0; rr < SIG_RC; ++rr) {
+        const cd x = pa[rr], y = pb[rr];
+        sr += x.re * y.re + x.im * y.im;
+        si += x.re * y.im - x.im * y.re;
+      }
+      accr[q] += sr;
+      acci[q] += si;
+    }
+  }
+  cd* P = part + (int64_t)np * blockIdx.x;
+  for (int q = 0; q < 8; ++q) {
+    const int pidx = threadIdx.x + 256 * q;
+    if (pidx >= np) break;
+    P[pidx] = cd{accr[q], acci[q]};
+  }
+}
+
+// zsec(m,m') += sum over partials in fixed order; then clip Im(zsec_mm) <= 0
+__global__ void sig_final_reduce_kernel(const cd* __restrict__ part, int nparts, int ntq, cd* __restrict__ zsec,
+                                        int64_t ldz) {
+  const int np = ntq * ntq;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int k = 0; k < nparts; ++k) {
+      cd x = part[(int64_t)np * k + p];
+      sr += x.re;
+      si += x.im;
+    }
+    const int m = p % ntq, mp = p / ntq;
+    cd* z = zsec + m + ldz * mp;
+    cd o = *z;
+    double zr = o.re + sr, zi = o.im + si;
+    if (m == mp && zi > 0.0) zi = 0.0;
+    *z = cd{zr, zi};
+  }
+}
+
+ecalj_status sig_contract(ecalj_ctx* h, int nbatch, const cd* zmel, int n, int nm, int ntq, int niw,
+                          const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
+  const int nw1 = niw + 1;
+  const int ncol = nm * ntq;
+  if (n == 0 || ncol == 0 || nbatch == 0) return ECALJ_OK;
+  const int64_t nn = (int64_t)n * n;
+  cd *wh, *cz, *part;
+  // W unpacked per internal k; processed in groups to bound scratch memory
+  const size_t wh_bytes_per = sizeof(cd) * nn * nw1;
+  const size_t cz_bytes_per = sizeof(cd) * (size_t)n * ncol;
+  int group = (int)std::max<size_t>(1, std::min<size_t>(nbatch, ((size_t)8 << 30) / (wh_bytes_per + cz_bytes_per)));
+  group = std::min(group, 65535);
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_WH, wh_bytes_per * group, (void**)&wh));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CZ, cz_bytes_per * group, (void**)&cz));
+  const int64_t R = (int64_t)n * nm;
+  int chunks = (int)std::max<int64_t>(1, std::min<int64_t>(64, (R + 4095) / 4096));
+  const int np = ntq * ntq;
+  if (np > 8 * 256) return ECALJ_ERR_DIM;
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_PART, sizeof(cd) * (size_t)np * chunks * nbatch, (void**)&part));
+  for (int b0 = 0; b0 < nbatch; b0 += group) {
+    const int gb = std::min(group, nbatch - b0);
+    const int64_t wstride = ldw * n * nw1;
+    const int64_t tot = nn * nw1;
+    const int ublocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+    for (int q = 0; q < gb; ++q) {
+      hipLaunchKernelGGL(sig_unpack_kernel, dim3(ublocks), dim3(256), 0, h->stream, wc + wstride * (b0 + q), ldw, ldw * n,
+                         n, nw1, wh + tot * q);
+      ECALJ_HIP(hipGetLastError());
+    }
+    int tev = -1;
+    ECALJ_TRY(timing_begin(h, 0, &tev));
+    dim3 grid((n + 63) / 64, (ncol + 63) / 64, gb);
+    if (grid.y > 65535) return ECALJ_ERR_DIM;
+    hipLaunchKernelGGL((sig_freq_gemm_kernel<64, 64>), grid, dim3(kThreads), 0, h->stream, wh, tot,
+                       zmel + (int64_t)n * ncol * b0, (int64_t)n * ncol, wgt + (int64_t)nw1 * ncol * b0,
+                       (int64_t)nw1 * ncol, n, nw1, ncol, cz, (int64_t)n * ncol);
+    ECALJ_HIP(hipGetLastError());
+    ECALJ_TRY(timing_end(h, 0, tev));
+    const size_t shm = sizeof(cd) * 2 * SIG_RC * ntq;
+    hipLaunchKernelGGL(sig_final_partial_kernel, dim3(chunks * gb), dim3(256), shm, h->stream, cz,
+                       zmel + (int64_t)n * ncol * b0, R, ntq, (int64_t)n * ncol, (int64_t)n * ncol, chunks,
+                       part + (int64_t)np * chunks * b0);
+    ECALJ_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(sig_final_reduce_kernel, dim3((np + 255) / 256), dim3(256), 0, h->stream, part, chunks * nbatch,
+                     ntq, zsec, ldz);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// --------------------------------------------------------------------------
+// Imaginary-axis weights (FP64), one thread per (batch, it, itp).
+__global__ void sig_weights_kernel(int nbatch, int niw, int nm, int ntq, int ns1, int nctot,
+                                   const double* __restrict__ omega, const double* __restrict__ ekc,
+                                   double esmr, double ua, const double* __restrict__ wkkr,
+                                   const double* __restrict__ x, const double* __restrict__ wt,
+                                   const double* __restrict__ expa, double* __restrict__ wgt) {
+  const int64_t tot = (int64_t)nbatch * nm * ntq;
+  const double pi = 3.14159265358979323846;
+  const double rmax = 2.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / ((int64_t)nm * ntq));
+    const int64_t r = e % ((int64_t)nm * ntq);
+    const int itl = (int)(r % nm);     // 0-based middle state
+    const int itp = (int)(r / nm);
+    const int it = ns1 + itl;          // Fortran index
+    const double we = 0.5 * (omega[(int64_t)ntq * b + itp] - ekc[(int64_t)nm * b + itl]);
+    const double aw = fabs(ua * we);
+    const double aw2 = aw * aw;
+    const double sgn = signbit(we) ? -1.0 : 1.0;
+    const double sig = 0.5 * esmr;
+    const double sig2 = 2.0 * sig * sig;
+    const double wk = wkkr[b];
+    double* out = wgt + (int64_t)(niw + 1) * e;
+    if (it <= nctot) {
+      double s = 0.0;
+      for (int i = 0; i < niw; ++i) {
+        const double cons = 1.0 / (we * we * x[i] * x[i] + (1.0 - x[i]) * (1.0 - x[i]));
+        const double w1 = we * cons * wt[i] * (-1.0 / pi);
+        out[1 + i] = wk * w1;
+        s += w1 * expa[i];
+      }
+      const double w0 = (fabs(we) < rmax / ua) ? (-s - 0.5 * sgn * exp(we * we * ua * ua) * erfc(ua * fabs(we))) : 0.0;
+      out[0] = wk * w0;
+    } else {
+      double s = 0.0;
+      for (int i = 0; i < niw; ++i) {
+        const double omd = 1.0 / x[i] - 1.0;
+        const double omd2w = omd * omd + we * we;
+        const double rr = omd2w / sig2;
+        double cons;
+        if (rr > 5e-3) {
+          cons = (1.0 - exp(-rr)) / omd2w;
+        } else {
+          const double s2 = sig2, s3 = s2 * s2 * s2, s4 = s3 * s2, s5 = s4 * s2, s6 = s5 * s2;
+          cons = 1.0 / s2 - omd2w / (s2 * s2) / 2.0 + omd2w * omd2w / s3 / 6.0 - omd2w * omd2w * omd2w / s4 / 24.0 +
+                 omd2w * omd2w * omd2w * omd2w / s5 / 120.0 - omd2w * omd2w * omd2w * omd2w * omd2w / s6 / 720.0;
+        }
+        out[1 + i] = wk * (-we * cons * wt[i] / (x[i] * x[i]) / pi);
+        s += cons * expa[i] * wt[i] / (x[i] * x[i]);
+      }
+      out[0] = wk * (we * s / pi + sgn * 0.5 * exp(aw2) * (erfc(sqrt(aw2 + we * we / sig2)) - erfc(aw)));
+    }
+  }
+}
+
+ecalj_status sig_weights(ecalj_ctx* h, int nbatch, int niw, int nm, int ntq, int ns1, int nctot,
+                         const double* omega, const double* ekc, double esmr, double ua, const double* wkkr,
+                         const double* x, const double* wt, const double* expa, double* wgt) {
+  const int64_t tot = (int64_t)nbatch * nm * ntq;
+  if (tot == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(sig_weights_kernel, dim3(blocks), dim3(256), 0, h->stream, nbatch, niw, nm, ntq, ns1, nctot,
+                     omega, ekc, esmr, ua, wkkr, x, wt, expa, wgt);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// --------------------------------------------------------------------------
+// Device utilities
+__global__ void zero_kernel(uint4* __restrict__ p, int64_t n16, unsigned char* __restrict__ tail, int ntail) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n16; e += stride) p[e] = uint4{0, 0, 0, 0};
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+ecalj_status dev_zero(ecalj_ctx* h, void* p, size_t bytes) {
+  if (bytes == 0) return ECALJ_OK;
+  if (reinterpret_cast<uintptr_t>(p) % 16 != 0) return ECALJ_ERR_ARG;
+  const int64_t n16 = bytes / 16;
+  const int ntail = (int)(bytes % 16);
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n16 + 255) / 256, 16384));
+  hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, h->stream, (uint4*)p, n16,
+                     (unsigned char*)p + n16 * 16, ntail);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// out[k][m](I,J) = d_k(I) * a[m](I,J) * conj(d_k(J))
+__global__ void phase_sim_kernel(const cd* __restrict__ a, int n, int nmat, const cd* __restrict__ d, int nk,
+                                 cd* __restrict__ out) {
+  const int64_t nn = (int64_t)n * n, per = nn * nmat, tot = per * nk;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e / per);
+    const int64_t r = e % per;
+    const int64_t ij = r % nn;
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    const cd x = a[r], di = d[(int64_t)n * k + i], dj = d[(int64_t)n * k + j];
+    // di * x
+    const double tr = di.re * x.re - di.im * x.im, ti = di.re * x.im + di.im * x.re;
+    // * conj(dj)
+    out[e] = cd{tr * dj.re + ti * dj.im, ti * dj.re - tr * dj.im};
+  }
+}
+
+ecalj_status phase_similarity(ecalj_ctx* h, const cd* a, int n, int nmat, const cd* d, int nk, cd* out) {
+  const int64_t tot = (int64_t)n * n * nmat * nk;
+  if (tot == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(phase_sim_kernel, dim3(blocks), dim3(256), 0, h->stream, a, n, nmat, d, nk, out);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+}  // namespace ecalj

@@ -1,0 +1,227 @@
+/*
+ * ecalj_gpu.h -- C ABI of the MI355X (gfx950) implementation of ecalj's QSGW
+ * chi0 -> W -> Sigma_c path.
+ *
+ * Conventions (those of the Fortran callers):
+ *   - arrays are column-major; complex arrays are interleaved (re, im)
+ *     doubles (Fortran complex(8)); index arrays hold 1-based values;
+ *   - functions without a suffix take HOST pointers: they copy inputs to the
+ *     device, compute, copy results back and synchronise;
+ *   - functions with suffix _dev take DEVICE pointers and are asynchronous on
+ *     the handle's stream (ecalj_sync waits);
+ *   - every function returns an ecalj_status; nothing aborts the process.
+ *     Callers map a nonzero status to their own error exit (the reference
+ *     calls rx(), S/rx.f90:2-12).
+ *   - one handle per host thread; a handle is not re-entrant.  Handles are
+ *     independent of each other.
+ *
+ * Each entry point names the reference routine it replaces (S/ =
+ * SRC/subroutines/ of ecalj).
+ */
+#ifndef ECALJ_GPU_H
+#define ECALJ_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ecalj_ctx* ecalj_handle;
+
+typedef enum {
+  ECALJ_OK = 0,
+  ECALJ_ERR_DIM = 1,         /* inconsistent or unsupported dimensions       */
+  ECALJ_ERR_SINGULAR = 2,    /* epsilon matrix singular (zero pivot)         */
+  ECALJ_ERR_HIP = 3,         /* HIP runtime error                            */
+  ECALJ_ERR_ARG = 4,         /* NULL pointer or bad enum                     */
+  ECALJ_ERR_NOMEM = 5,       /* device allocation failed                     */
+  ECALJ_ERR_STATE = 6        /* required set-up call missing (set_ppb, ...)  */
+} ecalj_status;
+
+/* Arithmetic of the contractions (zmel, chi0, M<->E transforms, Sigma_c).
+ * The epsilon inversion, the weights and the QP update are always FP64. */
+typedef enum {
+  ECALJ_PREC_F64 = 0,     /* complex128 data, FP64 MFMA                          */
+  ECALJ_PREC_F32 = 1,     /* complex64 products on FP32 MFMA, FP32 accumulation  */
+  ECALJ_PREC_BF16X3 = 2   /* hi/lo bf16 split, 3 MFMA products, FP32 accumulation */
+} ecalj_precision;
+
+/* ---- handle ------------------------------------------------------------ */
+/* Replaces gpu_init (S/m_gpu.f90:14-68): binds device `device` (the caller
+ * passes local_rank mod ndevs as the reference does) and creates a stream. */
+ecalj_status ecalj_create(ecalj_handle* h, int device);
+ecalj_status ecalj_destroy(ecalj_handle h);
+ecalj_status ecalj_set_precision(ecalj_handle h, int precision);
+/* Use an external hipStream_t (e.g. the caller's framework stream); NULL restores the handle's own. */
+ecalj_status ecalj_set_stream(ecalj_handle h, void* hip_stream);
+ecalj_status ecalj_sync(ecalj_handle h);
+const char* ecalj_status_string(int status);
+/* Kernel timing with HIP events on the handle's stream: when enabled, the
+ * launches of kernel class `kclass` are bracketed by events; ecalj_kernel_time
+ * returns their summed duration (ms) and count since the last reset.
+ * kclass: 0 = Sigma_c frequency GEMM, 1 = chi0 bin accumulation,
+ * 2 = zmel GEMMs, 3 = W build (transforms + inversion), 4 = Hilbert. */
+ecalj_status ecalj_kernel_timing(ecalj_handle h, int enable);
+ecalj_status ecalj_kernel_time(ecalj_handle h, int kclass, double* ms, int64_t* launches);
+
+/* ---- device memory ------------------------------------------------------
+ * The reference keeps its arrays on the device with OpenACC data directives
+ * (!$acc enter data create/copyin, !$acc update host; e.g.
+ * S/x0kf_v4h.f90:190-207, S/m_sxcf_gemm.f90:362).  A caller without such a
+ * runtime allocates and moves device memory through the library, so that the
+ * process holds one HIP runtime.  ecalj_dev_zero uses a fill kernel (no
+ * hipMemset), so it may be captured in a HIP graph.  ecalj_dev_phase_similarity
+ * forms out_k = diag(d_k) A diag(d_k)^H for nk phase vectors d_k (ngb
+ * complex each) and nmat matrices A (ngb x ngb): the synthetic W_c(k) inputs
+ * of the benchmark workload; it has no reference counterpart. */
+ecalj_status ecalj_dev_alloc(ecalj_handle h, size_t bytes, void** ptr);
+ecalj_status ecalj_dev_free(ecalj_handle h, void* ptr);
+ecalj_status ecalj_memcpy_h2d(ecalj_handle h, void* dst, const void* src, size_t bytes);
+ecalj_status ecalj_memcpy_d2h(ecalj_handle h, void* dst, const void* src, size_t bytes);
+ecalj_status ecalj_memcpy_d2d(ecalj_handle h, void* dst, const void* src, size_t bytes);
+ecalj_status ecalj_dev_zero(ecalj_handle h, void* ptr, size_t bytes);
+ecalj_status ecalj_dev_phase_similarity(ecalj_handle h, const void* a, int ngb, int nmat,
+                                        const void* d, int nk, void* out);
+
+/* ---- module state (set once per q / rotation, kept on the device) ------ */
+/* set_ppb (S/m_zmel.f90:110-145): real radial-angular integrals
+ * ppb(i1,i2,I) of each atom, blocks of nv[a] x nv[a] x mdim[a] concatenated.
+ * Atom a owns eigenvector rows [sum nv[<a], +nv[a]) and product-basis rows
+ * [sum mdim[<a], +mdim[a]).  Host pointer. */
+ecalj_status ecalj_set_ppb(ecalj_handle h, int natom, const int* nv, const int* mdim,
+                           const double* ppb);
+/* Readvcoud + set_m2e_prod_basis (S/m_readVcoud.f90:11-43,
+ * S/m_zmel.f90:46-68): eigenvalues vcoud(ngb) and eigenvectors
+ * zcousq(ngb,ngb) of the Coulomb matrix; vcousq = sqrt(vcoud).  Host pointers. */
+ecalj_status ecalj_set_coulomb(ecalj_handle h, int ngb, const double* vcoud, const double* zcousq);
+
+/* ---- zmel ---------------------------------------------------------------
+ * get_zmel_init_gemm (S/m_zmel.f90:146-597), MT valence part with identity
+ * rotation: zmel(I,n,m) = phase_a * sum_{i1,i2} conj(cphim(i2,n)) ppb(i1,i2,I) cphiq(i1,m)
+ * over the atom blocks; then zmel = zcousq^H zmel when is_m_basis == 0
+ * (:559-560), then conj when zmelconjg (:590-594).
+ * cphim: ndima x nm, cphiq: ndima x nq; zmel: ngb x nm x nq.
+ * phase: natom complex values (exp(-i 2pi kvec.tr)).
+ * nbatch > 1 evaluates nbatch independent problems with cphim, cphiq and
+ * zmel advanced by their own sizes (one call for all k-points of a q). */
+ecalj_status ecalj_zmel(ecalj_handle h, const double* phase, int ndima, int nm, const double* cphim,
+                        int nq, const double* cphiq, int is_m_basis, int zmelconjg, int nbatch,
+                        double* zmel);
+ecalj_status ecalj_zmel_dev(ecalj_handle h, const double* phase, int ndima, int nm,
+                            const void* cphim, int nq, const void* cphiq, int is_m_basis,
+                            int zmelconjg, int nbatch, void* zmel);
+/* As ecalj_zmel_dev with explicit batch strides (elements) of cphim and
+ * cphiq; a stride of 0 shares one block across the batch (the end states at
+ * q are the same for every internal k of Sigma_c, S/m_sxcf_gemm.f90:395). */
+ecalj_status ecalj_zmel_strided_dev(ecalj_handle h, const double* phase, int ndima, int nm,
+                                    const void* cphim, int64_t cphim_stride, int nq,
+                                    const void* cphiq, int64_t cphiq_stride, int is_m_basis,
+                                    int zmelconjg, int nbatch, void* zmel);
+
+/* ---- chi0: x0gemm -----------------------------------------------------
+ * x0gemm (S/x0gemm.f90:2-97) with its module inputs explicit
+ * (m_x0kf: icounkmink/icounkmaxk, iwini, iwend, itc, itpc, jpmc, icouini,
+ * whwc; m_zmel: zmel).  For every icoun in [icoun_lo, icoun_hi] with
+ * ns1 <= itc <= ns2 and every bin iw in [iwini, iwend]:
+ *   rcxq(:,:,iw*(3-2*jpm)) += conj(zmel(1:npr,it,itp)) (x) whw * zmel(ipr_col:ipr_col+npr_col-1,it,itp)
+ * rcxq: npr x npr_col x ((npm==1) ? nwhis+1 : 2*nwhis+1), Fortran lower
+ * bound (1-npm)*nwhis; zmel: nbb x (ns2-ns1+1) x nqtot.
+ * The _batched form accumulates nk such problems (one per k-point, zmel
+ * blocks of zmel_stride elements, index ranges icounkmin[k]..icounkmax[k],
+ * ns1[k]/ns2[k]) in one pass; this is the whole k loop of x0kf_zxq
+ * (S/x0kf_v4h.f90:233-290).  In the _dev form rcxq and zmel are device
+ * pointers; the index arrays stay on the host, where x0kf_v4hz_init builds
+ * them (S/x0kf_v4h.f90:48-142), and are turned into per-bin gather lists. */
+ecalj_status ecalj_x0gemm(ecalj_handle h, double* rcxq, int npr, int ipr_col, int npr_col,
+                          int nwhis, int npm, int ns1, int ns2, const double* zmel, int nbb,
+                          int nqtot, int icoun_lo, int icoun_hi, const int* iwini,
+                          const int* iwend, const int* itc, const int* itpc, const int* jpmc,
+                          const int* icouini, const double* whwc);
+ecalj_status ecalj_x0gemm_batched_dev(ecalj_handle h, void* rcxq, int npr, int ipr_col,
+                                      int npr_col, int nwhis, int npm, int nk, const int* ns1,
+                                      const int* ns2, const void* zmel, int64_t zmel_stride,
+                                      int nbb, int nqtot, const int* icounkmin,
+                                      const int* icounkmax, int ncoun, const int* iwini,
+                                      const int* iwend, const int* itc, const int* itpc,
+                                      const int* jpmc, const int* icouini, int ncount,
+                                      const double* whwc);
+
+/* ---- chi0: Hilbert transform to the imaginary axis ---------------------
+ * dpsion_chiq, imaginary branch with npm=1 (S/dpsion5.f90:211-232) with
+ * dpsion_init's imatt (S/dpsion5.f90:84-89,121-129, hilbertmat
+ * S/hilbertmat.f90:14-79) computed in FP64 on the host:
+ *   zxqi(:,:,iv) = sum_iw imatt(iv,iw) * (-wfac(iw)/(R-L)) * rcxq(:,:,iw)
+ * frhis(1:nwhis+1) bin edges, freqi(1:niw) imaginary frequencies.
+ * When scale_rcxq != 0 rcxq is scaled in place as the reference does. */
+ecalj_status ecalj_dpsion_imag(ecalj_handle h, double* rcxq, int npr, int npr_col, int nwhis,
+                               int niw, const double* frhis, const double* freqi, double ecut,
+                               int scale_rcxq, double* zxqi);
+ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_col, int nwhis,
+                                   int niw, const double* frhis, const double* freqi,
+                                   double ecut, int scale_rcxq, void* zxqi);
+
+/* ---- chi0: composed direct-frequency form ------------------------------
+ * chi(I,J,v) = sum_p w(v,p) conj(Z(I,p)) Z(J,p); equal to x0gemm followed by
+ * dpsion_chiq when w(v,p) = sum_bins whw * (-wfac/(R-L)) * imatt(v,bin). */
+ecalj_status ecalj_chi0_freq(ecalj_handle h, const double* z, const double* w, int ni,
+                             int npairs, int nf, double* chi);
+ecalj_status ecalj_chi0_freq_dev(ecalj_handle h, const void* z, const void* w, int ni,
+                                 int npairs, int nf, void* chi);
+
+/* ---- W_c build ---------------------------------------------------------
+ * WVIllwI body for q /= 0 (S/m_llw.f90:299-373), per frequency:
+ *   chi *= 2 when nspin == 1; [chi = zcousq^H chi zcousq];
+ *   eps = delta - vcousq_i chi_ij vcousq_j; eps^-1 (zminv, S/m_lapack.f90:21-47,108-176);
+ *   wc = vcousq_i eps^-1_ij vcousq_j - delta_ij vcoud_i; [wc = zcousq wc zcousq^H].
+ * chi: ngb x ngb x nw; wc: ldw x ngb x nw.  The inversion runs without
+ * pivoting: epsilon on the imaginary axis has a positive definite Hermitian
+ * part (chi is negative semidefinite), for which elimination without
+ * pivoting is stable; a pivot below 1e-300 returns ECALJ_ERR_SINGULAR. */
+ecalj_status ecalj_wc_build(ecalj_handle h, const double* chi, int ngb, int nw, int nspin,
+                            int is_x0_m_basis, int is_wc_m_basis, double* wc, int ldw);
+ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin,
+                                int is_x0_m_basis, int is_wc_m_basis, void* wc, int ldw);
+
+/* ---- Sigma_c ------------------------------------------------------------
+ * Imaginary-axis weights (S/m_sxcf_gemm.f90:432-458), FP64:
+ *   wgtim: (niw+1) x (ns2-ns1+1) x ntq, times wkkr.
+ * omega(ntq) end-state and ekc(ns1:ns2) middle-state energies in Ry; esmr in
+ * Ry; freqx/wwx: Gauss-Legendre points/weights on (0,1); expa = exp(-ua^2 w^2). */
+ecalj_status ecalj_sigc_weights(ecalj_handle h, int niw, int ns1, int ns2, int ntq, int nctot,
+                                const double* omega, const double* ekc, double esmr, double ua,
+                                double wkkr, const double* freqx, const double* wwx,
+                                const double* expa, double* wgtim);
+ecalj_status ecalj_sigc_weights_dev(ecalj_handle h, int nbatch, int niw, int ns1, int ns2,
+                                    int ntq, int nctot, const double* omega, const double* ekc,
+                                    double esmr, double ua, const double* wkkr,
+                                    const double* freqx, const double* wwx, const double* expa,
+                                    void* wgtim);
+/* Imaginary-axis contraction and final product (S/m_sxcf_gemm.f90:463-496,
+ * 599-608):
+ *   czmelwc = sum_iw wc(iw)^H (wgtim(iw,:,:) * zmel), wc(iw) unpacked from its
+ *             upper triangle as the keepwv path does (:337-344, 468-475);
+ *   zsec(1:ntq,1:ntq) += czmelwc^H zmel  (K = ngb*nm, ldc = ldz);
+ *   Im zsec(i,i) clipped to <= 0.
+ * zmel: ngb x nm x ntq; wc: ldw x ngb x (niw+1); wgtim (niw+1) x nm x ntq.
+ * nbatch > 1 accumulates nbatch internal k-points (zmel, wc, wgtim advanced
+ * by their sizes) into the same zsec; the clip is applied once at the end. */
+ecalj_status ecalj_sigc(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, int niw,
+                        const double* wc, int ldw, const double* wgtim, double* zsec, int ldz);
+ecalj_status ecalj_sigc_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm,
+                            int ntq, int niw, const void* wc, int ldw, const void* wgtim,
+                            void* zsec, int ldz);
+
+/* ---- QP update (host, FP64) -------------------------------------------
+ * S/main_hqpe.sc.f90:152-161: eqp = elda + (sex + sexcore)*hartree
+ * + Re(sec(i,i))*hartree - vxc, energies in eV, self-energies in Hartree. */
+ecalj_status ecalj_qp_energy(int ntq, const double* elda, const double* sex,
+                             const double* sexcore, const double* sec, int ldsec,
+                             const double* vxc, double* eqp);
+/* S/main_hqpe.sc.f90:175-181: se = sex2 + sexcore2 + (sec+sec^H)/2 - vxcmat. */
+ecalj_status ecalj_sigma_herm(int ntq, const double* sex2, const double* sexcore2,
+                              const double* sec, const double* vxcmat, double* se);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
