@@ -1,0 +1,162 @@
+"""Benchmark: chi0 + W_c + Sigma_c per q-point on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2] [--precision f64]
+                    [--no-cpu-baseline]
+
+A step is one q-point of the path (chi0(q) over all k and band pairs, W_c(q)
+at nf+1 frequencies, Sigma_c(q) over all internal k) on synthetic inputs
+already resident in device memory.  Prints one JSON line.  The kernel roofline
+uses HIP events recorded by the library on its own stream around every launch
+of the dominant kernel (the Sigma_c frequency contraction) during the timed
+steps.  cpu_baseline times the FP64 C oracle on a bounded sample of the same
+workload (one k-point of chi0 and of Sigma_c, one frequency of the W build),
+scaled to a whole q-point.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(qp, gpu):
+    """FP64 oracle on host cores: a bounded sample of the q-point, scaled."""
+    import oracle as orc
+    from ecalj_amd import synth
+    from oracle import pipeline
+
+    orc.build()
+    cfg = qp.cfg
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t = {}
+    # chi0 for one k-point (zmel + x0gemm) -> scaled by nk; Hilbert once
+    t0 = time.perf_counter()
+    chi, _ = pipeline.chi0(qp, k_subset=[0])
+    t["chi0_one_k_plus_hilbert"] = time.perf_counter() - t0
+    # Hilbert alone, to separate it from the per-k part
+    fr = synth.frhis(qp)
+    nwhis = len(fr) - 1
+    rc = np.zeros((cfg.ni, cfg.ni, nwhis + 1), np.complex128, order="F")
+    _, _, _, _, nu = pipeline.frequencies(cfg)
+    im = orc.imatt(nu, fr)
+    t0 = time.perf_counter()
+    orc.dpsion_imag(rc, fr, im)
+    t_hil = time.perf_counter() - t0
+    del rc
+    # W build for one frequency -> scaled by nf+1
+    t0 = time.perf_counter()
+    wc1 = orc.wc_build(chi[:, :, :1], qp.vcousq, qp.zcousq, nspin=cfg.nspin)
+    t["w_one_freq"] = time.perf_counter() - t0
+    # Sigma_c for one internal k -> scaled by nk
+    wck = [np.repeat(wc1, cfg.nf + 1, axis=2)]
+    t0 = time.perf_counter()
+    pipeline.sigc(qp, wck, k_subset=[0])
+    t["sigc_one_k"] = time.perf_counter() - t0
+    per_q = ((t["chi0_one_k_plus_hilbert"] - t_hil) * cfg.nk + t_hil + t["w_one_freq"] * (cfg.nf + 1)
+             + t["sigc_one_k"] * cfg.nk)
+    sample = (f"{cfg.name}: chi0 on 1 of {cfg.nk} k-points + full Hilbert transform, W build at 1 of "
+              f"{cfg.nf + 1} frequencies, Sigma_c on 1 of {cfg.nk} internal k-points "
+              f"({sum(t.values()):.1f} s measured); scaled to one q-point")
+    return {"value": per_q, "unit": "s/q-point", "cores": threads, "kind": "port", "sample": sample}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--precision", default="f64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one q-point per GPU; "
+              "see DESIGN.md for how the path shards). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from ecalj_amd import synth
+    from ecalj_amd.qsgw import QPointGPU, flops_per_q
+
+    cfg = synth.CONFIGS[args.config]
+    log(f"bench: generating {cfg.name} inputs")
+    qp = synth.make_qpoint(cfg)
+    g = QPointGPU(qp, args.precision)
+    log("bench: setup run (W_c at internal k from the first W_c(q))")
+    g.step()
+    g.make_wk()
+    g.h.sync()
+    for i in range(args.warmup):
+        g.step()
+        g.h.sync()
+        log(f"bench: warmup {i + 1}/{args.warmup} done")
+    g.h.kernel_timing(True)
+    g.h.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g.step()
+    g.h.sync()
+    dt = time.perf_counter() - t0
+    ktimes = {k: g.h.kernel_time(k) for k in ("sigc", "chi0", "zmel", "wbuild", "hilbert")}
+    g.h.kernel_timing(False)
+    s_per_q = dt / args.steps
+    log(f"bench: {args.steps} steps in {dt:.3f} s -> {s_per_q * 1e3:.1f} ms per q-point")
+
+    npb = int(sum(g.ix.iwend - g.ix.iwini + 1))  # (pair, bin) entries actually accumulated
+    fl = flops_per_q(cfg, nwhis_pairs_bins=npb)
+    tflops = sum(fl.values()) / s_per_q / 1e12
+    # dominant kernel: Sigma_c frequency contraction
+    ms, nl = ktimes["sigc"]
+    launches_per_step = max(1, nl // args.steps)
+    flop_launch = fl["sigc"] / launches_per_step
+    avg_s = ms / 1e3 / max(nl, 1)
+    achieved = flop_launch / avg_s / 1e12
+    roof = {"kernel": "sig_freq_gemm_kernel (Sigma_c W(iv)^H x weighted zmel, all internal k)",
+            "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK[args.precision], "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK[args.precision], 4), "traffic": None,
+            "flop_per_launch": flop_launch, "avg_launch_ms": round(avg_s * 1e3, 4)}
+    out = {
+        "metric": "chi0+Sigma_c s/q-point & TFLOP/s, mixed vs FP64 GPU form at 1 meV QP error",
+        "value": round(s_per_q, 6),
+        "unit": "s/q-point",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_per_q * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "complex128 (FP64 MFMA)", "f32": "complex64 (FP32 MFMA)",
+                  "bf16x3": "bf16x3 split (FP32 accumulate)"}[args.precision],
+        "data": "synthetic (seeded; see ecalj_amd/synth.py)",
+        "config": {"workload": f"{cfg.name} q-point: nk={cfg.nk}, nb={cfg.nb}, occ={cfg.no}, unocc={cfg.nu}, "
+                               f"nI={cfg.ni}, nf={cfg.nf}(+nu=0), ntq={cfg.nq}, nwhis={g.nwhis}",
+                   "form": args.precision},
+        "tflops": round(tflops, 3),
+        "flops_per_q": sum(fl.values()),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in ktimes.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU oracle on a bounded sample")
+        try:
+            out["cpu_baseline"] = cpu_baseline(qp, g)
+        except Exception as e:  # report, never hide
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
