@@ -221,6 +221,7 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
   if (h->d_ppb2) (void)hipFree(h->d_ppb2);
   if (h->d_vcousq) (void)hipFree(h->d_vcousq);
   if (h->d_zcousq) (void)hipFree(h->d_zcousq);
+  if (h->d_zcousq32) (void)hipFree(h->d_zcousq32);
   if (h->d_flag) (void)hipFree(h->d_flag);
   for (auto& pool : h->ev_pool)
     for (auto& p : pool) {
@@ -369,36 +370,94 @@ ecalj_status ecalj_set_coulomb(ecalj_handle h, int ngb, const double* vcoud, con
   ECALJ_HIP(hipStreamSynchronize(h->stream));
   if (h->d_vcousq) (void)hipFree(h->d_vcousq);
   if (h->d_zcousq) (void)hipFree(h->d_zcousq);
+  if (h->d_zcousq32) (void)hipFree(h->d_zcousq32);
   h->d_vcousq = nullptr;
   h->d_zcousq = nullptr;
+  h->d_zcousq32 = nullptr;
   if (hipMalloc(&h->d_vcousq, sizeof(double) * ngb) != hipSuccess) return ECALJ_ERR_NOMEM;
   if (hipMalloc(&h->d_zcousq, sizeof(cd) * (size_t)ngb * ngb) != hipSuccess) return ECALJ_ERR_NOMEM;
+  if (hipMalloc(&h->d_zcousq32, sizeof(ecalj::cf) * (size_t)ngb * ngb) != hipSuccess) return ECALJ_ERR_NOMEM;
   ECALJ_HIP(hipMemcpyAsync(h->d_vcousq, vs.data(), sizeof(double) * ngb, hipMemcpyHostToDevice, h->stream));
   ECALJ_HIP(hipMemcpyAsync(h->d_zcousq, zcousq, sizeof(cd) * (size_t)ngb * ngb, hipMemcpyHostToDevice, h->stream));
+  ECALJ_TRY(convert(h, h->d_zcousq, true, h->d_zcousq32, false, (int64_t)ngb * ngb));
   ECALJ_HIP(hipStreamSynchronize(h->stream));
   h->ngb = ngb;
   return ECALJ_OK;
 }
+
+// ------------------------------------------------------------------ helpers
+static inline bool wide(const ecalj_ctx* h) { return h->precision == ECALJ_PREC_F64; }
+static inline size_t celt(const ecalj_ctx* h) { return wide(h) ? sizeof(cd) : sizeof(ecalj::cf); }
+
+// Host staging for the host-pointer entry points: a complex128 host array
+// lives on the device in the handle's storage type.
+struct DevC {
+  DevBuf buf;
+  ecalj_status put(ecalj_ctx* h, const void* host, int64_t n) {
+    ECALJ_TRY(buf.alloc(celt(h) * std::max<int64_t>(n, 1)));
+    if (!host || n == 0) return ECALJ_OK;
+    if (wide(h)) {
+      ECALJ_HIP(hipMemcpyAsync(buf.p, host, sizeof(cd) * n, hipMemcpyHostToDevice, h->stream));
+      return ECALJ_OK;
+    }
+    DevBuf t;
+    ECALJ_TRY(t.put(h, host, sizeof(cd) * n));
+    ECALJ_TRY(convert(h, t.p, true, buf.p, false, n));
+    ECALJ_HIP(hipStreamSynchronize(h->stream));
+    return ECALJ_OK;
+  }
+  ecalj_status get(ecalj_ctx* h, void* host, int64_t n) {
+    if (n == 0) return ECALJ_OK;
+    if (wide(h)) {
+      ECALJ_HIP(hipMemcpyAsync(host, buf.p, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
+    } else {
+      DevBuf t;
+      ECALJ_TRY(t.alloc(sizeof(cd) * n));
+      ECALJ_TRY(convert(h, buf.p, false, t.p, true, n));
+      ECALJ_HIP(hipMemcpyAsync(host, t.p, sizeof(cd) * n, hipMemcpyDeviceToHost, h->stream));
+      ECALJ_HIP(hipStreamSynchronize(h->stream));
+    }
+    ECALJ_HIP(hipStreamSynchronize(h->stream));
+    return ECALJ_OK;
+  }
+};
 
 // ------------------------------------------------------------------ zmel
 // get_zmel_init_gemm (S/m_zmel.f90:146-597), MT valence part.  For each atom
 // block the smaller side is contracted with ppb first:
 //   m-first: T(b,I,n) = sum_a ppb(b,a,I) conj(cphim(a,n));  zmel(I,n,m) = phase sum_b T(b,I,n) cphiq(b,m)
 //   q-first: T(a,I,m) = sum_b ppb(b,a,I) cphiq(b,m);        zmel(I,n,m) = phase sum_a conj(cphim(a,n)) T(a,I,m)
-static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int nm, const cd* cphim,
-                              int64_t sm, int nq, const cd* cphiq, int64_t sq, int is_m_basis, int conjg,
-                              int nbatch, cd* zmel) {
+// cphim / cphiq are complex128; zmel is in the handle's storage type.
+static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int nm, const cd* cphim64,
+                              int64_t sm, int nq, const cd* cphiq64, int64_t sq, int is_m_basis, int conjg,
+                              int nbatch, void* zmel) {
   if (!h->d_ppb) return ECALJ_ERR_STATE;
   if (nm < 0 || nq < 0 || nbatch < 1 || ndima < h->ndima_used) return ECALJ_ERR_DIM;
   const int ngb = h->ngb_ppb;
   if (!is_m_basis && (!h->d_zcousq || h->ngb != ngb)) return ECALJ_ERR_STATE;
   if (nm == 0 || nq == 0) return ECALJ_OK;
-  const int64_t zsz = (int64_t)ngb * nm * nq;  // per batch
-  cd* zM = zmel;
-  if (!is_m_basis) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP2, sizeof(cd) * zsz * nbatch, (void**)&zM));
-  const bool conj_now = conjg && is_m_basis;
+  const int mode = h->precision;
+  const size_t es = celt(h);
   const int bm = (sm == 0) ? 1 : nbatch;
   const int bq = (sq == 0) ? 1 : nbatch;
+  // eigenvectors in the storage type
+  const void* cphim = cphim64;
+  const void* cphiq = cphiq64;
+  if (!wide(h)) {
+    void *m32, *q32;
+    const int64_t nme = (bm == 1) ? (int64_t)ndima * nm : sm * (nbatch - 1) + (int64_t)ndima * nm;
+    const int64_t nqe = (bq == 1) ? (int64_t)ndima * nq : sq * (nbatch - 1) + (int64_t)ndima * nq;
+    ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CVT1, sizeof(ecalj::cf) * nme, &m32));
+    ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CVT2, sizeof(ecalj::cf) * nqe, &q32));
+    ECALJ_TRY(convert(h, cphim64, true, m32, false, nme));
+    ECALJ_TRY(convert(h, cphiq64, true, q32, false, nqe));
+    cphim = m32;
+    cphiq = q32;
+  }
+  const int64_t zsz = (int64_t)ngb * nm * nq;  // per batch
+  char* zM = (char*)zmel;
+  if (!is_m_basis) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP2, es * zsz * nbatch, (void**)&zM));
+  const bool conj_now = conjg && is_m_basis;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 2, &tev));
   int64_t ias = 0, ims = 0;
@@ -406,14 +465,15 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
     const int nv = h->nv[a], md = h->mdim[a];
     const cd ph{phase ? phase[2 * a] : 1.0, phase ? phase[2 * a + 1] : 0.0};
     const int64_t nvmd = (int64_t)nv * md;
-    cd* T;
+    char* T;
     if ((int64_t)nm * bm <= (int64_t)nq * bq) {
-      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, sizeof(cd) * nvmd * nm * bm, (void**)&T));
+      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, es * nvmd * nm * bm, (void**)&T));
       GemmCall g;  // T[(b+nv*I), (n,k)] = P2 * conj(cphim)
+      g.mode = mode;
       g.areal = true; g.cb = true;
       g.M = (int)nvmd; g.K = nv;
       g.A = h->d_ppb2 + h->ppb_off[a]; g.lda = nvmd;
-      g.B = cphim + ias; g.ldb = ndima;
+      g.B = (const char*)cphim + es * ias; g.ldb = ndima;
       g.C = T; g.ldc = nvmd;
       if (bm == 1 || sm == (int64_t)ndima * nm) {
         g.N = nm * bm;
@@ -422,23 +482,25 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
         g.bB = {sm, 0}; g.bC = {nvmd * nm, 0};
         g.batch = bm; g.div = bm;
       }
-      ECALJ_TRY(gemm_f64(h, g));
+      ECALJ_TRY(gemm(h, g));
       GemmCall z;  // per (n,k): zmel(ims+I, n, m) = phase * sum_b T[b + nv*I + nvmd*(n + nm*k)] cphiq_k(ias+b, m)
+      z.mode = mode;
       z.ta = true;
       z.M = md; z.N = nq; z.K = nv;
       z.A = T; z.lda = nv; z.bA = {nvmd, (bm == 1) ? 0 : nvmd * nm};
-      z.B = cphiq + ias; z.ldb = ndima; z.bB = {0, sq};
-      z.C = zM + ims; z.ldc = (int64_t)ngb * nm; z.bC = {ngb, zsz};
+      z.B = (const char*)cphiq + es * ias; z.ldb = ndima; z.bB = {0, sq};
+      z.C = zM + es * ims; z.ldc = (int64_t)ngb * nm; z.bC = {ngb, zsz};
       z.batch = nm * nbatch; z.div = nm;
       z.alpha = ph; z.conjC = conj_now;
-      ECALJ_TRY(gemm_f64(h, z));
+      ECALJ_TRY(gemm(h, z));
     } else {
-      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, sizeof(cd) * nvmd * nq * bq, (void**)&T));
+      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, es * nvmd * nq * bq, (void**)&T));
       GemmCall g;  // T[(a+nv*I), (m,k)] = ppb^T * cphiq, A(r,b) = ppb[b + nv*r]
+      g.mode = mode;
       g.areal = true; g.ta = true;
       g.M = (int)nvmd; g.K = nv;
       g.A = h->d_ppb + h->ppb_off[a]; g.lda = nv;
-      g.B = cphiq + ias; g.ldb = ndima;
+      g.B = (const char*)cphiq + es * ias; g.ldb = ndima;
       g.C = T; g.ldc = nvmd;
       if (bq == 1 || sq == (int64_t)ndima * nq) {
         g.N = nq * bq;
@@ -447,29 +509,31 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
         g.bB = {sq, 0}; g.bC = {nvmd * nq, 0};
         g.batch = bq; g.div = bq;
       }
-      ECALJ_TRY(gemm_f64(h, g));
+      ECALJ_TRY(gemm(h, g));
       GemmCall z;  // per (m,k): zmel(ims+I, n, m) = phase * sum_a T[a + nv*I + nvmd*(m + nq*k)] conj(cphim_k(ias+a, n))
+      z.mode = mode;
       z.ta = true; z.cb = true;
       z.M = md; z.N = nm; z.K = nv;
       z.A = T; z.lda = nv; z.bA = {nvmd, (bq == 1) ? 0 : nvmd * nq};
-      z.B = cphim + ias; z.ldb = ndima; z.bB = {0, sm};
-      z.C = zM + ims; z.ldc = ngb; z.bC = {(int64_t)ngb * nm, zsz};
+      z.B = (const char*)cphim + es * ias; z.ldb = ndima; z.bB = {0, sm};
+      z.C = zM + es * ims; z.ldc = ngb; z.bC = {(int64_t)ngb * nm, zsz};
       z.batch = nq * nbatch; z.div = nq;
       z.alpha = ph; z.conjC = conj_now;
-      ECALJ_TRY(gemm_f64(h, z));
+      ECALJ_TRY(gemm(h, z));
     }
     ias += nv;
     ims += md;
   }
   if (!is_m_basis) {  // zmel = zcousq^H zmel_M  (S/m_zmel.f90:559-560), then conj
     GemmCall e;
+    e.mode = mode;
     e.ta = true; e.ca = true;
     e.M = ngb; e.N = (int)((int64_t)nm * nq * nbatch); e.K = ngb;
-    e.A = h->d_zcousq; e.lda = ngb;
+    e.A = wide(h) ? (const void*)h->d_zcousq : (const void*)h->d_zcousq32; e.lda = ngb;
     e.B = zM; e.ldb = ngb;
     e.C = zmel; e.ldc = ngb;
     e.conjC = conjg;
-    ECALJ_TRY(gemm_f64(h, e));
+    ECALJ_TRY(gemm(h, e));
   }
   ECALJ_TRY(timing_end(h, 2, tev));
   return ECALJ_OK;
@@ -478,40 +542,38 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
 ecalj_status ecalj_zmel_dev(ecalj_handle h, const double* phase, int ndima, int nm, const void* cphim, int nq,
                             const void* cphiq, int is_m_basis, int zmelconjg, int nbatch, void* zmel) {
   if (!h || !cphim || !cphiq || !zmel) return ECALJ_ERR_ARG;
-  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
   return zmel_impl(h, phase, ndima, nm, (const cd*)cphim, (int64_t)ndima * nm, nq, (const cd*)cphiq,
-                   (int64_t)ndima * nq, is_m_basis, zmelconjg, nbatch, (cd*)zmel);
+                   (int64_t)ndima * nq, is_m_basis, zmelconjg, nbatch, zmel);
 }
 
 ecalj_status ecalj_zmel_strided_dev(ecalj_handle h, const double* phase, int ndima, int nm, const void* cphim,
                                     int64_t cphim_stride, int nq, const void* cphiq, int64_t cphiq_stride,
                                     int is_m_basis, int zmelconjg, int nbatch, void* zmel) {
   if (!h || !cphim || !cphiq || !zmel) return ECALJ_ERR_ARG;
-  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
+  if (cphim_stride < 0 || cphiq_stride < 0) return ECALJ_ERR_DIM;
   return zmel_impl(h, phase, ndima, nm, (const cd*)cphim, cphim_stride, nq, (const cd*)cphiq, cphiq_stride,
-                   is_m_basis, zmelconjg, nbatch, (cd*)zmel);
+                   is_m_basis, zmelconjg, nbatch, zmel);
 }
 
 ecalj_status ecalj_zmel(ecalj_handle h, const double* phase, int ndima, int nm, const double* cphim, int nq,
                         const double* cphiq, int is_m_basis, int zmelconjg, int nbatch, double* zmel) {
   if (!h || !cphim || !cphiq || !zmel || nbatch < 1) return ECALJ_ERR_ARG;
   const size_t bm = sizeof(cd) * (size_t)ndima * nm * nbatch, bq = sizeof(cd) * (size_t)ndima * nq * nbatch;
-  const size_t bz = sizeof(cd) * (size_t)h->ngb_ppb * nm * nq * nbatch;
-  DevBuf dm, dq, dz;
+  const int64_t nz = (int64_t)h->ngb_ppb * nm * nq * nbatch;
+  DevBuf dm, dq;
+  DevC dz;
   ECALJ_TRY(dm.put(h, cphim, bm));
   ECALJ_TRY(dq.put(h, cphiq, bq));
-  ECALJ_TRY(dz.alloc(bz));
-  ECALJ_TRY(ecalj_zmel_dev(h, phase, ndima, nm, dm.p, nq, dq.p, is_m_basis, zmelconjg, nbatch, dz.p));
-  ECALJ_HIP(hipMemcpyAsync(zmel, dz.p, bz, hipMemcpyDeviceToHost, h->stream));
-  ECALJ_HIP(hipStreamSynchronize(h->stream));
-  return ECALJ_OK;
+  ECALJ_TRY(dz.put(h, nullptr, nz));
+  ECALJ_TRY(ecalj_zmel_dev(h, phase, ndima, nm, dm.p, nq, dq.p, is_m_basis, zmelconjg, nbatch, dz.buf.p));
+  return dz.get(h, zmel, nz);
 }
 
 // ------------------------------------------------------------------ x0gemm
 // Bin lists of x0gemm (S/x0gemm.f90:36-64), built over all k of the batch,
 // then the per-bin gather-Herk.
-static ecalj_status x0_impl(ecalj_ctx* h, cd* rcxq, int npr, int ipr_col, int npr_col, int nwhis, int npm, int nk,
-                            const int* ns1, const int* ns2, const cd* zmel, int64_t zstride, int nbb, int nqtot,
+static ecalj_status x0_impl(ecalj_ctx* h, void* rcxq, int npr, int ipr_col, int npr_col, int nwhis, int npm, int nk,
+                            const int* ns1, const int* ns2, const void* zmel, int64_t zstride, int nbb, int nqtot,
                             const int* kmin, const int* kmax, int ncoun, const int* iwini, const int* iwend,
                             const int* itc, const int* itpc, const int* jpmc, const int* icouini, int ncount,
                             const double* whwc) {
@@ -590,8 +652,8 @@ static ecalj_status x0_impl(ecalj_ctx* h, cd* rcxq, int npr, int ipr_col, int np
   const void* d_items = (const char*)d_w + sizeof(double) * wt.size();
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 1, &tev));
-  ECALJ_TRY(x0_herk(h, zmel, d_col, d_w, d_b, d_items, (int)sorted.size(), rcxq, (int64_t)npr * npr_col, npr,
-                    npr_col, ipr_col, mirror));
+  ECALJ_TRY(x0_herk(h, h->precision, zmel, d_col, d_w, d_b, d_items, (int)sorted.size(), rcxq,
+                    (int64_t)npr * npr_col, npr, npr_col, ipr_col, mirror));
   ECALJ_TRY(timing_end(h, 1, tev));
   return ECALJ_OK;
 }
@@ -605,9 +667,8 @@ ecalj_status ecalj_x0gemm_batched_dev(ecalj_handle h, void* rcxq, int npr, int i
   if (!h || !rcxq || !zmel || !ns1 || !ns2 || !icounkmin || !icounkmax || !iwini || !iwend || !itc || !itpc ||
       !jpmc || !icouini || !whwc)
     return ECALJ_ERR_ARG;
-  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
-  return x0_impl(h, (cd*)rcxq, npr, ipr_col, npr_col, nwhis, npm, nk, ns1, ns2, (const cd*)zmel, zmel_stride, nbb,
-                 nqtot, icounkmin, icounkmax, ncoun, iwini, iwend, itc, itpc, jpmc, icouini, ncount, whwc);
+  return x0_impl(h, rcxq, npr, ipr_col, npr_col, nwhis, npm, nk, ns1, ns2, zmel, zmel_stride, nbb, nqtot,
+                 icounkmin, icounkmax, ncoun, iwini, iwend, itc, itpc, jpmc, icouini, ncount, whwc);
 }
 
 ecalj_status ecalj_x0gemm(ecalj_handle h, double* rcxq, int npr, int ipr_col, int npr_col, int nwhis, int npm,
@@ -618,22 +679,19 @@ ecalj_status ecalj_x0gemm(ecalj_handle h, double* rcxq, int npr, int ipr_col, in
   if (icoun_hi < icoun_lo) return ECALJ_OK;
   const int nm = ns2 - ns1 + 1;
   if (nm < 1) return ECALJ_ERR_DIM;
-  // ncoun / ncount bounds: the caller's arrays cover indices up to icoun_hi
-  int ncount = 0;
+  int ncount = 0;  // the caller's whwc covers the indices its lists name
   for (int ic = icoun_lo; ic <= icoun_hi; ++ic)
     ncount = std::max(ncount, icouini[ic - 1] + iwend[ic - 1] - iwini[ic - 1]);
-  const size_t nslice = (npm == 1) ? (size_t)nwhis + 1 : 2 * (size_t)nwhis + 1;
-  const size_t br = sizeof(cd) * (size_t)npr * npr_col * nslice;
-  const size_t bz = sizeof(cd) * (size_t)nbb * nm * nqtot;
-  DevBuf dr, dz;
-  ECALJ_TRY(dr.put(h, rcxq, br));
-  ECALJ_TRY(dz.put(h, zmel, bz));
-  ECALJ_TRY(ecalj_x0gemm_batched_dev(h, dr.p, npr, ipr_col, npr_col, nwhis, npm, 1, &ns1, &ns2, dz.p, 0, nbb, nqtot,
-                                     &icoun_lo, &icoun_hi, icoun_hi, iwini, iwend, itc, itpc, jpmc, icouini, ncount,
-                                     whwc));
-  ECALJ_HIP(hipMemcpyAsync(rcxq, dr.p, br, hipMemcpyDeviceToHost, h->stream));
-  ECALJ_HIP(hipStreamSynchronize(h->stream));
-  return ECALJ_OK;
+  const int64_t nslice = (npm == 1) ? (int64_t)nwhis + 1 : 2 * (int64_t)nwhis + 1;
+  const int64_t nr = (int64_t)npr * npr_col * nslice;
+  const int64_t nz = (int64_t)nbb * nm * nqtot;
+  DevC dr, dz;
+  ECALJ_TRY(dr.put(h, rcxq, nr));
+  ECALJ_TRY(dz.put(h, zmel, nz));
+  ECALJ_TRY(ecalj_x0gemm_batched_dev(h, dr.buf.p, npr, ipr_col, npr_col, nwhis, npm, 1, &ns1, &ns2, dz.buf.p, 0, nbb,
+                                     nqtot, &icoun_lo, &icoun_hi, icoun_hi, iwini, iwend, itc, itpc, jpmc, icouini,
+                                     ncount, whwc));
+  return dr.get(h, rcxq, nr);
 }
 
 // ------------------------------------------------------------------ dpsion
@@ -642,7 +700,6 @@ ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_
                                    void* zxqi) {
   if (!h || !rcxq || !frhis || !freqi || !zxqi) return ECALJ_ERR_ARG;
   if (npr < 0 || npr_col < 0 || nwhis < 1 || niw < 1) return ECALJ_ERR_DIM;
-  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
   std::vector<double> coef, scale;
   hilbert_coef(nwhis, niw, frhis, freqi, ecut, coef, scale);
   std::vector<double> blob(coef);
@@ -652,7 +709,7 @@ ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_
   const int64_t n = (int64_t)npr * npr_col;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 4, &tev));
-  ECALJ_TRY(hilbert(h, (cd*)rcxq, n, nwhis, niw, d, scale_rcxq ? d + coef.size() : nullptr, (cd*)zxqi, n));
+  ECALJ_TRY(hilbert(h, wide(h), rcxq, n, nwhis, niw, d, scale_rcxq ? d + coef.size() : nullptr, (cd*)zxqi, n));
   ECALJ_TRY(timing_end(h, 4, tev));
   return ECALJ_OK;
 }
@@ -660,13 +717,14 @@ ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_
 ecalj_status ecalj_dpsion_imag(ecalj_handle h, double* rcxq, int npr, int npr_col, int nwhis, int niw,
                                const double* frhis, const double* freqi, double ecut, int scale_rcxq, double* zxqi) {
   if (!h || !rcxq || !zxqi) return ECALJ_ERR_ARG;
-  const size_t br = sizeof(cd) * (size_t)npr * npr_col * (nwhis + 1);
+  const int64_t nr = (int64_t)npr * npr_col * (nwhis + 1);
   const size_t bx = sizeof(cd) * (size_t)npr * npr_col * niw;
-  DevBuf dr, dx;
-  ECALJ_TRY(dr.put(h, rcxq, br));
+  DevC dr;
+  DevBuf dx;
+  ECALJ_TRY(dr.put(h, rcxq, nr));
   ECALJ_TRY(dx.alloc(bx));
-  ECALJ_TRY(ecalj_dpsion_imag_dev(h, dr.p, npr, npr_col, nwhis, niw, frhis, freqi, ecut, scale_rcxq, dx.p));
-  if (scale_rcxq) ECALJ_HIP(hipMemcpyAsync(rcxq, dr.p, br, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_TRY(ecalj_dpsion_imag_dev(h, dr.buf.p, npr, npr_col, nwhis, niw, frhis, freqi, ecut, scale_rcxq, dx.p));
+  if (scale_rcxq) ECALJ_TRY(dr.get(h, rcxq, nr));
   ECALJ_HIP(hipMemcpyAsync(zxqi, dx.p, bx, hipMemcpyDeviceToHost, h->stream));
   ECALJ_HIP(hipStreamSynchronize(h->stream));
   return ECALJ_OK;
@@ -677,10 +735,9 @@ ecalj_status ecalj_chi0_freq_dev(ecalj_handle h, const void* z, const void* w, i
                                  void* chi) {
   if (!h || !z || !w || !chi) return ECALJ_ERR_ARG;
   if (ni < 0 || npairs < 0 || nf < 0) return ECALJ_ERR_DIM;
-  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 1, &tev));
-  ECALJ_TRY(chi0_freq(h, (const cd*)z, (const double*)w, ni, npairs, nf, (cd*)chi));
+  ECALJ_TRY(chi0_freq(h, h->precision, z, (const double*)w, ni, npairs, nf, (cd*)chi));
   ECALJ_TRY(timing_end(h, 1, tev));
   return ECALJ_OK;
 }
@@ -688,24 +745,24 @@ ecalj_status ecalj_chi0_freq_dev(ecalj_handle h, const void* z, const void* w, i
 ecalj_status ecalj_chi0_freq(ecalj_handle h, const double* z, const double* w, int ni, int npairs, int nf,
                              double* chi) {
   if (!h || !z || !w || !chi) return ECALJ_ERR_ARG;
-  DevBuf dz, dw, dc;
-  ECALJ_TRY(dz.put(h, z, sizeof(cd) * (size_t)ni * npairs));
+  DevC dz;
+  DevBuf dw, dc;
+  ECALJ_TRY(dz.put(h, z, (int64_t)ni * npairs));
   ECALJ_TRY(dw.put(h, w, sizeof(double) * (size_t)nf * npairs));
   const size_t bc = sizeof(cd) * (size_t)ni * ni * nf;
   ECALJ_TRY(dc.alloc(bc));
-  ECALJ_TRY(ecalj_chi0_freq_dev(h, dz.p, dw.p, ni, npairs, nf, dc.p));
+  ECALJ_TRY(ecalj_chi0_freq_dev(h, dz.buf.p, dw.p, ni, npairs, nf, dc.p));
   ECALJ_HIP(hipMemcpyAsync(chi, dc.p, bc, hipMemcpyDeviceToHost, h->stream));
   ECALJ_HIP(hipStreamSynchronize(h->stream));
   return ECALJ_OK;
 }
 
-// ------------------------------------------------------------------ W build
+// ------------------------------------------------------------------ W build (FP64 in every form)
 ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
                                 int is_wc_m_basis, void* wc, int ldw) {
   if (!h || !chi || !wc) return ECALJ_ERR_ARG;
   if (ngb < 1 || nw < 1 || ldw < ngb || (nspin != 1 && nspin != 2)) return ECALJ_ERR_DIM;
   if (!h->d_vcousq || h->ngb != ngb) return ECALJ_ERR_STATE;
-  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
   const int n = ngb;
   const int64_t nn = (int64_t)n * n;
   const double fac = (nspin == 1) ? 2.0 : 1.0;  // S/m_llw.f90:299-303
@@ -721,7 +778,7 @@ ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw
     g1.B = h->d_zcousq; g1.ldb = n;
     g1.C = tmp; g1.ldc = n; g1.bC = {nn, 0};
     g1.batch = nw; g1.div = nw;
-    ECALJ_TRY(gemm_f64(h, g1));
+    ECALJ_TRY(gemm(h, g1));
     GemmCall g2;
     g2.ta = true; g2.ca = true;
     g2.M = n; g2.N = n; g2.K = n;
@@ -731,7 +788,7 @@ ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw
     g2.batch = nw; g2.div = nw;
     g2.alpha = cd{fac, 0.0};
     g2.epi = EPI_EPS; g2.svec = h->d_vcousq;
-    ECALJ_TRY(gemm_f64(h, g2));
+    ECALJ_TRY(gemm(h, g2));
   } else {
     ECALJ_TRY(eps_direct(h, (const cd*)chi, h->d_vcousq, fac, n, nw, eps));
   }
@@ -745,14 +802,14 @@ ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw
     g3.B = h->d_zcousq; g3.ldb = n;
     g3.C = eps; g3.ldc = n; g3.bC = {nn, 0};
     g3.batch = nw; g3.div = nw;
-    ECALJ_TRY(gemm_f64(h, g3));
+    ECALJ_TRY(gemm(h, g3));
     GemmCall g4;  // wc <- zcousq * (W_E zcousq^H)
     g4.M = n; g4.N = n; g4.K = n;
     g4.A = h->d_zcousq; g4.lda = n;
     g4.B = eps; g4.ldb = n; g4.bB = {nn, 0};
     g4.C = (cd*)wc; g4.ldc = ldw; g4.bC = {(int64_t)ldw * n, 0};
     g4.batch = nw; g4.div = nw;
-    ECALJ_TRY(gemm_f64(h, g4));
+    ECALJ_TRY(gemm(h, g4));
   } else {
     ECALJ_TRY(wc_form(h, eps, h->d_vcousq, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
   }
@@ -810,21 +867,21 @@ ecalj_status ecalj_sigc_dev(ecalj_handle h, int nbatch, const void* zmel, int ng
                             const void* wc, int ldw, const void* wgtim, void* zsec, int ldz) {
   if (!h || !zmel || !wc || !wgtim || !zsec) return ECALJ_ERR_ARG;
   if (ngb < 1 || nm < 0 || ntq < 0 || niw < 0 || ldw < ngb || ldz < ntq || nbatch < 1) return ECALJ_ERR_DIM;
-  if (h->precision != ECALJ_PREC_F64) return ECALJ_ERR_STATE;
-  return sig_contract(h, nbatch, (const cd*)zmel, ngb, nm, ntq, niw, (const cd*)wc, ldw, (const double*)wgtim,
+  return sig_contract(h, h->precision, nbatch, zmel, ngb, nm, ntq, niw, (const cd*)wc, ldw, (const double*)wgtim,
                       (cd*)zsec, ldz);
 }
 
 ecalj_status ecalj_sigc(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, int niw, const double* wc,
                         int ldw, const double* wgtim, double* zsec, int ldz) {
   if (!h || !zmel || !wc || !wgtim || !zsec) return ECALJ_ERR_ARG;
-  DevBuf dz, dw, dg, ds;
-  ECALJ_TRY(dz.put(h, zmel, sizeof(cd) * (size_t)ngb * nm * ntq));
+  DevC dz;
+  DevBuf dw, dg, ds;
+  ECALJ_TRY(dz.put(h, zmel, (int64_t)ngb * nm * ntq));
   ECALJ_TRY(dw.put(h, wc, sizeof(cd) * (size_t)ldw * ngb * (niw + 1)));
   ECALJ_TRY(dg.put(h, wgtim, sizeof(double) * (size_t)(niw + 1) * nm * ntq));
   const size_t bs = sizeof(cd) * (size_t)ldz * ntq;
   ECALJ_TRY(ds.put(h, zsec, bs));
-  ECALJ_TRY(ecalj_sigc_dev(h, 1, dz.p, ngb, nm, ntq, niw, dw.p, ldw, dg.p, ds.p, ldz));
+  ECALJ_TRY(ecalj_sigc_dev(h, 1, dz.buf.p, ngb, nm, ntq, niw, dw.p, ldw, dg.p, ds.p, ldz));
   ECALJ_HIP(hipMemcpyAsync(zsec, ds.p, bs, hipMemcpyDeviceToHost, h->stream));
   ECALJ_HIP(hipStreamSynchronize(h->stream));
   return ECALJ_OK;

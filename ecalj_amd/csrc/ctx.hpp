@@ -21,10 +21,11 @@ struct ecalj_ctx {
   int ngb = 0;
   double* d_vcousq = nullptr;     // sqrt(vcoud)
   ecalj::cd* d_zcousq = nullptr;  // ngb x ngb
+  ecalj::cf* d_zcousq32 = nullptr;  // complex64 copy for the mixed-precision M->E transform
 
   // grow-only scratch slots
   enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
-         S_NSLOT };
+         S_CVT1, S_CVT2, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag
@@ -46,12 +47,13 @@ ecalj_status timing_end(ecalj_ctx* h, int kclass, int idx);
 // Generic complex GEMM (FP64 MFMA).  ta/tb: operand stored transposed
 // (op = T); ca/cb: operand conjugated.  op 'C' of BLAS = ta + ca.  A may be real.
 struct GemmCall {
+  int mode = 0;  // M64 (complex128), M32 / MBF3 (complex64)
   bool ta = false, tb = false, ca = false, cb = false;
   bool areal = false;
   int M = 0, N = 0, K = 0;
   const void* A = nullptr; int64_t lda = 0; Batch2 bA{0, 0};
-  const cd* B = nullptr; int64_t ldb = 0; Batch2 bB{0, 0};
-  cd* C = nullptr; int64_t ldc = 0; Batch2 bC{0, 0};
+  const void* B = nullptr; int64_t ldb = 0; Batch2 bB{0, 0};
+  void* C = nullptr; int64_t ldc = 0; Batch2 bC{0, 0};
   int batch = 1, div = 1;
   cd alpha{1.0, 0.0}, beta{0.0, 0.0};
   int conjC = 0;
@@ -59,6 +61,6 @@ struct GemmCall {
   const double* svec = nullptr;
   int gj_k0 = 0, gj_k1 = 0;
 };
-ecalj_status gemm_f64(ecalj_ctx* h, const GemmCall& c);
+ecalj_status gemm(ecalj_ctx* h, const GemmCall& c);
 
 }  // namespace ecalj

@@ -1,11 +1,19 @@
-// gemm.hpp -- complex GEMM engine on CDNA4 MFMA (gfx950), FP64 path.
+// gemm.hpp -- complex GEMM engine on CDNA4 MFMA (gfx950).
 //
 // One workgroup = 256 threads = 4 wavefronts in a 2x2 grid; each wave owns a
 // (BM/2) x (BN/2) block of the output made of 16x16 MFMA tiles.  A and B
-// stages go global -> registers -> LDS (re and im in separate k-major
-// planes), with the next stage's global loads issued before the current
-// stage's MFMAs.  Complex products use four real MFMAs
-// (re += ar*br - ai*bi, im += ar*bi + ai*br); a real A operand uses two.
+// stages go global -> registers -> LDS, with the next stage's global loads
+// issued before the current stage's MFMAs.
+//
+// Three arithmetic modes share the loaders and epilogues:
+//   M64  : complex128 data, FP64 MFMA 16x16x4, four real MFMAs per complex
+//          product (re += ar*br - ai*bi, im += ar*bi + ai*br);
+//   M32  : complex64 data, FP32 MFMA 16x16x4, same scheme, FP32 accumulation;
+//   MBF3 : complex64 data split into bf16 hi + lo parts when written to LDS;
+//          each real product uses hi*hi + hi*lo + lo*hi (the lo*lo term,
+//          2^-16 relative, is dropped) on bf16 MFMA 16x16x32 with FP32
+//          accumulation: 12 MFMAs per complex 16x16x32 block.
+// A real A operand (ppb) skips the imaginary products.
 //
 // Operand loaders and epilogues are small structs so that the gathers and
 // scalings of the ecalj kernels (weighted band-pair columns for chi0,
@@ -19,30 +27,102 @@
 namespace ecalj {
 
 constexpr int kThreads = 256;
+enum Mode { M64 = 0, M32 = 1, MBF3 = 2 };
 
-template <int BM, int BN, int BK>
-struct StageF64 {
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------- elements
+template <class T> struct ElemT;
+template <> struct ElemT<cd> { typedef double V; };
+template <> struct ElemT<cf> { typedef float V; };
+
+template <class V, class T>
+__device__ inline void ld_c(const T* p, int64_t idx, V& re, V& im) {
+  const T v = p[idx];
+  re = (V)v.re;
+  im = (V)v.im;
+}
+
+// round-to-nearest-even float -> bf16 bits (finite inputs)
+__device__ inline unsigned short bf16_bits(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_float(unsigned short b) { return __uint_as_float(((unsigned int)b) << 16); }
+__device__ inline void split_bf16(float x, unsigned short& hi, unsigned short& lo) {
+  hi = bf16_bits(x);
+  lo = bf16_bits(x - bf16_float(hi));
+}
+
+// ---------------------------------------------------------------- LDS stages
+// Planar (M64, M32): separate re/im planes, k-major: plane[k*LD + m].
+template <int BM, int BN, int BK, class V>
+struct StagePlanar {
   static constexpr int LDA = BM + 4;
   static constexpr int LDB = BN + 4;
-  double ar[BK * LDA];
-  double ai[BK * LDA];
-  double br[BK * LDB];
-  double bi[BK * LDB];
+  V ar[BK * LDA];
+  V ai[BK * LDA];
+  V br[BK * LDB];
+  V bi[BK * LDB];
+  __device__ void putA(int i, int k, V re, V im) {
+    ar[k * LDA + i] = re;
+    ai[k * LDA + i] = im;
+  }
+  __device__ void putB(int k, int j, V re, V im) {
+    br[k * LDB + j] = re;
+    bi[k * LDB + j] = im;
+  }
 };
+
+// bf16 hi/lo planes, row-major with k contiguous: plane[m*LDK + k].
+template <int BM, int BN, int BK>
+struct StageBF3 {
+  static constexpr int LDK = BK + 8;
+  unsigned short arh[BM * LDK], arl[BM * LDK], aih[BM * LDK], ail[BM * LDK];
+  unsigned short brh[BN * LDK], brl[BN * LDK], bih[BN * LDK], bil[BN * LDK];
+  __device__ void putA(int i, int k, float re, float im) {
+    unsigned short h, l;
+    split_bf16(re, h, l);
+    arh[i * LDK + k] = h;
+    arl[i * LDK + k] = l;
+    split_bf16(im, h, l);
+    aih[i * LDK + k] = h;
+    ail[i * LDK + k] = l;
+  }
+  __device__ void putB(int k, int j, float re, float im) {
+    unsigned short h, l;
+    split_bf16(re, h, l);
+    brh[j * LDK + k] = h;
+    brl[j * LDK + k] = l;
+    split_bf16(im, h, l);
+    bih[j * LDK + k] = h;
+    bil[j * LDK + k] = l;
+  }
+};
+
+template <int MODE, int BM, int BN, int BK> struct StageOf;
+template <int BM, int BN, int BK> struct StageOf<M64, BM, BN, BK> { typedef StagePlanar<BM, BN, BK, double> type; };
+template <int BM, int BN, int BK> struct StageOf<M32, BM, BN, BK> { typedef StagePlanar<BM, BN, BK, float> type; };
+template <int BM, int BN, int BK> struct StageOf<MBF3, BM, BN, BK> { typedef StageBF3<BM, BN, BK> type; };
+
+template <int MODE> struct ModeBK { static constexpr int value = (MODE == MBF3) ? 32 : 16; };
 
 // --------------------------------------------------------------------------
 // Operand loaders.  Each owns a fixed thread -> element mapping chosen so
 // that consecutive lanes touch consecutive addresses of the contiguous
 // dimension.  fetch() reads one BK-deep stage into registers (zero outside
-// the matrix); store() writes them to the LDS planes.
+// the matrix); store() hands the values to the stage (putA / putB).
 
 // A(i,k) = a[i + lda*k]  (column-major, rows contiguous)
-template <int BM, int BK, bool REAL>
+template <int BM, int BK, class T, bool REAL>
 struct LdA_N {
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BM * BK / kThreads;
   static constexpr int KSTEP = kThreads / BM;
-  const void* a; int64_t lda; int M, K; double cs;
-  struct Regs { double re[R]; double im[R]; };
+  const void* a; int64_t lda; int M, K; V cs;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int m0, int k0, Regs& r) const {
     const int i = m0 + (threadIdx.x % BM);
     const int kb = k0 + threadIdx.x / BM;
@@ -51,12 +131,13 @@ struct LdA_N {
       const int k = kb + q * KSTEP;
       const bool ok = (i < M) && (k < K);
       if (REAL) {
-        r.re[q] = ok ? static_cast<const double*>(a)[i + lda * k] : 0.0;
-        r.im[q] = 0.0;
+        r.re[q] = ok ? (V) static_cast<const double*>(a)[i + lda * k] : (V)0;
+        r.im[q] = 0;
       } else {
-        cd v = ok ? static_cast<const cd*>(a)[i + lda * k] : cd{0.0, 0.0};
-        r.re[q] = v.re;
-        r.im[q] = cs * v.im;
+        V re = 0, im = 0;
+        if (ok) ld_c<V>(static_cast<const T*>(a), i + lda * k, re, im);
+        r.re[q] = re;
+        r.im[q] = cs * im;
       }
     }
   }
@@ -65,20 +146,18 @@ struct LdA_N {
     const int i = threadIdx.x % BM;
     const int kb = threadIdx.x / BM;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.ar[(kb + q * KSTEP) * S::LDA + i] = r.re[q];
-      s.ai[(kb + q * KSTEP) * S::LDA + i] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putA(i, kb + q * KSTEP, r.re[q], r.im[q]);
   }
 };
 
 // A(i,k) = a[k + lda*i]  (op(A) = A^T or A^H of a column-major matrix)
-template <int BM, int BK, bool REAL>
+template <int BM, int BK, class T, bool REAL>
 struct LdA_T {
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BM * BK / kThreads;
   static constexpr int ISTEP = kThreads / BK;
-  const void* a; int64_t lda; int M, K; double cs;
-  struct Regs { double re[R]; double im[R]; };
+  const void* a; int64_t lda; int M, K; V cs;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int m0, int k0, Regs& r) const {
     const int k = k0 + (threadIdx.x % BK);
     const int ib = m0 + threadIdx.x / BK;
@@ -87,12 +166,13 @@ struct LdA_T {
       const int i = ib + q * ISTEP;
       const bool ok = (i < M) && (k < K);
       if (REAL) {
-        r.re[q] = ok ? static_cast<const double*>(a)[k + lda * i] : 0.0;
-        r.im[q] = 0.0;
+        r.re[q] = ok ? (V) static_cast<const double*>(a)[k + lda * i] : (V)0;
+        r.im[q] = 0;
       } else {
-        cd v = ok ? static_cast<const cd*>(a)[k + lda * i] : cd{0.0, 0.0};
-        r.re[q] = v.re;
-        r.im[q] = cs * v.im;
+        V re = 0, im = 0;
+        if (ok) ld_c<V>(static_cast<const T*>(a), k + lda * i, re, im);
+        r.re[q] = re;
+        r.im[q] = cs * im;
       }
     }
   }
@@ -101,29 +181,28 @@ struct LdA_T {
     const int k = threadIdx.x % BK;
     const int ib = threadIdx.x / BK;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.ar[k * S::LDA + ib + q * ISTEP] = r.re[q];
-      s.ai[k * S::LDA + ib + q * ISTEP] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putA(ib + q * ISTEP, k, r.re[q], r.im[q]);
   }
 };
 
 // B(k,j) = b[k + ldb*j]  (column-major, k contiguous)
-template <int BN, int BK>
+template <int BN, int BK, class T>
 struct LdB_N {
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BN * BK / kThreads;
   static constexpr int JSTEP = kThreads / BK;
-  const cd* b; int64_t ldb; int N, K; double cs;
-  struct Regs { double re[R]; double im[R]; };
+  const T* b; int64_t ldb; int N, K; V cs;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int n0, int k0, Regs& r) const {
     const int k = k0 + (threadIdx.x % BK);
     const int jb = n0 + threadIdx.x / BK;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int j = jb + q * JSTEP;
-      cd v = (j < N && k < K) ? b[k + ldb * j] : cd{0.0, 0.0};
-      r.re[q] = v.re;
-      r.im[q] = cs * v.im;
+      V re = 0, im = 0;
+      if (j < N && k < K) ld_c<V>(b, k + ldb * j, re, im);
+      r.re[q] = re;
+      r.im[q] = cs * im;
     }
   }
   template <class S>
@@ -131,29 +210,28 @@ struct LdB_N {
     const int k = threadIdx.x % BK;
     const int jb = threadIdx.x / BK;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.br[k * S::LDB + jb + q * JSTEP] = r.re[q];
-      s.bi[k * S::LDB + jb + q * JSTEP] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putB(k, jb + q * JSTEP, r.re[q], r.im[q]);
   }
 };
 
 // B(k,j) = b[j + ldb*k]  (op(B) = B^T or B^H, j contiguous)
-template <int BN, int BK>
+template <int BN, int BK, class T>
 struct LdB_T {
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BN * BK / kThreads;
   static constexpr int KSTEP = kThreads / BN;
-  const cd* b; int64_t ldb; int N, K; double cs;
-  struct Regs { double re[R]; double im[R]; };
+  const T* b; int64_t ldb; int N, K; V cs;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int n0, int k0, Regs& r) const {
     const int j = n0 + (threadIdx.x % BN);
     const int kb = k0 + threadIdx.x / BN;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = kb + q * KSTEP;
-      cd v = (j < N && k < K) ? b[j + ldb * k] : cd{0.0, 0.0};
-      r.re[q] = v.re;
-      r.im[q] = cs * v.im;
+      V re = 0, im = 0;
+      if (j < N && k < K) ld_c<V>(b, j + ldb * k, re, im);
+      r.re[q] = re;
+      r.im[q] = cs * im;
     }
   }
   template <class S>
@@ -161,30 +239,37 @@ struct LdB_T {
     const int j = threadIdx.x % BN;
     const int kb = threadIdx.x / BN;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.br[(kb + q * KSTEP) * S::LDB + j] = r.re[q];
-      s.bi[(kb + q * KSTEP) * S::LDB + j] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putB(kb + q * KSTEP, j, r.re[q], r.im[q]);
   }
 };
 
 // --------------------------------------------------------------------------
-// MFMA core: wave (wm, wn) of the 2x2 grid accumulates TM x TN tiles.
-template <int BM, int BN, int BK, bool AREAL>
-struct CoreF64 {
+// MFMA cores: wave (wm, wn) of the 2x2 grid accumulates TM x TN tiles.
+template <int MODE, int BM, int BN, int BK, bool AREAL> struct Core;
+
+// Planar cores (FP64 and FP32 16x16x4)
+template <int BM, int BN, int BK, bool AREAL, class V, class ACC>
+struct CorePlanar {
   static constexpr int TM = BM / 32;
   static constexpr int TN = BN / 32;
-  d4 cr[TM][TN];
-  d4 ci[TM][TN];
+  ACC cr[TM][TN];
+  ACC ci[TM][TN];
 
   __device__ void zero() {
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        cr[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-        ci[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+        cr[i][j] = ACC{0, 0, 0, 0};
+        ci[i][j] = ACC{0, 0, 0, 0};
       }
+  }
+
+  __device__ static ACC mfma(V a, V b, ACC c) {
+    if constexpr (std::is_same<V, double>::value)
+      return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    else
+      return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
 
   template <class S>
@@ -192,7 +277,7 @@ struct CoreF64 {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int k = kk + (lane >> 4);
-      double ar[TM], ai[TM], an[TM], br[TN], bi[TN];
+      V ar[TM], ai[TM], an[TM], br[TN], bi[TN];
 #pragma unroll
       for (int t = 0; t < TM; ++t) {
         const int i = wm * TM * 16 + t * 16 + (lane & 15);
@@ -212,17 +297,16 @@ struct CoreF64 {
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], cr[i][j], 0, 0, 0);
-          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], ci[i][j], 0, 0, 0);
+          cr[i][j] = mfma(ar[i], br[j], cr[i][j]);
+          ci[i][j] = mfma(ar[i], bi[j], ci[i][j]);
           if (!AREAL) {
-            cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[i], bi[j], cr[i][j], 0, 0, 0);
-            ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], ci[i][j], 0, 0, 0);
+            cr[i][j] = mfma(an[i], bi[j], cr[i][j]);
+            ci[i][j] = mfma(ai[i], br[j], ci[i][j]);
           }
         }
     }
   }
 
-  // Visit every accumulator element with its (row, col) inside the block tile.
   template <class F>
   __device__ void for_each(int wm, int wn, int lane, F&& f) const {
 #pragma unroll
@@ -231,17 +315,114 @@ struct CoreF64 {
       for (int j = 0; j < TN; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int r = wm * TM * 16 + i * 16 + acc_row_f64(lane, v);
+          const int rr = std::is_same<V, double>::value ? acc_row_f64(lane, v) : acc_row_f32(lane, v);
+          const int r = wm * TM * 16 + i * 16 + rr;
           const int c = wn * TN * 16 + j * 16 + acc_col(lane, v);
-          f(r, c, cr[i][j][v], ci[i][j][v]);
+          f(r, c, (double)cr[i][j][v], (double)ci[i][j][v]);
+        }
+  }
+};
+
+template <int BM, int BN, int BK, bool AREAL>
+struct Core<M64, BM, BN, BK, AREAL> : CorePlanar<BM, BN, BK, AREAL, double, d4> {};
+template <int BM, int BN, int BK, bool AREAL>
+struct Core<M32, BM, BN, BK, AREAL> : CorePlanar<BM, BN, BK, AREAL, float, f4> {};
+
+// bf16x3 core (bf16 MFMA 16x16x32, FP32 accumulation)
+template <int BM, int BN, int BK, bool AREAL>
+struct Core<MBF3, BM, BN, BK, AREAL> {
+  static constexpr int TM = BM / 32;
+  static constexpr int TN = BN / 32;
+  f4 cr[TM][TN];
+  f4 ci[TM][TN];
+
+  __device__ void zero() {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        cr[i][j] = f4{0, 0, 0, 0};
+        ci[i][j] = f4{0, 0, 0, 0};
+      }
+  }
+
+  __device__ static f4 mfma(const us8& a, const us8& b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0,
+                                                   0);
+  }
+  __device__ static us8 ld8(const unsigned short* p) { return *reinterpret_cast<const us8*>(p); }
+  __device__ static us8 neg8(const us8& x) { return x ^ (unsigned short)0x8000; }
+
+  template <class S>
+  __device__ void compute(const S& s, int wm, int wn, int lane) {
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 32) {
+      const int ko = kk + 8 * (lane >> 4);
+      us8 brh[TN], brl[TN], bih[TN], bil[TN];
+#pragma unroll
+      for (int t = 0; t < TN; ++t) {
+        const int j = wn * TN * 16 + t * 16 + (lane & 15);
+        brh[t] = ld8(s.brh + j * S::LDK + ko);
+        brl[t] = ld8(s.brl + j * S::LDK + ko);
+        bih[t] = ld8(s.bih + j * S::LDK + ko);
+        bil[t] = ld8(s.bil + j * S::LDK + ko);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int ir = wm * TM * 16 + i * 16 + (lane & 15);
+        const us8 arh = ld8(s.arh + ir * S::LDK + ko);
+        const us8 arl = ld8(s.arl + ir * S::LDK + ko);
+        us8 aih, ail, nih, nil;
+        if (!AREAL) {
+          aih = ld8(s.aih + ir * S::LDK + ko);
+          ail = ld8(s.ail + ir * S::LDK + ko);
+          nih = neg8(aih);
+          nil = neg8(ail);
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          f4 r = cr[i][j], m = ci[i][j];
+          r = mfma(arh, brh[j], r);
+          r = mfma(arh, brl[j], r);
+          r = mfma(arl, brh[j], r);
+          m = mfma(arh, bih[j], m);
+          m = mfma(arh, bil[j], m);
+          m = mfma(arl, bih[j], m);
+          if (!AREAL) {
+            r = mfma(nih, bih[j], r);
+            r = mfma(nih, bil[j], r);
+            r = mfma(nil, bih[j], r);
+            m = mfma(aih, brh[j], m);
+            m = mfma(aih, brl[j], m);
+            m = mfma(ail, brh[j], m);
+          }
+          cr[i][j] = r;
+          ci[i][j] = m;
+        }
+      }
+    }
+  }
+
+  template <class F>
+  __device__ void for_each(int wm, int wn, int lane, F&& f) const {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = wm * TM * 16 + i * 16 + acc_row_f32(lane, v);
+          const int c = wn * TN * 16 + j * 16 + acc_col(lane, v);
+          f(r, c, (double)cr[i][j][v], (double)ci[i][j][v]);
         }
   }
 };
 
 // Main loop shared by every GEMM-shaped kernel.
-template <int BM, int BN, int BK, bool AREAL, class LA, class LB>
+template <int MODE, int BM, int BN, int BK, bool AREAL, class LA, class LB>
 __device__ inline void gemm_mainloop(const LA& la, const LB& lb, int m0, int n0, int K,
-                                     StageF64<BM, BN, BK>& s, CoreF64<BM, BN, BK, AREAL>& core) {
+                                     typename StageOf<MODE, BM, BN, BK>::type& s,
+                                     Core<MODE, BM, BN, BK, AREAL>& core) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -264,15 +445,20 @@ __device__ inline void gemm_mainloop(const LA& la, const LB& lb, int m0, int n0,
   }
 }
 
+// Store / load an output element as complex T.
+__device__ inline void st_c(cd* p, double re, double im) { *p = cd{re, im}; }
+__device__ inline void st_c(cf* p, double re, double im) { *p = cf{(float)re, (float)im}; }
+__device__ inline void ld_c2(const cd* p, double& re, double& im) { re = p->re; im = p->im; }
+__device__ inline void ld_c2(const cf* p, double& re, double& im) { re = p->re; im = p->im; }
+
 // --------------------------------------------------------------------------
 // Generic batched GEMM: C = alpha * op(A) op(B) + beta * C, with epilogue
-// variants.  op(A) is N (AT=false) or T/C (AT=true, conj via conjA).
-
+// variants.  op(A) is N (AT=false) or T (AT=true); conjugation via csA/csB.
 struct GemmArgs {
   int M, N, K, batch, div;
   const void* A; int64_t lda; Batch2 bA; double csA;
-  const cd* B; int64_t ldb; Batch2 bB; double csB;
-  cd* C; int64_t ldc; Batch2 bC;
+  const void* B; int64_t ldb; Batch2 bB; double csB;
+  void* C; int64_t ldc; Batch2 bC;
   double alr, ali, ber, bei;
   int conjC;
   int epi;
@@ -280,45 +466,49 @@ struct GemmArgs {
   int gj_k0, gj_k1;    // EPI_GJ: rows [k0,k1) skipped; cols [k0,k1) start from 0
 };
 
-template <int BM, int BN, bool AT, bool AREAL, bool BT>
-__global__ __launch_bounds__(kThreads) void zgemm_f64_kernel(GemmArgs g) {
-  constexpr int BK = 16;
-  __shared__ StageF64<BM, BN, BK> s;
+template <int MODE, class T, int BM, int BN, bool AT, bool AREAL, bool BT>
+__global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs g) {
+  constexpr int BK = ModeBK<MODE>::value;
+  typedef typename StageOf<MODE, BM, BN, BK>::type Stage;
+  typedef typename ElemT<T>::V V;
+  __shared__ Stage s;
   const int b = blockIdx.z;
   const int m0 = blockIdx.x * BM;
   const int n0 = blockIdx.y * BN;
-  const char* Ab = static_cast<const char*>(g.A) + g.bA.off(b, g.div) * (AREAL ? 8 : 16);
-  const cd* Bb = g.B + g.bB.off(b, g.div);
-  cd* Cb = g.C + g.bC.off(b, g.div);
-  CoreF64<BM, BN, BK, AREAL> core;
-  using LA = typename std::conditional<AT, LdA_T<BM, BK, AREAL>, LdA_N<BM, BK, AREAL>>::type;
-  using LB = typename std::conditional<BT, LdB_T<BN, BK>, LdB_N<BN, BK>>::type;
-  LA la{Ab, g.lda, g.M, g.K, g.csA};
-  LB lb{Bb, g.ldb, g.N, g.K, g.csB};
-  gemm_mainloop<BM, BN, BK, AREAL>(la, lb, m0, n0, g.K, s, core);
+  const char* Ab = static_cast<const char*>(g.A) + g.bA.off(b, g.div) * (AREAL ? 8 : (int)sizeof(T));
+  const T* Bb = static_cast<const T*>(g.B) + g.bB.off(b, g.div);
+  T* Cb = static_cast<T*>(g.C) + g.bC.off(b, g.div);
+  Core<MODE, BM, BN, BK, AREAL> core;
+  using LA = typename std::conditional<AT, LdA_T<BM, BK, T, AREAL>, LdA_N<BM, BK, T, AREAL>>::type;
+  using LB = typename std::conditional<BT, LdB_T<BN, BK, T>, LdB_N<BN, BK, T>>::type;
+  LA la{Ab, g.lda, g.M, g.K, (V)g.csA};
+  LB lb{Bb, g.ldb, g.N, g.K, (V)g.csB};
+  gemm_mainloop<MODE, BM, BN, BK, AREAL>(la, lb, m0, n0, g.K, s, core);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   core.for_each(wave >> 1, wave & 1, lane, [&](int r, int c, double xr, double xi) {
     const int i = m0 + r, j = n0 + c;
     if (i >= g.M || j >= g.N) return;
-    cd* p = Cb + i + g.ldc * j;
+    T* p = Cb + i + g.ldc * j;
     double vr = g.alr * xr - g.ali * xi;
     double vi = g.alr * xi + g.ali * xr;
     if (g.epi == EPI_STORE) {
       if (g.ber != 0.0 || g.bei != 0.0) {
-        cd o = *p;
-        vr += g.ber * o.re - g.bei * o.im;
-        vi += g.ber * o.im + g.bei * o.re;
+        double orr, oi;
+        ld_c2(p, orr, oi);
+        vr += g.ber * orr - g.bei * oi;
+        vi += g.ber * oi + g.bei * orr;
       }
       if (g.conjC) vi = -vi;
-      *p = cd{vr, vi};
+      st_c(p, vr, vi);
     } else if (g.epi == EPI_EPS) {
       const double sc = g.svec[i] * g.svec[j];
-      *p = cd{(i == j ? 1.0 : 0.0) - sc * vr, -sc * vi};
+      st_c(p, (i == j ? 1.0 : 0.0) - sc * vr, -sc * vi);
     } else {  // EPI_GJ
       if (i >= g.gj_k0 && i < g.gj_k1) return;
-      cd o = (j >= g.gj_k0 && j < g.gj_k1) ? cd{0.0, 0.0} : *p;
-      *p = cd{o.re - vr, o.im - vi};
+      double orr = 0.0, oi = 0.0;
+      if (!(j >= g.gj_k0 && j < g.gj_k1)) ld_c2(p, orr, oi);
+      st_c(p, orr - vr, oi - vi);
     }
   });
 }

@@ -13,27 +13,28 @@
 namespace ecalj {
 
 // ==========================================================================
-// Generic GEMM dispatcher
-template <int BM, int BN>
+// Generic GEMM dispatcher.  Complex operands are complex128 in mode M64 and
+// complex64 in M32 / MBF3.
+template <int MODE, class T, int BM, int BN>
 static ecalj_status launch_gemm(ecalj_ctx* h, const GemmArgs& g, bool at, bool areal, bool bt) {
   dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, g.batch);
   if (grid.y > 65535 || grid.z > 65535) return ECALJ_ERR_DIM;
   const int key = (at ? 4 : 0) | (areal ? 2 : 0) | (bt ? 1 : 0);
   switch (key) {
-    case 0: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, false, false>), grid, dim3(kThreads), 0, h->stream, g); break;
-    case 1: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, false, true>), grid, dim3(kThreads), 0, h->stream, g); break;
-    case 2: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, true, false>), grid, dim3(kThreads), 0, h->stream, g); break;
-    case 3: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, true, true>), grid, dim3(kThreads), 0, h->stream, g); break;
-    case 4: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, false, false>), grid, dim3(kThreads), 0, h->stream, g); break;
-    case 5: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, false, true>), grid, dim3(kThreads), 0, h->stream, g); break;
-    case 6: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, true, false>), grid, dim3(kThreads), 0, h->stream, g); break;
-    case 7: hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, true, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 0: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, false, false, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 1: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, false, false, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 2: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, false, true, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 3: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, false, true, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 4: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, true, false, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 5: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, true, false, true>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 6: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, true, true, false>), grid, dim3(kThreads), 0, h->stream, g); break;
+    case 7: hipLaunchKernelGGL((zgemm_kernel<MODE, T, BM, BN, true, true, true>), grid, dim3(kThreads), 0, h->stream, g); break;
   }
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
 
-ecalj_status gemm_f64(ecalj_ctx* h, const GemmCall& c) {
+ecalj_status gemm(ecalj_ctx* h, const GemmCall& c) {
   if (c.M < 0 || c.N < 0 || c.K < 0 || c.batch < 1 || c.div < 1) return ECALJ_ERR_DIM;
   if (c.M == 0 || c.N == 0) return ECALJ_OK;
   GemmArgs g;
@@ -44,8 +45,39 @@ ecalj_status gemm_f64(ecalj_ctx* h, const GemmCall& c) {
   g.alr = c.alpha.re; g.ali = c.alpha.im; g.ber = c.beta.re; g.bei = c.beta.im;
   g.conjC = c.conjC; g.epi = c.epi; g.svec = c.svec; g.gj_k0 = c.gj_k0; g.gj_k1 = c.gj_k1;
   const bool at = c.ta, bt = c.tb;
-  if (c.N <= 32) return launch_gemm<64, 32>(h, g, at, c.areal, bt);
-  return launch_gemm<64, 64>(h, g, at, c.areal, bt);
+  switch (c.mode) {
+    case M64:
+      if (c.N <= 32) return launch_gemm<M64, cd, 64, 32>(h, g, at, c.areal, bt);
+      return launch_gemm<M64, cd, 64, 64>(h, g, at, c.areal, bt);
+    case M32:
+      return launch_gemm<M32, cf, 64, 64>(h, g, at, c.areal, bt);
+    case MBF3:
+      return launch_gemm<MBF3, cf, 64, 64>(h, g, at, c.areal, bt);
+  }
+  return ECALJ_ERR_ARG;
+}
+
+// complex128 <-> complex64 conversion
+__global__ void cvt_kernel_d2f(const cd* __restrict__ a, cf* __restrict__ b, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    b[e] = cf{(float)a[e].re, (float)a[e].im};
+}
+__global__ void cvt_kernel_f2d(const cf* __restrict__ a, cd* __restrict__ b, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    b[e] = cd{(double)a[e].re, (double)a[e].im};
+}
+
+ecalj_status convert(ecalj_ctx* h, const void* src, bool src64, void* dst, bool dst64, int64_t n) {
+  if (n == 0 || src == dst) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  if (!src64 && dst64)
+    hipLaunchKernelGGL(cvt_kernel_f2d, dim3(blocks), dim3(256), 0, h->stream, (const cf*)src, (cd*)dst, n);
+  else if (src64 && !dst64)
+    hipLaunchKernelGGL(cvt_kernel_d2f, dim3(blocks), dim3(256), 0, h->stream, (const cd*)src, (cf*)dst, n);
+  else
+    ECALJ_HIP(hipMemcpyAsync(dst, src, n * (src64 ? sizeof(cd) : sizeof(cf)), hipMemcpyDeviceToDevice, h->stream));
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
 }
 
 // ==========================================================================
@@ -78,21 +110,23 @@ ecalj_status ppb_relayout(ecalj_ctx* h, const double* src, double* dst, int nv, 
 // Columns z_t live anywhere in `z` (offset col[t]); the bin's pair list is
 // [bstart[l], bstart[l+1]).  Work item = (list l, output slice, tile ti, tj).
 
-template <int BM, int BK>
+template <int BM, int BK, class T>
 struct LdGatherA {  // A(i,t) = conj(z[col[t] + i])
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BM * BK / kThreads;
   static constexpr int KSTEP = kThreads / BM;
-  const cd* z; const int64_t* col; int M, K;
-  struct Regs { double re[R]; double im[R]; };
+  const T* z; const int64_t* col; int M, K;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int m0, int k0, Regs& r) const {
     const int i = m0 + (threadIdx.x % BM);
     const int kb = k0 + threadIdx.x / BM;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = kb + q * KSTEP;
-      cd v = (i < M && k < K) ? z[col[k] + i] : cd{0.0, 0.0};
-      r.re[q] = v.re;
-      r.im[q] = -v.im;
+      V re = 0, im = 0;
+      if (i < M && k < K) ld_c<V>(z, col[k] + i, re, im);
+      r.re[q] = re;
+      r.im[q] = -im;
     }
   }
   template <class S>
@@ -100,34 +134,32 @@ struct LdGatherA {  // A(i,t) = conj(z[col[t] + i])
     const int i = threadIdx.x % BM;
     const int kb = threadIdx.x / BM;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.ar[(kb + q * KSTEP) * S::LDA + i] = r.re[q];
-      s.ai[(kb + q * KSTEP) * S::LDA + i] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putA(i, kb + q * KSTEP, r.re[q], r.im[q]);
   }
 };
 
-template <int BN, int BK>
+template <int BN, int BK, class T>
 struct LdGatherB {  // B(t,j) = w[t] * z[col[t] + off + j]
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BN * BK / kThreads;
   static constexpr int KSTEP = kThreads / BN;
-  const cd* z; const int64_t* col; const double* w; int64_t off; int N, K;
-  struct Regs { double re[R]; double im[R]; };
+  const T* z; const int64_t* col; const double* w; int64_t off; int N, K;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int n0, int k0, Regs& r) const {
     const int j = n0 + (threadIdx.x % BN);
     const int kb = k0 + threadIdx.x / BN;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = kb + q * KSTEP;
+      V re = 0, im = 0;
       if (j < N && k < K) {
-        const double wt = w[k];
-        cd v = z[col[k] + off + j];
-        r.re[q] = wt * v.re;
-        r.im[q] = wt * v.im;
-      } else {
-        r.re[q] = 0.0;
-        r.im[q] = 0.0;
+        const V wt = (V)w[k];
+        ld_c<V>(z, col[k] + off + j, re, im);
+        re *= wt;
+        im *= wt;
       }
+      r.re[q] = re;
+      r.im[q] = im;
     }
   }
   template <class S>
@@ -135,66 +167,72 @@ struct LdGatherB {  // B(t,j) = w[t] * z[col[t] + off + j]
     const int j = threadIdx.x % BN;
     const int kb = threadIdx.x / BN;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.br[(kb + q * KSTEP) * S::LDB + j] = r.re[q];
-      s.bi[(kb + q * KSTEP) * S::LDB + j] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putB(kb + q * KSTEP, j, r.re[q], r.im[q]);
   }
 };
 
 struct X0Item { int list, slice, ti, tj; };
 
-template <int BT>
-__global__ __launch_bounds__(kThreads) void x0_herk_kernel(const cd* __restrict__ z,
+template <int MODE, class T, int BT>
+__global__ __launch_bounds__(kThreads) void x0_herk_kernel(const T* __restrict__ z,
                                                            const int64_t* __restrict__ col,
                                                            const double* __restrict__ w,
                                                            const int64_t* __restrict__ bstart,
                                                            const X0Item* __restrict__ items,
-                                                           cd* __restrict__ rcxq, int64_t slice_elems,
+                                                           T* __restrict__ rcxq, int64_t slice_elems,
                                                            int npr, int npr_col, int ipr_col,
                                                            int mirror) {
-  constexpr int BK = 16;
-  __shared__ StageF64<BT, BT, BK> s;
+  constexpr int BK = ModeBK<MODE>::value;
+  __shared__ typename StageOf<MODE, BT, BT, BK>::type s;
   const X0Item it = items[blockIdx.x];
   const int64_t b0 = bstart[it.list];
   const int K = (int)(bstart[it.list + 1] - b0);
   const int m0 = it.ti * BT, n0 = it.tj * BT;
-  LdGatherA<BT, BK> la{z, col + b0, npr, K};
-  LdGatherB<BT, BK> lb{z, col + b0, w + b0, (int64_t)(ipr_col - 1), npr_col, K};
-  CoreF64<BT, BT, BK, false> core;
-  gemm_mainloop<BT, BT, BK, false>(la, lb, m0, n0, K, s, core);
-  cd* r = rcxq + slice_elems * it.slice;
+  LdGatherA<BT, BK, T> la{z, col + b0, npr, K};
+  LdGatherB<BT, BK, T> lb{z, col + b0, w + b0, (int64_t)(ipr_col - 1), npr_col, K};
+  Core<MODE, BT, BT, BK, false> core;
+  gemm_mainloop<MODE, BT, BT, BK, false>(la, lb, m0, n0, K, s, core);
+  T* r = rcxq + slice_elems * it.slice;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool mir = mirror && (it.ti != it.tj);
   core.for_each(wave >> 1, wave & 1, lane, [&](int rr, int cc, double xr, double xi) {
     const int i = m0 + rr, j = n0 + cc;
     if (i >= npr || j >= npr_col) return;
-    cd* p = r + i + (int64_t)npr * j;
-    cd o = *p;
-    *p = cd{o.re + xr, o.im + xi};
+    T* p = r + i + (int64_t)npr * j;
+    double orr, oi;
+    ld_c2(p, orr, oi);
+    st_c(p, orr + xr, oi + xi);
     if (mir) {
-      cd* q = r + j + (int64_t)npr * i;
-      cd oq = *q;
-      *q = cd{oq.re + xr, oq.im - xi};
+      T* q = r + j + (int64_t)npr * i;
+      ld_c2(q, orr, oi);
+      st_c(q, orr + xr, oi - xi);
     }
   });
 }
 
-ecalj_status x0_herk(ecalj_ctx* h, const cd* z, const int64_t* d_col, const double* d_w,
-                     const int64_t* d_bstart, const void* d_items, int nitems, cd* rcxq,
+ecalj_status x0_herk(ecalj_ctx* h, int mode, const void* z, const int64_t* d_col, const double* d_w,
+                     const int64_t* d_bstart, const void* d_items, int nitems, void* rcxq,
                      int64_t slice_elems, int npr, int npr_col, int ipr_col, int mirror) {
   if (nitems == 0) return ECALJ_OK;
-  hipLaunchKernelGGL(x0_herk_kernel<64>, dim3(nitems), dim3(kThreads), 0, h->stream, z, d_col, d_w,
-                     d_bstart, static_cast<const X0Item*>(d_items), rcxq, slice_elems, npr, npr_col,
-                     ipr_col, mirror);
+  const X0Item* it = static_cast<const X0Item*>(d_items);
+  if (mode == M64)
+    hipLaunchKernelGGL((x0_herk_kernel<M64, cd, 64>), dim3(nitems), dim3(kThreads), 0, h->stream, (const cd*)z, d_col,
+                       d_w, d_bstart, it, (cd*)rcxq, slice_elems, npr, npr_col, ipr_col, mirror);
+  else if (mode == M32)
+    hipLaunchKernelGGL((x0_herk_kernel<M32, cf, 64>), dim3(nitems), dim3(kThreads), 0, h->stream, (const cf*)z, d_col,
+                       d_w, d_bstart, it, (cf*)rcxq, slice_elems, npr, npr_col, ipr_col, mirror);
+  else
+    hipLaunchKernelGGL((x0_herk_kernel<MBF3, cf, 64>), dim3(nitems), dim3(kThreads), 0, h->stream, (const cf*)z, d_col,
+                       d_w, d_bstart, it, (cf*)rcxq, slice_elems, npr, npr_col, ipr_col, mirror);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
 
 // ==========================================================================
-// Hilbert transform to the imaginary axis: zxqi(e,v) = sum_iw coef(v,iw) rcxq(e,iw)
-template <int NV>
-__global__ __launch_bounds__(256) void hilbert_kernel(cd* __restrict__ rcxq, int64_t slice, int nwhis,
+// Hilbert transform to the imaginary axis: zxqi(e,v) = sum_iw coef(v,iw) rcxq(e,iw),
+// FP64 accumulation, complex128 output; rcxq complex128 or complex64.
+template <int NV, class T>
+__global__ __launch_bounds__(256) void hilbert_kernel(T* __restrict__ rcxq, int64_t slice, int nwhis,
                                                       int nv, const double* __restrict__ coef,
                                                       const double* __restrict__ scale,
                                                       cd* __restrict__ zxqi, int64_t n) {
@@ -204,17 +242,18 @@ __global__ __launch_bounds__(256) void hilbert_kernel(cd* __restrict__ rcxq, int
 #pragma unroll
   for (int v = 0; v < NV; ++v) { ar[v] = 0.0; ai[v] = 0.0; }
   for (int iw = 1; iw <= nwhis; ++iw) {
-    cd x = rcxq[e + slice * iw];
+    double xr, xi;
+    ld_c2(rcxq + e + slice * iw, xr, xi);
     if (scale) {
       const double s = scale[iw - 1];
-      rcxq[e + slice * iw] = cd{s * x.re, s * x.im};
+      st_c(rcxq + e + slice * iw, s * xr, s * xi);
     }
     const double* c = coef + (int64_t)nv * (iw - 1);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       if (v < nv) {
-        ar[v] += c[v] * x.re;
-        ai[v] += c[v] * x.im;
+        ar[v] += c[v] * xr;
+        ai[v] += c[v] * xi;
       }
     }
   }
@@ -223,47 +262,55 @@ __global__ __launch_bounds__(256) void hilbert_kernel(cd* __restrict__ rcxq, int
     if (v < nv) zxqi[e + n * v] = cd{ar[v], ai[v]};
 }
 
-ecalj_status hilbert(ecalj_ctx* h, cd* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
-                     const double* d_scale, cd* zxqi, int64_t n) {
-  if (n == 0) return ECALJ_OK;
+template <class T>
+static ecalj_status hilbert_t(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
+                              const double* d_scale, cd* zxqi, int64_t n) {
   const dim3 grid((unsigned)((n + 255) / 256));
   if (nv <= 8)
-    hipLaunchKernelGGL(hilbert_kernel<8>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+    hipLaunchKernelGGL((hilbert_kernel<8, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
   else if (nv <= 16)
-    hipLaunchKernelGGL(hilbert_kernel<16>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+    hipLaunchKernelGGL((hilbert_kernel<16, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
   else if (nv <= 32)
-    hipLaunchKernelGGL(hilbert_kernel<32>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+    hipLaunchKernelGGL((hilbert_kernel<32, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
   else if (nv <= 64)
-    hipLaunchKernelGGL(hilbert_kernel<64>, grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+    hipLaunchKernelGGL((hilbert_kernel<64, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
   else
     return ECALJ_ERR_DIM;
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
 
+ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
+                     const double* d_scale, cd* zxqi, int64_t n) {
+  if (n == 0) return ECALJ_OK;
+  if (wide) return hilbert_t<cd>(h, (cd*)rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+  return hilbert_t<cf>(h, (cf*)rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+}
+
 // ==========================================================================
 // Composed-weight chi0: B(p,j) = w(v,p) * z(j,p), A(i,p) = conj(z(i,p)).
-template <int BN, int BK>
+template <int BN, int BK, class T>
 struct LdWeightedZT {  // B(k=p, j) = w[v + nf*p] * z[j + ni*p]
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BN * BK / kThreads;
   static constexpr int KSTEP = kThreads / BN;
-  const cd* z; const double* w; int64_t ldz; int nf, v, N, K;
-  struct Regs { double re[R]; double im[R]; };
+  const T* z; const double* w; int64_t ldz; int nf, v, N, K;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int n0, int k0, Regs& r) const {
     const int j = n0 + (threadIdx.x % BN);
     const int kb = k0 + threadIdx.x / BN;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = kb + q * KSTEP;
+      V re = 0, im = 0;
       if (j < N && k < K) {
-        const double wt = w[v + (int64_t)nf * k];
-        cd x = z[j + ldz * k];
-        r.re[q] = wt * x.re;
-        r.im[q] = wt * x.im;
-      } else {
-        r.re[q] = 0.0;
-        r.im[q] = 0.0;
+        const V wt = (V)w[v + (int64_t)nf * k];
+        ld_c<V>(z, j + ldz * k, re, im);
+        re *= wt;
+        im *= wt;
       }
+      r.re[q] = re;
+      r.im[q] = im;
     }
   }
   template <class S>
@@ -271,24 +318,23 @@ struct LdWeightedZT {  // B(k=p, j) = w[v + nf*p] * z[j + ni*p]
     const int j = threadIdx.x % BN;
     const int kb = threadIdx.x / BN;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.br[(kb + q * KSTEP) * S::LDB + j] = r.re[q];
-      s.bi[(kb + q * KSTEP) * S::LDB + j] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putB(kb + q * KSTEP, j, r.re[q], r.im[q]);
   }
 };
 
-__global__ __launch_bounds__(kThreads) void chi0_freq_kernel(const cd* __restrict__ z,
+template <int MODE, class T>
+__global__ __launch_bounds__(kThreads) void chi0_freq_kernel(const T* __restrict__ z,
                                                              const double* __restrict__ w, int ni,
                                                              int npairs, int nf, cd* __restrict__ chi) {
-  constexpr int BT = 64, BK = 16;
-  __shared__ StageF64<BT, BT, BK> s;
+  constexpr int BT = 64, BK = ModeBK<MODE>::value;
+  typedef typename ElemT<T>::V V;
+  __shared__ typename StageOf<MODE, BT, BT, BK>::type s;
   const int v = blockIdx.z;
   const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
-  LdA_N<BT, BK, false> la{z, (int64_t)ni, ni, npairs, -1.0};   // A(i,p) = conj(z(i,p)): column-major, rows contiguous
-  LdWeightedZT<BT, BK> lb{z, w, (int64_t)ni, nf, v, ni, npairs};
-  CoreF64<BT, BT, BK, false> core;
-  gemm_mainloop<BT, BT, BK, false>(la, lb, m0, n0, npairs, s, core);
+  LdA_N<BT, BK, T, false> la{z, (int64_t)ni, ni, npairs, (V)-1};  // A(i,p) = conj(z(i,p))
+  LdWeightedZT<BT, BK, T> lb{z, w, (int64_t)ni, nf, v, ni, npairs};
+  Core<MODE, BT, BT, BK, false> core;
+  gemm_mainloop<MODE, BT, BT, BK, false>(la, lb, m0, n0, npairs, s, core);
   cd* c = chi + (int64_t)ni * ni * v;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   core.for_each(wave >> 1, wave & 1, lane, [&](int rr, int cc, double xr, double xi) {
@@ -297,11 +343,16 @@ __global__ __launch_bounds__(kThreads) void chi0_freq_kernel(const cd* __restric
   });
 }
 
-ecalj_status chi0_freq(ecalj_ctx* h, const cd* z, const double* w, int ni, int npairs, int nf, cd* chi) {
+ecalj_status chi0_freq(ecalj_ctx* h, int mode, const void* z, const double* w, int ni, int npairs, int nf, cd* chi) {
   if (ni == 0 || nf == 0) return ECALJ_OK;
   if (nf > 65535) return ECALJ_ERR_DIM;
   dim3 grid((ni + 63) / 64, (ni + 63) / 64, nf);
-  hipLaunchKernelGGL(chi0_freq_kernel, grid, dim3(kThreads), 0, h->stream, z, w, ni, npairs, nf, chi);
+  if (mode == M64)
+    hipLaunchKernelGGL((chi0_freq_kernel<M64, cd>), grid, dim3(kThreads), 0, h->stream, (const cd*)z, w, ni, npairs, nf, chi);
+  else if (mode == M32)
+    hipLaunchKernelGGL((chi0_freq_kernel<M32, cf>), grid, dim3(kThreads), 0, h->stream, (const cf*)z, w, ni, npairs, nf, chi);
+  else
+    hipLaunchKernelGGL((chi0_freq_kernel<MBF3, cf>), grid, dim3(kThreads), 0, h->stream, (const cf*)z, w, ni, npairs, nf, chi);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
@@ -446,7 +497,7 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
     r.B = a + k0; r.ldb = lda; r.bB = {astride, 0};
     r.C = a + k0; r.ldc = lda; r.bC = {astride, 0};
     r.batch = nmat; r.div = nmat;
-    ECALJ_TRY(gemm_f64(h, r));
+    ECALJ_TRY(gemm(h, r));
     // (3) trailing update over all rows not in K
     GemmCall t;
     t.M = n; t.N = n; t.K = b;
@@ -455,7 +506,7 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
     t.C = a; t.ldc = lda; t.bC = {astride, 0};
     t.batch = nmat; t.div = nmat;
     t.epi = EPI_GJ; t.gj_k0 = k0; t.gj_k1 = k0 + b;
-    ECALJ_TRY(gemm_f64(h, t));
+    ECALJ_TRY(gemm(h, t));
   }
   return ECALJ_OK;
 }
@@ -464,8 +515,10 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
 // Sigma_c
 // Unpack the keepwv upper-triangle form and take the conjugate transpose in
 // one pass:  A_v(I,J) = conj(wc_v(J,I)) = (I <= J) ? W_v(I,J) : conj(W_v(J,I)).
+// W is complex128 (FP64 W build); the output is in the contraction's type.
+template <class T>
 __global__ void sig_unpack_kernel(const cd* __restrict__ w, int64_t ldw, int64_t wstride, int n, int nw,
-                                  cd* __restrict__ out) {
+                                  T* __restrict__ out) {
   const int64_t nn = (int64_t)n * n, tot = nn * nw;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -473,24 +526,26 @@ __global__ void sig_unpack_kernel(const cd* __restrict__ w, int64_t ldw, int64_t
     const int64_t ij = e % nn;
     const int i = (int)(ij % n), j = (int)(ij / n);
     const cd* W = w + wstride * iw;
-    cd x;
+    double xr, xi;
     if (i <= j) {
-      x = W[i + ldw * j];
+      xr = W[i + ldw * j].re;
+      xi = W[i + ldw * j].im;
     } else {
-      cd y = W[j + ldw * i];
-      x = cd{y.re, -y.im};
+      xr = W[j + ldw * i].re;
+      xi = -W[j + ldw * i].im;
     }
-    out[e] = x;
+    st_c(out + e, xr, xi);
   }
 }
 
 // B(kk, c) = wgt[v + (niw+1)*c] * zmel[J + n*c],  v = kk / n, J = kk % n
-template <int BN, int BK>
+template <int BN, int BK, class T>
 struct LdSigB {
+  typedef typename ElemT<T>::V V;
   static constexpr int R = BN * BK / kThreads;
   static constexpr int JSTEP = kThreads / BK;
-  const cd* zmel; const double* wgt; int n, nw1, N, K;
-  struct Regs { double re[R]; double im[R]; };
+  const T* zmel; const double* wgt; int n, nw1, N, K;
+  struct Regs { V re[R]; V im[R]; };
   __device__ void fetch(int n0, int k0, Regs& r) const {
     const int kk = k0 + (threadIdx.x % BK);
     const int v = kk / n;
@@ -499,15 +554,15 @@ struct LdSigB {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int c = cb + q * JSTEP;
+      V re = 0, im = 0;
       if (c < N && kk < K) {
-        const double g = wgt[v + (int64_t)nw1 * c];
-        cd x = zmel[J + (int64_t)n * c];
-        r.re[q] = g * x.re;
-        r.im[q] = g * x.im;
-      } else {
-        r.re[q] = 0.0;
-        r.im[q] = 0.0;
+        const V g = (V)wgt[v + (int64_t)nw1 * c];
+        ld_c<V>(zmel, J + (int64_t)n * c, re, im);
+        re *= g;
+        im *= g;
       }
+      r.re[q] = re;
+      r.im[q] = im;
     }
   }
   template <class S>
@@ -515,101 +570,47 @@ struct LdSigB {
     const int k = threadIdx.x % BK;
     const int jb = threadIdx.x / BK;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      s.br[k * S::LDB + jb + q * JSTEP] = r.re[q];
-      s.bi[k * S::LDB + jb + q * JSTEP] = r.im[q];
-    }
+    for (int q = 0; q < R; ++q) s.putB(k, jb + q * JSTEP, r.re[q], r.im[q]);
   }
 };
 
 // czmelwc_b = Wh_b (n x nw1*n) * B_b ; one z-slice per batch (internal k)
-template <int BM, int BN>
-__global__ __launch_bounds__(kThreads) void sig_freq_gemm_kernel(const cd* __restrict__ wh, int64_t wh_stride,
-                                                                 const cd* __restrict__ zmel, int64_t z_stride,
+template <int MODE, class T, int BM, int BN>
+__global__ __launch_bounds__(kThreads) void sig_freq_gemm_kernel(const T* __restrict__ wh, int64_t wh_stride,
+                                                                 const T* __restrict__ zmel, int64_t z_stride,
                                                                  const double* __restrict__ wgt, int64_t g_stride,
-                                                                 int n, int nw1, int ncol, cd* __restrict__ cz,
+                                                                 int n, int nw1, int ncol, T* __restrict__ cz,
                                                                  int64_t cz_stride) {
-  constexpr int BK = 16;
-  __shared__ StageF64<BM, BN, BK> s;
+  constexpr int BK = ModeBK<MODE>::value;
+  typedef typename ElemT<T>::V V;
+  __shared__ typename StageOf<MODE, BM, BN, BK>::type s;
   const int b = blockIdx.z;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int K = n * nw1;
-  LdA_N<BM, BK, false> la{wh + wh_stride * b, (int64_t)n, n, K, 1.0};
-  LdSigB<BN, BK> lb{zmel + z_stride * b, wgt + g_stride * b, n, nw1, ncol, K};
-  CoreF64<BM, BN, BK, false> core;
-  gemm_mainloop<BM, BN, BK, false>(la, lb, m0, n0, K, s, core);
-  cd* c = cz + cz_stride * b;
+  LdA_N<BM, BK, T, false> la{wh + wh_stride * b, (int64_t)n, n, K, (V)1};
+  LdSigB<BN, BK, T> lb{zmel + z_stride * b, wgt + g_stride * b, n, nw1, ncol, K};
+  Core<MODE, BM, BN, BK, false> core;
+  gemm_mainloop<MODE, BM, BN, BK, false>(la, lb, m0, n0, K, s, core);
+  T* c = cz + cz_stride * b;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   core.for_each(wave >> 1, wave & 1, lane, [&](int rr, int cc, double xr, double xi) {
     const int i = m0 + rr, j = n0 + cc;
-    if (i < n && j < ncol) c[i + (int64_t)n * j] = cd{xr, xi};
+    if (i < n && j < ncol) st_c(c + i + (int64_t)n * j, xr, xi);
   });
 }
 
-// zsec partials: part[blk][m + ntq*m'] = sum_{r in chunk} conj(cz[r + R*m]) zmel[r + R*m']
-constexpr int SIG_RC = 32;
-__global__ __launch_bounds__(256) void sig_final_partial_kernel(const cd* __restrict__ cz, const cd* __restrict__ zmel,
-                                                                int64_t R, int ntq, int64_t cz_stride,
-                                                                int64_t z_stride, int chunks_per_batch,
-                                                                cd* __restrict__ part) {
-  extern __shared__ cd sh[];
-  cd* sa = sh;                        // SIG_RC x ntq
-  cd* sb = sh + SIG_RC * ntq;         // SIG_RC x ntq
-  const int b = blockIdx.x / chunks_per_batch;
-  const int ch = blockIdx.x % chunks_per_batch;
-  const cd* A = cz + cz_stride * b;
-  const cd* B = zmel + z_stride * b;
-  const int np = ntq * ntq;
-  double accr[8], acci[8];
-  for (int q = 0; q < 8; ++q) { accr[q] = 0.0; acci[q] = 0.0; }
-  const int64_t rows_per_chunk = (R + chunks_per_batch - 1) / chunks_per_batch;
-  const int64_t r_begin = ch * rows_per_chunk;
-  const int64_t r_end = std::min<int64_t>(R, r_begin + rows_per_chunk);
-  for (int64_t r0 = r_begin; r0 < r_end; r0 += SIG_RC) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < SIG_RC * ntq; e += blockDim.x) {
-      const int rr = e % SIG_RC, m = e / SIG_RC;
-      const int64_t r = r0 + rr;
-      const bool ok = r < r_end;
-      sa[e] = ok ? A[r + R * m] : cd{0.0, 0.0};
-      sb[e] = ok ? B[r + R * m] : cd{0.0, 0.0};
-    }
-    __syncthreads();
-    for (int q = 0; q < 8; ++q) {
-      const int pidx = threadIdx.x + 256 * q;
-      if (pidx >= np) break;
-      const int m = pidx % ntq, mp = pidx / ntq;
-      const cd* pa = sa + SIG_RC * m;
-      const cd* pb = sb + SIG_RC * mp;
-      double sr = 0.0, si = 0.0;
-#pragma unroll 8
-      for (int rr = 0; rr < SIG_RC; ++rr) {
-        const cd x = pa[rr], y = pb[rr];
-        sr += x.re * y.re + x.im * y.im;
-        si += x.re * y.im - x.im * y.re;
-      }
-      accr[q] += sr;
-      acci[q] += si;
-    }
-  }
-  cd* P = part + (int64_t)np * blockIdx.x;
-  for (int q = 0; q < 8; ++q) {
-    const int pidx = threadIdx.x + 256 * q;
-    if (pidx >= np) break;
-    P[pidx] = cd{accr[q], acci[q]};
-  }
-}
-
 // zsec(m,m') += sum over partials in fixed order; then clip Im(zsec_mm) <= 0
-__global__ void sig_final_reduce_kernel(const cd* __restrict__ part, int nparts, int ntq, cd* __restrict__ zsec,
+template <class T>
+__global__ void sig_final_reduce_kernel(const T* __restrict__ part, int nparts, int ntq, cd* __restrict__ zsec,
                                         int64_t ldz) {
   const int np = ntq * ntq;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
     double sr = 0.0, si = 0.0;
     for (int k = 0; k < nparts; ++k) {
-      cd x = part[(int64_t)np * k + p];
-      sr += x.re;
-      si += x.im;
+      double xr, xi;
+      ld_c2(part + (int64_t)np * k + p, xr, xi);
+      sr += xr;
+      si += xi;
     }
     const int m = p % ntq, mp = p / ntq;
     cd* z = zsec + m + ldz * mp;
@@ -620,54 +621,84 @@ __global__ void sig_final_reduce_kernel(const cd* __restrict__ part, int nparts,
   }
 }
 
-ecalj_status sig_contract(ecalj_ctx* h, int nbatch, const cd* zmel, int n, int nm, int ntq, int niw,
-                          const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
+template <int MODE, class T>
+static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int n, int nm, int ntq, int niw,
+                                   const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
   const int nw1 = niw + 1;
   const int ncol = nm * ntq;
-  if (n == 0 || ncol == 0 || nbatch == 0) return ECALJ_OK;
   const int64_t nn = (int64_t)n * n;
-  cd *wh, *cz, *part;
+  T *wh, *cz, *part;
   // W unpacked per internal k; processed in groups to bound scratch memory
-  const size_t wh_bytes_per = sizeof(cd) * nn * nw1;
-  const size_t cz_bytes_per = sizeof(cd) * (size_t)n * ncol;
+  const size_t wh_bytes_per = sizeof(T) * nn * nw1;
+  const size_t cz_bytes_per = sizeof(T) * (size_t)n * ncol;
   int group = (int)std::max<size_t>(1, std::min<size_t>(nbatch, ((size_t)8 << 30) / (wh_bytes_per + cz_bytes_per)));
   group = std::min(group, 65535);
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_WH, wh_bytes_per * group, (void**)&wh));
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CZ, cz_bytes_per * group, (void**)&cz));
+  // final product: split-K partials of czmelwc^H zmel (K = n*nm per internal k)
   const int64_t R = (int64_t)n * nm;
-  int chunks = (int)std::max<int64_t>(1, std::min<int64_t>(64, (R + 4095) / 4096));
+  const int KC = 2048;
+  const int chunks = (int)((R + KC - 1) / KC);
   const int np = ntq * ntq;
-  if (np > 8 * 256) return ECALJ_ERR_DIM;
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_PART, sizeof(cd) * (size_t)np * chunks * nbatch, (void**)&part));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_PART, sizeof(T) * (size_t)np * chunks * nbatch, (void**)&part));
+  const int64_t wstride = ldw * n * nw1;
+  const int64_t tot = nn * nw1;
+  const int ublocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
   for (int b0 = 0; b0 < nbatch; b0 += group) {
     const int gb = std::min(group, nbatch - b0);
-    const int64_t wstride = ldw * n * nw1;
-    const int64_t tot = nn * nw1;
-    const int ublocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
     for (int q = 0; q < gb; ++q) {
-      hipLaunchKernelGGL(sig_unpack_kernel, dim3(ublocks), dim3(256), 0, h->stream, wc + wstride * (b0 + q), ldw, ldw * n,
-                         n, nw1, wh + tot * q);
+      hipLaunchKernelGGL((sig_unpack_kernel<T>), dim3(ublocks), dim3(256), 0, h->stream, wc + wstride * (b0 + q), ldw,
+                         ldw * n, n, nw1, wh + tot * q);
       ECALJ_HIP(hipGetLastError());
     }
     int tev = -1;
     ECALJ_TRY(timing_begin(h, 0, &tev));
     dim3 grid((n + 63) / 64, (ncol + 63) / 64, gb);
     if (grid.y > 65535) return ECALJ_ERR_DIM;
-    hipLaunchKernelGGL((sig_freq_gemm_kernel<64, 64>), grid, dim3(kThreads), 0, h->stream, wh, tot,
+    hipLaunchKernelGGL((sig_freq_gemm_kernel<MODE, T, 64, 64>), grid, dim3(kThreads), 0, h->stream, wh, tot,
                        zmel + (int64_t)n * ncol * b0, (int64_t)n * ncol, wgt + (int64_t)nw1 * ncol * b0,
                        (int64_t)nw1 * ncol, n, nw1, ncol, cz, (int64_t)n * ncol);
     ECALJ_HIP(hipGetLastError());
     ECALJ_TRY(timing_end(h, 0, tev));
-    const size_t shm = sizeof(cd) * 2 * SIG_RC * ntq;
-    hipLaunchKernelGGL(sig_final_partial_kernel, dim3(chunks * gb), dim3(256), shm, h->stream, cz,
-                       zmel + (int64_t)n * ncol * b0, R, ntq, (int64_t)n * ncol, (int64_t)n * ncol, chunks,
-                       part + (int64_t)np * chunks * b0);
-    ECALJ_HIP(hipGetLastError());
+    // partial[chunk][k] (ntq x ntq) = cz_k(chunk rows)^H zmel_k(chunk rows)
+    GemmCall f;
+    f.mode = MODE;
+    f.ta = true; f.ca = true;
+    f.M = ntq; f.N = ntq; f.K = KC;
+    f.A = cz; f.lda = R; f.bA = {KC, (int64_t)n * ncol};
+    f.B = zmel + (int64_t)n * ncol * b0; f.ldb = R; f.bB = {KC, (int64_t)n * ncol};
+    f.C = part + (int64_t)np * chunks * b0; f.ldc = ntq; f.bC = {(int64_t)np, (int64_t)np * chunks};
+    f.batch = chunks * gb; f.div = chunks;
+    if (R % KC != 0) {
+      // last chunk is shorter: run full chunks, then the tail separately
+      f.batch = (chunks - 1) * gb;
+      f.div = chunks - 1;
+      f.bC = {(int64_t)np, (int64_t)np * chunks};
+      if (chunks > 1) ECALJ_TRY(gemm(h, f));
+      GemmCall t = f;
+      t.K = (int)(R - (int64_t)KC * (chunks - 1));
+      t.A = cz + (int64_t)KC * (chunks - 1);
+      t.B = zmel + (int64_t)n * ncol * b0 + (int64_t)KC * (chunks - 1);
+      t.C = part + (int64_t)np * chunks * b0 + (int64_t)np * (chunks - 1);
+      t.bA = {0, (int64_t)n * ncol}; t.bB = {0, (int64_t)n * ncol}; t.bC = {0, (int64_t)np * chunks};
+      t.batch = gb; t.div = 1;
+      ECALJ_TRY(gemm(h, t));
+    } else {
+      ECALJ_TRY(gemm(h, f));
+    }
   }
-  hipLaunchKernelGGL(sig_final_reduce_kernel, dim3((np + 255) / 256), dim3(256), 0, h->stream, part, chunks * nbatch,
-                     ntq, zsec, ldz);
+  hipLaunchKernelGGL((sig_final_reduce_kernel<T>), dim3((np + 255) / 256), dim3(256), 0, h->stream, part,
+                     chunks * nbatch, ntq, zsec, ldz);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
+}
+
+ecalj_status sig_contract(ecalj_ctx* h, int mode, int nbatch, const void* zmel, int n, int nm, int ntq, int niw,
+                          const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
+  if (n == 0 || nm * ntq == 0 || nbatch == 0) return ECALJ_OK;
+  if (mode == M64) return sig_contract_t<M64, cd>(h, nbatch, (const cd*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz);
+  if (mode == M32) return sig_contract_t<M32, cf>(h, nbatch, (const cf*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz);
+  return sig_contract_t<MBF3, cf>(h, nbatch, (const cf*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz);
 }
 
 // --------------------------------------------------------------------------

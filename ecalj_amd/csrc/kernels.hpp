@@ -5,17 +5,18 @@
 namespace ecalj {
 
 ecalj_status ppb_relayout(ecalj_ctx* h, const double* src, double* dst, int nv, int md);
-ecalj_status x0_herk(ecalj_ctx* h, const cd* z, const int64_t* d_col, const double* d_w,
-                     const int64_t* d_bstart, const void* d_items, int nitems, cd* rcxq,
+ecalj_status x0_herk(ecalj_ctx* h, int mode, const void* z, const int64_t* d_col, const double* d_w,
+                     const int64_t* d_bstart, const void* d_items, int nitems, void* rcxq,
                      int64_t slice_elems, int npr, int npr_col, int ipr_col, int mirror);
-ecalj_status hilbert(ecalj_ctx* h, cd* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
+ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
                      const double* d_scale, cd* zxqi, int64_t n);
-ecalj_status chi0_freq(ecalj_ctx* h, const cd* z, const double* w, int ni, int npairs, int nf, cd* chi);
+ecalj_status chi0_freq(ecalj_ctx* h, int mode, const void* z, const double* w, int ni, int npairs, int nf, cd* chi);
+ecalj_status convert(ecalj_ctx* h, const void* src, bool src64, void* dst, bool dst64, int64_t n);
 ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac, int n, int nw, cd* eps);
 ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const double* s, int n, int nw, cd* w, int64_t ldw,
                      int64_t wstride);
 ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat);
-ecalj_status sig_contract(ecalj_ctx* h, int nbatch, const cd* zmel, int n, int nm, int ntq, int niw,
+ecalj_status sig_contract(ecalj_ctx* h, int mode, int nbatch, const void* zmel, int n, int nm, int ntq, int niw,
                           const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz);
 ecalj_status sig_weights(ecalj_ctx* h, int nbatch, int niw, int nm, int ntq, int ns1, int nctot,
                          const double* omega, const double* ekc, double esmr, double ua, const double* wkkr,

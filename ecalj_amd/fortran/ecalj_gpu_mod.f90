@@ -1,0 +1,176 @@
+!> ISO_C_BINDING interfaces to libecalj_gpu.so (include/ecalj_gpu.h).
+!! The argument layout follows the reference routines each entry point
+!! replaces; Fortran arrays are passed by address (column-major, complex(8)
+!! interleaved), scalars by value.  Every function returns an integer
+!! status; 0 is success (see ecalj_status_string).
+module ecalj_gpu
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+  public :: ecalj_create, ecalj_destroy, ecalj_set_precision, ecalj_sync
+  public :: ecalj_set_ppb, ecalj_set_coulomb, ecalj_zmel, ecalj_x0gemm, ecalj_dpsion_imag
+  public :: ecalj_wc_build, ecalj_sigc_weights, ecalj_sigc, ecalj_qp_energy, ecalj_sigma_herm
+  public :: ecalj_zmel_dev, ecalj_x0gemm_batched_dev, ecalj_dpsion_imag_dev, ecalj_wc_build_dev
+  public :: ecalj_sigc_weights_dev, ecalj_sigc_dev
+  integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
+
+  interface
+    ! gpu_init (S/m_gpu.f90:14-68)
+    integer(c_int) function ecalj_create(h, device) bind(C, name="ecalj_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: device
+    end function
+    integer(c_int) function ecalj_destroy(h) bind(C, name="ecalj_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function ecalj_set_precision(h, prec) bind(C, name="ecalj_set_precision")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: prec
+    end function
+    integer(c_int) function ecalj_sync(h) bind(C, name="ecalj_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    ! set_ppb (S/m_zmel.f90:110-145)
+    integer(c_int) function ecalj_set_ppb(h, natom, nv, mdim, ppb) bind(C, name="ecalj_set_ppb")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: natom
+      integer(c_int), intent(in) :: nv(*), mdim(*)
+      real(c_double), intent(in) :: ppb(*)
+    end function
+    ! Readvcoud + set_m2e_prod_basis (S/m_readVcoud.f90:11-43)
+    integer(c_int) function ecalj_set_coulomb(h, ngb, vcoud, zcousq) bind(C, name="ecalj_set_coulomb")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      integer(c_int), value :: ngb
+      real(c_double), intent(in) :: vcoud(*)
+      complex(c_double_complex), intent(in) :: zcousq(*)
+    end function
+    ! get_zmel_init_gemm (S/m_zmel.f90:146-597)
+    integer(c_int) function ecalj_zmel(h, phase, ndima, nm, cphim, nq, cphiq, is_m_basis, zmelconjg, nbatch, zmel) &
+        bind(C, name="ecalj_zmel")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: phase(*)
+      integer(c_int), value :: ndima, nm, nq, is_m_basis, zmelconjg, nbatch
+      complex(c_double_complex), intent(in) :: cphim(*), cphiq(*)
+      complex(c_double_complex), intent(out) :: zmel(*)
+    end function
+    integer(c_int) function ecalj_zmel_dev(h, phase, ndima, nm, cphim, nq, cphiq, is_m_basis, zmelconjg, nbatch, &
+        zmel) bind(C, name="ecalj_zmel_dev")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: phase(*)
+      integer(c_int), value :: ndima, nm, nq, is_m_basis, zmelconjg, nbatch
+      type(c_ptr), value :: cphim, cphiq, zmel
+    end function
+    ! x0gemm (S/x0gemm.f90:2-97); module inputs of m_x0kf/m_zmel made explicit
+    integer(c_int) function ecalj_x0gemm(h, rcxq, npr, ipr_col, npr_col, nwhis, npm, ns1, ns2, zmel, nbb, nqtot, &
+        icoun_lo, icoun_hi, iwini, iwend, itc, itpc, jpmc, icouini, whwc) bind(C, name="ecalj_x0gemm")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(inout) :: rcxq(*)
+      integer(c_int), value :: npr, ipr_col, npr_col, nwhis, npm, ns1, ns2, nbb, nqtot, icoun_lo, icoun_hi
+      complex(c_double_complex), intent(in) :: zmel(*)
+      integer(c_int), intent(in) :: iwini(*), iwend(*), itc(*), itpc(*), jpmc(*), icouini(*)
+      real(c_double), intent(in) :: whwc(*)
+    end function
+    integer(c_int) function ecalj_x0gemm_batched_dev(h, rcxq, npr, ipr_col, npr_col, nwhis, npm, nk, ns1, ns2, &
+        zmel, zmel_stride, nbb, nqtot, icounkmin, icounkmax, ncoun, iwini, iwend, itc, itpc, jpmc, icouini, ncount, &
+        whwc) bind(C, name="ecalj_x0gemm_batched_dev")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h, rcxq, zmel
+      integer(c_int), value :: npr, ipr_col, npr_col, nwhis, npm, nk, nbb, nqtot, ncoun, ncount
+      integer(c_int64_t), value :: zmel_stride
+      integer(c_int), intent(in) :: ns1(*), ns2(*), icounkmin(*), icounkmax(*)
+      integer(c_int), intent(in) :: iwini(*), iwend(*), itc(*), itpc(*), jpmc(*), icouini(*)
+      real(c_double), intent(in) :: whwc(*)
+    end function
+    ! dpsion_chiq imaginary axis (S/dpsion5.f90:211-232)
+    integer(c_int) function ecalj_dpsion_imag(h, rcxq, npr, npr_col, nwhis, niw, frhis, freqi, ecut, scale_rcxq, &
+        zxqi) bind(C, name="ecalj_dpsion_imag")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(inout) :: rcxq(*)
+      integer(c_int), value :: npr, npr_col, nwhis, niw, scale_rcxq
+      real(c_double), intent(in) :: frhis(*), freqi(*)
+      real(c_double), value :: ecut
+      complex(c_double_complex), intent(out) :: zxqi(*)
+    end function
+    integer(c_int) function ecalj_dpsion_imag_dev(h, rcxq, npr, npr_col, nwhis, niw, frhis, freqi, ecut, &
+        scale_rcxq, zxqi) bind(C, name="ecalj_dpsion_imag_dev")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h, rcxq, zxqi
+      integer(c_int), value :: npr, npr_col, nwhis, niw, scale_rcxq
+      real(c_double), intent(in) :: frhis(*), freqi(*)
+      real(c_double), value :: ecut
+    end function
+    ! WVIllwI body, q /= 0 (S/m_llw.f90:299-373)
+    integer(c_int) function ecalj_wc_build(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw) &
+        bind(C, name="ecalj_wc_build")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: chi(*)
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, ldw
+      complex(c_double_complex), intent(inout) :: wc(*)
+    end function
+    integer(c_int) function ecalj_wc_build_dev(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw) &
+        bind(C, name="ecalj_wc_build_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, chi, wc
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, ldw
+    end function
+    ! imaginary-axis weights (S/m_sxcf_gemm.f90:432-458)
+    integer(c_int) function ecalj_sigc_weights(h, niw, ns1, ns2, ntq, nctot, omega, ekc, esmr, ua, wkkr, freqx, &
+        wwx, expa, wgtim) bind(C, name="ecalj_sigc_weights")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: niw, ns1, ns2, ntq, nctot
+      real(c_double), intent(in) :: omega(*), ekc(*), freqx(*), wwx(*), expa(*)
+      real(c_double), value :: esmr, ua, wkkr
+      real(c_double), intent(out) :: wgtim(*)
+    end function
+    integer(c_int) function ecalj_sigc_weights_dev(h, nbatch, niw, ns1, ns2, ntq, nctot, omega, ekc, esmr, ua, &
+        wkkr, freqx, wwx, expa, wgtim) bind(C, name="ecalj_sigc_weights_dev")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h, omega, ekc, wkkr, freqx, wwx, expa, wgtim
+      integer(c_int), value :: nbatch, niw, ns1, ns2, ntq, nctot
+      real(c_double), value :: esmr, ua
+    end function
+    ! Sigma_c contraction (S/m_sxcf_gemm.f90:463-496, 599-608)
+    integer(c_int) function ecalj_sigc(h, zmel, ngb, nm, ntq, niw, wc, ldw, wgtim, zsec, ldz) &
+        bind(C, name="ecalj_sigc")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: zmel(*), wc(*)
+      integer(c_int), value :: ngb, nm, ntq, niw, ldw, ldz
+      real(c_double), intent(in) :: wgtim(*)
+      complex(c_double_complex), intent(inout) :: zsec(*)
+    end function
+    integer(c_int) function ecalj_sigc_dev(h, nbatch, zmel, ngb, nm, ntq, niw, wc, ldw, wgtim, zsec, ldz) &
+        bind(C, name="ecalj_sigc_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, zmel, wc, wgtim, zsec
+      integer(c_int), value :: nbatch, ngb, nm, ntq, niw, ldw, ldz
+    end function
+    ! QP energy (S/main_hqpe.sc.f90:152-161) and hermitised Sigma (:175-181)
+    integer(c_int) function ecalj_qp_energy(ntq, elda, sex, sexcore, sec, ldsec, vxc, eqp) &
+        bind(C, name="ecalj_qp_energy")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: ntq, ldsec
+      real(c_double), intent(in) :: elda(*), sex(*), sexcore(*), vxc(*)
+      complex(c_double_complex), intent(in) :: sec(*)
+      real(c_double), intent(out) :: eqp(*)
+    end function
+    integer(c_int) function ecalj_sigma_herm(ntq, sex2, sexcore2, sec, vxcmat, se) bind(C, name="ecalj_sigma_herm")
+      import :: c_int, c_double_complex
+      integer(c_int), value :: ntq
+      complex(c_double_complex), intent(in) :: sex2(*), sexcore2(*), sec(*), vxcmat(*)
+      complex(c_double_complex), intent(out) :: se(*)
+    end function
+  end interface
+end module ecalj_gpu

@@ -37,6 +37,9 @@ class QPointGPU:
         self.cfg = cfg = qp.cfg
         self.h = h = Handle(device, precision)
         c128, f64 = np.complex128, np.float64
+        # bulk intermediates (zmel, rcxq) in the contraction's storage type
+        cst = c128 if precision == "f64" else np.complex64
+        self.cst = cst
         no, nu, nbt, nb, ni, nk, ntq = cfg.no, cfg.nu, cfg.nbt, cfg.nb, cfg.ni, cfg.nk, cfg.nq
         self.nw1 = cfg.nf + 1
         # module state
@@ -57,12 +60,12 @@ class QPointGPU:
         self.nwhis = len(self.fr) - 1
         self.ix = synth.x0kf_index(synth.pair_weights(qp, self.fr))
         # buffers
-        self.zmel_chi = h.array((ni, no, nu, nk), c128)
-        self.rcxq = h.array((ni, ni, self.nwhis + 1), c128)
+        self.zmel_chi = h.array((ni, no, nu, nk), cst)
+        self.rcxq = h.array((ni, ni, self.nwhis + 1), cst)
         self.chi = h.array((ni, ni, self.nw1), c128)
         self.wc = h.array((ni, ni, self.nw1), c128)
         self.wk = h.array((ni, ni, self.nw1, nk), c128)
-        self.zmel_sig = h.array((ni, nbt, ntq, nk), c128)
+        self.zmel_sig = h.array((ni, nbt, ntq, nk), cst)
         self.wgt = h.array((self.nw1, nbt, ntq, nk), f64)
         self.zsec = h.array((ntq, ntq), c128)
         # Sigma weights inputs

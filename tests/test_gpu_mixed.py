@@ -1,0 +1,109 @@
+"""Mixed-precision forms (complex64 storage; FP32 MFMA or bf16x3 split MFMA,
+FP32 accumulation) against the FP64 oracle.  Bars, stated per result as
+max|gpu - oracle| / max|oracle|:
+  f32    : 2e-5   (FP32 rounding of data and of K-long sums)
+  bf16x3 : 1e-4   (hi/lo bf16 split keeps ~16 mantissa bits, 2^-16 ~ 1.5e-5 per product)
+and |eQP(gpu) - eQP(oracle)| < 1 meV for whole q-points, the accuracy bar of
+the mixed form."""
+import sys
+
+import numpy as np
+import pytest
+
+from ecalj_amd import synth
+
+pytestmark = pytest.mark.gpu
+TOL = {"f32": 2e-5, "bf16x3": 1e-4}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def crand(rng, *shape):
+    return np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+
+def relerr(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+@pytest.fixture(scope="module", params=["f32", "bf16x3"])
+def hp(request):
+    import ecalj_amd
+
+    hd = ecalj_amd.Handle(0, request.param)
+    yield hd, request.param
+    hd.close()
+
+
+def test_zmel_mixed(hp, orc):
+    h, prec = hp
+    say("zmel", prec)
+    rng = np.random.default_rng(21)
+    nv, md = [5, 9, 3], [7, 11, 4]
+    ppb = np.concatenate([rng.standard_normal((v, v, m)).ravel(order="F") for v, m in zip(nv, md)])
+    phase = np.exp(1j * rng.uniform(0, 6, 3))
+    h.set_ppb(nv, md, ppb)
+    m2e, _ = np.linalg.qr(crand(rng, 22, 22))
+    h.set_coulomb(rng.uniform(0.1, 5.0, 22), np.asfortranarray(m2e))
+    cm, cq = crand(rng, 19, 4, 3), crand(rng, 19, 37, 3)
+    for is_m, conj in ((True, True), (False, False)):
+        g = h.zmel(cm, cq, phase=phase, is_m_basis=is_m, zmelconjg=conj)
+        for b in range(3):
+            ref = orc.zmel(nv, md, phase, ppb, cm[:, :, b], cq[:, :, b], m2e=None if is_m else m2e, conjg=conj)
+            assert relerr(g[..., b], ref) < TOL[prec]
+
+
+def test_x0gemm_mixed(hp, orc):
+    h, prec = hp
+    say("x0gemm", prec)
+    rng = np.random.default_rng(22)
+    npr, nwhis, ns1, ns2, nq, npairs = 97, 10, 1, 4, 6, 400
+    z = crand(rng, npr, ns2 - ns1 + 1, nq)
+    itc = rng.integers(1, ns2 + 1, npairs).astype(np.int32)
+    itpc = rng.integers(1, nq + 1, npairs).astype(np.int32)
+    iwini = rng.integers(1, nwhis - 2, npairs).astype(np.int32)
+    nhw = rng.integers(1, 3, npairs).astype(np.int32)
+    iwend = (iwini + nhw - 1).astype(np.int32)
+    icouini = np.concatenate([[1], 1 + np.cumsum(nhw)[:-1]]).astype(np.int32)
+    whwc = rng.uniform(0.1, 1.0, int(nhw.sum()))
+    jpmc = np.ones(npairs, np.int32)
+    r0 = np.zeros((npr, npr, nwhis + 1), complex, order="F")
+    args = (npr, 1, npr, nwhis, 1, ns1, ns2, z, 1, npairs, iwini, iwend, itc, itpc, jpmc, icouini, whwc)
+    g = h.x0gemm(r0.copy(order="F"), *args)
+    o = orc.x0gemm(r0.copy(order="F"), *args)
+    assert relerr(g, o) < TOL[prec]
+
+
+@pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10)])
+def test_sigc_mixed(hp, orc, ngb, nm, ntq, niw):
+    h, prec = hp
+    say("sigc", prec, ngb, nm, ntq, niw)
+    rng = np.random.default_rng(ngb)
+    z = crand(rng, ngb, nm, ntq)
+    wc = crand(rng, ngb, ngb, niw + 1)
+    g = np.asfortranarray(rng.standard_normal((niw + 1, nm, ntq)))
+    assert relerr(h.sigc(z, wc, g), orc.sigc_imag(z, wc, g)) < TOL[prec]
+
+
+@pytest.mark.parametrize("name", ["T3", "C1"])
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+def test_qpoint_mixed_qp_energy_within_1meV(orc, name, prec):
+    from ecalj_amd.qsgw import QPointGPU
+    from oracle import pipeline
+
+    say("qpoint mixed", name, prec)
+    qp = synth.make_qpoint(synth.CONFIGS[name])
+    ref = pipeline.run(qp)
+    g = QPointGPU(qp, prec)
+    g.chi0()
+    g.wbuild()
+    g.make_wk()
+    g.sigc()
+    chi, wc, zsec, eqp = g.results()
+    err_mev = np.abs(eqp - ref.eqp).max() * 1e3
+    say(f"  max |dEqp| = {err_mev:.4f} meV, chi relerr {relerr(chi, ref.chi):.2e}, W relerr {relerr(wc, ref.wc):.2e}")
+    assert err_mev < 1.0
+    assert relerr(chi, ref.chi) < TOL[prec]
