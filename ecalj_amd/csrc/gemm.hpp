@@ -80,8 +80,14 @@ struct StagePlanar {
 template <int BM, int BN, int BK>
 struct StageBF3 {
   static constexpr int LDK = BK + 8;
-  unsigned short arh[BM * LDK], arl[BM * LDK], aih[BM * LDK], ail[BM * LDK];
-  unsigned short brh[BN * LDK], brl[BN * LDK], bih[BN * LDK], bil[BN * LDK];
+  alignas(16) unsigned short arh[BM * LDK];
+  alignas(16) unsigned short arl[BM * LDK];
+  alignas(16) unsigned short aih[BM * LDK];
+  alignas(16) unsigned short ail[BM * LDK];
+  alignas(16) unsigned short brh[BN * LDK];
+  alignas(16) unsigned short brl[BN * LDK];
+  alignas(16) unsigned short bih[BN * LDK];
+  alignas(16) unsigned short bil[BN * LDK];
   __device__ void putA(int i, int k, float re, float im) {
     unsigned short h, l;
     split_bf16(re, h, l);
@@ -100,6 +106,34 @@ struct StageBF3 {
     bih[j * LDK + k] = h;
     bil[j * LDK + k] = l;
   }
+  // 8 consecutive k of row x (A) or column x (B), written as 16-byte vectors
+  template <bool IS_A, class V>
+  __device__ void put8(int x, int k, const V* re, const V* im) {
+    typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+    us8v rh, rl, ih, il;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      unsigned short h, l;
+      split_bf16((float)re[e], h, l);
+      rh[e] = h;
+      rl[e] = l;
+      split_bf16((float)im[e], h, l);
+      ih[e] = h;
+      il[e] = l;
+    }
+    const int o = x * LDK + k;
+    if (IS_A) {
+      *reinterpret_cast<us8v*>(arh + o) = rh;
+      *reinterpret_cast<us8v*>(arl + o) = rl;
+      *reinterpret_cast<us8v*>(aih + o) = ih;
+      *reinterpret_cast<us8v*>(ail + o) = il;
+    } else {
+      *reinterpret_cast<us8v*>(brh + o) = rh;
+      *reinterpret_cast<us8v*>(brl + o) = rl;
+      *reinterpret_cast<us8v*>(bih + o) = ih;
+      *reinterpret_cast<us8v*>(bil + o) = il;
+    }
+  }
 };
 
 template <int MODE, int BM, int BN, int BK> struct StageOf;
@@ -110,138 +144,115 @@ template <int BM, int BN, int BK> struct StageOf<MBF3, BM, BN, BK> { typedef Sta
 template <int MODE> struct ModeBK { static constexpr int value = (MODE == MBF3) ? 32 : 16; };
 
 // --------------------------------------------------------------------------
-// Operand loaders.  Each owns a fixed thread -> element mapping chosen so
-// that consecutive lanes touch consecutive addresses of the contiguous
-// dimension.  fetch() reads one BK-deep stage into registers (zero outside
-// the matrix); store() hands the values to the stage (putA / putB).
+// Operand loaders.  An accessor gives element (x, k) of an operand -- x is
+// the row of A or the column of B -- already conjugated / weighted / gathered
+// as the kernel needs, zero outside the matrix, and says which index is
+// contiguous in memory.  The generic loader Ld maps threads to elements so
+// that consecutive lanes touch consecutive addresses, reads one BK-deep stage
+// into registers (fetch) and hands it to the LDS stage (store).  With KV = 8
+// (bf16 stages) every thread owns runs of 8 consecutive k of one row, which
+// the stage splits and writes as 16-byte vectors.
 
-// A(i,k) = a[i + lda*k]  (column-major, rows contiguous)
-template <int BM, int BK, class T, bool REAL>
-struct LdA_N {
+// A(x,k) = a[x + ld*k]  (x contiguous)
+template <class T, bool REAL>
+struct AccN {
   typedef typename ElemT<T>::V V;
-  static constexpr int R = BM * BK / kThreads;
-  static constexpr int KSTEP = kThreads / BM;
-  const void* a; int64_t lda; int M, K; V cs;
+  static constexpr bool kcontig = false;
+  const void* a; int64_t ld; int X, K; V cs;
+  __device__ void get(int x, int k, V& re, V& im) const {
+    re = 0;
+    im = 0;
+    if (x >= X || k >= K) return;
+    if (REAL) {
+      re = (V) static_cast<const double*>(a)[x + ld * k];
+    } else {
+      ld_c<V>(static_cast<const T*>(a), x + ld * k, re, im);
+      im *= cs;
+    }
+  }
+};
+
+// A(x,k) = a[k + ld*x]  (k contiguous)
+template <class T, bool REAL>
+struct AccT {
+  typedef typename ElemT<T>::V V;
+  static constexpr bool kcontig = true;
+  const void* a; int64_t ld; int X, K; V cs;
+  __device__ void get(int x, int k, V& re, V& im) const {
+    re = 0;
+    im = 0;
+    if (x >= X || k >= K) return;
+    if (REAL) {
+      re = (V) static_cast<const double*>(a)[k + ld * x];
+    } else {
+      ld_c<V>(static_cast<const T*>(a), k + ld * x, re, im);
+      im *= cs;
+    }
+  }
+};
+
+template <class Acc, int BX, int BK, bool IS_A, int KV>
+struct Ld {
+  typedef typename Acc::V V;
+  static constexpr int R = BX * BK / kThreads;
+  static_assert(R % KV == 0, "stage too small for the k-vector mapping");
+  Acc acc;
   struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int m0, int k0, Regs& r) const {
-    const int i = m0 + (threadIdx.x % BM);
-    const int kb = k0 + threadIdx.x / BM;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = kb + q * KSTEP;
-      const bool ok = (i < M) && (k < K);
-      if (REAL) {
-        r.re[q] = ok ? (V) static_cast<const double*>(a)[i + lda * k] : (V)0;
-        r.im[q] = 0;
+
+  __device__ static void coords(int q, int& x, int& k) {
+    const int t = threadIdx.x;
+    if (KV == 1) {
+      if (!Acc::kcontig) {
+        x = t % BX;
+        k = t / BX + q * (kThreads / BX);
       } else {
-        V re = 0, im = 0;
-        if (ok) ld_c<V>(static_cast<const T*>(a), i + lda * k, re, im);
-        r.re[q] = re;
-        r.im[q] = cs * im;
+        k = t % BK;
+        x = t / BK + q * (kThreads / BK);
+      }
+    } else {
+      const int g = q / KV, e = q % KV;
+      if (!Acc::kcontig) {
+        x = t % BX;
+        k = KV * (t / BX + g * (kThreads / BX)) + e;
+      } else {
+        constexpr int NKV = BK / KV;
+        x = t / NKV + g * (kThreads / NKV);
+        k = KV * (t % NKV) + e;
       }
     }
   }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int i = threadIdx.x % BM;
-    const int kb = threadIdx.x / BM;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putA(i, kb + q * KSTEP, r.re[q], r.im[q]);
-  }
-};
-
-// A(i,k) = a[k + lda*i]  (op(A) = A^T or A^H of a column-major matrix)
-template <int BM, int BK, class T, bool REAL>
-struct LdA_T {
-  typedef typename ElemT<T>::V V;
-  static constexpr int R = BM * BK / kThreads;
-  static constexpr int ISTEP = kThreads / BK;
-  const void* a; int64_t lda; int M, K; V cs;
-  struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int m0, int k0, Regs& r) const {
-    const int k = k0 + (threadIdx.x % BK);
-    const int ib = m0 + threadIdx.x / BK;
+  __device__ void fetch(int x0, int k0, Regs& r) const {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const int i = ib + q * ISTEP;
-      const bool ok = (i < M) && (k < K);
-      if (REAL) {
-        r.re[q] = ok ? (V) static_cast<const double*>(a)[k + lda * i] : (V)0;
-        r.im[q] = 0;
-      } else {
-        V re = 0, im = 0;
-        if (ok) ld_c<V>(static_cast<const T*>(a), k + lda * i, re, im);
-        r.re[q] = re;
-        r.im[q] = cs * im;
+      int x, k;
+      coords(q, x, k);
+      acc.get(x0 + x, k0 + k, r.re[q], r.im[q]);
+    }
+  }
+  template <class S>
+  __device__ void store(S& s, const Regs& r) const {
+    if constexpr (KV == 1) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        int x, k;
+        coords(q, x, k);
+        if (IS_A)
+          s.putA(x, k, r.re[q], r.im[q]);
+        else
+          s.putB(k, x, r.re[q], r.im[q]);
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < R / KV; ++g) {
+        int x, k;
+        coords(g * KV, x, k);
+        s.template put8<IS_A>(x, k, r.re + g * KV, r.im + g * KV);
       }
     }
   }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int k = threadIdx.x % BK;
-    const int ib = threadIdx.x / BK;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putA(ib + q * ISTEP, k, r.re[q], r.im[q]);
-  }
 };
 
-// B(k,j) = b[k + ldb*j]  (column-major, k contiguous)
-template <int BN, int BK, class T>
-struct LdB_N {
-  typedef typename ElemT<T>::V V;
-  static constexpr int R = BN * BK / kThreads;
-  static constexpr int JSTEP = kThreads / BK;
-  const T* b; int64_t ldb; int N, K; V cs;
-  struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int n0, int k0, Regs& r) const {
-    const int k = k0 + (threadIdx.x % BK);
-    const int jb = n0 + threadIdx.x / BK;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int j = jb + q * JSTEP;
-      V re = 0, im = 0;
-      if (j < N && k < K) ld_c<V>(b, k + ldb * j, re, im);
-      r.re[q] = re;
-      r.im[q] = cs * im;
-    }
-  }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int k = threadIdx.x % BK;
-    const int jb = threadIdx.x / BK;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putB(k, jb + q * JSTEP, r.re[q], r.im[q]);
-  }
-};
-
-// B(k,j) = b[j + ldb*k]  (op(B) = B^T or B^H, j contiguous)
-template <int BN, int BK, class T>
-struct LdB_T {
-  typedef typename ElemT<T>::V V;
-  static constexpr int R = BN * BK / kThreads;
-  static constexpr int KSTEP = kThreads / BN;
-  const T* b; int64_t ldb; int N, K; V cs;
-  struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int n0, int k0, Regs& r) const {
-    const int j = n0 + (threadIdx.x % BN);
-    const int kb = k0 + threadIdx.x / BN;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = kb + q * KSTEP;
-      V re = 0, im = 0;
-      if (j < N && k < K) ld_c<V>(b, j + ldb * k, re, im);
-      r.re[q] = re;
-      r.im[q] = cs * im;
-    }
-  }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int j = threadIdx.x % BN;
-    const int kb = threadIdx.x / BN;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putB(kb + q * KSTEP, j, r.re[q], r.im[q]);
-  }
-};
+template <int MODE> struct ModeKV { static constexpr int value = (MODE == MBF3) ? 8 : 1; };
 
 // --------------------------------------------------------------------------
 // MFMA cores: wave (wm, wn) of the 2x2 grid accumulates TM x TN tiles.
@@ -479,10 +490,13 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs g) {
   const T* Bb = static_cast<const T*>(g.B) + g.bB.off(b, g.div);
   T* Cb = static_cast<T*>(g.C) + g.bC.off(b, g.div);
   Core<MODE, BM, BN, BK, AREAL> core;
-  using LA = typename std::conditional<AT, LdA_T<BM, BK, T, AREAL>, LdA_N<BM, BK, T, AREAL>>::type;
-  using LB = typename std::conditional<BT, LdB_T<BN, BK, T>, LdB_N<BN, BK, T>>::type;
-  LA la{Ab, g.lda, g.M, g.K, (V)g.csA};
-  LB lb{Bb, g.ldb, g.N, g.K, (V)g.csB};
+  constexpr int KV = ModeKV<MODE>::value;
+  // op(A)(i,k): stored A (N: A[i + lda k]) or A^T (T: A[k + lda i]);
+  // op(B)(k,j): stored B (N: B[k + ldb j], k contiguous) or B^T (T: B[j + ldb k])
+  using AccA = typename std::conditional<AT, AccT<T, AREAL>, AccN<T, AREAL>>::type;
+  using AccB = typename std::conditional<BT, AccN<T, false>, AccT<T, false>>::type;
+  Ld<AccA, BM, BK, true, KV> la{AccA{Ab, g.lda, g.M, g.K, (V)g.csA}};
+  Ld<AccB, BN, BK, false, KV> lb{AccB{Bb, g.ldb, g.N, g.K, (V)g.csB}};
   gemm_mainloop<MODE, BM, BN, BK, AREAL>(la, lb, m0, n0, g.K, s, core);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;

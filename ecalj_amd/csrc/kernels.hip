@@ -110,64 +110,33 @@ ecalj_status ppb_relayout(ecalj_ctx* h, const double* src, double* dst, int nv, 
 // Columns z_t live anywhere in `z` (offset col[t]); the bin's pair list is
 // [bstart[l], bstart[l+1]).  Work item = (list l, output slice, tile ti, tj).
 
-template <int BM, int BK, class T>
-struct LdGatherA {  // A(i,t) = conj(z[col[t] + i])
+template <class T>
+struct AccGatherA {  // A(i,t) = conj(z[col[t] + i])
   typedef typename ElemT<T>::V V;
-  static constexpr int R = BM * BK / kThreads;
-  static constexpr int KSTEP = kThreads / BM;
-  const T* z; const int64_t* col; int M, K;
-  struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int m0, int k0, Regs& r) const {
-    const int i = m0 + (threadIdx.x % BM);
-    const int kb = k0 + threadIdx.x / BM;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = kb + q * KSTEP;
-      V re = 0, im = 0;
-      if (i < M && k < K) ld_c<V>(z, col[k] + i, re, im);
-      r.re[q] = re;
-      r.im[q] = -im;
-    }
-  }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int i = threadIdx.x % BM;
-    const int kb = threadIdx.x / BM;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putA(i, kb + q * KSTEP, r.re[q], r.im[q]);
+  static constexpr bool kcontig = false;
+  const T* z; const int64_t* col; int X, K;
+  __device__ void get(int x, int k, V& re, V& im) const {
+    re = 0;
+    im = 0;
+    if (x >= X || k >= K) return;
+    ld_c<V>(z, col[k] + x, re, im);
+    im = -im;
   }
 };
 
-template <int BN, int BK, class T>
-struct LdGatherB {  // B(t,j) = w[t] * z[col[t] + off + j]
+template <class T>
+struct AccGatherB {  // B(t,j) = w[t] * z[col[t] + off + j]
   typedef typename ElemT<T>::V V;
-  static constexpr int R = BN * BK / kThreads;
-  static constexpr int KSTEP = kThreads / BN;
-  const T* z; const int64_t* col; const double* w; int64_t off; int N, K;
-  struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int n0, int k0, Regs& r) const {
-    const int j = n0 + (threadIdx.x % BN);
-    const int kb = k0 + threadIdx.x / BN;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = kb + q * KSTEP;
-      V re = 0, im = 0;
-      if (j < N && k < K) {
-        const V wt = (V)w[k];
-        ld_c<V>(z, col[k] + off + j, re, im);
-        re *= wt;
-        im *= wt;
-      }
-      r.re[q] = re;
-      r.im[q] = im;
-    }
-  }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int j = threadIdx.x % BN;
-    const int kb = threadIdx.x / BN;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putB(kb + q * KSTEP, j, r.re[q], r.im[q]);
+  static constexpr bool kcontig = false;
+  const T* z; const int64_t* col; const double* w; int64_t off; int X, K;
+  __device__ void get(int x, int k, V& re, V& im) const {
+    re = 0;
+    im = 0;
+    if (x >= X || k >= K) return;
+    const V wt = (V)w[k];
+    ld_c<V>(z, col[k] + off + x, re, im);
+    re *= wt;
+    im *= wt;
   }
 };
 
@@ -188,8 +157,9 @@ __global__ __launch_bounds__(kThreads) void x0_herk_kernel(const T* __restrict__
   const int64_t b0 = bstart[it.list];
   const int K = (int)(bstart[it.list + 1] - b0);
   const int m0 = it.ti * BT, n0 = it.tj * BT;
-  LdGatherA<BT, BK, T> la{z, col + b0, npr, K};
-  LdGatherB<BT, BK, T> lb{z, col + b0, w + b0, (int64_t)(ipr_col - 1), npr_col, K};
+  constexpr int KV = ModeKV<MODE>::value;
+  Ld<AccGatherA<T>, BT, BK, true, KV> la{AccGatherA<T>{z, col + b0, npr, K}};
+  Ld<AccGatherB<T>, BT, BK, false, KV> lb{AccGatherB<T>{z, col + b0, w + b0, (int64_t)(ipr_col - 1), npr_col, K}};
   Core<MODE, BT, BT, BK, false> core;
   gemm_mainloop<MODE, BT, BT, BK, false>(la, lb, m0, n0, K, s, core);
   T* r = rcxq + slice_elems * it.slice;
@@ -289,36 +259,19 @@ ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwh
 
 // ==========================================================================
 // Composed-weight chi0: B(p,j) = w(v,p) * z(j,p), A(i,p) = conj(z(i,p)).
-template <int BN, int BK, class T>
-struct LdWeightedZT {  // B(k=p, j) = w[v + nf*p] * z[j + ni*p]
+template <class T>
+struct AccWZT {  // B(k=p, j) = w[v + nf*p] * z[j + ni*p]
   typedef typename ElemT<T>::V V;
-  static constexpr int R = BN * BK / kThreads;
-  static constexpr int KSTEP = kThreads / BN;
-  const T* z; const double* w; int64_t ldz; int nf, v, N, K;
-  struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int n0, int k0, Regs& r) const {
-    const int j = n0 + (threadIdx.x % BN);
-    const int kb = k0 + threadIdx.x / BN;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = kb + q * KSTEP;
-      V re = 0, im = 0;
-      if (j < N && k < K) {
-        const V wt = (V)w[v + (int64_t)nf * k];
-        ld_c<V>(z, j + ldz * k, re, im);
-        re *= wt;
-        im *= wt;
-      }
-      r.re[q] = re;
-      r.im[q] = im;
-    }
-  }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int j = threadIdx.x % BN;
-    const int kb = threadIdx.x / BN;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putB(kb + q * KSTEP, j, r.re[q], r.im[q]);
+  static constexpr bool kcontig = false;
+  const T* z; const double* w; int64_t ldz; int nf, v, X, K;
+  __device__ void get(int x, int k, V& re, V& im) const {
+    re = 0;
+    im = 0;
+    if (x >= X || k >= K) return;
+    const V wt = (V)w[v + (int64_t)nf * k];
+    ld_c<V>(z, x + ldz * k, re, im);
+    re *= wt;
+    im *= wt;
   }
 };
 
@@ -331,8 +284,9 @@ __global__ __launch_bounds__(kThreads) void chi0_freq_kernel(const T* __restrict
   __shared__ typename StageOf<MODE, BT, BT, BK>::type s;
   const int v = blockIdx.z;
   const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
-  LdA_N<BT, BK, T, false> la{z, (int64_t)ni, ni, npairs, (V)-1};  // A(i,p) = conj(z(i,p))
-  LdWeightedZT<BT, BK, T> lb{z, w, (int64_t)ni, nf, v, ni, npairs};
+  constexpr int KV = ModeKV<MODE>::value;
+  Ld<AccN<T, false>, BT, BK, true, KV> la{AccN<T, false>{z, (int64_t)ni, ni, npairs, (V)-1}};  // conj(z(i,p))
+  Ld<AccWZT<T>, BT, BK, false, KV> lb{AccWZT<T>{z, w, (int64_t)ni, nf, v, ni, npairs}};
   Core<MODE, BT, BT, BK, false> core;
   gemm_mainloop<MODE, BT, BT, BK, false>(la, lb, m0, n0, npairs, s, core);
   cd* c = chi + (int64_t)ni * ni * v;
@@ -539,38 +493,21 @@ __global__ void sig_unpack_kernel(const cd* __restrict__ w, int64_t ldw, int64_t
 }
 
 // B(kk, c) = wgt[v + (niw+1)*c] * zmel[J + n*c],  v = kk / n, J = kk % n
-template <int BN, int BK, class T>
-struct LdSigB {
+template <class T>
+struct AccSigB {
   typedef typename ElemT<T>::V V;
-  static constexpr int R = BN * BK / kThreads;
-  static constexpr int JSTEP = kThreads / BK;
-  const T* zmel; const double* wgt; int n, nw1, N, K;
-  struct Regs { V re[R]; V im[R]; };
-  __device__ void fetch(int n0, int k0, Regs& r) const {
-    const int kk = k0 + (threadIdx.x % BK);
+  static constexpr bool kcontig = true;
+  const T* zmel; const double* wgt; int n, nw1, X, K;
+  __device__ void get(int x, int kk, V& re, V& im) const {
+    re = 0;
+    im = 0;
+    if (x >= X || kk >= K) return;
     const int v = kk / n;
     const int J = kk - v * n;
-    const int cb = n0 + threadIdx.x / BK;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int c = cb + q * JSTEP;
-      V re = 0, im = 0;
-      if (c < N && kk < K) {
-        const V g = (V)wgt[v + (int64_t)nw1 * c];
-        ld_c<V>(zmel, J + (int64_t)n * c, re, im);
-        re *= g;
-        im *= g;
-      }
-      r.re[q] = re;
-      r.im[q] = im;
-    }
-  }
-  template <class S>
-  __device__ void store(S& s, const Regs& r) const {
-    const int k = threadIdx.x % BK;
-    const int jb = threadIdx.x / BK;
-#pragma unroll
-    for (int q = 0; q < R; ++q) s.putB(k, jb + q * JSTEP, r.re[q], r.im[q]);
+    const V g = (V)wgt[v + (int64_t)nw1 * x];
+    ld_c<V>(zmel, J + (int64_t)n * x, re, im);
+    re *= g;
+    im *= g;
   }
 };
 
@@ -587,8 +524,9 @@ __global__ __launch_bounds__(kThreads) void sig_freq_gemm_kernel(const T* __rest
   const int b = blockIdx.z;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int K = n * nw1;
-  LdA_N<BM, BK, T, false> la{wh + wh_stride * b, (int64_t)n, n, K, (V)1};
-  LdSigB<BN, BK, T> lb{zmel + z_stride * b, wgt + g_stride * b, n, nw1, ncol, K};
+  constexpr int KV = ModeKV<MODE>::value;
+  Ld<AccN<T, false>, BM, BK, true, KV> la{AccN<T, false>{wh + wh_stride * b, (int64_t)n, n, K, (V)1}};
+  Ld<AccSigB<T>, BN, BK, false, KV> lb{AccSigB<T>{zmel + z_stride * b, wgt + g_stride * b, n, nw1, ncol, K}};
   Core<MODE, BM, BN, BK, false> core;
   gemm_mainloop<MODE, BM, BN, BK, false>(la, lb, m0, n0, K, s, core);
   T* c = cz + cz_stride * b;
@@ -653,9 +591,10 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
     }
     int tev = -1;
     ECALJ_TRY(timing_begin(h, 0, &tev));
-    dim3 grid((n + 63) / 64, (ncol + 63) / 64, gb);
+    constexpr int SB = (MODE == MBF3) ? 128 : 64;
+    dim3 grid((n + SB - 1) / SB, (ncol + SB - 1) / SB, gb);
     if (grid.y > 65535) return ECALJ_ERR_DIM;
-    hipLaunchKernelGGL((sig_freq_gemm_kernel<MODE, T, 64, 64>), grid, dim3(kThreads), 0, h->stream, wh, tot,
+    hipLaunchKernelGGL((sig_freq_gemm_kernel<MODE, T, SB, SB>), grid, dim3(kThreads), 0, h->stream, wh, tot,
                        zmel + (int64_t)n * ncol * b0, (int64_t)n * ncol, wgt + (int64_t)nw1 * ncol * b0,
                        (int64_t)nw1 * ncol, n, nw1, ncol, cz, (int64_t)n * ncol);
     ECALJ_HIP(hipGetLastError());
