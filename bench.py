@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["f64", "mixed"], default=None,
+                    help="profile one form only (prints a short line instead of the contract JSON)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one q-point per GPU; "
@@ -146,6 +148,11 @@ def main():
     cfg = synth.CONFIGS[args.config]
     log(f"bench: generating {cfg.name} inputs")
     qp = synth.make_qpoint(cfg)
+    if args.only:
+        info = run_form(qp, "f64" if args.only == "f64" else args.mixed, args.steps, args.warmup)
+        print(json.dumps({"form": args.only, "s_per_q": info["s_per_q"],
+                          "kernel_ms_per_step": {k: v[0] / args.steps for k, v in info["ktimes"].items()}}))
+        return
     f64 = run_form(qp, "f64", args.steps, args.warmup)
     mix = run_form(qp, args.mixed, args.steps, args.warmup)
     fl = flops_per_q(cfg, nwhis_pairs_bins=mix["npb"])

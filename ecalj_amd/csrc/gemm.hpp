@@ -370,6 +370,7 @@ struct Core<MBF3, BM, BN, BK, AREAL> {
     for (int kk = 0; kk < BK; kk += 32) {
       const int ko = kk + 8 * (lane >> 4);
       us8 brh[TN], brl[TN], bih[TN], bil[TN];
+      us8 arh[TM], arl[TM], aih[TM], ail[TM], nih[TM], nil[TM];
 #pragma unroll
       for (int t = 0; t < TN; ++t) {
         const int j = wn * TN * 16 + t * 16 + (lane & 15);
@@ -381,36 +382,36 @@ struct Core<MBF3, BM, BN, BK, AREAL> {
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
         const int ir = wm * TM * 16 + i * 16 + (lane & 15);
-        const us8 arh = ld8(s.arh + ir * S::LDK + ko);
-        const us8 arl = ld8(s.arl + ir * S::LDK + ko);
-        us8 aih, ail, nih, nil;
+        arh[i] = ld8(s.arh + ir * S::LDK + ko);
+        arl[i] = ld8(s.arl + ir * S::LDK + ko);
         if (!AREAL) {
-          aih = ld8(s.aih + ir * S::LDK + ko);
-          ail = ld8(s.ail + ir * S::LDK + ko);
-          nih = neg8(aih);
-          nil = neg8(ail);
-        }
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          f4 r = cr[i][j], m = ci[i][j];
-          r = mfma(arh, brh[j], r);
-          r = mfma(arh, brl[j], r);
-          r = mfma(arl, brh[j], r);
-          m = mfma(arh, bih[j], m);
-          m = mfma(arh, bil[j], m);
-          m = mfma(arl, bih[j], m);
-          if (!AREAL) {
-            r = mfma(nih, bih[j], r);
-            r = mfma(nih, bil[j], r);
-            r = mfma(nil, bih[j], r);
-            m = mfma(aih, brh[j], m);
-            m = mfma(aih, brl[j], m);
-            m = mfma(ail, brh[j], m);
-          }
-          cr[i][j] = r;
-          ci[i][j] = m;
+          aih[i] = ld8(s.aih + ir * S::LDK + ko);
+          ail[i] = ld8(s.ail + ir * S::LDK + ko);
+          nih[i] = neg8(aih[i]);
+          nil[i] = neg8(ail[i]);
         }
       }
+      // each product term sweeps all TM x TN tiles: consecutive MFMAs write
+      // different accumulators
+#define ECALJ_BF3_TERM(ACC, AF, BF)                   \
+  _Pragma("unroll") for (int i = 0; i < TM; ++i)      \
+  _Pragma("unroll") for (int j = 0; j < TN; ++j)      \
+    ACC[i][j] = mfma(AF[i], BF[j], ACC[i][j]);
+      ECALJ_BF3_TERM(cr, arh, brh)
+      ECALJ_BF3_TERM(ci, arh, bih)
+      ECALJ_BF3_TERM(cr, arh, brl)
+      ECALJ_BF3_TERM(ci, arh, bil)
+      ECALJ_BF3_TERM(cr, arl, brh)
+      ECALJ_BF3_TERM(ci, arl, bih)
+      if (!AREAL) {
+        ECALJ_BF3_TERM(cr, nih, bih)
+        ECALJ_BF3_TERM(ci, aih, brh)
+        ECALJ_BF3_TERM(cr, nih, bil)
+        ECALJ_BF3_TERM(ci, aih, brl)
+        ECALJ_BF3_TERM(cr, nil, bih)
+        ECALJ_BF3_TERM(ci, ail, brh)
+      }
+#undef ECALJ_BF3_TERM
     }
   }
 

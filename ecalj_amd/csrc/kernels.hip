@@ -566,8 +566,9 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
   const int ncol = nm * ntq;
   const int64_t nn = (int64_t)n * n;
   T *wh, *cz, *part;
-  // W unpacked per internal k; processed in groups to bound scratch memory
-  const size_t wh_bytes_per = sizeof(T) * nn * nw1;
+  // W unpacked (or, for bf16x3, split into bf16 planes) per internal k;
+  // processed in groups to bound scratch memory
+  const size_t wh_bytes_per = (MODE == MBF3) ? sig_bf3_bytes_per_k(n, ncol, nw1) : sizeof(T) * nn * nw1;
   const size_t cz_bytes_per = sizeof(T) * (size_t)n * ncol;
   int group = (int)std::max<size_t>(1, std::min<size_t>(nbatch, ((size_t)8 << 30) / (wh_bytes_per + cz_bytes_per)));
   group = std::min(group, 65535);
@@ -584,21 +585,28 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
   const int ublocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
   for (int b0 = 0; b0 < nbatch; b0 += group) {
     const int gb = std::min(group, nbatch - b0);
-    for (int q = 0; q < gb; ++q) {
-      hipLaunchKernelGGL((sig_unpack_kernel<T>), dim3(ublocks), dim3(256), 0, h->stream, wc + wstride * (b0 + q), ldw,
-                         ldw * n, n, nw1, wh + tot * q);
+    if constexpr (MODE == MBF3) {
+      unsigned short* as = reinterpret_cast<unsigned short*>(wh);
+      unsigned short* bs = as + sig_bf3_a_elems_per_k(n, nw1) * gb;
+      ECALJ_TRY(sig_bf3(h, gb, (const cf*)zmel + (int64_t)n * ncol * b0, (int64_t)n * ncol, n, ncol, nw1,
+                        wc + wstride * b0, ldw, wstride, wgt + (int64_t)nw1 * ncol * b0, (int64_t)nw1 * ncol,
+                        (cf*)cz, (int64_t)n * ncol, as, bs));
+    } else {
+      for (int q = 0; q < gb; ++q) {
+        hipLaunchKernelGGL((sig_unpack_kernel<T>), dim3(ublocks), dim3(256), 0, h->stream, wc + wstride * (b0 + q),
+                           ldw, ldw * n, n, nw1, wh + tot * q);
+        ECALJ_HIP(hipGetLastError());
+      }
+      int tev = -1;
+      ECALJ_TRY(timing_begin(h, 0, &tev));
+      dim3 grid((n + 63) / 64, (ncol + 63) / 64, gb);
+      if (grid.y > 65535) return ECALJ_ERR_DIM;
+      hipLaunchKernelGGL((sig_freq_gemm_kernel<MODE, T, 64, 64>), grid, dim3(kThreads), 0, h->stream, wh, tot,
+                         zmel + (int64_t)n * ncol * b0, (int64_t)n * ncol, wgt + (int64_t)nw1 * ncol * b0,
+                         (int64_t)nw1 * ncol, n, nw1, ncol, cz, (int64_t)n * ncol);
       ECALJ_HIP(hipGetLastError());
+      ECALJ_TRY(timing_end(h, 0, tev));
     }
-    int tev = -1;
-    ECALJ_TRY(timing_begin(h, 0, &tev));
-    constexpr int SB = (MODE == MBF3) ? 128 : 64;
-    dim3 grid((n + SB - 1) / SB, (ncol + SB - 1) / SB, gb);
-    if (grid.y > 65535) return ECALJ_ERR_DIM;
-    hipLaunchKernelGGL((sig_freq_gemm_kernel<MODE, T, SB, SB>), grid, dim3(kThreads), 0, h->stream, wh, tot,
-                       zmel + (int64_t)n * ncol * b0, (int64_t)n * ncol, wgt + (int64_t)nw1 * ncol * b0,
-                       (int64_t)nw1 * ncol, n, nw1, ncol, cz, (int64_t)n * ncol);
-    ECALJ_HIP(hipGetLastError());
-    ECALJ_TRY(timing_end(h, 0, tev));
     // partial[chunk][k] (ntq x ntq) = cz_k(chunk rows)^H zmel_k(chunk rows)
     GemmCall f;
     f.mode = MODE;

@@ -25,4 +25,10 @@ ecalj_status sig_weights(ecalj_ctx* h, int nbatch, int niw, int nm, int ntq, int
 ecalj_status dev_zero(ecalj_ctx* h, void* p, size_t bytes);
 ecalj_status phase_similarity(ecalj_ctx* h, const cd* a, int n, int nmat, const cd* d, int nk, cd* out);
 
+ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1,
+                     const cd* wc, int64_t ldw, int64_t w_stride, const double* wgt, int64_t g_stride, cf* cz,
+                     int64_t cz_stride, unsigned short* as, unsigned short* bs);
+size_t sig_bf3_bytes_per_k(int n, int ncol, int nw1);
+size_t sig_bf3_a_elems_per_k(int n, int nw1);
+
 }  // namespace ecalj
