@@ -129,11 +129,27 @@ static void hilbert_coef(int nwhis, int niw, const double* frhis, const double* 
 
 template <class T>
 static ecalj_status upload(ecalj_ctx* h, int slot, const std::vector<T>& v, T** out) {
-  ECALJ_TRY(slot_ensure(h, slot, sizeof(T) * v.size(), (void**)out));
-  if (!v.empty()) {
-    ECALJ_HIP(hipMemcpyAsync(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, h->stream));
-    ECALJ_HIP(hipStreamSynchronize(h->stream));
+  const size_t bytes = sizeof(T) * v.size();
+  ECALJ_TRY(slot_ensure(h, slot, bytes, (void**)out));
+  if (v.empty()) return ECALJ_OK;
+  // stage through pinned memory so the copy is asynchronous; the host waits
+  // only for the previous copy out of the same staging buffer
+  if (h->pin_ev[slot]) ECALJ_HIP(hipEventSynchronize(h->pin_ev[slot]));
+  else ECALJ_HIP(hipEventCreateWithFlags(&h->pin_ev[slot], hipEventDisableTiming));
+  if (h->pin_bytes[slot] < bytes) {
+    if (h->pin[slot]) ECALJ_HIP(hipHostFree(h->pin[slot]));
+    h->pin[slot] = nullptr;
+    h->pin_bytes[slot] = 0;
+    const size_t want = bytes + bytes / 4;
+    if (hipHostMalloc(&h->pin[slot], want, hipHostMallocDefault) != hipSuccess) {
+      h->pin[slot] = nullptr;
+      return ECALJ_ERR_NOMEM;
+    }
+    h->pin_bytes[slot] = want;
   }
+  memcpy(h->pin[slot], v.data(), bytes);
+  ECALJ_HIP(hipMemcpyAsync(*out, h->pin[slot], bytes, hipMemcpyHostToDevice, h->stream));
+  ECALJ_HIP(hipEventRecord(h->pin_ev[slot], h->stream));
   return ECALJ_OK;
 }
 
@@ -215,8 +231,11 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
   if (!h) return ECALJ_ERR_ARG;
   (void)hipSetDevice(h->device);
   (void)hipStreamSynchronize(h->stream);
-  for (int s = 0; s < ecalj_ctx::S_NSLOT; ++s)
+  for (int s = 0; s < ecalj_ctx::S_NSLOT; ++s) {
     if (h->slot[s]) (void)hipFree(h->slot[s]);
+    if (h->pin[s]) (void)hipHostFree(h->pin[s]);
+    if (h->pin_ev[s]) (void)hipEventDestroy(h->pin_ev[s]);
+  }
   if (h->d_ppb) (void)hipFree(h->d_ppb);
   if (h->d_ppb2) (void)hipFree(h->d_ppb2);
   if (h->d_vcousq) (void)hipFree(h->d_vcousq);

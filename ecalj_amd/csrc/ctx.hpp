@@ -29,6 +29,11 @@ struct ecalj_ctx {
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag
+  // pinned host staging per slot for asynchronous uploads; the event marks
+  // the end of the last copy out of it
+  void* pin[S_NSLOT] = {};
+  size_t pin_bytes[S_NSLOT] = {};
+  hipEvent_t pin_ev[S_NSLOT] = {};
 
   // kernel timing
   bool timing = false;

@@ -537,23 +537,36 @@ __global__ __launch_bounds__(kThreads) void sig_freq_gemm_kernel(const T* __rest
   });
 }
 
-// zsec(m,m') += sum over partials in fixed order; then clip Im(zsec_mm) <= 0
+// zsec(m,m') += sum over partials (fixed-order tree, FP64); then clip Im(zsec_mm) <= 0.
+// One block per output element.
 template <class T>
-__global__ void sig_final_reduce_kernel(const T* __restrict__ part, int nparts, int ntq, cd* __restrict__ zsec,
-                                        int64_t ldz) {
+__global__ __launch_bounds__(256) void sig_final_reduce_kernel(const T* __restrict__ part, int nparts, int ntq,
+                                                               cd* __restrict__ zsec, int64_t ldz) {
+  __shared__ double sr[256], si[256];
   const int np = ntq * ntq;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-    double sr = 0.0, si = 0.0;
-    for (int k = 0; k < nparts; ++k) {
-      double xr, xi;
-      ld_c2(part + (int64_t)np * k + p, xr, xi);
-      sr += xr;
-      si += xi;
+  const int p = blockIdx.x;
+  double a = 0.0, c = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    double xr, xi;
+    ld_c2(part + (int64_t)np * k + p, xr, xi);
+    a += xr;
+    c += xi;
+  }
+  sr[threadIdx.x] = a;
+  si[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sr[threadIdx.x] += sr[threadIdx.x + w];
+      si[threadIdx.x] += si[threadIdx.x + w];
     }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
     const int m = p % ntq, mp = p / ntq;
     cd* z = zsec + m + ldz * mp;
     cd o = *z;
-    double zr = o.re + sr, zi = o.im + si;
+    double zr = o.re + sr[0], zi = o.im + si[0];
     if (m == mp && zi > 0.0) zi = 0.0;
     *z = cd{zr, zi};
   }
@@ -634,8 +647,8 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
       ECALJ_TRY(gemm(h, f));
     }
   }
-  hipLaunchKernelGGL((sig_final_reduce_kernel<T>), dim3((np + 255) / 256), dim3(256), 0, h->stream, part,
-                     chunks * nbatch, ntq, zsec, ldz);
+  hipLaunchKernelGGL((sig_final_reduce_kernel<T>), dim3(np), dim3(256), 0, h->stream, part, chunks * nbatch, ntq,
+                     zsec, ldz);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }

@@ -38,26 +38,37 @@ __global__ __launch_bounds__(256) void split_w_kernel(const cd* __restrict__ w, 
   __syncthreads();
   unsigned short* A = as + as_stride * b;
   const int64_t pstride = (int64_t)nw1 * MP * NP;
-  for (int r = ty; r < SPT; r += 8) {
-    const int I = I0 + r, J = J0 + tx;
-    double xr = 0.0, xi = 0.0;
-    if (I < n && J < n) {
-      if (I <= J) {
-        xr = up[tx][r].re;
-        xi = up[tx][r].im;
-      } else {
-        xr = lo[r][tx].re;
-        xi = -lo[r][tx].im;
+  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+  if (threadIdx.x < SPT * (SPT / 8)) {
+    const int r = threadIdx.x / (SPT / 8), jg = 8 * (threadIdx.x % (SPT / 8));
+    const int I = I0 + r;
+    us8v rh, rl, ih, il;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int jj = jg + e, J = J0 + jj;
+      double xr = 0.0, xi = 0.0;
+      if (I < n && J < n) {
+        if (I <= J) {
+          xr = up[jj][r].re;
+          xi = up[jj][r].im;
+        } else {
+          xr = lo[r][jj].re;
+          xi = -lo[r][jj].im;
+        }
       }
+      unsigned short h, l;
+      split_bf16((float)xr, h, l);
+      rh[e] = h;
+      rl[e] = l;
+      split_bf16((float)xi, h, l);
+      ih[e] = h;
+      il[e] = l;
     }
-    unsigned short h, l;
-    const int64_t o = ((int64_t)v * MP + I) * NP + J;
-    split_bf16((float)xr, h, l);
-    A[o] = h;
-    A[o + pstride] = l;
-    split_bf16((float)xi, h, l);
-    A[o + 2 * pstride] = h;
-    A[o + 3 * pstride] = l;
+    const int64_t o = ((int64_t)v * MP + I) * NP + J0 + jg;
+    *reinterpret_cast<us8v*>(A + o) = rh;
+    *reinterpret_cast<us8v*>(A + o + pstride) = rl;
+    *reinterpret_cast<us8v*>(A + o + 2 * pstride) = ih;
+    *reinterpret_cast<us8v*>(A + o + 3 * pstride) = il;
   }
 }
 
@@ -88,7 +99,7 @@ __global__ void split_z_kernel(const cf* __restrict__ z, int64_t z_stride, int n
 }
 
 // ---------------------------------------------------------------- main kernel
-constexpr int SBM = 128, SBN = 64, SBK = 32;
+constexpr int SBM = 128, SBN = 64, SBK = 64;
 
 __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
                                                               int64_t as_stride,
@@ -122,37 +133,41 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
     }
   core.zero();
 
-  // register staging: A 8 chunks, B 4 chunks of 16 bytes per thread
-  us8v ra[8], rb[4];
+  // register staging of one stage: A and B chunks of 16 bytes (8 bf16 of one row)
+  constexpr int PARTS = SBK / 8;                        // 16-byte chunks per row and plane
+  constexpr int NA = 4 * SBM * PARTS / kThreads;        // A chunks per thread
+  constexpr int NB = 4 * SBN * PARTS / kThreads;        // B chunks per thread
+  constexpr int APL = SBM * PARTS, BPL = SBN * PARTS;   // chunks per plane
+  us8v ra[NA], rb[NB];
   auto fetch = [&](int st) {
     const int v = st / NS, ks = st - v * NS;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < NA; ++q) {
       const int cid = t + kThreads * q;
-      const int p = cid >> 9, rem = cid & 511, row = rem >> 2, part = rem & 3;
+      const int p = cid / APL, rem = cid % APL, row = rem / PARTS, part = rem % PARTS;
       const unsigned short* src = A + p * apl + ((int64_t)v * MP + I0 + row) * NP + ks * SBK + 8 * part;
       ra[q] = *reinterpret_cast<const us8v*>(src);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NB; ++q) {
       const int cid = t + kThreads * q;
-      const int p = cid >> 8, rem = cid & 255, col = rem >> 2, part = rem & 3;
+      const int p = cid / BPL, rem = cid % BPL, col = rem / PARTS, part = rem % PARTS;
       const unsigned short* src = B + p * bpl + (int64_t)(c0 + col) * NP + ks * SBK + 8 * part;
       rb[q] = *reinterpret_cast<const us8v*>(src);
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < NA; ++q) {
       const int cid = t + kThreads * q;
-      const int p = cid >> 9, rem = cid & 511, row = rem >> 2, part = rem & 3;
+      const int p = cid / APL, rem = cid % APL, row = rem / PARTS, part = rem % PARTS;
       unsigned short* dst = (p == 0 ? s.arh : p == 1 ? s.arl : p == 2 ? s.aih : s.ail) + row * S::LDK + 8 * part;
       *reinterpret_cast<us8v*>(dst) = ra[q];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NB; ++q) {
       const int cid = t + kThreads * q;
-      const int p = cid >> 8, rem = cid & 255, col = rem >> 2, part = rem & 3;
+      const int p = cid / BPL, rem = cid % BPL, col = rem / PARTS, part = rem % PARTS;
       unsigned short* dst = (p == 0 ? s.brh : p == 1 ? s.brl : p == 2 ? s.bih : s.bil) + col * S::LDK + 8 * part;
       *reinterpret_cast<us8v*>(dst) = rb[q];
     }
