@@ -5,10 +5,15 @@
 //   convention, S/m_sxcf_gemm.f90:468-475, after the conjugate transpose of :492)
 //
 // The weight depends only on (v, c), so it is applied to the per-frequency
-// partial sums in FP32 instead of to every B element: zmel is split into
-// bf16 hi/lo planes once, W_v once, both zero-padded (rows to MP, k to NP,
-// columns to NCP), and the main loop only moves 16-byte chunks into LDS and
-// issues MFMAs.  Plane p: 0 = re hi, 1 = re lo, 2 = im hi, 3 = im lo.
+// partial sums in FP32 instead of to every B element.  W_v and zmel are split
+// once into bf16 hi/lo planes (plane p: 0 = re hi, 1 = re lo, 2 = im hi,
+// 3 = im lo) and written pre-tiled: for each 32-wide k slice ks the operand
+// rows of one plane are contiguous, 32 bf16 (64 bytes) per row, with the four
+// 16-byte chunks of row r stored in the order chunk ^ ((r >> 2) & 3).  A block
+// tile of one stage is then a few contiguous 4-8 KB runs that the main loop
+// moves global -> LDS with direct-to-LDS loads (lane i of a wave writes LDS
+// bytes 16i..16i+15 of a 1 KB run), three stages in flight, and the chunk
+// order makes the MFMA operand reads (16 rows x one chunk) bank-conflict free.
 #include <algorithm>
 
 #include "ctx.hpp"
@@ -17,15 +22,20 @@
 
 namespace ecalj {
 
+constexpr int QBM = 128, QBN = 64, QBK = 32, QST = 3;  // block tile, k per stage, stages in flight
+constexpr int QNW = 32;                                // frequencies per launch (weight table in LDS)
+
+__device__ __host__ inline int chunk_slot(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
+
 // ---------------------------------------------------------------- pre-split
-// As[b][((p*nw1 + v)*MP + I)*NP + J] from W (complex128, ldw, per-batch stride w_stride)
+// A[b][(((v*NS + ks)*4 + p)*MP + I)*32 + slot] from W (complex128, ldw, per-batch stride w_stride)
 constexpr int SPT = 32;
 __global__ __launch_bounds__(256) void split_w_kernel(const cd* __restrict__ w, int64_t ldw, int64_t w_stride,
-                                                      int n, int nw1, int MP, int NP,
+                                                      int n, int nw1, int MP, int NS,
                                                       unsigned short* __restrict__ as, int64_t as_stride) {
   __shared__ cd up[SPT][SPT + 1];  // up[jj][ii] = W(I0+ii, J0+jj)
   __shared__ cd lo[SPT][SPT + 1];  // lo[ii][jj] = W(J0+jj, I0+ii)
-  const int J0 = blockIdx.x * SPT, I0 = blockIdx.y * SPT;
+  const int ks = blockIdx.x, J0 = ks * SPT, I0 = blockIdx.y * SPT;
   const int v = blockIdx.z % nw1, b = blockIdx.z / nw1;
   const cd* W = w + w_stride * b + (int64_t)ldw * n * v;
   const int tx = threadIdx.x % SPT, ty = threadIdx.x / SPT;  // 32 x 8
@@ -36,16 +46,14 @@ __global__ __launch_bounds__(256) void split_w_kernel(const cd* __restrict__ w, 
     lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
   }
   __syncthreads();
-  unsigned short* A = as + as_stride * b;
-  const int64_t pstride = (int64_t)nw1 * MP * NP;
   typedef unsigned short us8v __attribute__((ext_vector_type(8)));
-  if (threadIdx.x < SPT * (SPT / 8)) {
-    const int r = threadIdx.x / (SPT / 8), jg = 8 * (threadIdx.x % (SPT / 8));
+  if (threadIdx.x < SPT * 4) {
+    const int r = threadIdx.x >> 2, ch = threadIdx.x & 3;
     const int I = I0 + r;
     us8v rh, rl, ih, il;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int jj = jg + e, J = J0 + jj;
+      const int jj = 8 * ch + e, J = J0 + jj;
       double xr = 0.0, xi = 0.0;
       if (I < n && J < n) {
         if (I <= J) {
@@ -64,136 +72,195 @@ __global__ __launch_bounds__(256) void split_w_kernel(const cd* __restrict__ w, 
       ih[e] = h;
       il[e] = l;
     }
-    const int64_t o = ((int64_t)v * MP + I) * NP + J0 + jg;
-    *reinterpret_cast<us8v*>(A + o) = rh;
-    *reinterpret_cast<us8v*>(A + o + pstride) = rl;
-    *reinterpret_cast<us8v*>(A + o + 2 * pstride) = ih;
-    *reinterpret_cast<us8v*>(A + o + 3 * pstride) = il;
+    const int64_t pst = (int64_t)MP * QBK;
+    unsigned short* A = as + as_stride * b + ((int64_t)v * NS + ks) * 4 * pst + (int64_t)I * QBK +
+                        8 * chunk_slot(I, ch);
+    *reinterpret_cast<us8v*>(A) = rh;
+    *reinterpret_cast<us8v*>(A + pst) = rl;
+    *reinterpret_cast<us8v*>(A + 2 * pst) = ih;
+    *reinterpret_cast<us8v*>(A + 3 * pst) = il;
   }
 }
 
-// Bs[b][(p*NCP + c)*NP + J] from zmel (complex64, n x ncol per batch)
-__global__ void split_z_kernel(const cf* __restrict__ z, int64_t z_stride, int n, int ncol, int NCP, int NP,
-                               unsigned short* __restrict__ bs, int64_t bs_stride, int nb) {
-  const int64_t per = (int64_t)NCP * NP, tot = per * nb;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
-       e += (int64_t)gridDim.x * blockDim.x) {
+// B[b][((ks*4 + p)*NCP + c)*32 + slot] from zmel (complex64, n x ncol per batch);
+// one thread per (b, ks, c, chunk)
+__global__ __launch_bounds__(256) void split_z_kernel(const cf* __restrict__ z, int64_t z_stride, int n,
+                                                      int ncol, int NCP, int NS, unsigned short* __restrict__ bs,
+                                                      int64_t bs_stride, int nb) {
+  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+  const int64_t per = (int64_t)NS * NCP * 4, tot = per * nb;
+  const int64_t pst = (int64_t)NCP * QBK;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / per);
     const int64_t r = e % per;
-    const int c = (int)(r / NP), J = (int)(r % NP);
-    float xr = 0.f, xi = 0.f;
-    if (c < ncol && J < n) {
-      const cf x = z[z_stride * b + J + (int64_t)n * c];
-      xr = x.re;
-      xi = x.im;
+    const int ks = (int)(r / (4 * NCP));
+    const int c = (int)((r / 4) % NCP), ch = (int)(r % 4);
+    us8v rh, rl, ih, il;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int J = ks * QBK + 8 * ch + q;
+      float xr = 0.f, xi = 0.f;
+      if (c < ncol && J < n) {
+        const cf x = z[z_stride * b + J + (int64_t)n * c];
+        xr = x.re;
+        xi = x.im;
+      }
+      unsigned short h, l;
+      split_bf16(xr, h, l);
+      rh[q] = h;
+      rl[q] = l;
+      split_bf16(xi, h, l);
+      ih[q] = h;
+      il[q] = l;
     }
-    unsigned short* B = bs + bs_stride * b;
-    unsigned short h, l;
-    split_bf16(xr, h, l);
-    B[(0 * (int64_t)NCP + c) * NP + J] = h;
-    B[(1 * (int64_t)NCP + c) * NP + J] = l;
-    split_bf16(xi, h, l);
-    B[(2 * (int64_t)NCP + c) * NP + J] = h;
-    B[(3 * (int64_t)NCP + c) * NP + J] = l;
+    unsigned short* B = bs + bs_stride * b + (int64_t)ks * 4 * pst + (int64_t)c * QBK + 8 * chunk_slot(c, ch);
+    *reinterpret_cast<us8v*>(B) = rh;
+    *reinterpret_cast<us8v*>(B + pst) = rl;
+    *reinterpret_cast<us8v*>(B + 2 * pst) = ih;
+    *reinterpret_cast<us8v*>(B + 3 * pst) = il;
   }
 }
 
 // ---------------------------------------------------------------- main kernel
-constexpr int SBM = 128, SBN = 64, SBK = 64;
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+typedef __attribute__((address_space(1))) void* gbl_ptr_t;
+typedef unsigned short us8q __attribute__((ext_vector_type(8)));
+
+struct alignas(16) SigSmem {
+  unsigned short a[QST][4][QBM * QBK];  // 8 KB per plane
+  unsigned short b[QST][4][QBN * QBK];  // 4 KB per plane
+  float wt[QNW][QBN];                   // per-frequency weights of the block's columns
+};
+
+// s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt[3:0] + vmcnt[5:4] at [15:14],
+// expcnt [6:4] and lgkmcnt [11:8] left at their maximum)
+#define ECALJ_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(((N) & 0xF) | (((N) >> 4) << 14) | (0x7 << 4) | (0xF << 8))
 
 __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
                                                               int64_t as_stride,
                                                               const unsigned short* __restrict__ bs,
                                                               int64_t bs_stride, const double* __restrict__ wgt,
-                                                              int64_t g_stride, int n, int nw1, int ncol, int MP,
-                                                              int NP, int NCP, cf* __restrict__ cz,
-                                                              int64_t cz_stride) {
-  typedef StageBF3<SBM, SBN, SBK> S;
-  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
-  __shared__ S s;
+                                                              int64_t g_stride, int n, int nw1, int v0, int nwg,
+                                                              int ncol, int MP, int NS, int NCP, int accumulate,
+                                                              cf* __restrict__ cz, int64_t cz_stride) {
+  __shared__ SigSmem s;
   const int b = blockIdx.z;
-  const int I0 = blockIdx.x * SBM, c0 = blockIdx.y * SBN;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, wm = wave >> 1, wn = wave & 1;
-  const unsigned short* A = as + as_stride * b;
-  const unsigned short* B = bs + bs_stride * b;
-  const int64_t apl = (int64_t)nw1 * MP * NP;  // A plane stride
-  const int64_t bpl = (int64_t)NCP * NP;       // B plane stride
-  const int NS = NP / SBK;                     // k-stages per frequency
-  const int nst = NS * nw1;
+  const int I0 = blockIdx.x * QBM, c0 = blockIdx.y * QBN;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave >> 1, wn = wave & 1;
+  const int64_t apst = (int64_t)MP * QBK, bpst = (int64_t)NCP * QBK;
+  const unsigned short* A = as + as_stride * b + (int64_t)I0 * QBK + lane * 8;
+  const unsigned short* B = bs + bs_stride * b + (int64_t)c0 * QBK + lane * 8;
+  const int nst = nwg * NS;
 
-  Core<MBF3, SBM, SBN, SBK, false> core;
-  constexpr int TM = SBM / 32, TN = SBN / 32;
-  f4 tr[TM][TN], ti[TM][TN];
+  // weight table: wt[v][c] = wgt(v0+v, c0+c) (0 beyond ncol)
+  const double* g = wgt + g_stride * b;
+  for (int e = t; e < nwg * QBN; e += kThreads) {
+    const int v = e / QBN, c = c0 + e % QBN;
+    s.wt[v][e % QBN] = (c < ncol) ? (float)g[(v0 + v) + (int64_t)nw1 * c] : 0.f;
+  }
+  __syncthreads();
+
+  // stage st -> buffer: 32 A runs + 16 B runs of 1 KB; wave w issues A runs 8w..8w+7, B runs 4w..4w+3
+  auto issue = [&](int st, int buf) {
+    const int v = v0 + st / NS, ks = st % NS;
+    const unsigned short* a = A + ((int64_t)v * NS + ks) * 4 * apst;
+    const unsigned short* bb = B + (int64_t)ks * 4 * bpst;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int run = 8 * wave + q, p = run >> 3, sub = run & 7;
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + p * apst + sub * 512), (lds_ptr_t)(&s.a[buf][p][sub * 512]),
+                                       16, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int run = 4 * wave + q, p = run >> 2, sub = run & 3;
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(bb + p * bpst + sub * 512), (lds_ptr_t)(&s.b[buf][p][sub * 512]),
+                                       16, 0, 0);
+    }
+  };
+
+  constexpr int TM = 4, TN = 2;  // wave tile 64 x 32
+  f4 cr[TM][TN], ci[TM][TN], tr[TM][TN], ti[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      tr[i][j] = f4{0, 0, 0, 0};
-      ti[i][j] = f4{0, 0, 0, 0};
+      cr[i][j] = ci[i][j] = tr[i][j] = ti[i][j] = f4{0, 0, 0, 0};
     }
-  core.zero();
+  // operand offsets in a plane: row r, logical chunk lane>>4, stored at chunk_slot(r, lane>>4);
+  // rows are 16-aligned tile rows + (lane & 15), so the slot depends on the lane only
+  const int slot = chunk_slot(lane & 15, lane >> 4);
+  const int aoff = (wm * TM * 16 + (lane & 15)) * QBK + 8 * slot;
+  const int boff = (wn * TN * 16 + (lane & 15)) * QBK + 8 * slot;
 
-  // register staging of one stage: A and B chunks of 16 bytes (8 bf16 of one row)
-  constexpr int PARTS = SBK / 8;                        // 16-byte chunks per row and plane
-  constexpr int NA = 4 * SBM * PARTS / kThreads;        // A chunks per thread
-  constexpr int NB = 4 * SBN * PARTS / kThreads;        // B chunks per thread
-  constexpr int APL = SBM * PARTS, BPL = SBN * PARTS;   // chunks per plane
-  us8v ra[NA], rb[NB];
-  auto fetch = [&](int st) {
-    const int v = st / NS, ks = st - v * NS;
-#pragma unroll
-    for (int q = 0; q < NA; ++q) {
-      const int cid = t + kThreads * q;
-      const int p = cid / APL, rem = cid % APL, row = rem / PARTS, part = rem % PARTS;
-      const unsigned short* src = A + p * apl + ((int64_t)v * MP + I0 + row) * NP + ks * SBK + 8 * part;
-      ra[q] = *reinterpret_cast<const us8v*>(src);
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const int cid = t + kThreads * q;
-      const int p = cid / BPL, rem = cid % BPL, col = rem / PARTS, part = rem % PARTS;
-      const unsigned short* src = B + p * bpl + (int64_t)(c0 + col) * NP + ks * SBK + 8 * part;
-      rb[q] = *reinterpret_cast<const us8v*>(src);
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int q = 0; q < NA; ++q) {
-      const int cid = t + kThreads * q;
-      const int p = cid / APL, rem = cid % APL, row = rem / PARTS, part = rem % PARTS;
-      unsigned short* dst = (p == 0 ? s.arh : p == 1 ? s.arl : p == 2 ? s.aih : s.ail) + row * S::LDK + 8 * part;
-      *reinterpret_cast<us8v*>(dst) = ra[q];
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const int cid = t + kThreads * q;
-      const int p = cid / BPL, rem = cid % BPL, col = rem / PARTS, part = rem % PARTS;
-      unsigned short* dst = (p == 0 ? s.brh : p == 1 ? s.brl : p == 2 ? s.bih : s.bil) + col * S::LDK + 8 * part;
-      *reinterpret_cast<us8v*>(dst) = rb[q];
-    }
-  };
-
-  const double* g = wgt + g_stride * b;
-  fetch(0);
+  issue(0, 0);
+  if (nst > 1) issue(1, 1);
   for (int st = 0; st < nst; ++st) {
-    __syncthreads();
-    store();
-    __syncthreads();
-    if (st + 1 < nst) fetch(st + 1);
-    core.compute(s, wm, wn, lane);
-    const int v = st / NS;
-    if (st - v * NS == NS - 1) {  // end of frequency v: tot += wgt(v,c) * acc
+    const int buf = st % QST;
+    // stage st landed: at most the 12 loads of stage st+1 may still be in flight
+    if (st + 1 < nst) {
+      ECALJ_WAIT_VMCNT(12);
+    } else {
+      ECALJ_WAIT_VMCNT(0);
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // the buffer of stage st+2 was last read in iteration st-1, which every wave has left
+    if (st + 2 < nst) issue(st + 2, (st + 2) % QST);
+
+    us8q arh[TM], arl[TM], aih[TM], ail[TM], brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int o = boff + j * 16 * QBK;
+      brh[j] = *reinterpret_cast<const us8q*>(&s.b[buf][0][o]);
+      brl[j] = *reinterpret_cast<const us8q*>(&s.b[buf][1][o]);
+      bih[j] = *reinterpret_cast<const us8q*>(&s.b[buf][2][o]);
+      bil[j] = *reinterpret_cast<const us8q*>(&s.b[buf][3][o]);
+      nbh[j] = bih[j] ^ (unsigned short)0x8000;
+      nbl[j] = bil[j] ^ (unsigned short)0x8000;
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int o = aoff + i * 16 * QBK;
+      arh[i] = *reinterpret_cast<const us8q*>(&s.a[buf][0][o]);
+      arl[i] = *reinterpret_cast<const us8q*>(&s.a[buf][1][o]);
+      aih[i] = *reinterpret_cast<const us8q*>(&s.a[buf][2][o]);
+      ail[i] = *reinterpret_cast<const us8q*>(&s.a[buf][3][o]);
+    }
+#define ECALJ_Q_TERM(ACC, AF, BF)                                                                           \
+  _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < TN; ++j) ACC[i][j] = \
+      __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, AF[i]), __builtin_bit_cast(bf8, BF[j]), \
+                                              ACC[i][j], 0, 0, 0);
+    // re = ar br - ai bi, im = ar bi + ai br; hi*hi + hi*lo + lo*hi of each
+    ECALJ_Q_TERM(cr, arh, brh)
+    ECALJ_Q_TERM(ci, arh, bih)
+    ECALJ_Q_TERM(cr, arh, brl)
+    ECALJ_Q_TERM(ci, arh, bil)
+    ECALJ_Q_TERM(cr, arl, brh)
+    ECALJ_Q_TERM(ci, arl, bih)
+    ECALJ_Q_TERM(cr, aih, nbh)
+    ECALJ_Q_TERM(ci, aih, brh)
+    ECALJ_Q_TERM(cr, aih, nbl)
+    ECALJ_Q_TERM(ci, aih, brl)
+    ECALJ_Q_TERM(cr, ail, nbh)
+    ECALJ_Q_TERM(ci, ail, brh)
+#undef ECALJ_Q_TERM
+
+    const int vl = st / NS;
+    if (st - vl * NS == NS - 1) {  // end of frequency: tot += wgt(v,c) * acc
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const int c = c0 + wn * TN * 16 + j * 16 + (lane & 15);
-        const float wv = (c < ncol) ? (float)g[v + (int64_t)nw1 * c] : 0.f;
+        const float wv = s.wt[vl][wn * TN * 16 + j * 16 + (lane & 15)];
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
-          tr[i][j] += wv * core.cr[i][j];
-          ti[i][j] += wv * core.ci[i][j];
+          tr[i][j] += wv * cr[i][j];
+          ti[i][j] += wv * ci[i][j];
+          cr[i][j] = f4{0, 0, 0, 0};
+          ci[i][j] = f4{0, 0, 0, 0};
         }
       }
-      core.zero();
     }
   }
   cf* C = cz + cz_stride * b;
@@ -205,49 +272,57 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
       for (int e = 0; e < 4; ++e) {
         const int I = I0 + wm * TM * 16 + i * 16 + acc_row_f32(lane, e);
         const int c = c0 + wn * TN * 16 + j * 16 + (lane & 15);
-        if (I < n && c < ncol) C[I + (int64_t)n * c] = cf{tr[i][j][e], ti[i][j][e]};
+        if (I < n && c < ncol) {
+          cf* p = C + I + (int64_t)n * c;
+          if (accumulate) {
+            const cf o = *p;
+            *p = cf{o.re + tr[i][j][e], o.im + ti[i][j][e]};
+          } else {
+            *p = cf{tr[i][j][e], ti[i][j][e]};
+          }
+        }
       }
 }
 
 // ---------------------------------------------------------------- launcher
+static inline int pad_to(int x, int m) { return (x + m - 1) / m * m; }
+
 // Computes czmelwc for nb internal k-points; returns ECALJ_OK or a status.
 ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1,
                      const cd* wc, int64_t ldw, int64_t w_stride, const double* wgt, int64_t g_stride, cf* cz,
                      int64_t cz_stride, unsigned short* as, unsigned short* bs) {
-  const int MP = (n + SBM - 1) / SBM * SBM;
-  const int NP = (n + SBK - 1) / SBK * SBK;
-  const int NCP = (ncol + SBN - 1) / SBN * SBN;
+  if (n < 1 || ncol < 1 || nw1 < 1 || nb < 1) return ECALJ_ERR_DIM;
+  const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NCP = pad_to(ncol, QBN), NS = NP / QBK;
   const int64_t as_stride = 4LL * nw1 * MP * NP;
   const int64_t bs_stride = 4LL * NCP * NP;
-  // W planes (rows beyond n come out zero: the split kernel covers all of MP x NP)
-  dim3 gw(NP / SPT, MP / SPT, nw1 * nb);
-  if (gw.z > 65535) return ECALJ_ERR_DIM;
-  hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NP, as, as_stride);
+  // W planes: the split grid covers all MP x NP, so padding comes out zero
+  dim3 gw(NS, MP / SPT, nw1 * nb);
+  if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
+  hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
   ECALJ_HIP(hipGetLastError());
-  const int64_t totz = (int64_t)NCP * NP * nb;
+  const int64_t totz = (int64_t)NS * NCP * 4 * nb;
   const int bz = (int)std::min<int64_t>((totz + 255) / 256, 65536);
-  hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel, z_stride, n, ncol, NCP, NP, bs,
+  hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel, z_stride, n, ncol, NCP, NS, bs,
                      bs_stride, nb);
   ECALJ_HIP(hipGetLastError());
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 0, &tev));
-  dim3 grid(MP / SBM, NCP / SBN, nb);
+  dim3 grid(MP / QBM, NCP / QBN, nb);
   if (grid.y > 65535 || grid.z > 65535) return ECALJ_ERR_DIM;
-  hipLaunchKernelGGL(sig_bf3_kernel, grid, dim3(kThreads), 0, h->stream, as, as_stride, bs, bs_stride, wgt, g_stride,
-                     n, nw1, ncol, MP, NP, NCP, cz, cz_stride);
-  ECALJ_HIP(hipGetLastError());
+  for (int v0 = 0; v0 < nw1; v0 += QNW) {
+    const int nwg = std::min(QNW, nw1 - v0);
+    hipLaunchKernelGGL(sig_bf3_kernel, grid, dim3(kThreads), 0, h->stream, as, as_stride, bs, bs_stride, wgt,
+                       g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride);
+    ECALJ_HIP(hipGetLastError());
+  }
   ECALJ_TRY(timing_end(h, 0, tev));
   return ECALJ_OK;
 }
 
-size_t sig_bf3_a_elems_per_k(int n, int nw1) {
-  const size_t MP = (n + SBM - 1) / SBM * SBM, NP = (n + SBK - 1) / SBK * SBK;
-  return 4 * (size_t)nw1 * MP * NP;
-}
+size_t sig_bf3_a_elems_per_k(int n, int nw1) { return 4 * (size_t)nw1 * pad_to(n, QBM) * pad_to(n, QBK); }
 
 size_t sig_bf3_bytes_per_k(int n, int ncol, int nw1) {
-  const size_t MP = (n + SBM - 1) / SBM * SBM, NP = (n + SBK - 1) / SBK * SBK, NCP = (ncol + SBN - 1) / SBN * SBN;
-  return 2 * 4 * ((size_t)nw1 * MP * NP + NCP * NP);
+  return 2 * (sig_bf3_a_elems_per_k(n, nw1) + 4 * (size_t)pad_to(ncol, QBN) * pad_to(n, QBK));
 }
 
 }  // namespace ecalj

@@ -77,7 +77,9 @@ def test_x0gemm_mixed(hp, orc):
     assert relerr(g, o) < TOL[prec]
 
 
-@pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10)])
+# (ngb, nm, ntq, niw): ragged sizes, tile-aligned sizes, and more than 32
+# frequencies (the bf16x3 kernel then accumulates over two launches)
+@pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10), (161, 13, 5, 40), (1, 3, 1, 2)])
 def test_sigc_mixed(hp, orc, ngb, nm, ntq, niw):
     h, prec = hp
     say("sigc", prec, ngb, nm, ntq, niw)
