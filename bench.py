@@ -87,6 +87,9 @@ def run_form(qp, precision, steps, warmup):
     g.step()
     g.make_wk()
     g.h.sync()
+    tuning = g.h.tuning_report().strip()
+    if tuning:
+        log(f"bench[{precision}]: tuner: {tuning}")
     for i in range(warmup):
         g.step()
         g.h.sync()
@@ -102,7 +105,7 @@ def run_form(qp, precision, steps, warmup):
     g.h.kernel_timing(False)
     _, _, _, eqp = g.results()
     log(f"bench[{precision}]: {steps} steps in {dt:.3f} s -> {dt / steps * 1e3:.1f} ms per q-point")
-    info = {"s_per_q": dt / steps, "ktimes": ktimes, "eqp": eqp, "nwhis": g.nwhis,
+    info = {"s_per_q": dt / steps, "ktimes": ktimes, "eqp": eqp, "nwhis": g.nwhis, "tuning": tuning,
             "npb": int(sum(g.ix.iwend - g.ix.iwini + 1))}
     g.h.close()
     del g
@@ -185,6 +188,7 @@ def main():
         "flops_per_q": ftot,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in mix["ktimes"].items()},
         "roofline": roofline(fl, mix, args.mixed, args.steps),
+        "tuning": mix["tuning"],
     }
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on a bounded sample")

@@ -42,6 +42,8 @@ _SIGS = {
     "ecalj_sync": [_p],
     "ecalj_kernel_timing": [_p, _i],
     "ecalj_kernel_time": [_p, _i, C.POINTER(_d), C.POINTER(_l)],
+    "ecalj_set_tuning": [_p, _i],
+    "ecalj_tuning_report": [_p, C.c_char_p, _l],
     "ecalj_set_ppb": [_p, _i, _p, _p, _p],
     "ecalj_set_coulomb": [_p, _i, _p, _p],
     "ecalj_zmel": [_p, _p, _i, _i, _p, _i, _p, _i, _i, _i, _p],
@@ -216,6 +218,15 @@ class Handle:
         n = C.c_int64()
         _chk(lib().ecalj_kernel_time(self._h, KCLASS[kclass], C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
+
+    def set_tuning(self, mode: int):
+        """0: measure kernel variants on the first call of a shape; k > 0: force variant k."""
+        _chk(lib().ecalj_set_tuning(self._h, int(mode)), "set_tuning")
+
+    def tuning_report(self) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        _chk(lib().ecalj_tuning_report(self._h, buf, len(buf)), "tuning_report")
+        return buf.value.decode()
 
     def set_ppb(self, nv, mdim, ppb):
         """set_ppb (S/m_zmel.f90:110-145): per-atom real blocks nv x nv x mdim."""

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -12,6 +13,25 @@
 using ecalj::cd;
 
 namespace ecalj {
+
+bool tune_lookup(ecalj_ctx* h, const TuneKey& k, int* var) {
+  auto it = h->tuned.find(k);
+  if (it == h->tuned.end()) return false;
+  *var = it->second;
+  return true;
+}
+
+void tune_store(ecalj_ctx* h, const TuneKey& k, int var, const char* op, const std::vector<const char*>& names,
+                const std::vector<double>& ms) {
+  h->tuned[k] = var;
+  char line[512];
+  int p = snprintf(line, sizeof line, "%s [%lld %lld %lld %lld]:", op, (long long)k[1], (long long)k[2],
+                   (long long)k[3], (long long)k[4]);
+  for (size_t v = 0; v < names.size() && p < (int)sizeof line; ++v)
+    p += snprintf(line + p, sizeof line - p, " %s %.3f ms%s", names[v], ms[v], (int)v == var ? " *" : "");
+  h->tune_log += line;
+  h->tune_log += "\n";
+}
 
 ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
@@ -249,6 +269,20 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
     }
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_set_tuning(ecalj_handle h, int mode) {
+  if (!h || mode < 0) return ECALJ_ERR_ARG;
+  h->tune_mode = mode;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_tuning_report(ecalj_handle h, char* buf, int64_t len) {
+  if (!h || !buf || len < 1) return ECALJ_ERR_ARG;
+  const size_t n = std::min<size_t>(h->tune_log.size(), (size_t)len - 1);
+  memcpy(buf, h->tune_log.data(), n);
+  buf[n] = 0;
   return ECALJ_OK;
 }
 

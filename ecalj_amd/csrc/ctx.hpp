@@ -1,5 +1,8 @@
 // ctx.hpp -- handle state and host-side launch helpers.
 #pragma once
+#include <array>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -35,6 +38,12 @@ struct ecalj_ctx {
   size_t pin_bytes[S_NSLOT] = {};
   hipEvent_t pin_ev[S_NSLOT] = {};
 
+  // kernel-variant tuner: 0 = measure each variant on the first call of a shape and
+  // keep the fastest; k > 0 = always variant k (1-based)
+  int tune_mode = 0;
+  std::map<std::array<int64_t, 5>, int> tuned;
+  std::string tune_log;
+
   // kernel timing
   bool timing = false;
   struct EvPair { hipEvent_t a, b; };
@@ -45,6 +54,34 @@ struct ecalj_ctx {
 namespace ecalj {
 
 ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out);
+
+// ---- kernel-variant tuner
+enum TuneOp { TUNE_SIGC_BF3 = 0 };
+typedef std::array<int64_t, 5> TuneKey;
+bool tune_lookup(ecalj_ctx* h, const TuneKey& k, int* var);
+void tune_store(ecalj_ctx* h, const TuneKey& k, int var, const char* op, const std::vector<const char*>& names,
+                const std::vector<double>& ms);
+// Times f() (a launch sequence on h->stream) with HIP events; waits for it.
+template <class F>
+ecalj_status tune_time(ecalj_ctx* h, F&& f, double* ms) {
+  hipEvent_t a, b;
+  ECALJ_HIP(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return ECALJ_ERR_HIP;
+  }
+  ecalj_status st = ECALJ_OK;
+  if (hipEventRecord(a, h->stream) != hipSuccess) st = ECALJ_ERR_HIP;
+  if (st == ECALJ_OK) st = f();
+  if (st == ECALJ_OK && hipEventRecord(b, h->stream) != hipSuccess) st = ECALJ_ERR_HIP;
+  if (st == ECALJ_OK && hipEventSynchronize(b) != hipSuccess) st = ECALJ_ERR_HIP;
+  float t = 0.f;
+  if (st == ECALJ_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) st = ECALJ_ERR_HIP;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = t;
+  return st;
+}
 // Event bracketing for the kernel classes of ecalj_kernel_timing.
 ecalj_status timing_begin(ecalj_ctx* h, int kclass, int* idx);
 ecalj_status timing_end(ecalj_ctx* h, int kclass, int idx);

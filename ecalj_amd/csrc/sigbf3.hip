@@ -22,7 +22,7 @@
 
 namespace ecalj {
 
-constexpr int QBM = 128, QBN = 64, QBK = 32, QST = 3;  // block tile, k per stage, stages in flight
+constexpr int QBM = 128, QBK = 32;  // block rows, k per stage
 constexpr int QNW = 32;                                // frequencies per launch (weight table in LDS)
 
 __device__ __host__ inline int chunk_slot(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
@@ -126,16 +126,20 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 typedef __attribute__((address_space(1))) void* gbl_ptr_t;
 typedef unsigned short us8q __attribute__((ext_vector_type(8)));
 
+template <int BN, int NSTG>
 struct alignas(16) SigSmem {
-  unsigned short a[QST][4][QBM * QBK];  // 8 KB per plane
-  unsigned short b[QST][4][QBN * QBK];  // 4 KB per plane
-  float wt[QNW][QBN];                   // per-frequency weights of the block's columns
+  unsigned short a[NSTG][4][QBM * QBK];  // 8 KB per plane
+  unsigned short b[NSTG][4][BN * QBK];   // BN/16 KB per plane
+  float wt[QNW][BN];                     // per-frequency weights of the block's columns
 };
 
 // s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt[3:0] + vmcnt[5:4] at [15:14],
 // expcnt [6:4] and lgkmcnt [11:8] left at their maximum)
 #define ECALJ_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(((N) & 0xF) | (((N) >> 4) << 14) | (0x7 << 4) | (0xF << 8))
 
+// Block tile QBM x BN, four waves in 2 x 2, wave tile 64 x BN/2; NSTG stages of
+// QBK in LDS, NSTG-1 of them in flight.
+template <int BN, int NSTG>
 __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
                                                               int64_t as_stride,
                                                               const unsigned short* __restrict__ bs,
@@ -143,9 +147,14 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
                                                               int64_t g_stride, int n, int nw1, int v0, int nwg,
                                                               int ncol, int MP, int NS, int NCP, int accumulate,
                                                               cf* __restrict__ cz, int64_t cz_stride) {
-  __shared__ SigSmem s;
+  static_assert(NSTG == 2 || NSTG == 3, "two or three stages");
+  constexpr int TM = 4, TN = BN / 32;               // 16x16 tiles per wave
+  constexpr int ARW = 4 * QBM * QBK * 2 / 1024 / 4;  // 1 KB A runs per wave and stage
+  constexpr int BRW = 4 * BN * QBK * 2 / 1024 / 4;   // 1 KB B runs per wave and stage
+  constexpr int APP = QBM * QBK / 512, BPP = BN * QBK / 512;  // runs per plane
+  __shared__ SigSmem<BN, NSTG> s;
   const int b = blockIdx.z;
-  const int I0 = blockIdx.x * QBM, c0 = blockIdx.y * QBN;
+  const int I0 = blockIdx.x * QBM, c0 = blockIdx.y * BN;
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave >> 1, wn = wave & 1;
   const int64_t apst = (int64_t)MP * QBK, bpst = (int64_t)NCP * QBK;
@@ -155,32 +164,30 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
 
   // weight table: wt[v][c] = wgt(v0+v, c0+c) (0 beyond ncol)
   const double* g = wgt + g_stride * b;
-  for (int e = t; e < nwg * QBN; e += kThreads) {
-    const int v = e / QBN, c = c0 + e % QBN;
-    s.wt[v][e % QBN] = (c < ncol) ? (float)g[(v0 + v) + (int64_t)nw1 * c] : 0.f;
+  for (int e = t; e < nwg * BN; e += kThreads) {
+    const int v = e / BN, c = c0 + e % BN;
+    s.wt[v][e % BN] = (c < ncol) ? (float)g[(v0 + v) + (int64_t)nw1 * c] : 0.f;
   }
   __syncthreads();
 
-  // stage st -> buffer: 32 A runs + 16 B runs of 1 KB; wave w issues A runs 8w..8w+7, B runs 4w..4w+3
   auto issue = [&](int st, int buf) {
     const int v = v0 + st / NS, ks = st % NS;
     const unsigned short* a = A + ((int64_t)v * NS + ks) * 4 * apst;
     const unsigned short* bb = B + (int64_t)ks * 4 * bpst;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int run = 8 * wave + q, p = run >> 3, sub = run & 7;
+    for (int q = 0; q < ARW; ++q) {
+      const int run = ARW * wave + q, p = run / APP, sub = run % APP;
       __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + p * apst + sub * 512), (lds_ptr_t)(&s.a[buf][p][sub * 512]),
                                        16, 0, 0);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int run = 4 * wave + q, p = run >> 2, sub = run & 3;
+    for (int q = 0; q < BRW; ++q) {
+      const int run = BRW * wave + q, p = run / BPP, sub = run % BPP;
       __builtin_amdgcn_global_load_lds((gbl_ptr_t)(bb + p * bpst + sub * 512), (lds_ptr_t)(&s.b[buf][p][sub * 512]),
                                        16, 0, 0);
     }
   };
 
-  constexpr int TM = 4, TN = 2;  // wave tile 64 x 32
   f4 cr[TM][TN], ci[TM][TN], tr[TM][TN], ti[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -194,21 +201,21 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
   const int aoff = (wm * TM * 16 + (lane & 15)) * QBK + 8 * slot;
   const int boff = (wn * TN * 16 + (lane & 15)) * QBK + 8 * slot;
 
-  issue(0, 0);
-  if (nst > 1) issue(1, 1);
+  for (int q = 0; q < NSTG - 1; ++q)
+    if (q < nst) issue(q, q);
   for (int st = 0; st < nst; ++st) {
-    const int buf = st % QST;
-    // stage st landed: at most the 12 loads of stage st+1 may still be in flight
-    if (st + 1 < nst) {
-      ECALJ_WAIT_VMCNT(12);
+    const int buf = st % NSTG;
+    // stage st landed: the loads of later stages already issued may still be in flight
+    if (NSTG == 3 && st + 1 < nst) {
+      ECALJ_WAIT_VMCNT(ARW + BRW);
     } else {
       ECALJ_WAIT_VMCNT(0);
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_barrier();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    // the buffer of stage st+2 was last read in iteration st-1, which every wave has left
-    if (st + 2 < nst) issue(st + 2, (st + 2) % QST);
+    // the buffer of stage st+NSTG-1 was last read in iteration st-1, which every wave has left
+    if (st + NSTG - 1 < nst) issue(st + NSTG - 1, (st + NSTG - 1) % NSTG);
 
     us8q arh[TM], arl[TM], aih[TM], ail[TM], brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
 #pragma unroll
@@ -286,13 +293,34 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
 
 // ---------------------------------------------------------------- launcher
 static inline int pad_to(int x, int m) { return (x + m - 1) / m * m; }
+constexpr int QBN_MAX = 128;  // columns are padded for the widest variant
+
+// Kernel variants of the tuner: {column tile, stages}
+struct SigVariant { int bn, nstg; const char* name; };
+static const SigVariant kSigVariants[] = {{64, 3, "128x64x3"}, {128, 2, "128x128x2"}};
+constexpr int kNumSigVariants = 2;
+
+template <int BN, int NSTG>
+static ecalj_status sig_launch_t(ecalj_ctx* h, int nb, int n, int ncol, int nw1, int MP, int NS, int NCP,
+                                 const unsigned short* as, int64_t as_stride, const unsigned short* bs,
+                                 int64_t bs_stride, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride) {
+  dim3 grid(MP / QBM, NCP / BN, nb);
+  if (grid.y > 65535 || grid.z > 65535) return ECALJ_ERR_DIM;
+  for (int v0 = 0; v0 < nw1; v0 += QNW) {
+    const int nwg = std::min(QNW, nw1 - v0);
+    hipLaunchKernelGGL((sig_bf3_kernel<BN, NSTG>), grid, dim3(kThreads), 0, h->stream, as, as_stride, bs, bs_stride,
+                       wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride);
+    ECALJ_HIP(hipGetLastError());
+  }
+  return ECALJ_OK;
+}
 
 // Computes czmelwc for nb internal k-points; returns ECALJ_OK or a status.
 ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1,
                      const cd* wc, int64_t ldw, int64_t w_stride, const double* wgt, int64_t g_stride, cf* cz,
                      int64_t cz_stride, unsigned short* as, unsigned short* bs) {
   if (n < 1 || ncol < 1 || nw1 < 1 || nb < 1) return ECALJ_ERR_DIM;
-  const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NCP = pad_to(ncol, QBN), NS = NP / QBK;
+  const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NCP = pad_to(ncol, QBN_MAX), NS = NP / QBK;
   const int64_t as_stride = 4LL * nw1 * MP * NP;
   const int64_t bs_stride = 4LL * NCP * NP;
   // W planes: the split grid covers all MP x NP, so padding comes out zero
@@ -305,16 +333,34 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
   hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel, z_stride, n, ncol, NCP, NS, bs,
                      bs_stride, nb);
   ECALJ_HIP(hipGetLastError());
+  auto run = [&](int var) -> ecalj_status {
+    switch (var) {
+      case 0: return sig_launch_t<64, 3>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt,
+                                         g_stride, cz, cz_stride);
+      default: return sig_launch_t<128, 2>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt,
+                                           g_stride, cz, cz_stride);
+    }
+  };
+  // variant: forced by the handle, cached for this shape, or measured now
+  // (every variant computes the same sums in the same order, so whichever ran last leaves the result)
+  int var = -1;
+  const TuneKey key{TUNE_SIGC_BF3, n, ncol, nw1, nb};
+  if (h->tune_mode > 0) {
+    var = std::min(h->tune_mode, kNumSigVariants) - 1;
+  } else if (!tune_lookup(h, key, &var)) {
+    std::vector<double> ms(kNumSigVariants);
+    for (int v = 0; v < kNumSigVariants; ++v) {
+      ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
+    }
+    var = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    std::vector<const char*> names;
+    for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
+    tune_store(h, key, var, "sigc_bf3", names, ms);
+    return ECALJ_OK;
+  }
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 0, &tev));
-  dim3 grid(MP / QBM, NCP / QBN, nb);
-  if (grid.y > 65535 || grid.z > 65535) return ECALJ_ERR_DIM;
-  for (int v0 = 0; v0 < nw1; v0 += QNW) {
-    const int nwg = std::min(QNW, nw1 - v0);
-    hipLaunchKernelGGL(sig_bf3_kernel, grid, dim3(kThreads), 0, h->stream, as, as_stride, bs, bs_stride, wgt,
-                       g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride);
-    ECALJ_HIP(hipGetLastError());
-  }
+  ECALJ_TRY(run(var));
   ECALJ_TRY(timing_end(h, 0, tev));
   return ECALJ_OK;
 }
@@ -322,7 +368,7 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
 size_t sig_bf3_a_elems_per_k(int n, int nw1) { return 4 * (size_t)nw1 * pad_to(n, QBM) * pad_to(n, QBK); }
 
 size_t sig_bf3_bytes_per_k(int n, int ncol, int nw1) {
-  return 2 * (sig_bf3_a_elems_per_k(n, nw1) + 4 * (size_t)pad_to(ncol, QBN) * pad_to(n, QBK));
+  return 2 * (sig_bf3_a_elems_per_k(n, nw1) + 4 * (size_t)pad_to(ncol, QBN_MAX) * pad_to(n, QBK));
 }
 
 }  // namespace ecalj

@@ -63,6 +63,16 @@ const char* ecalj_status_string(int status);
  * 2 = zmel GEMMs, 3 = W build (transforms + inversion), 4 = Hilbert. */
 ecalj_status ecalj_kernel_timing(ecalj_handle h, int enable);
 ecalj_status ecalj_kernel_time(ecalj_handle h, int kclass, double* ms, int64_t* launches);
+/* Kernel-variant tuner (no counterpart in the reference, whose GEMMs have one
+ * fixed shape each).  mode 0 (default): on the first call of a shape, every
+ * variant of a tuned kernel runs once, timed with HIP events (that call
+ * waits for the stream), and the fastest is kept for the shape; mode k > 0:
+ * always variant k.  Tuned kernels: the bf16x3 Sigma_c contraction (variants
+ * 1 = 128x64 tiles, 3 stages; 2 = 128x128 tiles, 2 stages).
+ * ecalj_tuning_report writes the decisions so far, one line per shape, as a
+ * NUL-terminated string of at most len bytes. */
+ecalj_status ecalj_set_tuning(ecalj_handle h, int mode);
+ecalj_status ecalj_tuning_report(ecalj_handle h, char* buf, int64_t len);
 
 /* ---- device memory ------------------------------------------------------
  * The reference keeps its arrays on the device with OpenACC data directives

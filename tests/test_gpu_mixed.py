@@ -90,6 +90,37 @@ def test_sigc_mixed(hp, orc, ngb, nm, ntq, niw):
     assert relerr(h.sigc(z, wc, g), orc.sigc_imag(z, wc, g)) < TOL[prec]
 
 
+def test_sigc_bf3_tuner_variants(orc):
+    """Every kernel variant of the tuner gives the oracle's Sigma_c, bit-identical
+    to each other (same sums in the same order); auto mode records its choice."""
+    import ecalj_amd
+
+    rng = np.random.default_rng(7)
+    ngb, nm, ntq, niw = 203, 19, 9, 11
+    z = crand(rng, ngb, nm, ntq)
+    wc = crand(rng, ngb, ngb, niw + 1)
+    g = np.asfortranarray(rng.standard_normal((niw + 1, nm, ntq)))
+    ref = orc.sigc_imag(z, wc, g)
+    h = ecalj_amd.Handle(0, "bf16x3")
+    try:
+        outs = []
+        for mode in (1, 2):
+            say("sigc bf16x3 tuner variant", mode)
+            h.set_tuning(mode)
+            outs.append(h.sigc(z, wc, g))
+            assert relerr(outs[-1], ref) < TOL["bf16x3"]
+        assert np.array_equal(outs[0], outs[1])
+        h.set_tuning(0)
+        say("sigc bf16x3 tuner auto")
+        a = h.sigc(z, wc, g)
+        b = h.sigc(z, wc, g)  # cached choice
+        assert np.array_equal(a, outs[0]) and np.array_equal(b, outs[0])
+        rep = h.tuning_report()
+        assert rep.count("sigc_bf3") == 1 and "*" in rep
+    finally:
+        h.close()
+
+
 @pytest.mark.parametrize("name", ["T3", "C1"])
 @pytest.mark.parametrize("prec", ["f32", "bf16x3"])
 def test_qpoint_mixed_qp_energy_within_1meV(orc, name, prec):
