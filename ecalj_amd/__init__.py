@@ -71,6 +71,14 @@ _SIGS = {
     "ecalj_dev_zero": [_p, _p, C.c_size_t],
     "ecalj_dev_phase_similarity": [_p, _p, _i, _i, _p, _i, _p],
     "ecalj_sigma_herm": [_i, _p, _p, _p, _p, _p],
+    "ecalj_mlp_nparam": [_i, _p, C.POINTER(_l)],
+    "ecalj_mlp_forward_dev": [_p, _i, _p, _p, _p, _i, _p],
+    "ecalj_mlp_grad_dev": [_p, _i, _p, _p, _p, _p, _i, _p, _p],
+    "ecalj_mlp_adam_dev": [_p, _l, _p, _p, _p, _p, _i, _d, _d, _d, _d],
+    "ecalj_mlp_to_bf16_dev": [_p, _l, _p, _p, _p],
+    "ecalj_mlp_infer_bf16_dev": [_p, _i, _p, _p, _p, _p, _p, _i, _p],
+    "ecalj_tail_features_dev": [_p, _p, _i, _i, _i, _p, _p, _p, _p],
+    "ecalj_tail_apply_dev": [_p, _p, _i, _i, _i, _p, _p, _p, _p, _p],
 }
 
 
@@ -89,6 +97,13 @@ def lib():
         L.ecalj_status_string.restype = C.c_char_p
         _LIB = L
     return _LIB
+
+
+def mlp_nparam(dims) -> int:
+    d = _i32(dims)
+    n = C.c_int64()
+    _chk(lib().ecalj_mlp_nparam(len(d) - 1, _np_ptr(d), C.byref(n)), "mlp_nparam")
+    return n.value
 
 
 def exported_symbols():
@@ -227,6 +242,38 @@ class Handle:
         buf = C.create_string_buffer(1 << 16)
         _chk(lib().ecalj_tuning_report(self._h, buf, len(buf)), "tuning_report")
         return buf.value.decode()
+
+    # ---- learned chi0 tail (no reference counterpart; see include/ecalj_gpu.h)
+    # MLP batches are row-major (batch x width): a DevArray of Fortran shape (width, batch).
+    def mlp_forward_dev(self, dims, params, x, batch, y):
+        d = _i32(dims)
+        _chk(lib().ecalj_mlp_forward_dev(self._h, len(d) - 1, _np_ptr(d), _t_ptr(params), _t_ptr(x), batch,
+                                         _t_ptr(y)), "mlp_forward_dev")
+
+    def mlp_grad_dev(self, dims, params, x, target, batch, grad, loss=None):
+        d = _i32(dims)
+        _chk(lib().ecalj_mlp_grad_dev(self._h, len(d) - 1, _np_ptr(d), _t_ptr(params), _t_ptr(x), _t_ptr(target),
+                                      batch, _t_ptr(grad), _t_ptr(loss)), "mlp_grad_dev")
+
+    def mlp_adam_dev(self, nparam, params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        _chk(lib().ecalj_mlp_adam_dev(self._h, nparam, _t_ptr(params), _t_ptr(grad), _t_ptr(m), _t_ptr(v), step,
+                                      lr, beta1, beta2, eps), "mlp_adam_dev")
+
+    def mlp_to_bf16_dev(self, nparam, params, hi, lo=None):
+        _chk(lib().ecalj_mlp_to_bf16_dev(self._h, nparam, _t_ptr(params), _t_ptr(hi), _t_ptr(lo)), "mlp_to_bf16_dev")
+
+    def mlp_infer_bf16_dev(self, dims, params, w_hi, w_lo, x, batch, y):
+        d = _i32(dims)
+        _chk(lib().ecalj_mlp_infer_bf16_dev(self._h, len(d) - 1, _np_ptr(d), _t_ptr(params), _t_ptr(w_hi),
+                                            _t_ptr(w_lo), _t_ptr(x), batch, _t_ptr(y)), "mlp_infer_bf16_dev")
+
+    def tail_features_dev(self, chi, ni, nw, iref, mu, sd, feat, aux):
+        _chk(lib().ecalj_tail_features_dev(self._h, _t_ptr(chi), ni, nw, iref, _t_ptr(mu), _t_ptr(sd), _t_ptr(feat),
+                                           _t_ptr(aux)), "tail_features_dev")
+
+    def tail_apply_dev(self, chi, ni, nw, iref, pred, mu_y, sd_y, aux, chi_out):
+        _chk(lib().ecalj_tail_apply_dev(self._h, _t_ptr(chi), ni, nw, iref, _t_ptr(pred), _t_ptr(mu_y),
+                                        _t_ptr(sd_y), _t_ptr(aux), _t_ptr(chi_out)), "tail_apply_dev")
 
     def set_ppb(self, nv, mdim, ppb):
         """set_ppb (S/m_zmel.f90:110-145): per-atom real blocks nv x nv x mdim."""

@@ -231,6 +231,46 @@ ecalj_status ecalj_qp_energy(int ntq, const double* elda, const double* sex,
 ecalj_status ecalj_sigma_herm(int ntq, const double* sex2, const double* sexcore2,
                               const double* sec, const double* vxcmat, double* se);
 
+/* ---- learned chi0 tail ----------------------------------------------------
+ * No reference counterpart.  The reference truncates the chi0 band sum
+ * (nband_chi0 / emax_chi0, S/m_readgwinput.f90:34-35, applied in
+ * tetwt5x_dtet4 at S/m_tetwt.f90:140-145); these entry points predict what
+ * that truncation drops.  Network: nlayer layers of widths dims[0..nlayer],
+ *   z_l = h_l W_l^T + b_l, h_0 = x, h_{l+1} = silu(z_l) (l < nlayer-1), y = z_{nlayer-1};
+ * params (FP32, device): per layer W_l (dims[l+1] x dims[l], row-major) then
+ * b_l.  Batches are row-major FP32 device arrays (batch x width).
+ * ecalj_mlp_grad_dev: gradient of the mean squared error against target into
+ *   grad (same layout as params); loss (device float*, may be NULL) receives
+ *   the loss.  Deterministic (fixed-order reductions).
+ * ecalj_mlp_adam_dev: one Adam step (step >= 1 counts from 1).
+ * ecalj_mlp_to_bf16_dev: bf16 hi (and lo = x - hi, when lo != NULL) copies of params.
+ * ecalj_mlp_infer_bf16_dev: forward on bf16 MFMA with FP32 accumulation;
+ *   w_lo == NULL gives one bf16 product per MAC, else the hi/lo split
+ *   (hi*hi + hi*lo + lo*hi); biases come from params.
+ * ecalj_tail_features_dev: features of a truncated chi0 (ni x ni x nw,
+ *   complex128, frequencies {0, nu_1, ...}): per basis function I,
+ *   d_j = -Re chi(I,I,j), feat(I,:) = ((d_1..d_{nw-1})/d_0, log d_0),
+ *   standardised with mu/sd (nw floats each); aux (2 ni doubles) = {d_0, d_iref}.
+ * ecalj_tail_apply_dev: chi_out = chi + rho o sqrt(t t^T) per frequency, with
+ *   t(I,j) = max(0, d_0(I) (mu_y[j] + sd_y[j] pred(I,j))) and
+ *   rho(I,J) = chi(I,J,iref)/sqrt(d_iref(I) d_iref(J)): a Hermitian negative
+ *   semidefinite tail.  chi_out may equal chi. */
+ecalj_status ecalj_mlp_nparam(int nlayer, const int* dims, int64_t* nparam);
+ecalj_status ecalj_mlp_forward_dev(ecalj_handle h, int nlayer, const int* dims, const void* params,
+                                   const void* x, int batch, void* y);
+ecalj_status ecalj_mlp_grad_dev(ecalj_handle h, int nlayer, const int* dims, const void* params,
+                                const void* x, const void* target, int batch, void* grad, void* loss);
+ecalj_status ecalj_mlp_adam_dev(ecalj_handle h, int64_t nparam, void* params, const void* grad, void* m,
+                                void* v, int step, double lr, double beta1, double beta2, double eps);
+ecalj_status ecalj_mlp_to_bf16_dev(ecalj_handle h, int64_t nparam, const void* params, void* hi, void* lo);
+ecalj_status ecalj_mlp_infer_bf16_dev(ecalj_handle h, int nlayer, const int* dims, const void* params,
+                                      const void* w_hi, const void* w_lo, const void* x, int batch, void* y);
+ecalj_status ecalj_tail_features_dev(ecalj_handle h, const void* chi, int ni, int nw, int iref,
+                                     const void* mu, const void* sd, void* feat, void* aux);
+ecalj_status ecalj_tail_apply_dev(ecalj_handle h, const void* chi, int ni, int nw, int iref,
+                                  const void* pred, const void* mu_y, const void* sd_y, const void* aux,
+                                  void* chi_out);
+
 #ifdef __cplusplus
 }
 #endif

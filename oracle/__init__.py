@@ -284,3 +284,63 @@ def sigma_herm(sex2, sexcore2, sec, vxcmat):
     se = np.zeros((ntq, ntq), np.complex128, order="F")
     lib().orc_sigma_herm(ntq, _p(_cz(sex2)), _p(_cz(sexcore2)), _p(_cz(sec)), _p(_cz(vxcmat)), _p(se))
     return se
+
+
+# ---------------------------------------------------------------- learned-tail MLP (oracle/mlp.c)
+def _pr(a):
+    """Pointer to a C-ordered (row-major) float64 array; None passes NULL."""
+    if a is None:
+        return None
+    assert a.flags.c_contiguous and a.dtype == np.float64
+    return a.ctypes.data_as(_dp)
+
+
+def _mlp_sigs():
+    L = lib()
+    if getattr(L, "_mlp_ready", False):
+        return L
+    L.orc_mlp_nparam.restype = C.c_int64
+    L.orc_mlp_nparam.argtypes = [C.c_int, _ip]
+    L.orc_mlp_forward.argtypes = [C.c_int, _ip, _dp, _dp, C.c_int, _dp]
+    L.orc_mlp_loss_grad.restype = C.c_double
+    L.orc_mlp_loss_grad.argtypes = [C.c_int, _ip, _dp, _dp, _dp, C.c_int, _dp]
+    for n in ("orc_silu", "orc_dsilu"):
+        getattr(L, n).restype = C.c_double
+        getattr(L, n).argtypes = [C.c_double]
+    L._mlp_ready = True
+    return L
+
+
+def mlp_nparam(dims):
+    d = _ci(dims)
+    return int(_mlp_sigs().orc_mlp_nparam(len(d) - 1, _ip_(d)))
+
+
+def mlp_forward(dims, params, x):
+    """y = MLP(x); x: B x dims[0] (row-major), params float64 (nparam)."""
+    d = _ci(dims)
+    x = np.ascontiguousarray(x, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    y = np.zeros((x.shape[0], d[-1]))
+    _mlp_sigs().orc_mlp_forward(len(d) - 1, _ip_(d), _pr(p), _pr(x), x.shape[0], _pr(y))
+    return y
+
+
+def mlp_loss_grad(dims, params, x, t, want_grad=True):
+    """(mean squared error, its gradient w.r.t. params or None)."""
+    d = _ci(dims)
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    g = np.zeros(p.size) if want_grad else None
+    loss = _mlp_sigs().orc_mlp_loss_grad(len(d) - 1, _ip_(d), _pr(p), _pr(x), _pr(t), x.shape[0],
+                                         _pr(g) if want_grad else None)
+    return float(loss), g
+
+
+def silu(z):
+    return _mlp_sigs().orc_silu(float(z))
+
+
+def dsilu(z):
+    return _mlp_sigs().orc_dsilu(float(z))

@@ -136,6 +136,19 @@ void orc_qp_energy(int ntq, const double* elda, const double* sex, const double*
 void orc_sigma_herm(int ntq, const double complex* sex2, const double complex* sexcore2,
                     const double complex* sec, const double complex* vxcmat, double complex* se);
 
+/* Learned chi0-tail MLP (oracle/mlp.c): L layers of widths dims[0..L],
+ * silu between layers, linear output; params W_l (row-major out x in) then
+ * b_l per layer; batches row-major.  No reference counterpart (the reference
+ * truncates the band sum: nband_chi0, S/m_readgwinput.f90:34-35). */
+#include <stdint.h>
+int64_t orc_mlp_nparam(int L, const int* dims);
+void orc_mlp_forward(int L, const int* dims, const double* p, const double* x, int B, double* y);
+/* returns the mean-squared-error loss; g (nparam) receives its gradient when non-NULL */
+double orc_mlp_loss_grad(int L, const int* dims, const double* p, const double* x, const double* t, int B,
+                         double* g);
+double orc_silu(double z);
+double orc_dsilu(double z);
+
 #ifdef __cplusplus
 }
 #endif
