@@ -37,8 +37,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(qp, gpu):
-    """FP64 oracle on host cores: a bounded sample of the q-point, scaled."""
+def cpu_baseline(qp, nks=8, nws=3):
+    """FP64 oracle on host cores: a bounded sample of the q-point (chi0 and
+    Sigma_c on nks of the nk k-points, the W build at nws frequencies), scaled."""
     import oracle as orc
     from ecalj_amd import synth
     from oracle import pipeline
@@ -47,10 +48,12 @@ def cpu_baseline(qp, gpu):
     cfg = qp.cfg
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t = {}
-    # chi0 for one k-point (zmel + x0gemm) -> scaled by nk; Hilbert once
+    ks = list(range(min(nks, cfg.nk)))
+    nw = min(nws, cfg.nf + 1)
+    # chi0 for nks k-points (zmel + x0gemm) -> scaled by nk; Hilbert once
     t0 = time.perf_counter()
-    chi, _ = pipeline.chi0(qp, k_subset=[0])
-    t["chi0_one_k_plus_hilbert"] = time.perf_counter() - t0
+    chi, _ = pipeline.chi0(qp, k_subset=ks)
+    t["chi0_k_plus_hilbert"] = time.perf_counter() - t0
     # Hilbert alone, to separate it from the per-k part
     fr = synth.frhis(qp)
     nwhis = len(fr) - 1
@@ -61,19 +64,20 @@ def cpu_baseline(qp, gpu):
     orc.dpsion_imag(rc, fr, im)
     t_hil = time.perf_counter() - t0
     del rc
-    # W build for one frequency -> scaled by nf+1
+    # W build for nw frequencies -> scaled by nf+1
     t0 = time.perf_counter()
-    wc1 = orc.wc_build(chi[:, :, :1], qp.vcousq, qp.zcousq, nspin=cfg.nspin)
-    t["w_one_freq"] = time.perf_counter() - t0
-    # Sigma_c for one internal k -> scaled by nk
-    wck = [np.repeat(wc1, cfg.nf + 1, axis=2)]
+    wcs = orc.wc_build(np.asfortranarray(chi[:, :, :nw]), qp.vcousq, qp.zcousq, nspin=cfg.nspin)
+    t["w_freqs"] = time.perf_counter() - t0
+    # Sigma_c for nks internal k -> scaled by nk (W_c(k) = the sampled W_c(q) frequencies, repeated)
+    reps = [wcs[:, :, i % nw] for i in range(cfg.nf + 1)]
+    wck = [np.asfortranarray(np.stack(reps, axis=2))] * len(ks)
     t0 = time.perf_counter()
-    pipeline.sigc(qp, wck, k_subset=[0])
-    t["sigc_one_k"] = time.perf_counter() - t0
-    per_q = ((t["chi0_one_k_plus_hilbert"] - t_hil) * cfg.nk + t_hil + t["w_one_freq"] * (cfg.nf + 1)
-             + t["sigc_one_k"] * cfg.nk)
-    sample = (f"{cfg.name}: chi0 on 1 of {cfg.nk} k-points + full Hilbert transform, W build at 1 of "
-              f"{cfg.nf + 1} frequencies, Sigma_c on 1 of {cfg.nk} internal k-points "
+    pipeline.sigc(qp, wck, k_subset=ks)
+    t["sigc_k"] = time.perf_counter() - t0
+    per_q = ((t["chi0_k_plus_hilbert"] - t_hil) * cfg.nk / len(ks) + t_hil + t["w_freqs"] * (cfg.nf + 1) / nw
+             + t["sigc_k"] * cfg.nk / len(ks))
+    sample = (f"{cfg.name}: chi0 on {len(ks)} of {cfg.nk} k-points + full Hilbert transform, W build at {nw} of "
+              f"{cfg.nf + 1} frequencies, Sigma_c on {len(ks)} of {cfg.nk} internal k-points "
               f"({sum(t.values()):.1f} s measured); scaled to one q-point")
     return {"value": per_q, "unit": "s/q-point", "cores": threads, "kind": "port", "sample": sample}
 
@@ -137,6 +141,9 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--also", default="C3",
+                    help="comma-separated further configs timed in both forms (2 steps each) and reported "
+                         "under 'also'; 'none' to skip")
     ap.add_argument("--only", choices=["f64", "mixed"], default=None,
                     help="profile one form only (prints a short line instead of the contract JSON)")
     args = ap.parse_args()
@@ -193,9 +200,31 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle on a bounded sample")
         try:
-            out["cpu_baseline"] = cpu_baseline(qp, None)
+            out["cpu_baseline"] = cpu_baseline(qp)
         except Exception as e:  # report, never hide
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    also = [c for c in args.also.split(",") if c and c.lower() != "none" and c != args.config]
+    if also:
+        out["also"] = {}
+    for name in also:
+        c = synth.CONFIGS[name]
+        log(f"bench: generating {c.name} inputs")
+        q2 = synth.make_qpoint(c)
+        a64 = run_form(q2, "f64", 2, 1)
+        amx = run_form(q2, args.mixed, 2, 1)
+        f2 = flops_per_q(c, nwhis_pairs_bins=amx["npb"])
+        out["also"][name] = {
+            "workload": f"{c.name} q-point: nk={c.nk}, nb={c.nb}, occ={c.no}, unocc={c.nu}, nI={c.ni}, "
+                        f"nf={c.nf}(+nu=0), ntq={c.nq}",
+            "steps": 2, "warmup": 1,
+            "s_per_q": round(amx["s_per_q"], 6), "fp64_form_s_per_q": round(a64["s_per_q"], 6),
+            "speedup_vs_fp64_form": round(a64["s_per_q"] / amx["s_per_q"], 3),
+            "qp_err_meV_vs_fp64_form": round(float(np.abs(amx["eqp"] - a64["eqp"]).max() * 1e3), 5),
+            "tflops": round(sum(f2.values()) / amx["s_per_q"] / 1e12, 3),
+            "kernel_ms_per_step": {k: round(v[0] / 2, 3) for k, v in amx["ktimes"].items()},
+            "roofline": roofline(f2, amx, args.mixed, 2),
+        }
+        del q2
     print(json.dumps(out), flush=True)
 
 

@@ -43,6 +43,10 @@ _SIGS = {
     "ecalj_kernel_timing": [_p, _i],
     "ecalj_kernel_time": [_p, _i, C.POINTER(_d), C.POINTER(_l)],
     "ecalj_set_tuning": [_p, _i],
+    "ecalj_graph_begin": [_p],
+    "ecalj_graph_end": [_p, C.POINTER(C.c_void_p)],
+    "ecalj_graph_launch": [_p, _p],
+    "ecalj_graph_destroy": [_p],
     "ecalj_tuning_report": [_p, C.c_char_p, _l],
     "ecalj_set_ppb": [_p, _i, _p, _p, _p],
     "ecalj_set_coulomb": [_p, _i, _p, _p],
@@ -181,6 +185,27 @@ class DevArray:
             pass
 
 
+class Graph:
+    """A captured HIP graph of a handle's stream (ecalj_graph_*)."""
+
+    def __init__(self, h: "Handle", ptr):
+        self.h, self.ptr = h, ptr
+
+    def launch(self):
+        _chk(lib().ecalj_graph_launch(self.h.ptr, C.c_void_p(self.ptr)), "graph_launch")
+
+    def close(self):
+        if self.ptr:
+            lib().ecalj_graph_destroy(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Handle:
     """One library handle (device binding, stream, module state, scratch)."""
 
@@ -199,6 +224,11 @@ class Handle:
 
     def to_dev(self, a, dtype=None):
         return DevArray.from_host(self, a, dtype)
+
+    def memcpy_d2d(self, dst, src, nbytes=None):
+        n = dst.nbytes if nbytes is None else nbytes
+        assert nbytes is not None or dst.nbytes == src.nbytes
+        _chk(lib().ecalj_memcpy_d2d(self._h, _t_ptr(dst), _t_ptr(src), n), "memcpy_d2d")
 
     def phase_similarity(self, a, ngb, nmat, d, nk, out):
         _chk(lib().ecalj_dev_phase_similarity(self._h, _t_ptr(a), ngb, nmat, _t_ptr(d), nk, _t_ptr(out)),
@@ -233,6 +263,15 @@ class Handle:
         n = C.c_int64()
         _chk(lib().ecalj_kernel_time(self._h, KCLASS[kclass], C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
+
+    def graph_begin(self):
+        """Record the following _dev calls into a HIP graph (after a warm-up run of the same calls)."""
+        _chk(lib().ecalj_graph_begin(self._h), "graph_begin")
+
+    def graph_end(self) -> Graph:
+        g = C.c_void_p()
+        _chk(lib().ecalj_graph_end(self._h, C.byref(g)), "graph_end")
+        return Graph(self, g.value)
 
     def set_tuning(self, mode: int):
         """0: measure kernel variants on the first call of a shape; k > 0: force variant k."""

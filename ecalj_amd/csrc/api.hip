@@ -35,6 +35,7 @@ void tune_store(ecalj_ctx* h, const TuneKey& k, int var, const char* op, const s
 
 ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
+  if (h->slot_bytes[slot] < bytes && h->capturing) return ECALJ_ERR_STATE;  // warm up before capture
   if (h->slot_bytes[slot] < bytes) {
     if (h->slot[slot]) {
       ECALJ_HIP(hipStreamSynchronize(h->stream));
@@ -152,6 +153,14 @@ static ecalj_status upload(ecalj_ctx* h, int slot, const std::vector<T>& v, T** 
   const size_t bytes = sizeof(T) * v.size();
   ECALJ_TRY(slot_ensure(h, slot, bytes, (void**)out));
   if (v.empty()) return ECALJ_OK;
+  if (h->capturing) {  // the graph's copy node reads this buffer at every replay: give it its own
+    void* pin = nullptr;
+    if (hipHostMalloc(&pin, bytes, hipHostMallocDefault) != hipSuccess) return ECALJ_ERR_NOMEM;
+    h->cap_pinned.push_back(pin);
+    memcpy(pin, v.data(), bytes);
+    ECALJ_HIP(hipMemcpyAsync(*out, pin, bytes, hipMemcpyHostToDevice, h->stream));
+    return ECALJ_OK;
+  }
   // stage through pinned memory so the copy is asynchronous; the host waits
   // only for the previous copy out of the same staging buffer
   if (h->pin_ev[slot]) ECALJ_HIP(hipEventSynchronize(h->pin_ev[slot]));
@@ -269,6 +278,66 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
     }
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
+  return ECALJ_OK;
+}
+
+// ------------------------------------------------------------------ HIP graphs
+struct ecalj_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void*> pinned;
+};
+
+ecalj_status ecalj_graph_begin(ecalj_handle h) {
+  if (!h) return ECALJ_ERR_ARG;
+  if (h->capturing) return ECALJ_ERR_STATE;
+  ECALJ_HIP(hipSetDevice(h->device));
+  ECALJ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));  // pinned staging is allocated while capturing
+  h->capturing = true;
+  h->timing_before_capture = h->timing;
+  h->timing = false;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_graph_end(ecalj_handle h, ecalj_graph_t* out) {
+  if (!h || !out) return ECALJ_ERR_ARG;
+  if (!h->capturing) return ECALJ_ERR_STATE;
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  h->capturing = false;
+  h->timing = h->timing_before_capture;
+  auto* gr = new ecalj_graph_s;
+  gr->pinned.swap(h->cap_pinned);
+  if (e != hipSuccess || !g) {
+    for (void* p : gr->pinned) (void)hipHostFree(p);
+    delete gr;
+    return ECALJ_ERR_HIP;
+  }
+  gr->graph = g;
+  if (hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    for (void* p : gr->pinned) (void)hipHostFree(p);
+    delete gr;
+    return ECALJ_ERR_HIP;
+  }
+  *out = gr;
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_graph_launch(ecalj_handle h, ecalj_graph_t g) {
+  if (!h || !g) return ECALJ_ERR_ARG;
+  if (h->capturing) return ECALJ_ERR_STATE;
+  ECALJ_HIP(hipGraphLaunch(g->exec, h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_graph_destroy(ecalj_graph_t g) {
+  if (!g) return ECALJ_ERR_ARG;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (void* p : g->pinned) (void)hipHostFree(p);
+  delete g;
   return ECALJ_OK;
 }
 

@@ -38,6 +38,13 @@ struct ecalj_ctx {
   size_t pin_bytes[S_NSLOT] = {};
   hipEvent_t pin_ev[S_NSLOT] = {};
 
+  // HIP graph capture of the handle's stream (ecalj_graph_begin/end): scratch
+  // slots may not grow, uploads stage through fresh pinned buffers that the
+  // graph keeps, kernel timing is off, the tuner uses cached choices only
+  bool capturing = false;
+  bool timing_before_capture = false;
+  std::vector<void*> cap_pinned;
+
   // kernel-variant tuner: 0 = measure each variant on the first call of a shape and
   // keep the fastest; k > 0 = always variant k (1-based)
   int tune_mode = 0;

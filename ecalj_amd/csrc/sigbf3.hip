@@ -347,7 +347,10 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
   const TuneKey key{TUNE_SIGC_BF3, n, ncol, nw1, nb};
   if (h->tune_mode > 0) {
     var = std::min(h->tune_mode, kNumSigVariants) - 1;
-  } else if (!tune_lookup(h, key, &var)) {
+  } else if (tune_lookup(h, key, &var)) {
+  } else if (h->capturing) {  // no timing under capture: first variant
+    var = 0;
+  } else {
     std::vector<double> ms(kNumSigVariants);
     for (int v = 0; v < kNumSigVariants; ++v) {
       ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));

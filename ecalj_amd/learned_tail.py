@@ -1,0 +1,260 @@
+"""Learned chi0 tail: a small MLP predicts what truncating the chi0 band sum drops.
+
+ecalj bounds the unoccupied bands of the chi0 sum with nband_chi0 / emax_chi0
+(S/m_readgwinput.f90:34-35, applied in tetwt5x_dtet4 at S/m_tetwt.f90:140-145)
+and keeps whatever the truncation loses out of W.  Here the truncated sum
+chi_T (unoccupied bands 1..nu_t) is computed by the same GPU path, and the
+missing high-band part chi_F - chi_T is predicted from chi_T itself:
+
+* features, per product-basis function I (one training pair = one I of one
+  q-point): the truncated diagonal spectrum d_j = -Re chi_T(I,I,i nu_j) over
+  the nw = nf+1 frequencies, as (d_1..d_{nw-1})/d_0 and log d_0;
+* target: the missing diagonal (-Re (chi_F - chi_T)(I,I,j)) / d_0, j = 0..nw-1;
+* network: 3 layers of width 256 (nw -> 256 -> 256 -> nw), silu, FP32
+  weights, trained with Adam by the library's own forward/backward kernels;
+  inference on bf16 MFMA (hi/lo split by default);
+* reconstruction (ecalj_tail_apply_dev): chi_L = chi_T + D rho D with D the
+  predicted diagonal's square root and rho the correlation structure of
+  chi_T at the highest frequency, whose weights favour the largest
+  transition energies the truncated sum holds -- the closest stand-in for the
+  tail's own structure.  The result stays Hermitian negative semidefinite.
+
+The full band sum is the judge and the fallback: ``judge`` computes the QP
+energies of held-out q-points three ways (full, truncated, truncated +
+learned tail) and the model is accepted only if the learned form is within
+``tol_mev`` of the full sum; ``LearnedChi0`` computes the full sum whenever
+the model is not accepted.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import Handle, mlp_nparam, synth
+from .qsgw import QPointGPU
+
+WIDTH = 256
+
+
+def tail_dims(nw: int):
+    return [nw, WIDTH, WIDTH, nw]
+
+
+@dataclasses.dataclass
+class TailModel:
+    dims: list
+    params: np.ndarray      # float32, layout of ecalj_mlp_* (per layer W row-major, then b)
+    mu_x: np.ndarray        # float32 feature standardisation
+    sd_x: np.ndarray
+    mu_y: np.ndarray        # float32 target standardisation
+    sd_y: np.ndarray
+    iref: int               # frequency index of the correlation structure
+    nu_t: int               # unoccupied bands of the truncated sum
+    info: dict = dataclasses.field(default_factory=dict)
+
+    def save(self, path):
+        np.savez(path, dims=np.asarray(self.dims, np.int32), params=self.params, mu_x=self.mu_x, sd_x=self.sd_x,
+                 mu_y=self.mu_y, sd_y=self.sd_y, iref=self.iref, nu_t=self.nu_t)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path, allow_pickle=False)
+        return cls(list(z["dims"]), z["params"], z["mu_x"], z["sd_x"], z["mu_y"], z["sd_y"], int(z["iref"]),
+                   int(z["nu_t"]))
+
+
+# ---------------------------------------------------------------- data
+def _diag(chi_dev, ni):
+    c = chi_dev.download()
+    return -np.real(np.einsum("iij->ij", c))  # ni x nw
+
+
+class PairSet:
+    """chi_T (truncated at nu_t unoccupied bands) and chi_F (all bands) of one
+    q-point from one zmel pass on the GPU (same histogram mesh, same pairs up to nu_t)."""
+
+    def __init__(self, qp: synth.QPoint, nu_t: int, precision: str = "f64"):
+        cfg = qp.cfg.scaled(nu_chi0=0)
+        qp = dataclasses.replace(qp, cfg=cfg)
+        self.g = g = QPointGPU(qp, precision)
+        self.ni, self.nw = cfg.ni, cfg.nf + 1
+        g.chi0()                                  # full band sum -> g.chi
+        self.chi_f = g.h.array((self.ni, self.ni, self.nw), np.complex128)
+        g.h.memcpy_d2d(self.chi_f, g.chi)
+        self.ix_t = synth.x0kf_index(synth.pair_weights(qp, g.fr, nu_t))
+        self.chi_t = g.h.array((self.ni, self.ni, self.nw), np.complex128)
+        g.chi0_pairs(self.ix_t, self.chi_t)
+
+    def features_targets(self, iref):
+        """Raw (unstandardised) features and targets, one row per product-basis function."""
+        h, ni, nw = self.g.h, self.ni, self.nw
+        zero = h.to_dev(np.zeros(nw, np.float32))
+        one = h.to_dev(np.ones(nw, np.float32))
+        feat = h.array((nw, ni), np.float32)
+        aux = h.array((2, ni), np.float64)
+        h.tail_features_dev(self.chi_t, ni, nw, iref, zero, one, feat, aux)
+        x = feat.download().T.astype(np.float64)
+        s = aux.download()[0]
+        dt, df = _diag(self.chi_t, ni), _diag(self.chi_f, ni)
+        ok = s > 0
+        y = np.zeros((ni, nw))
+        y[ok] = (df[ok] - dt[ok]) / s[ok, None]
+        return x[ok], y[ok]
+
+    def close(self):
+        self.g.h.close()
+
+
+# ---------------------------------------------------------------- training (GPU)
+def train(h: Handle, x: np.ndarray, y: np.ndarray, dims, epochs=300, batch=256, lr=2e-3, seed=0, log=None):
+    """Adam on the library's MLP kernels; x, y standardised (N x nw).  Returns (params, losses)."""
+    rng = np.random.default_rng(seed)
+    n = mlp_nparam(dims)
+    p = np.empty(n, np.float32)
+    o = 0
+    for l in range(len(dims) - 1):
+        din, dout = dims[l], dims[l + 1]
+        p[o:o + dout * din] = rng.standard_normal(dout * din) * math.sqrt(1.0 / din)
+        o += dout * din
+        p[o:o + dout] = 0.0
+        o += dout
+    N = x.shape[0]
+    nb = N // batch
+    dp = h.to_dev(p)
+    dg, dm, dv = (h.array(n, np.float32) for _ in range(3))
+    dm.zero()
+    dv.zero()
+    dl = h.array(1, np.float32)
+    losses = []
+    step = 0
+    nsteps = epochs * nb
+    for ep in range(epochs):
+        perm = rng.permutation(N)[: nb * batch]
+        dx = h.to_dev(np.asarray(x[perm], np.float32).T)
+        dy = h.to_dev(np.asarray(y[perm], np.float32).T)
+        for b in range(nb):
+            step += 1
+            lr_t = 1e-5 + 0.5 * (lr - 1e-5) * (1 + math.cos(math.pi * (step - 1) / nsteps))
+            h.mlp_grad_dev(dims, dp, dx.ptr + 4 * b * batch * dims[0], dy.ptr + 4 * b * batch * dims[-1], batch,
+                           dg, dl if b == nb - 1 else None)
+            h.mlp_adam_dev(n, dp, dg, dm, dv, step, lr_t)
+        if ep % 25 == 0 or ep == epochs - 1:
+            losses.append(float(dl.download()[0]))
+            if log:
+                log(f"tail: epoch {ep} loss {losses[-1]:.3e}")
+        dx.free()
+        dy.free()
+    return dp.download(), losses
+
+
+def fit(qps, nu_t, npairs=4096, epochs=300, seed=0, log=None, h=None):
+    """Generate (truncated, full) pairs from the q-points qps with the GPU path and train."""
+    xs, ys = [], []
+    nw = qps[0].cfg.nf + 1
+    iref = nw - 1
+    t0 = time.perf_counter()
+    for qp in qps:
+        ps = PairSet(qp, nu_t)
+        x, y = ps.features_targets(iref)
+        ps.close()
+        xs.append(x)
+        ys.append(y)
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    rng = np.random.default_rng(seed)
+    sel = rng.permutation(len(x))[:npairs]
+    x, y = x[sel], y[sel]
+    t_data = time.perf_counter() - t0
+    mu_x, sd_x = x.mean(0), x.std(0) + 1e-12
+    mu_y, sd_y = y.mean(0), y.std(0) + 1e-12
+    xs_, ys_ = (x - mu_x) / sd_x, (y - mu_y) / sd_y
+    dims = tail_dims(nw)
+    own = h is None
+    if own:
+        h = Handle(0, "f64")
+    t0 = time.perf_counter()
+    params, losses = train(h, xs_, ys_, dims, epochs=epochs, seed=seed, log=log)
+    h.sync()
+    t_train = time.perf_counter() - t0
+    if own:
+        h.close()
+    f32 = lambda a: np.asarray(a, np.float32)
+    return TailModel(dims, params, f32(mu_x), f32(sd_x), f32(mu_y), f32(sd_y), iref, nu_t,
+                     info={"npairs": len(x), "losses": losses, "t_data_s": t_data, "t_train_s": t_train,
+                           "epochs": epochs})
+
+
+# ---------------------------------------------------------------- inference (GPU)
+class LearnedTail:
+    """A TailModel resident on a handle: chi_T -> chi_T + predicted tail, all on the device."""
+
+    def __init__(self, h: Handle, m: TailModel, split: bool = True):
+        self.h, self.m = h, m
+        n = mlp_nparam(m.dims)
+        self.p = h.to_dev(m.params)
+        self.hi = h.array(n, np.uint16)
+        self.lo = h.array(n, np.uint16) if split else None
+        h.mlp_to_bf16_dev(n, self.p, self.hi, self.lo)
+        self.mu_x, self.sd_x = h.to_dev(m.mu_x), h.to_dev(m.sd_x)
+        self.mu_y, self.sd_y = h.to_dev(m.mu_y), h.to_dev(m.sd_y)
+        self._bufs = {}
+
+    def apply(self, chi_t, ni, out):
+        nw = self.m.dims[0]
+        if ni not in self._bufs:
+            self._bufs[ni] = (self.h.array((nw, ni), np.float32), self.h.array((2, ni), np.float64),
+                              self.h.array((nw, ni), np.float32))
+        feat, aux, pred = self._bufs[ni]
+        self.h.tail_features_dev(chi_t, ni, nw, self.m.iref, self.mu_x, self.sd_x, feat, aux)
+        self.h.mlp_infer_bf16_dev(self.m.dims, self.p, self.hi, self.lo, feat, ni, pred)
+        self.h.tail_apply_dev(chi_t, ni, nw, self.m.iref, pred, self.mu_y, self.sd_y, aux, out)
+
+
+def qp_from_chi(g: QPointGPU, chi_dev):
+    """QP energies of g's q-point with chi0 replaced by chi_dev (W, W(k), Sigma_c on the GPU)."""
+    if chi_dev is not g.chi:
+        g.h.memcpy_d2d(g.chi, chi_dev)
+    g.wbuild()
+    g.make_wk()
+    g.sigc()
+    return g.results()[3]
+
+
+def judge(m: TailModel, qps, tol_mev=1.0, log=None):
+    """QP-energy errors (meV) of the truncated and the learned forms against the
+    full band sum on held-out q-points; the model is accepted if every learned
+    error is within tol_mev."""
+    rows = []
+    for qp in qps:
+        ps = PairSet(qp, m.nu_t)
+        g, ni = ps.g, ps.ni
+        lt = LearnedTail(g.h, m)
+        chi_l = g.h.array((ni, ni, ps.nw), np.complex128)
+        lt.apply(ps.chi_t, ni, chi_l)
+        e_f = qp_from_chi(g, ps.chi_f)
+        e_t = qp_from_chi(g, ps.chi_t)
+        e_l = qp_from_chi(g, chi_l)
+        cf, ct, cl = ps.chi_f.download(), ps.chi_t.download(), chi_l.download()
+        nrm = np.linalg.norm(cf)
+        dg = lambda c: np.einsum("iij->ij", c)
+        nd = np.linalg.norm(dg(cf))
+        row = {"seed": qp.cfg.seed,
+               "chi_err_truncated": float(np.linalg.norm(ct - cf) / nrm),
+               "chi_err_learned": float(np.linalg.norm(cl - cf) / nrm),
+               "chi_diag_err_truncated": float(np.linalg.norm(dg(ct) - dg(cf)) / nd),
+               "chi_diag_err_learned": float(np.linalg.norm(dg(cl) - dg(cf)) / nd),
+               "qp_err_truncated_meV": float(np.abs(e_t - e_f).max() * 1e3),
+               "qp_err_learned_meV": float(np.abs(e_l - e_f).max() * 1e3)}
+        rows.append(row)
+        if log:
+            log(f"tail judge: {row}")
+        ps.close()
+    worst = max(r["qp_err_learned_meV"] for r in rows)
+    return {"rows": rows, "judged_error_meV": worst, "accepted": worst <= tol_mev, "tol_meV": tol_mev}
+
+
+def seeded_qpoints(cfg: synth.Config, n: int, first: int = 0):
+    """n q-points of the family cfg (seeds cfg.seed + 100 (first + i))."""
+    return [synth.make_qpoint(cfg.scaled(seed=cfg.seed + 100 * (first + i))) for i in range(n)]

@@ -32,7 +32,10 @@ def gauss01(n):
 
 
 class QPointGPU:
-    def __init__(self, qp: synth.QPoint, precision: str = "f64", device: int = 0):
+    """tail_model: a learned_tail.TailModel; chi0 then sums the unoccupied bands
+    up to the model's nu_t and adds the predicted tail on the device."""
+
+    def __init__(self, qp: synth.QPoint, precision: str = "f64", device: int = 0, tail_model=None):
         self.qp = qp
         self.cfg = cfg = qp.cfg
         self.h = h = Handle(device, precision)
@@ -40,14 +43,18 @@ class QPointGPU:
         # bulk intermediates (zmel, rcxq) in the contraction's storage type
         cst = c128 if precision == "f64" else np.complex64
         self.cst = cst
-        no, nu, nbt, nb, ni, nk, ntq = cfg.no, cfg.nu, cfg.nbt, cfg.nb, cfg.ni, cfg.nk, cfg.nq
+        no, nbt, nb, ni, nk, ntq = cfg.no, cfg.nbt, cfg.nb, cfg.ni, cfg.nk, cfg.nq
+        # unoccupied bands in the chi0 band sum (nband_chi0, S/m_readgwinput.f90:34-35)
+        self.nuc = nu = cfg.nu_chi0 if cfg.nu_chi0 else cfg.nu
+        if tail_model is not None:
+            self.nuc = nu = tail_model.nu_t
         self.nw1 = cfg.nf + 1
         # module state
-        h.set_ppb([nb], [ni], qp.ppb)
+        h.set_ppb(cfg.nv_list, cfg.mdim_list, qp.ppb)
         h.set_coulomb(qp.vcoud, qp.zcousq)
         # eigenvectors on the device, one block per k
         self.c_occ = h.to_dev(qp.c_k[:, :no, :], c128)       # nb x no x nk
-        self.c_unocc = h.to_dev(qp.c_kq[:, no:, :], c128)    # nb x nu x nk
+        self.c_unocc = h.to_dev(qp.c_kq[:, no:no + nu, :], c128)    # nb x nu x nk
         self.c_qmk = h.to_dev(qp.c_qmk, c128)                # nb x nbt x nk
         self.c_q = h.to_dev(qp.c_q[:, :ntq], c128)           # nb x ntq
         # frequencies
@@ -58,7 +65,7 @@ class QPointGPU:
         # histogram and x0 index lists
         self.fr = synth.frhis(qp)
         self.nwhis = len(self.fr) - 1
-        self.ix = synth.x0kf_index(synth.pair_weights(qp, self.fr))
+        self.ix = synth.x0kf_index(synth.pair_weights(qp, self.fr, nu))
         # buffers
         self.zmel_chi = h.array((ni, no, nu, nk), cst)
         self.rcxq = h.array((ni, ni, self.nwhis + 1), cst)
@@ -76,19 +83,30 @@ class QPointGPU:
         self.d_wx = h.to_dev(self.wx, f64)
         self.d_expa = h.to_dev(self.expa, f64)
         self.phase = h.to_dev(synth.wc_phases(cfg), c128)   # ni x nk
+        self.tail = None
+        if tail_model is not None:
+            from .learned_tail import LearnedTail
+            self.tail = LearnedTail(h, tail_model)
 
     # ---------------------------------------------------------------- chi0
     def chi0(self):
         cfg = self.cfg
-        no, nu, ni, nb, nk = cfg.no, cfg.nu, cfg.ni, cfg.nb, cfg.nk
+        no, nu, nb, nk = cfg.no, self.nuc, cfg.nb, cfg.nk
         # zmel(I, n_occ, m_unocc) for every k, conj (zmelconjg=.true., S/x0kf_v4h.f90:273-275)
         self.h.zmel_dev(self.c_occ, self.c_unocc, self.zmel_chi, nm=no, nq=nu, ndima=nb, nbatch=nk,
                         is_m_basis=True, zmelconjg=True)
+        self.chi0_pairs(self.ix, self.chi)
+        if self.tail is not None:
+            self.tail.apply(self.chi, cfg.ni, self.chi)
+
+    def chi0_pairs(self, ix, out):
+        """x0gemm over the pairs of the index lists ix (from the zmel of chi0()) + Hilbert -> out."""
+        cfg = self.cfg
+        no, nu, ni, nk = cfg.no, self.nuc, cfg.ni, cfg.nk
         self.rcxq.zero()
-        ix = self.ix
         self.h.x0gemm_batched_dev(self.rcxq, ni, self.nwhis, nk, ix.nkmin, ix.nkmax, self.zmel_chi, ni * no * nu,
                                   ni, nu, ix)
-        self.h.dpsion_imag_dev(self.rcxq, ni, ni, self.nwhis, self.fr, self.nu, self.chi)
+        self.h.dpsion_imag_dev(self.rcxq, ni, ni, self.nwhis, self.fr, self.nu, out)
 
     def wbuild(self):
         cfg = self.cfg
@@ -117,6 +135,17 @@ class QPointGPU:
         self.wbuild()
         self.sigc()
 
+    def capture_step(self):
+        """A HIP graph of step() (call after one eager step of this q-point).  The
+        host-side index lists are fixed at capture; a replay recomputes the
+        q-point from the device buffers, e.g. in the next QSGW iteration."""
+        self.h.graph_begin()
+        try:
+            self.step()
+        finally:
+            g = self.h.graph_end()
+        return g
+
     # ---------------------------------------------------------------- results
     def results(self):
         self.h.sync()
@@ -131,13 +160,15 @@ class QPointGPU:
 
 def flops_per_q(cfg, nwhis_pairs_bins=None):
     """Algorithmic FLOPs of one q-point as executed (complex MAC = 8, real x complex = 4)."""
-    nk, nb, no, nu, ni, nw1, ntq, nbt = cfg.nk, cfg.nb, cfg.no, cfg.nu, cfg.ni, cfg.nf + 1, cfg.nq, cfg.nbt
+    nk, nb, no, ni, nw1, ntq, nbt = cfg.nk, cfg.nb, cfg.no, cfg.ni, cfg.nf + 1, cfg.nq, cfg.nbt
+    nu = cfg.nu_chi0 if cfg.nu_chi0 else cfg.nu
     f = {}
-    f["zmel_chi"] = nk * (4.0 * nb * nb * ni * no + 8.0 * ni * no * nu * nb)
+    na = cfg.natom  # atom blocks: sum_a nv_a^2 mdim_a = nb^2 ni / na^2, sum_a mdim_a nv_a = ni nb / na
+    f["zmel_chi"] = nk * (4.0 * nb * nb * ni * no / na ** 2 + 8.0 * ni * no * nu * nb / na)
     f["x0"] = 8.0 * ni * ni * (nwhis_pairs_bins if nwhis_pairs_bins is not None else nk * no * nu)
     f["hilbert"] = 0.0  # memory-bound, counted in bytes
     f["wbuild"] = nw1 * (8.0 * ni ** 3 + 4 * 8.0 * ni ** 3)
-    f["zmel_sig"] = 4.0 * nb * nb * ni * ntq + nk * 8.0 * ni * nbt * ntq * nb
+    f["zmel_sig"] = 4.0 * nb * nb * ni * ntq / na ** 2 + nk * 8.0 * ni * nbt * ntq * nb / na
     f["sigc"] = nk * 8.0 * ni * ni * nw1 * nbt * ntq
     f["sigc_final"] = nk * 8.0 * ntq * ntq * ni * nbt
     return f

@@ -53,6 +53,8 @@ class Config:
     hist_ratio: float = 1.03
     hist_dw: float = 1e-5
     window: float = 0.02
+    natom: int = 1   # atoms of the MT product basis: ppb in natom blocks of nb/natom x nb/natom x ni/natom
+    nu_chi0: int = 0  # unoccupied bands in the chi0 band sum (0 = all); nband_chi0, S/m_readgwinput.f90:34-35
     note: str = ""
 
     @property
@@ -62,6 +64,14 @@ class Config:
     @property
     def nq(self) -> int:
         return self.ntq if self.ntq else 2 * self.no
+
+    @property
+    def nv_list(self):
+        return [self.nb // self.natom] * self.natom
+
+    @property
+    def mdim_list(self):
+        return [self.ni // self.natom] * self.natom
 
     def scaled(self, **kw) -> "Config":
         return dataclasses.replace(self, **kw)
@@ -77,6 +87,10 @@ CONFIGS = {
     # orthonormal eigenvectors (300 < 318 is inconsistent).
     "C3": Config("C3", nk=64, nb=320, no=18, nu=300, ni=1200, nf=20, seed=20261205,
                  note="nb=320 (spec says 300 < nbt=318)"),
+    # Learned chi0 tail: chi0 truncated at 200 of 800 unoccupied bands; 8 atoms
+    # of 104 eigenvector rows and 150 product-basis functions each (nb = 832 >= nbt)
+    "C5": Config("C5", nk=8, nb=832, no=8, nu=800, ni=1200, nf=20, seed=20261207, natom=8, nu_chi0=200,
+                 note="nk, no and the atom blocks are not fixed by the spec; chosen here"),
     # parity-test shapes
     "T1": Config("T1", nk=2, nb=12, no=2, nu=5, ni=16, nf=6, seed=11),
     "T2": Config("T2", nk=3, nb=24, no=3, nu=13, ni=40, nf=10, seed=12),
@@ -125,7 +139,7 @@ class QPoint:
     e_qmk: np.ndarray      # nbt x nk
     c_q: np.ndarray        # nb x nbt
     c_qmk: np.ndarray      # nb x nbt x nk
-    ppb: np.ndarray        # nb x nb x ni (real)
+    ppb: np.ndarray        # nb x nb x ni (real); natom > 1: the atom blocks nv x nv x mdim, concatenated
     vcoud: np.ndarray      # ni
     zcousq: np.ndarray     # ni x ni
     wk: np.ndarray         # nk BZ weights
@@ -141,6 +155,8 @@ def make_qpoint(cfg: Config) -> QPoint:
     nk, nb, nbt, ni = cfg.nk, cfg.nb, cfg.nbt, cfg.ni
     if nb < nbt:
         raise ValueError(f"{cfg.name}: nb={nb} < nbt={nbt}; QR cannot give nbt orthonormal vectors")
+    if nb % cfg.natom or ni % cfg.natom:
+        raise ValueError(f"{cfg.name}: nb and ni must split evenly over {cfg.natom} atoms")
     # energies
     e_k = np.stack([_energies(rng, cfg.no, cfg.nu) for _ in range(nk)], axis=1)
     e_kq = np.stack([_energies(rng, cfg.no, cfg.nu) for _ in range(nk)], axis=1)
@@ -151,9 +167,14 @@ def make_qpoint(cfg: Config) -> QPoint:
     c_kq = np.stack([_qr_unitary(rng, nb, nbt) for _ in range(nk)], axis=2)
     c_q = _qr_unitary(rng, nb, nbt)
     c_qmk = np.stack([_qr_unitary(rng, nb, nbt) for _ in range(nk)], axis=2)
-    # ppb: real Gaussian with variance 1/(ni*Omega)
+    # ppb: real Gaussian with variance natom^2/(ni*Omega): an eigenvector keeps
+    # 1/natom of its norm on each atom, so sum_I |zmel|^2 ~ 1/Omega for any natom
     omega_cell = cfg.alat ** 3 / 4.0
-    ppb = np.asfortranarray(rng.standard_normal((nb, nb, ni)) / math.sqrt(ni * omega_cell))
+    if cfg.natom == 1:
+        ppb = np.asfortranarray(rng.standard_normal((nb, nb, ni)) / math.sqrt(ni * omega_cell))
+    else:
+        nv, md = nb // cfg.natom, ni // cfg.natom
+        ppb = rng.standard_normal(cfg.natom * nv * nv * md) * (cfg.natom / math.sqrt(ni * omega_cell))
     # v(q)
     q = np.array([0.1, 0.2, 0.3])
     g = gvectors(ni)
@@ -201,14 +222,15 @@ class PairWeights:
     whw: np.ndarray # all weights
 
 
-def pair_weights(qp: QPoint, fr: np.ndarray) -> PairWeights:
+def pair_weights(qp: QPoint, fr: np.ndarray, nu_max: int = 0) -> PairWeights:
+    """Pairs (occupied n, unoccupied m <= no + nu_max; nu_max = 0: all)."""
     cfg = qp.cfg
     nwhis = len(fr) - 1
     n1b, n2b, ihw, nhw, jhw = [], [], [], [], []
     whw_parts = []
     ptr = 1
     occ = np.arange(1, cfg.no + 1)
-    unocc = np.arange(cfg.no + 1, cfg.nbt + 1)
+    unocc = np.arange(cfg.no + 1, cfg.no + (nu_max if nu_max else cfg.nu) + 1)
     for k in range(cfg.nk):
         nn, mm = np.meshgrid(occ, unocc, indexing="ij")  # pair order: n outer, m inner
         nn = nn.ravel()

@@ -72,6 +72,22 @@ ecalj_status ecalj_kernel_time(ecalj_handle h, int kclass, double* ms, int64_t* 
  * ecalj_tuning_report writes the decisions so far, one line per shape, as a
  * NUL-terminated string of at most len bytes. */
 ecalj_status ecalj_set_tuning(ecalj_handle h, int mode);
+
+/* HIP graphs of the handle's stream (no counterpart in the reference, which
+ * issues every call eagerly).  Between ecalj_graph_begin and ecalj_graph_end
+ * the _dev entry points are recorded instead of run; ecalj_graph_launch
+ * replays them on the handle's stream.  Capture needs a warm-up: the same
+ * calls at the same sizes must have run once before (scratch may not grow
+ * under capture: ECALJ_ERR_STATE), and the tuner then reuses its choices.
+ * Index lists built on the host at capture time (x0gemm bins, Hilbert
+ * coefficients) are fixed in the graph: a replay repeats the captured
+ * problem on whatever data the device buffers hold.  Kernel timing is off
+ * while capturing.  Fill kernels, not memset nodes, clear memory. */
+typedef struct ecalj_graph_s* ecalj_graph_t;
+ecalj_status ecalj_graph_begin(ecalj_handle h);
+ecalj_status ecalj_graph_end(ecalj_handle h, ecalj_graph_t* graph);
+ecalj_status ecalj_graph_launch(ecalj_handle h, ecalj_graph_t graph);
+ecalj_status ecalj_graph_destroy(ecalj_graph_t graph);
 ecalj_status ecalj_tuning_report(ecalj_handle h, char* buf, int64_t len);
 
 /* ---- device memory ------------------------------------------------------

@@ -42,7 +42,8 @@ def chi0(qp: synth.QPoint, k_subset=None):
     cfg = qp.cfg
     fr = synth.frhis(qp)
     nwhis = len(fr) - 1
-    pw = synth.pair_weights(qp, fr)
+    nuc = cfg.nu_chi0 if cfg.nu_chi0 else cfg.nu   # nband_chi0 truncation (S/m_readgwinput.f90:34-35)
+    pw = synth.pair_weights(qp, fr, nuc)
     ix = synth.x0kf_index(pw)
     _, _, _, _, nu = frequencies(cfg)
     imatt = orc.imatt(nu, fr)
@@ -50,7 +51,8 @@ def chi0(qp: synth.QPoint, k_subset=None):
     rcxq = np.zeros((ni, ni, nwhis + 1), np.complex128, order="F")
     ks = range(cfg.nk) if k_subset is None else k_subset
     for k in ks:
-        z = orc.zmel([cfg.nb], [ni], [1.0], qp.ppb, qp.c_k[:, :no, k], qp.c_kq[:, no:, k], conjg=True)
+        z = orc.zmel(cfg.nv_list, cfg.mdim_list, [1.0] * cfg.natom, qp.ppb, qp.c_k[:, :no, k], qp.c_kq[:, no:no + nuc, k],
+                     conjg=True)
         orc.x0gemm(rcxq, ni, 1, ni, nwhis, 1, int(ix.nkmin[k]), int(ix.nkmax[k]), z,
                    int(ix.icounkmin[k]), int(ix.icounkmax[k]), ix.iwini, ix.iwend, ix.itc, ix.itpc,
                    ix.jpmc, ix.icouini, ix.whwc)
@@ -73,7 +75,7 @@ def sigc(qp: synth.QPoint, wck, k_subset=None):
     omega = qp.e_q[:ntq] / synth.RYDBERG_EV
     ks = range(cfg.nk) if k_subset is None else k_subset
     for k in ks:
-        z = orc.zmel([cfg.nb], [cfg.ni], [1.0], qp.ppb, qp.c_qmk[:, :, k], qp.c_q[:, :ntq], conjg=False)
+        z = orc.zmel(cfg.nv_list, cfg.mdim_list, [1.0] * cfg.natom, qp.ppb, qp.c_qmk[:, :, k], qp.c_q[:, :ntq], conjg=False)
         g = orc.wgtim(omega, qp.e_qmk[:, k] / synth.RYDBERG_EV, cfg.esmr, qp.wk[k], freqx, wx, expa, ua=cfg.ua)
         zsec = orc.sigc_imag(z, wck[k], g, zsec)
     return zsec
