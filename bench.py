@@ -116,6 +116,107 @@ def run_form(qp, precision, steps, warmup):
     return info
 
 
+def bench_c4(mixed, nq=16, log_=None):
+    """Batched uneven workload, one GPU's share: nq q-points of 27 k-points
+    with per-k band windows and per-q nI; FP64 form eager, mixed form as a
+    captured HIP graph per q-point (replayed, as the QSGW iterations do)."""
+    from ecalj_amd import synth
+    from ecalj_amd.batched import UnevenQPointGPU, flops_uneven
+
+    rows = []
+    t64 = tmx = tmx_eager = fl = 0.0
+    err = 0.0
+    for c in synth.c4_qpoints(nq):
+        qp = synth.make_uneven_qpoint(c, no_range=(8, 24), nu_range=(150, 400))
+        res = {}
+        for prec in ("f64", mixed):
+            g = UnevenQPointGPU(qp, prec)
+            g.step()
+            g.make_wk()
+            g.step()                      # warm-up at final shapes (tuner settles)
+            g.h.sync()
+            t0 = time.perf_counter()
+            g.step()
+            g.h.sync()
+            te = time.perf_counter() - t0
+            tg = None
+            if prec != "f64":
+                gr = g.capture_step()
+                gr.launch()
+                g.h.sync()
+                t0 = time.perf_counter()
+                gr.launch()
+                gr.launch()
+                g.h.sync()
+                tg = (time.perf_counter() - t0) / 2
+                gr.close()
+            res[prec] = (te, tg, g.results()[3], g.ix)
+            g.h.close()
+        npb = int(sum(res["f64"][3].iwend - res["f64"][3].iwini + 1))
+        f = flops_uneven(qp, npb)
+        fl += f
+        t64 += res["f64"][0]
+        tmx += res[mixed][1]
+        tmx_eager += res[mixed][0]
+        e = float(np.abs(res[mixed][2] - res["f64"][2]).max() * 1e3)
+        err = max(err, e)
+        rows.append({"ni": c.ni, "f64_s": round(res["f64"][0], 4), "mixed_graph_s": round(res[mixed][1], 4),
+                     "mixed_eager_s": round(res[mixed][0], 4), "qp_err_meV": round(e, 5)})
+        if log_:
+            log_(f"bench[C4]: q nI={c.ni}: f64 {res['f64'][0]:.3f} s, {mixed} graph {res[mixed][1]:.3f} s "
+                 f"(eager {res[mixed][0]:.3f} s), dQP {e:.4f} meV")
+    return {
+        "workload": f"C4 one-GPU share: {nq} q-points, nI ~ U[800,1600] per q, 27 k-points with per-k windows "
+                    f"U{{8..24}} occ x U{{150..400}} unocc, nf=20, ntq=16",
+        "s_per_q": round(tmx / nq, 6), "fp64_form_s_per_q": round(t64 / nq, 6),
+        "mixed_eager_s_per_q": round(tmx_eager / nq, 6),
+        "speedup_vs_fp64_form": round(t64 / tmx, 3), "qp_err_meV_vs_fp64_form": round(err, 5),
+        "tflops": round(fl / tmx / 1e12, 3), "per_q": rows,
+    }
+
+
+def bench_c5(log_=None):
+    """Learned chi0 tail: train on 4096 (truncated, full) pairs from 4 seeded
+    q-points, judge on a held-out one; time the learned form (chi0 truncated
+    at nu_t + tail) against the full band sum, both in the mixed form."""
+    from ecalj_amd import learned_tail as lt
+    from ecalj_amd import synth
+    from ecalj_amd.qsgw import QPointGPU
+
+    cfg = synth.CONFIGS["C5"]
+    m = lt.fit(lt.seeded_qpoints(cfg, 4), cfg.nu_chi0, npairs=4096, epochs=300, log=log_)
+    held = lt.seeded_qpoints(cfg, 1, first=4)
+    j = lt.judge(m, held, tol_mev=1.0, log=log_)
+    qp = held[0]
+    times = {}
+    for name, kw, q in (("full_fp64", {}, synth.dataclasses.replace(qp, cfg=cfg.scaled(nu_chi0=0))),
+                        ("full_mixed", {}, synth.dataclasses.replace(qp, cfg=cfg.scaled(nu_chi0=0))),
+                        ("learned_mixed", {"tail_model": m}, qp)):
+        g = QPointGPU(q, "f64" if name == "full_fp64" else "bf16x3", **kw)
+        g.step()
+        g.make_wk()
+        g.step()
+        g.h.sync()
+        t0 = time.perf_counter()
+        g.step()
+        g.h.sync()
+        times[name] = time.perf_counter() - t0
+        g.h.close()
+    r = j["rows"][0]
+    return {
+        "workload": f"C5: nI={cfg.ni}, nf={cfg.nf}, nk={cfg.nk}, {cfg.no} occ, chi0 truncated at {cfg.nu_chi0} of "
+                    f"{cfg.nu} unocc; MLP {m.dims} trained on {m.info['npairs']} pairs by the library's kernels",
+        "train_s": round(m.info["t_train_s"], 3), "final_loss": m.info["losses"][-1],
+        "qp_err_meV_truncated": round(r["qp_err_truncated_meV"], 3),
+        "qp_err_meV_learned": round(r["qp_err_learned_meV"], 3),
+        "chi_diag_err_truncated": round(r["chi_diag_err_truncated"], 5),
+        "chi_diag_err_learned": round(r["chi_diag_err_learned"], 5),
+        "judged_error_meV": round(j["judged_error_meV"], 3), "accepted": j["accepted"],
+        "form_used": "learned tail" if j["accepted"] else "full band sum (fallback)",
+        "s_per_q": {k: round(v, 5) for k, v in times.items()},
+    }
+
+
 def roofline(fl, info, precision, steps):
     ms, nl = info["ktimes"]["sigc"]
     launches_per_step = max(1, nl // steps)
@@ -141,7 +242,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3",
+    ap.add_argument("--also", default="C3,C4,C5",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--only", choices=["f64", "mixed"], default=None,
@@ -207,6 +308,12 @@ def main():
     if also:
         out["also"] = {}
     for name in also:
+        if name == "C4":
+            out["also"]["C4"] = bench_c4(args.mixed, log_=log)
+            continue
+        if name == "C5":
+            out["also"]["C5"] = bench_c5(log_=log)
+            continue
         c = synth.CONFIGS[name]
         log(f"bench: generating {c.name} inputs")
         q2 = synth.make_qpoint(c)

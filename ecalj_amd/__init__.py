@@ -66,6 +66,7 @@ _SIGS = {
     "ecalj_sigc_weights_dev": [_p, _i, _i, _i, _i, _i, _i, _p, _p, _d, _d, _p, _p, _p, _p, _p],
     "ecalj_sigc": [_p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
     "ecalj_sigc_dev": [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
+    "ecalj_sigc_grouped_dev": [_p, _i, _p, _p, _i, _i, _i, _p, _i, _p, _p, _i],
     "ecalj_qp_energy": [_i, _p, _p, _p, _p, _i, _p, _p],
     "ecalj_dev_alloc": [_p, C.c_size_t, C.POINTER(C.c_void_p)],
     "ecalj_dev_free": [_p, _p],
@@ -451,6 +452,13 @@ class Handle:
         _chk(lib().ecalj_sigc_dev(self._h, nbatch, _t_ptr(zmel), ngb, nm, ntq, niw, _t_ptr(wc),
                                   ngb if ldw is None else ldw, _t_ptr(wgtim), _t_ptr(zsec),
                                   ntq if ldz is None else ldz), "sigc_dev")
+
+    def sigc_grouped_dev(self, nm, zmel, ngb, ntq, niw, wc, wgtim, zsec, ldw=None, ldz=None):
+        """Internal k-points with their own middle-state counts nm[k]; zmel and wgtim packed per k."""
+        n = _i32(nm)
+        _chk(lib().ecalj_sigc_grouped_dev(self._h, len(n), _np_ptr(n), _t_ptr(zmel), ngb, ntq, niw, _t_ptr(wc),
+                                          ngb if ldw is None else ldw, _t_ptr(wgtim), _t_ptr(zsec),
+                                          ntq if ldz is None else ldz), "sigc_grouped_dev")
 
 
 def qp_energy(elda, sex, sexcore, sec, vxc):

@@ -182,6 +182,10 @@ static ecalj_status upload(ecalj_ctx* h, int slot, const std::vector<T>& v, T** 
   return ECALJ_OK;
 }
 
+ecalj_status upload_blob(ecalj_ctx* h, int slot, const std::vector<unsigned char>& v, unsigned char** out) {
+  return upload(h, slot, v, out);
+}
+
 // RAII device buffer for the host-pointer entry points.
 struct DevBuf {
   void* p = nullptr;
@@ -245,6 +249,9 @@ ecalj_status ecalj_create(ecalj_handle* out, int device) {
     return ECALJ_ERR_HIP;
   }
   h->stream = h->own_stream;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    h->ncu = ncu;
   if (hipMalloc(&h->d_flag, sizeof(int)) != hipSuccess) {
     (void)hipStreamDestroy(h->own_stream);
     delete h;
@@ -991,6 +998,15 @@ ecalj_status ecalj_sigc_dev(ecalj_handle h, int nbatch, const void* zmel, int ng
   if (ngb < 1 || nm < 0 || ntq < 0 || niw < 0 || ldw < ngb || ldz < ntq || nbatch < 1) return ECALJ_ERR_DIM;
   return sig_contract(h, h->precision, nbatch, zmel, ngb, nm, ntq, niw, (const cd*)wc, ldw, (const double*)wgtim,
                       (cd*)zsec, ldz);
+}
+
+ecalj_status ecalj_sigc_grouped_dev(ecalj_handle h, int nk, const int* nm, const void* zmel, int ngb, int ntq,
+                                    int niw, const void* wc, int ldw, const void* wgtim, void* zsec, int ldz) {
+  if (!h || !nm || !zmel || !wc || !wgtim || !zsec) return ECALJ_ERR_ARG;
+  if (ngb < 1 || ntq < 0 || niw < 0 || ldw < ngb || ldz < ntq || nk < 1) return ECALJ_ERR_DIM;
+  if (ntq == 0) return ECALJ_OK;
+  return sig_contract_grouped(h, h->precision, nk, nm, zmel, ngb, ntq, niw, (const cd*)wc, ldw,
+                              (const double*)wgtim, (cd*)zsec, ldz);
 }
 
 ecalj_status ecalj_sigc(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, int niw, const double* wc,

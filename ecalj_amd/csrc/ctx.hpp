@@ -9,6 +9,7 @@
 
 struct ecalj_ctx {
   int device = 0;
+  int ncu = 256;                 // compute units of the device (persistent grids)
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   int precision = ECALJ_PREC_F64;
@@ -28,7 +29,7 @@ struct ecalj_ctx {
 
   // grow-only scratch slots
   enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
-         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_NSLOT };
+         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag
@@ -61,9 +62,12 @@ struct ecalj_ctx {
 namespace ecalj {
 
 ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out);
+// Copies a host table to scratch slot `slot` on the handle's stream (pinned
+// staging; capture-safe); *out is its device address.
+ecalj_status upload_blob(ecalj_ctx* h, int slot, const std::vector<unsigned char>& v, unsigned char** out);
 
 // ---- kernel-variant tuner
-enum TuneOp { TUNE_SIGC_BF3 = 0 };
+enum TuneOp { TUNE_SIGC_BF3 = 0, TUNE_SIGC_BF3_GROUPED = 1 };
 typedef std::array<int64_t, 5> TuneKey;
 bool tune_lookup(ecalj_ctx* h, const TuneKey& k, int* var);
 void tune_store(ecalj_ctx* h, const TuneKey& k, int var, const char* op, const std::vector<const char*>& names,

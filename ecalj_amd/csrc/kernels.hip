@@ -572,6 +572,45 @@ __global__ __launch_bounds__(256) void sig_final_reduce_kernel(const T* __restri
   }
 }
 
+// Split-K partials of the final product (S/m_sxcf_gemm.f90:599):
+// part[chunk][b] (ntq x ntq) = cz_b(chunk rows)^H zmel_b(chunk rows) for gb
+// internal k-points of R = n*nm rows each (per-k strides zs), chunks of KC rows.
+constexpr int SIG_KC = 2048;
+static inline int sig_chunks(int64_t R) { return (int)((R + SIG_KC - 1) / SIG_KC); }
+
+template <int MODE, class T>
+static ecalj_status sig_final_partials(ecalj_ctx* h, const T* cz, const T* zmel, int64_t R, int64_t zs, int gb,
+                                       int ntq, T* part) {
+  const int KC = SIG_KC;
+  const int chunks = sig_chunks(R);
+  const int np = ntq * ntq;
+  GemmCall f;
+  f.mode = MODE;
+  f.ta = true; f.ca = true;
+  f.M = ntq; f.N = ntq; f.K = KC;
+  f.A = cz; f.lda = R; f.bA = {KC, zs};
+  f.B = zmel; f.ldb = R; f.bB = {KC, zs};
+  f.C = part; f.ldc = ntq; f.bC = {(int64_t)np, (int64_t)np * chunks};
+  f.batch = chunks * gb; f.div = chunks;
+  if (R % KC != 0) {
+    // last chunk is shorter: run full chunks, then the tail separately
+    f.batch = (chunks - 1) * gb;
+    f.div = chunks - 1;
+    if (chunks > 1) ECALJ_TRY(gemm(h, f));
+    GemmCall t = f;
+    t.K = (int)(R - (int64_t)KC * (chunks - 1));
+    t.A = cz + (int64_t)KC * (chunks - 1);
+    t.B = zmel + (int64_t)KC * (chunks - 1);
+    t.C = part + (int64_t)np * (chunks - 1);
+    t.bA = {0, zs}; t.bB = {0, zs}; t.bC = {0, (int64_t)np * chunks};
+    t.batch = gb; t.div = 1;
+    ECALJ_TRY(gemm(h, t));
+  } else {
+    ECALJ_TRY(gemm(h, f));
+  }
+  return ECALJ_OK;
+}
+
 template <int MODE, class T>
 static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int n, int nm, int ntq, int niw,
                                    const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
@@ -589,8 +628,7 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CZ, cz_bytes_per * group, (void**)&cz));
   // final product: split-K partials of czmelwc^H zmel (K = n*nm per internal k)
   const int64_t R = (int64_t)n * nm;
-  const int KC = 2048;
-  const int chunks = (int)((R + KC - 1) / KC);
+  const int chunks = sig_chunks(R);
   const int np = ntq * ntq;
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_PART, sizeof(T) * (size_t)np * chunks * nbatch, (void**)&part));
   const int64_t wstride = ldw * n * nw1;
@@ -620,35 +658,89 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
       ECALJ_HIP(hipGetLastError());
       ECALJ_TRY(timing_end(h, 0, tev));
     }
-    // partial[chunk][k] (ntq x ntq) = cz_k(chunk rows)^H zmel_k(chunk rows)
-    GemmCall f;
-    f.mode = MODE;
-    f.ta = true; f.ca = true;
-    f.M = ntq; f.N = ntq; f.K = KC;
-    f.A = cz; f.lda = R; f.bA = {KC, (int64_t)n * ncol};
-    f.B = zmel + (int64_t)n * ncol * b0; f.ldb = R; f.bB = {KC, (int64_t)n * ncol};
-    f.C = part + (int64_t)np * chunks * b0; f.ldc = ntq; f.bC = {(int64_t)np, (int64_t)np * chunks};
-    f.batch = chunks * gb; f.div = chunks;
-    if (R % KC != 0) {
-      // last chunk is shorter: run full chunks, then the tail separately
-      f.batch = (chunks - 1) * gb;
-      f.div = chunks - 1;
-      f.bC = {(int64_t)np, (int64_t)np * chunks};
-      if (chunks > 1) ECALJ_TRY(gemm(h, f));
-      GemmCall t = f;
-      t.K = (int)(R - (int64_t)KC * (chunks - 1));
-      t.A = cz + (int64_t)KC * (chunks - 1);
-      t.B = zmel + (int64_t)n * ncol * b0 + (int64_t)KC * (chunks - 1);
-      t.C = part + (int64_t)np * chunks * b0 + (int64_t)np * (chunks - 1);
-      t.bA = {0, (int64_t)n * ncol}; t.bB = {0, (int64_t)n * ncol}; t.bC = {0, (int64_t)np * chunks};
-      t.batch = gb; t.div = 1;
-      ECALJ_TRY(gemm(h, t));
-    } else {
-      ECALJ_TRY(gemm(h, f));
-    }
+    ECALJ_TRY((sig_final_partials<MODE, T>(h, cz, zmel + (int64_t)n * ncol * b0, R, (int64_t)n * ncol, gb, ntq,
+                                            part + (int64_t)np * chunks * b0)));
   }
   hipLaunchKernelGGL((sig_final_reduce_kernel<T>), dim3(np), dim3(256), 0, h->stream, part, chunks * nbatch, ntq,
                      zsec, ldz);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+// Internal k-points with their own middle-state counts nm[k] (host array):
+// zmel_k (n x nm[k] x ntq) and wgt_k ((niw+1) x nm[k] x ntq) packed one after
+// the other, W_c(k) at a uniform stride.  bf16x3: one grouped persistent
+// launch per scratch group, split-K partials per k, one reduction for all k.
+// Other forms: the uniform path per k.
+ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm, const void* zmel, int n, int ntq,
+                                  int niw, const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
+  const int nw1 = niw + 1;
+  const int64_t wstride = ldw * n * nw1;
+  std::vector<int64_t> zoff(nk + 1, 0);
+  for (int k = 0; k < nk; ++k) {
+    if (nm[k] < 0) return ECALJ_ERR_DIM;
+    zoff[k + 1] = zoff[k] + (int64_t)nm[k] * ntq;  // columns before k
+  }
+  if (mode != MBF3) {
+    const size_t es = (mode == M64) ? sizeof(cd) : sizeof(cf);
+    for (int k = 0; k < nk; ++k) {
+      if (nm[k] == 0) continue;
+      ECALJ_TRY(sig_contract(h, mode, 1, (const char*)zmel + es * (size_t)n * zoff[k], n, nm[k], ntq, niw,
+                             wc + wstride * k, ldw, wgt + nw1 * zoff[k], zsec, ldz));
+    }
+    return ECALJ_OK;
+  }
+  const cf* z = (const cf*)zmel;
+  const int np = ntq * ntq;
+  // scratch groups of consecutive k under 8 GB
+  std::vector<int> gstart{0};
+  size_t acc = 0;
+  size_t maxg = 0;
+  for (int k = 0; k < nk; ++k) {
+    const size_t b = sig_bf3_bytes_per_k(n, std::max(nm[k] * ntq, 1), nw1) + sizeof(cf) * (size_t)n * nm[k] * ntq;
+    if (acc > 0 && acc + b > ((size_t)8 << 30)) {
+      gstart.push_back(k);
+      maxg = std::max(maxg, acc);
+      acc = 0;
+    }
+    acc += b;
+  }
+  maxg = std::max(maxg, acc);
+  gstart.push_back(nk);
+  int64_t nparts = 0;
+  for (int k = 0; k < nk; ++k) nparts += nm[k] ? sig_chunks((int64_t)n * nm[k]) : 0;
+  unsigned char* wh;
+  cf* part;
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_WH, maxg, (void**)&wh));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_PART, sizeof(cf) * (size_t)np * std::max<int64_t>(nparts, 1),
+                        (void**)&part));
+  int64_t pofs = 0;
+  for (size_t gi = 0; gi + 1 < gstart.size(); ++gi) {
+    const int k0 = gstart[gi], k1 = gstart[gi + 1];
+    std::vector<int> ncol(k1 - k0);
+    size_t a_elems = sig_bf3_a_elems_per_k(n, nw1) * (k1 - k0);
+    size_t b_bytes = 0;
+    for (int k = k0; k < k1; ++k) {
+      ncol[k - k0] = nm[k] * ntq;
+      b_bytes += sig_bf3_bytes_per_k(n, std::max(ncol[k - k0], 1), nw1) - 2 * sig_bf3_a_elems_per_k(n, nw1);
+    }
+    unsigned short* as = reinterpret_cast<unsigned short*>(wh);
+    unsigned short* bs = as + a_elems;
+    cf* cz = reinterpret_cast<cf*>(reinterpret_cast<unsigned char*>(bs) + b_bytes);
+    ECALJ_TRY(sig_bf3_grouped(h, k1 - k0, ncol.data(), z + (int64_t)n * zoff[k0], n, nw1, wc + wstride * k0, ldw,
+                              wstride, wgt + nw1 * zoff[k0], cz, as, bs));
+    int64_t cofs = 0;
+    for (int k = k0; k < k1; ++k) {
+      if (nm[k] == 0) continue;
+      const int64_t R = (int64_t)n * nm[k];
+      ECALJ_TRY((sig_final_partials<MBF3, cf>(h, cz + cofs, z + (int64_t)n * zoff[k], R, 0, 1, ntq,
+                                              part + (int64_t)np * pofs)));
+      cofs += (int64_t)n * nm[k] * ntq;
+      pofs += sig_chunks(R);
+    }
+  }
+  hipLaunchKernelGGL((sig_final_reduce_kernel<cf>), dim3(np), dim3(256), 0, h->stream, part, (int)pofs, ntq, zsec,
+                     ldz);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }

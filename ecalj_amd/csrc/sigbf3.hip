@@ -15,6 +15,8 @@
 // bytes 16i..16i+15 of a 1 KB run), three stages in flight, and the chunk
 // order makes the MFMA operand reads (16 rows x one chunk) bank-conflict free.
 #include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "ctx.hpp"
 #include "gemm.hpp"
@@ -137,33 +139,28 @@ struct alignas(16) SigSmem {
 // expcnt [6:4] and lgkmcnt [11:8] left at their maximum)
 #define ECALJ_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(((N) & 0xF) | (((N) >> 4) << 14) | (0x7 << 4) | (0xF << 8))
 
-// Block tile QBM x BN, four waves in 2 x 2, wave tile 64 x BN/2; NSTG stages of
-// QBK in LDS, NSTG-1 of them in flight.
+// One block tile of czmelwc: rows I0..I0+QBM of one internal k, columns
+// c0..c0+BN; four waves in 2 x 2, wave tile 64 x BN/2; NSTG stages of QBK in
+// LDS, NSTG-1 of them in flight.  A and B point at the k's plane sets
+// (A: 4 planes of MP x QBK per (v, ks); B: 4 planes of NCP x QBK per ks).
 template <int BN, int NSTG>
-__global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
-                                                              int64_t as_stride,
-                                                              const unsigned short* __restrict__ bs,
-                                                              int64_t bs_stride, const double* __restrict__ wgt,
-                                                              int64_t g_stride, int n, int nw1, int v0, int nwg,
-                                                              int ncol, int MP, int NS, int NCP, int accumulate,
-                                                              cf* __restrict__ cz, int64_t cz_stride) {
+__device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned short* __restrict__ Ak,
+                                         const unsigned short* __restrict__ Bk, const double* __restrict__ g,
+                                         int n, int nw1, int v0, int nwg, int ncol, int MP, int NS, int NCP,
+                                         int I0, int c0, int accumulate, cf* __restrict__ C) {
   static_assert(NSTG == 2 || NSTG == 3, "two or three stages");
   constexpr int TM = 4, TN = BN / 32;               // 16x16 tiles per wave
   constexpr int ARW = 4 * QBM * QBK * 2 / 1024 / 4;  // 1 KB A runs per wave and stage
   constexpr int BRW = 4 * BN * QBK * 2 / 1024 / 4;   // 1 KB B runs per wave and stage
   constexpr int APP = QBM * QBK / 512, BPP = BN * QBK / 512;  // runs per plane
-  __shared__ SigSmem<BN, NSTG> s;
-  const int b = blockIdx.z;
-  const int I0 = blockIdx.x * QBM, c0 = blockIdx.y * BN;
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave >> 1, wn = wave & 1;
   const int64_t apst = (int64_t)MP * QBK, bpst = (int64_t)NCP * QBK;
-  const unsigned short* A = as + as_stride * b + (int64_t)I0 * QBK + lane * 8;
-  const unsigned short* B = bs + bs_stride * b + (int64_t)c0 * QBK + lane * 8;
+  const unsigned short* A = Ak + (int64_t)I0 * QBK + lane * 8;
+  const unsigned short* B = Bk + (int64_t)c0 * QBK + lane * 8;
   const int nst = nwg * NS;
 
   // weight table: wt[v][c] = wgt(v0+v, c0+c) (0 beyond ncol)
-  const double* g = wgt + g_stride * b;
   for (int e = t; e < nwg * BN; e += kThreads) {
     const int v = e / BN, c = c0 + e % BN;
     s.wt[v][e % BN] = (c < ncol) ? (float)g[(v0 + v) + (int64_t)nw1 * c] : 0.f;
@@ -270,7 +267,6 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
       }
     }
   }
-  cf* C = cz + cz_stride * b;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -289,6 +285,49 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
           }
         }
       }
+}
+
+// Uniform batch: block (x, y, z) = (row tile, column tile, internal k).
+template <int BN, int NSTG>
+__global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
+                                                              int64_t as_stride,
+                                                              const unsigned short* __restrict__ bs,
+                                                              int64_t bs_stride, const double* __restrict__ wgt,
+                                                              int64_t g_stride, int n, int nw1, int v0, int nwg,
+                                                              int ncol, int MP, int NS, int NCP, int accumulate,
+                                                              cf* __restrict__ cz, int64_t cz_stride) {
+  __shared__ SigSmem<BN, NSTG> s;
+  const int b = blockIdx.z;
+  sig_tile<BN, NSTG>(s, as + as_stride * b, bs + bs_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS,
+                     NCP, blockIdx.x * QBM, blockIdx.y * BN, accumulate, cz + cz_stride * b);
+}
+
+// Grouped, persistent: internal k-points with different column counts.  The
+// tile list holds (k, row tile, column tile) for every k; each block walks it
+// with a grid stride (all tiles cost the same) until the list is exhausted.
+struct SigGroup {
+  int64_t boff;   // B planes of k (elements)
+  int64_t goff;   // weights of k (elements)
+  int64_t czoff;  // output of k (elements)
+  int ncol, ncp;  // columns, padded columns
+};
+template <int BN, int NSTG>
+__global__ __launch_bounds__(kThreads, 1) void sig_bf3_grouped_kernel(const int* __restrict__ tiles, int ntiles,
+                                                                      const SigGroup* __restrict__ grp,
+                                                                      const unsigned short* __restrict__ as,
+                                                                      int64_t as_stride,
+                                                                      const unsigned short* __restrict__ bs,
+                                                                      const double* __restrict__ wgt, int n, int nw1,
+                                                                      int v0, int nwg, int MP, int NS, int accumulate,
+                                                                      cf* __restrict__ cz) {
+  __shared__ SigSmem<BN, NSTG> s;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int k = tiles[3 * t], it = tiles[3 * t + 1], jt = tiles[3 * t + 2];
+    const SigGroup gk = grp[k];
+    __syncthreads();  // the previous tile's LDS reads are done before this tile's loads land
+    sig_tile<BN, NSTG>(s, as + as_stride * k, bs + gk.boff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, gk.ncp,
+                       it * QBM, jt * BN, accumulate, cz + gk.czoff);
+  }
 }
 
 // ---------------------------------------------------------------- launcher
@@ -359,6 +398,96 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
     std::vector<const char*> names;
     for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
     tune_store(h, key, var, "sigc_bf3", names, ms);
+    return ECALJ_OK;
+  }
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 0, &tev));
+  ECALJ_TRY(run(var));
+  ECALJ_TRY(timing_end(h, 0, tev));
+  return ECALJ_OK;
+}
+
+// Grouped form: nk internal k-points with column counts ncol[k] (host); zmel,
+// wgt and cz packed per k (n x ncol[k], nw1 x ncol[k], n x ncol[k]).
+ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zmel, int n, int nw1, const cd* wc,
+                             int64_t ldw, int64_t w_stride, const double* wgt, cf* cz, unsigned short* as,
+                             unsigned short* bs) {
+  if (n < 1 || nw1 < 1 || nk < 1) return ECALJ_ERR_DIM;
+  const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NS = NP / QBK;
+  const int64_t as_stride = 4LL * nw1 * MP * NP;
+  std::vector<SigGroup> grp(nk);
+  int64_t bo = 0, go = 0, co = 0, cols = 0;
+  for (int k = 0; k < nk; ++k) {
+    if (ncol[k] < 0) return ECALJ_ERR_DIM;
+    grp[k].boff = bo;
+    grp[k].goff = go;
+    grp[k].czoff = co;
+    grp[k].ncol = ncol[k];
+    grp[k].ncp = pad_to(std::max(ncol[k], 1), QBN_MAX);
+    bo += 4LL * grp[k].ncp * NP;
+    go += (int64_t)nw1 * ncol[k];
+    co += (int64_t)n * ncol[k];
+    cols += ncol[k];
+  }
+  dim3 gw(NS, MP / SPT, nw1 * nk);
+  if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
+  hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
+  ECALJ_HIP(hipGetLastError());
+  for (int k = 0; k < nk; ++k) {
+    if (ncol[k] == 0) continue;
+    const int64_t totz = (int64_t)NS * grp[k].ncp * 4;
+    const int bz = (int)std::min<int64_t>((totz + 255) / 256, 65536);
+    hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel + grp[k].czoff, (int64_t)0, n,
+                       ncol[k], grp[k].ncp, NS, bs + grp[k].boff, (int64_t)0, 1);
+    ECALJ_HIP(hipGetLastError());
+  }
+  if (cols == 0) return ECALJ_OK;
+  // table: [SigGroup x nk][tiles: (k, row tile, column tile) x ntiles]
+  auto run = [&](int var) -> ecalj_status {
+    const int bn = kSigVariants[var].bn;
+    std::vector<int> tiles;
+    for (int k = 0; k < nk; ++k)
+      for (int it = 0; it < MP / QBM; ++it)
+        for (int jt = 0; jt < (ncol[k] + bn - 1) / bn; ++jt) {
+          tiles.push_back(k);
+          tiles.push_back(it);
+          tiles.push_back(jt);
+        }
+    const int ntiles = (int)(tiles.size() / 3);
+    std::vector<unsigned char> blob(sizeof(SigGroup) * nk + sizeof(int) * tiles.size());
+    memcpy(blob.data(), grp.data(), sizeof(SigGroup) * nk);
+    memcpy(blob.data() + sizeof(SigGroup) * nk, tiles.data(), sizeof(int) * tiles.size());
+    unsigned char* d;
+    ECALJ_TRY(upload_blob(h, ecalj_ctx::S_GTAB, blob, &d));
+    const SigGroup* dg = reinterpret_cast<const SigGroup*>(d);
+    const int* dt = reinterpret_cast<const int*>(d + sizeof(SigGroup) * nk);
+    const int grid = std::max(1, std::min(ntiles, h->ncu));
+    for (int v0 = 0; v0 < nw1; v0 += QNW) {
+      const int nwg = std::min(QNW, nw1 - v0);
+      if (var == 0)
+        hipLaunchKernelGGL((sig_bf3_grouped_kernel<64, 3>), dim3(grid), dim3(kThreads), 0, h->stream, dt, ntiles, dg,
+                           as, as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
+      else
+        hipLaunchKernelGGL((sig_bf3_grouped_kernel<128, 2>), dim3(grid), dim3(kThreads), 0, h->stream, dt, ntiles,
+                           dg, as, as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
+      ECALJ_HIP(hipGetLastError());
+    }
+    return ECALJ_OK;
+  };
+  int var = -1;
+  const TuneKey key{TUNE_SIGC_BF3_GROUPED, n, cols, nw1, nk};
+  if (h->tune_mode > 0) {
+    var = std::min(h->tune_mode, kNumSigVariants) - 1;
+  } else if (tune_lookup(h, key, &var)) {
+  } else if (h->capturing) {
+    var = 0;
+  } else {
+    std::vector<double> ms(kNumSigVariants);
+    for (int v = 0; v < kNumSigVariants; ++v) ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
+    var = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    std::vector<const char*> names;
+    for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
+    tune_store(h, key, var, "sigc_bf3_grouped", names, ms);
     return ECALJ_OK;
   }
   int tev = -1;

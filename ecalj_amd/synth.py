@@ -91,6 +91,11 @@ CONFIGS = {
     # of 104 eigenvector rows and 150 product-basis functions each (nb = 832 >= nbt)
     "C5": Config("C5", nk=8, nb=832, no=8, nu=800, ni=1200, nf=20, seed=20261207, natom=8, nu_chi0=200,
                  note="nk, no and the atom blocks are not fixed by the spec; chosen here"),
+    # Batched uneven workload, one GPU's share: 27 of the 216 k-points (6x6x6),
+    # per-k windows of U{8..24} occupied and U{150..400} unoccupied bands, nI per
+    # q-point from U[800, 1600] (c4_qpoints); nb = 8 atoms x 54 >= 424 bands
+    "C4": Config("C4", nk=27, nb=432, no=24, nu=400, ni=1200, nf=20, seed=20261206, natom=8, ntq=16,
+                 note="per-GPU share (27 k); nI drawn per q-point"),
     # parity-test shapes
     "T1": Config("T1", nk=2, nb=12, no=2, nu=5, ni=16, nf=6, seed=11),
     "T2": Config("T2", nk=3, nb=24, no=3, nu=13, ni=40, nf=10, seed=12),
@@ -222,16 +227,28 @@ class PairWeights:
     whw: np.ndarray # all weights
 
 
+def band_windows(qp: QPoint):
+    """Per-k (occupied, unoccupied) band counts: uniform (no, nu) unless the
+    q-point carries uneven windows (qp.extra['no_k'], qp.extra['nu_k'])."""
+    cfg = qp.cfg
+    no_k = qp.extra.get("no_k", np.full(cfg.nk, cfg.no))
+    nu_k = qp.extra.get("nu_k", np.full(cfg.nk, cfg.nu))
+    return np.asarray(no_k, int), np.asarray(nu_k, int)
+
+
 def pair_weights(qp: QPoint, fr: np.ndarray, nu_max: int = 0) -> PairWeights:
-    """Pairs (occupied n, unoccupied m <= no + nu_max; nu_max = 0: all)."""
+    """Pairs (occupied n <= no_k, unoccupied no_k < m <= no_k + min(nu_k, nu_max); nu_max = 0: all)."""
     cfg = qp.cfg
     nwhis = len(fr) - 1
     n1b, n2b, ihw, nhw, jhw = [], [], [], [], []
     whw_parts = []
     ptr = 1
-    occ = np.arange(1, cfg.no + 1)
-    unocc = np.arange(cfg.no + 1, cfg.no + (nu_max if nu_max else cfg.nu) + 1)
+    no_k, nu_k = band_windows(qp)
     for k in range(cfg.nk):
+        nok = int(no_k[k])
+        nuk = int(min(nu_k[k], nu_max) if nu_max else nu_k[k])
+        occ = np.arange(1, nok + 1)
+        unocc = np.arange(nok + 1, nok + nuk + 1)
         nn, mm = np.meshgrid(occ, unocc, indexing="ij")  # pair order: n outer, m inner
         nn = nn.ravel()
         mm = mm.ravel()
@@ -321,6 +338,35 @@ def x0kf_index(pw: PairWeights, nctot: int = 0, npm: int = 1) -> X0Index:
     return X0Index(cat(kc, np.int32), cat(itc, np.int32), cat(itpc, np.int32), cat(jpmc, np.int32),
                    cat(iwini, np.int32), cat(iwend, np.int32), cat(icouini, np.int32),
                    cat(whwc, np.float64), icounkmin, icounkmax, nkmin, nkmax, nkqmin, nkqmax)
+
+
+def make_uneven_qpoint(cfg: Config, no_range=(8, 24), nu_range=(150, 400)) -> QPoint:
+    """A q-point whose k-points have their own band windows: occupied counts
+    drawn from U{no_range}, unoccupied from U{nu_range} (per k, seeded), the
+    uneven per-k ranges that ecalj's x0kf_v4hz_init and sxcf_scz_count
+    handle (nkmin..nkqmax per k, S/x0kf_v4h.f90:91-137).  cfg.no / cfg.nu
+    must be the range maxima; the k-th window uses the first no_k + nu_k
+    bands, occupied energies for the first no_k of them."""
+    assert cfg.no == no_range[1] and cfg.nu == nu_range[1]
+    qp = make_qpoint(cfg)
+    rng = np.random.Generator(np.random.PCG64(cfg.seed + 3000))
+    no_k = rng.integers(no_range[0], no_range[1] + 1, cfg.nk)
+    nu_k = rng.integers(nu_range[0], nu_range[1] + 1, cfg.nk)
+    for k in range(cfg.nk):  # energies of the k-th window: no_k occupied, then unoccupied
+        for e in (qp.e_k, qp.e_kq, qp.e_qmk):
+            e[:, k] = _energies(rng, int(no_k[k]), cfg.nbt - int(no_k[k]))
+    qp.extra["no_k"] = no_k
+    qp.extra["nu_k"] = nu_k
+    return qp
+
+
+def c4_qpoints(nq: int = 16, cfg: Config = None):
+    """Configs of the batched uneven workload's q-points: nI from U[800, 1600]
+    rounded to a multiple of 8 (the atom count), one seed per q-point."""
+    cfg = cfg or CONFIGS["C4"]
+    rng = np.random.Generator(np.random.PCG64(cfg.seed))
+    nis = (rng.integers(800, 1601, nq) // cfg.natom) * cfg.natom
+    return [cfg.scaled(ni=int(n), seed=cfg.seed + 1 + i) for i, n in enumerate(nis)]
 
 
 def sigma_inputs(qp: QPoint, seed_offset: int = 1000):

@@ -236,6 +236,17 @@ ecalj_status ecalj_sigc(ecalj_handle h, const double* zmel, int ngb, int nm, int
 ecalj_status ecalj_sigc_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm,
                             int ntq, int niw, const void* wc, int ldw, const void* wgtim,
                             void* zsec, int ldz);
+/* Sigma_c over internal k-points whose middle-state windows differ, the
+ * per-k band ranges that sxcf_scz_count assigns (nstti/nstte,
+ * S/sxcf_fal2_count.sc.f90:118-236), in one call: k has nm[k] middle states
+ * (host array); zmel_k (ngb x nm[k] x ntq, precision storage type) and
+ * wgtim_k ((niw+1) x nm[k] x ntq, FP64) are packed one after the other; W_c(k)
+ * (ldw x ngb x (niw+1), complex128) are consecutive.  zsec accumulates all
+ * k, then Im(zsec_mm) <= 0.  The bf16x3 form runs one grouped persistent
+ * kernel over the (k, row tile, column tile) list. */
+ecalj_status ecalj_sigc_grouped_dev(ecalj_handle h, int nk, const int* nm, const void* zmel, int ngb,
+                                    int ntq, int niw, const void* wc, int ldw, const void* wgtim,
+                                    void* zsec, int ldz);
 
 /* ---- QP update (host, FP64) -------------------------------------------
  * S/main_hqpe.sc.f90:152-161: eqp = elda + (sex + sexcore)*hartree

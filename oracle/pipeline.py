@@ -50,9 +50,11 @@ def chi0(qp: synth.QPoint, k_subset=None):
     ni, no = cfg.ni, cfg.no
     rcxq = np.zeros((ni, ni, nwhis + 1), np.complex128, order="F")
     ks = range(cfg.nk) if k_subset is None else k_subset
+    no_k, nu_k = synth.band_windows(qp)
     for k in ks:
-        z = orc.zmel(cfg.nv_list, cfg.mdim_list, [1.0] * cfg.natom, qp.ppb, qp.c_k[:, :no, k], qp.c_kq[:, no:no + nuc, k],
-                     conjg=True)
+        nok, nuk = int(no_k[k]), int(min(nu_k[k], nuc))
+        z = orc.zmel(cfg.nv_list, cfg.mdim_list, [1.0] * cfg.natom, qp.ppb, qp.c_k[:, :nok, k],
+                     qp.c_kq[:, nok:nok + nuk, k], conjg=True)
         orc.x0gemm(rcxq, ni, 1, ni, nwhis, 1, int(ix.nkmin[k]), int(ix.nkmax[k]), z,
                    int(ix.icounkmin[k]), int(ix.icounkmax[k]), ix.iwini, ix.iwend, ix.itc, ix.itpc,
                    ix.jpmc, ix.icouini, ix.whwc)
@@ -74,9 +76,12 @@ def sigc(qp: synth.QPoint, wck, k_subset=None):
     zsec = np.zeros((ntq, ntq), np.complex128, order="F")
     omega = qp.e_q[:ntq] / synth.RYDBERG_EV
     ks = range(cfg.nk) if k_subset is None else k_subset
+    no_k, nu_k = synth.band_windows(qp)
     for k in ks:
-        z = orc.zmel(cfg.nv_list, cfg.mdim_list, [1.0] * cfg.natom, qp.ppb, qp.c_qmk[:, :, k], qp.c_q[:, :ntq], conjg=False)
-        g = orc.wgtim(omega, qp.e_qmk[:, k] / synth.RYDBERG_EV, cfg.esmr, qp.wk[k], freqx, wx, expa, ua=cfg.ua)
+        nbk = int(no_k[k] + nu_k[k])  # middle states of internal k
+        z = orc.zmel(cfg.nv_list, cfg.mdim_list, [1.0] * cfg.natom, qp.ppb, qp.c_qmk[:, :nbk, k], qp.c_q[:, :ntq],
+                     conjg=False)
+        g = orc.wgtim(omega, qp.e_qmk[:nbk, k] / synth.RYDBERG_EV, cfg.esmr, qp.wk[k], freqx, wx, expa, ua=cfg.ua)
         zsec = orc.sigc_imag(z, wck[k], g, zsec)
     return zsec
 
