@@ -11,7 +11,8 @@ module ecalj_gpu
   public :: ecalj_set_ppb, ecalj_set_coulomb, ecalj_zmel, ecalj_x0gemm, ecalj_dpsion_imag
   public :: ecalj_wc_build, ecalj_sigc_weights, ecalj_sigc, ecalj_qp_energy, ecalj_sigma_herm
   public :: ecalj_zmel_dev, ecalj_x0gemm_batched_dev, ecalj_dpsion_imag_dev, ecalj_wc_build_dev
-  public :: ecalj_sigc_weights_dev, ecalj_sigc_dev
+  public :: ecalj_sigc_weights_dev, ecalj_sigc_dev, ecalj_sigc_grouped_dev
+  public :: ecalj_set_tuning, ecalj_graph_begin, ecalj_graph_end, ecalj_graph_launch, ecalj_graph_destroy
   integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
 
   interface
@@ -156,6 +157,37 @@ module ecalj_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: h, zmel, wc, wgtim, zsec
       integer(c_int), value :: nbatch, ngb, nm, ntq, niw, ldw, ldz
+    end function
+    ! Sigma_c with per-k middle-state windows (S/sxcf_fal2_count.sc.f90:118-236)
+    integer(c_int) function ecalj_sigc_grouped_dev(h, nk, nm, zmel, ngb, ntq, niw, wc, ldw, wgtim, zsec, ldz) &
+        bind(C, name="ecalj_sigc_grouped_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, zmel, wc, wgtim, zsec
+      integer(c_int), value :: nk, ngb, ntq, niw, ldw, ldz
+      integer(c_int), intent(in) :: nm(*)
+    end function
+    ! kernel-variant tuner and HIP graphs (no reference counterpart)
+    integer(c_int) function ecalj_set_tuning(h, mode) bind(C, name="ecalj_set_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode
+    end function
+    integer(c_int) function ecalj_graph_begin(h) bind(C, name="ecalj_graph_begin")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function ecalj_graph_end(h, g) bind(C, name="ecalj_graph_end")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      type(c_ptr), intent(out) :: g
+    end function
+    integer(c_int) function ecalj_graph_launch(h, g) bind(C, name="ecalj_graph_launch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, g
+    end function
+    integer(c_int) function ecalj_graph_destroy(g) bind(C, name="ecalj_graph_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
     end function
     ! QP energy (S/main_hqpe.sc.f90:152-161) and hermitised Sigma (:175-181)
     integer(c_int) function ecalj_qp_energy(ntq, elda, sex, sexcore, sec, ldsec, vxc, eqp) &
