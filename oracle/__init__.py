@@ -344,3 +344,26 @@ def silu(z):
 
 def dsilu(z):
     return _mlp_sigs().orc_dsilu(float(z))
+
+
+def sigc_real(zmel_, wr, freq_r, omega, ekc, ef, esmr, wkkr, nctot=0, ns2r=None, zsec=None):
+    """Real-axis pole term of Sigma_c for one internal k (oracle.c orc_sigc_real).
+    zmel: ngb x nm x ntq; wr: ngb x ngb x (nw+1); freq_r: nw+1 (Hartree); energies Ry.
+    Returns (zsec, number of entries)."""
+    z = _cz(zmel_)
+    w = _cz(wr)
+    ngb, nm, ntq = z.shape
+    nw = w.shape[2] - 1
+    fr = _cd(freq_r)
+    assert fr.size == nw + 1
+    if zsec is None:
+        zsec = np.zeros((ntq, ntq), np.complex128, order="F")
+    L = lib()
+    L.orc_sigc_real.restype = C.c_int
+    L.orc_sigc_real.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double,
+                                C.c_double, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
+    ne = L.orc_sigc_real(ngb, nm, ntq, _p(z), _p(w), w.shape[0], nw, _p(fr), _p(_cd(omega)), _p(_cd(ekc)), ef, esmr,
+                         nctot, nm if ns2r is None else ns2r, wkkr, _p(zsec), zsec.shape[0])
+    if ne < 0:
+        raise ValueError("sigc_real: a pole frequency lies outside the real mesh")
+    return zsec, ne

@@ -584,6 +584,132 @@ void orc_sigc_imag(int ngb, int nm, int ntq, int niw, const double complex* zmel
   free(cz);
 }
 
+/* Real-axis pole term of Sigma_c for one internal k (S/m_sxcf_gemm.f90:501-594).
+ * Entry lists as the reference builds them (:512-552): for each external
+ * state itp and middle state it in the window [itini, itend] (:514-515), the
+ * pole weight wfac = wfacx2(omg, ef, ekc(it), smearing) * wkkr * sign(omg-ef)
+ * (skipped below 1e-8), the frequency we = |omg - weavx2(...)|/2 (Hartree),
+ * the first mesh point ixs with freq_r(ixs) > we, and the 3-point weights
+ * wfac * [1, we^2, we^4] * inverse([1, xi^2, xi^4]) of xi = freq_r(ixs-1..ixs+1)
+ * (:543-548).  Then for every real frequency iw in increasing order, the
+ * entries' columns get czmelwc(:,it,itp) += wgt * Wt(iw)^H zmel(:,it,itp)
+ * (:556-590) with the keepwv Hermitian packing of W (:560-566), and
+ * zsec += czmelwc^H zmel (:599), then Im(zsec_nn) <= 0.
+ * ekc(1..nm): middle-state energies (Ry), the first nctot are core; omega:
+ * ntq external energies (Ry); ef, esmr in Ry; freq_r(0..nw) in Hartree;
+ * wr: ldw x ngb x (nw+1).  Returns the number of entries, or -1 if a pole
+ * frequency falls outside freq_r(1..nw-1). */
+static void inv33(const double a[3][3], double r[3][3]) {
+  /* adjugate / determinant (a[i][j] = row i, column j) */
+  double c[3][3];
+  c[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+  c[0][1] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+  c[0][2] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+  c[1][0] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+  c[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+  c[1][2] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+  c[2][0] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+  c[2][1] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+  c[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+  double det = a[0][0] * c[0][0] + a[0][1] * c[1][0] + a[0][2] * c[2][0];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r[i][j] = c[i][j] / det;
+}
+
+int orc_sigc_real(int ngb, int nm, int ntq, const double complex* zmel, const double complex* wr, int ldw, int nw,
+                  const double* freq_r, const double* omega, const double* ekc, double ef, double esmr, int nctot,
+                  int ns2r, double wkkr, double complex* zsec, int ldz) {
+  const double ddw = 10.0, wfaccut = 1e-8;
+  int nt0p = 0, nt0m = 0;
+  for (int i = 0; i < nm; ++i) {
+    if (ekc[i] < ef + ddw * esmr) nt0p++;
+    if (ekc[i] < ef - ddw * esmr) nt0m++;
+  }
+  const long cap = 3L * nm * ntq + 3;
+  int* e_iw = (int*)malloc(sizeof(int) * cap);
+  int* e_col = (int*)malloc(sizeof(int) * cap);
+  double* e_w = (double*)malloc(sizeof(double) * cap);
+  long ne = 0;
+  for (int itp = 0; itp < ntq; ++itp) {
+    const double omg = omega[itp];
+    const int itini = (omg >= ef) ? (nt0m + 1 > 1 ? nt0m + 1 : 1) : 1;
+    const int itend = (omg >= ef) ? ns2r : (nt0p < ns2r ? nt0p : ns2r);
+    for (int it = itini; it <= itend; ++it) {
+      double wfac = orc_wfacx2(omg, ef, ekc[it - 1], it <= nctot ? 0.0 : esmr);
+      if (wfac < wfaccut) continue;
+      wfac *= wkkr * ((omg - ef) >= 0.0 ? 1.0 : -1.0);
+      const double we = 0.5 * fabs(omg - orc_weavx2(omg, ef, ekc[it - 1], esmr));
+      int ixs = 0;
+      for (int i = 1; i <= nw; ++i)
+        if (freq_r[i] > we) {
+          ixs = i;
+          break;
+        }
+      if (ixs < 1 || ixs + 1 > nw) {
+        free(e_iw);
+        free(e_col);
+        free(e_w);
+        return -1;
+      }
+      double a[3][3], ai[3][3];
+      for (int i = 0; i < 3; ++i) {
+        const double xi = freq_r[ixs - 1 + i];
+        a[i][0] = 1.0;
+        a[i][1] = xi * xi;
+        a[i][2] = xi * xi * xi * xi;
+      }
+      inv33(a, ai);
+      const double xv[3] = {1.0, we * we, we * we * we * we};
+      for (int j = 0; j < 3; ++j) {
+        double g = 0.0;
+        for (int i = 0; i < 3; ++i) g += xv[i] * ai[i][j];
+        e_iw[ne] = ixs - 1 + j;
+        e_col[ne] = (it - 1) + nm * itp;
+        e_w[ne] = wfac * g;
+        ne++;
+      }
+    }
+  }
+  const long n2 = (long)ngb * ngb;
+  double complex* w = (double complex*)malloc(sizeof(double complex) * n2);
+  double complex* cz = (double complex*)calloc((size_t)ngb * nm * ntq, sizeof(double complex));
+  for (int iw = 0; iw <= nw; ++iw) {
+    int any = 0;
+    for (long e = 0; e < ne; ++e) any |= (e_iw[e] == iw);
+    if (!any) continue;
+    const double complex* src = wr + (long)ldw * ngb * iw;
+    for (int j = 0; j < ngb; ++j)
+      for (int i = 0; i <= j; ++i) {
+        double complex u = src[i + (long)ldw * j];
+        w[i + (long)ngb * j] = u;
+        w[j + (long)ngb * i] = conj(u);
+      }
+    for (long e = 0; e < ne; ++e) {
+      if (e_iw[e] != iw) continue;
+      const double complex* z = zmel + (long)ngb * e_col[e];
+      double complex* c = cz + (long)ngb * e_col[e];
+      for (int i = 0; i < ngb; ++i) {
+        double complex s = 0.0;
+        for (int k = 0; k < ngb; ++k) s += conj(w[k + (long)ngb * i]) * (e_w[e] * z[k]);
+        c[i] += s;
+      }
+    }
+  }
+  orc_zgemm('C', 'N', ntq, ntq, ngb * nm, 1.0, cz, (long)ngb * nm, zmel, (long)ngb * nm, 1.0, zsec, ldz);
+  for (int i = 0; i < ntq; ++i) {
+    double complex z = zsec[i + (long)ldz * i];
+    double zi = cimag(z);
+    if (zi > 0.0) zi = 0.0;
+    zsec[i + (long)ldz * i] = creal(z) + I * zi;
+  }
+  free(w);
+  free(cz);
+  free(e_iw);
+  free(e_col);
+  free(e_w);
+  return (int)ne;
+}
+
 /* wfacx2 / weavx2 (S/wfacx.f90:19-49). */
 double orc_wfacx2(double e1, double e2, double ek, double esmr) {
   const double ewidthcut = 1e-6;

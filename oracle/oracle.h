@@ -122,6 +122,12 @@ void orc_sigc_imag(int ngb, int nm, int ntq, int niw, const double complex* zmel
                    const double complex* wc, int ldw, const double* wgtim,
                    double complex* zsec, int ldz);
 
+/* Real-axis pole term of Sigma_c for one internal k (S/m_sxcf_gemm.f90:501-594,
+ * 599-608); see oracle.c.  Returns the number of (iw, column) entries or -1. */
+int orc_sigc_real(int ngb, int nm, int ntq, const double complex* zmel, const double complex* wr, int ldw, int nw,
+                  const double* freq_r, const double* omega, const double* ekc, double ef, double esmr, int nctot,
+                  int ns2r, double wkkr, double complex* zsec, int ldz);
+
 /* wfacx2 / weavx2 (S/wfacx.f90:19-49). */
 double orc_wfacx2(double e1, double e2, double ek, double esmr);
 double orc_weavx2(double e1, double e2, double ek, double esmr);
