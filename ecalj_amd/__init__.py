@@ -67,6 +67,8 @@ _SIGS = {
     "ecalj_sigc": [_p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
     "ecalj_sigc_dev": [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
     "ecalj_sigc_grouped_dev": [_p, _i, _p, _p, _i, _i, _i, _p, _i, _p, _p, _i],
+    "ecalj_sigc_real_dev": [_p, _i, _p, _i, _i, _i, _p, _i, _l, _i, _p, _p, _p, _d, _d, _i, _i, _p, _p, _i],
+    "ecalj_sigc_real": [_p, _p, _i, _i, _i, _p, _i, _i, _p, _p, _p, _d, _d, _i, _i, _d, _p, _i],
     "ecalj_qp_energy": [_i, _p, _p, _p, _p, _i, _p, _p],
     "ecalj_dev_alloc": [_p, C.c_size_t, C.POINTER(C.c_void_p)],
     "ecalj_dev_free": [_p, _p],
@@ -452,6 +454,26 @@ class Handle:
         _chk(lib().ecalj_sigc_dev(self._h, nbatch, _t_ptr(zmel), ngb, nm, ntq, niw, _t_ptr(wc),
                                   ngb if ldw is None else ldw, _t_ptr(wgtim), _t_ptr(zsec),
                                   ntq if ldz is None else ldz), "sigc_dev")
+
+    def sigc_real(self, zmel, wr, freq_r, omega, ekc, ef, esmr, wkkr, nctot=0, ns2r=None, zsec=None):
+        """Real-axis pole term (S/m_sxcf_gemm.f90:501-608), one internal k, host arrays; returns zsec."""
+        z, w = _f(zmel, np.complex128), _f(wr, np.complex128)
+        ngb, nm, ntq = z.shape
+        nw = w.shape[2] - 1
+        s = np.zeros((ntq, ntq), np.complex128, order="F") if zsec is None else _f(zsec, np.complex128).copy(order="F")
+        fr, om, ek = (np.ascontiguousarray(a, np.float64) for a in (freq_r, omega, ekc))
+        _chk(lib().ecalj_sigc_real(self._h, _np_ptr(z), ngb, nm, ntq, _np_ptr(w), w.shape[0], nw, _np_ptr(fr),
+                                   _np_ptr(om), _np_ptr(ek), ef, esmr, nctot, nm if ns2r is None else ns2r, wkkr,
+                                   _np_ptr(s), ntq), "sigc_real")
+        return s
+
+    def sigc_real_dev(self, nbatch, zmel, ngb, nm, ntq, wr, wr_stride, nw, freq_r, omega, ekc, ef, esmr, wkkr, zsec,
+                      nctot=0, ns2r=None, ldw=None, ldz=None):
+        fr, om, ek, wk = (np.ascontiguousarray(a, np.float64) for a in (freq_r, omega, ekc, wkkr))
+        _chk(lib().ecalj_sigc_real_dev(self._h, nbatch, _t_ptr(zmel), ngb, nm, ntq, _t_ptr(wr),
+                                       ngb if ldw is None else ldw, wr_stride, nw, _np_ptr(fr), _np_ptr(om),
+                                       _np_ptr(ek), ef, esmr, nctot, nm if ns2r is None else ns2r, _np_ptr(wk),
+                                       _t_ptr(zsec), ntq if ldz is None else ldz), "sigc_real_dev")
 
     def sigc_grouped_dev(self, nm, zmel, ngb, ntq, niw, wc, wgtim, zsec, ldw=None, ldz=None):
         """Internal k-points with their own middle-state counts nm[k]; zmel and wgtim packed per k."""

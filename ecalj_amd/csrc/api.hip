@@ -1009,6 +1009,36 @@ ecalj_status ecalj_sigc_grouped_dev(ecalj_handle h, int nk, const int* nm, const
                               (const double*)wgtim, (cd*)zsec, ldz);
 }
 
+ecalj_status ecalj_sigc_real_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm, int ntq,
+                                 const void* wr, int ldw, int64_t wr_stride, int nw, const double* freq_r,
+                                 const double* omega, const double* ekc, double ef, double esmr, int nctot, int ns2r,
+                                 const double* wkkr, void* zsec, int ldz) {
+  if (!h || !zmel || !wr || !freq_r || !omega || !ekc || !wkkr || !zsec) return ECALJ_ERR_ARG;
+  if (ngb < 1 || nm < 1 || ntq < 0 || nbatch < 1 || ldw < ngb || ldz < ntq || nw < 2 || ns2r < 0 || ns2r > nm ||
+      nctot < 0 || wr_stride < 0)
+    return ECALJ_ERR_DIM;
+  if (ntq == 0) return ECALJ_OK;
+  return sig_real(h, h->precision, nbatch, zmel, ngb, nm, ntq, (const cd*)wr, ldw, wr_stride, nw, freq_r, omega,
+                  ekc, ef, esmr, nctot, ns2r, wkkr, (cd*)zsec, ldz);
+}
+
+ecalj_status ecalj_sigc_real(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, const double* wr, int ldw,
+                             int nw, const double* freq_r, const double* omega, const double* ekc, double ef,
+                             double esmr, int nctot, int ns2r, double wkkr, double* zsec, int ldz) {
+  if (!h || !zmel || !wr || !zsec) return ECALJ_ERR_ARG;
+  DevC dz;
+  DevBuf dw, ds;
+  ECALJ_TRY(dz.put(h, zmel, (int64_t)ngb * nm * ntq));
+  ECALJ_TRY(dw.put(h, wr, sizeof(cd) * (size_t)ldw * ngb * (nw + 1)));
+  const size_t bs = sizeof(cd) * (size_t)ldz * ntq;
+  ECALJ_TRY(ds.put(h, zsec, bs));
+  ECALJ_TRY(ecalj_sigc_real_dev(h, 1, dz.buf.p, ngb, nm, ntq, dw.p, ldw, 0, nw, freq_r, omega, ekc, ef, esmr, nctot,
+                                ns2r, &wkkr, ds.p, ldz));
+  ECALJ_HIP(hipMemcpyAsync(zsec, ds.p, bs, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
 ecalj_status ecalj_sigc(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, int niw, const double* wc,
                         int ldw, const double* wgtim, double* zsec, int ldz) {
   if (!h || !zmel || !wc || !wgtim || !zsec) return ECALJ_ERR_ARG;

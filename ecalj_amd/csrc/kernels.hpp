@@ -32,6 +32,10 @@ size_t sig_bf3_bytes_per_k(int n, int ncol, int nw1);
 ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zmel, int n, int nw1, const cd* wc,
                              int64_t ldw, int64_t w_stride, const double* wgt, cf* cz, unsigned short* as,
                              unsigned short* bs);
+ecalj_status sig_real(ecalj_ctx* h, int mode, int nb, const void* zmel, int n, int nm, int ntq, const cd* wr,
+                      int64_t ldw, int64_t wstride, int nw, const double* freq_r, const double* omega,
+                      const double* ekc, double ef, double esmr, int nctot, int ns2r, const double* wkkr, cd* zsec,
+                      int64_t ldz);
 ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm, const void* zmel, int n, int ntq,
                                   int niw, const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz);
 size_t sig_bf3_a_elems_per_k(int n, int nw1);

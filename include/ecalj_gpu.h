@@ -236,6 +236,23 @@ ecalj_status ecalj_sigc(ecalj_handle h, const double* zmel, int ngb, int nm, int
 ecalj_status ecalj_sigc_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm,
                             int ntq, int niw, const void* wc, int ldw, const void* wgtim,
                             void* zsec, int ldz);
+/* Real-axis pole term of Sigma_c (S/m_sxcf_gemm.f90:501-594, then :599-608),
+ * added to zsec with its own final product (the term is linear in czmelwc,
+ * so this equals adding it to czmelwc before the final GEMM), then
+ * Im(zsec_mm) <= 0.  zmel: ngb x nm x ntq per internal k (middle states
+ * 1..nm, the first nctot are core); wr: ldw x ngb x (nw+1) complex128 records
+ * W(iw), iw = 0..nw, per k at element stride wr_stride (0 = shared), used
+ * through their upper triangle (keepwv); freq_r(0..nw) Hartree; omega (ntq),
+ * ekc (nm per k), ef, esmr in Ry; ns2r: last middle state of the real-axis
+ * range (sxcf_scz_count's nstte2); wkkr: k weights.  Host arrays except zmel,
+ * wr, zsec.  A pole frequency outside freq_r(1..nw-1) returns ECALJ_ERR_DIM. */
+ecalj_status ecalj_sigc_real_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm, int ntq,
+                                 const void* wr, int ldw, int64_t wr_stride, int nw, const double* freq_r,
+                                 const double* omega, const double* ekc, double ef, double esmr, int nctot,
+                                 int ns2r, const double* wkkr, void* zsec, int ldz);
+ecalj_status ecalj_sigc_real(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, const double* wr,
+                             int ldw, int nw, const double* freq_r, const double* omega, const double* ekc,
+                             double ef, double esmr, int nctot, int ns2r, double wkkr, double* zsec, int ldz);
 /* Sigma_c over internal k-points whose middle-state windows differ, the
  * per-k band ranges that sxcf_scz_count assigns (nstti/nstte,
  * S/sxcf_fal2_count.sc.f90:118-236), in one call: k has nm[k] middle states
