@@ -68,6 +68,8 @@ _SIGS = {
     "ecalj_sigc_dev": [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
     "ecalj_sigc_grouped_dev": [_p, _i, _p, _p, _i, _i, _i, _p, _i, _p, _p, _i],
     "ecalj_sigc_real_dev": [_p, _i, _p, _i, _i, _i, _p, _i, _l, _i, _p, _p, _p, _d, _d, _i, _i, _p, _p, _i],
+    "ecalj_sigc_full_dev": [_p, _i, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i, _l, _i, _p, _p, _p, _d, _d, _i, _i,
+                            _p, _p, _i],
     "ecalj_sigc_real": [_p, _p, _i, _i, _i, _p, _i, _i, _p, _p, _p, _d, _d, _i, _i, _d, _p, _i],
     "ecalj_qp_energy": [_i, _p, _p, _p, _p, _i, _p, _p],
     "ecalj_dev_alloc": [_p, C.c_size_t, C.POINTER(C.c_void_p)],
@@ -469,11 +471,24 @@ class Handle:
 
     def sigc_real_dev(self, nbatch, zmel, ngb, nm, ntq, wr, wr_stride, nw, freq_r, omega, ekc, ef, esmr, wkkr, zsec,
                       nctot=0, ns2r=None, ldw=None, ldz=None):
-        fr, om, ek, wk = (np.ascontiguousarray(a, np.float64) for a in (freq_r, omega, ekc, wkkr))
+        # ekc: nm x nbatch, column k = the middle states of internal k (Fortran order)
+        fr, om, ek, wk = (np.ascontiguousarray(np.asarray(a, np.float64).ravel(order="F"))
+                          for a in (freq_r, omega, ekc, wkkr))
         _chk(lib().ecalj_sigc_real_dev(self._h, nbatch, _t_ptr(zmel), ngb, nm, ntq, _t_ptr(wr),
                                        ngb if ldw is None else ldw, wr_stride, nw, _np_ptr(fr), _np_ptr(om),
                                        _np_ptr(ek), ef, esmr, nctot, nm if ns2r is None else ns2r, _np_ptr(wk),
                                        _t_ptr(zsec), ntq if ldz is None else ldz), "sigc_real_dev")
+
+    def sigc_full_dev(self, nbatch, zmel, ngb, nm, ntq, niw, wc, wgtim, wr, wr_stride, nw, freq_r, omega, ekc, ef,
+                      esmr, wkkr, zsec, nctot=0, ns2r=None, ldw=None, ldwr=None, ldz=None):
+        """Imaginary axis + real-axis pole term in one czmelwc (S/m_sxcf_gemm.f90:423-608)."""
+        fr, om, ek, wk = (np.ascontiguousarray(np.asarray(a, np.float64).ravel(order="F"))
+                          for a in (freq_r, omega, ekc, wkkr))
+        _chk(lib().ecalj_sigc_full_dev(self._h, nbatch, _t_ptr(zmel), ngb, nm, ntq, niw, _t_ptr(wc),
+                                       ngb if ldw is None else ldw, _t_ptr(wgtim), _t_ptr(wr),
+                                       ngb if ldwr is None else ldwr, wr_stride, nw, _np_ptr(fr), _np_ptr(om),
+                                       _np_ptr(ek), ef, esmr, nctot, nm if ns2r is None else ns2r, _np_ptr(wk),
+                                       _t_ptr(zsec), ntq if ldz is None else ldz), "sigc_full_dev")
 
     def sigc_grouped_dev(self, nm, zmel, ngb, ntq, niw, wc, wgtim, zsec, ldw=None, ldz=None):
         """Internal k-points with their own middle-state counts nm[k]; zmel and wgtim packed per k."""

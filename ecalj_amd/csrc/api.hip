@@ -1018,8 +1018,29 @@ ecalj_status ecalj_sigc_real_dev(ecalj_handle h, int nbatch, const void* zmel, i
       nctot < 0 || wr_stride < 0)
     return ECALJ_ERR_DIM;
   if (ntq == 0) return ECALJ_OK;
-  return sig_real(h, h->precision, nbatch, zmel, ngb, nm, ntq, (const cd*)wr, ldw, wr_stride, nw, freq_r, omega,
-                  ekc, ef, esmr, nctot, ns2r, wkkr, (cd*)zsec, ldz);
+  RealSpec rs;
+  rs.wr = (const cd*)wr; rs.ldw = ldw; rs.wstride = wr_stride; rs.nw = nw;
+  rs.freq_r = freq_r; rs.omega = omega; rs.ekc = ekc; rs.wkkr = wkkr;
+  rs.ef = ef; rs.esmr = esmr; rs.nctot = nctot; rs.ns2r = ns2r;
+  return sig_real(h, h->precision, nbatch, zmel, ngb, nm, ntq, rs, (cd*)zsec, ldz);
+}
+
+ecalj_status ecalj_sigc_full_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm, int ntq, int niw,
+                                 const void* wc, int ldw, const void* wgtim, const void* wr, int ldwr,
+                                 int64_t wr_stride, int nw, const double* freq_r, const double* omega,
+                                 const double* ekc, double ef, double esmr, int nctot, int ns2r, const double* wkkr,
+                                 void* zsec, int ldz) {
+  if (!h || !zmel || !wc || !wgtim || !wr || !freq_r || !omega || !ekc || !wkkr || !zsec) return ECALJ_ERR_ARG;
+  if (ngb < 1 || nm < 1 || ntq < 0 || niw < 0 || nbatch < 1 || ldw < ngb || ldwr < ngb || ldz < ntq || nw < 2 ||
+      ns2r < 0 || ns2r > nm || nctot < 0 || wr_stride < 0)
+    return ECALJ_ERR_DIM;
+  if (ntq == 0) return ECALJ_OK;
+  RealSpec rs;
+  rs.wr = (const cd*)wr; rs.ldw = ldwr; rs.wstride = wr_stride; rs.nw = nw;
+  rs.freq_r = freq_r; rs.omega = omega; rs.ekc = ekc; rs.wkkr = wkkr;
+  rs.ef = ef; rs.esmr = esmr; rs.nctot = nctot; rs.ns2r = ns2r;
+  return sig_contract(h, h->precision, nbatch, zmel, ngb, nm, ntq, niw, (const cd*)wc, ldw, (const double*)wgtim,
+                      (cd*)zsec, ldz, &rs);
 }
 
 ecalj_status ecalj_sigc_real(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, const double* wr, int ldw,

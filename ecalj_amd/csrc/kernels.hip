@@ -537,39 +537,47 @@ __global__ __launch_bounds__(kThreads) void sig_freq_gemm_kernel(const T* __rest
   });
 }
 
-// zsec(m,m') += sum over partials (fixed-order tree, FP64); then clip Im(zsec_mm) <= 0.
-// One block per output element.
+// zsec(m,m') += the split-K partials of each internal k in turn (a fixed-order
+// tree per k, FP64), with Im(zsec_mm) <= 0 after every k: the reference clips
+// after every (k, band block) call (S/m_sxcf_gemm.f90:599-608).  The partials
+// of k are [kstart[k], kstart[k+1]) when kstart is given, else
+// [k*chunks, (k+1)*chunks).  One block per output element.
 template <class T>
-__global__ __launch_bounds__(256) void sig_final_reduce_kernel(const T* __restrict__ part, int nparts, int ntq,
+__global__ __launch_bounds__(256) void sig_final_reduce_kernel(const T* __restrict__ part, int nk, int chunks,
+                                                               const int* __restrict__ kstart, int ntq,
                                                                cd* __restrict__ zsec, int64_t ldz) {
   __shared__ double sr[256], si[256];
   const int np = ntq * ntq;
   const int p = blockIdx.x;
-  double a = 0.0, c = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-    double xr, xi;
-    ld_c2(part + (int64_t)np * k + p, xr, xi);
-    a += xr;
-    c += xi;
-  }
-  sr[threadIdx.x] = a;
-  si[threadIdx.x] = c;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sr[threadIdx.x] += sr[threadIdx.x + w];
-      si[threadIdx.x] += si[threadIdx.x + w];
+  const int m = p % ntq, mp = p / ntq;
+  cd* z = zsec + m + ldz * mp;
+  const cd z0 = *z;
+  double zr = z0.re, zi = z0.im;  // running value (thread 0's copy is the one written)
+  for (int k = 0; k < nk; ++k) {
+    const int q0 = kstart ? kstart[k] : k * chunks, q1 = kstart ? kstart[k + 1] : (k + 1) * chunks;
+    double a = 0.0, c = 0.0;
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+      double xr, xi;
+      ld_c2(part + (int64_t)np * q + p, xr, xi);
+      a += xr;
+      c += xi;
     }
+    sr[threadIdx.x] = a;
+    si[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        sr[threadIdx.x] += sr[threadIdx.x + w];
+        si[threadIdx.x] += si[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    zr += sr[0];
+    zi += si[0];
+    if (m == mp && zi > 0.0) zi = 0.0;
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const int m = p % ntq, mp = p / ntq;
-    cd* z = zsec + m + ldz * mp;
-    cd o = *z;
-    double zr = o.re + sr[0], zi = o.im + si[0];
-    if (m == mp && zi > 0.0) zi = 0.0;
-    *z = cd{zr, zi};
-  }
+  if (threadIdx.x == 0) *z = cd{zr, zi};
 }
 
 // Split-K partials of the final product (S/m_sxcf_gemm.f90:599):
@@ -613,7 +621,15 @@ static ecalj_status sig_final_partials(ecalj_ctx* h, const T* cz, const T* zmel,
 
 template <int MODE, class T>
 static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int n, int nm, int ntq, int niw,
-                                   const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
+                                   const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz,
+                                   const RealSpec* rs) {
+  // real-axis pole lists (host) for all k, applied per group below
+  RealPlanHandle* rp = nullptr;
+  if (rs) ECALJ_TRY(real_plan_make(h, *rs, nbatch, n, nm, ntq, &rp));
+  struct Free {
+    RealPlanHandle* p;
+    ~Free() { if (p) real_plan_free(p); }
+  } free_rp{rp};
   const int nw1 = niw + 1;
   const int ncol = nm * ntq;
   const int64_t nn = (int64_t)n * n;
@@ -658,11 +674,13 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
       ECALJ_HIP(hipGetLastError());
       ECALJ_TRY(timing_end(h, 0, tev));
     }
+    if (rp)  // czmelwc += real-axis pole columns (S/m_sxcf_gemm.f90:586-589)
+      ECALJ_TRY(real_plan_into_cz(h, rp, *rs, MODE == M64 ? M64 : M32, zmel, n, nm, ntq, b0, b0 + gb, cz));
     ECALJ_TRY((sig_final_partials<MODE, T>(h, cz, zmel + (int64_t)n * ncol * b0, R, (int64_t)n * ncol, gb, ntq,
                                             part + (int64_t)np * chunks * b0)));
   }
-  hipLaunchKernelGGL((sig_final_reduce_kernel<T>), dim3(np), dim3(256), 0, h->stream, part, chunks * nbatch, ntq,
-                     zsec, ldz);
+  hipLaunchKernelGGL((sig_final_reduce_kernel<T>), dim3(np), dim3(256), 0, h->stream, part, nbatch, chunks,
+                     (const int*)nullptr, ntq, zsec, ldz);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
@@ -739,18 +757,28 @@ ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm,
       pofs += sig_chunks(R);
     }
   }
-  hipLaunchKernelGGL((sig_final_reduce_kernel<cf>), dim3(np), dim3(256), 0, h->stream, part, (int)pofs, ntq, zsec,
-                     ldz);
+  std::vector<unsigned char> ks(sizeof(int) * (nk + 1));
+  {
+    int* kp = reinterpret_cast<int*>(ks.data());
+    kp[0] = 0;
+    for (int k = 0; k < nk; ++k) kp[k + 1] = kp[k] + (nm[k] ? sig_chunks((int64_t)n * nm[k]) : 0);
+  }
+  unsigned char* dks;
+  ECALJ_TRY(upload_blob(h, ecalj_ctx::S_KST, ks, &dks));
+  hipLaunchKernelGGL((sig_final_reduce_kernel<cf>), dim3(np), dim3(256), 0, h->stream, part, nk, 0,
+                     (const int*)dks, ntq, zsec, ldz);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
 
 ecalj_status sig_contract(ecalj_ctx* h, int mode, int nbatch, const void* zmel, int n, int nm, int ntq, int niw,
-                          const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz) {
+                          const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz, const RealSpec* rs) {
   if (n == 0 || nm * ntq == 0 || nbatch == 0) return ECALJ_OK;
-  if (mode == M64) return sig_contract_t<M64, cd>(h, nbatch, (const cd*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz);
-  if (mode == M32) return sig_contract_t<M32, cf>(h, nbatch, (const cf*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz);
-  return sig_contract_t<MBF3, cf>(h, nbatch, (const cf*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz);
+  if (mode == M64)
+    return sig_contract_t<M64, cd>(h, nbatch, (const cd*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz, rs);
+  if (mode == M32)
+    return sig_contract_t<M32, cf>(h, nbatch, (const cf*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz, rs);
+  return sig_contract_t<MBF3, cf>(h, nbatch, (const cf*)zmel, n, nm, ntq, niw, wc, ldw, wgt, zsec, ldz, rs);
 }
 
 // --------------------------------------------------------------------------

@@ -16,8 +16,11 @@ ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac
 ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const double* s, int n, int nw, cd* w, int64_t ldw,
                      int64_t wstride);
 ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat);
+struct RealSpec;
+// rs (optional): the real-axis pole term, added into czmelwc before the final product
 ecalj_status sig_contract(ecalj_ctx* h, int mode, int nbatch, const void* zmel, int n, int nm, int ntq, int niw,
-                          const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz);
+                          const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz,
+                          const RealSpec* rs = nullptr);
 ecalj_status sig_weights(ecalj_ctx* h, int nbatch, int niw, int nm, int ntq, int ns1, int nctot,
                          const double* omega, const double* ekc, double esmr, double ua, const double* wkkr,
                          const double* x, const double* wt, const double* expa, double* wgt);
@@ -32,10 +35,23 @@ size_t sig_bf3_bytes_per_k(int n, int ncol, int nw1);
 ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zmel, int n, int nw1, const cd* wc,
                              int64_t ldw, int64_t w_stride, const double* wgt, cf* cz, unsigned short* as,
                              unsigned short* bs);
-ecalj_status sig_real(ecalj_ctx* h, int mode, int nb, const void* zmel, int n, int nm, int ntq, const cd* wr,
-                      int64_t ldw, int64_t wstride, int nw, const double* freq_r, const double* omega,
-                      const double* ekc, double ef, double esmr, int nctot, int ns2r, const double* wkkr, cd* zsec,
-                      int64_t ldz);
+// Inputs of the real-axis pole term (S/m_sxcf_gemm.f90:501-594); host arrays
+// except wr.  ekc: nm per internal k.
+struct RealSpec {
+  const cd* wr = nullptr;
+  int64_t ldw = 0, wstride = 0;
+  int nw = 0;
+  const double *freq_r = nullptr, *omega = nullptr, *ekc = nullptr, *wkkr = nullptr;
+  double ef = 0.0, esmr = 0.0;
+  int nctot = 0, ns2r = 0;
+};
+ecalj_status sig_real(ecalj_ctx* h, int mode, int nb, const void* zmel, int n, int nm, int ntq, const RealSpec& rs,
+                      cd* zsec, int64_t ldz);
+struct RealPlanHandle;
+ecalj_status real_plan_make(ecalj_ctx* h, const RealSpec& rs, int nb, int n, int nm, int ntq, RealPlanHandle** out);
+void real_plan_free(RealPlanHandle* r);
+ecalj_status real_plan_into_cz(ecalj_ctx* h, const RealPlanHandle* r, const RealSpec& rs, int mode,
+                               const void* zmel, int n, int nm, int ntq, int k0, int k1, void* cz);
 ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm, const void* zmel, int n, int ntq,
                                   int niw, const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz);
 size_t sig_bf3_a_elems_per_k(int n, int nw1);

@@ -30,7 +30,7 @@ struct RealItem {
 };
 
 template <class T>
-__global__ __launch_bounds__(256) void sig_real_entries_kernel(const RealItem* __restrict__ items,
+__global__ __launch_bounds__(256) void sig_real_entries_kernel(const RealItem* __restrict__ items, int item0,
                                                                const int* __restrict__ perm,
                                                                const int* __restrict__ ecol,
                                                                const double* __restrict__ ew,
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void sig_real_entries_kernel(const RealItem* _
                                                                int64_t wstride, int n, cd* __restrict__ slot) {
   __shared__ cd wt[RT][RJ + 1];  // Wt(I0+r, J0+jj)
   __shared__ cd zb[RJ][RE];      // w_e * zmel(J0+jj, col_e)
-  const RealItem it = items[blockIdx.x];
+  const RealItem it = items[item0 + blockIdx.x];
   const int t = threadIdx.x;
   const cd* W = wr + wstride * it.k + ldw * n * (int64_t)it.iw;
   const T* Z = zmel + zstride * it.k;
@@ -96,52 +96,76 @@ __global__ __launch_bounds__(256) void sig_real_entries_kernel(const RealItem* _
   }
 }
 
-// zsec(m, m') += sum over poles p of external state m (fixed order) of
-// sum_I conj(v_p(I)) zmel_k(I, it_p, m'), v_p = slot[3p] + slot[3p+1] + slot[3p+2];
-// then Im(zsec_mm) <= 0.  One block per (m, m').
+// zsec(m, m') += sum over poles p of external state m, k by k, of
+// sum_I conj(v_p(I)) zmel_k(I, it_p, m'), v_p = slot[3p] + slot[3p+1] + slot[3p+2]
+// (frequency order); Im(zsec_mm) <= 0 after every k, as the reference clips
+// after every (k, band block) call.  plist holds each m's poles in k order.
+// One block per (m, m').
 template <class T>
 __global__ __launch_bounds__(256) void sig_real_final_kernel(const int* __restrict__ pstart,
                                                              const int* __restrict__ plist,
                                                              const int* __restrict__ pk,
                                                              const int* __restrict__ pit, const cd* __restrict__ slot,
                                                              const T* __restrict__ zmel, int64_t zstride, int n,
-                                                             int nm, int ntq, cd* __restrict__ zsec, int64_t ldz) {
+                                                             int nm, int ntq, int nk, cd* __restrict__ zsec,
+                                                             int64_t ldz) {
   __shared__ double sr[256], si[256];
   const int m = blockIdx.x % ntq, mp = blockIdx.x / ntq;
-  double tr = 0.0, ti = 0.0;
-  for (int q = pstart[m]; q < pstart[m + 1]; ++q) {
-    const int p = plist[q];
-    const cd* s0 = slot + (int64_t)n * 3 * p;
-    const T* Z = zmel + zstride * pk[p] + (int64_t)n * (pit[p] + (int64_t)nm * mp);
-    double a = 0.0, b = 0.0;
-    for (int I = threadIdx.x; I < n; I += 256) {
-      const cd v0 = s0[I], v1 = s0[n + I], v2 = s0[2 * n + I];
-      const double vr = (v0.re + v1.re) + v2.re, vi = (v0.im + v1.im) + v2.im;
-      double zr, zi;
-      ld_c2(Z + I, zr, zi);
-      a += vr * zr + vi * zi;  // conj(v) z
-      b += vr * zi - vi * zr;
-    }
-    sr[threadIdx.x] = a;
-    si[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) {
-        sr[threadIdx.x] += sr[threadIdx.x + w];
-        si[threadIdx.x] += si[threadIdx.x + w];
+  cd* zs = zsec + m + ldz * mp;
+  const cd z0 = *zs;
+  double tr = z0.re, ti = z0.im;
+  int q = pstart[m];
+  const int qend = pstart[m + 1];
+  for (int k = 0; k < nk; ++k) {
+    for (; q < qend && pk[plist[q]] == k; ++q) {
+      const int p = plist[q];
+      const cd* s0 = slot + (int64_t)n * 3 * p;
+      const T* Z = zmel + zstride * k + (int64_t)n * (pit[p] + (int64_t)nm * mp);
+      double a = 0.0, b = 0.0;
+      for (int I = threadIdx.x; I < n; I += 256) {
+        const cd v0 = s0[I], v1 = s0[n + I], v2 = s0[2 * n + I];
+        const double vr = (v0.re + v1.re) + v2.re, vi = (v0.im + v1.im) + v2.im;
+        double zr, zi;
+        ld_c2(Z + I, zr, zi);
+        a += vr * zr + vi * zi;  // conj(v) z
+        b += vr * zi - vi * zr;
       }
+      sr[threadIdx.x] = a;
+      si[threadIdx.x] = b;
+      __syncthreads();
+      for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+          sr[threadIdx.x] += sr[threadIdx.x + w];
+          si[threadIdx.x] += si[threadIdx.x + w];
+        }
+        __syncthreads();
+      }
+      tr += sr[0];
+      ti += si[0];
       __syncthreads();
     }
-    tr += sr[0];
-    ti += si[0];
-    __syncthreads();
+    if (m == mp && ti > 0.0) ti = 0.0;
   }
-  if (threadIdx.x == 0) {
-    cd* z = zsec + m + ldz * mp;
-    const cd o = *z;
-    double zr = o.re + tr, zi = o.im + ti;
-    if (m == mp && zi > 0.0) zi = 0.0;
-    *z = cd{zr, zi};
+  if (threadIdx.x == 0) *zs = cd{tr, ti};
+}
+
+// czmelwc_k(:, it_p + nm*itp_p) += slot[3p] + slot[3p+1] + slot[3p+2] (frequency
+// order) for the poles p of k in [k0, k1); cz of k at cz + czstride*(k - k0).
+// No two poles of one k share a column.
+template <class T>
+__global__ void sig_real_add_kernel(const int* __restrict__ pk, const int* __restrict__ pcol, int p0, int p1, int k0,
+                                    const cd* __restrict__ slot, int n, T* __restrict__ cz, int64_t czstride) {
+  const int64_t tot = (int64_t)(p1 - p0) * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int p = p0 + (int)(e / n), I = (int)(e % n);
+    const cd* s0 = slot + (int64_t)n * 3 * p;
+    const cd v0 = s0[I], v1 = s0[n + I], v2 = s0[2 * n + I];
+    T* c = cz + czstride * (pk[p] - k0) + (int64_t)n * pcol[p] + I;
+    double xr, xi;
+    ld_c2(c, xr, xi);
+    xr += (v0.re + v1.re) + v2.re;
+    xi += (v0.im + v1.im) + v2.im;
+    st_c(c, xr, xi);
   }
 }
 
@@ -176,85 +200,105 @@ static void inv3(const double a[3][3], double r[3][3]) {
     for (int j = 0; j < 3; ++j) r[i][j] = c[i][j] / det;
 }
 
-template <class T>
-static ecalj_status sig_real_t(ecalj_ctx* h, int nb, const T* zmel, int n, int nm, int ntq, const cd* wr,
-                               int64_t ldw, int64_t wstride, int nw, const double* freq_r, const double* omega,
-                               const double* ekc, double ef, double esmr, int nctot, int ns2r, const double* wkkr,
-                               cd* zsec, int64_t ldz) {
-  const double ddw = 10.0, wfaccut = 1e-8;  // S/m_sxcf_gemm.f90:264,408
-  // poles p (k, it, itp) with three entries 3p+j at iw = ixs-1+j
-  std::vector<int> pk, pit, pitp, eiw, ecol;
+// Pole lists of nb internal k-points (host, as S/m_sxcf_gemm.f90:512-552
+// builds them) and their device table.  Poles are generated k by k; the
+// entries of pole p are 3p..3p+2 (frequency order).
+struct RealPlan {
+  int nb = 0, np = 0, ne = 0;
+  std::vector<int> pk, pit, pitp, pcol, eiw, ecol;
   std::vector<double> ew;
+  std::vector<int> perm, pstart, plist;
+  std::vector<RealItem> items;
+  std::vector<int> kitem, kpole;  // first item / pole of each k (nb+1)
+  unsigned char* d = nullptr;     // device table
+  size_t o_it = 0, o_pm = 0, o_ec = 0, o_ps = 0, o_pl = 0, o_pk = 0, o_pt = 0, o_pc = 0, o_ew = 0;
+  cd* slot = nullptr;
+};
+
+static ecalj_status real_plan(ecalj_ctx* h, const RealSpec& rs, int nb, int n, int nm, int ntq, RealPlan& P) {
+  const double ddw = 10.0, wfaccut = 1e-8;  // S/m_sxcf_gemm.f90:264,408
+  const int nw = rs.nw;
+  P.nb = nb;
+  P.kpole.assign(nb + 1, 0);
   for (int k = 0; k < nb; ++k) {
-    const double* e = ekc + (int64_t)nm * k;
+    P.kpole[k] = (int)P.pk.size();
+    const double* e = rs.ekc + (int64_t)nm * k;
     int nt0p = 0, nt0m = 0;
     for (int i = 0; i < nm; ++i) {
-      nt0p += e[i] < ef + ddw * esmr;
-      nt0m += e[i] < ef - ddw * esmr;
+      nt0p += e[i] < rs.ef + ddw * rs.esmr;
+      nt0m += e[i] < rs.ef - ddw * rs.esmr;
     }
     for (int itp = 0; itp < ntq; ++itp) {
-      const double omg = omega[itp];
-      const int itini = (omg >= ef) ? std::max(1, nt0m + 1) : 1;
-      const int itend = (omg >= ef) ? ns2r : std::min(nt0p, ns2r);
+      const double omg = rs.omega[itp];
+      const int itini = (omg >= rs.ef) ? std::max(1, nt0m + 1) : 1;
+      const int itend = (omg >= rs.ef) ? rs.ns2r : std::min(nt0p, rs.ns2r);
       for (int it = itini; it <= itend; ++it) {
-        double wf = wfacx2_h(omg, ef, e[it - 1], it <= nctot ? 0.0 : esmr);
+        double wf = wfacx2_h(omg, rs.ef, e[it - 1], it <= rs.nctot ? 0.0 : rs.esmr);
         if (wf < wfaccut) continue;
-        wf *= wkkr[k] * ((omg - ef) >= 0.0 ? 1.0 : -1.0);
-        const double we = 0.5 * std::fabs(omg - weavx2_h(omg, ef, e[it - 1], esmr));
+        wf *= rs.wkkr[k] * ((omg - rs.ef) >= 0.0 ? 1.0 : -1.0);
+        const double we = 0.5 * std::fabs(omg - weavx2_h(omg, rs.ef, e[it - 1], rs.esmr));
         int ixs = 0;
         for (int i = 1; i <= nw; ++i)
-          if (freq_r[i] > we) {
+          if (rs.freq_r[i] > we) {
             ixs = i;
             break;
           }
         if (ixs < 1 || ixs + 1 > nw) return ECALJ_ERR_DIM;  // pole outside the real mesh
         double a[3][3], ai[3][3];
         for (int i = 0; i < 3; ++i) {
-          const double x = freq_r[ixs - 1 + i];
+          const double x = rs.freq_r[ixs - 1 + i];
           a[i][0] = 1.0;
           a[i][1] = x * x;
           a[i][2] = x * x * x * x;
         }
         inv3(a, ai);
         const double xv[3] = {1.0, we * we, we * we * we * we};
-        pk.push_back(k);
-        pit.push_back(it - 1);
-        pitp.push_back(itp);
+        P.pk.push_back(k);
+        P.pit.push_back(it - 1);
+        P.pitp.push_back(itp);
+        P.pcol.push_back((it - 1) + nm * itp);
         for (int j = 0; j < 3; ++j) {
-          eiw.push_back(ixs - 1 + j);
-          ecol.push_back((it - 1) + nm * itp);
-          ew.push_back(wf * (xv[0] * ai[0][j] + xv[1] * ai[1][j] + xv[2] * ai[2][j]));
+          P.eiw.push_back(ixs - 1 + j);
+          P.ecol.push_back((it - 1) + nm * itp);
+          P.ew.push_back(wf * (xv[0] * ai[0][j] + xv[1] * ai[1][j] + xv[2] * ai[2][j]));
         }
       }
     }
   }
-  const int np = (int)pk.size();
-  if (np == 0) return ECALJ_OK;
-  const int ne = 3 * np;
-  // entries ordered by (k, iw); items of <= RE entries x row tiles
-  std::vector<int> perm(ne);
-  for (int e = 0; e < ne; ++e) perm[e] = e;
-  std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
-    const int kx = pk[x / 3], ky = pk[y / 3];
-    return kx != ky ? kx < ky : eiw[x] < eiw[y];
+  P.kpole[nb] = P.np = (int)P.pk.size();
+  P.ne = 3 * P.np;
+  if (P.np == 0) return ECALJ_OK;
+  // entries ordered by (k, iw); items of <= RE entries x row tiles; items of k contiguous
+  P.perm.resize(P.ne);
+  for (int e = 0; e < P.ne; ++e) P.perm[e] = e;
+  std::stable_sort(P.perm.begin(), P.perm.end(), [&](int x, int y) {
+    const int kx = P.pk[x / 3], ky = P.pk[y / 3];
+    return kx != ky ? kx < ky : P.eiw[x] < P.eiw[y];
   });
-  std::vector<RealItem> items;
-  for (int s = 0; s < ne;) {
-    const int k = pk[perm[s] / 3], iw = eiw[perm[s]];
-    int e = s;
-    while (e < ne && pk[perm[e] / 3] == k && eiw[perm[e]] == iw && e - s < RE) ++e;
-    for (int i0 = 0; i0 < n; i0 += RT) items.push_back({k, iw, s, e - s, i0, 0});
-    s = e;
+  P.kitem.assign(nb + 1, 0);
+  for (int s0 = 0; s0 < P.ne;) {
+    const int k = P.pk[P.perm[s0] / 3], iw = P.eiw[P.perm[s0]];
+    int e = s0;
+    while (e < P.ne && P.pk[P.perm[e] / 3] == k && P.eiw[P.perm[e]] == iw && e - s0 < RE) ++e;
+    for (int i0 = 0; i0 < n; i0 += RT) P.items.push_back({k, iw, s0, e - s0, i0, 0});
+    s0 = e;
   }
-  // poles grouped by external state, in (k, it) order
-  std::vector<int> pstart(ntq + 1, 0), plist(np);
-  for (int p = 0; p < np; ++p) pstart[pitp[p] + 1]++;
-  for (int m = 0; m < ntq; ++m) pstart[m + 1] += pstart[m];
   {
-    std::vector<int> fill(pstart.begin(), pstart.end() - 1);
-    for (int p = 0; p < np; ++p) plist[fill[pitp[p]]++] = p;
+    int q = 0;
+    for (int k = 0; k <= nb; ++k) {
+      while (q < (int)P.items.size() && P.items[q].k < k) ++q;
+      P.kitem[k] = q;
+    }
   }
-  // one table upload: items | perm | ecol | pstart | plist | pk | pit | ew (8-byte aligned)
+  // poles grouped by external state, k order kept
+  P.pstart.assign(ntq + 1, 0);
+  P.plist.resize(P.np);
+  for (int p = 0; p < P.np; ++p) P.pstart[P.pitp[p] + 1]++;
+  for (int m = 0; m < ntq; ++m) P.pstart[m + 1] += P.pstart[m];
+  {
+    std::vector<int> fill(P.pstart.begin(), P.pstart.end() - 1);
+    for (int p = 0; p < P.np; ++p) P.plist[fill[P.pitp[p]]++] = p;
+  }
   std::vector<unsigned char> blob;
   auto put = [&](const void* src, size_t bytes) {
     const size_t o = blob.size();
@@ -262,40 +306,98 @@ static ecalj_status sig_real_t(ecalj_ctx* h, int nb, const T* zmel, int n, int n
     if (bytes) memcpy(blob.data() + o, src, bytes);
     return o;
   };
-  const size_t o_it = put(items.data(), sizeof(RealItem) * items.size());
-  const size_t o_pm = put(perm.data(), sizeof(int) * ne);
-  const size_t o_ec = put(ecol.data(), sizeof(int) * ne);
-  const size_t o_ps = put(pstart.data(), sizeof(int) * (ntq + 1));
-  const size_t o_pl = put(plist.data(), sizeof(int) * np);
-  const size_t o_pk = put(pk.data(), sizeof(int) * np);
-  const size_t o_pt = put(pit.data(), sizeof(int) * np);
-  const size_t o_ew = put(ew.data(), sizeof(double) * ne);
-  unsigned char* d;
-  ECALJ_TRY(upload_blob(h, ecalj_ctx::S_RTAB, blob, &d));
-  cd* slot;
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_RSLOT, sizeof(cd) * (size_t)n * ne, (void**)&slot));
-  const int64_t zstride = (int64_t)n * nm * ntq;
-  if (items.size() > 0x7fffffff) return ECALJ_ERR_DIM;
-  hipLaunchKernelGGL((sig_real_entries_kernel<T>), dim3((unsigned)items.size()), dim3(256), 0, h->stream,
-                     (const RealItem*)(d + o_it), (const int*)(d + o_pm), (const int*)(d + o_ec),
-                     (const double*)(d + o_ew), zmel, zstride, wr, ldw, wstride, n, slot);
-  ECALJ_HIP(hipGetLastError());
-  hipLaunchKernelGGL((sig_real_final_kernel<T>), dim3(ntq * ntq), dim3(256), 0, h->stream, (const int*)(d + o_ps),
-                     (const int*)(d + o_pl), (const int*)(d + o_pk), (const int*)(d + o_pt), slot, zmel, zstride, n,
-                     nm, ntq, zsec, ldz);
+  P.o_it = put(P.items.data(), sizeof(RealItem) * P.items.size());
+  P.o_pm = put(P.perm.data(), sizeof(int) * P.ne);
+  P.o_ec = put(P.ecol.data(), sizeof(int) * P.ne);
+  P.o_ps = put(P.pstart.data(), sizeof(int) * (ntq + 1));
+  P.o_pl = put(P.plist.data(), sizeof(int) * P.np);
+  P.o_pk = put(P.pk.data(), sizeof(int) * P.np);
+  P.o_pt = put(P.pit.data(), sizeof(int) * P.np);
+  P.o_pc = put(P.pcol.data(), sizeof(int) * P.np);
+  P.o_ew = put(P.ew.data(), sizeof(double) * P.ne);
+  ECALJ_TRY(upload_blob(h, ecalj_ctx::S_RTAB, blob, &P.d));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_RSLOT, sizeof(cd) * (size_t)n * P.ne, (void**)&P.slot));
+  return ECALJ_OK;
+}
+
+// entry columns of the k-points [k0, k1) into the slots
+template <class T>
+static ecalj_status real_entries(ecalj_ctx* h, const RealPlan& P, const RealSpec& rs, const T* zmel,
+                                 int64_t zstride, int n, int k0, int k1) {
+  const int i0 = P.kitem[k0], i1 = P.kitem[k1];
+  if (i1 <= i0) return ECALJ_OK;
+  hipLaunchKernelGGL((sig_real_entries_kernel<T>), dim3((unsigned)(i1 - i0)), dim3(256), 0, h->stream,
+                     (const RealItem*)(P.d + P.o_it), i0, (const int*)(P.d + P.o_pm), (const int*)(P.d + P.o_ec),
+                     (const double*)(P.d + P.o_ew), zmel, zstride, rs.wr, rs.ldw, rs.wstride, n, P.slot);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
 
-ecalj_status sig_real(ecalj_ctx* h, int mode, int nb, const void* zmel, int n, int nm, int ntq, const cd* wr,
-                      int64_t ldw, int64_t wstride, int nw, const double* freq_r, const double* omega,
-                      const double* ekc, double ef, double esmr, int nctot, int ns2r, const double* wkkr, cd* zsec,
-                      int64_t ldz) {
+template <class T>
+static ecalj_status real_into_cz(ecalj_ctx* h, const RealPlan& P, const RealSpec& rs, const T* zmel,
+                                 int64_t zstride, int n, int k0, int k1, T* cz, int64_t czstride) {
+  if (P.np == 0) return ECALJ_OK;
+  ECALJ_TRY(real_entries<T>(h, P, rs, zmel, zstride, n, k0, k1));
+  const int p0 = P.kpole[k0], p1 = P.kpole[k1];
+  if (p1 <= p0) return ECALJ_OK;
+  const int64_t tot = (int64_t)(p1 - p0) * n;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL((sig_real_add_kernel<T>), dim3(blocks), dim3(256), 0, h->stream, (const int*)(P.d + P.o_pk),
+                     (const int*)(P.d + P.o_pc), p0, p1, k0, P.slot, n, cz, czstride);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+template <class T>
+static ecalj_status sig_real_t(ecalj_ctx* h, int nb, const T* zmel, int n, int nm, int ntq, const RealSpec& rs,
+                               cd* zsec, int64_t ldz) {
+  RealPlan P;
+  ECALJ_TRY(real_plan(h, rs, nb, n, nm, ntq, P));
+  const int64_t zstride = (int64_t)n * nm * ntq;
+  if (P.np > 0) ECALJ_TRY(real_entries<T>(h, P, rs, zmel, zstride, n, 0, nb));
+  // the final kernel also clips k by k when there are no poles
+  static const int zero_list[2] = {0, 0};
+  (void)zero_list;
+  if (P.np == 0) {
+    std::vector<unsigned char> blob(sizeof(int) * (ntq + 1) + 8, 0);
+    ECALJ_TRY(upload_blob(h, ecalj_ctx::S_RTAB, blob, &P.d));
+    P.o_ps = P.o_pl = P.o_pk = P.o_pt = 0;
+  }
+  hipLaunchKernelGGL((sig_real_final_kernel<T>), dim3(ntq * ntq), dim3(256), 0, h->stream,
+                     (const int*)(P.d + P.o_ps), (const int*)(P.d + P.o_pl), (const int*)(P.d + P.o_pk),
+                     (const int*)(P.d + P.o_pt), P.slot, zmel, zstride, n, nm, ntq, nb, zsec, ldz);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+ecalj_status sig_real(ecalj_ctx* h, int mode, int nb, const void* zmel, int n, int nm, int ntq, const RealSpec& rs,
+                      cd* zsec, int64_t ldz) {
+  if (mode == M64) return sig_real_t<cd>(h, nb, (const cd*)zmel, n, nm, ntq, rs, zsec, ldz);
+  return sig_real_t<cf>(h, nb, (const cf*)zmel, n, nm, ntq, rs, zsec, ldz);
+}
+
+// The real-axis columns of the k-points [k0, k1) added into czmelwc
+// (cz of k at cz + czstride*(k - k0)), before the final product.
+struct RealPlanHandle {
+  RealPlan P;
+};
+ecalj_status real_plan_make(ecalj_ctx* h, const RealSpec& rs, int nb, int n, int nm, int ntq, RealPlanHandle** out) {
+  auto* r = new RealPlanHandle;
+  const ecalj_status st = real_plan(h, rs, nb, n, nm, ntq, r->P);
+  if (st != ECALJ_OK) {
+    delete r;
+    return st;
+  }
+  *out = r;
+  return ECALJ_OK;
+}
+void real_plan_free(RealPlanHandle* r) { delete r; }
+ecalj_status real_plan_into_cz(ecalj_ctx* h, const RealPlanHandle* r, const RealSpec& rs, int mode,
+                               const void* zmel, int n, int nm, int ntq, int k0, int k1, void* cz) {
+  const int64_t zstride = (int64_t)n * nm * ntq;
   if (mode == M64)
-    return sig_real_t<cd>(h, nb, (const cd*)zmel, n, nm, ntq, wr, ldw, wstride, nw, freq_r, omega, ekc, ef, esmr,
-                          nctot, ns2r, wkkr, zsec, ldz);
-  return sig_real_t<cf>(h, nb, (const cf*)zmel, n, nm, ntq, wr, ldw, wstride, nw, freq_r, omega, ekc, ef, esmr,
-                        nctot, ns2r, wkkr, zsec, ldz);
+    return real_into_cz<cd>(h, r->P, rs, (const cd*)zmel, zstride, n, k0, k1, (cd*)cz, zstride);
+  return real_into_cz<cf>(h, r->P, rs, (const cf*)zmel, zstride, n, k0, k1, (cf*)cz, zstride);
 }
 
 }  // namespace ecalj

@@ -250,6 +250,15 @@ ecalj_status ecalj_sigc_real_dev(ecalj_handle h, int nbatch, const void* zmel, i
                                  const void* wr, int ldw, int64_t wr_stride, int nw, const double* freq_r,
                                  const double* omega, const double* ekc, double ef, double esmr, int nctot,
                                  int ns2r, const double* wkkr, void* zsec, int ldz);
+/* get_correlation_block in full (S/m_sxcf_gemm.f90:423-608): the imaginary-
+ * axis contraction of ecalj_sigc_dev and the real-axis pole term of
+ * ecalj_sigc_real_dev added into the same czmelwc, one final product per
+ * internal k, Im(zsec_mm) <= 0 after every k -- the reference's order. */
+ecalj_status ecalj_sigc_full_dev(ecalj_handle h, int nbatch, const void* zmel, int ngb, int nm, int ntq,
+                                 int niw, const void* wc, int ldw, const void* wgtim, const void* wr,
+                                 int ldwr, int64_t wr_stride, int nw, const double* freq_r,
+                                 const double* omega, const double* ekc, double ef, double esmr,
+                                 int nctot, int ns2r, const double* wkkr, void* zsec, int ldz);
 ecalj_status ecalj_sigc_real(ecalj_handle h, const double* zmel, int ngb, int nm, int ntq, const double* wr,
                              int ldw, int nw, const double* freq_r, const double* omega, const double* ekc,
                              double ef, double esmr, int nctot, int ns2r, double wkkr, double* zsec, int ldz);

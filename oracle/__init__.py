@@ -367,3 +367,23 @@ def sigc_real(zmel_, wr, freq_r, omega, ekc, ef, esmr, wkkr, nctot=0, ns2r=None,
     if ne < 0:
         raise ValueError("sigc_real: a pole frequency lies outside the real mesh")
     return zsec, ne
+
+
+def sigc_full(zmel_, wc, wgt, wr, freq_r, omega, ekc, ef, esmr, wkkr, nctot=0, ns2r=None, zsec=None):
+    """Imaginary + real-axis Sigma_c of one internal k in one czmelwc (orc_sigc_full)."""
+    z, w, g, r = _cz(zmel_), _cz(wc), _cd(wgt), _cz(wr)
+    ngb, nm, ntq = z.shape
+    niw = w.shape[2] - 1
+    nw = r.shape[2] - 1
+    if zsec is None:
+        zsec = np.zeros((ntq, ntq), np.complex128, order="F")
+    L = lib()
+    L.orc_sigc_full.restype = C.c_int
+    L.orc_sigc_full.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
+    ne = L.orc_sigc_full(ngb, nm, ntq, niw, _p(z), _p(w), w.shape[0], _p(g), _p(r), r.shape[0], nw, _p(_cd(freq_r)),
+                         _p(_cd(omega)), _p(_cd(ekc)), ef, esmr, nctot, nm if ns2r is None else ns2r, wkkr, _p(zsec),
+                         zsec.shape[0])
+    if ne < 0:
+        raise ValueError("sigc_full: a pole frequency lies outside the real mesh")
+    return zsec, ne

@@ -545,14 +545,13 @@ void orc_wgtim(int niw, int ns1, int ns2, int ntq, int nctot, const double* omeg
 }
 
 /* Imaginary-axis Sigma_c + final GEMM + clip (S/m_sxcf_gemm.f90:463-496, 599-608). */
-void orc_sigc_imag(int ngb, int nm, int ntq, int niw, const double complex* zmel,
-                   const double complex* wc, int ldw, const double* wgtim,
-                   double complex* zsec, int ldz) {
+/* czmelwc of the imaginary axis (:463-496): cz = sum_iw Wt(iw)^H (wgtim(iw) o zmel) */
+static void sigc_imag_cz(int ngb, int nm, int ntq, int niw, const double complex* zmel, const double complex* wc,
+                         int ldw, const double* wgtim, double complex* cz) {
   long nc = (long)nm * ntq;
   long n2 = (long)ngb * ngb;
   double complex* w = (double complex*)malloc(sizeof(double complex) * n2);
   double complex* wz = (double complex*)malloc(sizeof(double complex) * ngb * nc);
-  double complex* cz = (double complex*)malloc(sizeof(double complex) * ngb * nc);
   for (int iw = 0; iw <= niw; ++iw) {
     const double complex* src = wc + (long)ldw * ngb * iw;
     /* keepwv unpack: wc(i,j)=upper(i,j), wc(j,i)=conj(upper(i,j)) for i<=j (:468-475) */
@@ -570,17 +569,28 @@ void orc_sigc_imag(int ngb, int nm, int ntq, int niw, const double complex* zmel
     /* czmelwc (+)= wc^H wzmel (:492) */
     orc_zgemm('C', 'N', ngb, (int)nc, ngb, 1.0, w, ngb, wz, ngb, iw == 0 ? 0.0 : 1.0, cz, ngb);
   }
-  /* zsec += czmelwc^H zmel, K = ngb*nm (:599) */
+  free(w);
+  free(wz);
+}
+
+/* zsec += czmelwc^H zmel, K = ngb*nm (:599); Im(zsec_nn) <= 0 (:601-608) */
+static void sigc_final(int ngb, int nm, int ntq, const double complex* cz, const double complex* zmel,
+                       double complex* zsec, int ldz) {
   orc_zgemm('C', 'N', ntq, ntq, ngb * nm, 1.0, cz, (long)ngb * nm, zmel, (long)ngb * nm, 1.0, zsec, ldz);
-  /* Im(zsec_nn) <= 0 (:601-608) */
   for (int i = 0; i < ntq; ++i) {
     double complex z = zsec[i + (long)ldz * i];
     double zi = cimag(z);
     if (zi > 0.0) zi = 0.0;
     zsec[i + (long)ldz * i] = creal(z) + I * zi;
   }
-  free(w);
-  free(wz);
+}
+
+void orc_sigc_imag(int ngb, int nm, int ntq, int niw, const double complex* zmel,
+                   const double complex* wc, int ldw, const double* wgtim,
+                   double complex* zsec, int ldz) {
+  double complex* cz = (double complex*)malloc(sizeof(double complex) * ngb * (long)nm * ntq);
+  sigc_imag_cz(ngb, nm, ntq, niw, zmel, wc, ldw, wgtim, cz);
+  sigc_final(ngb, nm, ntq, cz, zmel, zsec, ldz);
   free(cz);
 }
 
@@ -616,9 +626,9 @@ static void inv33(const double a[3][3], double r[3][3]) {
     for (int j = 0; j < 3; ++j) r[i][j] = c[i][j] / det;
 }
 
-int orc_sigc_real(int ngb, int nm, int ntq, const double complex* zmel, const double complex* wr, int ldw, int nw,
-                  const double* freq_r, const double* omega, const double* ekc, double ef, double esmr, int nctot,
-                  int ns2r, double wkkr, double complex* zsec, int ldz) {
+static int sigc_real_cz(int ngb, int nm, int ntq, const double complex* zmel, const double complex* wr, int ldw,
+                        int nw, const double* freq_r, const double* omega, const double* ekc, double ef,
+                        double esmr, int nctot, int ns2r, double wkkr, double complex* cz) {
   const double ddw = 10.0, wfaccut = 1e-8;
   int nt0p = 0, nt0m = 0;
   for (int i = 0; i < nm; ++i) {
@@ -672,7 +682,6 @@ int orc_sigc_real(int ngb, int nm, int ntq, const double complex* zmel, const do
   }
   const long n2 = (long)ngb * ngb;
   double complex* w = (double complex*)malloc(sizeof(double complex) * n2);
-  double complex* cz = (double complex*)calloc((size_t)ngb * nm * ntq, sizeof(double complex));
   for (int iw = 0; iw <= nw; ++iw) {
     int any = 0;
     for (long e = 0; e < ne; ++e) any |= (e_iw[e] == iw);
@@ -695,19 +704,35 @@ int orc_sigc_real(int ngb, int nm, int ntq, const double complex* zmel, const do
       }
     }
   }
-  orc_zgemm('C', 'N', ntq, ntq, ngb * nm, 1.0, cz, (long)ngb * nm, zmel, (long)ngb * nm, 1.0, zsec, ldz);
-  for (int i = 0; i < ntq; ++i) {
-    double complex z = zsec[i + (long)ldz * i];
-    double zi = cimag(z);
-    if (zi > 0.0) zi = 0.0;
-    zsec[i + (long)ldz * i] = creal(z) + I * zi;
-  }
   free(w);
-  free(cz);
   free(e_iw);
   free(e_col);
   free(e_w);
   return (int)ne;
+}
+
+int orc_sigc_real(int ngb, int nm, int ntq, const double complex* zmel, const double complex* wr, int ldw, int nw,
+                  const double* freq_r, const double* omega, const double* ekc, double ef, double esmr, int nctot,
+                  int ns2r, double wkkr, double complex* zsec, int ldz) {
+  double complex* cz = (double complex*)calloc((size_t)ngb * nm * ntq, sizeof(double complex));
+  const int ne = sigc_real_cz(ngb, nm, ntq, zmel, wr, ldw, nw, freq_r, omega, ekc, ef, esmr, nctot, ns2r, wkkr, cz);
+  if (ne >= 0) sigc_final(ngb, nm, ntq, cz, zmel, zsec, ldz);
+  free(cz);
+  return ne;
+}
+
+/* get_correlation_block (S/m_sxcf_gemm.f90:423-608): imaginary-axis and
+ * real-axis parts in one czmelwc, one final product, then the clip. */
+int orc_sigc_full(int ngb, int nm, int ntq, int niw, const double complex* zmel, const double complex* wc, int ldw,
+                  const double* wgtim, const double complex* wr, int ldwr, int nw, const double* freq_r,
+                  const double* omega, const double* ekc, double ef, double esmr, int nctot, int ns2r, double wkkr,
+                  double complex* zsec, int ldz) {
+  double complex* cz = (double complex*)malloc(sizeof(double complex) * ngb * (long)nm * ntq);
+  sigc_imag_cz(ngb, nm, ntq, niw, zmel, wc, ldw, wgtim, cz);
+  const int ne = sigc_real_cz(ngb, nm, ntq, zmel, wr, ldwr, nw, freq_r, omega, ekc, ef, esmr, nctot, ns2r, wkkr, cz);
+  if (ne >= 0) sigc_final(ngb, nm, ntq, cz, zmel, zsec, ldz);
+  free(cz);
+  return ne;
 }
 
 /* wfacx2 / weavx2 (S/wfacx.f90:19-49). */

@@ -128,6 +128,12 @@ int orc_sigc_real(int ngb, int nm, int ntq, const double complex* zmel, const do
                   const double* freq_r, const double* omega, const double* ekc, double ef, double esmr, int nctot,
                   int ns2r, double wkkr, double complex* zsec, int ldz);
 
+/* get_correlation_block (S/m_sxcf_gemm.f90:423-608): both parts in one czmelwc. */
+int orc_sigc_full(int ngb, int nm, int ntq, int niw, const double complex* zmel, const double complex* wc, int ldw,
+                  const double* wgtim, const double complex* wr, int ldwr, int nw, const double* freq_r,
+                  const double* omega, const double* ekc, double ef, double esmr, int nctot, int ns2r, double wkkr,
+                  double complex* zsec, int ldz);
+
 /* wfacx2 / weavx2 (S/wfacx.f90:19-49). */
 double orc_wfacx2(double e1, double e2, double ek, double esmr);
 double orc_weavx2(double e1, double e2, double ek, double esmr);

@@ -88,3 +88,36 @@ def test_sigc_real_pole_outside_mesh():
         assert e.value.status == 1
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("bf16x3", 1e-4)])
+def test_sigc_full_dev_vs_oracle(orc, prec, tol):
+    """Imaginary + real axis in one czmelwc, k by k with the clip after each k,
+    against orc_sigc_full called per k (the reference's order).  The real-axis
+    W has complex diagonals, so the clip acts."""
+    import ecalj_amd
+
+    say("sigc_full", prec)
+    ngb, nm, ntq, nk, niw = 64, 19, 6, 3, 6
+    z, wr, ekc, omega, wk, fr = _case(11, ngb, nm, ntq, nk)
+    rng = np.random.default_rng(12)
+    a = rng.standard_normal((ngb, ngb, niw + 1, nk)) + 1j * rng.standard_normal((ngb, ngb, niw + 1, nk))
+    wc = a + a.conj().transpose(1, 0, 2, 3)
+    g = rng.standard_normal((niw + 1, nm, ntq, nk))
+    ef, esmr = 0.0, 0.01
+    ref = np.zeros((ntq, ntq), complex, order="F")
+    for k in range(nk):
+        ref, _ = orc.sigc_full(z[..., k], wc[..., k], g[..., k], wr[..., k], fr, omega, ekc[:, k], ef, esmr, wk[k],
+                               zsec=ref)
+    h = ecalj_amd.Handle(0, prec)
+    try:
+        cst = np.complex128 if prec == "f64" else np.complex64
+        dz, dwc, dg, dwr = h.to_dev(z.astype(cst)), h.to_dev(wc), h.to_dev(g), h.to_dev(wr)
+        dsec = h.array((ntq, ntq))
+        dsec.zero()
+        h.sigc_full_dev(nk, dz, ngb, nm, ntq, niw, dwc, dg, dwr, ngb * ngb * len(fr), len(fr) - 1, fr, omega,
+                        np.asfortranarray(ekc), ef, esmr, wk, dsec)
+        got = dsec.download()
+    finally:
+        h.close()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < tol

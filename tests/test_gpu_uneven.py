@@ -41,7 +41,10 @@ def test_uneven_f64_parity(case):
 
     qp, ref = case
     g = UnevenQPointGPU(qp, "f64")
-    g.step()
+    g.chi0()
+    g.wbuild()
+    g.make_wk()     # W_c at the internal k from this W_c(q), as the oracle pipeline does
+    g.sigc()
     chi, wc, zsec, eqp = g.results()
     g.h.close()
     assert rel(chi, ref.chi) < 1e-10
@@ -58,7 +61,10 @@ def test_uneven_bf16x3_grouped(case, mode):
     qp, ref = case
     g = UnevenQPointGPU(qp, "bf16x3")
     g.h.set_tuning(mode)
-    g.step()
+    g.chi0()
+    g.wbuild()
+    g.make_wk()
+    g.sigc()
     _, _, zsec, eqp = g.results()
     g.h.close()
     assert rel(zsec, ref.zsec) < 1e-4
@@ -75,7 +81,8 @@ def test_grouped_equals_uniform_when_windows_agree():
         h = ecalj_amd.Handle(0, prec)
         cst = np.complex128 if prec == "f64" else np.complex64
         z = (rng.standard_normal((ngb, nm, ntq, nk)) + 1j * rng.standard_normal((ngb, nm, ntq, nk))).astype(cst)
-        w = rng.standard_normal((ngb, ngb, niw + 1, nk)) + 1j * rng.standard_normal((ngb, ngb, niw + 1, nk))
+        a = rng.standard_normal((ngb, ngb, niw + 1, nk)) + 1j * rng.standard_normal((ngb, ngb, niw + 1, nk))
+        w = a + a.conj().transpose(1, 0, 2, 3)  # Hermitian, as W_c(i nu) is: clipping per k or once then agree
         g = rng.standard_normal((niw + 1, nm, ntq, nk))
         dz, dw, dg = h.to_dev(z), h.to_dev(w), h.to_dev(g)
         s1, s2 = h.array((ntq, ntq)), h.array((ntq, ntq))

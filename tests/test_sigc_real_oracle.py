@@ -106,3 +106,22 @@ def test_real_axis_out_of_mesh_is_an_error(orc):
     wr = np.zeros((z.shape[0], z.shape[0], 21), complex)
     with pytest.raises(ValueError):
         orc.sigc_real(z, wr, fr, omega, ekc, 0.0, 0.01, 0.2)
+
+
+def test_full_equals_imag_plus_real_without_clipping(orc):
+    """orc_sigc_full puts both parts in one czmelwc: with a Hermitian W on
+    both axes (real diagonals) no clip acts, and it equals the sum of the
+    separate imaginary-axis and real-axis results."""
+    rng, z, ekc, omega = _setup(9)
+    ngb, nm, ntq = z.shape
+    niw, nw = 5, 270
+    fr = _mesh(nw)
+    herm = lambda a: a + a.conj().transpose(1, 0, 2)
+    wc = herm(rng.standard_normal((ngb, ngb, niw + 1)) + 1j * rng.standard_normal((ngb, ngb, niw + 1)))
+    wr = herm(rng.standard_normal((ngb, ngb, nw + 1)) + 1j * rng.standard_normal((ngb, ngb, nw + 1)))
+    g = rng.standard_normal((niw + 1, nm, ntq))
+    full, ne = orc.sigc_full(z, wc, g, wr, fr, omega, ekc, 0.0, 0.01, 0.3)
+    a = orc.sigc_imag(z, wc, g)
+    b, ne2 = orc.sigc_real(z, wr, fr, omega, ekc, 0.0, 0.01, 0.3)
+    assert ne == ne2 > 0
+    assert np.abs(full - (a + b)).max() < 1e-12 * np.abs(full).max()
