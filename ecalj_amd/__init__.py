@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -156,6 +157,7 @@ class DevArray:
         p = C.c_void_p()
         _chk(lib().ecalj_dev_alloc(h.ptr, self.nbytes, C.byref(p)), "dev_alloc")
         self.ptr = p.value
+        h._arrays.add(self)  # freed by the handle's close() at the latest
 
     @classmethod
     def from_host(cls, h, a, dtype=None):
@@ -216,11 +218,15 @@ class Handle:
 
     def __init__(self, device: int = 0, precision: str = "f64"):
         self._h = C.c_void_p()
+        self._arrays = weakref.WeakSet()
         _chk(lib().ecalj_create(C.byref(self._h), device), "ecalj_create")
         self.set_precision(precision)
 
     def close(self):
+        """Free every device buffer allocated through this handle, then the handle."""
         if self._h:
+            for a in list(self._arrays):
+                a.free()
             lib().ecalj_destroy(self._h)
             self._h = C.c_void_p()
 
