@@ -245,6 +245,8 @@ def main():
     ap.add_argument("--also", default="C3,C4,C5",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run the library's plain copy kernel on MIB MiB (HBM-counter calibration)")
     ap.add_argument("--only", choices=["f64", "mixed"], default=None,
                     help="profile one form only (prints a short line instead of the contract JSON)")
     args = ap.parse_args()
@@ -256,6 +258,18 @@ def main():
     from ecalj_amd import synth
     from ecalj_amd.qsgw import flops_per_q
 
+    if args.copy_calibration:
+        import ecalj_amd
+
+        hc = ecalj_amd.Handle(0)
+        nbytes = args.copy_calibration << 20
+        a, b = hc.array(nbytes, np.uint8), hc.array(nbytes, np.uint8)
+        a.zero()
+        for _ in range(3):
+            hc.dev_copy(b, a)
+        hc.sync()
+        log(f"bench: copy kernel ran 3x on {args.copy_calibration} MiB (reads and writes {nbytes} bytes each)")
+        hc.close()
     cfg = synth.CONFIGS[args.config]
     log(f"bench: generating {cfg.name} inputs")
     qp = synth.make_qpoint(cfg)

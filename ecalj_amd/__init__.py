@@ -79,6 +79,7 @@ _SIGS = {
     "ecalj_memcpy_d2h": [_p, _p, _p, C.c_size_t],
     "ecalj_memcpy_d2d": [_p, _p, _p, C.c_size_t],
     "ecalj_dev_zero": [_p, _p, C.c_size_t],
+    "ecalj_dev_copy": [_p, _p, _p, C.c_size_t],
     "ecalj_dev_phase_similarity": [_p, _p, _i, _i, _p, _i, _p],
     "ecalj_sigma_herm": [_i, _p, _p, _p, _p, _p],
     "ecalj_mlp_nparam": [_i, _p, C.POINTER(_l)],
@@ -235,6 +236,11 @@ class Handle:
 
     def to_dev(self, a, dtype=None):
         return DevArray.from_host(self, a, dtype)
+
+    def dev_copy(self, dst, src, nbytes=None):
+        """Plain copy kernel on the handle's stream (asynchronous)."""
+        n = dst.nbytes if nbytes is None else nbytes
+        _chk(lib().ecalj_dev_copy(self._h, _t_ptr(dst), _t_ptr(src), n), "dev_copy")
 
     def memcpy_d2d(self, dst, src, nbytes=None):
         n = dst.nbytes if nbytes is None else nbytes

@@ -451,6 +451,11 @@ ecalj_status ecalj_dev_zero(ecalj_handle h, void* ptr, size_t bytes) {
   return dev_zero(h, ptr, bytes);
 }
 
+ecalj_status ecalj_dev_copy(ecalj_handle h, void* dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!dst || !src))) return ECALJ_ERR_ARG;
+  return dev_copy(h, dst, src, bytes);
+}
+
 ecalj_status ecalj_dev_phase_similarity(ecalj_handle h, const void* a, int ngb, int nmat, const void* d, int nk,
                                         void* out) {
   if (!h || !a || !d || !out) return ECALJ_ERR_ARG;

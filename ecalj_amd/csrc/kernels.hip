@@ -870,6 +870,23 @@ ecalj_status dev_zero(ecalj_ctx* h, void* p, size_t bytes) {
   return ECALJ_OK;
 }
 
+// dst = src, 16-byte vectors (both 16-byte aligned, bytes a multiple of 16)
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n16; e += stride) dst[e] = src[e];
+}
+
+ecalj_status dev_copy(ecalj_ctx* h, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return ECALJ_OK;
+  if (reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16 || bytes % 16)
+    return ECALJ_ERR_ARG;
+  const int64_t n16 = bytes / 16;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n16 + 255) / 256, 16384));
+  hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, h->stream, (const uint4*)src, (uint4*)dst, n16);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
 // out[k][m](I,J) = d_k(I) * a[m](I,J) * conj(d_k(J))
 __global__ void phase_sim_kernel(const cd* __restrict__ a, int n, int nmat, const cd* __restrict__ d, int nk,
                                  cd* __restrict__ out) {

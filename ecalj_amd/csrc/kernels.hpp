@@ -25,6 +25,7 @@ ecalj_status sig_weights(ecalj_ctx* h, int nbatch, int niw, int nm, int ntq, int
                          const double* omega, const double* ekc, double esmr, double ua, const double* wkkr,
                          const double* x, const double* wt, const double* expa, double* wgt);
 
+ecalj_status dev_copy(ecalj_ctx* h, void* dst, const void* src, size_t bytes);
 ecalj_status dev_zero(ecalj_ctx* h, void* p, size_t bytes);
 ecalj_status phase_similarity(ecalj_ctx* h, const cd* a, int n, int nmat, const cd* d, int nk, cd* out);
 

@@ -287,7 +287,15 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
       }
 }
 
-// Uniform batch: block (x, y, z) = (row tile, column tile, internal k).
+// Uniform batch.  G == 0: block (x, y, z) = (row tile, column tile, internal k).
+// G > 0, XCD-aware: a 1-D grid whose blocks the dispatcher deals round-robin
+// to the 8 XCDs (block p on XCD p % 8, in increasing p).  XCD x's j-th block
+// belongs to super-tile (j / S)*8 + x of S = R*G blocks: all R row tiles times
+// G consecutive column tiles of one k.  The S blocks of a super-tile run
+// together on one XCD, read each W stage of their k together (L2 hits) and
+// keep their G zmel panels in that XCD's L2 across the frequency loop; the 8
+// XCDs work on consecutive super-tiles, mostly of the same k.  Blocks past
+// the last tile exit at once.
 template <int BN, int NSTG>
 __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
                                                               int64_t as_stride,
@@ -295,11 +303,25 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
                                                               int64_t bs_stride, const double* __restrict__ wgt,
                                                               int64_t g_stride, int n, int nw1, int v0, int nwg,
                                                               int ncol, int MP, int NS, int NCP, int accumulate,
-                                                              cf* __restrict__ cz, int64_t cz_stride) {
+                                                              cf* __restrict__ cz, int64_t cz_stride, int G,
+                                                              int nk) {
   __shared__ SigSmem<BN, NSTG> s;
-  const int b = blockIdx.z;
+  int b, r, c;
+  if (G == 0) {
+    r = blockIdx.x;
+    c = blockIdx.y;
+    b = blockIdx.z;
+  } else {
+    const int R = MP / QBM, Ct = NCP / BN, NG = (Ct + G - 1) / G, S = R * G;
+    const int p = blockIdx.x, x = p & 7, j = p >> 3;
+    const int sigma = (j / S) * 8 + x, loc = j % S;
+    b = sigma / NG;
+    r = loc % R;
+    c = (sigma % NG) * G + loc / R;
+    if (b >= nk || c >= Ct) return;
+  }
   sig_tile<BN, NSTG>(s, as + as_stride * b, bs + bs_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS,
-                     NCP, blockIdx.x * QBM, blockIdx.y * BN, accumulate, cz + cz_stride * b);
+                     NCP, r * QBM, c * BN, accumulate, cz + cz_stride * b);
 }
 
 // Grouped, persistent: internal k-points with different column counts.  The
@@ -334,21 +356,46 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_grouped_kernel(const int*
 static inline int pad_to(int x, int m) { return (x + m - 1) / m * m; }
 constexpr int QBN_MAX = 128;  // columns are padded for the widest variant
 
-// Kernel variants of the tuner: {column tile, stages}
-struct SigVariant { int bn, nstg; const char* name; };
-static const SigVariant kSigVariants[] = {{64, 3, "128x64x3"}, {128, 2, "128x128x2"}};
-constexpr int kNumSigVariants = 2;
+// Kernel variants of the tuner: {column tile, stages, XCD-aware order}
+struct SigVariant { int bn, nstg, xcd; const char* name; };
+static const SigVariant kSigVariants[] = {{64, 3, 0, "128x64x3"},
+                                          {128, 2, 0, "128x128x2"},
+                                          {64, 3, 1, "128x64x3/xcd"},
+                                          {128, 2, 1, "128x128x2/xcd"}};
+constexpr int kNumSigVariants = 4;
+constexpr int kNumSigGroupedVariants = 2;  // the grouped kernel's own tile order
+
+// Column tiles per super-tile of the XCD-aware order: the super-tile fills
+// about 32 CUs (one XCD) and its zmel panels (G x BN columns x NP x 8 bytes)
+// stay within ~3.5 MB of the XCD's 4 MB L2.
+static int xcd_group(int R, int Ct, int BN, int NP) {
+  int G = std::max(1, 32 / std::max(R, 1));
+  const int64_t panel = (int64_t)BN * NP * 8;
+  while (G > 1 && G * panel > (int64_t)3584 * 1024) --G;
+  return std::min(G, Ct);
+}
 
 template <int BN, int NSTG>
 static ecalj_status sig_launch_t(ecalj_ctx* h, int nb, int n, int ncol, int nw1, int MP, int NS, int NCP,
                                  const unsigned short* as, int64_t as_stride, const unsigned short* bs,
-                                 int64_t bs_stride, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride) {
+                                 int64_t bs_stride, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride,
+                                 bool xcd) {
   dim3 grid(MP / QBM, NCP / BN, nb);
-  if (grid.y > 65535 || grid.z > 65535) return ECALJ_ERR_DIM;
+  int G = 0;
+  if (xcd) {
+    const int R = MP / QBM, Ct = NCP / BN;
+    G = xcd_group(R, Ct, BN, NS * QBK);
+    const int64_t NG = (Ct + G - 1) / G, S = (int64_t)R * G;
+    const int64_t blocks = 8 * S * ((nb * NG + 7) / 8);
+    if (blocks > 0x7fffffff) return ECALJ_ERR_DIM;
+    grid = dim3((unsigned)blocks, 1, 1);
+  } else if (grid.y > 65535 || grid.z > 65535) {
+    return ECALJ_ERR_DIM;
+  }
   for (int v0 = 0; v0 < nw1; v0 += QNW) {
     const int nwg = std::min(QNW, nw1 - v0);
     hipLaunchKernelGGL((sig_bf3_kernel<BN, NSTG>), grid, dim3(kThreads), 0, h->stream, as, as_stride, bs, bs_stride,
-                       wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride);
+                       wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G, nb);
     ECALJ_HIP(hipGetLastError());
   }
   return ECALJ_OK;
@@ -373,12 +420,12 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
                      bs_stride, nb);
   ECALJ_HIP(hipGetLastError());
   auto run = [&](int var) -> ecalj_status {
-    switch (var) {
-      case 0: return sig_launch_t<64, 3>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt,
-                                         g_stride, cz, cz_stride);
-      default: return sig_launch_t<128, 2>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt,
-                                           g_stride, cz, cz_stride);
-    }
+    const bool xcd = kSigVariants[var].xcd != 0;
+    if (kSigVariants[var].bn == 64)
+      return sig_launch_t<64, 3>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride, cz,
+                                 cz_stride, xcd);
+    return sig_launch_t<128, 2>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride, cz,
+                                cz_stride, xcd);
   };
   // variant: forced by the handle, cached for this shape, or measured now
   // (every variant computes the same sums in the same order, so whichever ran last leaves the result)
@@ -477,16 +524,16 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
   int var = -1;
   const TuneKey key{TUNE_SIGC_BF3_GROUPED, n, cols, nw1, nk};
   if (h->tune_mode > 0) {
-    var = std::min(h->tune_mode, kNumSigVariants) - 1;
+    var = (h->tune_mode - 1) % kNumSigGroupedVariants;
   } else if (tune_lookup(h, key, &var)) {
   } else if (h->capturing) {
     var = 0;
   } else {
-    std::vector<double> ms(kNumSigVariants);
-    for (int v = 0; v < kNumSigVariants; ++v) ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
+    std::vector<double> ms(kNumSigGroupedVariants);
+    for (int v = 0; v < kNumSigGroupedVariants; ++v) ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
     var = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
     std::vector<const char*> names;
-    for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
+    for (int v = 0; v < kNumSigGroupedVariants; ++v) names.push_back(kSigVariants[v].name);
     tune_store(h, key, var, "sigc_bf3_grouped", names, ms);
     return ECALJ_OK;
   }

@@ -68,7 +68,9 @@ ecalj_status ecalj_kernel_time(ecalj_handle h, int kclass, double* ms, int64_t* 
  * variant of a tuned kernel runs once, timed with HIP events (that call
  * waits for the stream), and the fastest is kept for the shape; mode k > 0:
  * always variant k.  Tuned kernels: the bf16x3 Sigma_c contraction (variants
- * 1 = 128x64 tiles, 3 stages; 2 = 128x128 tiles, 2 stages).
+ * 1 = 128x64 tiles, 3 stages; 2 = 128x128 tiles, 2 stages; 3, 4 = the same
+ * with the XCD-aware block order), and its grouped form (variants 1, 2; k > 2
+ * wraps).
  * ecalj_tuning_report writes the decisions so far, one line per shape, as a
  * NUL-terminated string of at most len bytes. */
 ecalj_status ecalj_set_tuning(ecalj_handle h, int mode);
@@ -106,6 +108,10 @@ ecalj_status ecalj_memcpy_h2d(ecalj_handle h, void* dst, const void* src, size_t
 ecalj_status ecalj_memcpy_d2h(ecalj_handle h, void* dst, const void* src, size_t bytes);
 ecalj_status ecalj_memcpy_d2d(ecalj_handle h, void* dst, const void* src, size_t bytes);
 ecalj_status ecalj_dev_zero(ecalj_handle h, void* ptr, size_t bytes);
+/* dst = src by a plain vector-copy kernel (16-byte aligned, bytes % 16 == 0):
+ * asynchronous on the handle's stream; also the reference kernel of known
+ * traffic for HBM-counter calibration. */
+ecalj_status ecalj_dev_copy(ecalj_handle h, void* dst, const void* src, size_t bytes);
 ecalj_status ecalj_dev_phase_similarity(ecalj_handle h, const void* a, int ngb, int nmat,
                                         const void* d, int nk, void* out);
 
