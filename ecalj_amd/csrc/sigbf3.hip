@@ -41,11 +41,17 @@ __global__ __launch_bounds__(256) void split_w_kernel(const cd* __restrict__ w, 
   const int v = blockIdx.z % nw1, b = blockIdx.z / nw1;
   const cd* W = w + w_stride * b + (int64_t)ldw * n * v;
   const int tx = threadIdx.x % SPT, ty = threadIdx.x / SPT;  // 32 x 8
+  // a tile wholly above the diagonal reads only W(I,J), wholly below only W(J,I)
+  const bool need_up = I0 <= J0 + SPT - 1, need_lo = I0 + SPT - 1 > J0;
   for (int r = ty; r < SPT; r += 8) {
-    const int I = I0 + tx, J = J0 + r;
-    up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
-    const int J2 = J0 + tx, I2 = I0 + r;
-    lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
+    if (need_up) {
+      const int I = I0 + tx, J = J0 + r;
+      up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
+    }
+    if (need_lo) {
+      const int J2 = J0 + tx, I2 = I0 + r;
+      lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
+    }
   }
   __syncthreads();
   typedef unsigned short us8v __attribute__((ext_vector_type(8)));

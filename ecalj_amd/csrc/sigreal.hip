@@ -356,8 +356,6 @@ static ecalj_status sig_real_t(ecalj_ctx* h, int nb, const T* zmel, int n, int n
   const int64_t zstride = (int64_t)n * nm * ntq;
   if (P.np > 0) ECALJ_TRY(real_entries<T>(h, P, rs, zmel, zstride, n, 0, nb));
   // the final kernel also clips k by k when there are no poles
-  static const int zero_list[2] = {0, 0};
-  (void)zero_list;
   if (P.np == 0) {
     std::vector<unsigned char> blob(sizeof(int) * (ntq + 1) + 8, 0);
     ECALJ_TRY(upload_blob(h, ecalj_ctx::S_RTAB, blob, &P.d));
@@ -376,8 +374,9 @@ ecalj_status sig_real(ecalj_ctx* h, int mode, int nb, const void* zmel, int n, i
   return sig_real_t<cf>(h, nb, (const cf*)zmel, n, nm, ntq, rs, zsec, ldz);
 }
 
-// The real-axis columns of the k-points [k0, k1) added into czmelwc
-// (cz of k at cz + czstride*(k - k0)), before the final product.
+// A pole plan kept across the scratch groups of one Sigma_c call: the
+// real-axis columns of the k-points [k0, k1) are added into czmelwc (cz of k
+// at cz + (k - k0) * n*nm*ntq) before the final product.
 struct RealPlanHandle {
   RealPlan P;
 };
