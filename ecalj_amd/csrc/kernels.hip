@@ -368,11 +368,10 @@ ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const double* s, int n, int nw
 //   (1) gj_diag: P = A[K,K] -> Pinv (LDS elimination); Cp = A[:,K]; A[K,K] = I
 //   (2) row panel:  A[K,:] = Pinv * A[K,:]          (A[K,K] becomes Pinv)
 //   (3) trailing:   A[i,:] = (j in K ? 0 : A[i,:]) - Cp[i,:] * A[K,:],  i not in K
-// Pivot block width: 64 for large matrices (half the panel steps, K = 64 updates), else 32.
-constexpr int GJB_MAX = 64;
-static inline int gj_block(int n) { return n >= 512 ? 64 : 32; }
+// Pivot block width.  (64 halves the panel steps, but its 64-step diagonal
+// inverse on one block per matrix costs more than the wider updates save.)
+constexpr int GJB = 32;
 
-template <int GJB>
 __global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n, int64_t lda,
                                                       int64_t astride, int k0, int b,
                                                       cd* __restrict__ pinv, cd* __restrict__ cp,
@@ -439,18 +438,13 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n,
 
 ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat) {
   if (n == 0 || nmat == 0) return ECALJ_OK;
-  const int GJB = gj_block(n);
   cd *pinv, *cp;
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJP, sizeof(cd) * GJB_MAX * GJB_MAX * nmat, (void**)&pinv));
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJC, sizeof(cd) * (size_t)n * GJB_MAX * nmat, (void**)&cp));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJP, sizeof(cd) * GJB * GJB * nmat, (void**)&pinv));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJC, sizeof(cd) * (size_t)n * GJB * nmat, (void**)&cp));
   for (int k0 = 0; k0 < n; k0 += GJB) {
     const int b = std::min(GJB, n - k0);
-    if (GJB == 64)
-      hipLaunchKernelGGL((gj_diag_kernel<64>), dim3(nmat), dim3(256), 0, h->stream, a, n, lda, astride, k0, b, pinv,
-                         cp, h->d_flag);
-    else
-      hipLaunchKernelGGL((gj_diag_kernel<32>), dim3(nmat), dim3(256), 0, h->stream, a, n, lda, astride, k0, b, pinv,
-                         cp, h->d_flag);
+    hipLaunchKernelGGL(gj_diag_kernel, dim3(nmat), dim3(256), 0, h->stream, a, n, lda, astride, k0, b, pinv, cp,
+                       h->d_flag);
     ECALJ_HIP(hipGetLastError());
     // (2) row panel: A[K,:] = Pinv (b x b) * A[K,:] (b x n), in place per column tile
     GemmCall r;

@@ -136,7 +136,7 @@ def test_chi0_freq(h, orc):
 
 
 # ------------------------------------------------------------------ W
-# n >= 512 takes 64-wide pivot blocks (520: a ragged last block of 8)
+# large n, including a ragged last pivot block (520 = 16 x 32 + 8)
 @pytest.mark.parametrize("n,nw,xm,wm,nspin", [(72, 3, True, True, 1), (100, 2, False, False, 2),
                                              (33, 4, True, False, 1), (150, 2, True, True, 1),
                                              (520, 2, True, True, 1), (640, 1, False, True, 1)])
