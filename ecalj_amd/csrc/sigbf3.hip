@@ -133,6 +133,7 @@ __global__ __launch_bounds__(256) void split_z_kernel(const cf* __restrict__ z, 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 typedef __attribute__((address_space(1))) void* gbl_ptr_t;
 typedef unsigned short us8q __attribute__((ext_vector_type(8)));
+typedef unsigned int u4q __attribute__((ext_vector_type(4)));
 
 template <int BN, int NSTG>
 struct alignas(16) SigSmem {
@@ -146,28 +147,34 @@ struct alignas(16) SigSmem {
 #define ECALJ_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(((N) & 0xF) | (((N) >> 4) << 14) | (0x7 << 4) | (0xF << 8))
 
 // One block tile of czmelwc: rows I0..I0+QBM of one internal k, columns
-// c0..c0+BN; four waves in 2 x 2, wave tile 64 x BN/2; NSTG stages of QBK in
-// LDS, NSTG-1 of them in flight.  A and B point at the k's plane sets
+// c0..c0+BN; NW waves in 2 x NW/2, wave tile 64 x 2BN/NW; NSTG stages of QBK
+// in LDS, NSTG-1 of them in flight.  A and B point at the k's plane sets
 // (A: 4 planes of MP x QBK per (v, ks); B: 4 planes of NCP x QBK per ks).
-template <int BN, int NSTG>
+// NW = 8 puts two waves on every SIMD: one wave alone issues a 16x16x32
+// bf16 MFMA only every ~30 cycles, two interleave to ~20.
+template <int BN, int NSTG, int NW>
 __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned short* __restrict__ Ak,
                                          const unsigned short* __restrict__ Bk, const double* __restrict__ g,
                                          int n, int nw1, int v0, int nwg, int ncol, int MP, int NS, int NCP,
                                          int I0, int c0, int accumulate, cf* __restrict__ C) {
   static_assert(NSTG == 2 || NSTG == 3, "two or three stages");
-  constexpr int TM = 4, TN = BN / 32;               // 16x16 tiles per wave
-  constexpr int ARW = 4 * QBM * QBK * 2 / 1024 / 4;  // 1 KB A runs per wave and stage
-  constexpr int BRW = 4 * BN * QBK * 2 / 1024 / 4;   // 1 KB B runs per wave and stage
+  static_assert(NW == 4 || NW == 8, "four or eight waves");
+  constexpr int NT = NW * 64, WN = NW / 2;
+  constexpr int TM = 4, TN = BN / (16 * WN);          // 16x16 tiles per wave
+  constexpr int ARW = 4 * QBM * QBK * 2 / 1024 / NW;  // 1 KB A runs per wave and stage
+  constexpr int BRW = 4 * BN * QBK * 2 / 1024 / NW;   // 1 KB B runs per wave and stage
+  static_assert(TN >= 1 && ARW * NW == 4 * QBM * QBK * 2 / 1024 && BRW * NW == 4 * BN * QBK * 2 / 1024,
+                "runs split evenly over the waves");
   constexpr int APP = QBM * QBK / 512, BPP = BN * QBK / 512;  // runs per plane
   const int t = threadIdx.x, lane = t & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave >> 1, wn = wave & 1;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave / WN, wn = wave % WN;
   const int64_t apst = (int64_t)MP * QBK, bpst = (int64_t)NCP * QBK;
   const unsigned short* A = Ak + (int64_t)I0 * QBK + lane * 8;
   const unsigned short* B = Bk + (int64_t)c0 * QBK + lane * 8;
   const int nst = nwg * NS;
 
   // weight table: wt[v][c] = wgt(v0+v, c0+c) (0 beyond ncol)
-  for (int e = t; e < nwg * BN; e += kThreads) {
+  for (int e = t; e < nwg * BN; e += NT) {
     const int v = e / BN, c = c0 + e % BN;
     s.wt[v][e % BN] = (c < ncol) ? (float)g[(v0 + v) + (int64_t)nw1 * c] : 0.f;
   }
@@ -206,70 +213,75 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
 
   for (int q = 0; q < NSTG - 1; ++q)
     if (q < nst) issue(q, q);
-  for (int st = 0; st < nst; ++st) {
-    const int buf = st % NSTG;
-    // stage st landed: the loads of later stages already issued may still be in flight
-    if (NSTG == 3 && st + 1 < nst) {
-      ECALJ_WAIT_VMCNT(ARW + BRW);
-    } else {
-      ECALJ_WAIT_VMCNT(0);
-    }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_barrier();
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    // the buffer of stage st+NSTG-1 was last read in iteration st-1, which every wave has left
-    if (st + NSTG - 1 < nst) issue(st + NSTG - 1, (st + NSTG - 1) % NSTG);
+  // frequency loop outside, k-slice loop inside: the inner loop touches only the
+  // per-frequency accumulators (MFMA-resident), the weighted totals only at
+  // the end of each frequency
+  int st = 0;
+  for (int vl = 0; vl < nwg; ++vl) {
+    for (int ks = 0; ks < NS; ++ks, ++st) {
+      const int buf = st % NSTG;
+      // stage st landed: the loads of later stages already issued may still be in flight
+      if (NSTG == 3 && st + 1 < nst) {
+        ECALJ_WAIT_VMCNT(ARW + BRW);
+      } else {
+        ECALJ_WAIT_VMCNT(0);
+      }
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_barrier();
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      // the buffer of stage st+NSTG-1 was last read in iteration st-1, which every wave has left
+      if (st + NSTG - 1 < nst) issue(st + NSTG - 1, (st + NSTG - 1) % NSTG);
 
-    us8q arh[TM], arl[TM], aih[TM], ail[TM], brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int o = boff + j * 16 * QBK;
-      brh[j] = *reinterpret_cast<const us8q*>(&s.b[buf][0][o]);
-      brl[j] = *reinterpret_cast<const us8q*>(&s.b[buf][1][o]);
-      bih[j] = *reinterpret_cast<const us8q*>(&s.b[buf][2][o]);
-      bil[j] = *reinterpret_cast<const us8q*>(&s.b[buf][3][o]);
-      nbh[j] = bih[j] ^ (unsigned short)0x8000;
-      nbl[j] = bil[j] ^ (unsigned short)0x8000;
-    }
-#pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int o = aoff + i * 16 * QBK;
-      arh[i] = *reinterpret_cast<const us8q*>(&s.a[buf][0][o]);
-      arl[i] = *reinterpret_cast<const us8q*>(&s.a[buf][1][o]);
-      aih[i] = *reinterpret_cast<const us8q*>(&s.a[buf][2][o]);
-      ail[i] = *reinterpret_cast<const us8q*>(&s.a[buf][3][o]);
-    }
-#define ECALJ_Q_TERM(ACC, AF, BF)                                                                           \
-  _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < TN; ++j) ACC[i][j] = \
-      __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, AF[i]), __builtin_bit_cast(bf8, BF[j]), \
-                                              ACC[i][j], 0, 0, 0);
-    // re = ar br - ai bi, im = ar bi + ai br; hi*hi + hi*lo + lo*hi of each
-    ECALJ_Q_TERM(cr, arh, brh)
-    ECALJ_Q_TERM(ci, arh, bih)
-    ECALJ_Q_TERM(cr, arh, brl)
-    ECALJ_Q_TERM(ci, arh, bil)
-    ECALJ_Q_TERM(cr, arl, brh)
-    ECALJ_Q_TERM(ci, arl, bih)
-    ECALJ_Q_TERM(cr, aih, nbh)
-    ECALJ_Q_TERM(ci, aih, brh)
-    ECALJ_Q_TERM(cr, aih, nbl)
-    ECALJ_Q_TERM(ci, aih, brl)
-    ECALJ_Q_TERM(cr, ail, nbh)
-    ECALJ_Q_TERM(ci, ail, brh)
-#undef ECALJ_Q_TERM
-
-    const int vl = st / NS;
-    if (st - vl * NS == NS - 1) {  // end of frequency: tot += wgt(v,c) * acc
+      us8q brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const float wv = s.wt[vl][wn * TN * 16 + j * 16 + (lane & 15)];
+        const int o = boff + j * 16 * QBK;
+        brh[j] = *reinterpret_cast<const us8q*>(&s.b[buf][0][o]);
+        brl[j] = *reinterpret_cast<const us8q*>(&s.b[buf][1][o]);
+        bih[j] = *reinterpret_cast<const us8q*>(&s.b[buf][2][o]);
+        bil[j] = *reinterpret_cast<const us8q*>(&s.b[buf][3][o]);
+        // -b: flip the sign bit of every bf16 (two per 32-bit word)
+        nbh[j] = __builtin_bit_cast(us8q, __builtin_bit_cast(u4q, bih[j]) ^ 0x80008000u);
+        nbl[j] = __builtin_bit_cast(us8q, __builtin_bit_cast(u4q, bil[j]) ^ 0x80008000u);
+      }
+      // one A row tile at a time: its 4 planes meet the TN column tiles
 #pragma unroll
-        for (int i = 0; i < TM; ++i) {
-          tr[i][j] += wv * cr[i][j];
-          ti[i][j] += wv * ci[i][j];
-          cr[i][j] = f4{0, 0, 0, 0};
-          ci[i][j] = f4{0, 0, 0, 0};
-        }
+      for (int i = 0; i < TM; ++i) {
+        const int o = aoff + i * 16 * QBK;
+        const us8q arh = *reinterpret_cast<const us8q*>(&s.a[buf][0][o]);
+        const us8q arl = *reinterpret_cast<const us8q*>(&s.a[buf][1][o]);
+        const us8q aih = *reinterpret_cast<const us8q*>(&s.a[buf][2][o]);
+        const us8q ail = *reinterpret_cast<const us8q*>(&s.a[buf][3][o]);
+#define ECALJ_Q_TERM(ACC, AF, BF)                                                                     \
+  _Pragma("unroll") for (int j = 0; j < TN; ++j) ACC[i][j] =                                          \
+      __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, AF), __builtin_bit_cast(bf8, BF[j]), \
+                                              ACC[i][j], 0, 0, 0);
+        // re = ar br - ai bi, im = ar bi + ai br; hi*hi + hi*lo + lo*hi of each
+        ECALJ_Q_TERM(cr, arh, brh)
+        ECALJ_Q_TERM(ci, arh, bih)
+        ECALJ_Q_TERM(cr, arh, brl)
+        ECALJ_Q_TERM(ci, arh, bil)
+        ECALJ_Q_TERM(cr, arl, brh)
+        ECALJ_Q_TERM(ci, arl, bih)
+        ECALJ_Q_TERM(cr, aih, nbh)
+        ECALJ_Q_TERM(ci, aih, brh)
+        ECALJ_Q_TERM(cr, aih, nbl)
+        ECALJ_Q_TERM(ci, aih, brl)
+        ECALJ_Q_TERM(cr, ail, nbh)
+        ECALJ_Q_TERM(ci, ail, brh)
+#undef ECALJ_Q_TERM
+      }
+    }
+    // end of frequency: tot += wgt(v,c) * acc
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const float wv = s.wt[vl][wn * TN * 16 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        tr[i][j] += wv * cr[i][j];
+        ti[i][j] += wv * ci[i][j];
+        cr[i][j] = f4{0, 0, 0, 0};
+        ci[i][j] = f4{0, 0, 0, 0};
       }
     }
   }
@@ -302,8 +314,8 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
 // keep their G zmel panels in that XCD's L2 across the frequency loop; the 8
 // XCDs work on consecutive super-tiles, mostly of the same k.  Blocks past
 // the last tile exit at once.
-template <int BN, int NSTG>
-__global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
+template <int BN, int NSTG, int NW>
+__global__ __launch_bounds__(NW * 64, 1) void sig_bf3_kernel(const unsigned short* __restrict__ as,
                                                               int64_t as_stride,
                                                               const unsigned short* __restrict__ bs,
                                                               int64_t bs_stride, const double* __restrict__ wgt,
@@ -326,8 +338,8 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_kernel(const unsigned sho
     c = (sigma % NG) * G + loc / R;
     if (b >= nk || c >= Ct) return;
   }
-  sig_tile<BN, NSTG>(s, as + as_stride * b, bs + bs_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS,
-                     NCP, r * QBM, c * BN, accumulate, cz + cz_stride * b);
+  sig_tile<BN, NSTG, NW>(s, as + as_stride * b, bs + bs_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP,
+                         NS, NCP, r * QBM, c * BN, accumulate, cz + cz_stride * b);
 }
 
 // Grouped, persistent: internal k-points with different column counts.  The
@@ -339,8 +351,8 @@ struct SigGroup {
   int64_t czoff;  // output of k (elements)
   int ncol, ncp;  // columns, padded columns
 };
-template <int BN, int NSTG>
-__global__ __launch_bounds__(kThreads, 1) void sig_bf3_grouped_kernel(const int* __restrict__ tiles, int ntiles,
+template <int BN, int NSTG, int NW>
+__global__ __launch_bounds__(NW * 64, 1) void sig_bf3_grouped_kernel(const int* __restrict__ tiles, int ntiles,
                                                                       const SigGroup* __restrict__ grp,
                                                                       const unsigned short* __restrict__ as,
                                                                       int64_t as_stride,
@@ -353,8 +365,8 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_grouped_kernel(const int*
     const int k = tiles[3 * t], it = tiles[3 * t + 1], jt = tiles[3 * t + 2];
     const SigGroup gk = grp[k];
     __syncthreads();  // the previous tile's LDS reads are done before this tile's loads land
-    sig_tile<BN, NSTG>(s, as + as_stride * k, bs + gk.boff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, gk.ncp,
-                       it * QBM, jt * BN, accumulate, cz + gk.czoff);
+    sig_tile<BN, NSTG, NW>(s, as + as_stride * k, bs + gk.boff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS,
+                           gk.ncp, it * QBM, jt * BN, accumulate, cz + gk.czoff);
   }
 }
 
@@ -362,14 +374,14 @@ __global__ __launch_bounds__(kThreads, 1) void sig_bf3_grouped_kernel(const int*
 static inline int pad_to(int x, int m) { return (x + m - 1) / m * m; }
 constexpr int QBN_MAX = 128;  // columns are padded for the widest variant
 
-// Kernel variants of the tuner: {column tile, stages, XCD-aware order}
-struct SigVariant { int bn, nstg, xcd; const char* name; };
-static const SigVariant kSigVariants[] = {{64, 3, 0, "128x64x3"},
-                                          {128, 2, 0, "128x128x2"},
-                                          {64, 3, 1, "128x64x3/xcd"},
-                                          {128, 2, 1, "128x128x2/xcd"}};
+// Kernel variants of the tuner: {column tile, stages, waves, XCD-aware order}
+struct SigVariant { int bn, nstg, nw, xcd; const char* name; };
+static const SigVariant kSigVariants[] = {{64, 3, 4, 0, "128x64x3"},
+                                          {128, 2, 4, 0, "128x128x2"},
+                                          {128, 2, 8, 0, "128x128x2/8w"},
+                                          {128, 2, 8, 1, "128x128x2/8w/xcd"}};
 constexpr int kNumSigVariants = 4;
-constexpr int kNumSigGroupedVariants = 2;  // the grouped kernel's own tile order
+constexpr int kNumSigGroupedVariants = 3;  // the first three; the grouped kernel has its own tile order
 
 // Column tiles per super-tile of the XCD-aware order: the super-tile fills
 // about 32 CUs (one XCD) and its zmel panels (G x BN columns x NP x 8 bytes)
@@ -381,7 +393,7 @@ static int xcd_group(int R, int Ct, int BN, int NP) {
   return std::min(G, Ct);
 }
 
-template <int BN, int NSTG>
+template <int BN, int NSTG, int NW>
 static ecalj_status sig_launch_t(ecalj_ctx* h, int nb, int n, int ncol, int nw1, int MP, int NS, int NCP,
                                  const unsigned short* as, int64_t as_stride, const unsigned short* bs,
                                  int64_t bs_stride, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride,
@@ -400,8 +412,9 @@ static ecalj_status sig_launch_t(ecalj_ctx* h, int nb, int n, int ncol, int nw1,
   }
   for (int v0 = 0; v0 < nw1; v0 += QNW) {
     const int nwg = std::min(QNW, nw1 - v0);
-    hipLaunchKernelGGL((sig_bf3_kernel<BN, NSTG>), grid, dim3(kThreads), 0, h->stream, as, as_stride, bs, bs_stride,
-                       wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G, nb);
+    hipLaunchKernelGGL((sig_bf3_kernel<BN, NSTG, NW>), grid, dim3(NW * 64), 0, h->stream, as, as_stride, bs,
+                       bs_stride, wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G,
+                       nb);
     ECALJ_HIP(hipGetLastError());
   }
   return ECALJ_OK;
@@ -428,10 +441,13 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
   auto run = [&](int var) -> ecalj_status {
     const bool xcd = kSigVariants[var].xcd != 0;
     if (kSigVariants[var].bn == 64)
-      return sig_launch_t<64, 3>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride, cz,
-                                 cz_stride, xcd);
-    return sig_launch_t<128, 2>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride, cz,
-                                cz_stride, xcd);
+      return sig_launch_t<64, 3, 4>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride, cz,
+                                    cz_stride, xcd);
+    if (kSigVariants[var].nw == 4)
+      return sig_launch_t<128, 2, 4>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride,
+                                     cz, cz_stride, xcd);
+    return sig_launch_t<128, 2, 8>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride, cz,
+                                   cz_stride, xcd);
   };
   // variant: forced by the handle, cached for this shape, or measured now
   // (every variant computes the same sums in the same order, so whichever ran last leaves the result)
@@ -518,10 +534,13 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
     for (int v0 = 0; v0 < nw1; v0 += QNW) {
       const int nwg = std::min(QNW, nw1 - v0);
       if (var == 0)
-        hipLaunchKernelGGL((sig_bf3_grouped_kernel<64, 3>), dim3(grid), dim3(kThreads), 0, h->stream, dt, ntiles, dg,
+        hipLaunchKernelGGL((sig_bf3_grouped_kernel<64, 3, 4>), dim3(grid), dim3(256), 0, h->stream, dt, ntiles, dg,
                            as, as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
+      else if (var == 1)
+        hipLaunchKernelGGL((sig_bf3_grouped_kernel<128, 2, 4>), dim3(grid), dim3(256), 0, h->stream, dt, ntiles,
+                           dg, as, as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
       else
-        hipLaunchKernelGGL((sig_bf3_grouped_kernel<128, 2>), dim3(grid), dim3(kThreads), 0, h->stream, dt, ntiles,
+        hipLaunchKernelGGL((sig_bf3_grouped_kernel<128, 2, 8>), dim3(grid), dim3(512), 0, h->stream, dt, ntiles,
                            dg, as, as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
       ECALJ_HIP(hipGetLastError());
     }

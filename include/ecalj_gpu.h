@@ -68,9 +68,9 @@ ecalj_status ecalj_kernel_time(ecalj_handle h, int kclass, double* ms, int64_t* 
  * variant of a tuned kernel runs once, timed with HIP events (that call
  * waits for the stream), and the fastest is kept for the shape; mode k > 0:
  * always variant k.  Tuned kernels: the bf16x3 Sigma_c contraction (variants
- * 1 = 128x64 tiles, 3 stages; 2 = 128x128 tiles, 2 stages; 3, 4 = the same
- * with the XCD-aware block order), and its grouped form (variants 1, 2; k > 2
- * wraps).
+ * 1 = 128x64 tiles, 3 stages, 4 waves; 2 = 128x128 tiles, 2 stages, 4 waves;
+ * 3 = 128x128, 2 stages, 8 waves; 4 = variant 3 with the XCD-aware block
+ * order), and its grouped form (variants 1-3; k > 3 wraps).
  * ecalj_tuning_report writes the decisions so far, one line per shape, as a
  * NUL-terminated string of at most len bytes. */
 ecalj_status ecalj_set_tuning(ecalj_handle h, int mode);
