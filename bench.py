@@ -32,6 +32,13 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense
 
+# HBM traffic per launch of the dominant kernel, from the calibrated FETCH_SIZE +
+# WRITE_SIZE passes whose summary is committed (the counters cannot run inside
+# this timed process); None where no pass was taken.
+MEASURED_TRAFFIC = {
+    ("C2", "bf16x3"): (119.7e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -217,17 +224,24 @@ def bench_c5(log_=None):
     }
 
 
-def roofline(fl, info, precision, steps):
+def roofline(fl, info, precision, steps, config="C2"):
     ms, nl = info["ktimes"]["sigc"]
     launches_per_step = max(1, nl // steps)
     flop_launch = fl["sigc"] / launches_per_step
     avg_s = ms / 1e3 / max(nl, 1)
     achieved = flop_launch / avg_s / 1e12
     peak = PEAK[precision]
-    r = {"kernel": "sig_freq_gemm_kernel (Sigma_c: W_c(iv)^H x frequency-weighted zmel, all internal k)",
+    kname = ("sig_bf3_kernel (bf16x3 Sigma_c: per-frequency W_c(iv)^H zmel, weighted sum, all internal k)"
+             if precision == "bf16x3" else
+             "sig_freq_gemm_kernel (Sigma_c: W_c(iv)^H x frequency-weighted zmel, all internal k)")
+    tr = MEASURED_TRAFFIC.get((config, precision))
+    r = {"kernel": kname,
          "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-         "frac": round(achieved / peak, 4), "traffic": None,
+         "frac": round(achieved / peak, 4), "traffic": tr[0] if tr else None,
          "flop_per_launch": flop_launch, "avg_launch_ms": round(avg_s * 1e3, 4)}
+    if tr:
+        r["traffic_unit"] = "bytes per launch (HBM/fabric reads + writes, calibrated counters)"
+        r["traffic_source"] = tr[1]
     if precision == "bf16x3":
         r["note"] = ("achieved counts algorithmic complex flops (8 per complex MAC); each costs 3 bf16 MFMA "
                      "products, so the attainable algorithmic rate is peak/3")
@@ -304,12 +318,12 @@ def main():
         "tflops": round(ftot / s_per_q / 1e12, 3),
         "fp64_form": {"s_per_q": round(f64["s_per_q"], 6), "tflops": round(ftot / f64["s_per_q"] / 1e12, 3),
                       "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in f64["ktimes"].items()},
-                      "roofline": roofline(fl, f64, "f64", args.steps)},
+                      "roofline": roofline(fl, f64, "f64", args.steps, cfg.name)},
         "speedup_vs_fp64_form": round(f64["s_per_q"] / s_per_q, 3),
         "qp_err_meV_vs_fp64_form": round(qp_err_mev, 5),
         "flops_per_q": ftot,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in mix["ktimes"].items()},
-        "roofline": roofline(fl, mix, args.mixed, args.steps),
+        "roofline": roofline(fl, mix, args.mixed, args.steps, cfg.name),
         "tuning": mix["tuning"],
     }
     if not args.no_cpu_baseline:
@@ -343,7 +357,7 @@ def main():
             "qp_err_meV_vs_fp64_form": round(float(np.abs(amx["eqp"] - a64["eqp"]).max() * 1e3), 5),
             "tflops": round(sum(f2.values()) / amx["s_per_q"] / 1e12, 3),
             "kernel_ms_per_step": {k: round(v[0] / 2, 3) for k, v in amx["ktimes"].items()},
-            "roofline": roofline(f2, amx, args.mixed, 2),
+            "roofline": roofline(f2, amx, args.mixed, 2, name),
         }
         del q2
     print(json.dumps(out), flush=True)
