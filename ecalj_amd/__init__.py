@@ -63,6 +63,12 @@ _SIGS = {
     "ecalj_chi0_freq_dev": [_p, _p, _p, _i, _i, _i, _p],
     "ecalj_wc_build": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
     "ecalj_wc_build_dev": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "ecalj_wc_build_real": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "ecalj_wc_build_real_dev": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "ecalj_dpsion_real": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
+    "ecalj_dpsion_real_dev": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
+    "ecalj_zminv": [_p, _p, _i, _i],
+    "ecalj_zminv_dev": [_p, _p, _i, _i, _i, C.c_int64],
     "ecalj_sigc_weights": [_p, _i, _i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p, _p, _p],
     "ecalj_sigc_weights_dev": [_p, _i, _i, _i, _i, _i, _i, _p, _p, _d, _d, _p, _p, _p, _p, _p],
     "ecalj_sigc": [_p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
@@ -414,6 +420,23 @@ class Handle:
         _chk(lib().ecalj_dpsion_imag_dev(self._h, _t_ptr(rcxq), npr, npr_col, nwhis, len(fi), _np_ptr(fr),
                                          _np_ptr(fi), ecut, int(scale_rcxq), _t_ptr(zxqi)), "dpsion_imag_dev")
 
+    def dpsion_real(self, rcxq, frhis, freqr, ecut=1e10, rcxq_scaled=False):
+        """dpsion_chiq, real axis (S/dpsion5.f90:244-256); returns zxq(:,:,0:nw), rcxq unchanged."""
+        r = _f(rcxq, np.complex128)
+        npr, npr_col, ns = r.shape
+        fr = np.ascontiguousarray(frhis, np.float64)
+        fq = np.ascontiguousarray(freqr, np.float64)
+        zx = np.zeros((npr, npr_col, len(fq)), np.complex128, order="F")
+        _chk(lib().ecalj_dpsion_real(self._h, _np_ptr(r), npr, npr_col, ns - 1, len(fq) - 1, _np_ptr(fr), _np_ptr(fq),
+                                     ecut, int(rcxq_scaled), _np_ptr(zx)), "dpsion_real")
+        return zx
+
+    def dpsion_real_dev(self, rcxq, npr, npr_col, nwhis, frhis, freqr, zxq, ecut=1e10, rcxq_scaled=False):
+        fr = np.ascontiguousarray(frhis, np.float64)
+        fq = np.ascontiguousarray(freqr, np.float64)
+        _chk(lib().ecalj_dpsion_real_dev(self._h, _t_ptr(rcxq), npr, npr_col, nwhis, len(fq) - 1, _np_ptr(fr),
+                                         _np_ptr(fq), ecut, int(rcxq_scaled), _t_ptr(zxq)), "dpsion_real_dev")
+
     def chi0_freq(self, z, w):
         z, w = _f(z, np.complex128), _f(w, np.float64)
         ni, npairs = z.shape
@@ -435,6 +458,32 @@ class Handle:
     def wc_build_dev(self, chi, wc, ngb, nw, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True, ldw=None):
         _chk(lib().ecalj_wc_build_dev(self._h, _t_ptr(chi), ngb, nw, nspin, int(is_x0_m_basis), int(is_wc_m_basis),
                                       _t_ptr(wc), ngb if ldw is None else ldw), "wc_build_dev")
+
+    def wc_build_real(self, chi, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True):
+        """WVRllwR body for q /= 0 (S/m_llw.f90:88-180): pivoted inversion."""
+        c = _f(chi, np.complex128)
+        n, _, nw = c.shape
+        wc = np.zeros((n, n, nw), np.complex128, order="F")
+        _chk(lib().ecalj_wc_build_real(self._h, _np_ptr(c), n, nw, nspin, int(is_x0_m_basis), int(is_wc_m_basis),
+                                       _np_ptr(wc), n), "wc_build_real")
+        return wc
+
+    def wc_build_real_dev(self, chi, wc, ngb, nw, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True, ldw=None):
+        _chk(lib().ecalj_wc_build_real_dev(self._h, _t_ptr(chi), ngb, nw, nspin, int(is_x0_m_basis),
+                                           int(is_wc_m_basis), _t_ptr(wc), ngb if ldw is None else ldw),
+             "wc_build_real_dev")
+
+    def zminv(self, a):
+        """zminv (S/m_lapack.f90:21-47): returns inv(a), partial pivoting."""
+        x = np.array(a, np.complex128, order="F", copy=True)
+        n = x.shape[0]
+        _chk(lib().ecalj_zminv(self._h, _np_ptr(x), n, n), "zminv")
+        return x
+
+    def zminv_dev(self, a, n, nbatch=1, lda=None, stride=None):
+        lda = n if lda is None else lda
+        stride = lda * n if stride is None else stride
+        _chk(lib().ecalj_zminv_dev(self._h, _t_ptr(a), n, lda, nbatch, stride), "zminv_dev")
 
     # ---- Sigma_c
     def sigc_weights(self, omega, ekc, esmr, wkkr, freqx, wwx, expa, ua=1.0, ns1=1, nctot=0):

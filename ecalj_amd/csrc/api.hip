@@ -112,11 +112,14 @@ static void hilbertmat_host(std::complex<double> zz, int nwhis, const double* L,
   }
 }
 
-// imatt(iv, iw) * (-wfac(iw)/(R-L)) for npm = 1 (S/dpsion5.f90:84-89,121-129,215-220).
-static void hilbert_coef(int nwhis, int niw, const double* frhis, const double* freqi, double ecut,
-                         std::vector<double>& coef, std::vector<double>& scale) {
+// his_L/his_C/his_R (S/dpsion5.f90:84-86), index iw + nwhis, and the bin
+// scale -wfac/(R-L) of dpsion_chiq (S/dpsion5.f90:193-200).
+static void his_edges(int nwhis, const double* frhis, double ecut, std::vector<double>& L, std::vector<double>& C,
+                      std::vector<double>& R, std::vector<double>& scale) {
   const int n2 = 2 * nwhis + 1;
-  std::vector<double> L(n2), C(n2), R(n2);
+  L.assign(n2, 0.0);
+  C.assign(n2, 0.0);
+  R.assign(n2, 0.0);
   for (int iw = -nwhis; iw <= nwhis; ++iw) {
     const int j = iw + nwhis;
     if (iw < 0) {
@@ -137,6 +140,14 @@ static void hilbert_coef(int nwhis, int niw, const double* frhis, const double* 
     const double wfac = (ecut < 1e9) ? std::exp(-(hc / ecut) * (hc / ecut)) : 1.0;
     scale[iw - 1] = -wfac / (hr - hl);
   }
+}
+
+// imatt(iv, iw) * (-wfac(iw)/(R-L)) for npm = 1 (S/dpsion5.f90:84-89,121-129,215-220).
+static void hilbert_coef(int nwhis, int niw, const double* frhis, const double* freqi, double ecut,
+                         std::vector<double>& coef, std::vector<double>& scale) {
+  const int n2 = 2 * nwhis + 1;
+  std::vector<double> L, C, R;
+  his_edges(nwhis, frhis, ecut, L, C, R, scale);
   coef.assign((size_t)niw * nwhis, 0.0);
   std::vector<std::complex<double>> rr(n2);
   const double pi = 3.14159265358979323846;
@@ -145,6 +156,30 @@ static void hilbert_coef(int nwhis, int niw, const double* frhis, const double* 
     for (int iw = 1; iw <= nwhis; ++iw)
       coef[it + (size_t)niw * (iw - 1)] =
           std::real(rr[nwhis + iw] - rr[nwhis - iw]) / pi * scale[iw - 1];
+  }
+}
+
+// Real axis, npm = 1 without chipm (S/dpsion5.f90:91-103,244-256):
+// coef(it, iw) = rmatt(it, iw) * bscale(iw), rmatt = Re(rmat(iw) - rmat(-iw))/pi
+// of hilbertmat at zz = freq_r(it); dscale(it) multiplies the i*rcxq(:,:,it)
+// term: 1 for slot 0 (never scaled), bscale(it) for it >= 1.  bscale is the
+// bin scale -wfac/(R-L), or 1 when rcxq already holds the scaled values.
+static void hilbert_coef_real(int nwhis, int nw, const double* frhis, const double* freqr, double ecut, bool scaled,
+                              std::vector<double>& coef, std::vector<double>& dscale) {
+  std::vector<double> L, C, R, scale;
+  his_edges(nwhis, frhis, ecut, L, C, R, scale);
+  if (scaled) scale.assign(nwhis, 1.0);
+  const int n2 = 2 * nwhis + 1, nv = nw + 1;
+  coef.assign((size_t)nv * nwhis, 0.0);
+  dscale.assign(nwhis + 1, 0.0);
+  dscale[0] = 1.0;
+  for (int iw = 1; iw <= nwhis; ++iw) dscale[iw] = scale[iw - 1];
+  const double pi = 3.14159265358979323846;
+  std::vector<std::complex<double>> rr(n2);
+  for (int it = 0; it < nv; ++it) {
+    hilbertmat_host(std::complex<double>(freqr[it], 0.0), nwhis, L.data(), C.data(), R.data(), rr.data());
+    for (int iw = 1; iw <= nwhis; ++iw)
+      coef[it + (size_t)nv * (iw - 1)] = std::real(rr[nwhis + iw] - rr[nwhis - iw]) / pi * scale[iw - 1];
   }
 }
 
@@ -843,7 +878,8 @@ ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_
   const int64_t n = (int64_t)npr * npr_col;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 4, &tev));
-  ECALJ_TRY(hilbert(h, wide(h), rcxq, n, nwhis, niw, d, scale_rcxq ? d + coef.size() : nullptr, (cd*)zxqi, n));
+  ECALJ_TRY(hilbert(h, wide(h), rcxq, n, nwhis, niw, d, scale_rcxq ? d + coef.size() : nullptr, nullptr,
+                    (cd*)zxqi, n));
   ECALJ_TRY(timing_end(h, 4, tev));
   return ECALJ_OK;
 }
@@ -860,6 +896,43 @@ ecalj_status ecalj_dpsion_imag(ecalj_handle h, double* rcxq, int npr, int npr_co
   ECALJ_TRY(ecalj_dpsion_imag_dev(h, dr.buf.p, npr, npr_col, nwhis, niw, frhis, freqi, ecut, scale_rcxq, dx.p));
   if (scale_rcxq) ECALJ_TRY(dr.get(h, rcxq, nr));
   ECALJ_HIP(hipMemcpyAsync(zxqi, dx.p, bx, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_dpsion_real_dev(ecalj_handle h, const void* rcxq, int npr, int npr_col, int nwhis, int nw,
+                                   const double* frhis, const double* freqr, double ecut, int rcxq_scaled,
+                                   void* zxq) {
+  if (!h || !rcxq || !frhis || !freqr || !zxq) return ECALJ_ERR_ARG;
+  if (npr < 0 || npr_col < 0 || nwhis < 1 || nw < 0) return ECALJ_ERR_DIM;
+  if (nwhis <= nw) return ECALJ_ERR_DIM;  // S/dpsion5.f90:203
+  if (freqr[0] != 0.0) return ECALJ_ERR_ARG;  // S/dpsion5.f90:204
+  std::vector<double> coef, dscale;
+  hilbert_coef_real(nwhis, nw, frhis, freqr, ecut, rcxq_scaled != 0, coef, dscale);
+  std::vector<double> blob(coef);
+  blob.insert(blob.end(), dscale.begin(), dscale.end());
+  double* d;
+  ECALJ_TRY(upload(h, ecalj_ctx::S_COEF, blob, &d));
+  const int64_t n = (int64_t)npr * npr_col;
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 4, &tev));
+  ECALJ_TRY(hilbert(h, wide(h), const_cast<void*>(rcxq), n, nwhis, nw + 1, d, nullptr, d + coef.size(), (cd*)zxq,
+                    n));
+  ECALJ_TRY(timing_end(h, 4, tev));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_dpsion_real(ecalj_handle h, const double* rcxq, int npr, int npr_col, int nwhis, int nw,
+                               const double* frhis, const double* freqr, double ecut, int rcxq_scaled, double* zxq) {
+  if (!h || !rcxq || !zxq) return ECALJ_ERR_ARG;
+  const int64_t nr = (int64_t)npr * npr_col * (nwhis + 1);
+  const size_t bx = sizeof(cd) * (size_t)npr * npr_col * (nw + 1);
+  DevC dr;
+  DevBuf dx;
+  ECALJ_TRY(dr.put(h, rcxq, nr));
+  ECALJ_TRY(dx.alloc(bx));
+  ECALJ_TRY(ecalj_dpsion_real_dev(h, dr.buf.p, npr, npr_col, nwhis, nw, frhis, freqr, ecut, rcxq_scaled, dx.p));
+  ECALJ_HIP(hipMemcpyAsync(zxq, dx.p, bx, hipMemcpyDeviceToHost, h->stream));
   ECALJ_HIP(hipStreamSynchronize(h->stream));
   return ECALJ_OK;
 }
@@ -892,8 +965,10 @@ ecalj_status ecalj_chi0_freq(ecalj_handle h, const double* z, const double* w, i
 }
 
 // ------------------------------------------------------------------ W build (FP64 in every form)
-ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
-                                int is_wc_m_basis, void* wc, int ldw) {
+// WVIllwI (imaginary axis, epsilon Hermitian positive definite: in-order
+// elimination) and WVRllwR (real axis, epsilon general: row exchanges).
+static ecalj_status wc_build_impl(ecalj_handle h, const void* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
+                                  int is_wc_m_basis, void* wc, int ldw, bool pivot) {
   if (!h || !chi || !wc) return ECALJ_ERR_ARG;
   if (ngb < 1 || nw < 1 || ldw < ngb || (nspin != 1 && nspin != 2)) return ECALJ_ERR_DIM;
   if (!h->d_vcousq || h->ngb != ngb) return ECALJ_ERR_STATE;
@@ -926,9 +1001,10 @@ ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw
   } else {
     ECALJ_TRY(eps_direct(h, (const cd*)chi, h->d_vcousq, fac, n, nw, eps));
   }
-  ECALJ_TRY(gj_inverse(h, eps, n, n, nn, nw));  // zminv (S/m_llw.f90:350)
+  const int* perm = nullptr;
+  ECALJ_TRY(gj_inverse(h, eps, n, n, nn, nw, pivot ? &perm : nullptr));  // zminv (S/m_llw.f90:143,350)
   if (is_wc_m_basis) {  // W_M = zcousq W_E zcousq^H (S/m_llw.f90:366-367)
-    ECALJ_TRY(wc_form(h, eps, h->d_vcousq, n, nw, tmp, n, nn));
+    ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, tmp, n, nn));
     GemmCall g3;  // eps <- W_E zcousq^H
     g3.tb = true; g3.cb = true;
     g3.M = n; g3.N = n; g3.K = n;
@@ -945,10 +1021,57 @@ ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw
     g4.batch = nw; g4.div = nw;
     ECALJ_TRY(gemm(h, g4));
   } else {
-    ECALJ_TRY(wc_form(h, eps, h->d_vcousq, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
+    ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
   }
   ECALJ_TRY(timing_end(h, 3, tev));
   return ECALJ_OK;
+}
+
+ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
+                                int is_wc_m_basis, void* wc, int ldw) {
+  return wc_build_impl(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw, false);
+}
+
+ecalj_status ecalj_wc_build_real_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
+                                     int is_wc_m_basis, void* wc, int ldw) {
+  return wc_build_impl(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw, true);
+}
+
+ecalj_status ecalj_wc_build_real(ecalj_handle h, const double* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
+                                 int is_wc_m_basis, double* wc, int ldw) {
+  if (!h || !chi || !wc) return ECALJ_ERR_ARG;
+  DevBuf dc, dw;
+  ECALJ_TRY(dc.put(h, chi, sizeof(cd) * (size_t)ngb * ngb * nw));
+  const size_t bw = sizeof(cd) * (size_t)ldw * ngb * nw;
+  ECALJ_TRY(dw.put(h, wc, bw));
+  ECALJ_TRY(ecalj_wc_build_real_dev(h, dc.p, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, dw.p, ldw));
+  ECALJ_HIP(hipMemcpyAsync(wc, dw.p, bw, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return check_flag(h);
+}
+
+// zminv_d (S/m_lapack.f90:108-176), batched: in-place inverse with partial pivoting.
+ecalj_status ecalj_zminv_dev(ecalj_handle h, void* a, int n, int lda, int nbatch, int64_t stride) {
+  if (!h || !a) return ECALJ_ERR_ARG;
+  if (n < 0 || lda < std::max(n, 1) || nbatch < 0 || (nbatch > 1 && stride < (int64_t)lda * n))
+    return ECALJ_ERR_DIM;
+  if (n == 0 || nbatch == 0) return ECALJ_OK;
+  const int* perm = nullptr;
+  ECALJ_TRY(gj_inverse(h, (cd*)a, n, lda, stride, nbatch, &perm));
+  ECALJ_TRY(gj_unscramble(h, (cd*)a, n, lda, stride, nbatch, perm));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_zminv(ecalj_handle h, double* a, int n, int lda) {
+  if (!h || !a) return ECALJ_ERR_ARG;
+  if (n < 0 || lda < std::max(n, 1)) return ECALJ_ERR_DIM;
+  const size_t bytes = sizeof(cd) * (size_t)lda * n;
+  DevBuf d;
+  ECALJ_TRY(d.put(h, a, bytes));
+  ECALJ_TRY(ecalj_zminv_dev(h, d.p, n, lda, 1, (int64_t)lda * n));
+  ECALJ_HIP(hipMemcpyAsync(a, d.p, bytes, hipMemcpyDeviceToHost, h->stream));
+  ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return check_flag(h);
 }
 
 ecalj_status ecalj_wc_build(ecalj_handle h, const double* chi, int ngb, int nw, int nspin, int is_x0_m_basis,

@@ -199,13 +199,20 @@ ecalj_status x0_herk(ecalj_ctx* h, int mode, const void* z, const int64_t* d_col
 }
 
 // ==========================================================================
-// Hilbert transform to the imaginary axis: zxqi(e,v) = sum_iw coef(v,iw) rcxq(e,iw),
-// FP64 accumulation, complex128 output; rcxq complex128 or complex64.
+// Hilbert transforms of the binned Im chi (dpsion_chiq):
+//   out(e, v) = sum_iw coef(v, iw) rcxq(e, iw)                 (imaginary axis)
+//   out(e, v) = i dscale(v0+v) rcxq(e, v0+v) + sum_iw coef(v, iw) rcxq(e, iw)
+//                                                              (real axis, dscale != null)
+// for the output chunk v in [v0, v0+nv); coef(v, iw) = coef[v + ldc*(iw-1)]
+// (chunk base already applied).  FP64 accumulation, complex128 output;
+// rcxq complex128 or complex64.  With `scale`, rcxq(:, iw) *= scale(iw) is
+// stored back (the reference scales rcxq in place once).
 template <int NV, class T>
 __global__ __launch_bounds__(256) void hilbert_kernel(T* __restrict__ rcxq, int64_t slice, int nwhis,
-                                                      int nv, const double* __restrict__ coef,
-                                                      const double* __restrict__ scale,
-                                                      cd* __restrict__ zxqi, int64_t n) {
+                                                      int nv, int ldc, const double* __restrict__ coef,
+                                                      const double* __restrict__ scale, int v0,
+                                                      const double* __restrict__ dscale,
+                                                      cd* __restrict__ out, int64_t n) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= n) return;
   double ar[NV], ai[NV];
@@ -218,7 +225,7 @@ __global__ __launch_bounds__(256) void hilbert_kernel(T* __restrict__ rcxq, int6
       const double s = scale[iw - 1];
       st_c(rcxq + e + slice * iw, s * xr, s * xi);
     }
-    const double* c = coef + (int64_t)nv * (iw - 1);
+    const double* c = coef + (int64_t)ldc * (iw - 1);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       if (v < nv) {
@@ -226,35 +233,61 @@ __global__ __launch_bounds__(256) void hilbert_kernel(T* __restrict__ rcxq, int6
         ai[v] += c[v] * xi;
       }
     }
+    const int d = iw - v0;  // wave-uniform: only nv of the nwhis bins take this branch
+    if (dscale && d >= 0 && d < nv) {
+      const double s = dscale[iw];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        if (v == d) {
+          ar[v] -= s * xi;
+          ai[v] += s * xr;
+        }
+    }
+  }
+  if (dscale && v0 == 0) {  // slot 0: the zero-frequency term, never accumulated by x0gemm
+    double xr, xi;
+    ld_c2(rcxq + e, xr, xi);
+    ar[0] -= dscale[0] * xi;
+    ai[0] += dscale[0] * xr;
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v)
-    if (v < nv) zxqi[e + n * v] = cd{ar[v], ai[v]};
+    if (v < nv) out[e + n * v] = cd{ar[v], ai[v]};
+}
+
+template <int NV, class T>
+static void hilbert_launch(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, int nv, int ldc, const double* coef,
+                           const double* scale, int v0, const double* dscale, cd* out, int64_t n) {
+  const dim3 grid((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL((hilbert_kernel<NV, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, ldc, coef,
+                     scale, v0, dscale, out, n);
 }
 
 template <class T>
 static ecalj_status hilbert_t(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
-                              const double* d_scale, cd* zxqi, int64_t n) {
-  const dim3 grid((unsigned)((n + 255) / 256));
-  if (nv <= 8)
-    hipLaunchKernelGGL((hilbert_kernel<8, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
-  else if (nv <= 16)
-    hipLaunchKernelGGL((hilbert_kernel<16, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
-  else if (nv <= 32)
-    hipLaunchKernelGGL((hilbert_kernel<32, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
-  else if (nv <= 64)
-    hipLaunchKernelGGL((hilbert_kernel<64, T>), grid, dim3(256), 0, h->stream, rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
-  else
-    return ECALJ_ERR_DIM;
-  ECALJ_HIP(hipGetLastError());
+                              const double* d_scale, const double* d_dscale, cd* out, int64_t n) {
+  // chunks of at most 32 outputs (64 FP64 accumulators per lane); each chunk
+  // re-reads rcxq.  The chunks run last to first, so that the one that stores
+  // the scaled rcxq (chunk 0) runs after every other chunk has read it.
+  for (int v0 = ((nv - 1) / 32) * 32; v0 >= 0; v0 -= 32) {
+    const int c = std::min(32, nv - v0);
+    const double* sc = (v0 == 0) ? d_scale : nullptr;
+    if (c <= 8)
+      hilbert_launch<8, T>(h, rcxq, slice, nwhis, c, nv, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
+    else if (c <= 16)
+      hilbert_launch<16, T>(h, rcxq, slice, nwhis, c, nv, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
+    else
+      hilbert_launch<32, T>(h, rcxq, slice, nwhis, c, nv, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
+    ECALJ_HIP(hipGetLastError());
+  }
   return ECALJ_OK;
 }
 
 ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
-                     const double* d_scale, cd* zxqi, int64_t n) {
-  if (n == 0) return ECALJ_OK;
-  if (wide) return hilbert_t<cd>(h, (cd*)rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
-  return hilbert_t<cf>(h, (cf*)rcxq, slice, nwhis, nv, d_coef, d_scale, zxqi, n);
+                     const double* d_scale, const double* d_dscale, cd* out, int64_t n) {
+  if (n == 0 || nv == 0) return ECALJ_OK;
+  if (wide) return hilbert_t<cd>(h, (cd*)rcxq, slice, nwhis, nv, d_coef, d_scale, d_dscale, out, n);
+  return hilbert_t<cf>(h, (cf*)rcxq, slice, nwhis, nv, d_coef, d_scale, d_dscale, out, n);
 }
 
 // ==========================================================================
@@ -327,9 +360,12 @@ __global__ void eps_direct_kernel(const cd* __restrict__ chi, const double* __re
   }
 }
 
-// w = s_i inv_ij s_j - delta_ij s_i s_j, written with leading dimension ldw
-__global__ void wc_form_kernel(const cd* __restrict__ inv, const double* __restrict__ s, int n, int nw,
-                               cd* __restrict__ w, int64_t ldw, int64_t wstride) {
+// w = s_i inv_ij s_j - delta_ij s_i s_j, written with leading dimension ldw.
+// perm (optional): column j of inv is stored as column perm[j + n*iw]
+// (the pivoted inverse leaves its columns permuted).
+__global__ void wc_form_kernel(const cd* __restrict__ inv, const int* __restrict__ perm,
+                               const double* __restrict__ s, int n, int nw, cd* __restrict__ w, int64_t ldw,
+                               int64_t wstride) {
   const int64_t tot = (int64_t)n * n * nw;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -338,7 +374,8 @@ __global__ void wc_form_kernel(const cd* __restrict__ inv, const double* __restr
     const int64_t ij = e % nn;
     const int i = (int)(ij % n), j = (int)(ij / n);
     const double sc = s[i] * s[j];
-    cd x = inv[e];
+    const int jc = perm ? perm[j + (int64_t)n * iw] : j;
+    cd x = inv[nn * iw + i + (int64_t)n * jc];
     w[(int64_t)iw * wstride + i + ldw * j] = cd{sc * x.re - (i == j ? sc : 0.0), sc * x.im};
   }
 }
@@ -352,12 +389,12 @@ ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac
   return ECALJ_OK;
 }
 
-ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const double* s, int n, int nw, cd* w, int64_t ldw,
-                     int64_t wstride) {
+ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const int* perm, const double* s, int n, int nw, cd* w,
+                     int64_t ldw, int64_t wstride) {
   const int64_t tot = (int64_t)n * n * nw;
   if (tot == 0) return ECALJ_OK;
   const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
-  hipLaunchKernelGGL(wc_form_kernel, dim3(blocks), dim3(256), 0, h->stream, inv, s, n, nw, w, ldw, wstride);
+  hipLaunchKernelGGL(wc_form_kernel, dim3(blocks), dim3(256), 0, h->stream, inv, perm, s, n, nw, w, ldw, wstride);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
@@ -436,13 +473,190 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n,
   }
 }
 
-ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat) {
+// Row exchanges for pivot block K = [k0, k0+b) (partial pivoting, as
+// zgetrf): LU with exchanges of the panel rows k0..n-1 of columns K, on a
+// copy in `work` (leading dimension n).  Only the exchanges are kept:
+// ipiv[k0+kk] = row exchanged with row k0+kk (absolute, 0-based).  The pivot
+// is the first row of largest |re|+|im| (izamax).  The in-order elimination
+// of the exchanged block then meets exactly these pivots.
+__global__ __launch_bounds__(256) void gj_panel_kernel(const cd* __restrict__ a, int n, int64_t lda,
+                                                       int64_t astride, int k0, int b, cd* __restrict__ work,
+                                                       int* __restrict__ ipiv, int* __restrict__ flag) {
+  __shared__ double red_v[4];
+  __shared__ int red_i[4];
+  __shared__ cd prow[GJB];
+  __shared__ int s_p;
+  const int mat = blockIdx.x;
+  const cd* A = a + astride * mat;
+  cd* W = work + (int64_t)n * GJB * mat;
+  int* P = ipiv + (int64_t)n * mat;
+  const int m = n - k0;
+  const int tid = threadIdx.x;
+  for (int64_t e = tid; e < (int64_t)m * b; e += blockDim.x) {
+    const int i = (int)(e % m), l = (int)(e / m);
+    W[i + (int64_t)n * l] = A[(k0 + i) + lda * (k0 + l)];
+  }
+  __syncthreads();
+  for (int kk = 0; kk < b; ++kk) {
+    double bv = -1.0;
+    int bi = m;
+    for (int i = kk + tid; i < m; i += blockDim.x) {
+      const cd x = W[i + (int64_t)n * kk];
+      const double v = fabs(x.re) + fabs(x.im);
+      if (v > bv) { bv = v; bi = i; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { red_v[tid >> 6] = bv; red_i[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double v = red_v[0];
+      int i = red_i[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+        if (red_v[w] > v || (red_v[w] == v && red_i[w] < i)) { v = red_v[w]; i = red_i[w]; }
+      if (!(v > 1e-300) || i >= m) {  // zero (or NaN) column: singular
+        *flag = 1;
+        i = kk;
+      }
+      s_p = i;
+      P[k0 + kk] = k0 + i;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (tid < b) {
+      const cd x = W[kk + (int64_t)n * tid], y = W[p + (int64_t)n * tid];
+      if (p != kk) {
+        W[kk + (int64_t)n * tid] = y;
+        W[p + (int64_t)n * tid] = x;
+      }
+      prow[tid] = (p != kk) ? y : x;
+    }
+    __syncthreads();
+    const cd piv = prow[kk];
+    const double mag = piv.re * piv.re + piv.im * piv.im;
+    if (mag > 0.0) {
+      const double dr = piv.re / mag, di = -piv.im / mag;
+      for (int i = kk + 1 + tid; i < m; i += blockDim.x) {
+        const cd x = W[i + (int64_t)n * kk];
+        const cd l = cd{x.re * dr - x.im * di, x.re * di + x.im * dr};
+        W[i + (int64_t)n * kk] = l;
+        for (int j = kk + 1; j < b; ++j) {
+          const cd u = prow[j];
+          cd& y = W[i + (int64_t)n * j];
+          y = cd{y.re - (l.re * u.re - l.im * u.im), y.im - (l.re * u.im + l.im * u.re)};
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Apply the exchanges of block K to whole rows (all n columns).
+__global__ __launch_bounds__(256) void gj_swap_kernel(cd* __restrict__ a, int n, int64_t lda, int64_t astride,
+                                                      int k0, int b, const int* __restrict__ ipiv) {
+  const int mat = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  cd* col = a + astride * mat + lda * c;
+  const int* P = ipiv + (int64_t)n * mat;
+  for (int kk = 0; kk < b; ++kk) {
+    const int r = k0 + kk, p = P[r];
+    if (p != r) {
+      const cd x = col[r];
+      col[r] = col[p];
+      col[p] = x;
+    }
+  }
+}
+
+// Column order of the in-place result: undoing the row exchanges as column
+// exchanges in reverse order (j = n-1..0: swap columns j and ipiv[j]) is the
+// gather inv(:, j) = a(:, perm[j]) with perm from the same swaps on indices.
+__global__ __launch_bounds__(256) void gj_perm_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ perm) {
+  extern __shared__ int sh[];
+  int* idx = sh;
+  int* pv = sh + n;
+  const int mat = blockIdx.x;
+  const int* P = ipiv + (int64_t)n * mat;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    idx[j] = j;
+    pv[j] = P[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int j = n - 1; j >= 0; --j) {
+      const int p = pv[j];
+      if (p != j) {
+        const int t = idx[j];
+        idx[j] = idx[p];
+        idx[p] = t;
+      }
+    }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) perm[(int64_t)n * mat + j] = idx[j];
+}
+
+// out(:, j) = a(:, perm[j]) per matrix; out has leading dimension n.
+__global__ void gj_gather_kernel(const cd* __restrict__ a, int n, int64_t lda, int64_t astride,
+                                 const int* __restrict__ perm, int nmat, cd* __restrict__ out) {
+  const int64_t nn = (int64_t)n * n, tot = nn * nmat;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int mat = (int)(e / nn);
+    const int64_t ij = e % nn;
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    out[e] = a[astride * mat + i + lda * perm[(int64_t)n * mat + j]];
+  }
+}
+
+__global__ void gj_copy_back_kernel(const cd* __restrict__ src, int n, int nmat, cd* __restrict__ a, int64_t lda,
+                                    int64_t astride) {
+  const int64_t nn = (int64_t)n * n, tot = nn * nmat;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int mat = (int)(e / nn);
+    const int64_t ij = e % nn;
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    a[astride * mat + i + lda * j] = src[e];
+  }
+}
+
+ecalj_status gj_unscramble(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, const int* perm) {
+  const int64_t tot = (int64_t)n * n * nmat;
+  if (tot == 0) return ECALJ_OK;
+  cd* tmp;
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP2, sizeof(cd) * tot, (void**)&tmp));
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(gj_gather_kernel, dim3(blocks), dim3(256), 0, h->stream, a, n, lda, astride, perm, nmat, tmp);
+  ECALJ_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gj_copy_back_kernel, dim3(blocks), dim3(256), 0, h->stream, tmp, n, nmat, a, lda, astride);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, const int** perm) {
   if (n == 0 || nmat == 0) return ECALJ_OK;
+  if (nmat > 65535) return ECALJ_ERR_DIM;
+  const bool pivot = perm != nullptr;
+  if (pivot && n > 8192) return ECALJ_ERR_DIM;  // index tables of gj_perm_kernel live in LDS
   cd *pinv, *cp;
+  int* ipiv = nullptr;
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJP, sizeof(cd) * GJB * GJB * nmat, (void**)&pinv));
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJC, sizeof(cd) * (size_t)n * GJB * nmat, (void**)&cp));
+  if (pivot) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJPIV, sizeof(int) * 2 * (size_t)n * nmat, (void**)&ipiv));
   for (int k0 = 0; k0 < n; k0 += GJB) {
     const int b = std::min(GJB, n - k0);
+    if (pivot) {  // choose and apply the row exchanges of this block (cp is free until gj_diag)
+      hipLaunchKernelGGL(gj_panel_kernel, dim3(nmat), dim3(256), 0, h->stream, a, n, lda, astride, k0, b, cp, ipiv,
+                         h->d_flag);
+      ECALJ_HIP(hipGetLastError());
+      hipLaunchKernelGGL(gj_swap_kernel, dim3((n + 255) / 256, nmat), dim3(256), 0, h->stream, a, n, lda, astride,
+                         k0, b, ipiv);
+      ECALJ_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(gj_diag_kernel, dim3(nmat), dim3(256), 0, h->stream, a, n, lda, astride, k0, b, pinv, cp,
                        h->d_flag);
     ECALJ_HIP(hipGetLastError());
@@ -463,6 +677,12 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
     t.batch = nmat; t.div = nmat;
     t.epi = EPI_GJ; t.gj_k0 = k0; t.gj_k1 = k0 + b;
     ECALJ_TRY(gemm(h, t));
+  }
+  if (pivot) {
+    int* pm = ipiv + (size_t)n * nmat;
+    hipLaunchKernelGGL(gj_perm_kernel, dim3(nmat), dim3(256), sizeof(int) * 2 * (size_t)n, h->stream, ipiv, n, pm);
+    ECALJ_HIP(hipGetLastError());
+    *perm = pm;
   }
   return ECALJ_OK;
 }

@@ -8,14 +8,22 @@ ecalj_status ppb_relayout(ecalj_ctx* h, const double* src, double* dst, int nv, 
 ecalj_status x0_herk(ecalj_ctx* h, int mode, const void* z, const int64_t* d_col, const double* d_w,
                      const int64_t* d_bstart, const void* d_items, int nitems, void* rcxq,
                      int64_t slice_elems, int npr, int npr_col, int ipr_col, int mirror);
+// d_dscale (optional, real axis): adds i*dscale(it)*rcxq(:, it) to output it
 ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
-                     const double* d_scale, cd* zxqi, int64_t n);
+                     const double* d_scale, const double* d_dscale, cd* out, int64_t n);
 ecalj_status chi0_freq(ecalj_ctx* h, int mode, const void* z, const double* w, int ni, int npairs, int nf, cd* chi);
 ecalj_status convert(ecalj_ctx* h, const void* src, bool src64, void* dst, bool dst64, int64_t n);
 ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac, int n, int nw, cd* eps);
-ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const double* s, int n, int nw, cd* w, int64_t ldw,
-                     int64_t wstride);
-ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat);
+ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const int* perm, const double* s, int n, int nw, cd* w,
+                     int64_t ldw, int64_t wstride);
+// In-place inverse of nmat matrices.  perm == nullptr: no row exchanges (the
+// matrix must be safe to eliminate in order, e.g. Hermitian positive
+// definite).  Otherwise partial pivoting: on return column j of inv(A) is
+// column (*perm)[j + n*mat] of a (*perm: device, n ints per matrix).
+ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat,
+                        const int** perm = nullptr);
+// a(:, j) <- a(:, perm[j]) for each matrix (column unscramble after gj_inverse)
+ecalj_status gj_unscramble(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, const int* perm);
 struct RealSpec;
 // rs (optional): the real-axis pole term, added into czmelwc before the final product
 ecalj_status sig_contract(ecalj_ctx* h, int mode, int nbatch, const void* zmel, int n, int nm, int ntq, int niw,

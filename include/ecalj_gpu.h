@@ -192,6 +192,26 @@ ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_
                                    int niw, const double* frhis, const double* freqi,
                                    double ecut, int scale_rcxq, void* zxqi);
 
+/* ---- chi0: Hilbert transform to the real axis --------------------------
+ * dpsion_chiq, real branch with npm=1 and no chipm (S/dpsion5.f90:193-200,
+ * 244-256) with dpsion_init's rmatt (S/dpsion5.f90:91-103) computed in FP64
+ * on the host from hilbertmat at zz = freq_r(it):
+ *   zxq(:,:,it) = i*s(it)*rcxq(:,:,it) + sum_iw rmatt(it,iw)*s(iw)*rcxq(:,:,iw),
+ *   it = 0..nw,  s(iw) = -wfac(iw)/(R-L) for iw >= 1, s(0) = 1
+ * (rcxq slice it is the bin whose centre is freq_r(it); slice 0 is the
+ * zero-frequency slot).  rcxq_scaled != 0: rcxq already holds the scaled
+ * values (e.g. after ecalj_dpsion_imag with scale_rcxq), s = 1.  rcxq is
+ * read only; zxq (npr x npr_col x (nw+1), complex128) is a separate array,
+ * where the reference overwrites rcxq(:,:,0:nw).  Requires nw < nwhis and
+ * freqr[0] == 0 (the reference's own checks, S/dpsion5.f90:203-204). */
+ecalj_status ecalj_dpsion_real(ecalj_handle h, const double* rcxq, int npr, int npr_col,
+                               int nwhis, int nw, const double* frhis, const double* freqr,
+                               double ecut, int rcxq_scaled, double* zxq);
+ecalj_status ecalj_dpsion_real_dev(ecalj_handle h, const void* rcxq, int npr, int npr_col,
+                                   int nwhis, int nw, const double* frhis,
+                                   const double* freqr, double ecut, int rcxq_scaled,
+                                   void* zxq);
+
 /* ---- chi0: composed direct-frequency form ------------------------------
  * chi(I,J,v) = sum_p w(v,p) conj(Z(I,p)) Z(J,p); equal to x0gemm followed by
  * dpsion_chiq when w(v,p) = sum_bins whw * (-wfac/(R-L)) * imatt(v,bin). */
@@ -213,6 +233,26 @@ ecalj_status ecalj_wc_build(ecalj_handle h, const double* chi, int ngb, int nw, 
                             int is_x0_m_basis, int is_wc_m_basis, double* wc, int ldw);
 ecalj_status ecalj_wc_build_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin,
                                 int is_x0_m_basis, int is_wc_m_basis, void* wc, int ldw);
+
+/* WVRllwR body for q /= 0 (S/m_llw.f90:88-180): the same steps on the real
+ * frequency mesh (nw = number of frequencies 0..nw_max, i.e. nw_max+1).
+ * chi(omega) is not Hermitian there, so epsilon is a general complex
+ * matrix: the inversion uses partial pivoting (row exchanges chosen as
+ * zgetrf does, first row of largest |re|+|im|); a zero pivot column returns
+ * ECALJ_ERR_SINGULAR. */
+ecalj_status ecalj_wc_build_real(ecalj_handle h, const double* chi, int ngb, int nw, int nspin,
+                                 int is_x0_m_basis, int is_wc_m_basis, double* wc, int ldw);
+ecalj_status ecalj_wc_build_real_dev(ecalj_handle h, const void* chi, int ngb, int nw,
+                                     int nspin, int is_x0_m_basis, int is_wc_m_basis,
+                                     void* wc, int ldw);
+
+/* zminv_h / zminv_d (S/m_lapack.f90:21-47,108-176): in-place inverse of a
+ * general complex128 matrix (lda x n), with partial pivoting; the _dev form
+ * inverts nbatch matrices `stride` elements apart in device memory.  The
+ * plugin seam the reference aliases at compile time (S/m_llw.f90:25-29). */
+ecalj_status ecalj_zminv(ecalj_handle h, double* a, int n, int lda);
+ecalj_status ecalj_zminv_dev(ecalj_handle h, void* a, int n, int lda, int nbatch,
+                             int64_t stride);
 
 /* ---- Sigma_c ------------------------------------------------------------
  * Imaginary-axis weights (S/m_sxcf_gemm.f90:432-458), FP64:

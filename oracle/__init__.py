@@ -164,6 +164,16 @@ def imatt(freqi, frhis_):
     return out
 
 
+def rmatt(freqr, frhis_):
+    """dpsion_init real branch (npm=1): (nw+1) x nwhis, freqr = freq_r(0:nw)."""
+    freqr = _cd(freqr)
+    frhis_ = _cd(frhis_)
+    nw, nwhis = len(freqr) - 1, len(frhis_) - 1
+    out = np.zeros((nw + 1, nwhis), order="F")
+    lib().orc_rmatt(nw, _p(freqr), nwhis, _p(frhis_), _p(out))
+    return out
+
+
 def zmel(nv, mdim, phase, ppb, cphim, cphiq, m2e=None, conjg=False):
     nv = _ci(nv)
     mdim = _ci(mdim)
@@ -219,6 +229,19 @@ def chi0_freq(z, w):
     chi = np.zeros((ni, ni, nf), np.complex128, order="F")
     lib().orc_chi0_freq(ni, npairs, nf, _p(z), _p(w), _p(chi))
     return chi
+
+
+def dpsion_real(rcxq, frhis_, rmatt_, ecut=1e10, scaled=False):
+    """Real-axis branch of dpsion_chiq (npm=1).  rcxq (npr x npr_col x nwhis+1,
+    Fortran order) is scaled in place unless `scaled`; returns zxq(:,:,0:nw)."""
+    assert rcxq.dtype == np.complex128 and rcxq.flags.f_contiguous
+    npr, npr_col, ns = rcxq.shape
+    nwhis = ns - 1
+    nw = rmatt_.shape[0] - 1
+    zxq = np.zeros((npr, npr_col, nw + 1), np.complex128, order="F")
+    lib().orc_dpsion_real(npr, npr_col, nwhis, nw, _p(_cd(frhis_)), _p(_cd(rmatt_)), C.c_double(ecut),
+                          int(bool(scaled)), _p(rcxq), _p(zxq))
+    return zxq
 
 
 def wc_build(chi, vcousq, m2e, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True):

@@ -229,6 +229,53 @@ void orc_imatt(int niw, const double* freqi, int nwhis, const double* frhis, dou
   free(L);
 }
 
+/* dpsion_init real branch, npm=1 without chipm (S/dpsion5.f90:91-103):
+ * rmatt(it,iw) = Re(rmat(iw) - rmat(-iw))/pi of hilbertmat at zz = freqr(it)
+ * (real; hilbertmat adds i*1e-13).  rmatt is (nw+1) x nwhis column-major. */
+void orc_rmatt(int nw, const double* freqr, int nwhis, const double* frhis, double* rmatt) {
+  int n2 = 2 * nwhis + 1;
+  double* L = (double*)malloc(sizeof(double) * n2 * 3);
+  double* C = L + n2;
+  double* R = C + n2;
+  his_edges(nwhis, frhis, L, C, R);
+#pragma omp parallel for
+  for (int it = 0; it <= nw; ++it) {
+    double complex* rrr = (double complex*)malloc(sizeof(double complex) * n2);
+    orc_hilbertmat(freqr[it], nwhis, L, C, R, rrr);
+    for (int iw = 1; iw <= nwhis; ++iw)
+      rmatt[it + (long)(nw + 1) * (iw - 1)] = creal(rrr[nwhis + iw] - rrr[nwhis - iw]) / PI;
+    free(rrr);
+  }
+  free(L);
+}
+
+/* dpsion_chiq real axis, npm=1 without chipm (S/dpsion5.f90:193-200,
+ * 244-256).  Unless `scaled`, first scales rcxq(:,:,1:nwhis) in place by
+ * -wfac/(R-L) as dpsion_chiq does once for both branches; then
+ * zxq(:,:,it) = i*rcxq(:,:,it) + sum_iw rmatt(it,iw) rcxq(:,:,iw), it = 0..nw
+ * (slice it of rcxq is bin it: freq_r(it) is that bin's centre, and slice 0
+ * is the unaccumulated zero-frequency slot).                             */
+void orc_dpsion_real(int npr, int npr_col, int nwhis, int nw, const double* frhis,
+                     const double* rmatt, double ecut, int scaled, double complex* rcxq,
+                     double complex* zxq) {
+  long slice = (long)npr * npr_col;
+  if (!scaled)
+    for (int iw = 1; iw <= nwhis; ++iw) {
+      double hl = frhis[iw - 1], hr = frhis[iw], hc = 0.5 * (hl + hr);
+      double wfac = (ecut < 1e9) ? exp(-(hc / ecut) * (hc / ecut)) : 1.0;
+      double s = -wfac / (hr - hl);
+      double complex* r = rcxq + slice * iw;
+      for (long e = 0; e < slice; ++e) r[e] *= s;
+    }
+#pragma omp parallel for
+  for (long e = 0; e < slice; ++e)
+    for (int it = 0; it <= nw; ++it) {
+      double complex s = 0.0;
+      for (int iw = 1; iw <= nwhis; ++iw) s += rcxq[e + slice * iw] * rmatt[it + (long)(nw + 1) * (iw - 1)];
+      zxq[e + slice * it] = I * rcxq[e + slice * it] + s;
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* zmel (S/m_zmel.f90:146-597).  MT valence-valence block per atom
  * (:323-371):  ppbvphiq(:,:,i) = ppb(:,:,i)^T cphiq  (:361-362),

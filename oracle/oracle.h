@@ -21,6 +21,10 @@
  *   orc_zmel           numpy einsum on tiny inputs; unitary M->E invariance
  *   orc_x0gemm         brute-force numpy sum over pairs; Hermiticity
  *   orc_dpsion_imag    numpy; delta-pair limit -2D/(D^2+nu^2)
+ *   orc_rmatt          oracle/_ref hilbertmat at real zz; principal-value
+ *                      quadrature of the piecewise-linear interpolant
+ *   orc_dpsion_real    Kramers-Kronig pair of a two-pole response (closed
+ *                      form of Re chi from binned Im chi); slot-0 term
  *   orc_chi0_freq      equals x0gemm+dpsion with composed weights
  *   orc_wc_build       numpy.linalg.inv textbook W; Dyson residual
  *   orc_wgtim          scipy.integrate.quad of the smeared-Lorentzian integral
@@ -82,6 +86,15 @@ void orc_x0gemm(double complex* rcxq, int npr, int ipr_col, int npr_col, int nwh
                 int icoun_lo, int icoun_hi, const int* iwini, const int* iwend, const int* itc,
                 const int* itpc, const int* jpmc, const int* icouini, const double* whwc);
 
+/* dpsion_init, real branch with npm=1, no chipm (S/dpsion5.f90:91-103).
+ * rmatt is (nw+1) x nwhis column-major; freqr(0:nw). */
+void orc_rmatt(int nw, const double* freqr, int nwhis, const double* frhis, double* rmatt);
+/* dpsion_chiq, real axis with npm=1, no chipm (S/dpsion5.f90:193-200,244-256):
+ * scales rcxq in place unless `scaled`, then
+ * zxq(:,:,it) = i rcxq(:,:,it) + sum_iw rmatt(it,iw) rcxq(:,:,iw), it = 0..nw. */
+void orc_dpsion_real(int npr, int npr_col, int nwhis, int nw, const double* frhis,
+                     const double* rmatt, double ecut, int scaled, double complex* rcxq,
+                     double complex* zxq);
 /* dpsion_chiq, imaginary axis with npm=1 (S/dpsion5.f90:211-232): scales rcxq
  * bins 1..nwhis in place by -wfac/(R-L) and sets
  * zxqi(:,:,iv) = sum_iw imatt(iv,iw) rcxq(:,:,iw).  rcxq has nwhis+1 slices
