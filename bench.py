@@ -224,6 +224,46 @@ def bench_c5(log_=None):
     }
 
 
+def bench_real_axis(mixed, name="C2", log_=None):
+    """The q-point with the real-axis chain added: chi(omega) on freq_r from the
+    same rcxq, W_c(omega) (WVRllwR, pivoted inversion) and the Sigma_c pole
+    term in the same contraction.  Both forms, stage times with a sync between
+    stages, one warm-up step."""
+    from ecalj_amd import synth
+    from ecalj_amd.qsgw import QPointGPU
+
+    c = synth.CONFIGS[name]
+    qp = synth.make_qpoint(c)
+    res = {}
+    for prec in ("f64", mixed):
+        g = QPointGPU(qp, prec, real_axis=True)
+        g.step()
+        g.make_wk()
+        g.step()
+        g.h.sync()
+        st = {}
+        t0 = time.perf_counter()
+        for stage in ("chi0", "wbuild", "sigc"):
+            t1 = time.perf_counter()
+            getattr(g, stage)()
+            g.h.sync()
+            st[stage] = round(time.perf_counter() - t1, 5)
+        tot = time.perf_counter() - t0
+        res[prec] = (tot, st, g.results()[3], g.nwr)
+        if log_:
+            log_(f"bench[{name} real axis]: {prec} {tot:.4f} s {st}")
+        g.h.close()
+    e = float(np.abs(res[mixed][2] - res["f64"][2]).max() * 1e3)
+    return {
+        "workload": f"{name} q-point plus the real-axis chain: chi(omega) and W_c(omega) at nw+1={res['f64'][3] + 1} "
+                    f"real frequencies (freq_r from getwemax/getfreq), pole term of Sigma_c for all {c.nk} k",
+        "s_per_q": round(res[mixed][0], 5), "fp64_form_s_per_q": round(res["f64"][0], 5),
+        "speedup_vs_fp64_form": round(res["f64"][0] / res[mixed][0], 3),
+        "qp_err_meV_vs_fp64_form": round(e, 5),
+        "stage_s": {"f64": res["f64"][1], mixed: res[mixed][1]},
+    }
+
+
 def roofline(fl, info, precision, steps, config="C2"):
     ms, nl = info["ktimes"]["sigc"]
     launches_per_step = max(1, nl // steps)
@@ -256,7 +296,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5",
+    ap.add_argument("--also", default="C3,C4,C5,C2R",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -341,6 +381,9 @@ def main():
             continue
         if name == "C5":
             out["also"]["C5"] = bench_c5(log_=log)
+            continue
+        if name.endswith("R") and name[:-1] in synth.CONFIGS:
+            out["also"][name] = bench_real_axis(args.mixed, name[:-1], log_=log)
             continue
         c = synth.CONFIGS[name]
         log(f"bench: generating {c.name} inputs")

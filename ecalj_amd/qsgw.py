@@ -33,9 +33,16 @@ def gauss01(n):
 
 class QPointGPU:
     """tail_model: a learned_tail.TailModel; chi0 then sums the unoccupied bands
-    up to the model's nu_t and adds the predicted tail on the device."""
+    up to the model's nu_t and adds the predicted tail on the device.
 
-    def __init__(self, qp: synth.QPoint, precision: str = "f64", device: int = 0, tail_model=None):
+    real_axis: also build chi(omega) and W_c(omega) on the real mesh freq_r
+    (dpsion_chiq real branch, WVRllwR) from the same rcxq, and add the
+    real-axis pole term to Sigma_c (sxcf_scz_correlation, S/m_sxcf_gemm.f90:
+    501-594) with the imaginary-axis part in one contraction.  W_c(omega) of
+    q serves every internal k (the pole term reads a few frequencies each)."""
+
+    def __init__(self, qp: synth.QPoint, precision: str = "f64", device: int = 0, tail_model=None,
+                 real_axis: bool = False):
         self.qp = qp
         self.cfg = cfg = qp.cfg
         self.h = h = Handle(device, precision)
@@ -83,6 +90,16 @@ class QPointGPU:
         self.d_wx = h.to_dev(self.wx, f64)
         self.d_expa = h.to_dev(self.expa, f64)
         self.phase = h.to_dev(synth.wc_phases(cfg), c128)   # ni x nk
+        self.real_axis = real_axis
+        if real_axis:
+            if tail_model is not None:
+                raise ValueError("the learned tail models the imaginary axis only")
+            self.freqr = synth.real_mesh(qp, self.fr)
+            self.nwr = len(self.freqr) - 1
+            self.chi_r = h.array((ni, ni, self.nwr + 1), c128)
+            self.w_r = h.array((ni, ni, self.nwr + 1), c128)
+            self.d_omega1 = qp.e_q[:ntq] / RY
+            self.ekc_h = np.asfortranarray(qp.e_qmk / RY)
         self.tail = None
         if tail_model is not None:
             from .learned_tail import LearnedTail
@@ -107,10 +124,14 @@ class QPointGPU:
         self.h.x0gemm_batched_dev(self.rcxq, ni, self.nwhis, nk, ix.nkmin, ix.nkmax, self.zmel_chi, ni * no * nu,
                                   ni, nu, ix)
         self.h.dpsion_imag_dev(self.rcxq, ni, ni, self.nwhis, self.fr, self.nu, out)
+        if self.real_axis:
+            self.h.dpsion_real_dev(self.rcxq, ni, ni, self.nwhis, self.fr, self.freqr, self.chi_r)
 
     def wbuild(self):
         cfg = self.cfg
         self.h.wc_build_dev(self.chi, self.wc, cfg.ni, self.nw1, nspin=cfg.nspin)
+        if self.real_axis:
+            self.h.wc_build_real_dev(self.chi_r, self.w_r, cfg.ni, self.nwr + 1, nspin=cfg.nspin)
 
     def make_wk(self):
         """W_c at the internal k: D_k W_c(q) D_k^H (inputs for Sigma_c)."""
@@ -127,7 +148,12 @@ class QPointGPU:
         self.h.sigc_weights_dev(nk, cfg.nf, nbt, ntq, self.d_omega, self.d_ekc, cfg.esmr, cfg.ua, self.d_wk,
                                 self.d_x, self.d_wx, self.d_expa, self.wgt)
         self.zsec.zero()
-        self.h.sigc_dev(nk, self.zmel_sig, ni, nbt, ntq, cfg.nf, self.wk, self.wgt, self.zsec)
+        if self.real_axis:
+            self.h.sigc_full_dev(nk, self.zmel_sig, ni, nbt, ntq, cfg.nf, self.wk, self.wgt, self.w_r, 0, self.nwr,
+                                 self.freqr, self.d_omega1, self.ekc_h, 0.0, cfg.esmr, self.qp.wk,
+                                 self.zsec)
+        else:
+            self.h.sigc_dev(nk, self.zmel_sig, ni, nbt, ntq, cfg.nf, self.wk, self.wgt, self.zsec)
 
     def step(self):
         """chi0(q) + W_c(q) + Sigma_c(q): the per-q-point unit of work."""

@@ -215,6 +215,41 @@ def frhis(qp: QPoint) -> np.ndarray:
     return np.array([bb * (math.exp(aa * (i - 1)) - 1.0) for i in range(1, nwhis + 2)])
 
 
+def real_mesh(qp: QPoint, fr: np.ndarray, deltaw: float = 0.02, ffac: float = 10.0) -> np.ndarray:
+    """freq_r(0:nw) in Hartree for the real-axis pole term.
+
+    getwemax (S/m_freq.f90:197-268): the largest |e - E_F| among the external
+    bands (+-2 deltaw) and the bands within ffac*esmr of E_F, plus ffac*esmr
+    (Ry); getfreq (S/m_freq.f90:110-129): nw is the first iw >= 3 whose
+    (frhis(iw-2)+frhis(iw-1))/2 exceeds wemax/2 (Hartree), at most nwhis-1;
+    freq_r(0) = 0 and freq_r(i) the bin centres.  E_F = 0; deltaw is the
+    GWinput default of the reference's samples (0.02)."""
+    cfg = qp.cfg
+    ef = 0.0
+    eq = qp.e_q[:cfg.nq] / RYDBERG_EV
+    below = eq - 2 * deltaw
+    above = eq + 2 * deltaw
+    omegav = min(0.0, float(below[below < ef].min())) if np.any(below < ef) else 0.0
+    omegac = max(0.0, float(above[above > ef].max())) if np.any(above > ef) else 0.0
+    allE = np.concatenate([qp.e_k.ravel(), qp.e_kq.ravel()]) / RYDBERG_EV
+    sel = allE[allE < ef + ffac * cfg.esmr]
+    emaxv = max(0.0, float(sel.max())) if sel.size else 0.0
+    sel = allE[allE > ef - ffac * cfg.esmr]
+    eminc = min(0.0, float(sel.min())) if sel.size else 0.0
+    we = max(abs(emaxv - ef), abs(omegav - ef), abs(omegac - ef), abs(eminc - ef))
+    wemax = we + ffac * cfg.esmr
+    nwhis = len(fr) - 1
+    nw = None
+    for iw in range(3, nwhis + 1):
+        if 0.5 * (fr[iw - 3] + fr[iw - 2]) > wemax / 2.0:
+            nw = iw
+            break
+    if nw is None:
+        raise ValueError("real_mesh: histogram mesh does not reach wemax")
+    nw = min(nw, nwhis - 1)
+    return np.concatenate([[0.0], 0.5 * (fr[:nw] + fr[1:nw + 1])])
+
+
 @dataclass
 class PairWeights:
     """Per-k band pairs with histogram weights, the tetwt5 output layout

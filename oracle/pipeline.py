@@ -30,6 +30,8 @@ class QPResult:
     zsec: np.ndarray     # ntq x ntq, Hartree
     eqp: np.ndarray      # ntq, eV
     nwhis: int
+    chi_r: np.ndarray = None   # real axis (run(..., real_axis=True)): ni x ni x (nw+1)
+    w_r: np.ndarray = None
 
 
 def frequencies(cfg):
@@ -38,7 +40,7 @@ def frequencies(cfg):
     return freqx, freqw, wx, expa, nu
 
 
-def chi0(qp: synth.QPoint, k_subset=None):
+def chi0(qp: synth.QPoint, k_subset=None, real_axis=False):
     cfg = qp.cfg
     fr = synth.frhis(qp)
     nwhis = len(fr) - 1
@@ -59,6 +61,10 @@ def chi0(qp: synth.QPoint, k_subset=None):
                    int(ix.icounkmin[k]), int(ix.icounkmax[k]), ix.iwini, ix.iwend, ix.itc, ix.itpc,
                    ix.jpmc, ix.icouini, ix.whwc)
     chi = orc.dpsion_imag(rcxq, fr, imatt)
+    if real_axis:  # the same dpsion_chiq call's real branch, on the scaled rcxq (S/dpsion5.f90:243-256)
+        freqr = synth.real_mesh(qp, fr)
+        chi_r = orc.dpsion_real(rcxq, fr, orc.rmatt(freqr, fr), scaled=True)
+        return chi, nwhis, chi_r, freqr
     return chi, nwhis
 
 
@@ -69,7 +75,9 @@ def wc_k_list(qp: synth.QPoint, wc: np.ndarray):
             for k in range(qp.cfg.nk)]
 
 
-def sigc(qp: synth.QPoint, wck, k_subset=None):
+def sigc(qp: synth.QPoint, wck, k_subset=None, w_r=None, freqr=None):
+    """w_r (real-axis W_c(omega) of q, shared by every internal k) adds the
+    pole term in the same czmelwc (orc_sigc_full)."""
     cfg = qp.cfg
     freqx, _, wx, expa, _ = frequencies(cfg)
     ntq = cfg.nq
@@ -82,15 +90,25 @@ def sigc(qp: synth.QPoint, wck, k_subset=None):
         z = orc.zmel(cfg.nv_list, cfg.mdim_list, [1.0] * cfg.natom, qp.ppb, qp.c_qmk[:, :nbk, k], qp.c_q[:, :ntq],
                      conjg=False)
         g = orc.wgtim(omega, qp.e_qmk[:nbk, k] / synth.RYDBERG_EV, cfg.esmr, qp.wk[k], freqx, wx, expa, ua=cfg.ua)
-        zsec = orc.sigc_imag(z, wck[k], g, zsec)
+        if w_r is None:
+            zsec = orc.sigc_imag(z, wck[k], g, zsec)
+        else:
+            zsec, _ = orc.sigc_full(z, wck[k], g, w_r, freqr, omega, qp.e_qmk[:nbk, k] / synth.RYDBERG_EV, 0.0,
+                                    cfg.esmr, qp.wk[k], zsec=zsec)
     return zsec
 
 
-def run(qp: synth.QPoint) -> QPResult:
+def run(qp: synth.QPoint, real_axis: bool = False) -> QPResult:
+    """real_axis: also chi(omega), W_c(omega) (WVRllwR) and the pole term."""
     cfg = qp.cfg
-    chi, nwhis = chi0(qp)
+    chi_r = w_r = freqr = None
+    if real_axis:
+        chi, nwhis, chi_r, freqr = chi0(qp, real_axis=True)
+        w_r = orc.wc_build(chi_r, qp.vcousq, qp.zcousq, nspin=cfg.nspin)
+    else:
+        chi, nwhis = chi0(qp)
     wc = orc.wc_build(chi, qp.vcousq, qp.zcousq, nspin=cfg.nspin)
-    zsec = sigc(qp, wc_k_list(qp, wc))
+    zsec = sigc(qp, wc_k_list(qp, wc), w_r=w_r, freqr=freqr)
     sex, sexcore, vxc = synth.sigma_inputs(qp)
     eqp = orc.qp_energy(qp.e_q[:cfg.nq], sex, sexcore, zsec, vxc)
-    return QPResult(chi, wc, zsec, eqp, nwhis)
+    return QPResult(chi, wc, zsec, eqp, nwhis, chi_r, w_r)

@@ -215,3 +215,52 @@ def test_qpoint_pipeline(orc, name):
     assert relerr(wc, ref.wc) < TOL
     assert relerr(zsec, ref.zsec) < TOL
     assert np.abs(eqp - ref.eqp).max() < 1e-9  # eV
+
+
+@pytest.mark.parametrize("name", ["T1", "T2", "T3"])
+def test_qpoint_pipeline_real_axis(orc, name):
+    """The q-point with the real-axis chain: chi(omega), W_c(omega) (WVRllwR)
+    and the pole term of Sigma_c in the same contraction as the imaginary axis."""
+    from ecalj_amd.qsgw import QPointGPU
+    from oracle import pipeline
+
+    say("qpoint real axis", name)
+    qp = synth.make_qpoint(synth.CONFIGS[name])
+    ref = pipeline.run(qp, real_axis=True)
+    cfg = qp.cfg
+    g = QPointGPU(qp, "f64", real_axis=True)
+    g.chi0()
+    g.wbuild()
+    g.make_wk()
+    g.sigc()
+    chi, wc, zsec, eqp = g.results()
+    chi_r, w_r = g.chi_r.download(), g.w_r.download()
+    g.h.close()
+    e_chi = relerr(chi_r, ref.chi_r)
+    assert e_chi < TOL
+    s, fac = qp.vcousq, 2.0 if cfg.nspin == 1 else 1.0
+    kappa = max(np.linalg.cond(np.eye(cfg.ni) - s[:, None] * (fac * qp.zcousq.conj().T @ ref.chi_r[:, :, i]
+                                                                @ qp.zcousq) * s[None, :])
+                for i in range(ref.chi_r.shape[2]))
+    bar = max(TOL, 10 * kappa * e_chi)
+    assert relerr(w_r, ref.w_r) < bar
+    assert relerr(zsec, ref.zsec) < bar
+    assert np.abs(eqp - ref.eqp).max() < 1e-6  # eV
+
+
+def test_qpoint_real_axis_mixed_within_1meV():
+    from ecalj_amd.qsgw import QPointGPU
+
+    say("qpoint real axis mixed T3")
+    qp = synth.make_qpoint(synth.CONFIGS["T3"])
+    e = {}
+    for prec in ("f64", "bf16x3", "f32"):
+        g = QPointGPU(qp, prec, real_axis=True)
+        g.chi0()
+        g.wbuild()
+        g.make_wk()
+        g.sigc()
+        e[prec] = g.results()[3]
+        g.h.close()
+    assert np.abs(e["bf16x3"] - e["f64"]).max() < 1e-3
+    assert np.abs(e["f32"] - e["f64"]).max() < 1e-3
