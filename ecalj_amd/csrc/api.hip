@@ -148,13 +148,14 @@ static void hilbert_coef(int nwhis, int niw, const double* frhis, const double* 
   const int n2 = 2 * nwhis + 1;
   std::vector<double> L, C, R;
   his_edges(nwhis, frhis, ecut, L, C, R, scale);
-  coef.assign((size_t)niw * nwhis, 0.0);
+  const int ldc = hilbert_ldc(niw);
+  coef.assign((size_t)ldc * nwhis, 0.0);
   std::vector<std::complex<double>> rr(n2);
   const double pi = 3.14159265358979323846;
   for (int it = 0; it < niw; ++it) {
     hilbertmat_host(std::complex<double>(0.0, freqi[it]), nwhis, L.data(), C.data(), R.data(), rr.data());
     for (int iw = 1; iw <= nwhis; ++iw)
-      coef[it + (size_t)niw * (iw - 1)] =
+      coef[it + (size_t)ldc * (iw - 1)] =
           std::real(rr[nwhis + iw] - rr[nwhis - iw]) / pi * scale[iw - 1];
   }
 }
@@ -169,8 +170,8 @@ static void hilbert_coef_real(int nwhis, int nw, const double* frhis, const doub
   std::vector<double> L, C, R, scale;
   his_edges(nwhis, frhis, ecut, L, C, R, scale);
   if (scaled) scale.assign(nwhis, 1.0);
-  const int n2 = 2 * nwhis + 1, nv = nw + 1;
-  coef.assign((size_t)nv * nwhis, 0.0);
+  const int n2 = 2 * nwhis + 1, nv = nw + 1, ldc = hilbert_ldc(nv);
+  coef.assign((size_t)ldc * nwhis, 0.0);
   dscale.assign(nwhis + 1, 0.0);
   dscale[0] = 1.0;
   for (int iw = 1; iw <= nwhis; ++iw) dscale[iw] = scale[iw - 1];
@@ -179,7 +180,7 @@ static void hilbert_coef_real(int nwhis, int nw, const double* frhis, const doub
   for (int it = 0; it < nv; ++it) {
     hilbertmat_host(std::complex<double>(freqr[it], 0.0), nwhis, L.data(), C.data(), R.data(), rr.data());
     for (int iw = 1; iw <= nwhis; ++iw)
-      coef[it + (size_t)nv * (iw - 1)] = std::real(rr[nwhis + iw] - rr[nwhis - iw]) / pi * scale[iw - 1];
+      coef[it + (size_t)ldc * (iw - 1)] = std::real(rr[nwhis + iw] - rr[nwhis - iw]) / pi * scale[iw - 1];
   }
 }
 
@@ -878,8 +879,8 @@ ecalj_status ecalj_dpsion_imag_dev(ecalj_handle h, void* rcxq, int npr, int npr_
   const int64_t n = (int64_t)npr * npr_col;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 4, &tev));
-  ECALJ_TRY(hilbert(h, wide(h), rcxq, n, nwhis, niw, d, scale_rcxq ? d + coef.size() : nullptr, nullptr,
-                    (cd*)zxqi, n));
+  ECALJ_TRY(hilbert(h, wide(h), rcxq, n, nwhis, niw, hilbert_ldc(niw), d, scale_rcxq ? d + coef.size() : nullptr,
+                    nullptr, (cd*)zxqi, n));
   ECALJ_TRY(timing_end(h, 4, tev));
   return ECALJ_OK;
 }
@@ -916,8 +917,8 @@ ecalj_status ecalj_dpsion_real_dev(ecalj_handle h, const void* rcxq, int npr, in
   const int64_t n = (int64_t)npr * npr_col;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 4, &tev));
-  ECALJ_TRY(hilbert(h, wide(h), const_cast<void*>(rcxq), n, nwhis, nw + 1, d, nullptr, d + coef.size(), (cd*)zxq,
-                    n));
+  ECALJ_TRY(hilbert(h, wide(h), const_cast<void*>(rcxq), n, nwhis, nw + 1, hilbert_ldc(nw + 1), d, nullptr,
+                    d + coef.size(), (cd*)zxq, n));
   ECALJ_TRY(timing_end(h, 4, tev));
   return ECALJ_OK;
 }

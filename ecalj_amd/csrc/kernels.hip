@@ -204,9 +204,11 @@ ecalj_status x0_herk(ecalj_ctx* h, int mode, const void* z, const int64_t* d_col
 //   out(e, v) = i dscale(v0+v) rcxq(e, v0+v) + sum_iw coef(v, iw) rcxq(e, iw)
 //                                                              (real axis, dscale != null)
 // for the output chunk v in [v0, v0+nv); coef(v, iw) = coef[v + ldc*(iw-1)]
-// (chunk base already applied).  FP64 accumulation, complex128 output;
-// rcxq complex128 or complex64.  With `scale`, rcxq(:, iw) *= scale(iw) is
-// stored back (the reference scales rcxq in place once).
+// (chunk base already applied), zero-padded so that NV coefficients can be
+// read for every chunk: the FMA loop runs over all NV outputs without a
+// per-output test.  FP64 accumulation, complex128 output; rcxq complex128
+// or complex64.  With `scale`, rcxq(:, iw) *= scale(iw) is stored back (the
+// reference scales rcxq in place once).
 template <int NV, class T>
 __global__ __launch_bounds__(256) void hilbert_kernel(T* __restrict__ rcxq, int64_t slice, int nwhis,
                                                       int nv, int ldc, const double* __restrict__ coef,
@@ -228,27 +230,20 @@ __global__ __launch_bounds__(256) void hilbert_kernel(T* __restrict__ rcxq, int6
     const double* c = coef + (int64_t)ldc * (iw - 1);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      if (v < nv) {
-        ar[v] += c[v] * xr;
-        ai[v] += c[v] * xi;
-      }
-    }
-    const int d = iw - v0;  // wave-uniform: only nv of the nwhis bins take this branch
-    if (dscale && d >= 0 && d < nv) {
-      const double s = dscale[iw];
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-        if (v == d) {
-          ar[v] -= s * xi;
-          ai[v] += s * xr;
-        }
+      ar[v] += c[v] * xr;
+      ai[v] += c[v] * xi;
     }
   }
-  if (dscale && v0 == 0) {  // slot 0: the zero-frequency term, never accumulated by x0gemm
-    double xr, xi;
-    ld_c2(rcxq + e, xr, xi);
-    ar[0] -= dscale[0] * xi;
-    ai[0] += dscale[0] * xr;
+  if (dscale) {  // i*dscale*rcxq(:, it): slot it of rcxq is the bin centred at freq_r(it); slot 0 the zero frequency
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      if (v < nv) {
+        double xr, xi;
+        ld_c2(rcxq + e + slice * (v0 + v), xr, xi);
+        const double s = dscale[v0 + v];
+        ar[v] -= s * xi;
+        ai[v] += s * xr;
+      }
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v)
@@ -264,7 +259,7 @@ static void hilbert_launch(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, int 
 }
 
 template <class T>
-static ecalj_status hilbert_t(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
+static ecalj_status hilbert_t(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, int nv, int ldc, const double* d_coef,
                               const double* d_scale, const double* d_dscale, cd* out, int64_t n) {
   // chunks of at most 32 outputs (64 FP64 accumulators per lane); each chunk
   // re-reads rcxq.  The chunks run last to first, so that the one that stores
@@ -273,21 +268,22 @@ static ecalj_status hilbert_t(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, i
     const int c = std::min(32, nv - v0);
     const double* sc = (v0 == 0) ? d_scale : nullptr;
     if (c <= 8)
-      hilbert_launch<8, T>(h, rcxq, slice, nwhis, c, nv, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
+      hilbert_launch<8, T>(h, rcxq, slice, nwhis, c, ldc, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
     else if (c <= 16)
-      hilbert_launch<16, T>(h, rcxq, slice, nwhis, c, nv, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
+      hilbert_launch<16, T>(h, rcxq, slice, nwhis, c, ldc, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
     else
-      hilbert_launch<32, T>(h, rcxq, slice, nwhis, c, nv, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
+      hilbert_launch<32, T>(h, rcxq, slice, nwhis, c, ldc, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
     ECALJ_HIP(hipGetLastError());
   }
   return ECALJ_OK;
 }
 
-ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
-                     const double* d_scale, const double* d_dscale, cd* out, int64_t n) {
+ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, int ldc,
+                     const double* d_coef, const double* d_scale, const double* d_dscale, cd* out, int64_t n) {
   if (n == 0 || nv == 0) return ECALJ_OK;
-  if (wide) return hilbert_t<cd>(h, (cd*)rcxq, slice, nwhis, nv, d_coef, d_scale, d_dscale, out, n);
-  return hilbert_t<cf>(h, (cf*)rcxq, slice, nwhis, nv, d_coef, d_scale, d_dscale, out, n);
+  if (ldc < hilbert_ldc(nv)) return ECALJ_ERR_ARG;
+  if (wide) return hilbert_t<cd>(h, (cd*)rcxq, slice, nwhis, nv, ldc, d_coef, d_scale, d_dscale, out, n);
+  return hilbert_t<cf>(h, (cf*)rcxq, slice, nwhis, nv, ldc, d_coef, d_scale, d_dscale, out, n);
 }
 
 // ==========================================================================
@@ -408,6 +404,10 @@ ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const int* perm, const double*
 // Pivot block width.  (64 halves the panel steps, but its 64-step diagonal
 // inverse on one block per matrix costs more than the wider updates save.)
 constexpr int GJB = 32;
+// Outer pivot block.  The trailing update of each outer step is a rank-GJO
+// GEMM (K = GJO), so the matrix is read and written n/GJO times instead of
+// n/GJB times; the GJO x GJO diagonal block is inverted by the GJB scheme.
+constexpr int GJO = 128;
 
 __global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n, int64_t lda,
                                                       int64_t astride, int k0, int b,
@@ -473,9 +473,9 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n,
   }
 }
 
-// Row exchanges for pivot block K = [k0, k0+b) (partial pivoting, as
-// zgetrf): LU with exchanges of the panel rows k0..n-1 of columns K, on a
-// copy in `work` (leading dimension n).  Only the exchanges are kept:
+// Row exchanges for pivot block K = [k0, k0+b), b <= GJO (partial pivoting,
+// as zgetrf): LU with exchanges of the panel rows k0..n-1 of columns K, on a
+// copy in `work` (leading dimension n, n*b per matrix).  Only the exchanges are kept:
 // ipiv[k0+kk] = row exchanged with row k0+kk (absolute, 0-based).  The pivot
 // is the first row of largest |re|+|im| (izamax).  The in-order elimination
 // of the exchanged block then meets exactly these pivots.
@@ -484,11 +484,11 @@ __global__ __launch_bounds__(256) void gj_panel_kernel(const cd* __restrict__ a,
                                                        int* __restrict__ ipiv, int* __restrict__ flag) {
   __shared__ double red_v[4];
   __shared__ int red_i[4];
-  __shared__ cd prow[GJB];
+  __shared__ cd prow[GJO];
   __shared__ int s_p;
   const int mat = blockIdx.x;
   const cd* A = a + astride * mat;
-  cd* W = work + (int64_t)n * GJB * mat;
+  cd* W = work + (int64_t)n * b * mat;
   int* P = ipiv + (int64_t)n * mat;
   const int m = n - k0;
   const int tid = threadIdx.x;
@@ -637,16 +637,12 @@ ecalj_status gj_unscramble(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astr
   return ECALJ_OK;
 }
 
-ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, const int** perm) {
-  if (n == 0 || nmat == 0) return ECALJ_OK;
-  if (nmat > 65535) return ECALJ_ERR_DIM;
-  const bool pivot = perm != nullptr;
-  if (pivot && n > 8192) return ECALJ_ERR_DIM;  // index tables of gj_perm_kernel live in LDS
+// One level: pivot blocks of GJB columns.
+static ecalj_status gj_inverse_1(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, int* ipiv) {
+  const bool pivot = ipiv != nullptr;
   cd *pinv, *cp;
-  int* ipiv = nullptr;
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJP, sizeof(cd) * GJB * GJB * nmat, (void**)&pinv));
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJC, sizeof(cd) * (size_t)n * GJB * nmat, (void**)&cp));
-  if (pivot) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJPIV, sizeof(int) * 2 * (size_t)n * nmat, (void**)&ipiv));
   for (int k0 = 0; k0 < n; k0 += GJB) {
     const int b = std::min(GJB, n - k0);
     if (pivot) {  // choose and apply the row exchanges of this block (cp is free until gj_diag)
@@ -677,6 +673,77 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
     t.batch = nmat; t.div = nmat;
     t.epi = EPI_GJ; t.gj_k0 = k0; t.gj_k1 = k0 + b;
     ECALJ_TRY(gemm(h, t));
+  }
+  return ECALJ_OK;
+}
+
+// Outer step copies: D = A[K,K] (ld GJO), Cp = A[:,K] (ld n), R = A[K,:] with
+// R[:,K] = I (ld GJO), for K = [k0, k0+b).
+__global__ void gj_outer_copy_kernel(const cd* __restrict__ a, int n, int64_t lda, int64_t astride, int k0, int b,
+                                     int nmat, cd* __restrict__ D, cd* __restrict__ Cp, cd* __restrict__ R) {
+  const int64_t per = (int64_t)n * b, tot = per * nmat;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int mat = (int)(e / per);
+    const int64_t r = e % per;
+    const cd* A = a + astride * mat;
+    {  // Cp(i, l), i over n
+      const int i = (int)(r % n), l = (int)(r / n);
+      const cd x = A[i + lda * (k0 + l)];
+      Cp[(int64_t)n * b * mat + i + (int64_t)n * l] = x;
+      if (i >= k0 && i < k0 + b) D[(int64_t)GJO * GJO * mat + (i - k0) + (int64_t)GJO * l] = x;
+    }
+    {  // R(l, j), j over n
+      const int l = (int)(r % b), j = (int)(r / b);
+      const cd x = (j >= k0 && j < k0 + b) ? cd{(j - k0) == l ? 1.0 : 0.0, 0.0} : A[(k0 + l) + lda * j];
+      R[(int64_t)GJO * n * mat + l + (int64_t)GJO * j] = x;
+    }
+  }
+}
+
+ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, const int** perm) {
+  if (n == 0 || nmat == 0) return ECALJ_OK;
+  if (nmat > 65535) return ECALJ_ERR_DIM;
+  const bool pivot = perm != nullptr;
+  if (pivot && n > 8192) return ECALJ_ERR_DIM;  // index tables of gj_perm_kernel live in LDS
+  int* ipiv = nullptr;
+  if (pivot) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJPIV, sizeof(int) * 2 * (size_t)n * nmat, (void**)&ipiv));
+  // Pivoted inverses stay one-level: choosing the exchanges of a GJO-wide
+  // block in one workgroup per matrix (gj_panel_kernel's sequential column
+  // steps) costs more than the wider trailing updates save.
+  if (n <= GJO || pivot) {
+    ECALJ_TRY(gj_inverse_1(h, a, n, lda, astride, nmat, ipiv));
+  } else {
+    cd *D, *Cp, *R;
+    ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJD, sizeof(cd) * GJO * GJO * (size_t)nmat, (void**)&D));
+    ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJOC, sizeof(cd) * (size_t)n * GJO * nmat, (void**)&Cp));
+    ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJR, sizeof(cd) * (size_t)n * GJO * nmat, (void**)&R));
+    for (int k0 = 0; k0 < n; k0 += GJO) {
+      const int b = std::min(GJO, n - k0);
+      const int64_t tot = (int64_t)n * b * nmat;
+      hipLaunchKernelGGL(gj_outer_copy_kernel, dim3((unsigned)std::min<int64_t>((tot + 255) / 256, 65536)),
+                         dim3(256), 0, h->stream, a, n, lda, astride, k0, b, nmat, D, Cp, R);
+      ECALJ_HIP(hipGetLastError());
+      // D <- D^-1 (in order: the matrices taking this path are safe to eliminate without exchanges)
+      ECALJ_TRY(gj_inverse_1(h, D, b, GJO, (int64_t)GJO * GJO, nmat, nullptr));
+      // A[K,:] = D^-1 R   (A[K,K] becomes D^-1)
+      GemmCall r;
+      r.M = b; r.N = n; r.K = b;
+      r.A = D; r.lda = GJO; r.bA = {(int64_t)GJO * GJO, 0};
+      r.B = R; r.ldb = GJO; r.bB = {(int64_t)GJO * n, 0};
+      r.C = a + k0; r.ldc = lda; r.bC = {astride, 0};
+      r.batch = nmat; r.div = nmat;
+      ECALJ_TRY(gemm(h, r));
+      // A[i,:] = (j in K ? 0 : A[i,:]) - Cp[i,:] A[K,:],  i not in K
+      GemmCall t;
+      t.M = n; t.N = n; t.K = b;
+      t.A = Cp; t.lda = n; t.bA = {(int64_t)n * b, 0};
+      t.B = a + k0; t.ldb = lda; t.bB = {astride, 0};
+      t.C = a; t.ldc = lda; t.bC = {astride, 0};
+      t.batch = nmat; t.div = nmat;
+      t.epi = EPI_GJ; t.gj_k0 = k0; t.gj_k1 = k0 + b;
+      ECALJ_TRY(gemm(h, t));
+    }
   }
   if (pivot) {
     int* pm = ipiv + (size_t)n * nmat;

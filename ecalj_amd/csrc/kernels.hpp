@@ -8,9 +8,11 @@ ecalj_status ppb_relayout(ecalj_ctx* h, const double* src, double* dst, int nv, 
 ecalj_status x0_herk(ecalj_ctx* h, int mode, const void* z, const int64_t* d_col, const double* d_w,
                      const int64_t* d_bstart, const void* d_items, int nitems, void* rcxq,
                      int64_t slice_elems, int npr, int npr_col, int ipr_col, int mirror);
-// d_dscale (optional, real axis): adds i*dscale(it)*rcxq(:, it) to output it
-ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, const double* d_coef,
-                     const double* d_scale, const double* d_dscale, cd* out, int64_t n);
+// d_coef: coef(v, iw) at [v + ldc*(iw-1)], ldc >= hilbert_ldc(nv), zero beyond nv.
+// d_dscale (optional, real axis): adds i*dscale(it)*rcxq(:, it) to output it.
+inline int hilbert_ldc(int nv) { return ((nv + 31) / 32) * 32; }
+ecalj_status hilbert(ecalj_ctx* h, bool wide, void* rcxq, int64_t slice, int nwhis, int nv, int ldc,
+                     const double* d_coef, const double* d_scale, const double* d_dscale, cd* out, int64_t n);
 ecalj_status chi0_freq(ecalj_ctx* h, int mode, const void* z, const double* w, int ni, int npairs, int nf, cd* chi);
 ecalj_status convert(ecalj_ctx* h, const void* src, bool src64, void* dst, bool dst64, int64_t n);
 ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac, int n, int nw, cd* eps);

@@ -29,11 +29,14 @@ struct RealItem {
   int k, iw, e0, ne, i0, pad;
 };
 
+// pk (optional): the internal k of each pole, for items whose entries come
+// from several k-points (W shared by every k: items grouped by frequency only).
 template <class T>
 __global__ __launch_bounds__(256) void sig_real_entries_kernel(const RealItem* __restrict__ items, int item0,
                                                                const int* __restrict__ perm,
                                                                const int* __restrict__ ecol,
                                                                const double* __restrict__ ew,
+                                                               const int* __restrict__ pk,
                                                                const T* __restrict__ zmel, int64_t zstride,
                                                                const cd* __restrict__ wr, int64_t ldw,
                                                                int64_t wstride, int n, cd* __restrict__ slot) {
@@ -67,8 +70,9 @@ __global__ __launch_bounds__(256) void sig_real_entries_kernel(const RealItem* _
       cd v{0.0, 0.0};
       if (e < it.ne && J < n) {
         const int id = perm[it.e0 + e];
+        const T* Ze = pk ? zmel + zstride * pk[id / 3] : Z;
         double xr, xi;
-        ld_c2(Z + (int64_t)n * ecol[id] + J, xr, xi);
+        ld_c2(Ze + (int64_t)n * ecol[id] + J, xr, xi);
         v = cd{ew[id] * xr, ew[id] * xi};
       }
       zb[jj][e] = v;
@@ -210,6 +214,8 @@ struct RealPlan {
   std::vector<int> perm, pstart, plist;
   std::vector<RealItem> items;
   std::vector<int> kitem, kpole;  // first item / pole of each k (nb+1)
+  bool shared = false;            // one W for every k: items grouped by frequency across k
+  mutable bool entries_done = false;
   unsigned char* d = nullptr;     // device table
   size_t o_it = 0, o_pm = 0, o_ec = 0, o_ps = 0, o_pl = 0, o_pk = 0, o_pt = 0, o_pc = 0, o_ew = 0;
   cd* slot = nullptr;
@@ -268,22 +274,27 @@ static ecalj_status real_plan(ecalj_ctx* h, const RealSpec& rs, int nb, int n, i
   P.kpole[nb] = P.np = (int)P.pk.size();
   P.ne = 3 * P.np;
   if (P.np == 0) return ECALJ_OK;
-  // entries ordered by (k, iw); items of <= RE entries x row tiles; items of k contiguous
+  // Entries ordered by (k, iw), items of <= RE entries of one (k, iw) x row
+  // tiles, items of k contiguous.  With one W for every k (wstride 0) the
+  // entries of all k are ordered by iw and an item takes entries of several
+  // k: each W(iw) row tile is then read once per RE entries of the whole
+  // call, not once per k.
+  P.shared = rs.wstride == 0;
   P.perm.resize(P.ne);
   for (int e = 0; e < P.ne; ++e) P.perm[e] = e;
   std::stable_sort(P.perm.begin(), P.perm.end(), [&](int x, int y) {
-    const int kx = P.pk[x / 3], ky = P.pk[y / 3];
+    const int kx = P.shared ? 0 : P.pk[x / 3], ky = P.shared ? 0 : P.pk[y / 3];
     return kx != ky ? kx < ky : P.eiw[x] < P.eiw[y];
   });
   P.kitem.assign(nb + 1, 0);
   for (int s0 = 0; s0 < P.ne;) {
-    const int k = P.pk[P.perm[s0] / 3], iw = P.eiw[P.perm[s0]];
+    const int k = P.shared ? 0 : P.pk[P.perm[s0] / 3], iw = P.eiw[P.perm[s0]];
     int e = s0;
-    while (e < P.ne && P.pk[P.perm[e] / 3] == k && P.eiw[P.perm[e]] == iw && e - s0 < RE) ++e;
+    while (e < P.ne && (P.shared || P.pk[P.perm[e] / 3] == k) && P.eiw[P.perm[e]] == iw && e - s0 < RE) ++e;
     for (int i0 = 0; i0 < n; i0 += RT) P.items.push_back({k, iw, s0, e - s0, i0, 0});
     s0 = e;
   }
-  {
+  if (!P.shared) {
     int q = 0;
     for (int k = 0; k <= nb; ++k) {
       while (q < (int)P.items.size() && P.items[q].k < k) ++q;
@@ -320,15 +331,23 @@ static ecalj_status real_plan(ecalj_ctx* h, const RealSpec& rs, int nb, int n, i
   return ECALJ_OK;
 }
 
-// entry columns of the k-points [k0, k1) into the slots
+// entry columns of the k-points [k0, k1) into the slots (shared W: the
+// entries of every k, on the first call)
 template <class T>
 static ecalj_status real_entries(ecalj_ctx* h, const RealPlan& P, const RealSpec& rs, const T* zmel,
                                  int64_t zstride, int n, int k0, int k1) {
-  const int i0 = P.kitem[k0], i1 = P.kitem[k1];
+  int i0 = P.kitem[k0], i1 = P.kitem[k1];
+  if (P.shared) {
+    if (P.entries_done) return ECALJ_OK;
+    i0 = 0;
+    i1 = (int)P.items.size();
+    P.entries_done = true;
+  }
   if (i1 <= i0) return ECALJ_OK;
   hipLaunchKernelGGL((sig_real_entries_kernel<T>), dim3((unsigned)(i1 - i0)), dim3(256), 0, h->stream,
                      (const RealItem*)(P.d + P.o_it), i0, (const int*)(P.d + P.o_pm), (const int*)(P.d + P.o_ec),
-                     (const double*)(P.d + P.o_ew), zmel, zstride, rs.wr, rs.ldw, rs.wstride, n, P.slot);
+                     (const double*)(P.d + P.o_ew), P.shared ? (const int*)(P.d + P.o_pk) : nullptr, zmel, zstride,
+                     rs.wr, rs.ldw, rs.wstride, n, P.slot);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
