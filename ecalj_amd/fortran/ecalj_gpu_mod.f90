@@ -13,6 +13,8 @@ module ecalj_gpu
   public :: ecalj_zmel_dev, ecalj_x0gemm_batched_dev, ecalj_dpsion_imag_dev, ecalj_wc_build_dev
   public :: ecalj_sigc_weights_dev, ecalj_sigc_dev, ecalj_sigc_grouped_dev
   public :: ecalj_set_tuning, ecalj_graph_begin, ecalj_graph_end, ecalj_graph_launch, ecalj_graph_destroy
+  public :: ecalj_dpsion_real, ecalj_dpsion_real_dev, ecalj_wc_build_real, ecalj_wc_build_real_dev
+  public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
   integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
 
   interface
@@ -110,6 +112,53 @@ module ecalj_gpu
       real(c_double), intent(in) :: frhis(*), freqi(*)
       real(c_double), value :: ecut
     end function
+    ! dpsion_chiq real branch, npm=1 (S/dpsion5.f90:193-200,244-256); rcxq read only, zxq(:,:,0:nw) returned
+    integer(c_int) function ecalj_dpsion_real(h, rcxq, npr, npr_col, nwhis, nw, frhis, freqr, ecut, rcxq_scaled, &
+        zxq) bind(C, name="ecalj_dpsion_real")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: rcxq(*)
+      integer(c_int), value :: npr, npr_col, nwhis, nw, rcxq_scaled
+      real(c_double), intent(in) :: frhis(*), freqr(*)
+      real(c_double), value :: ecut
+      complex(c_double_complex), intent(out) :: zxq(*)
+    end function
+    integer(c_int) function ecalj_dpsion_real_dev(h, rcxq, npr, npr_col, nwhis, nw, frhis, freqr, ecut, &
+        rcxq_scaled, zxq) bind(C, name="ecalj_dpsion_real_dev")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h, rcxq, zxq
+      integer(c_int), value :: npr, npr_col, nwhis, nw, rcxq_scaled
+      real(c_double), intent(in) :: frhis(*), freqr(*)
+      real(c_double), value :: ecut
+    end function
+    ! WVRllwR body, q /= 0 (S/m_llw.f90:88-180): pivoted inversion
+    integer(c_int) function ecalj_wc_build_real(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw) &
+        bind(C, name="ecalj_wc_build_real")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: chi(*)
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, ldw
+      complex(c_double_complex), intent(inout) :: wc(*)
+    end function
+    integer(c_int) function ecalj_wc_build_real_dev(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw) &
+        bind(C, name="ecalj_wc_build_real_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, chi, wc
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, ldw
+    end function
+    ! zminv_h / zminv_d (S/m_lapack.f90:21-47,108-176)
+    integer(c_int) function ecalj_zminv(h, a, n, lda) bind(C, name="ecalj_zminv")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(inout) :: a(*)
+      integer(c_int), value :: n, lda
+    end function
+    integer(c_int) function ecalj_zminv_dev(h, a, n, lda, nbatch, stride) bind(C, name="ecalj_zminv_dev")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: h, a
+      integer(c_int), value :: n, lda, nbatch
+      integer(c_int64_t), value :: stride
+    end function
     ! WVIllwI body, q /= 0 (S/m_llw.f90:299-373)
     integer(c_int) function ecalj_wc_build(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw) &
         bind(C, name="ecalj_wc_build")
@@ -157,6 +206,36 @@ module ecalj_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: h, zmel, wc, wgtim, zsec
       integer(c_int), value :: nbatch, ngb, nm, ntq, niw, ldw, ldz
+    end function
+    ! real-axis pole term (S/m_sxcf_gemm.f90:501-608)
+    integer(c_int) function ecalj_sigc_real(h, zmel, ngb, nm, ntq, wr, ldw, nw, freq_r, omega, ekc, ef, esmr, &
+        nctot, ns2r, wkkr, zsec, ldz) bind(C, name="ecalj_sigc_real")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: zmel(*), wr(*)
+      integer(c_int), value :: ngb, nm, ntq, ldw, nw, nctot, ns2r, ldz
+      real(c_double), intent(in) :: freq_r(*), omega(*), ekc(*)
+      real(c_double), value :: ef, esmr, wkkr
+      complex(c_double_complex), intent(inout) :: zsec(*)
+    end function
+    integer(c_int) function ecalj_sigc_real_dev(h, nbatch, zmel, ngb, nm, ntq, wr, ldw, wr_stride, nw, freq_r, &
+        omega, ekc, ef, esmr, nctot, ns2r, wkkr, zsec, ldz) bind(C, name="ecalj_sigc_real_dev")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h, zmel, wr, zsec
+      integer(c_int), value :: nbatch, ngb, nm, ntq, ldw, nw, nctot, ns2r, ldz
+      integer(c_int64_t), value :: wr_stride
+      real(c_double), intent(in) :: freq_r(*), omega(*), ekc(*), wkkr(*)
+      real(c_double), value :: ef, esmr
+    end function
+    ! get_correlation_block in full (S/m_sxcf_gemm.f90:423-608)
+    integer(c_int) function ecalj_sigc_full_dev(h, nbatch, zmel, ngb, nm, ntq, niw, wc, ldw, wgtim, wr, ldwr, &
+        wr_stride, nw, freq_r, omega, ekc, ef, esmr, nctot, ns2r, wkkr, zsec, ldz) bind(C, name="ecalj_sigc_full_dev")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h, zmel, wc, wgtim, wr, zsec
+      integer(c_int), value :: nbatch, ngb, nm, ntq, niw, ldw, ldwr, nw, nctot, ns2r, ldz
+      integer(c_int64_t), value :: wr_stride
+      real(c_double), intent(in) :: freq_r(*), omega(*), ekc(*), wkkr(*)
+      real(c_double), value :: ef, esmr
     end function
     ! Sigma_c with per-k middle-state windows (S/sxcf_fal2_count.sc.f90:118-236)
     integer(c_int) function ecalj_sigc_grouped_dev(h, nk, nm, zmel, ngb, ntq, niw, wc, ldw, wgtim, zsec, ldz) &
