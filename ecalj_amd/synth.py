@@ -215,6 +215,99 @@ def frhis(qp: QPoint) -> np.ndarray:
     return np.array([bb * (math.exp(aa * (i - 1)) - 1.0) for i in range(1, nwhis + 2)])
 
 
+# --------------------------------------------------------------------------
+# k-mesh, tetrahedra and smooth bands for the tetrahedron weights
+@dataclass
+class TetraMesh:
+    """The BZ-mesh arrays that tetwt5x_dtet4 takes (getbzdata1's output,
+    S/getbzdata1.f90:247-293): qbzw the (n1+1)(n2+1)(n3+1) extended mesh
+    (3 x nqbzw, 2pi/alat units), ib1bz its 1-based k index in the n1 n2 n3
+    mesh, idtetf 4 x ntetf 1-based qbzw indices of the tetrahedra (6 per
+    micro-cell), qlat the reciprocal vectors (columns)."""
+    n: tuple
+    qlat: np.ndarray
+    qbzw: np.ndarray
+    ib1bz: np.ndarray
+    idtetf: np.ndarray
+
+    @property
+    def nqbz(self):
+        return int(np.prod(self.n))
+
+    def frac(self):
+        """Fractional coordinates (3 x nqbz) of the k-points, k index i1 + n1*(i2 + n2*i3)."""
+        n1, n2, n3 = self.n
+        i = np.arange(self.nqbz)
+        return np.stack([(i % n1) / n1, ((i // n1) % n2) / n2, (i // (n1 * n2)) / n3])
+
+
+def tetra_mesh(n1: int, n2: int, n3: int, qlat=None) -> TetraMesh:
+    """Each micro-cell is cut into the 6 tetrahedra that share its (0,0,0)-(1,1,1)
+    diagonal (the reference picks the shortest diagonal; for a cubic cell all
+    four are equal)."""
+    ql = np.eye(3) if qlat is None else np.asarray(qlat, float)
+    ext = [(i1, i2, i3) for i3 in range(n3 + 1) for i2 in range(n2 + 1) for i1 in range(n1 + 1)]
+    widx = {e: j for j, e in enumerate(ext)}
+    qbzw = np.array([ql @ np.array([e[0] / n1, e[1] / n2, e[2] / n3]) for e in ext]).T
+    ib1bz = np.array([1 + (e[0] % n1) + n1 * ((e[1] % n2) + n2 * (e[2] % n3)) for e in ext], np.int32)
+    paths = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = []
+    for i3 in range(n3):
+        for i2 in range(n2):
+            for i1 in range(n1):
+                for p in paths:
+                    c = [0, 0, 0]
+                    corners = [tuple(c)]
+                    for ax in p:
+                        c[ax] += 1
+                        corners.append(tuple(c))
+                    tets.append([widx[(i1 + a, i2 + b, i3 + d)] + 1 for a, b, d in corners])
+    return TetraMesh((n1, n2, n3), ql, np.asfortranarray(qbzw), ib1bz, np.asfortranarray(np.array(tets, np.int32).T))
+
+
+def tetra_bands(mesh: TetraMesh, nocc: int, nunocc: int, seed: int = 20261210, nmetal: int = 2,
+                degenerate: bool = True, iq=(1, 0, 0)):
+    """Smooth bands on the mesh: E_n(k) = c_n + w_n (cos 2pi f1 + cos 2pi f2 + cos 2pi f3)/3
+    + a_n sin 2pi(f1 + 2 f2 - f3)/4 (eV), sorted per k as eigenvalues are.
+    nmetal of the occupied bands cross E_F = 0 (Fermi-surface cuts); with
+    `degenerate` the lowest band is doubled (degenerate-band symmetrisation).
+    Returns ek1 (k), ek2 (k+q, q = iq/n on the mesh), both nband x nqbz in Ry,
+    and E_F = 0."""
+    rng = np.random.default_rng(seed)
+    c = np.concatenate([np.sort(rng.uniform(-12.0, -2.0, nocc - nmetal)), rng.uniform(-0.6, 0.6, nmetal),
+                        np.sort(rng.uniform(1.0, 60.0, nunocc))])
+    w = np.concatenate([rng.uniform(0.5, 2.0, nocc - nmetal), rng.uniform(3.0, 5.0, nmetal),
+                        rng.uniform(1.0, 4.0, nunocc)])
+    a = rng.uniform(-0.3, 0.3, nocc + nunocc)
+    if degenerate and nocc - nmetal >= 2:
+        c[1], w[1], a[1] = c[0], w[0], a[0]
+
+    def bands(f):
+        g = (np.cos(2 * np.pi * f[0]) + np.cos(2 * np.pi * f[1]) + np.cos(2 * np.pi * f[2])) / 3.0
+        h = np.sin(2 * np.pi * (f[0] + 2 * f[1] - f[2])) / 4.0
+        e = c[:, None] + w[:, None] * g[None, :] + a[:, None] * h[None, :]
+        return np.sort(e, axis=0) / RYDBERG_EV
+
+    f = mesh.frac()
+    fq = (f + np.array([iq[0] / mesh.n[0], iq[1] / mesh.n[1], iq[2] / mesh.n[2]])[:, None]) % 1.0
+    return np.asfortranarray(bands(f)), np.asfortranarray(bands(fq)), 0.0
+
+
+def tetra_frhis(ek1, ek2, oratio=1.03, dw=1e-5):
+    """frhis for the band range of ek1/ek2 (getfreq2's omg2max, S/m_freq.f90:63-64)."""
+    allE = np.concatenate([np.ravel(ek1), np.ravel(ek2)])
+    omg = (allE.max() - allE.min()) * 0.5 + 0.2
+    aa = oratio - 1.0
+    bb = dw / aa
+    iw = 0
+    while True:
+        iw += 1
+        if bb * (math.exp(aa * (iw - 1)) - 1.0) > omg + 1e-6:
+            break
+    nwhis = iw + 2
+    return np.array([bb * (math.exp(aa * (i - 1)) - 1.0) for i in range(1, nwhis + 2)])
+
+
 def real_mesh(qp: QPoint, fr: np.ndarray, deltaw: float = 0.02, ffac: float = 10.0) -> np.ndarray:
     """freq_r(0:nw) in Hartree for the real-axis pole term.
 

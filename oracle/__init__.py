@@ -164,6 +164,59 @@ def imatt(freqi, frhis_):
     return out
 
 
+def lindtet6(kvec, ea, eb, x, efa, efb, frhis_):
+    """Histogram of one tetrahedron (kvec: 4 x 3 corners)."""
+    fr = _cd(frhis_)
+    nwhis = len(fr) - 1
+    out = np.zeros(nwhis)
+    k = np.ascontiguousarray(np.asarray(kvec, np.float64))
+    rc = lib().orc_lindtet6(k.ctypes.data_as(_dp), _p(_cd(ea)), _p(_cd(eb)), _p(_cd(x)), C.c_double(efa),
+                            C.c_double(efb), _p(fr), nwhis, _p(out))
+    if rc != 0:
+        raise ValueError(f"lindtet6 rc={rc}")
+    return out
+
+
+def chkdgn(ene):
+    e = _cd(ene)
+    a = np.zeros(len(e), np.int32)
+    b = np.zeros(len(e), np.int32)
+    n = lib().orc_chkdgn(_p(e), len(e), _ip_(a), _ip_(b))
+    return a[:n], b[:n]
+
+
+def tetwt(mesh, ek1, ek2, ef, frhis_, ebmx=1e10):
+    """gettetwt for npm=1: returns a PairWeights-like dict of the reference's
+    index sets (1-based values, per k) and whw.  mesh: synth.TetraMesh."""
+    ek1 = _cd(ek1)
+    ek2 = _cd(ek2)
+    nband, nqbz = ek1.shape
+    fr = _cd(frhis_)
+    nwhis = len(fr) - 1
+    idt = np.asfortranarray(mesh.idtetf, np.int32)
+    qw = np.asfortranarray(mesh.qbzw, np.float64)
+    ib1 = np.ascontiguousarray(mesh.ib1bz, np.int32)
+    ql = np.asfortranarray(mesh.qlat, np.float64)
+    cap_b, cap_h = 1, 1
+    for _ in range(3):  # sizes: nbnbx first, then nhwtot
+        nbnb = np.zeros(nqbz, np.int32)
+        arrs = [np.zeros((cap_b, nqbz), np.int32, order="F") for _ in range(5)]
+        whw = np.zeros(cap_h)
+        nbx, nht = C.c_int(0), C.c_int(0)
+        rc = lib().orc_tetwt(nband, nqbz, idt.shape[1], _ip_(idt), _p(qw), _ip_(ib1), _p(ek1), _p(ek2),
+                             C.c_double(ef), C.c_double(ebmx), _p(ql), _p(fr), nwhis, cap_b, cap_h, _ip_(nbnb),
+                             *[_ip_(a) for a in arrs], _p(whw), C.byref(nbx), C.byref(nht))
+        if rc == -1:
+            cap_b, cap_h = max(nbx.value, 1), max(nht.value, 1)
+            continue
+        if rc != 0:
+            raise ValueError(f"orc_tetwt rc={rc}")
+        n1b, n2b, ihw, nhw, jhw = arrs
+        return dict(nbnb=nbnb, n1b=n1b, n2b=n2b, ihw=ihw, nhw=nhw, jhw=jhw, whw=whw[:nht.value],
+                    nbnbx=nbx.value, nhwtot=nht.value)
+    raise RuntimeError("orc_tetwt: capacity loop did not converge")
+
+
 def rmatt(freqr, frhis_):
     """dpsion_init real branch (npm=1): (nw+1) x nwhis, freqr = freq_r(0:nw)."""
     freqr = _cd(freqr)

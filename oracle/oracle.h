@@ -30,6 +30,10 @@
  *   orc_wgtim          scipy.integrate.quad of the smeared-Lorentzian integral
  *   orc_sigc_imag      numpy GEMM chain; Im(Sigma_nn) clip
  *   orc_qp_energy      closed form of S/main_hqpe.sc.f90:160-161
+ *   orc_lindtet6       Monte Carlo of f(ea)(1-f(eb)) delta(omega+x) over the
+ *                      tetrahedron, every occupied/unoccupied cut figure
+ *   orc_tetwt          brute-force pair sets; degenerate-band equality;
+ *                      Monte Carlo of the whole-mesh histogram
  */
 #ifndef ECALJ_ORACLE_H
 #define ECALJ_ORACLE_H
@@ -160,6 +164,24 @@ void orc_qp_energy(int ntq, const double* elda, const double* sex, const double*
  * se = sex2 + sexcore2 + (sec + sec^H)/2 - vxcmat, all ntq x ntq. */
 void orc_sigma_herm(int ntq, const double complex* sex2, const double complex* sexcore2,
                     const double complex* sec, const double complex* vxcmat, double complex* se);
+
+/* Tetrahedron histogram weights (oracle/tetwt.c): gettetwt + tetwt5x_dtet4
+ * (S/m_tetwt.f90:36-256, S/tetwt5.f90:9-486) for npm=1, no core states,
+ * matrix_linear=F.  idtetf: 4 x ntetf 1-based indices into qbzw (3 x nqbzw);
+ * ib1bz: nqbzw 1-based k indices; ek1/ek2: nband x nqbz energies (Ry) at k
+ * and k+q; qlat columns = reciprocal vectors.  Outputs with leading dimension
+ * nbnbx_cap: n1b, n2b, ihw, nhw, jhw (1-based values, Fortran layout); whw
+ * (nhw_cap).  Returns 0; -1 if a capacity is short (nbnbx_out / nhwtot_out
+ * hold the sizes needed); < -1 on the reference's own error exits. */
+int orc_tetwt(int nband, int nqbz, int ntetf, const int* idtetf, const double* qbzw, const int* ib1bz,
+              const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
+              const double* frhis, int nwhis, int nbnbx_cap, int nhw_cap, int* nbnb, int* n1b, int* n2b,
+              int* ihw, int* nhw, int* jhw, double* whw, int* nbnbx_out, int* nhwtot_out);
+/* lindtet6 (S/tetwt5.f90:514-600): histogram of one tetrahedron; kvec[corner][xyz] */
+int orc_lindtet6(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
+                 double efermia, double efermib, const double* frhis, int nwhis, double* wtthis);
+int orc_hisrange(const double* frhis, int nwhis, double demin, double demax, int* ihw, int* nhw);
+int orc_chkdgn(const double* ene, int ndat, int* ixini, int* ixend);
 
 /* Learned chi0-tail MLP (oracle/mlp.c): L layers of widths dims[0..L],
  * silu between layers, linear output; params W_l (row-major out x in) then
