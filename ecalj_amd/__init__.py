@@ -68,6 +68,8 @@ _SIGS = {
     "ecalj_dpsion_real": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
     "ecalj_dpsion_real_dev": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
     "ecalj_zminv": [_p, _p, _i, _i],
+    "ecalj_tetwt": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _l, _p, _p, _p, _p, _p, _p,
+                    _p, _p, _p],
     "ecalj_zminv_dev": [_p, _p, _i, _i, _i, C.c_int64],
     "ecalj_sigc_weights": [_p, _i, _i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p, _p, _p],
     "ecalj_sigc_weights_dev": [_p, _i, _i, _i, _i, _i, _i, _p, _p, _d, _d, _p, _p, _p, _p, _p],
@@ -472,6 +474,33 @@ class Handle:
         _chk(lib().ecalj_wc_build_real_dev(self._h, _t_ptr(chi), ngb, nw, nspin, int(is_x0_m_basis),
                                            int(is_wc_m_basis), _t_ptr(wc), ngb if ldw is None else ldw),
              "wc_build_real_dev")
+
+    def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10):
+        """gettetwt (S/m_tetwt.f90:36-256), npm=1: pair lists and histogram weights.
+        mesh: synth.TetraMesh (or any object with idtetf, qbzw, ib1bz, qlat)."""
+        e1 = np.asfortranarray(ek1, np.float64)
+        e2 = np.asfortranarray(ek2, np.float64)
+        nband, nqbz = e1.shape
+        idt = np.asfortranarray(mesh.idtetf, np.int32)
+        qw = np.asfortranarray(mesh.qbzw, np.float64)
+        ib1 = np.ascontiguousarray(mesh.ib1bz, np.int32)
+        ql = np.asfortranarray(mesh.qlat, np.float64)
+        fr = np.ascontiguousarray(frhis, np.float64)
+        nbx, nht = C.c_int(0), C.c_int64(0)
+        args = (self._h, nband, nqbz, idt.shape[1], _np_ptr(idt), qw.shape[1], _np_ptr(qw), _np_ptr(ib1),
+                _np_ptr(e1), _np_ptr(e2), ef, ebmx, _np_ptr(ql), _np_ptr(fr), len(fr) - 1)
+        rc = lib().ecalj_tetwt(*args, 0, 0, None, None, None, None, None, None, None, C.byref(nbx), C.byref(nht))
+        if rc not in (0, 1):
+            _chk(rc, "tetwt")
+        cap = max(nbx.value, 1)
+        nbnb = np.zeros(nqbz, np.int32)
+        arrs = [np.zeros((cap, nqbz), np.int32, order="F") for _ in range(5)]
+        whw = np.zeros(max(nht.value, 1))
+        _chk(lib().ecalj_tetwt(*args, cap, len(whw), _np_ptr(nbnb), *[_np_ptr(a) for a in arrs], _np_ptr(whw),
+                               C.byref(nbx), C.byref(nht)), "tetwt")
+        n1b, n2b, ihw, nhw, jhw = arrs
+        return dict(nbnb=nbnb, n1b=n1b, n2b=n2b, ihw=ihw, nhw=nhw, jhw=jhw, whw=whw[:nht.value],
+                    nbnbx=nbx.value, nhwtot=nht.value)
 
     def zminv(self, a):
         """zminv (S/m_lapack.f90:21-47): returns inv(a), partial pivoting."""

@@ -1,0 +1,716 @@
+// tetwt.hip -- tetrahedron histogram weights of chi0 (gettetwt +
+// tetwt5x_dtet4, S/m_tetwt.f90:36-256, S/tetwt5.f90:9-486) for npm = 1, no
+// core states, tetrahedron_matrix_linear = F.
+//
+// Device work, in the reference's two passes:
+//   job 0  tw_job0_kernel: one thread per (tetrahedron, band pair) marks the
+//          pairs of each corner k and widens their excitation window
+//          [demin, demax] (atomic min / max: exact, so the index maps are
+//          bit-identical to a sequential loop); tw_sym0_kernel merges the
+//          windows of degenerate bands; tw_count_kernel / tw_lists_kernel
+//          build the per-k pair lists (ib-major, rsvwwk00_4's order), the
+//          bin windows (hisrange) and the whw pointers (block scans).
+//   job 1  tw_job1_kernel: one thread per (tetrahedron, pair) cuts the
+//          tetrahedron at E_F for the occupied band, each piece at E_F for
+//          the unoccupied band (Rath & Freeman), integrates the distribution
+//          of the excitation energy over every bin, and adds it to the pair's
+//          histogram at the four corners (FP64 atomics); tw_sym1_kernel
+//          averages degenerate bands.
+// The host keeps what the reference computes once per call outside the
+// loops: per-tetrahedron band counts and the degenerate groups (chkdgn).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+#include "ecalj_gpu.h"
+
+namespace ecalj {
+namespace {
+
+struct TwBuf {
+  void* p = nullptr;
+  ~TwBuf() {
+    if (p) (void)hipFree(p);
+  }
+  ecalj_status alloc(size_t bytes) {
+    if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return ECALJ_ERR_NOMEM;
+    return ECALJ_OK;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+__device__ inline size_t idx3(int ib, int jb, int k, int nband) {
+  return (size_t)ib + (size_t)nband * ((size_t)jb + (size_t)nband * (size_t)k);
+}
+
+__device__ inline void atomic_min_d(double* a, double v) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(a);
+  unsigned long long old = *p;
+  while (v < __longlong_as_double((long long)old)) {
+    const unsigned long long prev = atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+__device__ inline void atomic_max_d(double* a, double v) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(a);
+  unsigned long long old = *p;
+  while (v > __longlong_as_double((long long)old)) {
+    const unsigned long long prev = atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+// ---------------------------------------------------------------- geometry
+__device__ inline void sort4(const double* e, int* ord) {  // stable insertion sort, ascending (sortea)
+  for (int i = 0; i < 4; ++i) ord[i] = i;
+  for (int ix = 1; ix < 4; ++ix)
+    for (int i = ix; i >= 1; --i) {
+      if (e[ord[i - 1]] > e[ord[i]]) {
+        const int t = ord[i - 1];
+        ord[i - 1] = ord[i];
+        ord[i] = t;
+        continue;
+      }
+      break;
+    }
+}
+
+__device__ inline double det3(const double a[3][3]) {  // a[col][row]
+  return a[0][0] * a[1][1] * a[2][2] - a[0][0] * a[1][2] * a[2][1] - a[0][1] * a[1][0] * a[2][2] +
+         a[0][1] * a[1][2] * a[2][0] + a[0][2] * a[1][0] * a[2][1] - a[0][2] * a[1][1] * a[2][0];
+}
+
+// integtetn (S/m_tetrakbt.f90:441-466); e ascending.  false on ee > e4.
+__device__ inline bool integtetn(const double* e, double ee, double& out) {
+  const double e1 = e[0] - 3e-8, e2 = e[1] - 2e-8, e3 = e[2] - 1e-8, e4 = e[3];
+  const double V1 = ee - e1, V2 = ee - e2, V4 = ee - e4;
+  if (ee < e1) {
+    out = 0.0;
+  } else if (e1 <= ee && ee < e2) {
+    out = V1 * V1 * V1 / ((e4 - e1) * (e3 - e1) * (e2 - e1));
+  } else if (e2 <= ee && ee < e3) {
+    const double a = V1 / pow((e4 - e1) * (e3 - e1) * (e2 - e1), 1.0 / 3.0);
+    const double b = V2 / pow(-(e4 - e2) * (e3 - e2) * (e1 - e2), 1.0 / 3.0);
+    out = (a - b) * (a * a + a * b + b * b);
+  } else if (e3 <= ee && ee < e4) {
+    out = 1.0 - V4 * V4 * V4 / ((e1 - e4) * (e2 - e4) * (e3 - e4));
+  } else if (ee == e4) {
+    out = 1.0;
+  } else {
+    return false;
+  }
+  return true;
+}
+
+// The four corners' histogram windows of this pair; weights go straight to whw.
+struct Sink {
+  int ihw[4], nhw[4], jhw[4];
+  double scale;  // pi / (4 vol_BZ) * 4 * (1/4)
+  double* whw;
+  int* err;
+  __device__ void add(int ihis, double w) const {
+    const double v = w * scale;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (ihis >= ihw[c] && ihis < ihw[c] + nhw[c]) atomicAdd(whw + (jhw[c] - 1) + (ihis - ihw[c]), v);
+  }
+};
+
+// intttvc6 with matrix_linear = F (S/tetwt5.f90:714-787)
+__device__ void intttvc6(const double v[4], double voltet, const double* frhis, int nwhis, const Sink& s) {
+  double mv[4], WW[4];
+  int ord[4];
+  for (int i = 0; i < 4; ++i) mv[i] = -v[i];
+  sort4(mv, ord);
+  for (int i = 0; i < 4; ++i) WW[i] = -v[ord[i]];
+  int inihis = -999, iedhis = -999, ix = 1;
+  for (int ihis = 1; ihis <= nwhis; ++ihis) {
+    if (ix == 1 && WW[0] < frhis[ihis]) {
+      inihis = ihis;
+      ix = 4;
+    }
+    if (ix == 4 && WW[3] < frhis[ihis]) {
+      iedhis = ihis;
+      break;
+    }
+  }
+  if (iedhis == -999 || inihis == -999) {
+    *s.err = 3;
+    return;
+  }
+  double norm;
+  if (!integtetn(WW, WW[3], norm)) {
+    *s.err = 4;
+    return;
+  }
+  const double pvn = voltet / norm;
+  double intega = 0.0, integb;
+  for (int ihis = inihis; ihis <= iedhis; ++ihis) {
+    if (frhis[ihis] > WW[3]) {
+      integb = norm;
+    } else if (!integtetn(WW, frhis[ihis], integb)) {
+      *s.err = 4;
+      return;
+    }
+    s.add(ihis, pvn * (integb - intega));
+    intega = integb;
+  }
+}
+
+__device__ inline void inttetrac6(const double kk[8][3], const double xx[8], int t0, int t1, int t2, int t3,
+                                  const double* frhis, int nwhis, const Sink& s) {
+  const int it[4] = {t0 - 1, t1 - 1, t2 - 1, t3 - 1};
+  double xin[4], am[3][3];
+  for (int i = 0; i < 4; ++i) xin[i] = xx[it[i]];
+  for (int i = 0; i < 3; ++i)
+    for (int d = 0; d < 3; ++d) am[i][d] = kk[it[i]][d] - kk[it[3]][d];
+  intttvc6(xin, fabs(det3(am) / 6.0), frhis, nwhis, s);
+}
+
+__device__ inline void midk(double kk[8][3], const double* ee, double* xx, int i, int j, int out) {
+  const double ratio = ee[i - 1] / (ee[i - 1] - ee[j - 1]);
+  xx[out - 1] = xx[i - 1] + ratio * (xx[j - 1] - xx[i - 1]);
+  for (int d = 0; d < 3; ++d) kk[out - 1][d] = kk[i - 1][d] + ratio * (kk[j - 1][d] - kk[i - 1][d]);
+}
+
+__device__ inline void midk3(double kk[8][3], const double* ee, double* xx, double* yy, int i, int j, int out) {
+  const double ratio = ee[i - 1] / (ee[i - 1] - ee[j - 1]);
+  xx[out - 1] = xx[i - 1] + ratio * (xx[j - 1] - xx[i - 1]);
+  yy[out - 1] = yy[i - 1] + ratio * (yy[j - 1] - yy[i - 1]);
+  for (int d = 0; d < 3; ++d) kk[out - 1][d] = kk[i - 1][d] + ratio * (kk[j - 1][d] - kk[i - 1][d]);
+}
+
+// inttetra6 (S/tetwt5.f90:601-672): the unoccupied cut of one occupied piece
+__device__ void inttetra6(const double kk_[8][3], const double xx_[8], const double ebf[8], int t0, int t1, int t2,
+                          int t3, const double* frhis, int nwhis, const Sink& s) {
+  const int it[4] = {t0 - 1, t1 - 1, t2 - 1, t3 - 1};
+  double ebfin[4], kk[8][3], xx[8], ee[4];
+  int ord[4];
+  for (int i = 0; i < 4; ++i) ebfin[i] = ebf[it[i]];
+  sort4(ebfin, ord);
+  for (int i = 0; i < 4; ++i) {
+    ee[i] = ebfin[ord[i]];
+    xx[i] = xx_[it[ord[i]]];
+    for (int d = 0; d < 3; ++d) kk[i][d] = kk_[it[ord[i]]][d];
+  }
+  if (0.0 >= ee[3]) return;
+  if (ee[3] > 0.0 && 0.0 >= ee[2]) {
+    midk(kk, ee, xx, 4, 2, 5);
+    midk(kk, ee, xx, 4, 1, 6);
+    midk(kk, ee, xx, 4, 3, 7);
+    inttetrac6(kk, xx, 4, 5, 6, 7, frhis, nwhis, s);
+  } else if (ee[2] > 0.0 && 0.0 >= ee[1]) {
+    midk(kk, ee, xx, 4, 2, 5);
+    midk(kk, ee, xx, 4, 1, 6);
+    midk(kk, ee, xx, 3, 1, 7);
+    midk(kk, ee, xx, 3, 2, 8);
+    inttetrac6(kk, xx, 4, 3, 5, 6, frhis, nwhis, s);
+    inttetrac6(kk, xx, 3, 6, 7, 5, frhis, nwhis, s);
+    inttetrac6(kk, xx, 3, 5, 7, 8, frhis, nwhis, s);
+  } else if (ee[1] > 0.0 && 0.0 >= ee[0]) {
+    midk(kk, ee, xx, 1, 4, 5);
+    midk(kk, ee, xx, 1, 2, 6);
+    midk(kk, ee, xx, 1, 3, 7);
+    inttetrac6(kk, xx, 3, 4, 7, 2, frhis, nwhis, s);
+    inttetrac6(kk, xx, 4, 5, 6, 7, frhis, nwhis, s);
+    inttetrac6(kk, xx, 4, 2, 6, 7, frhis, nwhis, s);
+  } else {
+    inttetrac6(kk, xx, 1, 2, 3, 4, frhis, nwhis, s);
+  }
+}
+
+// lindtet6 (S/tetwt5.f90:514-600): the occupied cut (ea < E_F)
+__device__ void lindtet6(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
+                         double ef, const double* frhis, int nwhis, const Sink& s) {
+  int ord[4];
+  sort4(ea, ord);
+  const int r[4] = {ord[3], ord[2], ord[1], ord[0]};
+  double kk[8][3], xx[8], ee[4], ebf[8];
+  for (int i = 0; i < 4; ++i) {
+    for (int d = 0; d < 3; ++d) kk[i][d] = kvec[r[i]][d];
+    ee[i] = ea[r[i]] - ef;
+    xx[i] = x[r[i]];
+    ebf[i] = eb[r[i]] - ef;
+  }
+  if (0.0 <= ee[3]) return;
+  if (ee[3] < 0.0 && 0.0 <= ee[2]) {
+    midk3(kk, ee, xx, ebf, 4, 2, 5);
+    midk3(kk, ee, xx, ebf, 4, 1, 6);
+    midk3(kk, ee, xx, ebf, 4, 3, 7);
+    inttetra6(kk, xx, ebf, 4, 5, 6, 7, frhis, nwhis, s);
+  } else if (ee[2] < 0.0 && 0.0 <= ee[1]) {
+    midk3(kk, ee, xx, ebf, 4, 2, 5);
+    midk3(kk, ee, xx, ebf, 4, 1, 6);
+    midk3(kk, ee, xx, ebf, 3, 1, 7);
+    midk3(kk, ee, xx, ebf, 3, 2, 8);
+    inttetra6(kk, xx, ebf, 4, 3, 5, 6, frhis, nwhis, s);
+    inttetra6(kk, xx, ebf, 3, 6, 7, 5, frhis, nwhis, s);
+    inttetra6(kk, xx, ebf, 3, 5, 7, 8, frhis, nwhis, s);
+  } else if (ee[1] < 0.0 && 0.0 <= ee[0]) {
+    midk3(kk, ee, xx, ebf, 1, 4, 5);
+    midk3(kk, ee, xx, ebf, 1, 2, 6);
+    midk3(kk, ee, xx, ebf, 1, 3, 7);
+    inttetra6(kk, xx, ebf, 3, 4, 7, 2, frhis, nwhis, s);
+    inttetra6(kk, xx, ebf, 4, 5, 6, 7, frhis, nwhis, s);
+    inttetra6(kk, xx, ebf, 4, 2, 6, 7, frhis, nwhis, s);
+  } else {
+    inttetra6(kk, xx, ebf, 1, 2, 3, 4, frhis, nwhis, s);
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+struct TetRow {
+  int k[4];   // corner k-points, 0-based
+  int w[4];   // corner indices into qbzw, 0-based
+  int nocc;   // pairs: ib < nocc, jb < nbmx
+  int nbmx;
+};
+
+// One (tetrahedron, pair) per thread; the checks of S/tetwt5.f90:252-288.
+__device__ inline bool tw_pair(const TetRow& t, int p, const double* ek1, const double* ek2, int nband, double ef,
+                               int& ib, int& jb, double eo[4], double eu[4]) {
+  if (p >= t.nocc * t.nbmx) return false;
+  ib = p / t.nbmx;
+  jb = p % t.nbmx;
+  double mino = 1e300, maxu = -1e300, maxd = -1e300;
+  for (int c = 0; c < 4; ++c) {
+    eo[c] = ek1[ib + (size_t)nband * t.k[c]];
+    eu[c] = ek2[jb + (size_t)nband * t.k[c]];
+    mino = fmin(mino, eo[c]);
+    maxu = fmax(maxu, eu[c]);
+    maxd = fmax(maxd, eu[c] - eo[c]);
+  }
+  if (!(mino <= ef && maxu >= ef)) return false;
+  return !(maxd < 0.0);
+}
+
+__global__ __launch_bounds__(256) void tw_job0_kernel(const TetRow* __restrict__ tets, const double* __restrict__ ek1,
+                                                      const double* __restrict__ ek2, int nband, double ef,
+                                                      unsigned char* __restrict__ iwgt, double* __restrict__ demin,
+                                                      double* __restrict__ demax) {
+  const TetRow t = tets[blockIdx.y];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int ib, jb;
+  double eo[4], eu[4];
+  if (!tw_pair(t, p, ek1, ek2, nband, ef, ib, jb, eo, eu)) return;
+  double dmx = -0.5 * (eo[0] - eu[0]), dmn = dmx;
+  for (int c = 1; c < 4; ++c) {
+    const double v = -0.5 * (eo[c] - eu[c]);
+    dmx = fmax(dmx, v);
+    dmn = fmin(dmn, v);
+  }
+  for (int c = 0; c < 4; ++c) {
+    const size_t e = idx3(ib, jb, t.k[c], nband);
+    iwgt[e] = 1;
+    atomic_max_d(demax + e, dmx);
+    atomic_min_d(demin + e, dmn);
+  }
+}
+
+struct DgBlock {
+  int k, a0, a1, b0, b1;  // 0-based inclusive band ranges at k
+};
+
+__global__ void tw_sym0_kernel(const DgBlock* __restrict__ blk, int nblk, int nband, unsigned char* __restrict__ iwgt,
+                               double* __restrict__ demin, double* __restrict__ demax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nblk) return;
+  const DgBlock b = blk[i];
+  unsigned char w = 0;
+  double mx = -1e300, mn = 1e300;
+  for (int a = b.a0; a <= b.a1; ++a)
+    for (int c = b.b0; c <= b.b1; ++c) {
+      const size_t e = idx3(a, c, b.k, nband);
+      w |= iwgt[e];
+      mx = fmax(mx, demax[e]);
+      mn = fmin(mn, demin[e]);
+    }
+  for (int a = b.a0; a <= b.a1; ++a)
+    for (int c = b.b0; c <= b.b1; ++c) {
+      const size_t e = idx3(a, c, b.k, nband);
+      iwgt[e] = w;
+      demax[e] = mx;
+      demin[e] = mn;
+    }
+}
+
+__global__ void tw_fill_kernel(double* __restrict__ a, double v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+// block-wide exclusive scan of ints (blockDim.x = 1024 = 16 waves)
+__device__ inline int block_exscan(int v, int* total, int* sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      const int t = sh[w];
+      sh[w] = s;
+      s += t;
+    }
+    sh[16] = s;
+  }
+  __syncthreads();
+  const int r = sh[wave] + x - v;
+  *total = sh[16];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(1024) void tw_count_kernel(const unsigned char* __restrict__ iwgt, int nband,
+                                                        int* __restrict__ nbnb) {
+  __shared__ int sh[17];
+  const int k = blockIdx.x;
+  int c = 0;
+  const size_t nn = (size_t)nband * nband;
+  for (size_t q = threadIdx.x; q < nn; q += blockDim.x) c += iwgt[q + nn * k];
+  int tot;
+  block_exscan(c, &tot, sh);
+  if (threadIdx.x == 0) nbnb[k] = tot;
+}
+
+// Pair lists of k in ib-major order (rsvwwk00_4), bin windows (hisrange), the
+// pair's offset within k's part of whw, ibjb, and k's total bin count.
+__global__ __launch_bounds__(1024) void tw_lists_kernel(const unsigned char* __restrict__ iwgt,
+                                                        const double* __restrict__ demin,
+                                                        const double* __restrict__ demax, int nband, int ld,
+                                                        const double* __restrict__ frhis, int nwhis,
+                                                        int* __restrict__ n1b, int* __restrict__ n2b,
+                                                        int* __restrict__ ihw, int* __restrict__ nhw,
+                                                        int* __restrict__ jloc, int* __restrict__ ibjb,
+                                                        int* __restrict__ ktot, int* __restrict__ err) {
+  __shared__ int sh[17];
+  const int k = blockIdx.x;
+  int base = 0, hbase = 0;
+  const int nn = nband * nband;
+  for (int q0 = 0; q0 < nn; q0 += blockDim.x) {
+    const int q = q0 + threadIdx.x;
+    int f = 0, ih = 0, nh = 0, ib = 0, jb = 0;
+    if (q < nn) {
+      ib = q / nband;
+      jb = q % nband;
+      const size_t e = idx3(ib, jb, k, nband);
+      f = iwgt[e];
+      if (f) {  // hisrange (S/tetwt5.f90:487-513)
+        const double lo = demin[e], hi = demax[e];
+        ih = -9999;
+        for (int i = 1; i <= nwhis; ++i)
+          if (lo < frhis[i]) {
+            ih = i;
+            break;
+          }
+        nh = -9999;
+        if (ih > 0)
+          for (int i = ih; i <= nwhis; ++i)
+            if (hi < frhis[i]) {
+              nh = i - ih + 1;
+              break;
+            }
+        if (ih < 0 || nh < 0) {
+          *err = 2;
+          nh = 0;
+        }
+      }
+    }
+    int tf, th;
+    const int pos = block_exscan(f, &tf, sh);
+    const int hpos = block_exscan(f ? nh : 0, &th, sh);
+    if (f) {
+      const size_t o = (size_t)(base + pos) + (size_t)ld * k;
+      n1b[o] = ib + 1;
+      n2b[o] = jb + 1;
+      ihw[o] = ih;
+      nhw[o] = nh;
+      jloc[o] = hbase + hpos;
+      ibjb[idx3(ib, jb, k, nband)] = base + pos + 1;
+    }
+    base += tf;
+    hbase += th;
+  }
+  if (threadIdx.x == 0) ktot[k] = hbase;
+}
+
+__global__ void tw_jhw_kernel(const int* __restrict__ nbnb, const long long* __restrict__ kbase, int ld, int nqbz,
+                              const int* __restrict__ jloc, int* __restrict__ jhw) {
+  const size_t tot = (size_t)ld * nqbz;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e / ld), i = (int)(e % ld);
+    if (i < nbnb[k]) jhw[e] = (int)(kbase[k] + jloc[e] + 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void tw_job1_kernel(const TetRow* __restrict__ tets, const double* __restrict__ qbzw,
+                                                      const double* __restrict__ ek1, const double* __restrict__ ek2,
+                                                      int nband, double ef, const int* __restrict__ ibjb, int ld,
+                                                      const int* __restrict__ ihw, const int* __restrict__ nhw,
+                                                      const int* __restrict__ jhw, const double* __restrict__ frhis,
+                                                      int nwhis, double scale, double* __restrict__ whw,
+                                                      int* __restrict__ err) {
+  const TetRow t = tets[blockIdx.y];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int ib, jb;
+  double eo[4], eu[4];
+  if (!tw_pair(t, p, ek1, ek2, nband, ef, ib, jb, eo, eu)) return;
+  Sink s;
+  s.scale = scale;
+  s.whw = whw;
+  s.err = err;
+  double kv[4][3], x[4];
+  for (int c = 0; c < 4; ++c) {
+    const size_t o = (size_t)(ibjb[idx3(ib, jb, t.k[c], nband)] - 1) + (size_t)ld * t.k[c];
+    s.ihw[c] = ihw[o];
+    s.nhw[c] = nhw[o];
+    s.jhw[c] = jhw[o];
+    for (int d = 0; d < 3; ++d) kv[c][d] = qbzw[d + 3 * (size_t)t.w[c]];
+    x[c] = 0.5 * (eo[c] - eu[c]);
+  }
+  lindtet6(kv, eo, eu, x, ef, frhis, nwhis, s);
+}
+
+// degenerate bands: every pair of a block takes the block's average histogram
+__global__ void tw_sym1_kernel(const DgBlock* __restrict__ blk, int nblk, int nband, const int* __restrict__ ibjb,
+                               int ld, const int* __restrict__ nhw, const int* __restrict__ jhw,
+                               double* __restrict__ whw, int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nblk) return;
+  const DgBlock b = blk[i];
+  const int bs = (b.a1 - b.a0 + 1) * (b.b1 - b.b0 + 1);
+  int isum = 0;
+  for (int a = b.a0; a <= b.a1; ++a)
+    for (int c = b.b0; c <= b.b1; ++c) isum += ibjb[idx3(a, c, b.k, nband)] != 0;
+  if (isum == 0) return;
+  if (isum != bs) {
+    *err = 5;
+    return;
+  }
+  const size_t o0 = (size_t)(ibjb[idx3(b.a0, b.b0, b.k, nband)] - 1) + (size_t)ld * b.k;
+  const int nh = nhw[o0];
+  for (int a = b.a0; a <= b.a1; ++a)  // one window for the block (its windows were merged in job 0)
+    for (int c = b.b0; c <= b.b1; ++c)
+      if (nhw[(size_t)(ibjb[idx3(a, c, b.k, nband)] - 1) + (size_t)ld * b.k] != nh) {
+        *err = 5;
+        return;
+      }
+  for (int h = 0; h < nh; ++h) {
+    double sum = 0.0;
+    for (int a = b.a0; a <= b.a1; ++a)
+      for (int c = b.b0; c <= b.b1; ++c) {
+        const size_t o = (size_t)(ibjb[idx3(a, c, b.k, nband)] - 1) + (size_t)ld * b.k;
+        sum += whw[jhw[o] - 1 + h];
+      }
+    const double avg = sum / bs;
+    for (int a = b.a0; a <= b.a1; ++a)
+      for (int c = b.b0; c <= b.b1; ++c) {
+        const size_t o = (size_t)(ibjb[idx3(a, c, b.k, nband)] - 1) + (size_t)ld * b.k;
+        whw[jhw[o] - 1 + h] = avg;
+      }
+  }
+}
+
+// chkdgn (S/tetwt5.f90:868-912): 0-based inclusive ranges
+int chkdgn(const double* e, int n, std::vector<int>& lo, std::vector<int>& hi) {
+  lo.assign(n, 0);
+  hi.assign(n, 0);
+  if (n < 1) return 0;
+  lo[0] = 0;
+  if (n == 1) {
+    hi[0] = 0;
+    return 1;
+  }
+  int i = 0;
+  for (int ix = 1; ix < n; ++ix) {
+    if (std::fabs(e[ix] - e[ix - 1]) > 1e-4) {
+      hi[i] = ix - 1;
+      ++i;
+      lo[i] = hi[i - 1] + 1;
+      if (ix == n - 1) hi[i] = ix;
+    } else if (ix == n - 1) {
+      hi[i] = n - 1;
+    }
+  }
+  return i + 1;
+}
+
+}  // namespace
+}  // namespace ecalj
+
+using namespace ecalj;
+
+extern "C" ecalj_status ecalj_tetwt(ecalj_handle h, int nband, int nqbz, int ntetf, const int* idtetf, int nqbzw,
+                                    const double* qbzw, const int* ib1bz, const double* ekxx1, const double* ekxx2,
+                                    double ef, double ebmx, const double* qlat, const double* frhis, int nwhis,
+                                    int nbnbx_cap, int64_t nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw,
+                                    int* nhw, int* jhw, double* whw, int* nbnbx_out, int64_t* nhwtot_out) {
+  if (!h || !idtetf || !qbzw || !ib1bz || !ekxx1 || !ekxx2 || !qlat || !frhis || !nbnbx_out || !nhwtot_out)
+    return ECALJ_ERR_ARG;
+  if (nband < 1 || nqbz < 1 || ntetf < 1 || nqbzw < 1 || nwhis < 1) return ECALJ_ERR_DIM;
+  if (std::fabs(frhis[0]) > 1e-12) return ECALJ_ERR_ARG;  // S/m_tetwt.f90:176
+  if (h->capturing) return ECALJ_ERR_STATE;  // synchronises with the host between its passes
+  ECALJ_HIP(hipSetDevice(h->device));
+  hipStream_t st = h->stream;
+  // per-tetrahedron rows: corners and band counts (S/tetwt5.f90:223-238)
+  std::vector<TetRow> rows(ntetf);
+  int maxp = 0;
+  const double ecut = std::min(1e10, ebmx);
+  for (int t = 0; t < ntetf; ++t) {
+    TetRow& r = rows[t];
+    r.nocc = 0;
+    r.nbmx = nband;
+    for (int c = 0; c < 4; ++c) {
+      const int w = idtetf[c + 4 * (size_t)t] - 1;
+      if (w < 0 || w >= nqbzw) return ECALJ_ERR_ARG;
+      r.w[c] = w;
+      r.k[c] = ib1bz[w] - 1;
+      if (r.k[c] < 0 || r.k[c] >= nqbz) return ECALJ_ERR_ARG;
+      int n1 = 0, n2 = 0;
+      for (int b = 0; b < nband; ++b) {
+        n1 += ekxx1[b + (size_t)nband * r.k[c]] < ef;
+        n2 += ekxx2[b + (size_t)nband * r.k[c]] < ecut;
+      }
+      r.nocc = std::max(r.nocc, n1);
+      r.nbmx = std::min(r.nbmx, n2);
+    }
+    maxp = std::max(maxp, r.nocc * r.nbmx);
+  }
+  // degenerate blocks with more than one pair (singletons need no merging)
+  std::vector<DgBlock> blocks;
+  {
+    std::vector<int> l1, h1, l2, h2;
+    for (int k = 0; k < nqbz; ++k) {
+      const int r1 = chkdgn(ekxx1 + (size_t)nband * k, nband, l1, h1);
+      const int r2 = chkdgn(ekxx2 + (size_t)nband * k, nband, l2, h2);
+      for (int a = 0; a < r1; ++a)
+        for (int b = 0; b < r2; ++b)
+          if (h1[a] > l1[a] || h2[b] > l2[b]) blocks.push_back({k, l1[a], h1[a], l2[b], h2[b]});
+    }
+  }
+  const size_t nbb = (size_t)nband * nband * nqbz;
+  TwBuf d_rows, d_ek1, d_ek2, d_qw, d_fr, d_iw, d_dmn, d_dmx, d_blk, d_nbnb, d_err, d_ibjb;
+  ECALJ_TRY(d_rows.alloc(sizeof(TetRow) * ntetf));
+  ECALJ_TRY(d_ek1.alloc(sizeof(double) * nband * (size_t)nqbz));
+  ECALJ_TRY(d_ek2.alloc(sizeof(double) * nband * (size_t)nqbz));
+  ECALJ_TRY(d_qw.alloc(sizeof(double) * 3 * (size_t)nqbzw));
+  ECALJ_TRY(d_fr.alloc(sizeof(double) * (nwhis + 1)));
+  ECALJ_TRY(d_iw.alloc(nbb));
+  ECALJ_TRY(d_dmn.alloc(sizeof(double) * nbb));
+  ECALJ_TRY(d_dmx.alloc(sizeof(double) * nbb));
+  ECALJ_TRY(d_ibjb.alloc(sizeof(int) * nbb));
+  ECALJ_TRY(d_blk.alloc(sizeof(DgBlock) * std::max<size_t>(blocks.size(), 1)));
+  ECALJ_TRY(d_nbnb.alloc(sizeof(int) * nqbz));
+  ECALJ_TRY(d_err.alloc(sizeof(int)));
+  ECALJ_HIP(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(TetRow) * ntetf, hipMemcpyHostToDevice, st));
+  ECALJ_HIP(hipMemcpyAsync(d_ek1.p, ekxx1, sizeof(double) * nband * (size_t)nqbz, hipMemcpyHostToDevice, st));
+  ECALJ_HIP(hipMemcpyAsync(d_ek2.p, ekxx2, sizeof(double) * nband * (size_t)nqbz, hipMemcpyHostToDevice, st));
+  ECALJ_HIP(hipMemcpyAsync(d_qw.p, qbzw, sizeof(double) * 3 * (size_t)nqbzw, hipMemcpyHostToDevice, st));
+  ECALJ_HIP(hipMemcpyAsync(d_fr.p, frhis, sizeof(double) * (nwhis + 1), hipMemcpyHostToDevice, st));
+  if (!blocks.empty())
+    ECALJ_HIP(hipMemcpyAsync(d_blk.p, blocks.data(), sizeof(DgBlock) * blocks.size(), hipMemcpyHostToDevice, st));
+  ECALJ_HIP(hipMemsetAsync(d_iw.p, 0, nbb, st));
+  ECALJ_HIP(hipMemsetAsync(d_ibjb.p, 0, sizeof(int) * nbb, st));
+  ECALJ_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), st));
+  const unsigned fb = (unsigned)std::min<size_t>((nbb + 255) / 256, 65536);
+  hipLaunchKernelGGL(tw_fill_kernel, dim3(fb), dim3(256), 0, st, d_dmn.as<double>(), 1e10, nbb);
+  hipLaunchKernelGGL(tw_fill_kernel, dim3(fb), dim3(256), 0, st, d_dmx.as<double>(), -1e10, nbb);
+  const int nblk = (int)blocks.size();
+  // job 0
+  if (maxp > 0) {
+    hipLaunchKernelGGL(tw_job0_kernel, dim3((maxp + 255) / 256, ntetf), dim3(256), 0, st, d_rows.as<TetRow>(),
+                       d_ek1.as<double>(), d_ek2.as<double>(), nband, ef, d_iw.as<unsigned char>(),
+                       d_dmn.as<double>(), d_dmx.as<double>());
+    ECALJ_HIP(hipGetLastError());
+  }
+  if (nblk)
+    hipLaunchKernelGGL(tw_sym0_kernel, dim3((nblk + 255) / 256), dim3(256), 0, st, d_blk.as<DgBlock>(), nblk, nband,
+                       d_iw.as<unsigned char>(), d_dmn.as<double>(), d_dmx.as<double>());
+  hipLaunchKernelGGL(tw_count_kernel, dim3(nqbz), dim3(1024), 0, st, d_iw.as<unsigned char>(), nband,
+                     d_nbnb.as<int>());
+  ECALJ_HIP(hipGetLastError());
+  std::vector<int> hb(nqbz);
+  ECALJ_HIP(hipMemcpyAsync(hb.data(), d_nbnb.p, sizeof(int) * nqbz, hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipStreamSynchronize(st));
+  int nbnbx = 1;
+  for (int k = 0; k < nqbz; ++k) nbnbx = std::max(nbnbx, hb[k]);
+  *nbnbx_out = nbnbx;
+  const int ld = std::max(nbnbx, nbnbx_cap);
+  TwBuf d_n1b, d_n2b, d_ihw, d_nhw, d_jloc, d_jhw, d_ktot, d_kbase;
+  const size_t nl = (size_t)ld * nqbz;
+  for (TwBuf* b : {&d_n1b, &d_n2b, &d_ihw, &d_nhw, &d_jloc, &d_jhw}) ECALJ_TRY(b->alloc(sizeof(int) * nl));
+  ECALJ_TRY(d_ktot.alloc(sizeof(int) * nqbz));
+  ECALJ_TRY(d_kbase.alloc(sizeof(long long) * nqbz));
+  for (TwBuf* b : {&d_n1b, &d_n2b, &d_ihw, &d_nhw, &d_jhw}) ECALJ_HIP(hipMemsetAsync(b->p, 0, sizeof(int) * nl, st));
+  hipLaunchKernelGGL(tw_lists_kernel, dim3(nqbz), dim3(1024), 0, st, d_iw.as<unsigned char>(), d_dmn.as<double>(),
+                     d_dmx.as<double>(), nband, ld, d_fr.as<double>(), nwhis, d_n1b.as<int>(), d_n2b.as<int>(),
+                     d_ihw.as<int>(), d_nhw.as<int>(), d_jloc.as<int>(), d_ibjb.as<int>(), d_ktot.as<int>(),
+                     d_err.as<int>());
+  ECALJ_HIP(hipGetLastError());
+  std::vector<int> kt(nqbz);
+  int herr = 0;
+  ECALJ_HIP(hipMemcpyAsync(kt.data(), d_ktot.p, sizeof(int) * nqbz, hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipMemcpyAsync(&herr, d_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipStreamSynchronize(st));
+  if (herr) return ECALJ_ERR_DIM;  // a window outside the histogram mesh (hisrange)
+  std::vector<long long> kb(nqbz);
+  long long tot = 0;
+  for (int k = 0; k < nqbz; ++k) {
+    kb[k] = tot;
+    tot += kt[k];
+  }
+  *nhwtot_out = tot;
+  if (nbnbx > nbnbx_cap || tot > nhw_cap) return ECALJ_ERR_DIM;  // sizes reported; call again
+  if (!nbnb || !n1b || !n2b || !ihw || !nhw || !jhw || !whw) return ECALJ_ERR_ARG;
+  ECALJ_HIP(hipMemcpyAsync(d_kbase.p, kb.data(), sizeof(long long) * nqbz, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(tw_jhw_kernel, dim3((unsigned)std::min<size_t>((nl + 255) / 256, 65536)), dim3(256), 0, st,
+                     d_nbnb.as<int>(), d_kbase.as<long long>(), ld, nqbz, d_jloc.as<int>(), d_jhw.as<int>());
+  ECALJ_HIP(hipGetLastError());
+  // job 1
+  TwBuf d_whw;
+  ECALJ_TRY(d_whw.alloc(sizeof(double) * std::max<long long>(tot, 1)));
+  ECALJ_HIP(hipMemsetAsync(d_whw.p, 0, sizeof(double) * std::max<long long>(tot, 1), st));
+  double qb[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) qb[c][r] = qlat[r + 3 * c];
+  const double voltot = std::fabs(qb[0][0] * qb[1][1] * qb[2][2] - qb[0][0] * qb[1][2] * qb[2][1] -
+                                  qb[0][1] * qb[1][0] * qb[2][2] + qb[0][1] * qb[1][2] * qb[2][0] +
+                                  qb[0][2] * qb[1][0] * qb[2][1] - qb[0][2] * qb[1][1] * qb[2][0]);
+  const double pi = 3.14159265358979323846;
+  const double scale = pi / 4.0 / voltot * 4.0 * 0.25;  // piofvoltot * 4 * wtet (S/tetwt5.f90:342-343)
+  if (maxp > 0) {
+    hipLaunchKernelGGL(tw_job1_kernel, dim3((maxp + 255) / 256, ntetf), dim3(256), 0, st, d_rows.as<TetRow>(),
+                       d_qw.as<double>(), d_ek1.as<double>(), d_ek2.as<double>(), nband, ef, d_ibjb.as<int>(), ld,
+                       d_ihw.as<int>(), d_nhw.as<int>(), d_jhw.as<int>(), d_fr.as<double>(), nwhis, scale,
+                       d_whw.as<double>(), d_err.as<int>());
+    ECALJ_HIP(hipGetLastError());
+  }
+  if (nblk)
+    hipLaunchKernelGGL(tw_sym1_kernel, dim3((nblk + 255) / 256), dim3(256), 0, st, d_blk.as<DgBlock>(), nblk, nband,
+                       d_ibjb.as<int>(), ld, d_nhw.as<int>(), d_jhw.as<int>(), d_whw.as<double>(), d_err.as<int>());
+  ECALJ_HIP(hipGetLastError());
+  ECALJ_HIP(hipMemcpyAsync(nbnb, d_nbnb.p, sizeof(int) * nqbz, hipMemcpyDeviceToHost, st));
+  // outputs have leading dimension nbnbx_cap (= ld)
+  ECALJ_HIP(hipMemcpyAsync(n1b, d_n1b.p, sizeof(int) * nl, hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipMemcpyAsync(n2b, d_n2b.p, sizeof(int) * nl, hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipMemcpyAsync(ihw, d_ihw.p, sizeof(int) * nl, hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipMemcpyAsync(nhw, d_nhw.p, sizeof(int) * nl, hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipMemcpyAsync(jhw, d_jhw.p, sizeof(int) * nl, hipMemcpyDeviceToHost, st));
+  if (tot > 0) ECALJ_HIP(hipMemcpyAsync(whw, d_whw.p, sizeof(double) * tot, hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipMemcpyAsync(&herr, d_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  ECALJ_HIP(hipStreamSynchronize(st));
+  if (herr) return ECALJ_ERR_DIM;
+  return ECALJ_OK;
+}

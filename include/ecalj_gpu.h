@@ -178,6 +178,29 @@ ecalj_status ecalj_x0gemm_batched_dev(ecalj_handle h, void* rcxq, int npr, int i
                                       const int* jpmc, const int* icouini, int ncount,
                                       const double* whwc);
 
+/* ---- chi0: tetrahedron histogram weights --------------------------------
+ * gettetwt + tetwt5x_dtet4 (S/m_tetwt.f90:36-256, S/tetwt5.f90:9-486) for
+ * npm=1 (time reversal), no core states, tetrahedron_matrix_linear=F: the
+ * (occupied n at k, unoccupied m at k+q) pairs with tetrahedron weight and
+ * their histogram weights whw on the bins frhis(1:nwhis+1) (Hartree), the
+ * input of x0kf_v4hz_init.  idtetf: 4 x ntetf 1-based indices into qbzw
+ * (3 x nqbzw); ib1bz: nqbzw 1-based k indices; ekxx1, ekxx2: nband x nqbz
+ * energies (Ry) at k and k+q; ef (Ry); ebmx: band cutoff (Ry; 1e10 = none);
+ * qlat: reciprocal vectors (columns).  Outputs, Fortran layout with leading
+ * dimension nbnbx_cap: n1b, n2b (1-based bands), ihw, nhw (first bin,
+ * count), jhw (1-based pointer into whw); nbnb(nqbz); whw(nhw_cap).
+ * nbnbx_out / nhwtot_out receive the sizes needed; when a capacity is short
+ * the call returns ECALJ_ERR_DIM after setting them (call again with
+ * enough room; output pointers may then be NULL on the first call).
+ * Index maps are exact (same as a sequential loop); whw is summed with FP64
+ * atomics (order-dependent rounding).  Host pointers; not capturable. */
+ecalj_status ecalj_tetwt(ecalj_handle h, int nband, int nqbz, int ntetf, const int* idtetf,
+                         int nqbzw, const double* qbzw, const int* ib1bz, const double* ekxx1,
+                         const double* ekxx2, double ef, double ebmx, const double* qlat,
+                         const double* frhis, int nwhis, int nbnbx_cap, int64_t nhw_cap,
+                         int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw,
+                         double* whw, int* nbnbx_out, int64_t* nhwtot_out);
+
 /* ---- chi0: Hilbert transform to the imaginary axis ---------------------
  * dpsion_chiq, imaginary branch with npm=1 (S/dpsion5.f90:211-232) with
  * dpsion_init's imatt (S/dpsion5.f90:84-89,121-129, hilbertmat

@@ -1,0 +1,89 @@
+"""Tetrahedron histogram weights on the GPU (ecalj_tetwt) against the oracle
+(orc_tetwt, pinned in test_tetwt_oracle.py): index maps bit-exact (n1b, n2b,
+ihw, nhw, jhw, nbnb).  Weights: each (k, pair) total within 1e-12 (the bin
+sums of a piece telescope to its volume), single bins within 1e-8 of the
+largest weight.  The per-bin bar is wide because the reference's integtetn
+separates equal corner energies by 1e-8 offsets: where two corners of a
+piece share an energy (frequent on a symmetric mesh), the bin integral
+divides by ~1e-8, so two correct evaluations (FMA contraction, pow's last
+bit, the FP64 atomics' summation order) differ at ~1e-16/1e-8 of a weight."""
+import sys
+
+import numpy as np
+import pytest
+
+from ecalj_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def h():
+    import ecalj_amd
+
+    hd = ecalj_amd.Handle(0, "f64")
+    yield hd
+    hd.close()
+
+
+@pytest.mark.parametrize("n,nocc,nun,seed,degenerate", [((2, 2, 2), 4, 5, 1, True), ((3, 3, 3), 6, 20, 2, True),
+                                                        ((4, 4, 4), 9, 40, 3, False), ((3, 2, 4), 5, 12, 4, True)])
+def test_tetwt_vs_oracle(h, orc, n, nocc, nun, seed, degenerate):
+    say("tetwt", n, nocc, nun, seed)
+    m = synth.tetra_mesh(*n)
+    ek1, ek2, ef = synth.tetra_bands(m, nocc, nun, seed=seed, degenerate=degenerate)
+    fr = synth.tetra_frhis(ek1, ek2)
+    g = h.tetwt(m, ek1, ek2, ef, fr)
+    o = orc.tetwt(m, ek1, ek2, ef, fr)
+    assert g["nbnbx"] == o["nbnbx"] and g["nhwtot"] == o["nhwtot"]
+    np.testing.assert_array_equal(g["nbnb"], o["nbnb"])
+    for key in ("n1b", "n2b", "ihw", "nhw", "jhw"):
+        np.testing.assert_array_equal(g[key], o[key], err_msg=key)
+    _check_weights(g, o)
+
+
+def _check_weights(g, o):
+    assert np.abs(g["whw"] - o["whw"]).max() <= 1e-8 * np.abs(o["whw"]).max()
+    for k in range(len(o["nbnb"])):
+        for i in range(o["nbnb"][k]):
+            j0, nh = o["jhw"][i, k] - 1, o["nhw"][i, k]
+            tg, to = g["whw"][j0:j0 + nh].sum(), o["whw"][j0:j0 + nh].sum()
+            assert abs(tg - to) <= 1e-12 * max(abs(to), 1e-3)
+
+
+def test_tetwt_band_cutoff(h, orc):
+    """ebmx drops the bands above it at any corner (nbandmx, S/tetwt5.f90:238)."""
+    say("tetwt ebmx")
+    m = synth.tetra_mesh(3, 3, 3)
+    ek1, ek2, ef = synth.tetra_bands(m, 5, 15, seed=9)
+    fr = synth.tetra_frhis(ek1, ek2)
+    ebmx = 30.0 / synth.RYDBERG_EV
+    g = h.tetwt(m, ek1, ek2, ef, fr, ebmx=ebmx)
+    o = orc.tetwt(m, ek1, ek2, ef, fr, ebmx=ebmx)
+    for key in ("nbnb", "n1b", "n2b", "ihw", "nhw", "jhw"):
+        np.testing.assert_array_equal(g[key], o[key], err_msg=key)
+    _check_weights(g, o)
+    assert g["n2b"].max() < ek2.shape[0]
+
+
+def test_tetwt_feeds_x0gemm(h, orc):
+    """The weights drive x0gemm through the reference's index lists
+    (x0kf_v4hz_init): rcxq from the device weights equals rcxq from the
+    oracle weights."""
+    say("tetwt -> x0kf lists -> x0gemm")
+    m = synth.tetra_mesh(2, 2, 2)
+    ek1, ek2, ef = synth.tetra_bands(m, 4, 6, seed=5)
+    fr = synth.tetra_frhis(ek1, ek2)
+    g = h.tetwt(m, ek1, ek2, ef, fr)
+    o = orc.tetwt(m, ek1, ek2, ef, fr)
+    ixg = synth.x0kf_index(synth.pair_weights_from_tetwt(g))
+    ixo = synth.x0kf_index(synth.pair_weights_from_tetwt(o))
+    for a, b in zip((ixg.itc, ixg.itpc, ixg.iwini, ixg.iwend, ixg.icouini), (ixo.itc, ixo.itpc, ixo.iwini,
+                                                                             ixo.iwend, ixo.icouini)):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(ixg.whwc - ixo.whwc).max() <= 1e-8 * np.abs(ixo.whwc).max()
