@@ -264,6 +264,46 @@ def bench_real_axis(mixed, name="C2", log_=None):
     }
 
 
+def bench_tetwt(log_=None):
+    """Tetrahedron histogram weights (gettetwt + tetwt5x_dtet4): the C2-like
+    mesh (4x4x4 k, 9 occupied + 150 unoccupied bands) on the GPU and in the
+    FP64 oracle (one host thread), and the C4-like mesh (6x6x6 k, 24 + 400
+    bands) on the GPU.  Smooth synthetic bands, two crossing E_F."""
+    from ecalj_amd import Handle, synth
+
+    out = {}
+    h = Handle(0, "f64")
+    try:
+        for name, n, nocc, nun, cpu in (("C2-like", (4, 4, 4), 9, 150, True), ("C4-like", (6, 6, 6), 24, 400, False)):
+            m = synth.tetra_mesh(*n)
+            ek1, ek2, ef = synth.tetra_bands(m, nocc, nun)
+            fr = synth.tetra_frhis(ek1, ek2)
+            h.tetwt(m, ek1, ek2, ef, fr)
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                r = h.tetwt(m, ek1, ek2, ef, fr)
+                ts.append(time.perf_counter() - t0)
+            row = {"nqbz": m.nqbz, "ntetf": int(m.idtetf.shape[1]), "nband": nocc + nun,
+                   "pairs": int(r["nbnb"].sum()), "nhwtot": int(r["nhwtot"]), "gpu_s": round(float(np.median(ts)), 5)}
+            if cpu:
+                import oracle as orc
+                orc.build()
+                t0 = time.perf_counter()
+                o = orc.tetwt(m, ek1, ek2, ef, fr)
+                row["cpu_oracle_s"] = round(time.perf_counter() - t0, 4)
+                row["cpu_threads"] = 1
+                row["index_maps_equal"] = bool(all(np.array_equal(r[k], o[k]) for k in ("n1b", "n2b", "ihw", "nhw", "jhw")))
+            if log_:
+                log_(f"bench[tetwt {name}]: {row}")
+            out[name] = row
+    finally:
+        h.close()
+    out["note"] = ("host-pointer call: includes the uploads, two small host round trips between the passes and "
+                   "the copy-out of the lists and whw")
+    return out
+
+
 def roofline(fl, info, precision, steps, config="C2"):
     ms, nl = info["ktimes"]["sigc"]
     launches_per_step = max(1, nl // steps)
@@ -296,7 +336,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5,C2R",
+    ap.add_argument("--also", default="C3,C4,C5,C2R,TW",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -381,6 +421,9 @@ def main():
             continue
         if name == "C5":
             out["also"]["C5"] = bench_c5(log_=log)
+            continue
+        if name == "TW":
+            out["also"]["TW"] = bench_tetwt(log_=log)
             continue
         if name.endswith("R") and name[:-1] in synth.CONFIGS:
             out["also"][name] = bench_real_axis(args.mixed, name[:-1], log_=log)
