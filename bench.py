@@ -341,6 +341,8 @@ def main():
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run the library's plain copy kernel on MIB MiB (HBM-counter calibration)")
+    ap.add_argument("--also-only", action="store_true",
+                    help="run only the --also legs (prints their JSON; for profiling them)")
     ap.add_argument("--only", choices=["f64", "mixed"], default=None,
                     help="profile one form only (prints a short line instead of the contract JSON)")
     args = ap.parse_args()
@@ -364,6 +366,21 @@ def main():
         hc.sync()
         log(f"bench: copy kernel ran 3x on {args.copy_calibration} MiB (reads and writes {nbytes} bytes each)")
         hc.close()
+    if args.also_only:
+        res = {}
+        for name in [c for c in args.also.split(",") if c and c.lower() != "none"]:
+            if name == "TW":
+                res[name] = bench_tetwt(log_=log)
+            elif name == "C4":
+                res[name] = bench_c4(args.mixed, log_=log)
+            elif name == "C5":
+                res[name] = bench_c5(log_=log)
+            elif name.endswith("R") and name[:-1] in synth.CONFIGS:
+                res[name] = bench_real_axis(args.mixed, name[:-1], log_=log)
+            else:
+                raise SystemExit(f"--also-only: {name} is not a separate leg")
+        print(json.dumps(res), flush=True)
+        return
     cfg = synth.CONFIGS[args.config]
     log(f"bench: generating {cfg.name} inputs")
     qp = synth.make_qpoint(cfg)
