@@ -216,6 +216,90 @@ def frhis(qp: QPoint) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------
+# interstitial plane-wave (IPW) part of zmel
+@dataclass
+class IPWCase:
+    """Inputs of zmel's IPW block (S/m_zmel.f90:418-538): G-vector lists of
+    the IPW product basis (ngveccR), of the plane-wave basis at q (ngvecpB1)
+    and at q-k (ngvecpB2); the m_read_ppovl tables of q: ggg (Fourier
+    coefficients of the interstitial step function) with its index cube
+    igggi over [lo, lo+n), the G list nvgcgp2 (every Gc+G2 and Gc-G1 needed)
+    with its index cube igcgp2i; PW coefficients geigq (ngpmx x ntp0, end
+    states at q) and dgeigqk (ngpmx x nm, middle states at q-k); nadd; the
+    phase exp(-i 2pi (R k + G).shtv) per IPW function."""
+    ngveccR: np.ndarray
+    ngvecpB1: np.ndarray
+    ngvecpB2: np.ndarray
+    nadd: np.ndarray
+    ggg: np.ndarray
+    igggi: np.ndarray
+    ggg_lo: np.ndarray
+    nvgcgp2: np.ndarray
+    igcgp2i: np.ndarray
+    gcp_lo: np.ndarray
+    geigq: np.ndarray
+    dgeigqk: np.ndarray
+    phase: np.ndarray
+
+
+def _cube(vecs, lo, hi):
+    n = hi - lo + 1
+    cube = np.zeros(tuple(n), np.int32, order="F")
+    for i, v in enumerate(vecs):
+        cube[tuple(v - lo)] = i + 1
+    return cube
+
+
+def ipw_case(seed=1, ngc=40, ngp1=30, ngp2=35, nm=7, ntp0=5, ngpmx=None, nadd=(0, 1, -1), gcut2=12,
+             absent=0.1) -> IPWCase:
+    """Seeded IPW inputs: integer G vectors sorted by shell, a step-function
+    table with Hermitian symmetry ggg(-G) = conj(ggg(G)) inside |G|^2 <=
+    gcut2 (entries beyond it, and a fraction `absent` inside, have index 0:
+    no contribution), random PW coefficients."""
+    rng = np.random.default_rng(seed)
+    G = gvectors(max(ngc, ngp1, ngp2) + 8)
+    gc = G[:ngc].copy()
+    g1 = G[rng.permutation(ngp1)].copy()
+    g2 = G[rng.permutation(ngp2)].copy()
+    nadd = np.asarray(nadd, np.int32)
+    combos = np.unique(np.concatenate([(gc[:, None, :] + g2[None, :, :]).reshape(-1, 3),
+                                       (gc[:, None, :] - g1[None, :, :]).reshape(-1, 3)]), axis=0)
+    lo2, hi2 = combos.min(0), combos.max(0)
+    igcgp2i = _cube(combos, lo2, hi2)
+    # every G1 - X - nadd and -G2 - X - nadd the two branches look up
+    need = np.concatenate([(g1[None, :, :] - combos[:, None, :] - nadd).reshape(-1, 3),
+                           (-g2[None, :, :] - combos[:, None, :] - nadd).reshape(-1, 3)])
+    lo, hi = need.min(0) - 1, need.max(0) + 1
+    r = int(np.abs(np.concatenate([lo, hi])).max())
+    ax = np.arange(-r, r + 1)
+    allg = np.array(np.meshgrid(ax, ax, ax, indexing="ij")).reshape(3, -1).T
+    inside = allg[(allg ** 2).sum(1) <= gcut2]
+    half = [v for v in inside if tuple(v) >= tuple(-v)]      # one of each +-G pair
+    vals = {}
+    for v in half:
+        if rng.random() < absent and np.any(v != 0):
+            continue
+        z = complex(rng.normal(), rng.normal()) / (1.0 + float((v ** 2).sum()))
+        if np.all(v == 0):
+            z = complex(0.7, 0.0)
+        vals[tuple(v)] = z
+        vals[tuple(-v)] = np.conj(z)
+    keys = sorted(vals)
+    ggg = np.array([vals[k] for k in keys], np.complex128)
+    kv = np.array(keys, np.int32)
+    cube = np.zeros(tuple(hi - lo + 1), np.int32, order="F")
+    for i, v in enumerate(kv):
+        if np.all(v >= lo) and np.all(v <= hi):
+            cube[tuple(v - lo)] = i + 1
+    ngpmx = ngpmx or max(ngp1, ngp2) + 3
+    crand = lambda *sh: np.asfortranarray(rng.normal(size=sh) + 1j * rng.normal(size=sh))
+    return IPWCase(np.asfortranarray(gc.T, np.int32), np.asfortranarray(g1.T, np.int32),
+                   np.asfortranarray(g2.T, np.int32), nadd, ggg, cube, lo.astype(np.int32),
+                   np.asfortranarray(combos.T, np.int32), igcgp2i, lo2.astype(np.int32),
+                   crand(ngpmx, ntp0), crand(ngpmx, nm), np.exp(1j * rng.uniform(0, 2 * np.pi, ngc)))
+
+
+# --------------------------------------------------------------------------
 # k-mesh, tetrahedra and smooth bands for the tetrahedron weights
 @dataclass
 class TetraMesh:

@@ -217,6 +217,26 @@ def tetwt(mesh, ek1, ek2, ef, frhis_, ebmx=1e10):
     raise RuntimeError("orc_tetwt: capacity loop did not converge")
 
 
+def zmel_ipw(case, branch=0):
+    """IPW block of zmel for a synth.IPWCase: ngc x nm x ntp0."""
+    c = case
+    ngc = c.ngveccR.shape[1]
+    nm, ntp0 = c.dgeigqk.shape[1], c.geigq.shape[1]
+    out = np.zeros((ngc, nm, ntp0), np.complex128, order="F")
+    i32 = lambda a: np.asfortranarray(a, np.int32)
+    args = [i32(c.ngveccR), i32(c.nadd), _cz(c.phase), i32(c.ngvecpB1), _cz(c.geigq), i32(c.ngvecpB2),
+            _cz(c.dgeigqk), _cz(c.ggg), i32(c.ggg_lo), i32(np.array(c.igggi.shape)), i32(c.igggi),
+            i32(c.nvgcgp2), i32(c.gcp_lo), i32(np.array(c.igcgp2i.shape)), i32(c.igcgp2i)]
+    gcR, nadd, ph, g1, geq, g2, dgk, ggg, glo, gn, igg, nvg, clo, cn, icg = args
+    rc = lib().orc_zmel_ipw(ngc, _ip_(gcR), _ip_(nadd), _p(ph), c.geigq.shape[0], g1.shape[1], _ip_(g1), _p(geq),
+                            ntp0, g2.shape[1], _ip_(g2), _p(dgk), nm, _p(ggg), _ip_(glo), _ip_(gn), _ip_(igg),
+                            nvg.shape[1], _ip_(nvg), _ip_(clo), _ip_(cn), _ip_(icg), branch, _p(out),
+                            C.c_long(ngc))
+    if rc != 0:
+        raise ValueError(f"orc_zmel_ipw rc={rc}")
+    return out
+
+
 def rmatt(freqr, frhis_):
     """dpsion_init real branch (npm=1): (nw+1) x nwhis, freqr = freq_r(0:nw)."""
     freqr = _cd(freqr)

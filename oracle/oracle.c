@@ -276,6 +276,77 @@ void orc_dpsion_real(int npr, int npr_col, int nwhis, int nw, const double* frhi
     }
 }
 
+/* zmel IPW block (S/m_zmel.f90:418-538).  gggmat(X, G) = ggg(igggi(s G - X -
+ * nadd)) (0 outside the cube or for index 0); branch 1 (G1 first, taken when
+ * nm > ntp0): ggitp = gggmat(X, G1) geigq, zmelp0(:, n, itp) =
+ * ggitp(igcgp2i(Gc + G2), itp) dgeigqk(:, n); branch 2: ggit = gggmat(X, G2)
+ * dgeigqk with s = -1, zmelp0(:, it, :) = ggit(igcgp2i(Gc - G1), it) geigq;
+ * zmel(igc, n, m) = phase(igc) zmelp0.  branch 0 = the reference's choice.
+ * Returns -1 when an igcgp2i lookup leaves its cube or finds no entry. */
+static long cube_at(const int* v, const int* lo, const int* n) {
+  long i0 = v[0] - lo[0], i1 = v[1] - lo[1], i2 = v[2] - lo[2];
+  if (i0 < 0 || i0 >= n[0] || i1 < 0 || i1 >= n[1] || i2 < 0 || i2 >= n[2]) return -1;
+  return i0 + (long)n[0] * (i1 + (long)n[1] * i2);
+}
+
+int orc_zmel_ipw(int ngc, const int* ngveccR, const int* nadd, const double complex* phase, int ngpmx, int ngp1,
+                 const int* ngvecpB1, const double complex* geigq, int ntp0, int ngp2, const int* ngvecpB2,
+                 const double complex* dgeigqk, int nm, const double complex* ggg, const int* glo, const int* gn,
+                 const int* igggi, int ngcgp, const int* nvgcgp2, const int* clo, const int* cn,
+                 const int* igcgp2i, int branch, double complex* zmel, long ldz) {
+  if (branch == 0) branch = (nm > ntp0) ? 1 : 2;
+  const int s = (branch == 1) ? 1 : -1;
+  const int ngp = (branch == 1) ? ngp1 : ngp2;
+  const int* gp = (branch == 1) ? ngvecpB1 : ngvecpB2;
+  const double complex* B = (branch == 1) ? geigq : dgeigqk; /* first contraction */
+  const int nB = (branch == 1) ? ntp0 : nm;
+  double complex* mat = (double complex*)calloc((size_t)ngcgp * ngp, sizeof(double complex));
+  for (int c = 0; c < ngp; ++c)
+    for (int r = 0; r < ngcgp; ++r) {
+      int v[3];
+      for (int d = 0; d < 3; ++d) v[d] = s * gp[d + 3 * c] - nvgcgp2[d + 3 * r] - nadd[d];
+      long at = cube_at(v, glo, gn);
+      if (at >= 0 && igggi[at] > 0) mat[r + (long)ngcgp * c] = ggg[igggi[at] - 1];
+    }
+  double complex* gg = (double complex*)malloc(sizeof(double complex) * (size_t)ngcgp * nB);
+  orc_zgemm('N', 'N', ngcgp, nB, ngp, 1.0, mat, ngcgp, B, ngpmx, 0.0, gg, ngcgp);
+  free(mat);
+  /* second contraction over the other plane-wave basis */
+  const int ngq = (branch == 1) ? ngp2 : ngp1;
+  const int* gq = (branch == 1) ? ngvecpB2 : ngvecpB1;
+  const int t = (branch == 1) ? 1 : -1; /* Gc + G2 or Gc - G1 */
+  const double complex* C2 = (branch == 1) ? dgeigqk : geigq;
+  const int nC = (branch == 1) ? nm : ntp0;
+  double complex* work = (double complex*)malloc(sizeof(double complex) * (size_t)ngc * ngq);
+  double complex* out = (double complex*)malloc(sizeof(double complex) * (size_t)ngc * nC);
+  int rc = 0;
+  for (int j = 0; j < nB && rc == 0; ++j) {
+    for (int q = 0; q < ngq && rc == 0; ++q)
+      for (int c = 0; c < ngc; ++c) {
+        int v[3];
+        for (int d = 0; d < 3; ++d) v[d] = ngveccR[d + 3 * c] + t * gq[d + 3 * q];
+        long at = cube_at(v, clo, cn);
+        if (at < 0 || igcgp2i[at] <= 0) {
+          rc = -1;
+          break;
+        }
+        work[c + (long)ngc * q] = gg[(igcgp2i[at] - 1) + (long)ngcgp * j];
+      }
+    if (rc) break;
+    orc_zgemm('N', 'N', ngc, nC, ngq, 1.0, work, ngc, C2, ngpmx, 0.0, out, ngc);
+    for (int i = 0; i < nC; ++i)
+      for (int c = 0; c < ngc; ++c) {
+        /* branch 1: j = itp, i = n; branch 2: j = it, i = itp */
+        const int n = (branch == 1) ? i : j, m = (branch == 1) ? j : i;
+        zmel[c + ldz * (n + (long)nm * m)] = phase[c] * out[c + (long)ngc * i];
+      }
+  }
+  free(gg);
+  free(work);
+  free(out);
+  return rc;
+}
+
 /* ---------------------------------------------------------------------- */
 /* zmel (S/m_zmel.f90:146-597).  MT valence-valence block per atom
  * (:323-371):  ppbvphiq(:,:,i) = ppb(:,:,i)^T cphiq  (:361-362),

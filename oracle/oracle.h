@@ -19,6 +19,7 @@
  *                      analytic Hilbert transform of piecewise-linear f
  *   orc_wfacx2/weavx2  oracle/_ref (S/wfacx.f90 compiled with flang)
  *   orc_zmel           numpy einsum on tiny inputs; unitary M->E invariance
+ *   orc_zmel_ipw       numpy triple sum over (G1, G2); both branches agree
  *   orc_x0gemm         brute-force numpy sum over pairs; Hermiticity
  *   orc_dpsion_imag    numpy; delta-pair limit -2D/(D^2+nu^2)
  *   orc_rmatt          oracle/_ref hilbertmat at real zz; principal-value
@@ -99,6 +100,15 @@ void orc_rmatt(int nw, const double* freqr, int nwhis, const double* frhis, doub
 void orc_dpsion_real(int npr, int npr_col, int nwhis, int nw, const double* frhis,
                      const double* rmatt, double ecut, int scaled, double complex* rcxq,
                      double complex* zxq);
+/* zmel IPW block (S/m_zmel.f90:418-538): rows igc of zmel (leading dimension
+ * ldz) for nm middle x ntp0 end states.  G lists 3 x n (int), cubes Fortran
+ * order with lower bounds lo and extents n.  branch: 0 = as the reference
+ * (nm > ntp0: G1 first), 1 or 2 forced. */
+int orc_zmel_ipw(int ngc, const int* ngveccR, const int* nadd, const double complex* phase, int ngpmx, int ngp1,
+                 const int* ngvecpB1, const double complex* geigq, int ntp0, int ngp2, const int* ngvecpB2,
+                 const double complex* dgeigqk, int nm, const double complex* ggg, const int* glo, const int* gn,
+                 const int* igggi, int ngcgp, const int* nvgcgp2, const int* clo, const int* cn,
+                 const int* igcgp2i, int branch, double complex* zmel, long ldz);
 /* dpsion_chiq, imaginary axis with npm=1 (S/dpsion5.f90:211-232): scales rcxq
  * bins 1..nwhis in place by -wfac/(R-L) and sets
  * zxqi(:,:,iv) = sum_iw imatt(iv,iw) rcxq(:,:,iw).  rcxq has nwhis+1 slices
