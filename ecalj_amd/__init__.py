@@ -68,6 +68,10 @@ _SIGS = {
     "ecalj_dpsion_real": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
     "ecalj_dpsion_real_dev": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
     "ecalj_zminv": [_p, _p, _i, _i],
+    "ecalj_set_ppovl": [_p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p],
+    "ecalj_zmel_ipw": [_p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _i, _p, _p, _i, _p, _i, _i],
+    "ecalj_zmel_ipw_dev": [_p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _i, _p, _p, _i, _p, _i, _i],
+    "ecalj_zmel_assemble_dev": [_p, _p, _i, _p, _i, _l, _i, _i, _p],
     "ecalj_tetwt": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _l, _p, _p, _p, _p, _p, _p,
                     _p, _p, _p],
     "ecalj_zminv_dev": [_p, _p, _i, _i, _i, C.c_int64],
@@ -474,6 +478,46 @@ class Handle:
         _chk(lib().ecalj_wc_build_real_dev(self._h, _t_ptr(chi), ngb, nw, nspin, int(is_x0_m_basis),
                                            int(is_wc_m_basis), _t_ptr(wc), ngb if ldw is None else ldw),
              "wc_build_real_dev")
+
+    def set_ppovl(self, case):
+        """m_read_ppovl tables of q (synth.IPWCase fields ggg, igggi, ggg_lo, nvgcgp2, igcgp2i, gcp_lo)."""
+        i32 = lambda a: np.asfortranarray(a, np.int32)
+        ggg = np.ascontiguousarray(case.ggg, np.complex128)
+        igg, nvg, icg = i32(case.igggi), i32(case.nvgcgp2), i32(case.igcgp2i)
+        glo, gn = i32(case.ggg_lo), i32(np.array(igg.shape))
+        clo, cn = i32(case.gcp_lo), i32(np.array(icg.shape))
+        _chk(lib().ecalj_set_ppovl(self._h, len(ggg), _np_ptr(ggg), _np_ptr(glo), _np_ptr(gn), _np_ptr(igg),
+                                   nvg.shape[1], _np_ptr(nvg), _np_ptr(clo), _np_ptr(cn), _np_ptr(icg)), "set_ppovl")
+
+    def _ipw_args(self, case):
+        i32 = lambda a: np.asfortranarray(a, np.int32)
+        self._ipw_keep = [i32(case.ngveccR), i32(case.nadd), np.ascontiguousarray(case.phase, np.complex128),
+                          i32(case.ngvecpB1), i32(case.ngvecpB2)]
+        return self._ipw_keep
+
+    def zmel_ipw(self, case, zmel=None, row0=0):
+        """IPW rows of zmel (S/m_zmel.f90:418-538), host arrays; returns ngc x nm x ntp0 (or fills zmel)."""
+        gc, nadd, ph, g1, g2 = self._ipw_args(case)
+        geq = np.asfortranarray(case.geigq, np.complex128)
+        dgk = np.asfortranarray(case.dgeigqk, np.complex128)
+        ngc, nm, ntp0 = gc.shape[1], dgk.shape[1], geq.shape[1]
+        out = np.zeros((ngc, nm, ntp0), np.complex128, order="F") if zmel is None else zmel
+        _chk(lib().ecalj_zmel_ipw(self._h, ngc, _np_ptr(gc), _np_ptr(nadd), _np_ptr(ph), geq.shape[0], g1.shape[1],
+                                  _np_ptr(g1), _np_ptr(geq), ntp0, g2.shape[1], _np_ptr(g2), _np_ptr(dgk), nm,
+                                  _np_ptr(out), out.shape[0], row0), "zmel_ipw")
+        return out
+
+    def zmel_ipw_dev(self, case, geigq, dgeigqk, zmel, ldz, row0=0):
+        gc, nadd, ph, g1, g2 = self._ipw_args(case)
+        ngpmx, ntp0 = case.geigq.shape
+        nm = case.dgeigqk.shape[1]
+        _chk(lib().ecalj_zmel_ipw_dev(self._h, gc.shape[1], _np_ptr(gc), _np_ptr(nadd), _np_ptr(ph), ngpmx,
+                                      g1.shape[1], _np_ptr(g1), _t_ptr(geigq), ntp0, g2.shape[1], _np_ptr(g2),
+                                      _t_ptr(dgeigqk), nm, _t_ptr(zmel), ldz, row0), "zmel_ipw_dev")
+
+    def zmel_assemble_dev(self, mt, nbloch, ipw, ngc, ncol, zmel, is_m_basis=False, zmelconjg=False):
+        _chk(lib().ecalj_zmel_assemble_dev(self._h, _t_ptr(mt), nbloch, _t_ptr(ipw), ngc, ncol, int(is_m_basis),
+                                           int(zmelconjg), _t_ptr(zmel)), "zmel_assemble_dev")
 
     def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10):
         """gettetwt (S/m_tetwt.f90:36-256), npm=1: pair lists and histogram weights.

@@ -313,6 +313,8 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
   if (h->d_vcousq) (void)hipFree(h->d_vcousq);
   if (h->d_zcousq) (void)hipFree(h->d_zcousq);
   if (h->d_zcousq32) (void)hipFree(h->d_zcousq32);
+  for (void* p : {(void*)h->d_ggg, (void*)h->d_igggi, (void*)h->d_nvgcgp2, (void*)h->d_igcgp2i})
+    if (p) (void)hipFree(p);
   if (h->d_flag) (void)hipFree(h->d_flag);
   for (auto& pool : h->ev_pool)
     for (auto& p : pool) {

@@ -26,10 +26,21 @@ struct ecalj_ctx {
   double* d_vcousq = nullptr;     // sqrt(vcoud)
   ecalj::cd* d_zcousq = nullptr;  // ngb x ngb
   ecalj::cf* d_zcousq32 = nullptr;  // complex64 copy for the mixed-precision M->E transform
+  // set_ppovl state (m_read_ppovl of the current q): the interstitial step
+  // function's Fourier table ggg with its index cube, and the combined-G list
+  // with its index cube (host copies for argument checks)
+  ecalj::cd* d_ggg = nullptr;
+  int* d_igggi = nullptr;
+  int ggg_lo[3] = {0, 0, 0}, ggg_n[3] = {0, 0, 0};
+  int* d_nvgcgp2 = nullptr;
+  int ngcgp = 0;
+  int* d_igcgp2i = nullptr;
+  int gcp_lo[3] = {0, 0, 0}, gcp_n[3] = {0, 0, 0};
+  std::vector<int> h_igcgp2i;
 
   // grow-only scratch slots
   enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
-         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_NSLOT };
+         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag

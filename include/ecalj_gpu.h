@@ -150,6 +150,43 @@ ecalj_status ecalj_zmel_strided_dev(ecalj_handle h, const double* phase, int ndi
                                     const void* cphiq, int64_t cphiq_stride, int is_m_basis,
                                     int zmelconjg, int nbatch, void* zmel);
 
+/* ---- zmel: interstitial plane-wave rows ---------------------------------
+ * m_read_ppovl tables of the current q (S/m_zmel.f90:419): ggg (nggg complex,
+ * Fourier coefficients of the interstitial step function) with its index
+ * cube igggi (1-based into ggg, 0 = none; Fortran order over
+ * [lo, lo+n) per axis), the combined G list nvgcgp2 (3 x ngcgp) with its
+ * index cube igcgp2i.  Host pointers; kept on the device. */
+ecalj_status ecalj_set_ppovl(ecalj_handle h, int nggg, const double* ggg, const int* ggg_lo,
+                             const int* ggg_n, const int* igggi, int ngcgp, const int* nvgcgp2,
+                             const int* gcp_lo, const int* gcp_n, const int* igcgp2i);
+/* ZmelIPW block of get_zmel_init_gemm (S/m_zmel.f90:418-538):
+ *   zmel(row0+c, n, m) = phase(c) sum_{G1,G2} ggg(G1 - Gc - G2 - nadd) geigq(G1, m) dgeigqk(G2, n)
+ * for c < ngc (Gc = ngveccR(:,c)), n < nm, m < ntp0; G1 over ngvecpB1
+ * (3 x ngp1, the PW basis of q), G2 over ngvecpB2 (3 x ngp2, of q-k).
+ * geigq: ngpmx x ntp0, dgeigqk: ngpmx x nm (complex128; device in _dev);
+ * zmel: ldz x nm x ntp0 in the precision's storage type (_dev) or complex128
+ * (host), only rows row0..row0+ngc-1 written.  phase: ngc complex (host).
+ * Contraction order as the reference: G1 first when nm > ntp0.  Every
+ * Gc + G2 (G1 first) or Gc - G1 (G2 first) must be in the combined list
+ * (ECALJ_ERR_ARG otherwise). */
+ecalj_status ecalj_zmel_ipw(ecalj_handle h, int ngc, const int* ngveccR, const int* nadd,
+                            const double* phase, int ngpmx, int ngp1, const int* ngvecpB1,
+                            const double* geigq, int ntp0, int ngp2, const int* ngvecpB2,
+                            const double* dgeigqk, int nm, double* zmel, int ldz, int row0);
+ecalj_status ecalj_zmel_ipw_dev(ecalj_handle h, int ngc, const int* ngveccR, const int* nadd,
+                                const double* phase, int ngpmx, int ngp1, const int* ngvecpB1,
+                                const void* geigq, int ntp0, int ngp2, const int* ngvecpB2,
+                                const void* dgeigqk, int nm, void* zmel, int ldz, int row0);
+/* The rows of zmelt (MT rows 0..nbloch-1 from ecalj_zmel_dev with
+ * is_m_basis = 1, IPW rows nbloch..nbloch+ngc-1 from ecalj_zmel_ipw_dev) put
+ * together, then zmel = zcousq^H zmelt when is_m_basis == 0
+ * (S/m_zmel.f90:559-560; ngb of ecalj_set_coulomb = nbloch + ngc) and conj
+ * when zmelconjg (:590-594).  mt: nbloch x ncol, ipw: ngc x ncol, zmel:
+ * (nbloch+ngc) x ncol, device arrays in the storage type. */
+ecalj_status ecalj_zmel_assemble_dev(ecalj_handle h, const void* mt, int nbloch, const void* ipw,
+                                     int ngc, int64_t ncol, int is_m_basis, int zmelconjg,
+                                     void* zmel);
+
 /* ---- chi0: x0gemm -----------------------------------------------------
  * x0gemm (S/x0gemm.f90:2-97) with its module inputs explicit
  * (m_x0kf: icounkmink/icounkmaxk, iwini, iwend, itc, itpc, jpmc, icouini,
