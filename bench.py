@@ -304,6 +304,49 @@ def bench_tetwt(log_=None):
     return out
 
 
+def bench_ipw(mixed, log_=None):
+    """Interstitial plane-wave rows of zmel for one k at GaAs-like sizes: 150
+    IPW product functions, 180 plane waves at q and at q-k; the Sigma_c shape
+    (159 middle x 18 end states: G1 first) and the chi0 shape (9 occupied x
+    150 unoccupied: G2 first).  Device time per call (inputs resident), both
+    forms; the FP64 oracle on one host thread for comparison."""
+    import oracle as orc
+    from ecalj_amd import Handle, synth
+
+    orc.build()
+    out = {}
+    for name, nm, ntp0 in (("sigma_shape", 159, 18), ("chi0_shape", 9, 150)):
+        c = synth.ipw_case(7, 150, 180, 180, nm, ntp0, gcut2=40)
+        row = {"ngc": 150, "ngp1": 180, "ngp2": 180, "nm": nm, "ntp0": ntp0, "ngcgp": int(c.nvgcgp2.shape[1])}
+        ref = None
+        for prec in ("f64", mixed):
+            h = Handle(0, prec)
+            try:
+                h.set_ppovl(c)
+                st = np.complex128 if prec == "f64" else np.complex64
+                dq, dk = h.to_dev(c.geigq, np.complex128), h.to_dev(c.dgeigqk, np.complex128)
+                dz = h.array((150, nm, ntp0), st)
+                h.zmel_ipw_dev(c, dq, dk, dz, 150)
+                h.sync()
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    h.zmel_ipw_dev(c, dq, dk, dz, 150)
+                h.sync()
+                row[f"{prec}_ms"] = round((time.perf_counter() - t0) / 10 * 1e3, 4)
+                got = dz.download().astype(np.complex128)
+            finally:
+                h.close()
+            if ref is None:
+                t0 = time.perf_counter()
+                ref = orc.zmel_ipw(c)
+                row["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+            row[f"{prec}_relerr"] = float(np.abs(got - ref).max() / np.abs(ref).max())
+        if log_:
+            log_(f"bench[IPW {name}]: {row}")
+        out[name] = row
+    return out
+
+
 def roofline(fl, info, precision, steps, config="C2"):
     ms, nl = info["ktimes"]["sigc"]
     launches_per_step = max(1, nl // steps)
@@ -336,7 +379,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5,C2R,TW",
+    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -371,6 +414,8 @@ def main():
         for name in [c for c in args.also.split(",") if c and c.lower() != "none"]:
             if name == "TW":
                 res[name] = bench_tetwt(log_=log)
+            elif name == "IPW":
+                res[name] = bench_ipw(args.mixed, log_=log)
             elif name == "C4":
                 res[name] = bench_c4(args.mixed, log_=log)
             elif name == "C5":
@@ -441,6 +486,9 @@ def main():
             continue
         if name == "TW":
             out["also"]["TW"] = bench_tetwt(log_=log)
+            continue
+        if name == "IPW":
+            out["also"]["IPW"] = bench_ipw(args.mixed, log_=log)
             continue
         if name.endswith("R") and name[:-1] in synth.CONFIGS:
             out["also"][name] = bench_real_axis(args.mixed, name[:-1], log_=log)
