@@ -15,6 +15,7 @@ module ecalj_gpu
   public :: ecalj_set_tuning, ecalj_graph_begin, ecalj_graph_end, ecalj_graph_launch, ecalj_graph_destroy
   public :: ecalj_dpsion_real, ecalj_dpsion_real_dev, ecalj_wc_build_real, ecalj_wc_build_real_dev
   public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
+  public :: ecalj_tetwt, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
   integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
 
   interface
@@ -158,6 +159,54 @@ module ecalj_gpu
       type(c_ptr), value :: h, a
       integer(c_int), value :: n, lda, nbatch
       integer(c_int64_t), value :: stride
+    end function
+    ! gettetwt + tetwt5x_dtet4, npm=1 (S/m_tetwt.f90:36-256, S/tetwt5.f90:9-486)
+    integer(c_int) function ecalj_tetwt(h, nband, nqbz, ntetf, idtetf, nqbzw, qbzw, ib1bz, ekxx1, ekxx2, ef, ebmx, &
+        qlat, frhis, nwhis, nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, jhw, whw, nbnbx_out, nhwtot_out) &
+        bind(C, name="ecalj_tetwt")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: nband, nqbz, ntetf, nqbzw, nwhis, nbnbx_cap
+      integer(c_int64_t), value :: nhw_cap
+      integer(c_int), intent(in) :: idtetf(*), ib1bz(*)
+      real(c_double), intent(in) :: qbzw(*), ekxx1(*), ekxx2(*), qlat(*), frhis(*)
+      real(c_double), value :: ef, ebmx
+      integer(c_int), intent(out) :: nbnb(*), n1b(*), n2b(*), ihw(*), nhw(*), jhw(*), nbnbx_out
+      real(c_double), intent(out) :: whw(*)
+      integer(c_int64_t), intent(out) :: nhwtot_out
+    end function
+    ! m_read_ppovl tables of q, and the IPW rows of zmel (S/m_zmel.f90:418-538)
+    integer(c_int) function ecalj_set_ppovl(h, nggg, ggg, ggg_lo, ggg_n, igggi, ngcgp, nvgcgp2, gcp_lo, gcp_n, &
+        igcgp2i) bind(C, name="ecalj_set_ppovl")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      integer(c_int), value :: nggg, ngcgp
+      complex(c_double_complex), intent(in) :: ggg(*)
+      integer(c_int), intent(in) :: ggg_lo(3), ggg_n(3), igggi(*), nvgcgp2(*), gcp_lo(3), gcp_n(3), igcgp2i(*)
+    end function
+    integer(c_int) function ecalj_zmel_ipw(h, ngc, ngveccR, nadd, phase, ngpmx, ngp1, ngvecpB1, geigq, ntp0, ngp2, &
+        ngvecpB2, dgeigqk, nm, zmel, ldz, row0) bind(C, name="ecalj_zmel_ipw")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      integer(c_int), value :: ngc, ngpmx, ngp1, ntp0, ngp2, nm, ldz, row0
+      integer(c_int), intent(in) :: ngveccR(*), nadd(3), ngvecpB1(*), ngvecpB2(*)
+      complex(c_double_complex), intent(in) :: phase(*), geigq(*), dgeigqk(*)
+      complex(c_double_complex), intent(inout) :: zmel(*)
+    end function
+    integer(c_int) function ecalj_zmel_ipw_dev(h, ngc, ngveccR, nadd, phase, ngpmx, ngp1, ngvecpB1, geigq, ntp0, &
+        ngp2, ngvecpB2, dgeigqk, nm, zmel, ldz, row0) bind(C, name="ecalj_zmel_ipw_dev")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h, geigq, dgeigqk, zmel
+      integer(c_int), value :: ngc, ngpmx, ngp1, ntp0, ngp2, nm, ldz, row0
+      integer(c_int), intent(in) :: ngveccR(*), nadd(3), ngvecpB1(*), ngvecpB2(*)
+      complex(c_double_complex), intent(in) :: phase(*)
+    end function
+    integer(c_int) function ecalj_zmel_assemble_dev(h, mt, nbloch, ipw, ngc, ncol, is_m_basis, zmelconjg, zmel) &
+        bind(C, name="ecalj_zmel_assemble_dev")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: h, mt, ipw, zmel
+      integer(c_int), value :: nbloch, ngc, is_m_basis, zmelconjg
+      integer(c_int64_t), value :: ncol
     end function
     ! WVIllwI body, q /= 0 (S/m_llw.f90:299-373)
     integer(c_int) function ecalj_wc_build(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw) &
