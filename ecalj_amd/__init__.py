@@ -68,6 +68,14 @@ _SIGS = {
     "ecalj_dpsion_real": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
     "ecalj_dpsion_real_dev": [_p, _p, _i, _i, _i, _i, _p, _p, _d, _i, _p],
     "ecalj_zminv": [_p, _p, _i, _i],
+    "ecalj_vcoud_write": [C.c_char_p, _i, _p, _p, _p],
+    "ecalj_vcoud_read": [C.c_char_p, _p, _p, _p, _p, _i],
+    "ecalj_sec2_write": [C.c_char_p, _p, _i, _i, _p, _p, _p],
+    "ecalj_sec2_read": [C.c_char_p, _p, _i, _i, _p, _p, _p],
+    "ecalj_wv_d_write": [C.c_char_p, _p],
+    "ecalj_wv_d_read": [C.c_char_p, _p],
+    "ecalj_wv_write_dev": [_p, C.c_char_p, _i, _i, _i, _p],
+    "ecalj_wv_read_dev": [_p, C.c_char_p, _i, _i, _i, _p],
     "ecalj_set_ppovl": [_p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p],
     "ecalj_zmel_ipw": [_p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _i, _p, _p, _i, _p, _i, _i],
     "ecalj_zmel_ipw_dev": [_p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _i, _p, _p, _i, _p, _i, _i],
@@ -519,6 +527,15 @@ class Handle:
         _chk(lib().ecalj_zmel_assemble_dev(self._h, _t_ptr(mt), nbloch, _t_ptr(ipw), ngc, ncol, int(is_m_basis),
                                            int(zmelconjg), _t_ptr(zmel)), "zmel_assemble_dev")
 
+    def wv_write_dev(self, path, rec0, w, ld):
+        """Records rec0.. of a __WVI/__WVR file from device W (ld x ld x nrec, complex128)."""
+        nrec = w.nbytes // (16 * ld * ld)
+        _chk(lib().ecalj_wv_write_dev(self._h, str(path).encode(), rec0, nrec, ld, _t_ptr(w)), "wv_write_dev")
+
+    def wv_read_dev(self, path, rec0, w, ld):
+        nrec = w.nbytes // (16 * ld * ld)
+        _chk(lib().ecalj_wv_read_dev(self._h, str(path).encode(), rec0, nrec, ld, _t_ptr(w)), "wv_read_dev")
+
     def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10):
         """gettetwt (S/m_tetwt.f90:36-256), npm=1: pair lists and histogram weights.
         mesh: synth.TetraMesh (or any object with idtetf, qbzw, ib1bz, qlat)."""
@@ -630,6 +647,65 @@ class Handle:
         _chk(lib().ecalj_sigc_grouped_dev(self._h, len(n), _np_ptr(n), _t_ptr(zmel), ngb, ntq, niw, _t_ptr(wc),
                                           ngb if ldw is None else ldw, _t_ptr(wgtim), _t_ptr(zsec),
                                           ntq if ldz is None else ldz), "sigc_grouped_dev")
+
+
+# ---------------------------------------------------------------- files shared with the Fortran steps
+def vcoud_write(path, q, vcoud, zcousq):
+    """__Vcoud.iq (S/main_hvccfp0.f90:396-401)."""
+    v = np.ascontiguousarray(vcoud, np.float64)
+    z = _f(zcousq, np.complex128)
+    qq = np.ascontiguousarray(q, np.float64)
+    _chk(lib().ecalj_vcoud_write(str(path).encode(), len(v), _np_ptr(qq), _np_ptr(v), _np_ptr(z)), "vcoud_write")
+
+
+def vcoud_read(path):
+    """Returns (q, vcoud, zcousq) of a __Vcoud.iq file (S/m_readVcoud.f90:36-43)."""
+    n = C.c_int(0)
+    rc = lib().ecalj_vcoud_read(str(path).encode(), C.byref(n), None, None, None, 0)
+    if rc not in (0, 1):
+        _chk(rc, "vcoud_read")
+    q = np.zeros(3)
+    v = np.zeros(n.value)
+    z = np.zeros((n.value, n.value), np.complex128, order="F")
+    _chk(lib().ecalj_vcoud_read(str(path).encode(), C.byref(n), _np_ptr(q), _np_ptr(v), _np_ptr(z), n.value),
+         "vcoud_read")
+    return q, v, z
+
+
+def sec2_write(path, header8, isp, q, zsec):
+    """SEC2U (S/main_hsfp0.sc.f90:339-359); zsec ntq x ntq x nq."""
+    h8 = np.ascontiguousarray(header8, np.int32)
+    z = _f(zsec, np.complex128)
+    ntq, nq = z.shape[0], z.shape[2]
+    i = np.ascontiguousarray(isp, np.int32)
+    qq = np.asfortranarray(q, np.float64)
+    _chk(lib().ecalj_sec2_write(str(path).encode(), _np_ptr(h8), nq, ntq, _np_ptr(i), _np_ptr(qq), _np_ptr(z)),
+         "sec2_write")
+
+
+def sec2_read(path):
+    h8 = np.zeros(8, np.int32)
+    rc = lib().ecalj_sec2_read(str(path).encode(), _np_ptr(h8), 0, 0, None, None, None)
+    if rc not in (0, 1):
+        _chk(rc, "sec2_read")
+    nq, ntq = int(h8[1]), int(h8[2])
+    i = np.zeros(nq, np.int32)
+    q = np.zeros((3, nq), order="F")
+    z = np.zeros((ntq, ntq, nq), np.complex128, order="F")
+    _chk(lib().ecalj_sec2_read(str(path).encode(), _np_ptr(h8), nq, ntq, _np_ptr(i), _np_ptr(q), _np_ptr(z)),
+         "sec2_read")
+    return h8, i, q, z
+
+
+def wv_d_write(path, nprecx, mrecl, nblochpmx, nwp, niw, nq, nw_i):
+    v = np.array([nprecx, mrecl, nblochpmx, nwp, niw, nq, nw_i], np.int64)
+    _chk(lib().ecalj_wv_d_write(str(path).encode(), _np_ptr(v)), "wv_d_write")
+
+
+def wv_d_read(path):
+    v = np.zeros(7, np.int64)
+    _chk(lib().ecalj_wv_d_read(str(path).encode(), _np_ptr(v)), "wv_d_read")
+    return v
 
 
 def qp_energy(elda, sex, sexcore, sec, vxc):

@@ -16,6 +16,7 @@ module ecalj_gpu
   public :: ecalj_dpsion_real, ecalj_dpsion_real_dev, ecalj_wc_build_real, ecalj_wc_build_real_dev
   public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
   public :: ecalj_tetwt, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
+  public :: ecalj_wv_write_dev, ecalj_wv_read_dev
   integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
 
   interface
@@ -207,6 +208,19 @@ module ecalj_gpu
       type(c_ptr), value :: h, mt, ipw, zmel
       integer(c_int), value :: nbloch, ngc, is_m_basis, zmelconjg
       integer(c_int64_t), value :: ncol
+    end function
+    ! W records of __WVI/__WVR (S/m_llw.f90:102,177,308,373) to / from device memory
+    integer(c_int) function ecalj_wv_write_dev(h, path, rec0, nrec, ld, w) bind(C, name="ecalj_wv_write_dev")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, w
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: rec0, nrec, ld
+    end function
+    integer(c_int) function ecalj_wv_read_dev(h, path, rec0, nrec, ld, w) bind(C, name="ecalj_wv_read_dev")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, w
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: rec0, nrec, ld
     end function
     ! WVIllwI body, q /= 0 (S/m_llw.f90:299-373)
     integer(c_int) function ecalj_wc_build(h, chi, ngb, nw, nspin, is_x0_m_basis, is_wc_m_basis, wc, ldw) &

@@ -314,6 +314,33 @@ ecalj_status ecalj_zminv(ecalj_handle h, double* a, int n, int lda);
 ecalj_status ecalj_zminv_dev(ecalj_handle h, void* a, int n, int lda, int nbatch,
                              int64_t stride);
 
+/* ---- on-disk records shared with the Fortran steps ----------------------
+ * __Vcoud.iq (S/main_hvccfp0.f90:396-401, read by S/m_readVcoud.f90:36-43):
+ * sequential unformatted records ngb; q(3); vcoud(ngb); zcousq(ngb,ngb)
+ * (4-byte record markers).  _read: first call with ngb_cap = 0 reports ngb
+ * (ECALJ_ERR_DIM), then with room.  No handle needed. */
+ecalj_status ecalj_vcoud_write(const char* path, int ngb, const double* q, const double* vcoud,
+                               const double* zcousq);
+ecalj_status ecalj_vcoud_read(const char* path, int* ngb, double* q, double* vcoud, double* zcousq,
+                              int ngb_cap);
+/* SEC2U (S/main_hsfp0.sc.f90:339-359, read by S/main_hqpe.sc.f90:70):
+ * header8 = nspin nq ntq nqbz nqibz n1 n2 n3, then per q: is, q(3),
+ * zsec(ntq,ntq) complex(8).  _read reports the header first (capacities 0). */
+ecalj_status ecalj_sec2_write(const char* path, const int* header8, int nq, int ntq, const int* is,
+                              const double* q, const double* zsec);
+ecalj_status ecalj_sec2_read(const char* path, int* header8, int nq_cap, int ntq_cap, int* is,
+                             double* q, double* zsec);
+/* __WV.d (S/main_hrcxq.f90:134-136): nprecx mrecl nblochpmx nw+1 niw nq nw_i. */
+ecalj_status ecalj_wv_d_write(const char* path, const int64_t* v7);
+ecalj_status ecalj_wv_d_read(const char* path, int64_t* v7);
+/* __WVI.iq / __WVR.iq (S/m_llw.f90:102,177,308,373; S/m_mpiio.f90:12-74):
+ * fixed-length records of W(nblochpmx, nblochpmx) complex(8), record rec at
+ * byte (rec-1)*16*ld^2.  nrec records rec0.. to / from device memory (ld x
+ * ld each, contiguous), through pinned staging; synchronous. */
+ecalj_status ecalj_wv_write_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld,
+                                const void* w);
+ecalj_status ecalj_wv_read_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld, void* w);
+
 /* ---- Sigma_c ------------------------------------------------------------
  * Imaginary-axis weights (S/m_sxcf_gemm.f90:432-458), FP64:
  *   wgtim: (niw+1) x (ns2-ns1+1) x ntq, times wkkr.
