@@ -347,6 +347,101 @@ def bench_ipw(mixed, log_=None):
     return out
 
 
+def bench_gamma(log_=None):
+    """The Gamma point of the W build at C2 sizes (nI = 600): W(q=0) on the 16
+    imaginary and the nw+1 real frequencies of the C2 real mesh (mu = 1 left
+    out of the inversion), llw = 1/epsinv(1,1) at six offset points on both
+    meshes (the wing at omega = 0), getw0 for both, and modifyWV0 with the
+    E -> M transform.  FP64 (the W build is FP64 in every form); inputs
+    resident; device time per Gamma point.  The FP64 oracle on a one-frequency
+    sample of each inversion kind, scaled to the same work."""
+    import oracle as orc
+    from ecalj_amd import Handle, synth
+
+    orc.build()
+    c = synth.CONFIGS["C2"]
+    qp = synth.make_qpoint(c)
+    nwr = len(synth.real_mesh(qp, synth.frhis(qp)))
+    n, niw = c.ni, c.nf
+    rng = np.random.default_rng(5)
+    gc = synth.gamma_cell(lmxax=4)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    Q = np.asfortranarray(Q)
+    vc = rng.uniform(0.5, 40.0, n)
+    vcou1 = 4 * np.pi / np.sum(gc.q0i[:, 0] ** 2)
+    vc[0] = vcou1 * 0.999
+
+    def chi(nw, real):
+        x = (rng.standard_normal((n, 40)) + 1j * rng.standard_normal((n, 40))) * 0.004
+        base = -(x @ x.conj().T)
+        out = np.empty((n, n, nw), complex, order="F")
+        for i in range(nw):
+            out[:, :, i] = base / (1 + 0.1 * i)
+            if real:
+                out[:, :, i] += 1j * 0.3 * base / (1 + 0.05 * i)
+        out[0, :, :] /= np.sqrt(vcou1)
+        out[:, 0, :] /= np.sqrt(vcou1)
+        return out
+
+    ci, cr = chi(niw, False), chi(nwr, True)
+    h = Handle(0, "f64")
+    row = {"ngb": n, "niw": niw, "nw_real": nwr, "nq0i": 6}
+    try:
+        h.set_coulomb(vc, Q)
+        di, dr = h.to_dev(ci), h.to_dev(cr)
+        wi, wr = h.array((n, n, niw)), h.array((n, n, nwr))
+        li, lr = h.array((niw,)), h.array((nwr,))
+        wing = h.array((n - 1,))
+        llw_i = np.zeros((niw, 6), complex)
+        llw_r = np.zeros((nwr, 6), complex)
+
+        def step():
+            for k in range(6):  # one offset point each; same chi for the timing
+                h.eps_head_dev(di, n, niw, vcou1, li, real_axis=False)
+                h.eps_head_dev(dr, n, nwr, vcou1, lr, real_axis=True, wing_iw=0, wing=wing)
+            h.sync()
+            llw_i[:] = li.download()[:, None]
+            llw_r[:] = lr.download()[:, None]
+            w0i, _ = ecalj_getw0(1, llw_i, gc)
+            w0r, _ = ecalj_getw0(0, llw_r, gc)
+            h.wc_build_gamma_dev(di, n, niw, wi, real_axis=False)
+            h.wc_build_gamma_dev(dr, n, nwr, wr, real_axis=True)
+            h.wv0_patch_dev(wi, n, niw, w0i)
+            h.wv0_patch_dev(wr, n, nwr, w0r)
+            h.sync()
+
+        step()
+        t0 = time.perf_counter()
+        step()
+        row["s_per_gamma_point"] = round(time.perf_counter() - t0, 5)
+        for name, fn in (("eps_head_imag_ms", lambda: h.eps_head_dev(di, n, niw, vcou1, li)),
+                         ("eps_head_real_ms", lambda: h.eps_head_dev(dr, n, nwr, vcou1, lr, real_axis=True)),
+                         ("wc_build_gamma_real_ms", lambda: h.wc_build_gamma_dev(dr, n, nwr, wr, real_axis=True))):
+            h.sync()
+            t0 = time.perf_counter()
+            fn()
+            h.sync()
+            row[name] = round((time.perf_counter() - t0) * 1e3, 3)
+    finally:
+        h.close()
+    t0 = time.perf_counter()
+    orc.eps_head(ci[:, :, :1], np.sqrt(vc), Q, vcou1)
+    t1 = time.perf_counter()
+    orc.wc_build_gamma(ci[:, :, :1], np.sqrt(vc), Q)
+    t2 = time.perf_counter()
+    row["cpu_oracle_s_scaled"] = round((t1 - t0) * 6 * (niw + nwr) + (t2 - t1) * (niw + nwr), 2)
+    row["cpu_sample"] = "one eps_head and one wc_build_gamma frequency at nI=600 (oracle, host cores), scaled"
+    if log_:
+        log_(f"bench[GAM]: {row}")
+    return row
+
+
+def ecalj_getw0(ii, llw, gc):
+    import ecalj_amd
+
+    return ecalj_amd.getw0(ii, llw, gc.dmlx, gc.epinvq0i, gc.wklm, gc.wbz, gc.lmxax, gc.epinv)
+
+
 def roofline(fl, info, precision, steps, config="C2"):
     ms, nl = info["ktimes"]["sigc"]
     launches_per_step = max(1, nl // steps)
@@ -379,7 +474,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW",
+    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -416,6 +511,8 @@ def main():
                 res[name] = bench_tetwt(log_=log)
             elif name == "IPW":
                 res[name] = bench_ipw(args.mixed, log_=log)
+            elif name == "GAM":
+                res[name] = bench_gamma(log_=log)
             elif name == "C4":
                 res[name] = bench_c4(args.mixed, log_=log)
             elif name == "C5":
@@ -489,6 +586,9 @@ def main():
             continue
         if name == "IPW":
             out["also"]["IPW"] = bench_ipw(args.mixed, log_=log)
+            continue
+        if name == "GAM":
+            out["also"]["GAM"] = bench_gamma(log_=log)
             continue
         if name.endswith("R") and name[:-1] in synth.CONFIGS:
             out["also"][name] = bench_real_axis(args.mixed, name[:-1], log_=log)

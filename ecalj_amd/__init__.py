@@ -83,6 +83,13 @@ _SIGS = {
     "ecalj_tetwt": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _l, _p, _p, _p, _p, _p, _p,
                     _p, _p, _p],
     "ecalj_zminv_dev": [_p, _p, _i, _i, _i, C.c_int64],
+    "ecalj_wc_build_gamma": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "ecalj_wc_build_gamma_dev": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
+    "ecalj_eps_head": [_p, _p, _i, _i, _i, _i, _d, _i, _p, _i, _p],
+    "ecalj_eps_head_dev": [_p, _p, _i, _i, _i, _i, _d, _i, _p, _i, _p],
+    "ecalj_getw0": [_i, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _p],
+    "ecalj_wv0_patch": [_p, _p, _i, _i, _i, _p, _i],
+    "ecalj_wv0_patch_dev": [_p, _p, _i, _i, _i, _p, _i],
     "ecalj_sigc_weights": [_p, _i, _i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p, _p, _p],
     "ecalj_sigc_weights_dev": [_p, _i, _i, _i, _i, _i, _i, _p, _p, _d, _d, _p, _p, _p, _p, _p],
     "ecalj_sigc": [_p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
@@ -128,6 +135,18 @@ def lib():
         L.ecalj_status_string.restype = C.c_char_p
         _LIB = L
     return _LIB
+
+
+def getw0(ii, llw, dmlx, epinvq0i, wklm, wbz, lmxax, epinv):
+    """getw0 (S/m_w0w0i.f90:146-311), host FP64: w0 for frequencies ii..ii+nw-1 and llmat (iw = 0)."""
+    L = np.asfortranarray(llw, np.complex128)
+    nw, nq0i = L.shape
+    w0 = np.zeros(nw, np.complex128)
+    llmat = np.zeros((3, 3), np.complex128, order="F")
+    arrs = [np.asfortranarray(a, np.float64) for a in (dmlx, epinvq0i, wklm, epinv)]
+    _chk(lib().ecalj_getw0(ii, ii + nw - 1, nq0i, lmxax, _np_ptr(L), *[_np_ptr(a) for a in arrs[:3]], float(wbz),
+                           _np_ptr(arrs[3]), _np_ptr(w0), _np_ptr(llmat)), "getw0")
+    return w0, llmat
 
 
 def mlp_nparam(dims) -> int:
@@ -574,6 +593,50 @@ class Handle:
         lda = n if lda is None else lda
         stride = lda * n if stride is None else stride
         _chk(lib().ecalj_zminv_dev(self._h, _t_ptr(a), n, lda, nbatch, stride), "zminv_dev")
+
+    # ---- the Gamma point of the W build
+    def wc_build_gamma(self, chi, nspin=1, is_x0_m_basis=True, real_axis=False):
+        """q = 0 body (S/m_llw.f90:98-178, 299-373 with ix = 1): W in the E basis, row/column 1 zero."""
+        c = _f(chi, np.complex128)
+        ngb, _, nw = c.shape
+        w = np.zeros((ngb, ngb, nw), np.complex128, order="F")
+        _chk(lib().ecalj_wc_build_gamma(self._h, _np_ptr(c), ngb, nw, nspin, int(is_x0_m_basis), int(real_axis),
+                                        _np_ptr(w), ngb), "wc_build_gamma")
+        return w
+
+    def wc_build_gamma_dev(self, chi, ngb, nw, wc, ldw=None, nspin=1, is_x0_m_basis=True, real_axis=False):
+        _chk(lib().ecalj_wc_build_gamma_dev(self._h, _t_ptr(chi), ngb, nw, nspin, int(is_x0_m_basis),
+                                            int(real_axis), _t_ptr(wc), ngb if ldw is None else ldw),
+             "wc_build_gamma_dev")
+
+    def eps_head(self, chi, vcou1, nspin=1, is_x0_m_basis=True, real_axis=False, wing_iw=-1):
+        """Offset point q0i (S/m_llw.f90:183-256, 378-440): llw(nw) = 1/epsinv(1,1), and the wing at wing_iw."""
+        c = _f(chi, np.complex128)
+        ngb, _, nw = c.shape
+        llw = np.zeros(nw, np.complex128)
+        wing = np.zeros(ngb - 1, np.complex128)
+        _chk(lib().ecalj_eps_head(self._h, _np_ptr(c), ngb, nw, nspin, int(is_x0_m_basis), float(vcou1),
+                                  int(real_axis), _np_ptr(llw), wing_iw, _np_ptr(wing)), "eps_head")
+        return llw, wing
+
+    def eps_head_dev(self, chi, ngb, nw, vcou1, llw, nspin=1, is_x0_m_basis=True, real_axis=False, wing_iw=-1,
+                     wing=None):
+        _chk(lib().ecalj_eps_head_dev(self._h, _t_ptr(chi), ngb, nw, nspin, int(is_x0_m_basis), float(vcou1),
+                                      int(real_axis), _t_ptr(llw), wing_iw,
+                                      None if wing is None else _t_ptr(wing)), "eps_head_dev")
+
+    def wv0_patch(self, w, w0, is_wc_m_basis=True):
+        """modifyWV0 (S/m_w0w0i.f90:89-140): W(1,1,iw) = w0(iw), then E -> M."""
+        x = np.array(w, np.complex128, order="F", copy=True)
+        ngb, _, nw = x.shape
+        z0 = np.ascontiguousarray(w0, np.complex128)
+        _chk(lib().ecalj_wv0_patch(self._h, _np_ptr(x), ngb, ngb, nw, _np_ptr(z0), int(is_wc_m_basis)), "wv0_patch")
+        return x
+
+    def wv0_patch_dev(self, w, ngb, nw, w0, ldw=None, is_wc_m_basis=True):
+        z0 = np.ascontiguousarray(w0, np.complex128)
+        _chk(lib().ecalj_wv0_patch_dev(self._h, _t_ptr(w), ngb, ngb if ldw is None else ldw, nw, _np_ptr(z0),
+                                       int(is_wc_m_basis)), "wv0_patch_dev")
 
     # ---- Sigma_c
     def sigc_weights(self, omega, ekc, esmr, wkkr, freqx, wwx, expa, ua=1.0, ns1=1, nctot=0):

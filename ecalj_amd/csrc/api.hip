@@ -222,24 +222,7 @@ ecalj_status upload_blob(ecalj_ctx* h, int slot, const std::vector<unsigned char
   return upload(h, slot, v, out);
 }
 
-// RAII device buffer for the host-pointer entry points.
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  ecalj_status alloc(size_t bytes) {
-    if (bytes == 0) bytes = 16;
-    return hipMalloc(&p, bytes) == hipSuccess ? ECALJ_OK : ECALJ_ERR_NOMEM;
-  }
-  ecalj_status put(ecalj_ctx* h, const void* src, size_t bytes) {
-    ECALJ_TRY(alloc(bytes));
-    if (src && bytes) ECALJ_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, h->stream));
-    return ECALJ_OK;
-  }
-};
-
-static ecalj_status check_flag(ecalj_ctx* h) {
+ecalj_status check_flag(ecalj_ctx* h) {
   int f = 0;
   ECALJ_HIP(hipMemcpyAsync(&f, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   ECALJ_HIP(hipStreamSynchronize(h->stream));

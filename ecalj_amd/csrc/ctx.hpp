@@ -40,7 +40,7 @@ struct ecalj_ctx {
 
   // grow-only scratch slots
   enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
-         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_NSLOT };
+         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_GAM, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag
@@ -73,6 +73,25 @@ struct ecalj_ctx {
 namespace ecalj {
 
 ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out);
+// Device buffer owned by one host-pointer entry point (freed on return).
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  ecalj_status alloc(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    return hipMalloc(&p, bytes) == hipSuccess ? ECALJ_OK : ECALJ_ERR_NOMEM;
+  }
+  ecalj_status put(ecalj_ctx* h, const void* src, size_t bytes) {
+    ECALJ_TRY(alloc(bytes));
+    if (src && bytes) ECALJ_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, h->stream));
+    return ECALJ_OK;
+  }
+};
+// Waits for the stream; ECALJ_ERR_SINGULAR (and the flag cleared) when a
+// kernel met a zero pivot since the last check.
+ecalj_status check_flag(ecalj_ctx* h);
 // Copies a host table to scratch slot `slot` on the handle's stream (pinned
 // staging; capture-safe); *out is its device address.
 ecalj_status upload_blob(ecalj_ctx* h, int slot, const std::vector<unsigned char>& v, unsigned char** out);

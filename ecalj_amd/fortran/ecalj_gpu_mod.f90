@@ -17,6 +17,8 @@ module ecalj_gpu
   public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
   public :: ecalj_tetwt, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
   public :: ecalj_wv_write_dev, ecalj_wv_read_dev
+  public :: ecalj_wc_build_gamma, ecalj_wc_build_gamma_dev, ecalj_eps_head, ecalj_eps_head_dev, ecalj_getw0
+  public :: ecalj_wv0_patch, ecalj_wv0_patch_dev
   integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
 
   interface
@@ -345,6 +347,63 @@ module ecalj_gpu
       integer(c_int), value :: ntq
       complex(c_double_complex), intent(in) :: sex2(*), sexcore2(*), sec(*), vxcmat(*)
       complex(c_double_complex), intent(out) :: se(*)
+    end function
+    ! q = 0 body of WVIllwI / WVRllwR with ix = 1 (S/m_llw.f90:98-178, 299-373); W in the E basis
+    integer(c_int) function ecalj_wc_build_gamma(h, chi, ngb, nw, nspin, is_x0_m_basis, real_axis, wc, ldw) &
+        bind(C, name="ecalj_wc_build_gamma")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: chi(*)
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, real_axis, ldw
+      complex(c_double_complex), intent(inout) :: wc(*)
+    end function
+    integer(c_int) function ecalj_wc_build_gamma_dev(h, chi, ngb, nw, nspin, is_x0_m_basis, real_axis, wc, ldw) &
+        bind(C, name="ecalj_wc_build_gamma_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, chi, wc
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, real_axis, ldw
+    end function
+    ! offset points q0i: llw = 1/epsinv(1,1) and the wing (S/m_llw.f90:183-256, 378-440)
+    integer(c_int) function ecalj_eps_head(h, chi, ngb, nw, nspin, is_x0_m_basis, vcou1, real_axis, llw, wing_iw, &
+        wing) bind(C, name="ecalj_eps_head")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(in) :: chi(*)
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, real_axis, wing_iw
+      real(c_double), value :: vcou1
+      complex(c_double_complex), intent(out) :: llw(*), wing(*)
+    end function
+    integer(c_int) function ecalj_eps_head_dev(h, chi, ngb, nw, nspin, is_x0_m_basis, vcou1, real_axis, llw, &
+        wing_iw, wing) bind(C, name="ecalj_eps_head_dev")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h, chi, llw, wing
+      integer(c_int), value :: ngb, nw, nspin, is_x0_m_basis, real_axis, wing_iw
+      real(c_double), value :: vcou1
+    end function
+    ! getw0 (S/m_w0w0i.f90:146-311), host, no handle
+    integer(c_int) function ecalj_getw0(ii, ie, nq0i, lmxax, llw, dmlx, epinvq0i, wklm, wbz, epinv, w0, llmat) &
+        bind(C, name="ecalj_getw0")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: ii, ie, nq0i, lmxax
+      complex(c_double_complex), intent(in) :: llw(*)
+      real(c_double), intent(in) :: dmlx(*), epinvq0i(*), wklm(*), epinv(*)
+      real(c_double), value :: wbz
+      complex(c_double_complex), intent(out) :: w0(*), llmat(*)
+    end function
+    ! modifyWV0 on records in memory (S/m_w0w0i.f90:89-140)
+    integer(c_int) function ecalj_wv0_patch(h, w, ngb, ldw, nw, w0, is_wc_m_basis) bind(C, name="ecalj_wv0_patch")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(inout) :: w(*)
+      integer(c_int), value :: ngb, ldw, nw, is_wc_m_basis
+      complex(c_double_complex), intent(in) :: w0(*)
+    end function
+    integer(c_int) function ecalj_wv0_patch_dev(h, w, ngb, ldw, nw, w0, is_wc_m_basis) &
+        bind(C, name="ecalj_wv0_patch_dev")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h, w
+      integer(c_int), value :: ngb, ldw, nw, is_wc_m_basis
+      complex(c_double_complex), intent(in) :: w0(*)
     end function
   end interface
 end module ecalj_gpu

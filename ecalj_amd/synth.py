@@ -608,3 +608,113 @@ def wc_phases(cfg: Config, seed_offset: int = 2000) -> np.ndarray:
     ph = rng.uniform(0.0, 2.0 * math.pi, (cfg.ni, cfg.nk))
     ph[:, 0] = 0.0
     return np.asfortranarray(np.exp(1j * ph))
+
+
+# --------------------------------------------------------------------------
+# Gamma cell: offset directions q0i with their invariant tensors
+def real_ylm(lmax: int, v: np.ndarray) -> np.ndarray:
+    """Normalized real harmonics Y_L(v/|v|), L = l*l + l + m (0-based): m > 0 ~
+    cos(m phi), m < 0 ~ sin(|m| phi), no Condon-Shortley sign (the convention
+    of sylm/sylmnc, S/sylm.f90:1-121).  v: (..., 3) -> (..., (lmax+1)^2)."""
+    v = np.asarray(v, np.float64)
+    r = np.linalg.norm(v, axis=-1)
+    ct = v[..., 2] / r
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    ph = np.arctan2(v[..., 1], v[..., 0])
+    out = np.zeros(v.shape[:-1] + ((lmax + 1) ** 2,))
+    for m in range(lmax + 1):
+        pmm = np.ones_like(ct)
+        for k in range(1, m + 1):
+            pmm = pmm * (2 * k - 1) * st
+        p2, p1 = None, pmm
+        for l in range(m, lmax + 1):
+            if l == m:
+                p = pmm
+            elif l == m + 1:
+                p = ct * (2 * m + 1) * pmm
+            else:
+                p = ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m)
+            if l > m:
+                p2, p1 = p1, p
+            nrm = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - m) / math.factorial(l + m))
+            c = l * l + l
+            if m == 0:
+                out[..., c] = nrm * p
+            else:
+                out[..., c + m] = math.sqrt(2.0) * nrm * p * np.cos(m * ph)
+                out[..., c - m] = math.sqrt(2.0) * nrm * p * np.sin(m * ph)
+    return out
+
+
+def sphere_rule(n: int = 24):
+    """Product rule on the unit sphere: points (N, 3) and weights (N,), exact
+    for polynomials of degree < min(2n, 2n)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    ph = 2 * np.pi * np.arange(2 * n) / (2 * n)
+    ct, p = np.meshgrid(x, ph, indexing="ij")
+    st = np.sqrt(1 - ct ** 2)
+    pts = np.stack([st * np.cos(p), st * np.sin(p), ct], -1).reshape(-1, 3)
+    wts = (w[:, None] * np.full(2 * n, 2 * np.pi / (2 * n))[None, :]).ravel()
+    return pts, wts
+
+
+@dataclass
+class GammaCell:
+    q0i: np.ndarray        # (3, nq0i) offset directions (Cartesian, 2 pi/a)
+    epinv: np.ndarray      # (3, 3, nq0i) invariant tensors
+    epinvq0i: np.ndarray   # (nq0i, nq0i): <q0i_i| epinv_j |q0i_i> / |q0i_i|^2
+    dmlx: np.ndarray       # (nq0i, 9): <e|epinv_i|e> = sum_L dmlx(i, L) Y_L(e)
+    wklm: np.ndarray       # ((lmxax+1)^2,) Gamma-cell weights
+    wbz: float             # BZ weight of q = 0
+    lmxax: int
+
+
+def gamma_cell(seed: int = 7, lmxax: int = 4, wbz: float = 1.0 / 512, aniso: bool = True) -> GammaCell:
+    """Six offset directions and their invariant tensors without symmetry, as
+    getallq0p / diele_invariant / m_q0p build them (S/m_q0p.f90:225-296,
+    S/q0irre.f90:151-266): q0x = the three reciprocal axes (a slightly
+    distorted cubic lattice) and three pair combinations; epinv_ = q q^T
+    normalized, Gram-Schmidt orthonormalized in the Frobenius product;
+    epinvq0i(i, j) = e_i . epinv_j . e_i; dmlx(i, L) = int e.epinv_i.e Y_L by
+    quadrature.  wklm: positive l = 0 weight, small even l > 0 components
+    (zero when aniso=False)."""
+    rng = np.random.default_rng(seed)
+    qlat = np.eye(3) + 0.08 * rng.standard_normal((3, 3))
+    q0x = [qlat[:, i] for i in range(3)]
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        u, v = q0x[a], q0x[b]
+        q0x.append((u - v) / 2 if np.sum((u - v) ** 2) < np.sum((u + v) ** 2) else (u + v) / 2)
+    q0x = np.array(q0x).T * 0.01
+    basis = []
+    for i in range(6):
+        e = q0x[:, i] / np.linalg.norm(q0x[:, i])
+        t = np.outer(e, e)
+        t = t / np.linalg.norm(t)
+        for b in basis:
+            t = t - np.sum(t * b) * b
+        nt = np.linalg.norm(t)
+        assert nt > 1e-6
+        basis.append(t / nt)
+    epinv = np.asfortranarray(np.stack(basis, -1))
+    nq0i = 6
+    epq = np.zeros((nq0i, nq0i))
+    for i in range(nq0i):
+        e = q0x[:, i] / np.linalg.norm(q0x[:, i])
+        for j in range(nq0i):
+            epq[i, j] = e @ epinv[:, :, j] @ e
+    pts, wts = sphere_rule(12)
+    Y = real_ylm(2, pts)
+    dmlx = np.zeros((nq0i, 9))
+    for i in range(nq0i):
+        f = np.einsum("pa,ab,pb->p", pts, epinv[:, :, i], pts)
+        dmlx[i] = (wts * f) @ Y
+    nlk = (lmxax + 1) ** 2
+    wklm = np.zeros(nlk)
+    wklm[0] = 0.35 * wbz
+    if aniso:
+        for lm in range(1, nlk):
+            l = int(math.isqrt(lm))
+            if l % 2 == 0:
+                wklm[lm] = 0.02 * wbz * rng.standard_normal() / l
+    return GammaCell(np.asfortranarray(q0x), epinv, np.asfortranarray(epq), np.asfortranarray(dmlx),
+                     wklm, float(wbz), lmxax)

@@ -314,6 +314,44 @@ ecalj_status ecalj_zminv(ecalj_handle h, double* a, int n, int lda);
 ecalj_status ecalj_zminv_dev(ecalj_handle h, void* a, int n, int lda, int nbatch,
                              int64_t stride);
 
+/* ---- the Gamma point of the W build ------------------------------------
+ * q = 0 body (iq = 1) of WVIllwI / WVRllwR (S/m_llw.f90:98-178, 299-373 with
+ * ix = 1): epsilon restricted to mu, nu >= 2 is inverted (partial pivoting
+ * when real_axis), W(1,:) = W(:,1) = 0, and W stays in the E basis (the
+ * E -> M step skips iq = 1; modifyWV0 applies it).  Same arguments as
+ * ecalj_wc_build otherwise; the Coulomb state is that of q = 0 (vcousq(1)
+ * is not used). */
+ecalj_status ecalj_wc_build_gamma(ecalj_handle h, const double* chi, int ngb, int nw, int nspin,
+                                  int is_x0_m_basis, int real_axis, double* wc, int ldw);
+ecalj_status ecalj_wc_build_gamma_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin,
+                                      int is_x0_m_basis, int real_axis, void* wc, int ldw);
+/* Offset points q0i near Gamma (iq > nqibz) of WVRllwR / WVIllwI
+ * (S/m_llw.f90:183-256, 378-440): epsilon with the head 1 - vcou1*chi(1,1)
+ * (vcou1 = 4 pi/|q|^2, :185, :380), full inverse (pivoted when real_axis),
+ * llw(iw) = 1/epsinv(1,1) (complex, nw); when 0 <= wing_iw < nw and wing is
+ * given, wing(mu-1) = epsinv(1,mu)/epsinv(1,1), mu = 2..ngb, at that
+ * frequency (wmuk, :238-244).  The Coulomb state is that of q0i. */
+ecalj_status ecalj_eps_head(ecalj_handle h, const double* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
+                            double vcou1, int real_axis, double* llw, int wing_iw, double* wing);
+ecalj_status ecalj_eps_head_dev(ecalj_handle h, const void* chi, int ngb, int nw, int nspin,
+                                int is_x0_m_basis, double vcou1, int real_axis, void* llw, int wing_iw,
+                                void* wing);
+/* getw0 (S/m_w0w0i.f90:146-311), host FP64, no handle: the Gamma-cell
+ * average of W - v, w0(ii:ie), from llw(ii:ie, nq0i) (complex, leading
+ * dimension ie-ii+1), dmlx(nq0i, 9), epinvq0i(nq0i, nq0i),
+ * wklm((lmxax+1)^2), wbz = wbz(1), epinv(3, 3, nq0i); llmat(3, 3) complex
+ * (iw = 0) when ii <= 0 and llmat is given.  lmxax <= 12. */
+ecalj_status ecalj_getw0(int ii, int ie, int nq0i, int lmxax, const double* llw, const double* dmlx,
+                         const double* epinvq0i, const double* wklm, double wbz, const double* epinv,
+                         double* w0, double* llmat);
+/* modifyWV0 (S/m_w0w0i.f90:89-140) on W records held in memory (ldw x ngb x
+ * nw, as written by ecalj_wc_build_gamma): W(1,1,iw) = w0(iw) (host array,
+ * complex nw), then W <- zcousq W zcousq^H when is_wc_m_basis. */
+ecalj_status ecalj_wv0_patch(ecalj_handle h, double* w, int ngb, int ldw, int nw, const double* w0,
+                             int is_wc_m_basis);
+ecalj_status ecalj_wv0_patch_dev(ecalj_handle h, void* w, int ngb, int ldw, int nw, const double* w0,
+                                 int is_wc_m_basis);
+
 /* ---- on-disk records shared with the Fortran steps ----------------------
  * __Vcoud.iq (S/main_hvccfp0.f90:396-401, read by S/m_readVcoud.f90:36-43):
  * sequential unformatted records ngb; q(3); vcoud(ngb); zcousq(ngb,ngb)

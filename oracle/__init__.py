@@ -483,3 +483,67 @@ def sigc_full(zmel_, wc, wgt, wr, freq_r, omega, ekc, ef, esmr, wkkr, nctot=0, n
     if ne < 0:
         raise ValueError("sigc_full: a pole frequency lies outside the real mesh")
     return zsec, ne
+
+
+# --------------------------------------------------------------------------
+# Gamma point (oracle/gamma.c)
+def sylm(r, lmx):
+    """sylm * sylmnc: normalized real harmonics at the unit vector of r."""
+    r = _cd(r)
+    n = (lmx + 1) ** 2
+    yl, cy = np.zeros(n), np.zeros(n)
+    lib().orc_sylm(_p(r), _p(yl), lmx)
+    lib().orc_sylmnc(_p(cy), lmx)
+    return yl * cy / np.linalg.norm(r) ** np.array([int(np.sqrt(i)) for i in range(n)])
+
+
+def gaunt(l12, l3):
+    """g[L1, L2, L3] = int Y_L1 Y_L2 Y_L3 (0-based L)."""
+    n12, n3 = (l12 + 1) ** 2, (l3 + 1) ** 2
+    g = np.zeros((n12, n12, n3), order="F")
+    if lib().orc_gaunt(l12, l3, _p(g)) != 0:
+        raise ValueError("orc_gaunt: l out of range")
+    return g
+
+
+def getw0(ii, llw, dmlx, epinvq0i, wklm, wbz, lmxax, epinv):
+    """getw0: llw (nw, nq0i) complex for frequencies ii..ii+nw-1."""
+    llw = _cz(llw)
+    nw, nq0i = llw.shape
+    w0 = np.zeros(nw, np.complex128)
+    llmat = np.zeros((3, 3), np.complex128, order="F")
+    rc = lib().orc_getw0(ii, ii + nw - 1, nq0i, lmxax, _p(llw), _p(_cd(dmlx)), _p(_cd(epinvq0i)), _p(_cd(wklm)),
+                         C.c_double(wbz), _p(_cd(epinv)), _p(w0), _p(llmat))
+    if rc != 0:
+        raise np.linalg.LinAlgError("orc_getw0: singular")
+    return w0, llmat
+
+
+def wc_build_gamma(chi, vcousq, m2e, nspin=1, is_x0_m_basis=True):
+    chi = _cz(chi)
+    ngb, _, nw = chi.shape
+    wc = np.zeros((ngb, ngb, nw), np.complex128, order="F")
+    rc = lib().orc_wc_build_gamma(ngb, nw, nspin, int(is_x0_m_basis), _p(chi), _p(_cd(vcousq)), _p(_cz(m2e)),
+                                  _p(wc), ngb)
+    if rc != 0:
+        raise np.linalg.LinAlgError("orc_wc_build_gamma: singular epsilon")
+    return wc
+
+
+def eps_head(chi, vcousq, m2e, vcou1, nspin=1, is_x0_m_basis=True, wing_iw=-1):
+    chi = _cz(chi)
+    ngb, _, nw = chi.shape
+    llw = np.zeros(nw, np.complex128)
+    wing = np.zeros(max(ngb - 1, 1), np.complex128)
+    rc = lib().orc_eps_head(ngb, nw, nspin, int(is_x0_m_basis), _p(chi), _p(_cd(vcousq)), _p(_cz(m2e)),
+                            C.c_double(vcou1), _p(llw), wing_iw, _p(wing))
+    if rc != 0:
+        raise np.linalg.LinAlgError("orc_eps_head: singular epsilon")
+    return llw, wing[:ngb - 1]
+
+
+def wv0_patch(w, w0, m2e, is_wc_m_basis=True):
+    w = _cz(w).copy(order="F")
+    ngb, _, nw = w.shape
+    lib().orc_wv0_patch(ngb, nw, _p(w), ngb, _p(_cz(w0)), int(is_wc_m_basis), _p(_cz(m2e)))
+    return w

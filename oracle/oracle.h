@@ -35,6 +35,16 @@
  *                      tetrahedron, every occupied/unoccupied cut figure
  *   orc_tetwt          brute-force pair sets; degenerate-band equality;
  *                      Monte Carlo of the whole-mesh histogram
+ *   orc_sylm/sylmnc    oracle/_ref (S/sylm.f90 compiled with flang)
+ *   orc_gaunt          oracle/_ref scg (S/scg.f90 compiled with flang) via
+ *                      the table layout of rotcg; sympy real_gaunt
+ *   orc_getw0          isotropic closed form wklm(1)(4pi)^1.5/wbz (1/eps-1);
+ *                      Galerkin residual of 1/L(q) by dense quadrature of the
+ *                      quadratic form q.epinv.q itself; llmat sum
+ *   orc_eps_head       diagonal chi (llw = the no-LFC head); Schur complement
+ *                      eps11 - eps1r eps_rr^-1 eps_r1 by numpy.linalg.solve
+ *   orc_wc_build_gamma numpy.linalg.inv of the mu,nu >= 2 block
+ *   orc_wv0_patch      W(1,1) set; unitary E->M by numpy
  */
 #ifndef ECALJ_ORACLE_H
 #define ECALJ_ORACLE_H
@@ -192,6 +202,32 @@ int orc_lindtet6(const double kvec[4][3], const double ea[4], const double eb[4]
                  double efermia, double efermib, const double* frhis, int nwhis, double* wtthis);
 int orc_hisrange(const double* frhis, int nwhis, double demin, double demax, int* ihw, int* nhw);
 int orc_chkdgn(const double* ene, int ndat, int* ixini, int* ixend);
+
+/* Gamma point (oracle/gamma.c).  sylm/sylmnc (S/sylm.f90:1-121): lm index
+ * 0-based in yl / cy.  orc_gaunt: g[L1 + n12*(L2 + n12*L3)] = int Y_L1 Y_L2
+ * Y_L3 for L1, L2 < n12 = (l12+1)^2, L3 < (l3+1)^2 (rotcg with the identity,
+ * S/rotcg.f90:54-69). */
+void orc_sylm(const double* r, double* yl, int lmx);
+void orc_sylmnc(double* cy, int lmx);
+int orc_gaunt(int l12, int l3, double* g);
+/* getw0 (S/m_w0w0i.f90:146-311): llw(ii:ie, nq0i), dmlx(nq0i,9),
+ * epinvq0i(nq0i,nq0i), wklm((lmxax+1)^2), epinv(3,3,nq0i) -> w0(ii:ie),
+ * llmat(3,3) at iw = 0 when ii <= 0.  Returns 0 or 2 (singular). */
+int orc_getw0(int ii, int ie, int nq0i, int lmxax, const double complex* llw, const double* dmlx,
+              const double* epinvq0i, const double* wklm, double wbz, const double* epinv,
+              double complex* w0, double complex* llmat);
+/* q = 0 body of the W build (S/m_llw.f90:98-178, 299-373 with ix = 1): W in
+ * the E basis, row and column 1 zero.  wc: ldw x ngb x nw. */
+int orc_wc_build_gamma(int ngb, int nw, int nspin, int is_x0_m_basis, const double complex* chi,
+                       const double* vcousq, const double complex* m2e, double complex* wc, int ldw);
+/* Offset points q0i (S/m_llw.f90:183-256, 378-440): llw(iw) = 1/epsinv(1,1)
+ * with the head 1 - vcou1 chi(1,1); wing(0..ngb-2) = epsinv(1,2:)/epsinv(1,1)
+ * at frequency wing_iw (skipped when out of range). */
+int orc_eps_head(int ngb, int nw, int nspin, int is_x0_m_basis, const double complex* chi, const double* vcousq,
+                 const double complex* m2e, double vcou1, double complex* llw, int wing_iw, double complex* wing);
+/* modifyWV0 (S/m_w0w0i.f90:89-140): W(1,1,iw) = w0(iw), then E -> M. */
+void orc_wv0_patch(int ngb, int nw, double complex* w, int ldw, const double complex* w0, int is_wc_m_basis,
+                   const double complex* m2e);
 
 /* Learned chi0-tail MLP (oracle/mlp.c): L layers of widths dims[0..L],
  * silu between layers, linear output; params W_l (row-major out x in) then
