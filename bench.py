@@ -347,6 +347,58 @@ def bench_ipw(mixed, log_=None):
     return out
 
 
+def bench_rotwave(log_=None):
+    """Eigenfunctions at 66 points (3 irreducible + 21 rotations of each) from
+    the 3 irreducible ones (readcphi + readgeig): 180 bands, 34 muffin-tin rows
+    with l <= 3 on two atoms, plane-wave lists of |q+G| < 4.6 (up to 414 G).  Device
+    time per batch (inputs resident, host index maps included), bytes moved
+    (read + write of the rotated blocks), and the FP64 oracle on one target."""
+    import oracle as orc
+    from ecalj_amd import Handle, synth
+
+    orc.build()
+    c = synth.rotation_case(seed=4, nqi=3, ntarget=21, qcut=4.6)
+    nk, nband = c.qtt.shape[1], 180
+    rng = np.random.default_rng(1)
+    cphi = (rng.standard_normal((c.ndima, nband, c.nqi)) + 1j * rng.standard_normal((c.ndima, nband, c.nqi)))
+    geig = (rng.standard_normal((c.ngpmx, nband, c.nqi)) + 1j * rng.standard_normal((c.ngpmx, nband, c.nqi)))
+    iq = np.arange(1, nk + 1)
+    h = Handle(0, "f64")
+    row = {"targets": nk, "nband": nband, "ndima": c.ndima, "ngpmx": c.ngpmx}
+    try:
+        h.set_hamindex(c)
+        h.set_mnla(c.ibas_tbl, c.l_tbl, c.k_tbl, c.offset_tbl)
+        dc, dg = h.to_dev(np.asfortranarray(cphi)), h.to_dev(np.asfortranarray(geig))
+        oc, og = h.array((c.ndima, nband, nk)), h.array((c.ngpmx, nband, nk))
+        for _ in range(2):
+            h.readcphi_dev(iq, 1, nband, dc, c.ndima, oc)
+            h.readgeig_dev(iq, 1, nband, dg, c.ngpmx, og)
+        h.sync()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            h.readcphi_dev(iq, 1, nband, dc, c.ndima, oc)
+            h.readgeig_dev(iq, 1, nband, dg, c.ngpmx, og)
+        h.sync()
+        dt = (time.perf_counter() - t0) / reps
+        moved = 2 * 16 * nband * nk * (c.ndima + c.ngpmx) + 16 * nband * nk * c.ngpmx  # + zero fill of geig
+        row["ms_per_batch"] = round(dt * 1e3, 4)
+        row["bytes_per_batch"] = moved
+        row["GB_per_s"] = round(moved / dt / 1e9, 1)
+        got_c, got_g = oc.download(), og.download()
+    finally:
+        h.close()
+    t0 = time.perf_counter()
+    wc = orc.rotmto(c, nk, cphi[:, :, c.iqimap[nk - 1] - 1])
+    wg = orc.rotipw(c, nk, np.asfortranarray(geig[:c.ngp[c.iqmap[nk - 1] - 1], :, c.iqimap[nk - 1] - 1]))
+    row["cpu_oracle_ms_per_target"] = round((time.perf_counter() - t0) * 1e3, 3)
+    row["relerr_vs_oracle"] = float(max(np.abs(got_c[:, :, -1] - wc).max() / np.abs(wc).max(),
+                                        np.abs(got_g[:wg.shape[0], :, -1] - wg).max() / np.abs(wg).max()))
+    if log_:
+        log_(f"bench[ROT]: {row}")
+    return row
+
+
 def bench_gamma(log_=None):
     """The Gamma point of the W build at C2 sizes (nI = 600): W(q=0) on the 16
     imaginary and the nw+1 real frequencies of the C2 real mesh (mu = 1 left
@@ -474,7 +526,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM",
+    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM,ROT",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -513,6 +565,8 @@ def main():
                 res[name] = bench_ipw(args.mixed, log_=log)
             elif name == "GAM":
                 res[name] = bench_gamma(log_=log)
+            elif name == "ROT":
+                res[name] = bench_rotwave(log_=log)
             elif name == "C4":
                 res[name] = bench_c4(args.mixed, log_=log)
             elif name == "C5":
@@ -589,6 +643,9 @@ def main():
             continue
         if name == "GAM":
             out["also"]["GAM"] = bench_gamma(log_=log)
+            continue
+        if name == "ROT":
+            out["also"]["ROT"] = bench_rotwave(log_=log)
             continue
         if name.endswith("R") and name[:-1] in synth.CONFIGS:
             out["also"][name] = bench_real_axis(args.mixed, name[:-1], log_=log)

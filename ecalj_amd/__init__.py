@@ -90,6 +90,14 @@ _SIGS = {
     "ecalj_getw0": [_i, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _p],
     "ecalj_wv0_patch": [_p, _p, _i, _i, _i, _p, _i],
     "ecalj_wv0_patch_dev": [_p, _p, _i, _i, _i, _p, _i],
+    "ecalj_set_hamindex": [_p, _i, _p, _p, _p, _p, _p, _i, _i, _p, _p, _i, _p, _p, _i, _p, _p, _i, _p, _p, _p],
+    "ecalj_set_mnla": [_p, _i, _p, _p, _p, _p],
+    "ecalj_readcphi_dev": [_p, _i, _p, _i, _i, _p, _i, _p],
+    "ecalj_readgeig_dev": [_p, _i, _p, _i, _i, _p, _i, _p],
+    "ecalj_mnla_read": [C.c_char_p, _i, _p, _p, _p, _p, _p],
+    "ecalj_qgpsi_read": [C.c_char_p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p],
+    "ecalj_evalue_read": [C.c_char_p, _p, _l, _p, _p],
+    "ecalj_rec_read_dev": [_p, C.c_char_p, _l, _i, _i, _i, _l, _p, _l],
     "ecalj_sigc_weights": [_p, _i, _i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p, _p, _p],
     "ecalj_sigc_weights_dev": [_p, _i, _i, _i, _i, _i, _i, _p, _p, _d, _d, _p, _p, _p, _p, _p],
     "ecalj_sigc": [_p, _p, _i, _i, _i, _i, _p, _i, _p, _p, _i],
@@ -147,6 +155,50 @@ def getw0(ii, llw, dmlx, epinvq0i, wklm, wbz, lmxax, epinv):
     _chk(lib().ecalj_getw0(ii, ii + nw - 1, nq0i, lmxax, _np_ptr(L), *[_np_ptr(a) for a in arrs[:3]], float(wbz),
                            _np_ptr(arrs[3]), _np_ptr(w0), _np_ptr(llmat)), "getw0")
     return w0, llmat
+
+
+def mnla_read(path):
+    """@MNLA_CPHI tables: (ibas_tbl, l_tbl, k_tbl, offset_tbl)."""
+    n = C.c_int(0)
+    rc = lib().ecalj_mnla_read(str(path).encode(), 0, C.byref(n), None, None, None, None)
+    if rc not in (0, 1):
+        _chk(rc, "mnla_read")
+    t = [np.zeros(n.value, np.int32) for _ in range(4)]
+    _chk(lib().ecalj_mnla_read(str(path).encode(), n.value, C.byref(n), *[_np_ptr(a) for a in t]), "mnla_read")
+    return tuple(t)
+
+
+def qgpsi_read(path):
+    """__QGpsi: dict(nqnum, ngpmx, nqbz, nqi, imx, nqibz, qpgcut, qtt, ngp, irr, ngvecp, ngvecprev)."""
+    h6 = np.zeros(6, np.int32)
+    cut = C.c_double(0.0)
+    rc = lib().ecalj_qgpsi_read(str(path).encode(), _np_ptr(h6), C.byref(cut), 0, 0, 0, None, None, None, None, None)
+    if rc not in (0, 1):
+        _chk(rc, "qgpsi_read")
+    nq, ngpmx, _, _, imx, _ = (int(x) for x in h6)
+    nc = 2 * imx + 1
+    qtt = np.zeros((3, nq), order="F")
+    ngp, irr = np.zeros(nq, np.int32), np.zeros(nq, np.int32)
+    gv = np.zeros((3, ngpmx, nq), np.int32, order="F")
+    gr = np.zeros((nc, nc, nc, nq), np.int32, order="F")
+    _chk(lib().ecalj_qgpsi_read(str(path).encode(), _np_ptr(h6), C.byref(cut), nq, ngpmx, imx, _np_ptr(qtt),
+                                _np_ptr(ngp), _np_ptr(irr), _np_ptr(gv), _np_ptr(gr)), "qgpsi_read")
+    keys = ("nqnum", "ngpmx", "nqbz", "nqi", "imx", "nqibz")
+    return dict(zip(keys, (int(x) for x in h6)), qpgcut=cut.value, qtt=qtt, ngp=ngp, irr=irr, ngvecp=gv,
+                ngvecprev=gr)
+
+
+def evalue_read(path):
+    """__EValue: dict(nband, nqi, nsp, nspc, nspx, qtti, evud)."""
+    h5 = np.zeros(5, np.int32)
+    rc = lib().ecalj_evalue_read(str(path).encode(), _np_ptr(h5), 0, None, None)
+    if rc not in (0, 1):
+        _chk(rc, "evalue_read")
+    nband, nqi, nsp, nspc, nspx = (int(x) for x in h5)
+    qtti = np.zeros((3, nqi), order="F")
+    ev = np.zeros((nband, nqi, nspx), order="F")
+    _chk(lib().ecalj_evalue_read(str(path).encode(), _np_ptr(h5), ev.size, _np_ptr(qtti), _np_ptr(ev)), "evalue_read")
+    return dict(nband=nband, nqi=nqi, nsp=nsp, nspc=nspc, nspx=nspx, qtti=qtti, evud=ev)
 
 
 def mlp_nparam(dims) -> int:
@@ -637,6 +689,41 @@ class Handle:
         z0 = np.ascontiguousarray(w0, np.complex128)
         _chk(lib().ecalj_wv0_patch_dev(self._h, _t_ptr(w), ngb, ngb if ldw is None else ldw, nw, _np_ptr(z0),
                                        int(is_wc_m_basis)), "wv0_patch_dev")
+
+    # ---- eigenfunctions on the full mesh
+    def set_hamindex(self, case):
+        """m_hamindex state from a synth.RotationCase-like object."""
+        c = case
+        i32 = lambda a: np.asfortranarray(a, np.int32)
+        f64 = lambda a: np.asfortranarray(a, np.float64)
+        self._rot_keep = [f64(c.qtt), i32(c.iqmap), i32(c.igmap), i32(c.iqimap), i32(c.ngp), i32(c.ngvecp),
+                          i32(c.ngvecprev), f64(c.symops), f64(c.shtvg), i32(c.miat), f64(c.tiat), f64(c.dlmm),
+                          f64(c.plat), f64(c.qlat)]
+        qtt, iqm, igm, iqim, ngp, gv, gr, so, sh, mi, ti, dl, pl, ql = self._rot_keep
+        _chk(lib().ecalj_set_hamindex(self._h, qtt.shape[1], _np_ptr(qtt), _np_ptr(iqm), _np_ptr(igm), _np_ptr(iqim),
+                                      _np_ptr(ngp), int(c.ngpmx), int(c.imx), _np_ptr(gv), _np_ptr(gr), so.shape[2],
+                                      _np_ptr(so), _np_ptr(sh), mi.shape[0], _np_ptr(mi), _np_ptr(ti), int(c.lmxax),
+                                      _np_ptr(dl), _np_ptr(pl), _np_ptr(ql)), "set_hamindex")
+
+    def set_mnla(self, ibas_tbl, l_tbl, k_tbl, offset_tbl):
+        t = [np.ascontiguousarray(a, np.int32) for a in (ibas_tbl, l_tbl, k_tbl, offset_tbl)]
+        _chk(lib().ecalj_set_mnla(self._h, len(t[0]), *[_np_ptr(a) for a in t]), "set_mnla")
+
+    def readcphi_dev(self, iq, nspc, nband, cphi_irr, ldc, cphi_out):
+        q = np.ascontiguousarray(iq, np.int32)
+        _chk(lib().ecalj_readcphi_dev(self._h, len(q), _np_ptr(q), nspc, nband, _t_ptr(cphi_irr), ldc,
+                                      _t_ptr(cphi_out)), "readcphi_dev")
+
+    def readgeig_dev(self, iq, nspc, nband, geig_irr, ldg, geig_out):
+        q = np.ascontiguousarray(iq, np.int32)
+        _chk(lib().ecalj_readgeig_dev(self._h, len(q), _np_ptr(q), nspc, nband, _t_ptr(geig_irr), ldg,
+                                      _t_ptr(geig_out)), "readgeig_dev")
+
+    def rec_read_dev(self, path, recl, rec0, step, nrec, nbytes, dst, dst_stride=None):
+        """Fixed-length records (__CPHI / __GEIG) to device memory."""
+        _chk(lib().ecalj_rec_read_dev(self._h, str(path).encode(), int(recl), rec0, step, nrec, int(nbytes),
+                                      _t_ptr(dst), int(nbytes if dst_stride is None else dst_stride)),
+             "rec_read_dev")
 
     # ---- Sigma_c
     def sigc_weights(self, omega, ekc, esmr, wkkr, freqx, wwx, expa, ua=1.0, ns1=1, nctot=0):

@@ -299,6 +299,7 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
   for (void* p : {(void*)h->d_ggg, (void*)h->d_igggi, (void*)h->d_nvgcgp2, (void*)h->d_igcgp2i})
     if (p) (void)hipFree(p);
   if (h->d_flag) (void)hipFree(h->d_flag);
+  rot_free(h);
   for (auto& pool : h->ev_pool)
     for (auto& p : pool) {
       (void)hipEventDestroy(p.a);

@@ -7,6 +7,8 @@
 
 #include "common.hpp"
 
+struct RotState;  // eigenfunction-rotation tables (rotwave.hip)
+
 struct ecalj_ctx {
   int device = 0;
   int ncu = 256;                 // compute units of the device (persistent grids)
@@ -37,10 +39,12 @@ struct ecalj_ctx {
   int* d_igcgp2i = nullptr;
   int gcp_lo[3] = {0, 0, 0}, gcp_n[3] = {0, 0, 0};
   std::vector<int> h_igcgp2i;
+  // m_hamindex and @MNLA_CPHI state of readcphi / readgeig
+  RotState* rot = nullptr;
 
   // grow-only scratch slots
   enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
-         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_GAM, S_NSLOT };
+         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_GAM, S_ROT1, S_ROT2, S_ROT3, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag
@@ -73,6 +77,7 @@ struct ecalj_ctx {
 namespace ecalj {
 
 ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out);
+void rot_free(ecalj_ctx* h);
 // Device buffer owned by one host-pointer entry point (freed on return).
 struct DevBuf {
   void* p = nullptr;

@@ -19,6 +19,8 @@ module ecalj_gpu
   public :: ecalj_wv_write_dev, ecalj_wv_read_dev
   public :: ecalj_wc_build_gamma, ecalj_wc_build_gamma_dev, ecalj_eps_head, ecalj_eps_head_dev, ecalj_getw0
   public :: ecalj_wv0_patch, ecalj_wv0_patch_dev
+  public :: ecalj_set_hamindex, ecalj_set_mnla, ecalj_readcphi_dev, ecalj_readgeig_dev, ecalj_mnla_read
+  public :: ecalj_qgpsi_read, ecalj_evalue_read, ecalj_rec_read_dev
   integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
 
   interface
@@ -404,6 +406,68 @@ module ecalj_gpu
       type(c_ptr), value :: h, w
       integer(c_int), value :: ngb, ldw, nw, is_wc_m_basis
       complex(c_double_complex), intent(in) :: w0(*)
+    end function
+    ! m_hamindex state of readcphi / readgeig (S/readeigen.f90:11-12)
+    integer(c_int) function ecalj_set_hamindex(h, nqtt, qtt, iqmap, igmap, iqimap, ngp, ngpmx, imx, ngvecp, ngvecprev, &
+        ngrp, symops, shtvg, nbas, miat, tiat, lmxax, dlmm, plat, qlat) bind(C, name="ecalj_set_hamindex")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nqtt, ngpmx, imx, ngrp, nbas, lmxax
+      real(c_double), intent(in) :: qtt(*), symops(*), shtvg(*), tiat(*), dlmm(*), plat(*), qlat(*)
+      integer(c_int), intent(in) :: iqmap(*), igmap(*), iqimap(*), ngp(*), ngvecp(*), ngvecprev(*), miat(*)
+    end function
+    ! readmnla_cphi tables (S/readeigen.f90:487-521)
+    integer(c_int) function ecalj_set_mnla(h, norbt, ibas_tbl, l_tbl, k_tbl, offset_tbl) bind(C, name="ecalj_set_mnla")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: norbt
+      integer(c_int), intent(in) :: ibas_tbl(*), l_tbl(*), k_tbl(*), offset_tbl(*)
+    end function
+    ! readcphi / readgeig for a batch of target points (S/readeigen.f90:71-181)
+    integer(c_int) function ecalj_readcphi_dev(h, nk, iq, nspc, nband, cphi_irr, ldc, cphi_out) &
+        bind(C, name="ecalj_readcphi_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, cphi_irr, cphi_out
+      integer(c_int), value :: nk, nspc, nband, ldc
+      integer(c_int), intent(in) :: iq(*)
+    end function
+    integer(c_int) function ecalj_readgeig_dev(h, nk, iq, nspc, nband, geig_irr, ldg, geig_out) &
+        bind(C, name="ecalj_readgeig_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, geig_irr, geig_out
+      integer(c_int), value :: nk, nspc, nband, ldg
+      integer(c_int), intent(in) :: iq(*)
+    end function
+    ! readers of @MNLA_CPHI, __QGpsi, __EValue and the __CPHI / __GEIG records
+    integer(c_int) function ecalj_mnla_read(path, cap, norbt, ibas_tbl, l_tbl, k_tbl, offset_tbl) &
+        bind(C, name="ecalj_mnla_read")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: cap
+      integer(c_int), intent(out) :: norbt, ibas_tbl(*), l_tbl(*), k_tbl(*), offset_tbl(*)
+    end function
+    integer(c_int) function ecalj_qgpsi_read(path, hdr6, qpgcut, nq_cap, ngpmx_cap, imx_cap, qtt, ngp, irr, ngvecp, &
+        ngvecprev) bind(C, name="ecalj_qgpsi_read")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: hdr6(6), ngp(*), irr(*), ngvecp(*), ngvecprev(*)
+      real(c_double), intent(out) :: qpgcut, qtt(*)
+      integer(c_int), value :: nq_cap, ngpmx_cap, imx_cap
+    end function
+    integer(c_int) function ecalj_evalue_read(path, hdr5, cap, qtti, evud) bind(C, name="ecalj_evalue_read")
+      import :: c_int, c_char, c_double, c_int64_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: hdr5(5)
+      integer(c_int64_t), value :: cap
+      real(c_double), intent(out) :: qtti(*), evud(*)
+    end function
+    integer(c_int) function ecalj_rec_read_dev(h, path, recl, rec0, step, nrec, nbytes, dst, dst_stride) &
+        bind(C, name="ecalj_rec_read_dev")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: h, dst
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: recl, nbytes, dst_stride
+      integer(c_int), value :: rec0, step, nrec
     end function
   end interface
 end module ecalj_gpu

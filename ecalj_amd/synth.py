@@ -718,3 +718,155 @@ def gamma_cell(seed: int = 7, lmxax: int = 4, wbz: float = 1.0 / 512, aniso: boo
                 wklm[lm] = 0.02 * wbz * rng.standard_normal() / l
     return GammaCell(np.asfortranarray(q0x), epinv, np.asfortranarray(epq), np.asfortranarray(dmlx),
                      wklm, float(wbz), lmxax)
+
+
+# --------------------------------------------------------------------------
+# Eigenfunction rotation: a two-atom cubic crystal with its space group
+def cubic_point_group() -> np.ndarray:
+    """The 48 signed permutation matrices (O_h), identity first: (3, 3, 48)."""
+    import itertools
+
+    ops = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            g = np.zeros((3, 3))
+            for i in range(3):
+                g[i, perm[i]] = signs[i]
+            ops.append(g)
+    ops.sort(key=lambda g: (not np.allclose(g, np.eye(3)),))
+    return np.stack(ops, -1)
+
+
+def rotation_dlmm(g: np.ndarray, lmax: int) -> np.ndarray:
+    """D_l(M, L) = int Y_M(r) Y_L(g^-1 r) dOmega in the real harmonics of
+    real_ylm: the coefficients of the rotated function Y_L(g^-1 r).
+    Shape (2 lmax+1, 2 lmax+1, lmax+1), index m + lmax."""
+    pts, wts = sphere_rule(lmax + 2)
+    gi = np.linalg.inv(g)
+    Y = real_ylm(lmax, pts)
+    Yr = real_ylm(lmax, pts @ gi.T)
+    d = np.zeros((2 * lmax + 1, 2 * lmax + 1, lmax + 1))
+    for l in range(lmax + 1):
+        s = slice(l * l, l * l + 2 * l + 1)
+        blk = (Y[:, s] * wts[:, None]).T @ Yr[:, s]
+        d[lmax - l:lmax + l + 1, lmax - l:lmax + l + 1, l] = blk
+    return d
+
+
+@dataclass
+class RotationCase:
+    plat: np.ndarray
+    qlat: np.ndarray
+    pos: np.ndarray          # (3, nbas)
+    nbas: int
+    symops: np.ndarray       # (3, 3, ngrp)
+    shtvg: np.ndarray        # (3, ngrp)
+    miat: np.ndarray         # (nbas, ngrp), 1-based
+    tiat: np.ndarray         # (3, nbas, ngrp)
+    dlmm: np.ndarray         # (2 lmxax+1, 2 lmxax+1, lmxax+1, ngrp)
+    lmxax: int
+    ibas_tbl: np.ndarray     # @MNLA_CPHI tables (1-based ibas, l, k; offset = first row - 1)
+    l_tbl: np.ndarray
+    k_tbl: np.ndarray
+    offset_tbl: np.ndarray
+    ndima: int
+    qtt: np.ndarray          # (3, nqtt); the first nqi are the irreducible points
+    nqi: int
+    iqmap: np.ndarray        # (nqtt,) 1-based: the irreducible point each q is rotated from
+    igmap: np.ndarray        # (nqtt,) 1-based operation
+    iqimap: np.ndarray       # (nqtt,) 1-based index among the irreducible points
+    ngp: np.ndarray          # (nqtt,)
+    ngpmx: int
+    imx: int
+    ngvecp: np.ndarray       # (3, ngpmx, nqtt)
+    ngvecprev: np.ndarray    # (2 imx+1,)*3 + (nqtt,), 1-based, 9999 where absent
+    alpha: dict              # Gaussian exponent of each radial function (l, k)
+
+
+def rotation_case(seed: int = 3, nqi: int = 3, ntarget: int = 5, lmxax: int = 3, qcut: float = 2.3) -> RotationCase:
+    """Simple cubic lattice (a = 1) with identical atoms at (0,0,0) and
+    (1/2,1/2,1/2): the 48 cubic point operations with shift 0 and with shift
+    (1/2,1/2,1/2) map it to itself (miat, tiat: g tau_a + s = tau_b + t_a).
+    Orbitals per atom: (l,k) = (0,1) (0,2) (1,1) (2,1) (3,1), rows m = -l..l,
+    numbered as sugw writes @MNLA_CPHI.  qtt: nqi irreducible points, then for
+    each ntarget random operations with qtarget = g q + G0 (G0 folds into the
+    first cell, plus a random lattice vector).  G lists: |q+G| < qcut."""
+    rng = np.random.default_rng(seed)
+    plat = np.eye(3)
+    qlat = np.eye(3)
+    pos = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5]]).T
+    nbas = 2
+    pg = cubic_point_group()
+    ops, shifts = [], []
+    for s in (np.zeros(3), np.full(3, 0.5)):
+        for i in range(pg.shape[2]):
+            ops.append(pg[:, :, i])
+            shifts.append(s)
+    ngrp = len(ops)
+    symops = np.asfortranarray(np.stack(ops, -1))
+    shtvg = np.asfortranarray(np.stack(shifts, -1))
+    miat = np.zeros((nbas, ngrp), np.int32)
+    tiat = np.zeros((3, nbas, ngrp))
+    for ig in range(ngrp):
+        for a in range(nbas):
+            x = symops[:, :, ig] @ pos[:, a] + shtvg[:, ig]
+            for b in range(nbas):
+                t = x - pos[:, b]
+                if np.allclose(t, np.round(t), atol=1e-12):
+                    miat[a, ig] = b + 1
+                    tiat[:, a, ig] = np.round(t)
+                    break
+            assert miat[a, ig] > 0
+    dlmm = np.asfortranarray(np.stack([rotation_dlmm(symops[:, :, ig], lmxax) for ig in range(ngrp)], -1))
+    orbs = [(0, 1), (0, 2), (1, 1), (2, 1), (3, 1)]
+    ibas_tbl, l_tbl, k_tbl, off = [], [], [], []
+    row = 0
+    for a in range(nbas):
+        for l, k in sorted(orbs):
+            ibas_tbl.append(a + 1)
+            l_tbl.append(l)
+            k_tbl.append(k)
+            off.append(row)
+            row += 2 * l + 1
+    ndima = row
+    qi = rng.uniform(-0.5, 0.5, (3, nqi))
+    qtt = [qi[:, i] for i in range(nqi)]
+    iqmap, igmap, iqimap = list(range(1, nqi + 1)), [1] * nqi, list(range(1, nqi + 1))
+    for i in range(nqi):
+        for _ in range(ntarget):
+            ig = int(rng.integers(1, ngrp))
+            gq = symops[:, :, ig] @ qi[:, i]
+            g0 = -np.round(gq) + rng.integers(-1, 2, 3)
+            qtt.append(gq + qlat @ g0)
+            iqmap.append(i + 1)
+            igmap.append(ig + 1)
+            iqimap.append(i + 1)
+    qtt = np.asfortranarray(np.array(qtt).T)
+    nqtt = qtt.shape[1]
+    lists = []
+    r = int(np.ceil(qcut)) + 3
+    grid = np.array(np.meshgrid(*[np.arange(-r, r + 1)] * 3, indexing="ij")).reshape(3, -1).T
+    for iq in range(nqtt):
+        d2 = np.sum((qtt[:, iq][None, :] + grid @ qlat.T) ** 2, axis=1)
+        keep = d2 < qcut ** 2
+        gap = np.abs(d2 - qcut ** 2).min()
+        assert gap > 1e-9, "cutoff sits on a shell"
+        sel = grid[keep]
+        order = np.lexsort((sel[:, 2], sel[:, 1], sel[:, 0], np.round(d2[keep], 12)))
+        lists.append(sel[order])
+    ngp = np.array([len(x) for x in lists], np.int32)
+    ngpmx = int(ngp.max())
+    imx = int(max(np.abs(x).max() for x in lists))
+    nc = 2 * imx + 1
+    ngvecp = np.zeros((3, ngpmx, nqtt), np.int32, order="F")
+    ngvecprev = np.full((nc, nc, nc, nqtt), 9999, np.int32, order="F")
+    for iq, x in enumerate(lists):
+        ngvecp[:, :len(x), iq] = x.T
+        for ig, g in enumerate(x):
+            ngvecprev[g[0] + imx, g[1] + imx, g[2] + imx, iq] = ig + 1
+    alpha = {(l, k): 9.0 + 3.0 * k + l for l, k in orbs}
+    return RotationCase(plat, qlat, pos, nbas, symops, shtvg, miat, tiat, dlmm, lmxax,
+                        np.array(ibas_tbl, np.int32), np.array(l_tbl, np.int32), np.array(k_tbl, np.int32),
+                        np.array(off, np.int32), ndima, qtt, nqi, np.array(iqmap, np.int32),
+                        np.array(igmap, np.int32), np.array(iqimap, np.int32), ngp, ngpmx, imx, ngvecp, ngvecprev,
+                        alpha)

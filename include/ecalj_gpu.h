@@ -352,6 +352,54 @@ ecalj_status ecalj_wv0_patch(ecalj_handle h, double* w, int ngb, int ldw, int nw
 ecalj_status ecalj_wv0_patch_dev(ecalj_handle h, void* w, int ngb, int ldw, int nw, const double* w0,
                                  int is_wc_m_basis);
 
+/* ---- eigenfunctions on the full mesh (readcphi / readgeig) -------------
+ * m_hamindex state (S/readeigen.f90:11-12, from HAMindex0 and __QGpsi):
+ * qtt(3,nqtt); iqmap/igmap/iqimap(nqtt) 1-based (the point each q is
+ * rotated from, the operation, its index among the irreducible points);
+ * ngp(nqtt); ngvecp(3,ngpmx,nqtt) and ngvecprev((2imx+1)^3,nqtt) (1-based
+ * indices; may be NULL when ngpmx = 0); symops(3,3,ngrp), shtvg(3,ngrp),
+ * miat(nbas,ngrp) 1-based, tiat(3,nbas,ngrp), dlmm(-lmxax:lmxax,
+ * -lmxax:lmxax,0:lmxax,ngrp) (rotdlmm's real-harmonic D^l), plat, qlat. */
+ecalj_status ecalj_set_hamindex(ecalj_handle h, int nqtt, const double* qtt, const int* iqmap, const int* igmap,
+                                const int* iqimap, const int* ngp, int ngpmx, int imx, const int* ngvecp,
+                                const int* ngvecprev, int ngrp, const double* symops, const double* shtvg,
+                                int nbas, const int* miat, const double* tiat, int lmxax, const double* dlmm,
+                                const double* plat, const double* qlat);
+/* readmnla_cphi (S/readeigen.f90:487-521): per orbital block (ibas, l, k)
+ * 1-based and offset = first row - 1 of the block in cphi. */
+ecalj_status ecalj_set_mnla(ecalj_handle h, int norbt, const int* ibas_tbl, const int* l_tbl, const int* k_tbl,
+                            const int* offset_tbl);
+/* readcphi (S/readeigen.f90:140-181, rotmto S/rotwave.f90:156-192) for nk
+ * target points iq(nk) (host, 1-based into qtt) at once: cphi_irr holds the
+ * irreducible eigenvectors of one spin, ldc x nband per irreducible point
+ * (ldc >= ndima*nspc), cphi_out ldc x nband per target.  Device pointers. */
+ecalj_status ecalj_readcphi_dev(ecalj_handle h, int nk, const int* iq, int nspc, int nband, const void* cphi_irr,
+                                int ldc, void* cphi_out);
+/* readgeig (S/readeigen.f90:71-131, rotipw S/rotwave.f90:223-245): the same
+ * for the plane-wave coefficients, ldg >= ngpmx*nspc rows; rows without a
+ * rotated G are zero.  A rotated G outside the target's list, or ngp(iq) /=
+ * ngp(iqmap(iq)), returns ECALJ_ERR_DIM. */
+ecalj_status ecalj_readgeig_dev(ecalj_handle h, int nk, const int* iq, int nspc, int nband, const void* geig_irr,
+                                int ldg, void* geig_out);
+/* Readers of the front end's files (host, no handle except rec_read_dev).
+ * @MNLA_CPHI (S/sugw.f90:261-277): first call with cap = 0 reports norbt
+ * (ECALJ_ERR_DIM), then with room.
+ * __QGpsi (S/mkqg.f90:298,349-356): hdr6 = nqnum ngpmx nqbz nqi imx nqibz and
+ * QpGcut_psi; first call with capacities 0 reports them; then qtt(3,nqnum),
+ * ngp, irr (nqnum), ngvecp(3,ngpmx,nqnum), ngvecprev((2imx+1)^3,nqnum).
+ * __EValue (S/readeigen.f90:428-440): hdr5 = nband nqi nsp nspc nspx,
+ * qtti(3,nqi), evud(nband,nqi,nspx); cap = elements of evud.
+ * Fixed-length records (__CPHI recl mrecb, __GEIG recl mrecg,
+ * S/sugw.f90:234-236): records rec0, rec0+step, ..., the first nbytes of
+ * each, to device memory dst + i*dst_stride bytes. */
+ecalj_status ecalj_mnla_read(const char* path, int cap, int* norbt, int* ibas_tbl, int* l_tbl, int* k_tbl,
+                             int* offset_tbl);
+ecalj_status ecalj_qgpsi_read(const char* path, int* hdr6, double* qpgcut, int nq_cap, int ngpmx_cap, int imx_cap,
+                              double* qtt, int* ngp, int* irr, int* ngvecp, int* ngvecprev);
+ecalj_status ecalj_evalue_read(const char* path, int* hdr5, int64_t cap, double* qtti, double* evud);
+ecalj_status ecalj_rec_read_dev(ecalj_handle h, const char* path, int64_t recl, int rec0, int step, int nrec,
+                                int64_t nbytes, void* dst, int64_t dst_stride);
+
 /* ---- on-disk records shared with the Fortran steps ----------------------
  * __Vcoud.iq (S/main_hvccfp0.f90:396-401, read by S/m_readVcoud.f90:36-43):
  * sequential unformatted records ngb; q(3); vcoud(ngb); zcousq(ngb,ngb)

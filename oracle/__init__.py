@@ -547,3 +547,38 @@ def wv0_patch(w, w0, m2e, is_wc_m_basis=True):
     ngb, _, nw = w.shape
     lib().orc_wv0_patch(ngb, nw, _p(w), ngb, _p(_cz(w0)), int(is_wc_m_basis), _p(_cz(m2e)))
     return w
+
+
+# --------------------------------------------------------------------------
+# Eigenfunction rotation (oracle/rotwave.c)
+def rotmto(case, iq, cphi_in):
+    """rotmto for target iq (1-based into case.qtt): cphi_in ndima x nband at qtt(:, iqmap(iq))."""
+    c = case
+    iqq, igg = c.iqmap[iq - 1], c.igmap[iq - 1]
+    cin = _cz(cphi_in)
+    out = np.zeros_like(cin, order="F")
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    rc = lib().orc_rotmto(_p(_cd(c.qtt[:, iqq - 1])), cin.shape[0], cin.shape[1], len(c.ibas_tbl), _ip_(i32(c.ibas_tbl)),
+                          _ip_(i32(c.l_tbl)), _ip_(i32(c.k_tbl)), _ip_(i32(c.offset_tbl)),
+                          _p(_cd(c.symops[:, :, igg - 1])), _p(_cd(c.dlmm[:, :, :, igg - 1])), c.lmxax,
+                          _ip_(i32(c.miat[:, igg - 1])), _p(_cd(c.tiat[:, :, igg - 1])), c.nbas, _p(cin), _p(out))
+    if rc != 0:
+        raise ValueError("orc_rotmto: orbital tables do not map a block")
+    return out
+
+
+def rotipw(case, iq, geig_in):
+    """rotipw for target iq: geig_in ngp(iqq) x nband at qtt(:, iqmap(iq))."""
+    c = case
+    iqq, igg = c.iqmap[iq - 1], c.igmap[iq - 1]
+    gin = _cz(geig_in)
+    out = np.zeros_like(gin, order="F")
+    platt = np.asfortranarray(c.plat.T)
+    rc = lib().orc_rotipw(_p(_cd(c.qtt[:, iqq - 1])), _p(_cd(c.qtt[:, iq - 1])), int(c.ngp[iqq - 1]), gin.shape[1],
+                          _p(platt), _p(_cd(c.qlat)), _p(_cd(c.symops[:, :, igg - 1])),
+                          _ip_(np.asfortranarray(c.ngvecp[:, :, iqq - 1], np.int32)),
+                          _ip_(np.asfortranarray(c.ngvecprev[:, :, :, iq - 1], np.int32)), c.imx,
+                          _p(_cd(c.shtvg[:, igg - 1])), _p(gin), _p(out), C.c_long(gin.shape[0]))
+    if rc != 0:
+        raise ValueError("orc_rotipw: rotated G outside the target list")
+    return out

@@ -45,6 +45,10 @@
  *                      eps11 - eps1r eps_rr^-1 eps_r1 by numpy.linalg.solve
  *   orc_wc_build_gamma numpy.linalg.inv of the mu,nu >= 2 block
  *   orc_wv0_patch      W(1,1) set; unitary E->M by numpy
+ *   orc_rotmto/rotipw  the rotated Bloch function psi(g^-1 (r - s)) of a
+ *                      synthetic two-atom crystal evaluated in real space
+ *                      (muffin-tin Bloch sums, plane waves) against the one
+ *                      built from the rotated coefficients at g k + G0
  */
 #ifndef ECALJ_ORACLE_H
 #define ECALJ_ORACLE_H
@@ -228,6 +232,17 @@ int orc_eps_head(int ngb, int nw, int nspin, int is_x0_m_basis, const double com
 /* modifyWV0 (S/m_w0w0i.f90:89-140): W(1,1,iw) = w0(iw), then E -> M. */
 void orc_wv0_patch(int ngb, int nw, double complex* w, int ldw, const double complex* w0, int is_wc_m_basis,
                    const double complex* m2e);
+
+/* Eigenfunction rotation (oracle/rotwave.c): rotmto (S/rotwave.f90:156-192)
+ * with the @MNLA_CPHI tables (1-based ibas/l/k, offset = first row - 1), and
+ * rotipw (S/rotwave.f90:223-245).  sym, platt, qlat: 3x3 column-major;
+ * dlmm(-lmxax:lmxax, -lmxax:lmxax, 0:lmxax) of the operation. */
+int orc_rotmto(const double* qin, int ndima, int nband, int norbt, const int* ibas_tbl, const int* l_tbl,
+               const int* k_tbl, const int* offset_tbl, const double* sym, const double* dlmm, int lmxax,
+               const int* miat, const double* tiat, int nbas, const double complex* cin, double complex* cout);
+int orc_rotipw(const double* qin, const double* qtarget, int ngp, int nband, const double* platt, const double* qlat,
+               const double* sym, const int* ngvecp, const int* ngvecprev, int imx, const double* shtvg,
+               const double complex* gin, double complex* gout, long ldg);
 
 /* Learned chi0-tail MLP (oracle/mlp.c): L layers of widths dims[0..L],
  * silu between layers, linear output; params W_l (row-major out x in) then
