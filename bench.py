@@ -602,8 +602,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16x3": "bf16x3 split products, FP32 accumulation, complex64 storage; FP64 W build",
-                  "f32": "complex64, FP32 MFMA; FP64 W build"}[args.mixed],
+        "dtype": {"bf16x3": "bf16x3 split products, FP32 accumulation, complex64 storage; FP64 epsilon and its inverse",
+                  "f32": "complex64, FP32 MFMA; FP64 epsilon and its inverse"}[args.mixed],
         "data": "synthetic (seeded; see ecalj_amd/synth.py)",
         "config": {"workload": f"{cfg.name} q-point: nk={cfg.nk}, nb={cfg.nb}, occ={cfg.no}, unocc={cfg.nu}, "
                                f"nI={cfg.ni}, nf={cfg.nf}(+nu=0), ntq={cfg.nq}, nwhis={mix['nwhis']}",

@@ -951,9 +951,13 @@ ecalj_status ecalj_chi0_freq(ecalj_handle h, const double* z, const double* w, i
   return ECALJ_OK;
 }
 
-// ------------------------------------------------------------------ W build (FP64 in every form)
+// ------------------------------------------------------------------ W build
 // WVIllwI (imaginary axis, epsilon Hermitian positive definite: in-order
 // elimination) and WVRllwR (real axis, epsilon general: row exchanges).
+// FP64 form: every step in complex128.  Mixed forms: the reference's __MP
+// split (S/m_kind.f90:3-6, S/m_llw.f90:31-39,62-64): the M <-> E transforms
+// on the precision's GEMM core with complex64 chi_E and W_E (cmm_d), epsilon
+// and its inverse in complex128 (epstilde is complex(8) in every build).
 static ecalj_status wc_build_impl(ecalj_handle h, const void* chi, int ngb, int nw, int nspin, int is_x0_m_basis,
                                   int is_wc_m_basis, void* wc, int ldw, bool pivot) {
   if (!h || !chi || !wc) return ECALJ_ERR_ARG;
@@ -965,48 +969,69 @@ static ecalj_status wc_build_impl(ecalj_handle h, const void* chi, int ngb, int 
   cd *eps, *tmp;
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_EPS, sizeof(cd) * nn * nw, (void**)&eps));
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP, sizeof(cd) * nn * nw, (void**)&tmp));
+  const bool narrow = !wide(h);
+  cf* a32 = (cf*)tmp;       // complex64 halves of tmp (mixed forms)
+  cf* b32 = a32 + nn * nw;
+  const int mode = h->precision;  // the GEMM core of the form (FP64 for ECALJ_PREC_F64)
+  const void* Z = narrow ? (const void*)h->d_zcousq32 : (const void*)h->d_zcousq;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 3, &tev));
-  if (is_x0_m_basis) {  // chi_E = zcousq^H (chi zcousq) (S/m_llw.f90:336-337), eps fused
+  if (is_x0_m_basis) {  // chi_E = zcousq^H (chi zcousq) (S/m_llw.f90:336-337)
+    if (narrow) ECALJ_TRY(convert(h, chi, true, a32, false, nn * nw));
     GemmCall g1;
+    g1.mode = mode;
     g1.M = n; g1.N = n; g1.K = n;
-    g1.A = chi; g1.lda = n; g1.bA = {nn, 0};
-    g1.B = h->d_zcousq; g1.ldb = n;
-    g1.C = tmp; g1.ldc = n; g1.bC = {nn, 0};
+    g1.A = narrow ? (const void*)a32 : chi; g1.lda = n; g1.bA = {nn, 0};
+    g1.B = Z; g1.ldb = n;
+    g1.C = narrow ? (void*)b32 : (void*)tmp; g1.ldc = n; g1.bC = {nn, 0};
     g1.batch = nw; g1.div = nw;
     ECALJ_TRY(gemm(h, g1));
     GemmCall g2;
+    g2.mode = mode;
     g2.ta = true; g2.ca = true;
     g2.M = n; g2.N = n; g2.K = n;
-    g2.A = h->d_zcousq; g2.lda = n;
-    g2.B = tmp; g2.ldb = n; g2.bB = {nn, 0};
-    g2.C = eps; g2.ldc = n; g2.bC = {nn, 0};
+    g2.A = Z; g2.lda = n;
+    g2.B = narrow ? (const void*)b32 : (const void*)tmp; g2.ldb = n; g2.bB = {nn, 0};
+    g2.C = narrow ? (void*)a32 : (void*)eps; g2.ldc = n; g2.bC = {nn, 0};
     g2.batch = nw; g2.div = nw;
     g2.alpha = cd{fac, 0.0};
-    g2.epi = EPI_EPS; g2.svec = h->d_vcousq;
-    ECALJ_TRY(gemm(h, g2));
+    if (narrow) {
+      ECALJ_TRY(gemm(h, g2));
+      ECALJ_TRY(eps_narrow(h, a32, h->d_vcousq, n, nw, eps));  // epstilde in complex(8)
+    } else {
+      g2.epi = EPI_EPS; g2.svec = h->d_vcousq;  // eps fused into the epilogue
+      ECALJ_TRY(gemm(h, g2));
+    }
   } else {
     ECALJ_TRY(eps_direct(h, (const cd*)chi, h->d_vcousq, fac, n, nw, eps));
   }
   const int* perm = nullptr;
   ECALJ_TRY(gj_inverse(h, eps, n, n, nn, nw, pivot ? &perm : nullptr));  // zminv (S/m_llw.f90:143,350)
   if (is_wc_m_basis) {  // W_M = zcousq W_E zcousq^H (S/m_llw.f90:366-367)
-    ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, tmp, n, nn));
-    GemmCall g3;  // eps <- W_E zcousq^H
+    if (narrow) ECALJ_TRY(wc_form32(h, eps, perm, h->d_vcousq, n, nw, b32));
+    else ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, tmp, n, nn));
+    GemmCall g3;  // W_E zcousq^H
+    g3.mode = mode;
     g3.tb = true; g3.cb = true;
     g3.M = n; g3.N = n; g3.K = n;
-    g3.A = tmp; g3.lda = n; g3.bA = {nn, 0};
-    g3.B = h->d_zcousq; g3.ldb = n;
-    g3.C = eps; g3.ldc = n; g3.bC = {nn, 0};
+    g3.A = narrow ? (const void*)b32 : (const void*)tmp; g3.lda = n; g3.bA = {nn, 0};
+    g3.B = Z; g3.ldb = n;
+    g3.C = narrow ? (void*)a32 : (void*)eps; g3.ldc = n; g3.bC = {nn, 0};
     g3.batch = nw; g3.div = nw;
     ECALJ_TRY(gemm(h, g3));
-    GemmCall g4;  // wc <- zcousq * (W_E zcousq^H)
+    GemmCall g4;  // zcousq * (W_E zcousq^H)
+    g4.mode = mode;
     g4.M = n; g4.N = n; g4.K = n;
-    g4.A = h->d_zcousq; g4.lda = n;
-    g4.B = eps; g4.ldb = n; g4.bB = {nn, 0};
-    g4.C = (cd*)wc; g4.ldc = ldw; g4.bC = {(int64_t)ldw * n, 0};
+    g4.A = Z; g4.lda = n;
+    g4.B = narrow ? (const void*)a32 : (const void*)eps; g4.ldb = n; g4.bB = {nn, 0};
+    if (narrow) {
+      g4.C = b32; g4.ldc = n; g4.bC = {nn, 0};
+    } else {
+      g4.C = (cd*)wc; g4.ldc = ldw; g4.bC = {(int64_t)ldw * n, 0};
+    }
     g4.batch = nw; g4.div = nw;
     ECALJ_TRY(gemm(h, g4));
+    if (narrow) ECALJ_TRY(widen_ld(h, b32, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
   } else {
     ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
   }

@@ -376,6 +376,77 @@ __global__ void wc_form_kernel(const cd* __restrict__ inv, const int* __restrict
   }
 }
 
+// eps = delta - s_i chiE_ij s_j from a complex64 chi_E (already scaled), in FP64
+__global__ void eps_narrow_kernel(const cf* __restrict__ chi, const double* __restrict__ s, int n, int nw,
+                                  cd* __restrict__ eps) {
+  const int64_t tot = (int64_t)n * n * nw;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ij = e % ((int64_t)n * n);
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    const double sc = s[i] * s[j];
+    const cf x = chi[e];
+    eps[e] = cd{(i == j ? 1.0 : 0.0) - sc * (double)x.re, -sc * (double)x.im};
+  }
+}
+
+// wc_form into complex64, leading dimension n (the E -> M input of the mixed form)
+__global__ void wc_form32_kernel(const cd* __restrict__ inv, const int* __restrict__ perm,
+                                 const double* __restrict__ s, int n, int nw, cf* __restrict__ w) {
+  const int64_t nn = (int64_t)n * n, tot = nn * nw;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int iw = (int)(e / nn);
+    const int64_t ij = e % nn;
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    const double sc = s[i] * s[j];
+    const int jc = perm ? perm[j + (int64_t)n * iw] : j;
+    const cd x = inv[nn * iw + i + (int64_t)n * jc];
+    w[e] = cf{(float)(sc * x.re - (i == j ? sc : 0.0)), (float)(sc * x.im)};
+  }
+}
+
+// dst(i, j, iw) (leading dimension ldw, record stride wstride) = src(i, j, iw) (ld n)
+__global__ void widen_ld_kernel(const cf* __restrict__ src, int n, int nw, cd* __restrict__ dst, int64_t ldw,
+                                int64_t wstride) {
+  const int64_t nn = (int64_t)n * n, tot = nn * nw;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int iw = (int)(e / nn);
+    const int64_t ij = e % nn;
+    const int i = (int)(ij % n), j = (int)(ij / n);
+    const cf x = src[e];
+    dst[wstride * iw + i + ldw * j] = cd{(double)x.re, (double)x.im};
+  }
+}
+
+ecalj_status eps_narrow(ecalj_ctx* h, const cf* chi, const double* s, int n, int nw, cd* eps) {
+  const int64_t tot = (int64_t)n * n * nw;
+  if (tot == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(eps_narrow_kernel, dim3(blocks), dim3(256), 0, h->stream, chi, s, n, nw, eps);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+ecalj_status wc_form32(ecalj_ctx* h, const cd* inv, const int* perm, const double* s, int n, int nw, cf* w) {
+  const int64_t tot = (int64_t)n * n * nw;
+  if (tot == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(wc_form32_kernel, dim3(blocks), dim3(256), 0, h->stream, inv, perm, s, n, nw, w);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+ecalj_status widen_ld(ecalj_ctx* h, const cf* src, int n, int nw, cd* dst, int64_t ldw, int64_t wstride) {
+  const int64_t tot = (int64_t)n * n * nw;
+  if (tot == 0) return ECALJ_OK;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(widen_ld_kernel, dim3(blocks), dim3(256), 0, h->stream, src, n, nw, dst, ldw, wstride);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
 ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac, int n, int nw, cd* eps) {
   const int64_t tot = (int64_t)n * n * nw;
   if (tot == 0) return ECALJ_OK;

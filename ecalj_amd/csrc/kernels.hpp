@@ -18,6 +18,11 @@ ecalj_status convert(ecalj_ctx* h, const void* src, bool src64, void* dst, bool 
 ecalj_status eps_direct(ecalj_ctx* h, const cd* chi, const double* s, double fac, int n, int nw, cd* eps);
 ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const int* perm, const double* s, int n, int nw, cd* w,
                      int64_t ldw, int64_t wstride);
+// Mixed form of the W build (the reference's __MP split): chi_E and W_E in
+// complex64 around the FP64 epsilon and its inverse.
+ecalj_status eps_narrow(ecalj_ctx* h, const cf* chi, const double* s, int n, int nw, cd* eps);
+ecalj_status wc_form32(ecalj_ctx* h, const cd* inv, const int* perm, const double* s, int n, int nw, cf* w);
+ecalj_status widen_ld(ecalj_ctx* h, const cf* src, int n, int nw, cd* dst, int64_t ldw, int64_t wstride);
 // In-place inverse of nmat matrices.  perm == nullptr: no row exchanges (the
 // matrix must be safe to eliminate in order, e.g. Hermitian positive
 // definite).  Otherwise partial pivoting: on return column j of inv(A) is

@@ -141,3 +141,35 @@ def test_qpoint_mixed_qp_energy_within_1meV(orc, name, prec):
     say(f"  max |dEqp| = {err_mev:.4f} meV, chi relerr {relerr(chi, ref.chi):.2e}, W relerr {relerr(wc, ref.wc):.2e}")
     assert err_mev < 1.0
     assert relerr(chi, ref.chi) < TOL[prec]
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_wc_build_mixed(hp, orc, real):
+    """The reference's __MP split of the W build: M <-> E transforms on the
+    precision's GEMM core (complex64 chi_E, W_E), epsilon and its inverse in
+    FP64.  Bar: the storage rounding of chi_E and W_E times cond(eps)."""
+    h, prec = hp
+    n, nw = 150, 3
+    say("wc_build mixed", prec, n, nw, real)
+    rng = np.random.default_rng(31)
+    vc = rng.uniform(0.5, 40.0, n)
+    Q, _ = np.linalg.qr(crand(rng, n, n))
+    Q = np.asfortranarray(Q)
+    chi = np.zeros((n, n, nw), complex, order="F")
+    for i in range(nw):
+        X = crand(rng, n, 40) * 0.01 / (1 + i)
+        chi[:, :, i] = -(X @ X.conj().T)
+        if real:
+            chi[:, :, i] += 0.3j * (X[:, :20] @ X[:, :20].conj().T)
+    h.set_coulomb(vc, Q)
+    for wm in (True, False):
+        g = (h.wc_build_real if real else h.wc_build)(chi, nspin=1, is_x0_m_basis=True, is_wc_m_basis=wm)
+        o = orc.wc_build(chi, np.sqrt(vc), Q, nspin=1, is_x0_m_basis=True, is_wc_m_basis=wm)
+        kap = 1.0
+        s = np.sqrt(vc)
+        for i in range(nw):
+            e = np.eye(n) - s[:, None] * (Q.conj().T @ (2 * chi[:, :, i]) @ Q) * s[None, :]
+            kap = max(kap, np.linalg.cond(e))
+        err = relerr(g, o)
+        say(f"  wm={wm} relerr {err:.2e} cond {kap:.1f}")
+        assert err < TOL[prec] * kap

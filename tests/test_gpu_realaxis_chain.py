@@ -223,9 +223,9 @@ def test_wc_build_real_zero_leading_pivot(h, orc):
 def test_chain_rcxq_to_sigma_real(orc, prec):
     """rcxq -> chi(omega) -> W_c(omega) -> real-axis Sigma_c pole term, each
     step on the device, against the same chain in the oracle.  For complex64
-    storage the oracle gets the same rounded rcxq: everything after it runs in
-    FP64 in both.  Near a zero of eps(omega) (the plasmon) cond(eps) is large;
-    the bar scales with it."""
+    storage the oracle gets the same rounded rcxq; the device then keeps chi_E
+    and W_E in complex64 around the FP64 inverse.  Near a zero of eps(omega)
+    (the plasmon) cond(eps) is large; the bars scale with it."""
     import ecalj_amd
 
     n, nm, ntq = 48, 13, 4
@@ -275,11 +275,23 @@ def test_chain_rcxq_to_sigma_real(orc, prec):
     e_chi = relerr(chi_g, chi_o)
     assert e_chi < 1e-10
     w_o2 = orc.wc_build(chi_g, np.sqrt(vc), Q, nspin=1, is_x0_m_basis=True, is_wc_m_basis=True)
-    assert relerr(w_g, w_o2) < tol
+    kappa = tol / 1e-15
+    if prec == "f64":
+        assert relerr(w_g, w_o2) < tol
+    else:
+        # the complex64 form stores chi_E and W_E in complex64 around the FP64
+        # inverse (the reference's __MP split): per frequency the error is the
+        # FP32 rounding (2^-24, with a 16x margin for the K-long FP32 sums)
+        # amplified by that frequency's cond(eps)
+        s = np.sqrt(vc)
+        for i in range(nw + 1):
+            x = Q.conj().T @ (2 * chi_g[:, :, i]) @ Q
+            k_i = np.linalg.cond(np.eye(n) - s[:, None] * x * s[None, :])
+            assert relerr(w_g[:, :, i], w_o2[:, :, i]) < max(1e-6, 1e-6 * k_i), i
     s_o2, _ = orc.sigc_real(z, w_g, freqr, omega, ekc, ef, esmr, wk)
     assert relerr(s_g, s_o2) < 1e-10
     # end to end: the chi difference (summation order over ~300 bins) is
     # amplified by cond(eps) near its zero
-    kappa = tol / 1e-15
-    assert relerr(w_g, w_o) < max(1e-10, 10 * kappa * e_chi)
-    assert relerr(s_g, s_o) < max(1e-10, 10 * kappa * e_chi)
+    e_w = 10 * kappa * e_chi if prec == "f64" else 1e-6 * kappa
+    assert relerr(w_g, w_o) < max(1e-10, e_w)
+    assert relerr(s_g, s_o) < max(1e-10, e_w)
