@@ -976,7 +976,18 @@ static ecalj_status wc_build_impl(ecalj_handle h, const void* chi, int ngb, int 
   const void* Z = narrow ? (const void*)h->d_zcousq32 : (const void*)h->d_zcousq;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 3, &tev));
-  if (is_x0_m_basis) {  // chi_E = zcousq^H (chi zcousq) (S/m_llw.f90:336-337)
+  // bf16x3 form: the four transforms on the pre-split bf16x3 core of the Sigma_c kernel
+  const bool q3 = h->precision == ECALJ_PREC_BF16X3;
+  auto opnd = [&](const void* p, bool w, int64_t stride, bool herm) {
+    QOperand o;
+    o.p = p; o.ld = n; o.stride = stride; o.wide = w; o.herm = herm;
+    return o;
+  };
+  if (is_x0_m_basis && q3) {  // chi_E = fac zcousq^H (chi zcousq) (S/m_llw.f90:336-337)
+    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(chi, true, nn, false), opnd(h->d_zcousq, true, 0, false), 1.0, b32, nn));
+    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(h->d_zcousq, true, 0, true), opnd(b32, false, nn, false), fac, a32, nn));
+    ECALJ_TRY(eps_narrow(h, a32, h->d_vcousq, n, nw, eps));  // epstilde in complex(8)
+  } else if (is_x0_m_basis) {
     if (narrow) ECALJ_TRY(convert(h, chi, true, a32, false, nn * nw));
     GemmCall g1;
     g1.mode = mode;
@@ -1007,7 +1018,12 @@ static ecalj_status wc_build_impl(ecalj_handle h, const void* chi, int ngb, int 
   }
   const int* perm = nullptr;
   ECALJ_TRY(gj_inverse(h, eps, n, n, nn, nw, pivot ? &perm : nullptr));  // zminv (S/m_llw.f90:143,350)
-  if (is_wc_m_basis) {  // W_M = zcousq W_E zcousq^H (S/m_llw.f90:366-367)
+  if (is_wc_m_basis && q3) {  // W_M = zcousq (W_E zcousq^H) (S/m_llw.f90:366-367)
+    ECALJ_TRY(wc_form32(h, eps, perm, h->d_vcousq, n, nw, b32));
+    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(b32, false, nn, false), opnd(h->d_zcousq, true, 0, true), 1.0, a32, nn));
+    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(h->d_zcousq, true, 0, false), opnd(a32, false, nn, false), 1.0, b32, nn));
+    ECALJ_TRY(widen_ld(h, b32, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
+  } else if (is_wc_m_basis) {
     if (narrow) ECALJ_TRY(wc_form32(h, eps, perm, h->d_vcousq, n, nw, b32));
     else ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, tmp, n, nn));
     GemmCall g3;  // W_E zcousq^H

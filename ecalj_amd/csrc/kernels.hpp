@@ -71,5 +71,16 @@ ecalj_status real_plan_into_cz(ecalj_ctx* h, const RealPlanHandle* r, const Real
 ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm, const void* zmel, int n, int ntq,
                                   int niw, const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz);
 size_t sig_bf3_a_elems_per_k(int n, int nw1);
+// Batched complex GEMM on the bf16x3 core of the Sigma_c kernel:
+// C_b = alpha op(A_b) op(B_b) (m x ncol, leading dimension m, stride c_stride,
+// complex64).  An operand is complex128 (wide) or complex64 with leading
+// dimension ld and batch stride (0: shared); herm: the operand is X^H.
+struct QOperand {
+  const void* p = nullptr;
+  int64_t ld = 0, stride = 0;
+  bool wide = false, herm = false;
+};
+ecalj_status qgemm_bf3(ecalj_ctx* h, int nb, int m, int ncol, int k, const QOperand& A, const QOperand& B,
+                       double alpha, cf* C, int64_t c_stride);
 
 }  // namespace ecalj

@@ -571,6 +571,139 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
 
 size_t sig_bf3_a_elems_per_k(int n, int nw1) { return 4 * (size_t)nw1 * pad_to(n, QBM) * pad_to(n, QBK); }
 
+// ---------------------------------------------------------------- plain batched GEMM on the same core
+// C_b (m x ncol, leading dimension m, complex64) = alpha * op(A_b) op(B_b),
+// K = k: the operands are split into the pre-tiled bf16 hi/lo planes of the
+// Sigma_c kernel (one frequency, weights alpha) and the Sigma_c kernel runs.
+// op(X) = X, or X^H when the flag is set.  A stride / B stride 0: one operand
+// shared by every batch (split once).
+
+// A planes [b][((ks*4 + p)*MP + I)*32 + slot]; one thread per (b, ks, chunk, I), I fastest
+template <class TIN>
+__global__ __launch_bounds__(256) void split_a_kernel(const TIN* __restrict__ a, int64_t lda, int64_t a_stride,
+                                                      int m, int k, int herm, int MP, int NS,
+                                                      unsigned short* __restrict__ as, int64_t as_stride, int nb) {
+  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+  const int64_t per = (int64_t)NS * 4 * MP, tot = per * nb;
+  const int64_t pst = (int64_t)MP * QBK;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e % per;
+    const int I = (int)(r % MP);
+    const int ch = (int)((r / MP) % 4), ks = (int)(r / (4 * (int64_t)MP));
+    us8v rh, rl, ih, il;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int J = ks * QBK + 8 * ch + q;
+      float xr = 0.f, xi = 0.f;
+      if (I < m && J < k) {
+        const TIN x = herm ? a[a_stride * b + J + lda * I] : a[a_stride * b + I + lda * J];
+        xr = (float)x.re;
+        xi = herm ? -(float)x.im : (float)x.im;
+      }
+      unsigned short h, l;
+      split_bf16(xr, h, l);
+      rh[q] = h;
+      rl[q] = l;
+      split_bf16(xi, h, l);
+      ih[q] = h;
+      il[q] = l;
+    }
+    unsigned short* A = as + as_stride * b + (int64_t)ks * 4 * pst + (int64_t)I * QBK + 8 * chunk_slot(I, ch);
+    *reinterpret_cast<us8v*>(A) = rh;
+    *reinterpret_cast<us8v*>(A + pst) = rl;
+    *reinterpret_cast<us8v*>(A + 2 * pst) = ih;
+    *reinterpret_cast<us8v*>(A + 3 * pst) = il;
+  }
+}
+
+// B planes [b][((ks*4 + p)*NCP + c)*32 + slot]; one thread per (b, ks, chunk, c), c fastest
+template <class TIN>
+__global__ __launch_bounds__(256) void split_b_kernel(const TIN* __restrict__ bm, int64_t ldb, int64_t b_stride,
+                                                      int k, int ncol, int herm, int NCP, int NS,
+                                                      unsigned short* __restrict__ bs, int64_t bs_stride, int nb) {
+  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+  const int64_t per = (int64_t)NS * 4 * NCP, tot = per * nb;
+  const int64_t pst = (int64_t)NCP * QBK;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e % per;
+    const int c = (int)(r % NCP);
+    const int ch = (int)((r / NCP) % 4), ks = (int)(r / (4 * (int64_t)NCP));
+    us8v rh, rl, ih, il;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int J = ks * QBK + 8 * ch + q;
+      float xr = 0.f, xi = 0.f;
+      if (c < ncol && J < k) {
+        const TIN x = herm ? bm[b_stride * b + c + ldb * J] : bm[b_stride * b + J + ldb * c];
+        xr = (float)x.re;
+        xi = herm ? -(float)x.im : (float)x.im;
+      }
+      unsigned short h, l;
+      split_bf16(xr, h, l);
+      rh[q] = h;
+      rl[q] = l;
+      split_bf16(xi, h, l);
+      ih[q] = h;
+      il[q] = l;
+    }
+    unsigned short* B = bs + bs_stride * b + (int64_t)ks * 4 * pst + (int64_t)c * QBK + 8 * chunk_slot(c, ch);
+    *reinterpret_cast<us8v*>(B) = rh;
+    *reinterpret_cast<us8v*>(B + pst) = rl;
+    *reinterpret_cast<us8v*>(B + 2 * pst) = ih;
+    *reinterpret_cast<us8v*>(B + 3 * pst) = il;
+  }
+}
+
+template <class TIN>
+static ecalj_status split_a(ecalj_ctx* h, const void* a, int64_t lda, int64_t a_stride, int m, int k, bool herm,
+                            int MP, int NS, unsigned short* as, int64_t as_stride, int nb) {
+  const int64_t tot = (int64_t)NS * 4 * MP * nb;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(split_a_kernel<TIN>, dim3(blocks), dim3(256), 0, h->stream, (const TIN*)a, lda, a_stride, m, k,
+                     herm ? 1 : 0, MP, NS, as, as_stride, nb);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+template <class TIN>
+static ecalj_status split_b(ecalj_ctx* h, const void* b, int64_t ldb, int64_t b_stride, int k, int ncol, bool herm,
+                            int NCP, int NS, unsigned short* bs, int64_t bs_stride, int nb) {
+  const int64_t tot = (int64_t)NS * 4 * NCP * nb;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(split_b_kernel<TIN>, dim3(blocks), dim3(256), 0, h->stream, (const TIN*)b, ldb, b_stride, k,
+                     ncol, herm ? 1 : 0, NCP, NS, bs, bs_stride, nb);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
+}
+
+ecalj_status qgemm_bf3(ecalj_ctx* h, int nb, int m, int ncol, int k, const QOperand& A, const QOperand& B,
+                       double alpha, cf* C, int64_t c_stride) {
+  if (nb < 1 || m < 1 || ncol < 1 || k < 1 || nb > 65535) return ECALJ_ERR_DIM;
+  if (!A.p || !B.p || !C) return ECALJ_ERR_ARG;
+  const int MP = pad_to(m, QBM), NP = pad_to(k, QBK), NCP = pad_to(ncol, QBN_MAX), NS = NP / QBK;
+  if (MP / QBM > 65535 || NCP / QBN_MAX > 65535) return ECALJ_ERR_DIM;
+  const int na = A.stride ? nb : 1, nbb = B.stride ? nb : 1;
+  const int64_t ape = 4LL * MP * NP, bpe = 4LL * NCP * NP;
+  unsigned short *as, *bs;
+  double* w;
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_QA, sizeof(unsigned short) * (size_t)ape * na, (void**)&as));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_QB, sizeof(unsigned short) * (size_t)bpe * nbb, (void**)&bs));
+  std::vector<unsigned char> wb(sizeof(double) * (size_t)ncol);
+  for (int c = 0; c < ncol; ++c) std::memcpy(wb.data() + sizeof(double) * c, &alpha, sizeof(double));
+  unsigned char* dw = nullptr;
+  ECALJ_TRY(upload_blob(h, ecalj_ctx::S_QW, wb, &dw));
+  w = (double*)dw;
+  if (A.wide) ECALJ_TRY(split_a<cd>(h, A.p, A.ld, A.stride, m, k, A.herm, MP, NS, as, ape, na));
+  else ECALJ_TRY(split_a<cf>(h, A.p, A.ld, A.stride, m, k, A.herm, MP, NS, as, ape, na));
+  if (B.wide) ECALJ_TRY(split_b<cd>(h, B.p, B.ld, B.stride, k, ncol, B.herm, NCP, NS, bs, bpe, nbb));
+  else ECALJ_TRY(split_b<cf>(h, B.p, B.ld, B.stride, k, ncol, B.herm, NCP, NS, bs, bpe, nbb));
+  // the 8-wave 128 x 128 variant (the tuner's choice at the Sigma_c shapes); the output has leading dimension m
+  return sig_launch_t<128, 2, 8>(h, nb, m, ncol, 1, MP, NS, NCP, as, A.stride ? ape : 0, bs, B.stride ? bpe : 0, w,
+                                 0, C, c_stride, false);
+}
+
 size_t sig_bf3_bytes_per_k(int n, int ncol, int nw1) {
   return 2 * (sig_bf3_a_elems_per_k(n, nw1) + 4 * (size_t)pad_to(ncol, QBN_MAX) * pad_to(n, QBK));
 }
