@@ -625,6 +625,130 @@ __global__ __launch_bounds__(256) void gj_panel_kernel(const cd* __restrict__ a,
   }
 }
 
+// Blocked LU with partial pivoting of the outer panel A[k0:n, K], |K| = b <= GJO,
+// on a copy W (leading dimension n, n*GJO per matrix): 32-column sub-panels
+// are factored in turn (gj_panel_lu_kernel), their exchanges and the U rows
+// carried to the columns on their right (gj_panel_fix_kernel), and the Schur
+// complement of those columns updated by a GEMM.  Only the exchanges are
+// kept (ipiv[k0+kk], absolute rows, in sequence); they are those of zgetrf on
+// the panel, so the in-order elimination of the exchanged block meets the
+// partial pivots.
+__global__ void gj_wcopy_kernel(const cd* __restrict__ a, int n, int64_t lda, int64_t astride, int k0, int b,
+                                int nmat, cd* __restrict__ w) {
+  const int m = n - k0;
+  const int64_t per = (int64_t)m * b, tot = per * nmat;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int mat = (int)(e / per);
+    const int64_t r = e % per;
+    const int i = (int)(r % m), c = (int)(r / m);
+    w[(int64_t)n * GJO * mat + i + (int64_t)n * c] = a[astride * mat + (k0 + i) + lda * (k0 + c)];
+  }
+}
+
+// LU with row exchanges of the sub-panel W[r0:m, c0:c0+b), b <= GJB, in place;
+// ipiv[k0+r0+kk] = k0 + (exchanged row of W).
+__global__ __launch_bounds__(256) void gj_panel_lu_kernel(cd* __restrict__ w, int n, int m, int r0, int c0, int b,
+                                                          int k0, int* __restrict__ ipiv, int* __restrict__ flag) {
+  __shared__ double red_v[4];
+  __shared__ int red_i[4];
+  __shared__ cd prow[GJB];
+  __shared__ int s_p;
+  const int mat = blockIdx.x;
+  cd* W = w + (int64_t)n * GJO * mat + r0 + (int64_t)n * c0;
+  int* P = ipiv + (int64_t)n * mat;
+  const int mm = m - r0;
+  const int tid = threadIdx.x;
+  for (int kk = 0; kk < b; ++kk) {
+    double bv = -1.0;
+    int bi = mm;
+    for (int i = kk + tid; i < mm; i += blockDim.x) {
+      const cd x = W[i + (int64_t)n * kk];
+      const double v = fabs(x.re) + fabs(x.im);
+      if (v > bv) { bv = v; bi = i; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { red_v[tid >> 6] = bv; red_i[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double v = red_v[0];
+      int i = red_i[0];
+      for (int wv = 1; wv < (int)(blockDim.x >> 6); ++wv)
+        if (red_v[wv] > v || (red_v[wv] == v && red_i[wv] < i)) { v = red_v[wv]; i = red_i[wv]; }
+      if (!(v > 1e-300) || i >= mm) {  // zero (or NaN) column: singular
+        *flag = 1;
+        i = kk;
+      }
+      s_p = i;
+      P[k0 + r0 + kk] = k0 + r0 + i;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (tid < b) {
+      const cd x = W[kk + (int64_t)n * tid], y = W[p + (int64_t)n * tid];
+      if (p != kk) {
+        W[kk + (int64_t)n * tid] = y;
+        W[p + (int64_t)n * tid] = x;
+      }
+      prow[tid] = (p != kk) ? y : x;
+    }
+    __syncthreads();
+    const cd piv = prow[kk];
+    const double mag = piv.re * piv.re + piv.im * piv.im;
+    if (mag > 0.0) {
+      const double dr = piv.re / mag, di = -piv.im / mag;
+      for (int i = kk + 1 + tid; i < mm; i += blockDim.x) {
+        const cd x = W[i + (int64_t)n * kk];
+        const cd l = cd{x.re * dr - x.im * di, x.re * di + x.im * dr};
+        W[i + (int64_t)n * kk] = l;
+        for (int j = kk + 1; j < b; ++j) {
+          const cd u = prow[j];
+          cd& y = W[i + (int64_t)n * j];
+          y = cd{y.re - (l.re * u.re - l.im * u.im), y.im - (l.re * u.im + l.im * u.re)};
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// For the panel columns j in [c0+b, bw): the sub-panel's exchanges, then
+// U12 = L11^-1 W[r0:r0+b, j] (L11 unit lower triangular).  One thread per (matrix, column).
+__global__ void gj_panel_fix_kernel(cd* __restrict__ w, int n, int r0, int c0, int b, int bw, int k0,
+                                    const int* __restrict__ ipiv, int nmat) {
+  const int ncol = bw - c0 - b;
+  const int64_t tot = (int64_t)ncol * nmat;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int mat = (int)(e / ncol), j = c0 + b + (int)(e % ncol);
+    cd* Wm = w + (int64_t)n * GJO * mat;
+    cd* col = Wm + (int64_t)n * j;
+    const int* P = ipiv + (int64_t)n * mat;
+    for (int kk = 0; kk < b; ++kk) {
+      const int r = r0 + kk, p = P[k0 + r] - k0;
+      if (p != r) {
+        const cd x = col[r];
+        col[r] = col[p];
+        col[p] = x;
+      }
+    }
+    const cd* L = Wm + r0 + (int64_t)n * c0;
+    for (int i = 1; i < b; ++i) {
+      double sr = col[r0 + i].re, si = col[r0 + i].im;
+      for (int kk = 0; kk < i; ++kk) {
+        const cd l = L[i + (int64_t)n * kk], u = col[r0 + kk];
+        sr -= l.re * u.re - l.im * u.im;
+        si -= l.re * u.im + l.im * u.re;
+      }
+      col[r0 + i] = cd{sr, si};
+    }
+  }
+}
+
 // Apply the exchanges of block K to whole rows (all n columns).
 __global__ __launch_bounds__(256) void gj_swap_kernel(cd* __restrict__ a, int n, int64_t lda, int64_t astride,
                                                       int k0, int b, const int* __restrict__ ipiv) {
@@ -779,19 +903,51 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
   if (pivot && n > 8192) return ECALJ_ERR_DIM;  // index tables of gj_perm_kernel live in LDS
   int* ipiv = nullptr;
   if (pivot) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJPIV, sizeof(int) * 2 * (size_t)n * nmat, (void**)&ipiv));
-  // Pivoted inverses stay one-level: choosing the exchanges of a GJO-wide
-  // block in one workgroup per matrix (gj_panel_kernel's sequential column
-  // steps) costs more than the wider trailing updates save.
-  if (n <= GJO || pivot) {
+  // Two levels above GJO: outer blocks of GJO columns.  With pivoting, the
+  // exchanges of an outer block come from a blocked LU of its panel (32-wide
+  // sub-panels on a copy), then the exchanged block is eliminated in order.
+  if (n <= GJO) {
     ECALJ_TRY(gj_inverse_1(h, a, n, lda, astride, nmat, ipiv));
   } else {
-    cd *D, *Cp, *R;
+    cd *D, *Cp, *R, *W = nullptr;
     ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJD, sizeof(cd) * GJO * GJO * (size_t)nmat, (void**)&D));
     ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJOC, sizeof(cd) * (size_t)n * GJO * nmat, (void**)&Cp));
     ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJR, sizeof(cd) * (size_t)n * GJO * nmat, (void**)&R));
+    if (pivot) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_GJW, sizeof(cd) * (size_t)n * GJO * nmat, (void**)&W));
     for (int k0 = 0; k0 < n; k0 += GJO) {
       const int b = std::min(GJO, n - k0);
       const int64_t tot = (int64_t)n * b * nmat;
+      if (pivot) {
+        const int m = n - k0;
+        const int64_t tw = (int64_t)m * b * nmat;
+        hipLaunchKernelGGL(gj_wcopy_kernel, dim3((unsigned)std::min<int64_t>((tw + 255) / 256, 65536)), dim3(256), 0,
+                           h->stream, a, n, lda, astride, k0, b, nmat, W);
+        ECALJ_HIP(hipGetLastError());
+        for (int c0 = 0; c0 < b; c0 += GJB) {
+          const int bs = std::min(GJB, b - c0);
+          hipLaunchKernelGGL(gj_panel_lu_kernel, dim3(nmat), dim3(256), 0, h->stream, W, n, m, c0, c0, bs, k0, ipiv,
+                             h->d_flag);
+          ECALJ_HIP(hipGetLastError());
+          const int right = b - c0 - bs;
+          if (right <= 0) continue;
+          const int64_t tf = (int64_t)right * nmat;
+          hipLaunchKernelGGL(gj_panel_fix_kernel, dim3((unsigned)std::min<int64_t>((tf + 255) / 256, 65536)),
+                             dim3(256), 0, h->stream, W, n, c0, c0, bs, b, k0, ipiv, nmat);
+          ECALJ_HIP(hipGetLastError());
+          if (m - c0 - bs <= 0) continue;
+          GemmCall u;  // W[c0+bs:m, c0+bs:b] -= W[c0+bs:m, c0:c0+bs] W[c0:c0+bs, c0+bs:b]
+          u.M = m - c0 - bs; u.N = right; u.K = bs;
+          u.A = W + (c0 + bs) + (int64_t)n * c0; u.lda = n; u.bA = {(int64_t)n * GJO, 0};
+          u.B = W + c0 + (int64_t)n * (c0 + bs); u.ldb = n; u.bB = {(int64_t)n * GJO, 0};
+          u.C = W + (c0 + bs) + (int64_t)n * (c0 + bs); u.ldc = n; u.bC = {(int64_t)n * GJO, 0};
+          u.batch = nmat; u.div = nmat;
+          u.alpha = cd{-1.0, 0.0}; u.beta = cd{1.0, 0.0};
+          ECALJ_TRY(gemm(h, u));
+        }
+        hipLaunchKernelGGL(gj_swap_kernel, dim3((n + 255) / 256, nmat), dim3(256), 0, h->stream, a, n, lda, astride,
+                           k0, b, ipiv);
+        ECALJ_HIP(hipGetLastError());
+      }
       hipLaunchKernelGGL(gj_outer_copy_kernel, dim3((unsigned)std::min<int64_t>((tot + 255) / 256, 65536)),
                          dim3(256), 0, h->stream, a, n, lda, astride, k0, b, nmat, D, Cp, R);
       ECALJ_HIP(hipGetLastError());
