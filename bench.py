@@ -118,7 +118,7 @@ def run_form(qp, precision, steps, warmup):
     log(f"bench[{precision}]: {steps} steps in {dt:.3f} s -> {dt / steps * 1e3:.1f} ms per q-point")
     info = {"s_per_q": dt / steps, "ktimes": ktimes, "eqp": eqp, "nwhis": g.nwhis, "tuning": tuning,
             "npb": int(sum(g.ix.iwend - g.ix.iwini + 1))}
-    g.h.close()
+    g.close()
     del g
     return info
 

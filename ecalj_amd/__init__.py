@@ -41,6 +41,7 @@ _SIGS = {
     "ecalj_set_precision": [_p, _i],
     "ecalj_set_stream": [_p, _p],
     "ecalj_sync": [_p],
+    "ecalj_stream_wait": [_p, _p],
     "ecalj_kernel_timing": [_p, _i],
     "ecalj_kernel_time": [_p, _i, C.POINTER(_d), C.POINTER(_l)],
     "ecalj_set_tuning": [_p, _i],
@@ -362,6 +363,10 @@ class Handle:
 
     def sync(self):
         _chk(lib().ecalj_sync(self._h), "sync")
+
+    def stream_wait(self, other: "Handle"):
+        """This handle's stream waits (on the device) for the work issued so far on other's."""
+        _chk(lib().ecalj_stream_wait(self._h, other._h), "stream_wait")
 
     def kernel_timing(self, on: bool):
         _chk(lib().ecalj_kernel_timing(self._h, int(on)), "kernel_timing")

@@ -305,6 +305,7 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);
     }
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
   return ECALJ_OK;
@@ -394,6 +395,17 @@ ecalj_status ecalj_set_precision(ecalj_handle h, int p) {
 ecalj_status ecalj_set_stream(ecalj_handle h, void* s) {
   if (!h) return ECALJ_ERR_ARG;
   h->stream = s ? static_cast<hipStream_t>(s) : h->own_stream;
+  return ECALJ_OK;
+}
+
+// h's stream waits (on the device) for the work issued so far on other's stream.
+ecalj_status ecalj_stream_wait(ecalj_handle h, ecalj_handle other) {
+  if (!h || !other) return ECALJ_ERR_ARG;
+  if (h->capturing || other->capturing) return ECALJ_ERR_STATE;
+  if (h->stream == other->stream) return ECALJ_OK;
+  if (!other->ev_join) ECALJ_HIP(hipEventCreateWithFlags(&other->ev_join, hipEventDisableTiming));
+  ECALJ_HIP(hipEventRecord(other->ev_join, other->stream));
+  ECALJ_HIP(hipStreamWaitEvent(h->stream, other->ev_join, 0));
   return ECALJ_OK;
 }
 

@@ -58,6 +58,7 @@ struct ecalj_ctx {
   // slots may not grow, uploads stage through fresh pinned buffers that the
   // graph keeps, kernel timing is off, the tuner uses cached choices only
   bool capturing = false;
+  hipEvent_t ev_join = nullptr;   // ecalj_stream_wait: last point of this stream another waited for
   bool timing_before_capture = false;
   std::vector<void*> cap_pinned;
 

@@ -7,7 +7,7 @@ module ecalj_gpu
   use, intrinsic :: iso_c_binding
   implicit none
   private
-  public :: ecalj_create, ecalj_destroy, ecalj_set_precision, ecalj_sync
+  public :: ecalj_create, ecalj_destroy, ecalj_set_precision, ecalj_sync, ecalj_stream_wait
   public :: ecalj_set_ppb, ecalj_set_coulomb, ecalj_zmel, ecalj_x0gemm, ecalj_dpsion_imag
   public :: ecalj_wc_build, ecalj_sigc_weights, ecalj_sigc, ecalj_qp_energy, ecalj_sigma_herm
   public :: ecalj_zmel_dev, ecalj_x0gemm_batched_dev, ecalj_dpsion_imag_dev, ecalj_wc_build_dev
@@ -42,6 +42,11 @@ module ecalj_gpu
     integer(c_int) function ecalj_sync(h) bind(C, name="ecalj_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: h
+    end function
+    ! no counterpart in the reference: h's stream waits for the work issued on other's
+    integer(c_int) function ecalj_stream_wait(h, other) bind(C, name="ecalj_stream_wait")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, other
     end function
     ! set_ppb (S/m_zmel.f90:110-145)
     integer(c_int) function ecalj_set_ppb(h, natom, nv, mdim, ppb) bind(C, name="ecalj_set_ppb")

@@ -35,6 +35,10 @@ class QPointGPU:
     """tail_model: a learned_tail.TailModel; chi0 then sums the unoccupied bands
     up to the model's nu_t and adds the predicted tail on the device.
 
+    overlap: Sigma_c's zmel and weights (independent of chi0 and W) run on a
+    second handle's stream while W_c(q) is built; the two streams join before
+    the contraction (``Handle.stream_wait``).  Same results, same kernels.
+
     real_axis: also build chi(omega) and W_c(omega) on the real mesh freq_r
     (dpsion_chiq real branch, WVRllwR) from the same rcxq, and add the
     real-axis pole term to Sigma_c (sxcf_scz_correlation, S/m_sxcf_gemm.f90:
@@ -42,7 +46,7 @@ class QPointGPU:
     q serves every internal k (the pole term reads a few frequencies each)."""
 
     def __init__(self, qp: synth.QPoint, precision: str = "f64", device: int = 0, tail_model=None,
-                 real_axis: bool = False):
+                 real_axis: bool = False, overlap: bool = False):
         self.qp = qp
         self.cfg = cfg = qp.cfg
         self.h = h = Handle(device, precision)
@@ -100,6 +104,10 @@ class QPointGPU:
             self.w_r = h.array((ni, ni, self.nwr + 1), c128)
             self.d_omega1 = qp.e_q[:ntq] / RY
             self.ekc_h = np.asfortranarray(qp.e_qmk / RY)
+        self.h2 = None
+        if overlap:
+            self.h2 = Handle(device, precision)
+            self.h2.set_ppb(cfg.nv_list, cfg.mdim_list, qp.ppb)
         self.tail = None
         if tail_model is not None:
             from .learned_tail import LearnedTail
@@ -139,14 +147,24 @@ class QPointGPU:
         self.h.phase_similarity(self.wc, cfg.ni, self.nw1, self.phase, cfg.nk, self.wk)
 
     # ---------------------------------------------------------------- Sigma_c
-    def sigc(self):
+    def sigc_prep(self, h=None):
+        """Sigma_c inputs that do not depend on W: zmel of the internal k and the weights."""
+        h = self.h if h is None else h
         cfg = self.cfg
-        ni, nb, nbt, ntq, nk = cfg.ni, cfg.nb, cfg.nbt, cfg.nq, cfg.nk
+        nb, nbt, ntq, nk = cfg.nb, cfg.nbt, cfg.nq, cfg.nk
         # zmel(I, n_middle, m_end): middle states at q-k, end states at q shared (S/m_sxcf_gemm.f90:395-399)
-        self.h.zmel_dev(self.c_qmk, self.c_q, self.zmel_sig, nm=nbt, nq=ntq, ndima=nb, nbatch=nk,
-                        is_m_basis=True, zmelconjg=False, cphiq_stride=0)
-        self.h.sigc_weights_dev(nk, cfg.nf, nbt, ntq, self.d_omega, self.d_ekc, cfg.esmr, cfg.ua, self.d_wk,
-                                self.d_x, self.d_wx, self.d_expa, self.wgt)
+        h.zmel_dev(self.c_qmk, self.c_q, self.zmel_sig, nm=nbt, nq=ntq, ndima=nb, nbatch=nk,
+                   is_m_basis=True, zmelconjg=False, cphiq_stride=0)
+        h.sigc_weights_dev(nk, cfg.nf, nbt, ntq, self.d_omega, self.d_ekc, cfg.esmr, cfg.ua, self.d_wk,
+                           self.d_x, self.d_wx, self.d_expa, self.wgt)
+
+    def sigc(self):
+        self.sigc_prep()
+        self.sigc_contract()
+
+    def sigc_contract(self):
+        cfg = self.cfg
+        ni, nbt, ntq, nk = cfg.ni, cfg.nbt, cfg.nq, cfg.nk
         self.zsec.zero()
         if self.real_axis:
             self.h.sigc_full_dev(nk, self.zmel_sig, ni, nbt, ntq, cfg.nf, self.wk, self.wgt, self.w_r, 0, self.nwr,
@@ -158,19 +176,40 @@ class QPointGPU:
     def step(self):
         """chi0(q) + W_c(q) + Sigma_c(q): the per-q-point unit of work."""
         self.chi0()
+        if self.h2 is None:
+            self.wbuild()
+            self.sigc()
+            return
+        # zmel/weights of Sigma_c beside the W build; the event after chi0 also
+        # orders them after the previous step's contraction, which read them
+        self.h2.stream_wait(self.h)
+        self.sigc_prep(self.h2)
         self.wbuild()
-        self.sigc()
+        self.h.stream_wait(self.h2)
+        self.sigc_contract()
 
     def capture_step(self):
-        """A HIP graph of step() (call after one eager step of this q-point).  The
-        host-side index lists are fixed at capture; a replay recomputes the
-        q-point from the device buffers, e.g. in the next QSGW iteration."""
+        """A HIP graph of step() on one stream (call after one eager step of this
+        q-point).  The host-side index lists are fixed at capture; a replay
+        recomputes the q-point from the device buffers, e.g. in the next QSGW
+        iteration."""
+        if self.h2 is not None:
+            self.h.stream_wait(self.h2)
         self.h.graph_begin()
         try:
-            self.step()
+            self.chi0()
+            self.wbuild()
+            self.sigc()
         finally:
             g = self.h.graph_end()
         return g
+
+    def close(self):
+        if self.h2 is not None:
+            self.h2.sync()
+            self.h2.close()
+            self.h2 = None
+        self.h.close()
 
     # ---------------------------------------------------------------- results
     def results(self):

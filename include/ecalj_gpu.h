@@ -55,6 +55,11 @@ ecalj_status ecalj_set_precision(ecalj_handle h, int precision);
 /* Use an external hipStream_t (e.g. the caller's framework stream); NULL restores the handle's own. */
 ecalj_status ecalj_set_stream(ecalj_handle h, void* hip_stream);
 ecalj_status ecalj_sync(ecalj_handle h);
+/* No counterpart in the reference (stream plumbing): h's stream waits on the
+ * device for the work issued so far on other's stream (an event, no host
+ * wait), so that independent steps of one q-point on two handles of the same
+ * device can overlap.  Not under graph capture (ECALJ_ERR_STATE). */
+ecalj_status ecalj_stream_wait(ecalj_handle h, ecalj_handle other);
 const char* ecalj_status_string(int status);
 /* Kernel timing with HIP events on the handle's stream: when enabled, the
  * launches of kernel class `kclass` are bracketed by events; ecalj_kernel_time
