@@ -399,14 +399,15 @@ def bench_rotwave(log_=None):
     return row
 
 
-def bench_gamma(log_=None):
+def bench_gamma(mixed, log_=None):
     """The Gamma point of the W build at C2 sizes (nI = 600): W(q=0) on the 16
     imaginary and the nw+1 real frequencies of the C2 real mesh (mu = 1 left
     out of the inversion), llw = 1/epsinv(1,1) at six offset points on both
     meshes (the wing at omega = 0), getw0 for both, and modifyWV0 with the
-    E -> M transform.  FP64 (the W build is FP64 in every form); inputs
-    resident; device time per Gamma point.  The FP64 oracle on a one-frequency
-    sample of each inversion kind, scaled to the same work."""
+    E -> M transform.  Both forms (the mixed one with the M <-> E transforms on
+    its core around the FP64 inverse); inputs resident; device time per Gamma
+    point.  The FP64 oracle on a one-frequency sample of each inversion kind,
+    scaled to the same work."""
     import oracle as orc
     from ecalj_amd import Handle, synth
 
@@ -436,46 +437,52 @@ def bench_gamma(log_=None):
         return out
 
     ci, cr = chi(niw, False), chi(nwr, True)
-    h = Handle(0, "f64")
     row = {"ngb": n, "niw": niw, "nw_real": nwr, "nq0i": 6}
-    try:
-        h.set_coulomb(vc, Q)
-        di, dr = h.to_dev(ci), h.to_dev(cr)
-        wi, wr = h.array((n, n, niw)), h.array((n, n, nwr))
-        li, lr = h.array((niw,)), h.array((nwr,))
-        wing = h.array((n - 1,))
-        llw_i = np.zeros((niw, 6), complex)
-        llw_r = np.zeros((nwr, 6), complex)
+    w0_forms = {}
+    for prec in ("f64", mixed):
+        h = Handle(0, prec)
+        try:
+            h.set_coulomb(vc, Q)
+            di, dr = h.to_dev(ci), h.to_dev(cr)
+            wi, wr = h.array((n, n, niw)), h.array((n, n, nwr))
+            li, lr = h.array((niw,)), h.array((nwr,))
+            wing = h.array((n - 1,))
+            llw_i = np.zeros((niw, 6), complex)
+            llw_r = np.zeros((nwr, 6), complex)
 
-        def step():
-            for k in range(6):  # one offset point each; same chi for the timing
-                h.eps_head_dev(di, n, niw, vcou1, li, real_axis=False)
-                h.eps_head_dev(dr, n, nwr, vcou1, lr, real_axis=True, wing_iw=0, wing=wing)
-            h.sync()
-            llw_i[:] = li.download()[:, None]
-            llw_r[:] = lr.download()[:, None]
-            w0i, _ = ecalj_getw0(1, llw_i, gc)
-            w0r, _ = ecalj_getw0(0, llw_r, gc)
-            h.wc_build_gamma_dev(di, n, niw, wi, real_axis=False)
-            h.wc_build_gamma_dev(dr, n, nwr, wr, real_axis=True)
-            h.wv0_patch_dev(wi, n, niw, w0i)
-            h.wv0_patch_dev(wr, n, nwr, w0r)
-            h.sync()
+            def step():
+                for k in range(6):  # one offset point each; same chi for the timing
+                    h.eps_head_dev(di, n, niw, vcou1, li, real_axis=False)
+                    h.eps_head_dev(dr, n, nwr, vcou1, lr, real_axis=True, wing_iw=0, wing=wing)
+                h.sync()
+                llw_i[:] = li.download()[:, None]
+                llw_r[:] = lr.download()[:, None]
+                w0i, _ = ecalj_getw0(1, llw_i, gc)
+                w0r, _ = ecalj_getw0(0, llw_r, gc)
+                h.wc_build_gamma_dev(di, n, niw, wi, real_axis=False)
+                h.wc_build_gamma_dev(dr, n, nwr, wr, real_axis=True)
+                h.wv0_patch_dev(wi, n, niw, w0i)
+                h.wv0_patch_dev(wr, n, nwr, w0r)
+                h.sync()
+                return w0i, w0r
 
-        step()
-        t0 = time.perf_counter()
-        step()
-        row["s_per_gamma_point"] = round(time.perf_counter() - t0, 5)
-        for name, fn in (("eps_head_imag_ms", lambda: h.eps_head_dev(di, n, niw, vcou1, li)),
-                         ("eps_head_real_ms", lambda: h.eps_head_dev(dr, n, nwr, vcou1, lr, real_axis=True)),
-                         ("wc_build_gamma_real_ms", lambda: h.wc_build_gamma_dev(dr, n, nwr, wr, real_axis=True))):
-            h.sync()
+            step()
             t0 = time.perf_counter()
-            fn()
-            h.sync()
-            row[name] = round((time.perf_counter() - t0) * 1e3, 3)
-    finally:
-        h.close()
+            w0_forms[prec] = step()
+            row[f"{prec}_s_per_gamma_point"] = round(time.perf_counter() - t0, 5)
+            for name, fn in (("eps_head_real_ms", lambda: h.eps_head_dev(dr, n, nwr, vcou1, lr, real_axis=True)),
+                             ("wc_build_gamma_real_ms",
+                              lambda: h.wc_build_gamma_dev(dr, n, nwr, wr, real_axis=True))):
+                h.sync()
+                t0 = time.perf_counter()
+                fn()
+                h.sync()
+                row[f"{prec}_{name}"] = round((time.perf_counter() - t0) * 1e3, 3)
+        finally:
+            h.close()
+    a, b = w0_forms["f64"], w0_forms[mixed]
+    row["w0_relerr_mixed_vs_f64"] = float(max(np.abs(a[0] - b[0]).max() / np.abs(a[0]).max(),
+                                              np.abs(a[1] - b[1]).max() / np.abs(a[1]).max()))
     t0 = time.perf_counter()
     orc.eps_head(ci[:, :, :1], np.sqrt(vc), Q, vcou1)
     t1 = time.perf_counter()
@@ -564,7 +571,7 @@ def main():
             elif name == "IPW":
                 res[name] = bench_ipw(args.mixed, log_=log)
             elif name == "GAM":
-                res[name] = bench_gamma(log_=log)
+                res[name] = bench_gamma(args.mixed, log_=log)
             elif name == "ROT":
                 res[name] = bench_rotwave(log_=log)
             elif name == "C4":
@@ -642,7 +649,7 @@ def main():
             out["also"]["IPW"] = bench_ipw(args.mixed, log_=log)
             continue
         if name == "GAM":
-            out["also"]["GAM"] = bench_gamma(log_=log)
+            out["also"]["GAM"] = bench_gamma(args.mixed, log_=log)
             continue
         if name == "ROT":
             out["also"]["ROT"] = bench_rotwave(log_=log)

@@ -981,88 +981,12 @@ static ecalj_status wc_build_impl(ecalj_handle h, const void* chi, int ngb, int 
   cd *eps, *tmp;
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_EPS, sizeof(cd) * nn * nw, (void**)&eps));
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP, sizeof(cd) * nn * nw, (void**)&tmp));
-  const bool narrow = !wide(h);
-  cf* a32 = (cf*)tmp;       // complex64 halves of tmp (mixed forms)
-  cf* b32 = a32 + nn * nw;
-  const int mode = h->precision;  // the GEMM core of the form (FP64 for ECALJ_PREC_F64)
-  const void* Z = narrow ? (const void*)h->d_zcousq32 : (const void*)h->d_zcousq;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 3, &tev));
-  // bf16x3 form: the four transforms on the pre-split bf16x3 core of the Sigma_c kernel
-  const bool q3 = h->precision == ECALJ_PREC_BF16X3;
-  auto opnd = [&](const void* p, bool w, int64_t stride, bool herm) {
-    QOperand o;
-    o.p = p; o.ld = n; o.stride = stride; o.wide = w; o.herm = herm;
-    return o;
-  };
-  if (is_x0_m_basis && q3) {  // chi_E = fac zcousq^H (chi zcousq) (S/m_llw.f90:336-337)
-    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(chi, true, nn, false), opnd(h->d_zcousq, true, 0, false), 1.0, b32, nn));
-    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(h->d_zcousq, true, 0, true), opnd(b32, false, nn, false), fac, a32, nn));
-    ECALJ_TRY(eps_narrow(h, a32, h->d_vcousq, n, nw, eps));  // epstilde in complex(8)
-  } else if (is_x0_m_basis) {
-    if (narrow) ECALJ_TRY(convert(h, chi, true, a32, false, nn * nw));
-    GemmCall g1;
-    g1.mode = mode;
-    g1.M = n; g1.N = n; g1.K = n;
-    g1.A = narrow ? (const void*)a32 : chi; g1.lda = n; g1.bA = {nn, 0};
-    g1.B = Z; g1.ldb = n;
-    g1.C = narrow ? (void*)b32 : (void*)tmp; g1.ldc = n; g1.bC = {nn, 0};
-    g1.batch = nw; g1.div = nw;
-    ECALJ_TRY(gemm(h, g1));
-    GemmCall g2;
-    g2.mode = mode;
-    g2.ta = true; g2.ca = true;
-    g2.M = n; g2.N = n; g2.K = n;
-    g2.A = Z; g2.lda = n;
-    g2.B = narrow ? (const void*)b32 : (const void*)tmp; g2.ldb = n; g2.bB = {nn, 0};
-    g2.C = narrow ? (void*)a32 : (void*)eps; g2.ldc = n; g2.bC = {nn, 0};
-    g2.batch = nw; g2.div = nw;
-    g2.alpha = cd{fac, 0.0};
-    if (narrow) {
-      ECALJ_TRY(gemm(h, g2));
-      ECALJ_TRY(eps_narrow(h, a32, h->d_vcousq, n, nw, eps));  // epstilde in complex(8)
-    } else {
-      g2.epi = EPI_EPS; g2.svec = h->d_vcousq;  // eps fused into the epilogue
-      ECALJ_TRY(gemm(h, g2));
-    }
-  } else {
-    ECALJ_TRY(eps_direct(h, (const cd*)chi, h->d_vcousq, fac, n, nw, eps));
-  }
+  ECALJ_TRY(eps_build(h, chi, n, nw, fac, is_x0_m_basis != 0, 0.0, eps, tmp));  // (S/m_llw.f90:336-348)
   const int* perm = nullptr;
   ECALJ_TRY(gj_inverse(h, eps, n, n, nn, nw, pivot ? &perm : nullptr));  // zminv (S/m_llw.f90:143,350)
-  if (is_wc_m_basis && q3) {  // W_M = zcousq (W_E zcousq^H) (S/m_llw.f90:366-367)
-    ECALJ_TRY(wc_form32(h, eps, perm, h->d_vcousq, n, nw, b32));
-    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(b32, false, nn, false), opnd(h->d_zcousq, true, 0, true), 1.0, a32, nn));
-    ECALJ_TRY(qgemm_bf3(h, nw, n, n, n, opnd(h->d_zcousq, true, 0, false), opnd(a32, false, nn, false), 1.0, b32, nn));
-    ECALJ_TRY(widen_ld(h, b32, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
-  } else if (is_wc_m_basis) {
-    if (narrow) ECALJ_TRY(wc_form32(h, eps, perm, h->d_vcousq, n, nw, b32));
-    else ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, tmp, n, nn));
-    GemmCall g3;  // W_E zcousq^H
-    g3.mode = mode;
-    g3.tb = true; g3.cb = true;
-    g3.M = n; g3.N = n; g3.K = n;
-    g3.A = narrow ? (const void*)b32 : (const void*)tmp; g3.lda = n; g3.bA = {nn, 0};
-    g3.B = Z; g3.ldb = n;
-    g3.C = narrow ? (void*)a32 : (void*)eps; g3.ldc = n; g3.bC = {nn, 0};
-    g3.batch = nw; g3.div = nw;
-    ECALJ_TRY(gemm(h, g3));
-    GemmCall g4;  // zcousq * (W_E zcousq^H)
-    g4.mode = mode;
-    g4.M = n; g4.N = n; g4.K = n;
-    g4.A = Z; g4.lda = n;
-    g4.B = narrow ? (const void*)a32 : (const void*)eps; g4.ldb = n; g4.bB = {nn, 0};
-    if (narrow) {
-      g4.C = b32; g4.ldc = n; g4.bC = {nn, 0};
-    } else {
-      g4.C = (cd*)wc; g4.ldc = ldw; g4.bC = {(int64_t)ldw * n, 0};
-    }
-    g4.batch = nw; g4.div = nw;
-    ECALJ_TRY(gemm(h, g4));
-    if (narrow) ECALJ_TRY(widen_ld(h, b32, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
-  } else {
-    ECALJ_TRY(wc_form(h, eps, perm, h->d_vcousq, n, nw, (cd*)wc, ldw, (int64_t)ldw * n));
-  }
+  ECALJ_TRY(w_form(h, eps, perm, n, nw, is_wc_m_basis != 0, (cd*)wc, ldw, tmp));  // (S/m_llw.f90:353-367)
   ECALJ_TRY(timing_end(h, 3, tev));
   return ECALJ_OK;
 }

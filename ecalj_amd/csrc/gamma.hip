@@ -12,7 +12,7 @@
 //     (S/m_w0w0i.f90:89-140).
 // The inversions are the batched Gauss-Jordan of kernels.hip (partial
 // pivoting on the real axis, where epsilon is not Hermitian); the M <-> E
-// transforms are batched FP64 GEMMs.
+// transforms are those of the W build in the handle's form (wbuild.hip).
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -31,21 +31,6 @@ constexpr double kPi = 3.14159265358979323846;
 __device__ inline cd cdiv(cd a, cd b) {
   const double d = b.re * b.re + b.im * b.im;
   return cd{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
-}
-
-// eps = delta - fac s_i chi_ij s_j, and eps(1,1) = 1 - fac head chi(1,1) when
-// head > 0.  chi and eps may be the same buffer (elementwise).
-__global__ void eps_head_kernel(const cd* chi, const double* __restrict__ s, double fac, double head, int n,
-                                int nw, cd* eps) {
-  const int64_t nn = (int64_t)n * n, tot = nn * nw;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t ij = e % nn;
-    const int i = (int)(ij % n), j = (int)(ij / n);
-    const cd x = chi[e];
-    const double sc = (head > 0.0 && ij == 0) ? fac * head : fac * s[i] * s[j];
-    eps[e] = cd{(i == j ? 1.0 : 0.0) - sc * x.re, -sc * x.im};
-  }
 }
 
 // dst(i, j, iw) = src(i+1, j+1, iw): the mu, nu >= 2 block, m = n-1
@@ -100,44 +85,6 @@ __global__ void set_w0_kernel(cd* __restrict__ w, int64_t wstride, int nw, const
 }
 
 inline int blocks_for(int64_t tot) { return (int)std::max<int64_t>(1, std::min<int64_t>((tot + 255) / 256, 65536)); }
-
-// chi (M basis when is_x0_m_basis) -> epsilon over all ngb, head override
-// when head > 0.  eps and tmp: nn*nw each.
-ecalj_status build_eps(ecalj_ctx* h, const void* chi, int n, int nw, double fac, int is_x0_m_basis, double head,
-                       cd* eps, cd* tmp) {
-  const int64_t nn = (int64_t)n * n;
-  if (is_x0_m_basis) {  // chi_E = zcousq^H (chi zcousq) (S/m_llw.f90:122-125, 336-337)
-    GemmCall g1;
-    g1.M = n; g1.N = n; g1.K = n;
-    g1.A = chi; g1.lda = n; g1.bA = {nn, 0};
-    g1.B = h->d_zcousq; g1.ldb = n;
-    g1.C = tmp; g1.ldc = n; g1.bC = {nn, 0};
-    g1.batch = nw; g1.div = nw;
-    ECALJ_TRY(gemm(h, g1));
-    GemmCall g2;
-    g2.ta = true; g2.ca = true;
-    g2.M = n; g2.N = n; g2.K = n;
-    g2.A = h->d_zcousq; g2.lda = n;
-    g2.B = tmp; g2.ldb = n; g2.bB = {nn, 0};
-    g2.C = eps; g2.ldc = n; g2.bC = {nn, 0};
-    g2.batch = nw; g2.div = nw;
-    if (head > 0.0) {  // plain product; the head is applied by eps_head_kernel below
-      g2.alpha = cd{fac, 0.0};
-      ECALJ_TRY(gemm(h, g2));
-      hipLaunchKernelGGL(eps_head_kernel, dim3(blocks_for(nn * nw)), dim3(256), 0, h->stream, eps, h->d_vcousq,
-                         1.0, head, n, nw, eps);
-    } else {  // epsilon fused into the GEMM epilogue
-      g2.alpha = cd{fac, 0.0};
-      g2.epi = EPI_EPS; g2.svec = h->d_vcousq;
-      ECALJ_TRY(gemm(h, g2));
-    }
-  } else {
-    hipLaunchKernelGGL(eps_head_kernel, dim3(blocks_for(nn * nw)), dim3(256), 0, h->stream, (const cd*)chi,
-                       h->d_vcousq, fac, head, n, nw, eps);
-  }
-  ECALJ_HIP(hipGetLastError());
-  return ECALJ_OK;
-}
 
 ecalj_status check_common(ecalj_ctx* h, const void* chi, int ngb, int nw, int nspin) {
   if (!h || !chi) return ECALJ_ERR_ARG;
@@ -278,7 +225,7 @@ ecalj_status ecalj_wc_build_gamma_dev(ecalj_handle h, const void* chi, int ngb, 
   const double fac = (nspin == 1) ? 2.0 : 1.0;  // S/m_llw.f90:88-92, 299-303
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 3, &tev));
-  ECALJ_TRY(build_eps(h, chi, n, nw, fac, is_x0_m_basis, 0.0, eps, tmp));
+  ECALJ_TRY(eps_build(h, chi, n, nw, fac, is_x0_m_basis != 0, 0.0, eps, tmp));
   hipLaunchKernelGGL(block_tail_kernel, dim3(blocks_for((int64_t)m * m * nw)), dim3(256), 0, h->stream, eps, n, nw,
                      tmp);
   ECALJ_HIP(hipGetLastError());
@@ -320,7 +267,7 @@ ecalj_status ecalj_eps_head_dev(ecalj_handle h, const void* chi, int ngb, int nw
   const double fac = (nspin == 1) ? 2.0 : 1.0;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 3, &tev));
-  ECALJ_TRY(build_eps(h, chi, n, nw, fac, is_x0_m_basis, vcou1, eps, tmp));
+  ECALJ_TRY(eps_build(h, chi, n, nw, fac, is_x0_m_basis != 0, vcou1, eps, tmp));
   const int* perm = nullptr;
   ECALJ_TRY(gj_inverse(h, eps, n, n, nn, nw, real_axis ? &perm : nullptr));  // zminv (S/m_llw.f90:234, 427)
   const int nwing = (wing && wing_iw >= 0 && wing_iw < nw) ? n - 1 : 0;
@@ -364,21 +311,7 @@ ecalj_status ecalj_wv0_patch_dev(ecalj_handle h, void* w, int ngb, int ldw, int 
   if (!is_wc_m_basis) return ECALJ_OK;
   cd* tmp;
   ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP, sizeof(cd) * nn * nw, (void**)&tmp));
-  GemmCall g3;  // tmp = W_E zcousq^H (S/m_w0w0i.f90:130)
-  g3.tb = true; g3.cb = true;
-  g3.M = n; g3.N = n; g3.K = n;
-  g3.A = w; g3.lda = ldw; g3.bA = {ws, 0};
-  g3.B = h->d_zcousq; g3.ldb = n;
-  g3.C = tmp; g3.ldc = n; g3.bC = {nn, 0};
-  g3.batch = nw; g3.div = nw;
-  ECALJ_TRY(gemm(h, g3));
-  GemmCall g4;  // W_M = zcousq tmp (:131)
-  g4.M = n; g4.N = n; g4.K = n;
-  g4.A = h->d_zcousq; g4.lda = n;
-  g4.B = tmp; g4.ldb = n; g4.bB = {nn, 0};
-  g4.C = w; g4.ldc = ldw; g4.bC = {ws, 0};
-  g4.batch = nw; g4.div = nw;
-  return gemm(h, g4);
+  return w_e2m_inplace(h, (cd*)w, n, nw, (int64_t)ldw, tmp);  // (S/m_w0w0i.f90:130-131)
 }
 
 ecalj_status ecalj_wv0_patch(ecalj_handle h, double* w, int ngb, int ldw, int nw, const double* w0,

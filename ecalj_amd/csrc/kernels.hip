@@ -377,14 +377,15 @@ __global__ void wc_form_kernel(const cd* __restrict__ inv, const int* __restrict
 }
 
 // eps = delta - s_i chiE_ij s_j from a complex64 chi_E (already scaled), in FP64
-__global__ void eps_narrow_kernel(const cf* __restrict__ chi, const double* __restrict__ s, int n, int nw,
-                                  cd* __restrict__ eps) {
+// (head > 0: eps(1,1) = 1 - head chiE(1,1), the offset points near Gamma)
+__global__ void eps_narrow_kernel(const cf* __restrict__ chi, const double* __restrict__ s, double head, int n,
+                                  int nw, cd* __restrict__ eps) {
   const int64_t tot = (int64_t)n * n * nw;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t ij = e % ((int64_t)n * n);
     const int i = (int)(ij % n), j = (int)(ij / n);
-    const double sc = s[i] * s[j];
+    const double sc = (head > 0.0 && ij == 0) ? head : s[i] * s[j];
     const cf x = chi[e];
     eps[e] = cd{(i == j ? 1.0 : 0.0) - sc * (double)x.re, -sc * (double)x.im};
   }
@@ -420,11 +421,11 @@ __global__ void widen_ld_kernel(const cf* __restrict__ src, int n, int nw, cd* _
   }
 }
 
-ecalj_status eps_narrow(ecalj_ctx* h, const cf* chi, const double* s, int n, int nw, cd* eps) {
+ecalj_status eps_narrow(ecalj_ctx* h, const cf* chi, const double* s, double head, int n, int nw, cd* eps) {
   const int64_t tot = (int64_t)n * n * nw;
   if (tot == 0) return ECALJ_OK;
   const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536);
-  hipLaunchKernelGGL(eps_narrow_kernel, dim3(blocks), dim3(256), 0, h->stream, chi, s, n, nw, eps);
+  hipLaunchKernelGGL(eps_narrow_kernel, dim3(blocks), dim3(256), 0, h->stream, chi, s, head, n, nw, eps);
   ECALJ_HIP(hipGetLastError());
   return ECALJ_OK;
 }
@@ -985,7 +986,7 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
 // Sigma_c
 // Unpack the keepwv upper-triangle form and take the conjugate transpose in
 // one pass:  A_v(I,J) = conj(wc_v(J,I)) = (I <= J) ? W_v(I,J) : conj(W_v(J,I)).
-// W is complex128 (FP64 W build); the output is in the contraction's type.
+// W is complex128 (as the W build writes it); the output is in the contraction's type.
 template <class T>
 __global__ void sig_unpack_kernel(const cd* __restrict__ w, int64_t ldw, int64_t wstride, int n, int nw,
                                   T* __restrict__ out) {

@@ -20,7 +20,7 @@ ecalj_status wc_form(ecalj_ctx* h, const cd* inv, const int* perm, const double*
                      int64_t ldw, int64_t wstride);
 // Mixed form of the W build (the reference's __MP split): chi_E and W_E in
 // complex64 around the FP64 epsilon and its inverse.
-ecalj_status eps_narrow(ecalj_ctx* h, const cf* chi, const double* s, int n, int nw, cd* eps);
+ecalj_status eps_narrow(ecalj_ctx* h, const cf* chi, const double* s, double head, int n, int nw, cd* eps);
 ecalj_status wc_form32(ecalj_ctx* h, const cd* inv, const int* perm, const double* s, int n, int nw, cf* w);
 ecalj_status widen_ld(ecalj_ctx* h, const cf* src, int n, int nw, cd* dst, int64_t ldw, int64_t wstride);
 // In-place inverse of nmat matrices.  perm == nullptr: no row exchanges (the
@@ -82,5 +82,16 @@ struct QOperand {
 };
 ecalj_status qgemm_bf3(ecalj_ctx* h, int nb, int m, int ncol, int k, const QOperand& A, const QOperand& B,
                        double alpha, cf* C, int64_t c_stride);
+// The transforms around the epsilon inversion (wbuild.hip), in the handle's
+// form.  eps_build: eps (n x n x nw) from chi (complex128, M basis when
+// m_basis), head > 0 replaces the (1,1) element by 1 - head fac chi_E(1,1).
+// w_form: W = s inv s - diag(s^2) (columns through perm), to the M basis when
+// m_basis, into wc (leading dimension ldw); inv is overwritten.  w_e2m_inplace:
+// W records (ld ldw) <- zcousq W zcousq^H.  tmp: n*n*nw complex128.
+ecalj_status eps_build(ecalj_ctx* h, const void* chi, int n, int nw, double fac, bool m_basis, double head, cd* eps,
+                       cd* tmp);
+ecalj_status w_form(ecalj_ctx* h, cd* inv, const int* perm, int n, int nw, bool m_basis, cd* wc, int64_t ldw,
+                    cd* tmp);
+ecalj_status w_e2m_inplace(ecalj_ctx* h, cd* w, int n, int nw, int64_t ldw, cd* tmp);
 
 }  // namespace ecalj

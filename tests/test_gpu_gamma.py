@@ -198,3 +198,33 @@ def test_gamma_chain(h, orc, real):
     wg = h.wv0_patch(wg, w0_g, is_wc_m_basis=True)
     wo = orc.wv0_patch(orc.wc_build_gamma(chi, np.sqrt(vc), Q, nspin=1, is_x0_m_basis=True), w0_o, Q)
     assert relerr(wg, wo) < max(1e-10, 1e-14 * kap)
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-5), ("bf16x3", 1e-4)])
+def test_gamma_mixed_forms(orc, prec, tol):
+    """The mixed forms: M <-> E transforms on the form's core around the FP64
+    inverse (the reference's __MP split); bar = the form's per-result bar
+    times cond(eps)."""
+    import ecalj_amd
+
+    n, nw = 140, 3
+    say("gamma mixed", prec, n, nw)
+    rng = np.random.default_rng(77)
+    vc, Q = _coulomb(rng, n)
+    vcou1 = vc[0] * 1.02
+    chi = _chi_imag(rng, n, nw)
+    hd = ecalj_amd.Handle(0, prec)
+    try:
+        hd.set_coulomb(vc, Q)
+        llw, _ = hd.eps_head(chi, vcou1, nspin=1, is_x0_m_basis=True)
+        w = hd.wc_build_gamma(chi, nspin=1, is_x0_m_basis=True)
+        w0 = crand(rng, nw)
+        wp = hd.wv0_patch(w, w0, is_wc_m_basis=True)
+    finally:
+        hd.close()
+    lo, _ = orc.eps_head(chi, np.sqrt(vc), Q, vcou1, nspin=1, is_x0_m_basis=True)
+    wo = orc.wc_build_gamma(chi, np.sqrt(vc), Q, nspin=1, is_x0_m_basis=True)
+    kap = _cond(chi, vc, Q, 1, True, vcou1)
+    assert relerr(llw, lo) < tol * kap
+    assert relerr(w, wo) < tol * kap
+    assert relerr(wp, orc.wv0_patch(w, w0, Q)) < tol
