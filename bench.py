@@ -35,9 +35,23 @@ PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense
 # HBM traffic per launch of the dominant kernel, from the calibrated FETCH_SIZE +
 # WRITE_SIZE passes whose summary is committed (the counters cannot run inside
 # this timed process); None where no pass was taken.
+# HBM/fabric bytes per launch of the Sigma_c kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# (calibrated on the copy kernel), per workload, form and the variant the tuner keeps
 MEASURED_TRAFFIC = {
-    ("C2", "bf16x3"): (119.7e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
+    ("C2", "bf16x3", "128x128x2/8w/xcd"): (49.7e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
+    ("C2", "bf16x3", "128x128x2/8w"): (124.0e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
 }
+
+
+def chosen_variant(tuning):
+    """The Sigma_c variant the tuner kept (marked * in its report), or None."""
+    for line in (tuning or "").splitlines():
+        if line.startswith("sigc_bf3 ") and "*" in line:
+            toks = line.split("]: ", 1)[-1].split()  # name time "ms" ["*"] ...
+            for i, t in enumerate(toks):
+                if t == "*" and i >= 3:
+                    return toks[i - 3]
+    return None
 
 
 def log(*a):
@@ -511,7 +525,7 @@ def roofline(fl, info, precision, steps, config="C2"):
     kname = ("sig_bf3_kernel (bf16x3 Sigma_c: per-frequency W_c(iv)^H zmel, weighted sum, all internal k)"
              if precision == "bf16x3" else
              "sig_freq_gemm_kernel (Sigma_c: W_c(iv)^H x frequency-weighted zmel, all internal k)")
-    tr = MEASURED_TRAFFIC.get((config, precision))
+    tr = MEASURED_TRAFFIC.get((config, precision, chosen_variant(info.get("tuning"))))
     r = {"kernel": kname,
          "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
          "frac": round(achieved / peak, 4), "traffic": tr[0] if tr else None,

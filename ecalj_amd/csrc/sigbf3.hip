@@ -464,6 +464,13 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
       ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
     }
     var = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    // One launch per variant shows the first, cool launch.  Over consecutive
+    // launches the XCD-aware order, which fetches ~3x fewer bytes through the
+    // fabric, runs faster than a variant that beat it by a few percent once
+    // (C2: 22.6 against 23.2 ms per launch over 8 q-point steps), so it is
+    // kept when within 3% of the fastest single launch.
+    for (int v = 0; v < kNumSigVariants; ++v)
+      if (kSigVariants[v].xcd && ms[v] <= 1.03 * ms[var]) var = v;
     std::vector<const char*> names;
     for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
     tune_store(h, key, var, "sigc_bf3", names, ms);
