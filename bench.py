@@ -1,6 +1,6 @@
 """Benchmark: chi0 + W_c + Sigma_c per q-point on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2] [--mixed bf16x3|f32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2] [--mixed bf16x3|bf16f8|f32]
                     [--no-cpu-baseline]
 
 A step is one q-point of the path (chi0(q) over all k and band pairs, W_c(q)
@@ -30,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "bf16f8": 2500.0}  # TFLOP/s, MI355X dense
 
 # HBM traffic per launch of the dominant kernel, from the calibrated FETCH_SIZE +
 # WRITE_SIZE passes whose summary is committed (the counters cannot run inside
@@ -522,9 +522,10 @@ def roofline(fl, info, precision, steps, config="C2"):
     avg_s = ms / 1e3 / max(nl, 1)
     achieved = flop_launch / avg_s / 1e12
     peak = PEAK[precision]
-    kname = ("sig_bf3_kernel (bf16x3 Sigma_c: per-frequency W_c(iv)^H zmel, weighted sum, all internal k)"
-             if precision == "bf16x3" else
-             "sig_freq_gemm_kernel (Sigma_c: W_c(iv)^H x frequency-weighted zmel, all internal k)")
+    kname = {"bf16x3": "sig_bf3_kernel (bf16x3 Sigma_c: per-frequency W_c(iv)^H zmel, weighted sum, all internal k)",
+             "bf16f8": "sig_f8_kernel (Sigma_c: bf16 hi*hi + scaled-fp8 cross terms, per-frequency W_c(iv)^H zmel, "
+                       "weighted sum, all internal k)"}.get(
+        precision, "sig_freq_gemm_kernel (Sigma_c: W_c(iv)^H x frequency-weighted zmel, all internal k)")
     tr = MEASURED_TRAFFIC.get((config, precision, chosen_variant(info.get("tuning"))))
     r = {"kernel": kname,
          "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
@@ -536,6 +537,10 @@ def roofline(fl, info, precision, steps, config="C2"):
     if precision == "bf16x3":
         r["note"] = ("achieved counts algorithmic complex flops (8 per complex MAC); each costs 3 bf16 MFMA "
                      "products, so the attainable algorithmic rate is peak/3")
+    if precision == "bf16f8":
+        r["note"] = ("achieved counts algorithmic complex flops (8 per complex MAC); each costs one bf16 MFMA "
+                     "product and two fp8 products at twice the bf16 rate, so the attainable algorithmic rate "
+                     "is peak/2")
     return r
 
 
@@ -545,7 +550,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="C2")
-    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "f32"])
+    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "bf16f8", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM,ROT",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
@@ -624,6 +629,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"bf16x3": "bf16x3 split products, FP32 accumulation, complex64 storage; FP64 epsilon and its inverse",
+                  "bf16f8": "bf16x3 split products; Sigma_c as bf16 hi*hi + scaled fp8 (E4M3) cross terms; FP32 "
+                            "accumulation, complex64 storage; FP64 epsilon and its inverse",
                   "f32": "complex64, FP32 MFMA; FP64 epsilon and its inverse"}[args.mixed],
         "data": "synthetic (seeded; see ecalj_amd/synth.py)",
         "config": {"workload": f"{cfg.name} q-point: nk={cfg.nk}, nb={cfg.nb}, occ={cfg.no}, unocc={cfg.nu}, "

@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libecalj_gpu.so")
 _LIB = None
 
-PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16f8": 3}
 KCLASS = {"sigc": 0, "chi0": 1, "zmel": 2, "wbuild": 3, "hilbert": 4}
 
 

@@ -387,8 +387,10 @@ ecalj_status ecalj_tuning_report(ecalj_handle h, char* buf, int64_t len) {
 
 ecalj_status ecalj_set_precision(ecalj_handle h, int p) {
   if (!h) return ECALJ_ERR_ARG;
-  if (p != ECALJ_PREC_F64 && p != ECALJ_PREC_F32 && p != ECALJ_PREC_BF16X3) return ECALJ_ERR_ARG;
-  h->precision = p;
+  if (p != ECALJ_PREC_F64 && p != ECALJ_PREC_F32 && p != ECALJ_PREC_BF16X3 && p != ECALJ_PREC_BF16X3_F8)
+    return ECALJ_ERR_ARG;
+  h->sigc_fp8 = p == ECALJ_PREC_BF16X3_F8;
+  h->precision = h->sigc_fp8 ? ECALJ_PREC_BF16X3 : p;
   return ECALJ_OK;
 }
 

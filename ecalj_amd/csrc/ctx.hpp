@@ -14,7 +14,8 @@ struct ecalj_ctx {
   int ncu = 256;                 // compute units of the device (persistent grids)
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
-  int precision = ECALJ_PREC_F64;
+  int precision = ECALJ_PREC_F64;  // F64, F32 or BF16X3 (BF16X3_F8 is BF16X3 + sigc_fp8)
+  bool sigc_fp8 = false;           // Sigma_c contraction with fp8 cross terms
 
   // set_ppb state: per-atom blocks
   int natom = 0;
@@ -103,7 +104,7 @@ ecalj_status check_flag(ecalj_ctx* h);
 ecalj_status upload_blob(ecalj_ctx* h, int slot, const std::vector<unsigned char>& v, unsigned char** out);
 
 // ---- kernel-variant tuner
-enum TuneOp { TUNE_SIGC_BF3 = 0, TUNE_SIGC_BF3_GROUPED = 1 };
+enum TuneOp { TUNE_SIGC_BF3 = 0, TUNE_SIGC_BF3_GROUPED = 1, TUNE_SIGC_F8 = 2, TUNE_SIGC_F8_GROUPED = 3 };
 typedef std::array<int64_t, 5> TuneKey;
 bool tune_lookup(ecalj_ctx* h, const TuneKey& k, int* var);
 void tune_store(ecalj_ctx* h, const TuneKey& k, int var, const char* op, const std::vector<const char*>& names,

@@ -1172,9 +1172,10 @@ static ecalj_status sig_contract_t(ecalj_ctx* h, int nbatch, const T* zmel, int 
     if constexpr (MODE == MBF3) {
       unsigned short* as = reinterpret_cast<unsigned short*>(wh);
       unsigned short* bs = as + sig_bf3_a_elems_per_k(n, nw1) * gb;
-      ECALJ_TRY(sig_bf3(h, gb, (const cf*)zmel + (int64_t)n * ncol * b0, (int64_t)n * ncol, n, ncol, nw1,
-                        wc + wstride * b0, ldw, wstride, wgt + (int64_t)nw1 * ncol * b0, (int64_t)nw1 * ncol,
-                        (cf*)cz, (int64_t)n * ncol, as, bs));
+      ECALJ_TRY((h->sigc_fp8 ? sig_f8 : sig_bf3)(h, gb, (const cf*)zmel + (int64_t)n * ncol * b0,
+                                                 (int64_t)n * ncol, n, ncol, nw1, wc + wstride * b0, ldw, wstride,
+                                                 wgt + (int64_t)nw1 * ncol * b0, (int64_t)nw1 * ncol, (cf*)cz,
+                                                 (int64_t)n * ncol, as, bs));
     } else {
       for (int q = 0; q < gb; ++q) {
         hipLaunchKernelGGL((sig_unpack_kernel<T>), dim3(ublocks), dim3(256), 0, h->stream, wc + wstride * (b0 + q),
@@ -1262,8 +1263,9 @@ ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm,
     unsigned short* as = reinterpret_cast<unsigned short*>(wh);
     unsigned short* bs = as + a_elems;
     cf* cz = reinterpret_cast<cf*>(reinterpret_cast<unsigned char*>(bs) + b_bytes);
-    ECALJ_TRY(sig_bf3_grouped(h, k1 - k0, ncol.data(), z + (int64_t)n * zoff[k0], n, nw1, wc + wstride * k0, ldw,
-                              wstride, wgt + nw1 * zoff[k0], cz, as, bs));
+    ECALJ_TRY((h->sigc_fp8 ? sig_f8_grouped : sig_bf3_grouped)(h, k1 - k0, ncol.data(), z + (int64_t)n * zoff[k0],
+                                                               n, nw1, wc + wstride * k0, ldw, wstride,
+                                                               wgt + nw1 * zoff[k0], cz, as, bs));
     int64_t cofs = 0;
     for (int k = k0; k < k1; ++k) {
       if (nm[k] == 0) continue;

@@ -48,6 +48,13 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
                      const cd* wc, int64_t ldw, int64_t w_stride, const double* wgt, int64_t g_stride, cf* cz,
                      int64_t cz_stride, unsigned short* as, unsigned short* bs);
 size_t sig_bf3_bytes_per_k(int n, int ncol, int nw1);
+// The same contraction with fp8 cross terms (scratch as for sig_bf3)
+ecalj_status sig_f8(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1, const cd* wc,
+                    int64_t ldw, int64_t w_stride, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride,
+                    unsigned short* as, unsigned short* bs);
+ecalj_status sig_f8_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zmel, int n, int nw1, const cd* wc,
+                            int64_t ldw, int64_t w_stride, const double* wgt, cf* cz, unsigned short* as,
+                            unsigned short* bs);
 ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zmel, int n, int nw1, const cd* wc,
                              int64_t ldw, int64_t w_stride, const double* wgt, cf* cz, unsigned short* as,
                              unsigned short* bs);

@@ -21,7 +21,8 @@ module ecalj_gpu
   public :: ecalj_wv0_patch, ecalj_wv0_patch_dev
   public :: ecalj_set_hamindex, ecalj_set_mnla, ecalj_readcphi_dev, ecalj_readgeig_dev, ecalj_mnla_read
   public :: ecalj_qgpsi_read, ecalj_evalue_read, ecalj_rec_read_dev
-  integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2
+  integer(c_int), parameter, public :: ECALJ_PREC_F64 = 0, ECALJ_PREC_F32 = 1, ECALJ_PREC_BF16X3 = 2, &
+                                        ECALJ_PREC_BF16X3_F8 = 3
 
   interface
     ! gpu_init (S/m_gpu.f90:14-68)

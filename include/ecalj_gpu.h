@@ -43,7 +43,10 @@ typedef enum {
 typedef enum {
   ECALJ_PREC_F64 = 0,     /* complex128 data, FP64 MFMA                          */
   ECALJ_PREC_F32 = 1,     /* complex64 products on FP32 MFMA, FP32 accumulation  */
-  ECALJ_PREC_BF16X3 = 2   /* hi/lo bf16 split, 3 MFMA products, FP32 accumulation */
+  ECALJ_PREC_BF16X3 = 2,  /* hi/lo bf16 split, 3 MFMA products, FP32 accumulation */
+  ECALJ_PREC_BF16X3_F8 = 3 /* bf16x3, except the Sigma_c frequency contraction:
+                              hi*hi on bf16 MFMA, hi*lo + lo*hi on the scaled fp8
+                              (E4M3) MFMA, ~2^-12 per product                    */
 } ecalj_precision;
 
 /* ---- handle ------------------------------------------------------------ */
