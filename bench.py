@@ -552,7 +552,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "bf16f8", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM,ROT",
+    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM,ROT,F8,C3F8",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -656,6 +656,7 @@ def main():
     also = [c for c in args.also.split(",") if c and c.lower() != "none" and c != args.config]
     if also:
         out["also"] = {}
+    fp64_runs = {}  # config -> (inputs, FP64-form run) kept for a later F8 leg of the same config
     for name in also:
         if name == "C4":
             out["also"]["C4"] = bench_c4(args.mixed, log_=log)
@@ -674,6 +675,31 @@ def main():
             continue
         if name == "ROT":
             out["also"]["ROT"] = bench_rotwave(log_=log)
+            continue
+        if name.endswith("F8") and (name[:-2] or args.config) in synth.CONFIGS:
+            # the same q-point in the bf16f8 form (Sigma_c with scaled-fp8 cross terms)
+            base = name[:-2] or args.config
+            if base == args.config:
+                q2, a64, st2 = qp, f64, args.steps
+            elif base in fp64_runs:
+                (q2, a64), st2 = fp64_runs.pop(base), 2
+            else:
+                log(f"bench: generating {base} inputs")
+                q2 = synth.make_qpoint(synth.CONFIGS[base])
+                a64, st2 = run_form(q2, "f64", 2, 1), 2
+            a8 = run_form(q2, "bf16f8", st2, 1)
+            f2 = flops_per_q(synth.CONFIGS[base], nwhis_pairs_bins=a8["npb"])
+            out["also"][name] = {
+                "workload": f"{base} q-point, precision bf16f8: bf16x3 except Sigma_c (bf16 hi*hi + scaled fp8 "
+                            f"E4M3 cross terms)",
+                "steps": st2, "s_per_q": round(a8["s_per_q"], 6),
+                "speedup_vs_fp64_form": round(a64["s_per_q"] / a8["s_per_q"], 3),
+                "qp_err_meV_vs_fp64_form": round(float(np.abs(a8["eqp"] - a64["eqp"]).max() * 1e3), 5),
+                "tflops": round(sum(f2.values()) / a8["s_per_q"] / 1e12, 3),
+                "kernel_ms_per_step": {k: round(v[0] / st2, 3) for k, v in a8["ktimes"].items()},
+                "roofline": roofline(f2, a8, "bf16f8", st2, base),
+                "tuning": a8["tuning"],
+            }
             continue
         if name.endswith("R") and name[:-1] in synth.CONFIGS:
             out["also"][name] = bench_real_axis(args.mixed, name[:-1], log_=log)
@@ -695,6 +721,8 @@ def main():
             "kernel_ms_per_step": {k: round(v[0] / 2, 3) for k, v in amx["ktimes"].items()},
             "roofline": roofline(f2, amx, args.mixed, 2, name),
         }
+        if name + "F8" in also:
+            fp64_runs[name] = (q2, a64)
         del q2
     print(json.dumps(out), flush=True)
 

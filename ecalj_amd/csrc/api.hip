@@ -784,6 +784,15 @@ static ecalj_status x0_impl(ecalj_ctx* h, void* rcxq, int npr, int ipr_col, int 
   }));
   // work items; square output -> upper tiles + mirror
   const int mirror = (ipr_col == 1 && npr_col == npr) ? 1 : 0;
+  if (h->precision == ECALJ_PREC_BF16X3) {  // bins gathered into bf16 planes, grouped GEMM on the Sigma_c core
+    std::vector<int> slice(nbin);
+    for (int b = 0; b < nbin; ++b) {
+      const int iw = b % nwhis + 1, jpm = b / nwhis + 1;
+      slice[b] = iw * (3 - 2 * jpm) - (1 - npm) * nwhis;
+    }
+    return x0_bf3(h, (const ecalj::cf*)zmel, npr, npr_col, ipr_col, nbin, cnt.data(), col.data(), wt.data(),
+                  slice.data(), (ecalj::cf*)rcxq, (int64_t)npr * npr_col, mirror);
+  }
   const int tm = (npr + 63) / 64, tn = (npr_col + 63) / 64;
   struct Item { int list, slice, ti, tj; };
   std::vector<Item> items;

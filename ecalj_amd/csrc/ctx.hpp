@@ -45,7 +45,7 @@ struct ecalj_ctx {
 
   // grow-only scratch slots
   enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
-         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_GAM, S_ROT1, S_ROT2, S_ROT3, S_QA, S_QB, S_QW, S_GJW, S_NSLOT };
+         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_GAM, S_ROT1, S_ROT2, S_ROT3, S_QA, S_QB, S_QW, S_GJW, S_X0A, S_X0B, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag
@@ -104,7 +104,7 @@ ecalj_status check_flag(ecalj_ctx* h);
 ecalj_status upload_blob(ecalj_ctx* h, int slot, const std::vector<unsigned char>& v, unsigned char** out);
 
 // ---- kernel-variant tuner
-enum TuneOp { TUNE_SIGC_BF3 = 0, TUNE_SIGC_BF3_GROUPED = 1, TUNE_SIGC_F8 = 2, TUNE_SIGC_F8_GROUPED = 3 };
+enum TuneOp { TUNE_SIGC_BF3 = 0, TUNE_SIGC_BF3_GROUPED = 1 };
 typedef std::array<int64_t, 5> TuneKey;
 bool tune_lookup(ecalj_ctx* h, const TuneKey& k, int* var);
 void tune_store(ecalj_ctx* h, const TuneKey& k, int var, const char* op, const std::vector<const char*>& names,

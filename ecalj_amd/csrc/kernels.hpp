@@ -78,6 +78,12 @@ ecalj_status real_plan_into_cz(ecalj_ctx* h, const RealPlanHandle* r, const Real
 ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm, const void* zmel, int n, int ntq,
                                   int niw, const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz);
 size_t sig_bf3_a_elems_per_k(int n, int nw1);
+// x0gemm in the bf16x3 form: per-bin entry lists on the host (bin b: entries
+// cnt[b]..cnt[b+1]-1 with zmel column offsets col[] and weights wt[]), output
+// slice slice[b] of rcxq (npr x npr_col, complex64, slice stride slice_elems)
+ecalj_status x0_bf3(ecalj_ctx* h, const cf* z, int npr, int npr_col, int ipr_col, int nbin, const int64_t* cnt,
+                    const int64_t* col, const double* wt, const int* slice, cf* rcxq, int64_t slice_elems,
+                    int mirror);
 // Batched complex GEMM on the bf16x3 core of the Sigma_c kernel:
 // C_b = alpha op(A_b) op(B_b) (m x ncol, leading dimension m, stride c_stride,
 // complex64).  An operand is complex128 (wide) or complex64 with leading

@@ -18,6 +18,7 @@
 // bytes) at a time: rows of equal parity within 8 and rows equal mod 4 within
 // 16 land in distinct 16-byte bank groups.
 #include <algorithm>
+#include <numeric>
 #include <cstring>
 #include <vector>
 
@@ -160,7 +161,7 @@ template <int BN, int NSTG, int NW>
 __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned short* __restrict__ Ak,
                                          const unsigned short* __restrict__ Bk, const double* __restrict__ g,
                                          int n, int nw1, int v0, int nwg, int ncol, int MP, int NS, int NCP,
-                                         int I0, int c0, int accumulate, cf* __restrict__ C) {
+                                         int I0, int c0, int accumulate, cf* __restrict__ C, int mirror = 0) {
   static_assert(NSTG == 2 || NSTG == 3, "two or three stages");
   static_assert(NW == 4 || NW == 8, "four or eight waves");
   constexpr int NT = NW * 64, WN = NW / 2;
@@ -304,6 +305,11 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
             *p = cf{o.re + tr[i][j][e], o.im + ti[i][j][e]};
           } else {
             *p = cf{tr[i][j][e], ti[i][j][e]};
+          }
+          if (mirror) {  // Hermitian output, off-diagonal tile: C(c, I) += conj
+            cf* q = C + c + (int64_t)n * I;
+            const cf o = *q;
+            *q = cf{o.re + tr[i][j][e], o.im - ti[i][j][e]};
           }
         }
       }
@@ -902,17 +908,17 @@ typedef int i4q __attribute__((ext_vector_type(4)));
 typedef int i8q __attribute__((ext_vector_type(8)));
 typedef int i12q __attribute__((ext_vector_type(12)));
 
-// One 128 x 128 block tile, two stages; NW waves in 2 x NW/2: 8 waves (two
-// per SIMD, wave tile 64 x 32, ~235 VGPRs) or 16 (four per SIMD, 64 x 16,
-// within 128 VGPRs: more waves to cover the barrier and load waits, twice the
-// A-operand LDS reads).
-template <int NW>
+// One 128 x 128 block tile, two stages; 8 waves in 2 x 4 (two per SIMD), wave
+// tile 64 x 32 (~235 VGPRs).  A 16-wave form (64 x 16 wave tiles within 128
+// VGPRs) measured the same time: per 32-wide k slice the block's MFMA work
+// and its 65 KB of L2 -> LDS loads each take ~1.8 us at C2 and overlap, so
+// neither more waves nor fewer loads alone shortens a stage.
+constexpr int F8_NW = 8;
 __device__ __forceinline__ void sig8_tile(F8Smem& s, const unsigned char* __restrict__ Ak,
                                           const unsigned char* __restrict__ Bk, const double* __restrict__ g,
                                           int n, int nw1, int v0, int nwg, int ncol, int MP, int NS, int NCP, int I0,
                                           int c0, int accumulate, cf* __restrict__ C) {
-  static_assert(NW == 8 || NW == 16, "8 or 16 waves");
-  constexpr int WN = NW / 2, TM = 4, TN = F8_BN / (16 * WN), NT = NW * 64, RPW = 64 / NW;
+  constexpr int NW = F8_NW, WN = NW / 2, TM = 4, TN = F8_BN / (16 * WN), NT = NW * 64, RPW = 64 / NW;
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave / WN, wn = wave % WN;
   const int64_t asec = (int64_t)F8_ROWB * MP, bsec = (int64_t)F8_ROWB * NCP;
@@ -1077,8 +1083,7 @@ __device__ __forceinline__ void sig8_tile(F8Smem& s, const unsigned char* __rest
 }
 
 // Uniform batch in the XCD-aware 1-D order of sig_bf3_kernel (G > 0)
-template <int NW>
-__global__ __launch_bounds__(NW * 64, 1) void sig_f8_kernel(const unsigned char* __restrict__ as,
+__global__ __launch_bounds__(F8_NW * 64, 1) void sig_f8_kernel(const unsigned char* __restrict__ as,
                                                                int64_t as_stride,
                                                                const unsigned char* __restrict__ bs,
                                                                int64_t bs_stride, const double* __restrict__ wgt,
@@ -1091,13 +1096,12 @@ __global__ __launch_bounds__(NW * 64, 1) void sig_f8_kernel(const unsigned char*
   const int sigma = (j / S) * 8 + x, loc = j % S;
   const int b = sigma / NG, r = loc % R, c = (sigma % NG) * G + loc / R;
   if (b >= nk || c >= Ct) return;
-  sig8_tile<NW>(s, as + as_stride * b, bs + bs_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS, NCP,
+  sig8_tile(s, as + as_stride * b, bs + bs_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS, NCP,
             r * QBM, c * F8_BN, accumulate, cz + cz_stride * b);
 }
 
 // Grouped, persistent (tile list of (k, row tile, column tile)), as sig_bf3_grouped_kernel
-template <int NW>
-__global__ __launch_bounds__(NW * 64, 1) void sig_f8_grouped_kernel(const int* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(F8_NW * 64, 1) void sig_f8_grouped_kernel(const int* __restrict__ tiles, int ntiles,
                                                                        const SigGroup* __restrict__ grp,
                                                                        const unsigned char* __restrict__ as,
                                                                        int64_t as_stride,
@@ -1110,33 +1114,9 @@ __global__ __launch_bounds__(NW * 64, 1) void sig_f8_grouped_kernel(const int* _
     const int k = tiles[3 * t], it = tiles[3 * t + 1], jt = tiles[3 * t + 2];
     const SigGroup gk = grp[k];
     __syncthreads();
-    sig8_tile<NW>(s, as + as_stride * k, bs + gk.boff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, gk.ncp,
+    sig8_tile(s, as + as_stride * k, bs + gk.boff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, gk.ncp,
               it * QBM, jt * F8_BN, accumulate, cz + gk.czoff);
   }
-}
-
-// Variant of the fp8 kernels (0: 8 waves, 1: 16 waves): forced by the
-// handle, cached for this shape, or measured now (*measured: every variant
-// computes the same sums in the same order, so the last timed run left the
-// result).
-static const char* const kF8Names[] = {"128x128x2/8w/xcd", "128x128x2/16w/xcd"};
-template <class Run>
-static ecalj_status sig_f8_pick(ecalj_ctx* h, const TuneKey& key, const char* op, Run&& run, int* var,
-                                bool* measured) {
-  *measured = false;
-  if (h->tune_mode > 0) {
-    *var = (h->tune_mode - 1) % 2;
-  } else if (tune_lookup(h, key, var)) {
-  } else if (h->capturing) {
-    *var = 1;
-  } else {
-    std::vector<double> ms(2);
-    for (int v = 0; v < 2; ++v) ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
-    *var = ms[1] <= ms[0] ? 1 : 0;
-    tune_store(h, key, *var, op, {kF8Names[0], kF8Names[1]}, ms);
-    *measured = true;
-  }
-  return ECALJ_OK;
 }
 
 ecalj_status sig_f8(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1, const cd* wc,
@@ -1162,29 +1142,15 @@ ecalj_status sig_f8(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int 
   const int64_t NG = (Ct + G - 1) / G, S = (int64_t)R * G;
   const int64_t blocks = 8 * S * ((nb * NG + 7) / 8);
   if (blocks > 0x7fffffff) return ECALJ_ERR_DIM;
-  auto run = [&](int var) -> ecalj_status {
-    for (int v0 = 0; v0 < nw1; v0 += QNW) {
-      const int nwg = std::min(QNW, nw1 - v0);
-      if (var == 0)
-        hipLaunchKernelGGL(sig_f8_kernel<8>, dim3((unsigned)blocks), dim3(512), 0, h->stream, as, as_stride, bs,
-                           bs_stride, wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz,
-                           cz_stride, G, nb);
-      else
-        hipLaunchKernelGGL(sig_f8_kernel<16>, dim3((unsigned)blocks), dim3(1024), 0, h->stream, as, as_stride, bs,
-                           bs_stride, wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz,
-                           cz_stride, G, nb);
-      ECALJ_HIP(hipGetLastError());
-    }
-    return ECALJ_OK;
-  };
-  int var = -1;
-  const TuneKey key{TUNE_SIGC_F8, n, ncol, nw1, nb};
-  bool measured = false;
-  ECALJ_TRY(sig_f8_pick(h, key, "sigc_f8", run, &var, &measured));
-  if (measured) return ECALJ_OK;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 0, &tev));
-  ECALJ_TRY(run(var));
+  for (int v0 = 0; v0 < nw1; v0 += QNW) {
+    const int nwg = std::min(QNW, nw1 - v0);
+    hipLaunchKernelGGL(sig_f8_kernel, dim3((unsigned)blocks), dim3(F8_NW * 64), 0, h->stream, as, as_stride, bs,
+                       bs_stride, wgt, g_stride, n, nw1, v0, nwg, ncol, MP, NS, NCP, v0 > 0 ? 1 : 0, cz, cz_stride,
+                       G, nb);
+    ECALJ_HIP(hipGetLastError());
+  }
   ECALJ_TRY(timing_end(h, 0, tev));
   return ECALJ_OK;
 }
@@ -1241,28 +1207,193 @@ ecalj_status sig_f8_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zme
   const SigGroup* dg = reinterpret_cast<const SigGroup*>(d);
   const int* dt = reinterpret_cast<const int*>(d + sizeof(SigGroup) * nk);
   const int grid = std::max(1, std::min(ntiles, h->ncu));
-  auto run = [&](int var) -> ecalj_status {
-    for (int v0 = 0; v0 < nw1; v0 += QNW) {
-      const int nwg = std::min(QNW, nw1 - v0);
-      if (var == 0)
-        hipLaunchKernelGGL(sig_f8_grouped_kernel<8>, dim3(grid), dim3(512), 0, h->stream, dt, ntiles, dg, as,
-                           as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
-      else
-        hipLaunchKernelGGL(sig_f8_grouped_kernel<16>, dim3(grid), dim3(1024), 0, h->stream, dt, ntiles, dg, as,
-                           as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
-      ECALJ_HIP(hipGetLastError());
-    }
-    return ECALJ_OK;
-  };
-  int var = -1;
-  const TuneKey key{TUNE_SIGC_F8_GROUPED, n, (int)cols, nw1, nk};
-  bool measured = false;
-  ECALJ_TRY(sig_f8_pick(h, key, "sigc_f8_grouped", run, &var, &measured));
-  if (measured) return ECALJ_OK;
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 0, &tev));
-  ECALJ_TRY(run(var));
+  for (int v0 = 0; v0 < nw1; v0 += QNW) {
+    const int nwg = std::min(QNW, nw1 - v0);
+    hipLaunchKernelGGL(sig_f8_grouped_kernel, dim3(grid), dim3(F8_NW * 64), 0, h->stream, dt, ntiles, dg, as,
+                       as_stride, bs, wgt, n, nw1, v0, nwg, MP, NS, v0 > 0 ? 1 : 0, cz);
+    ECALJ_HIP(hipGetLastError());
+  }
   ECALJ_TRY(timing_end(h, 0, tev));
+  return ECALJ_OK;
+}
+
+
+// ================================================================ x0gemm on the bf16x3 core
+// rcxq(:,:,slice(bin)) += sum_{t in bin} conj(z_t(i)) w_t z_t(ipr_col-1+j)
+// (S/x0gemm.f90:51-97) as one grouped GEMM per histogram bin: the bin's
+// columns are gathered once into pre-split bf16 planes (A = conj(z_t), B =
+// w_t z_t, K = the bin's entries padded to 32 with zeros), and every (bin,
+// row tile, column tile) runs the Sigma_c tile code (one frequency, unit
+// weights, accumulating).  A square output (ipr_col = 1, npr_col = npr) is
+// Hermitian: only tiles ti <= tj run, off-diagonal ones also add the
+// conjugate transpose.
+
+// one thread per (global K slice s, row r, chunk ch), ch fastest: the four
+// chunks of a row are adjacent lanes, so a wave writes 16 whole 64-byte rows
+// of each plane
+__global__ __launch_bounds__(256) void x0_split_kernel(const cf* __restrict__ z, const int64_t* __restrict__ colp,
+                                                       const double* __restrict__ wp, int64_t nslice, int npr,
+                                                       int npr_col, int64_t boff, int MP, int NCP, int RP,
+                                                       unsigned short* __restrict__ as,
+                                                       unsigned short* __restrict__ bs) {
+  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+  const int64_t tot = nslice * 4 * RP;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int ch = (int)(e % 4);
+    const int r = (int)((e / 4) % RP);
+    const int64_t sl = e / (4 * (int64_t)RP);
+    us8v arh, arl, aih, ail, brh, brl, bih, bil;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t t = sl * QBK + 8 * ch + q;
+      const int64_t c = colp[t];
+      float xr = 0.f, xi = 0.f, yr = 0.f, yi = 0.f;
+      if (c >= 0) {
+        if (r < npr) {
+          const cf x = z[c + r];
+          xr = x.re;
+          xi = -x.im;
+        }
+        if (r < npr_col) {
+          const float w = (float)wp[t];
+          const cf y = z[c + boff + r];
+          yr = w * y.re;
+          yi = w * y.im;
+        }
+      }
+      unsigned short h, l;
+      split_bf16(xr, h, l);
+      arh[q] = h;
+      arl[q] = l;
+      split_bf16(xi, h, l);
+      aih[q] = h;
+      ail[q] = l;
+      split_bf16(yr, h, l);
+      brh[q] = h;
+      brl[q] = l;
+      split_bf16(yi, h, l);
+      bih[q] = h;
+      bil[q] = l;
+    }
+    if (r < MP) {
+      const int64_t pst = (int64_t)MP * QBK;
+      unsigned short* A = as + sl * 4 * pst + (int64_t)r * QBK + 8 * chunk_slot(r, ch);
+      *reinterpret_cast<us8v*>(A) = arh;
+      *reinterpret_cast<us8v*>(A + pst) = arl;
+      *reinterpret_cast<us8v*>(A + 2 * pst) = aih;
+      *reinterpret_cast<us8v*>(A + 3 * pst) = ail;
+    }
+    if (r < NCP) {
+      const int64_t pst = (int64_t)NCP * QBK;
+      unsigned short* B = bs + sl * 4 * pst + (int64_t)r * QBK + 8 * chunk_slot(r, ch);
+      *reinterpret_cast<us8v*>(B) = brh;
+      *reinterpret_cast<us8v*>(B + pst) = brl;
+      *reinterpret_cast<us8v*>(B + 2 * pst) = bih;
+      *reinterpret_cast<us8v*>(B + 3 * pst) = bil;
+    }
+  }
+}
+
+struct X0Bin {
+  int64_t aoff, boff;  // plane offsets (elements) of the bin's first K slice
+  int ns, slice;       // K slices, output slice
+};
+
+__global__ __launch_bounds__(512, 1) void x0_bf3_kernel(const int* __restrict__ tiles, int ntiles,
+                                                        const X0Bin* __restrict__ bins,
+                                                        const unsigned short* __restrict__ as,
+                                                        const unsigned short* __restrict__ bs,
+                                                        const double* __restrict__ ones, int npr, int npr_col,
+                                                        int MP, int NCP, int mirror, cf* __restrict__ rcxq,
+                                                        int64_t slice_elems) {
+  __shared__ SigSmem<128, 2> s;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int b = tiles[3 * t], ti = tiles[3 * t + 1], tj = tiles[3 * t + 2];
+    const X0Bin bb = bins[b];
+    __syncthreads();  // the previous tile's LDS reads are done before this tile's loads land
+    sig_tile<128, 2, 8>(s, as + bb.aoff, bs + bb.boff, ones, npr, 1, 0, 1, npr_col, MP, bb.ns, NCP, ti * QBM,
+                        tj * 128, 1, rcxq + slice_elems * bb.slice, mirror && ti != tj);
+  }
+}
+
+ecalj_status x0_bf3(ecalj_ctx* h, const cf* z, int npr, int npr_col, int ipr_col, int nbin, const int64_t* cnt,
+                    const int64_t* col, const double* wt, const int* slice, cf* rcxq, int64_t slice_elems,
+                    int mirror) {
+  const int MP = pad_to(std::max(npr, 1), QBM), NCP = pad_to(std::max(npr_col, 1), 128);
+  const int RP = std::max(MP, NCP);
+  // padded K per bin, host tables
+  std::vector<X0Bin> bins;
+  int64_t kp = 0;
+  for (int b = 0; b < nbin; ++b) {
+    const int64_t K = cnt[b + 1] - cnt[b];
+    if (K == 0) continue;
+    const int64_t k32 = (K + QBK - 1) / QBK * QBK;
+    if (k32 / QBK > 0x7fffffff) return ECALJ_ERR_DIM;
+    bins.push_back({kp / QBK * 4 * (int64_t)MP * QBK, kp / QBK * 4 * (int64_t)NCP * QBK, (int)(k32 / QBK), slice[b]});
+    kp += k32;
+  }
+  if (bins.empty()) return ECALJ_OK;
+  std::vector<int64_t> colp(kp, -1);
+  std::vector<double> wp(kp, 0.0);
+  {
+    int64_t o = 0;
+    for (int b = 0; b < nbin; ++b) {
+      const int64_t K = cnt[b + 1] - cnt[b];
+      if (K == 0) continue;
+      for (int64_t t = 0; t < K; ++t) {
+        colp[o + t] = col[cnt[b] + t];
+        wp[o + t] = wt[cnt[b] + t];
+      }
+      o += (K + QBK - 1) / QBK * QBK;
+    }
+  }
+  // tiles: longest K first (persistent grid)
+  const int R = MP / QBM, Ct = NCP / 128;
+  std::vector<int> order(bins.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bins[x].ns > bins[y].ns; });
+  std::vector<int> tiles;
+  for (int b : order)
+    for (int ti = 0; ti < R; ++ti)
+      for (int tj = mirror ? ti : 0; tj < Ct; ++tj) {
+        tiles.push_back(b);
+        tiles.push_back(ti);
+        tiles.push_back(tj);
+      }
+  const int ntiles = (int)(tiles.size() / 3);
+  // one upload: [colp | wp | ones (NCP) | bins | tiles]
+  const size_t o_w = sizeof(int64_t) * kp, o_one = o_w + sizeof(double) * kp, o_bin = o_one + sizeof(double) * NCP,
+               o_til = o_bin + sizeof(X0Bin) * bins.size(), nbytes = o_til + sizeof(int) * tiles.size();
+  std::vector<unsigned char> blob(nbytes);
+  memcpy(blob.data(), colp.data(), o_w);
+  memcpy(blob.data() + o_w, wp.data(), o_one - o_w);
+  for (int c = 0; c < NCP; ++c) {
+    const double one = 1.0;
+    memcpy(blob.data() + o_one + sizeof(double) * c, &one, sizeof(double));
+  }
+  memcpy(blob.data() + o_bin, bins.data(), o_til - o_bin);
+  memcpy(blob.data() + o_til, tiles.data(), nbytes - o_til);
+  unsigned char* d;
+  ECALJ_TRY(upload_blob(h, ecalj_ctx::S_LIST, blob, &d));
+  unsigned short *as, *bs;
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0A, sizeof(unsigned short) * 4 * (size_t)MP * kp, (void**)&as));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0B, sizeof(unsigned short) * 4 * (size_t)NCP * kp, (void**)&bs));
+  const int64_t nslice = kp / QBK;
+  const int64_t tot = nslice * 4 * RP;
+  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
+  int tev = -1;
+  ECALJ_TRY(timing_begin(h, 1, &tev));
+  hipLaunchKernelGGL(x0_split_kernel, dim3(blocks), dim3(256), 0, h->stream, z, (const int64_t*)d,
+                     (const double*)(d + o_w), nslice, npr, npr_col, (int64_t)(ipr_col - 1), MP, NCP, RP, as, bs);
+  ECALJ_HIP(hipGetLastError());
+  const int grid = std::max(1, std::min(ntiles, h->ncu));
+  hipLaunchKernelGGL(x0_bf3_kernel, dim3(grid), dim3(512), 0, h->stream, (const int*)(d + o_til), ntiles,
+                     (const X0Bin*)(d + o_bin), as, bs, (const double*)(d + o_one), npr, npr_col, MP, NCP, mirror,
+                     rcxq, slice_elems);
+  ECALJ_HIP(hipGetLastError());
+  ECALJ_TRY(timing_end(h, 1, tev));
   return ECALJ_OK;
 }
 

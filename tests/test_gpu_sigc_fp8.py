@@ -58,36 +58,6 @@ def test_sigc_fp8(h8, orc, ngb, nm, ntq, niw):
     assert err < TOL_SIGC
 
 
-def test_sigc_fp8_variants(orc):
-    """The 8-wave and 16-wave kernels compute every output with the same MFMA
-    sequence: bit-identical to each other; the tuner records its choice."""
-    import ecalj_amd
-
-    rng = np.random.default_rng(8)
-    ngb, nm, ntq, niw = 203, 19, 9, 11
-    z = crand(rng, ngb, nm, ntq)
-    wc = crand(rng, ngb, ngb, niw + 1)
-    g = np.asfortranarray(rng.standard_normal((niw + 1, nm, ntq)))
-    ref = orc.sigc_imag(z, wc, g)
-    h = ecalj_amd.Handle(0, "bf16f8")
-    try:
-        outs = []
-        for mode in (1, 2):
-            say("sigc bf16f8 variant", mode)
-            h.set_tuning(mode)
-            outs.append(h.sigc(z, wc, g))
-            assert relerr(outs[-1], ref) < TOL_SIGC
-        assert np.array_equal(outs[0], outs[1])
-        h.set_tuning(0)
-        a = h.sigc(z, wc, g)
-        b = h.sigc(z, wc, g)
-        assert np.array_equal(a, outs[0]) and np.array_equal(b, outs[0])
-        rep = h.tuning_report()
-        assert rep.count("sigc_f8") == 1 and "*" in rep
-    finally:
-        h.close()
-
-
 def test_sigc_fp8_dynamic_range(h8, orc):
     """Rows, columns and frequencies of W over 10^-6..10^2 and zmel columns
     over 10^-3..10^3: each 32-value block gets its own scale, so no block
