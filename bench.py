@@ -40,6 +40,7 @@ PEAK = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "bf16f8": 2500.0}  # TFLOP/
 MEASURED_TRAFFIC = {
     ("C2", "bf16x3", "128x128x2/8w/xcd"): (49.7e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
     ("C2", "bf16x3", "128x128x2/8w"): (124.0e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
+    ("C2", "bf16f8", None): (48.8e9, "profiles/sigc_f8_c2_hbm_traffic.md"),  # one kernel, XCD-aware order
 }
 
 
