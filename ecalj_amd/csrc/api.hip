@@ -662,7 +662,12 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
       z.C = zM + es * ims; z.ldc = (int64_t)ngb * nm; z.bC = {ngb, zsz};
       z.batch = nm * nbatch; z.div = nm;
       z.alpha = ph; z.conjC = conj_now;
-      ECALJ_TRY(gemm(h, z));
+      if (mode == ECALJ_PREC_BF16X3)
+        ECALJ_TRY(zmel_z_bf3(h, md, nq, nv, (const ecalj::cf*)T, nv, nvmd, (bm == 1) ? 0 : nvmd * nm,
+                             (const ecalj::cf*)cphiq + ias, ndima, sq, false, ph, conj_now,
+                             (ecalj::cf*)zM + ims, (int64_t)ngb * nm, ngb, zsz, nm * nbatch, nm));
+      else
+        ECALJ_TRY(gemm(h, z));
     } else {
       ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, es * nvmd * nq * bq, (void**)&T));
       GemmCall g;  // T[(a+nv*I), (m,k)] = ppb^T * cphiq, A(r,b) = ppb[b + nv*r]
@@ -689,7 +694,12 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
       z.C = zM + es * ims; z.ldc = ngb; z.bC = {(int64_t)ngb * nm, zsz};
       z.batch = nq * nbatch; z.div = nq;
       z.alpha = ph; z.conjC = conj_now;
-      ECALJ_TRY(gemm(h, z));
+      if (mode == ECALJ_PREC_BF16X3)
+        ECALJ_TRY(zmel_z_bf3(h, md, nm, nv, (const ecalj::cf*)T, nv, nvmd, (bq == 1) ? 0 : nvmd * nq,
+                             (const ecalj::cf*)cphim + ias, ndima, sm, true, ph, conj_now, (ecalj::cf*)zM + ims,
+                             ngb, (int64_t)ngb * nm, zsz, nq * nbatch, nq));
+      else
+        ECALJ_TRY(gemm(h, z));
     }
     ias += nv;
     ims += md;

@@ -84,6 +84,13 @@ size_t sig_bf3_a_elems_per_k(int n, int nw1);
 ecalj_status x0_bf3(ecalj_ctx* h, const cf* z, int npr, int npr_col, int ipr_col, int nbin, const int64_t* cnt,
                     const int64_t* col, const double* wt, const int* slice, cf* rcxq, int64_t slice_elems,
                     int mirror);
+// zmel's second contraction in the bf16x3 form: C_b = s(alpha T_b^T opB(X_b)),
+// T_b (K x M, column stride lda) at A + (b % div) a_s0 + (b / div) a_s1, X_b
+// (K x N, ldb) at B + (b / div) b_s1 (b_s1 = 0: shared), opB = conj when
+// conjB, s = conj when conjC; C_b (M x N, ldc) at C + (b % div) c_s0 + (b / div) c_s1
+ecalj_status zmel_z_bf3(ecalj_ctx* h, int M, int N, int K, const cf* A, int64_t lda, int64_t a_s0, int64_t a_s1,
+                        const cf* B, int64_t ldb, int64_t b_s1, bool conjB, cd alpha, bool conjC, cf* C,
+                        int64_t ldc, int64_t c_s0, int64_t c_s1, int batch, int div);
 // Batched complex GEMM on the bf16x3 core of the Sigma_c kernel:
 // C_b = alpha op(A_b) op(B_b) (m x ncol, leading dimension m, stride c_stride,
 // complex64).  An operand is complex128 (wide) or complex64 with leading

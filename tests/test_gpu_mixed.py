@@ -48,12 +48,14 @@ def test_zmel_mixed(hp, orc):
     h.set_ppb(nv, md, ppb)
     m2e, _ = np.linalg.qr(crand(rng, 22, 22))
     h.set_coulomb(rng.uniform(0.1, 5.0, 22), np.asfortranarray(m2e))
-    cm, cq = crand(rng, 19, 4, 3), crand(rng, 19, 37, 3)
-    for is_m, conj in ((True, True), (False, False)):
-        g = h.zmel(cm, cq, phase=phase, is_m_basis=is_m, zmelconjg=conj)
-        for b in range(3):
-            ref = orc.zmel(nv, md, phase, ppb, cm[:, :, b], cq[:, :, b], m2e=None if is_m else m2e, conjg=conj)
-            assert relerr(g[..., b], ref) < TOL[prec]
+    # both contraction orders: occupied side first (nm <= nq) and end side first (nm > nq)
+    for nmc, nqc in ((4, 37), (37, 4)):
+        cm, cq = crand(rng, 19, nmc, 3), crand(rng, 19, nqc, 3)
+        for is_m, conj in ((True, True), (False, False), (True, False)):
+            g = h.zmel(cm, cq, phase=phase, is_m_basis=is_m, zmelconjg=conj)
+            for b in range(3):
+                ref = orc.zmel(nv, md, phase, ppb, cm[:, :, b], cq[:, :, b], m2e=None if is_m else m2e, conjg=conj)
+                assert relerr(g[..., b], ref) < TOL[prec], (nmc, nqc, is_m, conj)
 
 
 def test_x0gemm_mixed(hp, orc):
