@@ -271,6 +271,8 @@ static ecalj_status hilbert_t(ecalj_ctx* h, T* rcxq, int64_t slice, int nwhis, i
       hilbert_launch<8, T>(h, rcxq, slice, nwhis, c, ldc, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
     else if (c <= 16)
       hilbert_launch<16, T>(h, rcxq, slice, nwhis, c, ldc, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
+    else if (c <= 24)  // nf + 1 = 17..24 imaginary frequencies: no FMAs on 8 padded outputs
+      hilbert_launch<24, T>(h, rcxq, slice, nwhis, c, ldc, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
     else
       hilbert_launch<32, T>(h, rcxq, slice, nwhis, c, ldc, d_coef + v0, sc, v0, d_dscale, out + n * v0, n);
     ECALJ_HIP(hipGetLastError());

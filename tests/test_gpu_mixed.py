@@ -79,6 +79,31 @@ def test_x0gemm_mixed(hp, orc):
     assert relerr(g, o) < TOL[prec]
 
 
+@pytest.mark.parametrize("npm,ipr_col,npr_col", [(2, 1, 97), (1, 5, 40), (2, 9, 88)])
+def test_x0gemm_mixed_blocks(hp, orc, npm, ipr_col, npr_col):
+    """Column blocks of rcxq (ipr_col > 1: no Hermitian mirror) and both
+    signs of the frequency (npm = 2: slices -nwhis..nwhis)."""
+    h, prec = hp
+    say("x0gemm blocks", prec, npm, ipr_col, npr_col)
+    rng = np.random.default_rng(100 + 10 * npm + ipr_col)
+    npr, nwhis, ns1, ns2, nq, npairs = 97, 9, 2, 5, 5, 300
+    z = crand(rng, npr, ns2 - ns1 + 1, nq)
+    itc = rng.integers(ns1, ns2 + 1, npairs).astype(np.int32)
+    itpc = rng.integers(1, nq + 1, npairs).astype(np.int32)
+    iwini = rng.integers(1, nwhis - 2, npairs).astype(np.int32)
+    nhw = rng.integers(1, 3, npairs).astype(np.int32)
+    iwend = (iwini + nhw - 1).astype(np.int32)
+    icouini = np.concatenate([[1], 1 + np.cumsum(nhw)[:-1]]).astype(np.int32)
+    whwc = rng.uniform(0.1, 1.0, int(nhw.sum()))
+    jpmc = rng.integers(1, npm + 1, npairs).astype(np.int32)
+    nsl = nwhis + 1 if npm == 1 else 2 * nwhis + 1
+    r0 = np.asfortranarray(crand(rng, npr, npr_col, nsl) * 0.1)
+    args = (npr, ipr_col, npr_col, nwhis, npm, ns1, ns2, z, 1, npairs, iwini, iwend, itc, itpc, jpmc, icouini, whwc)
+    g = h.x0gemm(r0.copy(order="F"), *args)
+    o = orc.x0gemm(r0.copy(order="F"), *args)
+    assert relerr(g, o) < TOL[prec]
+
+
 # (ngb, nm, ntq, niw): ragged sizes, tile-aligned sizes, and more than 32
 # frequencies (the bf16x3 kernel then accumulates over two launches)
 @pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10), (161, 13, 5, 40), (1, 3, 1, 2)])
