@@ -37,36 +37,43 @@ __device__ __host__ inline int chunk_slot(int row, int chunk) { return chunk ^ (
 // ---------------------------------------------------------------- pre-split
 // A[b][(((v*NS + ks)*4 + p)*MP + I)*32 + slot] from W (complex128, ldw, per-batch stride w_stride)
 constexpr int SPT = 32;
+constexpr int SPLIT_KS = 4;  // k slices per block of split_w_kernel / split_w8_kernel
 __global__ __launch_bounds__(256) void split_w_kernel(const cd* __restrict__ w, int64_t ldw, int64_t w_stride,
                                                       int n, int nw1, int MP, int NS,
                                                       unsigned short* __restrict__ as, int64_t as_stride) {
   __shared__ cd up[SPT][SPT + 1];  // up[jj][ii] = W(I0+ii, J0+jj)
   __shared__ cd lo[SPT][SPT + 1];  // lo[ii][jj] = W(J0+jj, I0+ii)
-  const int ks = blockIdx.x, J0 = ks * SPT, I0 = blockIdx.y * SPT;
+  const int I0 = blockIdx.y * SPT;
   const int v = blockIdx.z % nw1, b = blockIdx.z / nw1;
   const cd* W = w + w_stride * b + (int64_t)ldw * n * v;
   const int tx = threadIdx.x % SPT, ty = threadIdx.x / SPT;  // 32 x 8
-  // a tile wholly above the diagonal reads only W(I,J), wholly below only W(J,I)
-  const bool need_up = I0 <= J0 + SPT - 1, need_lo = I0 + SPT - 1 > J0;
-  for (int r = ty; r < SPT; r += 8) {
-    if (need_up) {
-      const int I = I0 + tx, J = J0 + r;
-      up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
+  typedef unsigned short us4v __attribute__((ext_vector_type(4)));
+  // SPLIT_KS consecutive k slices per block
+  for (int kq = 0; kq < SPLIT_KS; ++kq) {
+    const int ks = blockIdx.x * SPLIT_KS + kq;
+    if (ks >= NS) break;
+    const int J0 = ks * SPT;
+    if (kq) __syncthreads();  // the previous slice's LDS reads are done
+    // a tile wholly above the diagonal reads only W(I,J), wholly below only W(J,I)
+    const bool need_up = I0 <= J0 + SPT - 1, need_lo = I0 + SPT - 1 > J0;
+    for (int r = ty; r < SPT; r += 8) {
+      if (need_up) {
+        const int I = I0 + tx, J = J0 + r;
+        up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
+      }
+      if (need_lo) {
+        const int J2 = J0 + tx, I2 = I0 + r;
+        lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
+      }
     }
-    if (need_lo) {
-      const int J2 = J0 + tx, I2 = I0 + r;
-      lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
-    }
-  }
-  __syncthreads();
-  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
-  if (threadIdx.x < SPT * 4) {
-    const int r = threadIdx.x >> 2, ch = threadIdx.x & 3;
+    __syncthreads();
+    // every thread: row r, 4 k values (half hf of chunk ch)
+    const int r = threadIdx.x >> 3, ch = (threadIdx.x >> 1) & 3, hf = threadIdx.x & 1;
     const int I = I0 + r;
-    us8v rh, rl, ih, il;
+    us4v rh, rl, ih, il;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int jj = 8 * ch + e, J = J0 + jj;
+    for (int e = 0; e < 4; ++e) {
+      const int jj = 8 * ch + 4 * hf + e, J = J0 + jj;
       double xr = 0.0, xi = 0.0;
       if (I < n && J < n) {
         if (I <= J) {
@@ -87,11 +94,11 @@ __global__ __launch_bounds__(256) void split_w_kernel(const cd* __restrict__ w, 
     }
     const int64_t pst = (int64_t)MP * QBK;
     unsigned short* A = as + as_stride * b + ((int64_t)v * NS + ks) * 4 * pst + (int64_t)I * QBK +
-                        8 * chunk_slot(I, ch);
-    *reinterpret_cast<us8v*>(A) = rh;
-    *reinterpret_cast<us8v*>(A + pst) = rl;
-    *reinterpret_cast<us8v*>(A + 2 * pst) = ih;
-    *reinterpret_cast<us8v*>(A + 3 * pst) = il;
+                        8 * chunk_slot(I, ch) + 4 * hf;
+    *reinterpret_cast<us4v*>(A) = rh;
+    *reinterpret_cast<us4v*>(A + pst) = rl;
+    *reinterpret_cast<us4v*>(A + 2 * pst) = ih;
+    *reinterpret_cast<us4v*>(A + 3 * pst) = il;
   }
 }
 
@@ -441,7 +448,7 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
   const int64_t as_stride = 4LL * nw1 * MP * NP;
   const int64_t bs_stride = 4LL * NCP * NP;
   // W planes: the split grid covers all MP x NP, so padding comes out zero
-  dim3 gw(NS, MP / SPT, nw1 * nb);
+  dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nb);
   if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
   hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
   ECALJ_HIP(hipGetLastError());
@@ -517,7 +524,7 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
     co += (int64_t)n * ncol[k];
     cols += ncol[k];
   }
-  dim3 gw(NS, MP / SPT, nw1 * nk);
+  dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nk);
   if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
   hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
   ECALJ_HIP(hipGetLastError());
@@ -826,43 +833,49 @@ __global__ __launch_bounds__(256) void split_w8_kernel(const cd* __restrict__ w,
                                                        unsigned char* __restrict__ as, int64_t as_stride) {
   __shared__ cd up[SPT][SPT + 1];  // up[jj][ii] = W(I0+ii, J0+jj)
   __shared__ cd lo[SPT][SPT + 1];  // lo[ii][jj] = W(J0+jj, I0+ii)
-  const int ks = blockIdx.x, J0 = ks * SPT, I0 = blockIdx.y * SPT;
+  const int I0 = blockIdx.y * SPT;
   const int v = blockIdx.z % nw1, b = blockIdx.z / nw1;
   const cd* W = w + w_stride * b + (int64_t)ldw * n * v;
   const int tx = threadIdx.x % SPT, ty = threadIdx.x / SPT;
-  const bool need_up = I0 <= J0 + SPT - 1, need_lo = I0 + SPT - 1 > J0;
-  for (int r = ty; r < SPT; r += 8) {
-    if (need_up) {
-      const int I = I0 + tx, J = J0 + r;
-      up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
-    }
-    if (need_lo) {
-      const int J2 = J0 + tx, I2 = I0 + r;
-      lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < SPT * 4) {  // waves 0 and 1, whole: 4 consecutive lanes per row
-    const int r = threadIdx.x >> 2, ch = threadIdx.x & 3;
-    const int I = I0 + r;
-    float xr[8], xi[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int jj = 8 * ch + e, J = J0 + jj;
-      double a = 0.0, c = 0.0;
-      if (I < n && J < n) {
-        if (I <= J) {
-          a = up[jj][r].re;
-          c = up[jj][r].im;
-        } else {
-          a = lo[r][jj].re;
-          c = -lo[r][jj].im;
-        }
+  for (int kq = 0; kq < SPLIT_KS; ++kq) {  // SPLIT_KS consecutive k slices per block
+    const int ks = blockIdx.x * SPLIT_KS + kq;
+    if (ks >= NS) break;
+    const int J0 = ks * SPT;
+    if (kq) __syncthreads();  // the previous slice's LDS reads are done
+    const bool need_up = I0 <= J0 + SPT - 1, need_lo = I0 + SPT - 1 > J0;
+    for (int r = ty; r < SPT; r += 8) {
+      if (need_up) {
+        const int I = I0 + tx, J = J0 + r;
+        up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
       }
-      xr[e] = (float)a;
-      xi[e] = (float)c;
+      if (need_lo) {
+        const int J2 = J0 + tx, I2 = I0 + r;
+        lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
+      }
     }
-    f8_put_row(xr, xi, I, ch, MP, as + as_stride * b + ((int64_t)v * NS + ks) * F8_ROWB * MP);
+    __syncthreads();
+    if (threadIdx.x < SPT * 4) {  // waves 0 and 1, whole: 4 consecutive lanes per row
+      const int r = threadIdx.x >> 2, ch = threadIdx.x & 3;
+      const int I = I0 + r;
+      float xr[8], xi[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int jj = 8 * ch + e, J = J0 + jj;
+        double a = 0.0, c = 0.0;
+        if (I < n && J < n) {
+          if (I <= J) {
+            a = up[jj][r].re;
+            c = up[jj][r].im;
+          } else {
+            a = lo[r][jj].re;
+            c = -lo[r][jj].im;
+          }
+        }
+        xr[e] = (float)a;
+        xi[e] = (float)c;
+      }
+      f8_put_row(xr, xi, I, ch, MP, as + as_stride * b + ((int64_t)v * NS + ks) * F8_ROWB * MP);
+    }
   }
 }
 
@@ -1130,7 +1143,7 @@ ecalj_status sig_f8(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int 
   const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NCP = pad_to(ncol, QBN_MAX), NS = NP / QBK;
   const int64_t as_stride = (int64_t)F8_ROWB * MP * NS * nw1;  // bytes
   const int64_t bs_stride = (int64_t)F8_ROWB * NCP * NS;
-  dim3 gw(NS, MP / SPT, nw1 * nb);
+  dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nb);
   if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
   hipLaunchKernelGGL(split_w8_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
   ECALJ_HIP(hipGetLastError());
@@ -1179,7 +1192,7 @@ ecalj_status sig_f8_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zme
     co += (int64_t)n * ncol[k];
     cols += ncol[k];
   }
-  dim3 gw(NS, MP / SPT, nw1 * nk);
+  dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nk);
   if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
   hipLaunchKernelGGL(split_w8_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
   ECALJ_HIP(hipGetLastError());
