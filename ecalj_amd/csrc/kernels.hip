@@ -503,7 +503,10 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n,
     p[i][l] = A[(k0 + i) + lda * (k0 + l)];
   }
   __syncthreads();
-  // in-place Gauss-Jordan on the b x b block
+  // in-place Gauss-Jordan on the b x b block; each step reads the old values
+  // of a thread's elements (b*b <= 4 * blockDim), then writes: two barriers
+  //   row kk: x/piv; column kk: -f/piv; pivot: 1/piv; others: y - f * (x/piv)
+  constexpr int EPT = GJB * GJB / 256;
   for (int kk = 0; kk < b; ++kk) {
     const cd piv = p[kk][kk];
     if (!(hypot(piv.re, piv.im) > 1e-300)) {
@@ -511,29 +514,27 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(cd* __restrict__ a, int n,
     }
     const double mag = piv.re * piv.re + piv.im * piv.im;
     const double dr = piv.re / mag, di = -piv.im / mag;  // 1/piv
-    __syncthreads();
-    // row kk scaled by 1/piv (except the pivot column)
-    for (int l = tid; l < b; l += blockDim.x) {
-      if (l == kk) continue;
-      cd x = p[kk][l];
-      p[kk][l] = cd{x.re * dr - x.im * di, x.re * di + x.im * dr};
-    }
-    __syncthreads();
-    // other rows: p[i][l] -= p[i][kk] * p[kk][l]
-    for (int e = tid; e < b * b; e += blockDim.x) {
-      const int i = e % b, l = e / b;
-      if (i == kk || l == kk) continue;
-      cd f = p[i][kk], x = p[kk][l];
-      p[i][l] = cd{p[i][l].re - (f.re * x.re - f.im * x.im), p[i][l].im - (f.re * x.im + f.im * x.re)};
-    }
-    __syncthreads();
-    // pivot column: p[i][kk] = -p[i][kk]/piv ; pivot = 1/piv
-    for (int i = tid; i < b; i += blockDim.x) {
-      cd f = p[i][kk];
-      if (i == kk)
-        p[i][kk] = cd{dr, di};
+    cd nv[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = tid + 256 * q, i = e % GJB, l = e / GJB;
+      if (i >= b || l >= b) continue;
+      const cd x = p[kk][l], f = p[i][kk];
+      const cd xs = cd{x.re * dr - x.im * di, x.re * di + x.im * dr};
+      if (i == kk && l == kk)
+        nv[q] = cd{dr, di};
+      else if (i == kk)
+        nv[q] = xs;
+      else if (l == kk)
+        nv[q] = cd{-(f.re * dr - f.im * di), -(f.re * di + f.im * dr)};
       else
-        p[i][kk] = cd{-(f.re * dr - f.im * di), -(f.re * di + f.im * dr)};
+        nv[q] = cd{p[i][l].re - (f.re * xs.re - f.im * xs.im), p[i][l].im - (f.re * xs.im + f.im * xs.re)};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = tid + 256 * q, i = e % GJB, l = e / GJB;
+      if (i < b && l < b) p[i][l] = nv[q];
     }
     __syncthreads();
   }
