@@ -220,7 +220,31 @@ __global__ __launch_bounds__(256) void hilbert_kernel(T* __restrict__ rcxq, int6
   double ar[NV], ai[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) { ar[v] = 0.0; ai[v] = 0.0; }
-  for (int iw = 1; iw <= nwhis; ++iw) {
+  // four bins per round: their loads are issued before any store-back, so
+  // four reads of rcxq per lane are in flight (the loop is latency bound)
+  int iw = 1;
+  for (; iw + 3 <= nwhis; iw += 4) {
+    double xr[4], xi[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ld_c2(rcxq + e + slice * (iw + u), xr[u], xi[u]);
+    if (scale) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double s = scale[iw + u - 1];
+        st_c(rcxq + e + slice * (iw + u), s * xr[u], s * xi[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double* c = coef + (int64_t)ldc * (iw + u - 1);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        ar[v] += c[v] * xr[u];
+        ai[v] += c[v] * xi[u];
+      }
+    }
+  }
+  for (; iw <= nwhis; ++iw) {
     double xr, xi;
     ld_c2(rcxq + e + slice * iw, xr, xi);
     if (scale) {
