@@ -19,6 +19,7 @@
 // 16 land in distinct 16-byte bank groups.
 #include <algorithm>
 #include <numeric>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -1333,81 +1334,121 @@ __global__ __launch_bounds__(512, 1) void x0_bf3_kernel(const int* __restrict__ 
   }
 }
 
+// Scratch for the gathered planes of one group of bins (A and B planes):
+// larger entry lists run as several groups, each accumulating into rcxq.
+constexpr size_t X0_SCRATCH = (size_t)4 << 30;
+
 ecalj_status x0_bf3(ecalj_ctx* h, const cf* z, int npr, int npr_col, int ipr_col, int nbin, const int64_t* cnt,
                     const int64_t* col, const double* wt, const int* slice, cf* rcxq, int64_t slice_elems,
                     int mirror) {
   const int MP = pad_to(std::max(npr, 1), QBM), NCP = pad_to(std::max(npr_col, 1), 128);
   const int RP = std::max(MP, NCP);
-  // padded K per bin, host tables
-  std::vector<X0Bin> bins;
-  int64_t kp = 0;
+  const int R = MP / QBM, Ct = NCP / 128;
+  // entries per group (multiple of 32): the group's planes fit X0_SCRATCH;
+  // ECALJ_X0_GROUP_ENTRIES sets a smaller group (tests of the grouping)
+  int64_t kcap = std::max<int64_t>(QBK, (int64_t)(X0_SCRATCH / (8 * (size_t)(MP + NCP))) / QBK * QBK);
+  if (const char* e = std::getenv("ECALJ_X0_GROUP_ENTRIES")) {
+    const long long v = std::atoll(e);
+    if (v > 0) kcap = std::min<int64_t>(kcap, std::max<int64_t>(QBK, (int64_t)v / QBK * QBK));
+  }
+  // pieces: (bin, first entry, entries) with at most kcap entries; groups of consecutive pieces
+  struct Piece { int bin; int64_t t0, k; };
+  std::vector<std::vector<Piece>> groups(1);
+  int64_t gk = 0;
   for (int b = 0; b < nbin; ++b) {
     const int64_t K = cnt[b + 1] - cnt[b];
-    if (K == 0) continue;
-    const int64_t k32 = (K + QBK - 1) / QBK * QBK;
-    if (k32 / QBK > 0x7fffffff) return ECALJ_ERR_DIM;
-    bins.push_back({kp / QBK * 4 * (int64_t)MP * QBK, kp / QBK * 4 * (int64_t)NCP * QBK, (int)(k32 / QBK), slice[b]});
-    kp += k32;
-  }
-  if (bins.empty()) return ECALJ_OK;
-  std::vector<int64_t> colp(kp, -1);
-  std::vector<double> wp(kp, 0.0);
-  {
-    int64_t o = 0;
-    for (int b = 0; b < nbin; ++b) {
-      const int64_t K = cnt[b + 1] - cnt[b];
-      if (K == 0) continue;
-      for (int64_t t = 0; t < K; ++t) {
-        colp[o + t] = col[cnt[b] + t];
-        wp[o + t] = wt[cnt[b] + t];
+    for (int64_t t0 = 0; t0 < K; t0 += kcap) {
+      const int64_t k = std::min(kcap, K - t0), k32 = (k + QBK - 1) / QBK * QBK;
+      if (gk + k32 > kcap) {
+        groups.emplace_back();
+        gk = 0;
       }
-      o += (K + QBK - 1) / QBK * QBK;
+      groups.back().push_back({b, cnt[b] + t0, k});
+      gk += k32;
     }
   }
-  // tiles: longest K first (persistent grid)
-  const int R = MP / QBM, Ct = NCP / 128;
-  std::vector<int> order(bins.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bins[x].ns > bins[y].ns; });
-  std::vector<int> tiles;
-  for (int b : order)
-    for (int ti = 0; ti < R; ++ti)
-      for (int tj = mirror ? ti : 0; tj < Ct; ++tj) {
-        tiles.push_back(b);
-        tiles.push_back(ti);
-        tiles.push_back(tj);
+  if (groups.back().empty()) groups.pop_back();
+  if (groups.empty()) return ECALJ_OK;
+  // blobs of every group first: the list slot and the planes are sized once (no growth between groups)
+  struct Blob { std::vector<unsigned char> v; size_t o_w, o_one, o_bin, o_til; int64_t kp; int ntiles; };
+  std::vector<Blob> blobs(groups.size());
+  size_t maxblob = 0;
+  int64_t maxkp = 0;
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const std::vector<Piece>& g = groups[gi];
+    std::vector<X0Bin> bins;
+    int64_t kp = 0;
+    for (const Piece& pc : g) {
+      const int64_t k32 = (pc.k + QBK - 1) / QBK * QBK;
+      if (k32 / QBK > 0x7fffffff) return ECALJ_ERR_DIM;
+      bins.push_back({kp / QBK * 4 * (int64_t)MP * QBK, kp / QBK * 4 * (int64_t)NCP * QBK, (int)(k32 / QBK),
+                      slice[pc.bin]});
+      kp += k32;
+    }
+    std::vector<int64_t> colp(kp, -1);
+    std::vector<double> wp(kp, 0.0);
+    int64_t o = 0;
+    for (const Piece& pc : g) {
+      for (int64_t t = 0; t < pc.k; ++t) {
+        colp[o + t] = col[pc.t0 + t];
+        wp[o + t] = wt[pc.t0 + t];
       }
-  const int ntiles = (int)(tiles.size() / 3);
-  // one upload: [colp | wp | ones (NCP) | bins | tiles]
-  const size_t o_w = sizeof(int64_t) * kp, o_one = o_w + sizeof(double) * kp, o_bin = o_one + sizeof(double) * NCP,
-               o_til = o_bin + sizeof(X0Bin) * bins.size(), nbytes = o_til + sizeof(int) * tiles.size();
-  std::vector<unsigned char> blob(nbytes);
-  memcpy(blob.data(), colp.data(), o_w);
-  memcpy(blob.data() + o_w, wp.data(), o_one - o_w);
-  for (int c = 0; c < NCP; ++c) {
-    const double one = 1.0;
-    memcpy(blob.data() + o_one + sizeof(double) * c, &one, sizeof(double));
+      o += (pc.k + QBK - 1) / QBK * QBK;
+    }
+    // tiles: longest K first (persistent grid)
+    std::vector<int> order(bins.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bins[x].ns > bins[y].ns; });
+    std::vector<int> tiles;
+    for (int b : order)
+      for (int ti = 0; ti < R; ++ti)
+        for (int tj = mirror ? ti : 0; tj < Ct; ++tj) {
+          tiles.push_back(b);
+          tiles.push_back(ti);
+          tiles.push_back(tj);
+        }
+    // blob: [colp | wp | ones (NCP) | bins | tiles]
+    Blob& B = blobs[gi];
+    B.kp = kp;
+    B.ntiles = (int)(tiles.size() / 3);
+    B.o_w = sizeof(int64_t) * kp;
+    B.o_one = B.o_w + sizeof(double) * kp;
+    B.o_bin = B.o_one + sizeof(double) * NCP;
+    B.o_til = B.o_bin + sizeof(X0Bin) * bins.size();
+    B.v.resize(B.o_til + sizeof(int) * tiles.size());
+    memcpy(B.v.data(), colp.data(), B.o_w);
+    memcpy(B.v.data() + B.o_w, wp.data(), B.o_one - B.o_w);
+    for (int c = 0; c < NCP; ++c) {
+      const double one = 1.0;
+      memcpy(B.v.data() + B.o_one + sizeof(double) * c, &one, sizeof(double));
+    }
+    memcpy(B.v.data() + B.o_bin, bins.data(), B.o_til - B.o_bin);
+    memcpy(B.v.data() + B.o_til, tiles.data(), sizeof(int) * tiles.size());
+    maxblob = std::max(maxblob, B.v.size());
+    maxkp = std::max(maxkp, kp);
   }
-  memcpy(blob.data() + o_bin, bins.data(), o_til - o_bin);
-  memcpy(blob.data() + o_til, tiles.data(), nbytes - o_til);
   unsigned char* d;
-  ECALJ_TRY(upload_blob(h, ecalj_ctx::S_LIST, blob, &d));
   unsigned short *as, *bs;
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0A, sizeof(unsigned short) * 4 * (size_t)MP * kp, (void**)&as));
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0B, sizeof(unsigned short) * 4 * (size_t)NCP * kp, (void**)&bs));
-  const int64_t nslice = kp / QBK;
-  const int64_t tot = nslice * 4 * RP;
-  const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_LIST, maxblob, (void**)&d));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0A, sizeof(unsigned short) * 4 * (size_t)MP * maxkp, (void**)&as));
+  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0B, sizeof(unsigned short) * 4 * (size_t)NCP * maxkp, (void**)&bs));
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 1, &tev));
-  hipLaunchKernelGGL(x0_split_kernel, dim3(blocks), dim3(256), 0, h->stream, z, (const int64_t*)d,
-                     (const double*)(d + o_w), nslice, npr, npr_col, (int64_t)(ipr_col - 1), MP, NCP, RP, as, bs);
-  ECALJ_HIP(hipGetLastError());
-  const int grid = std::max(1, std::min(ntiles, h->ncu));
-  hipLaunchKernelGGL(x0_bf3_kernel, dim3(grid), dim3(512), 0, h->stream, (const int*)(d + o_til), ntiles,
-                     (const X0Bin*)(d + o_bin), as, bs, (const double*)(d + o_one), npr, npr_col, MP, NCP, mirror,
-                     rcxq, slice_elems);
-  ECALJ_HIP(hipGetLastError());
+  for (const Blob& B : blobs) {
+    ECALJ_TRY(upload_blob(h, ecalj_ctx::S_LIST, B.v, &d));
+    const int64_t nslice = B.kp / QBK;
+    const int64_t tot = nslice * 4 * RP;
+    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
+    hipLaunchKernelGGL(x0_split_kernel, dim3(blocks), dim3(256), 0, h->stream, z, (const int64_t*)d,
+                       (const double*)(d + B.o_w), nslice, npr, npr_col, (int64_t)(ipr_col - 1), MP, NCP, RP, as,
+                       bs);
+    ECALJ_HIP(hipGetLastError());
+    const int grid = std::max(1, std::min(B.ntiles, h->ncu));
+    hipLaunchKernelGGL(x0_bf3_kernel, dim3(grid), dim3(512), 0, h->stream, (const int*)(d + B.o_til), B.ntiles,
+                       (const X0Bin*)(d + B.o_bin), as, bs, (const double*)(d + B.o_one), npr, npr_col, MP, NCP,
+                       mirror, rcxq, slice_elems);
+    ECALJ_HIP(hipGetLastError());
+  }
   ECALJ_TRY(timing_end(h, 1, tev));
   return ECALJ_OK;
 }

@@ -104,6 +104,39 @@ def test_x0gemm_mixed_blocks(hp, orc, npm, ipr_col, npr_col):
     assert relerr(g, o) < TOL[prec]
 
 
+def test_x0gemm_bf16x3_groups(orc, monkeypatch):
+    """The bf16x3 x0gemm gathers each bin's entries into scratch planes in
+    groups of bounded size; with tiny groups (64 entries) bins are split over
+    groups and accumulate into rcxq: same result as one group, to rounding."""
+    import ecalj_amd
+
+    rng = np.random.default_rng(23)
+    npr, nwhis, ns1, ns2, nq, npairs = 130, 7, 1, 4, 6, 500
+    z = crand(rng, npr, ns2 - ns1 + 1, nq)
+    itc = rng.integers(1, ns2 + 1, npairs).astype(np.int32)
+    itpc = rng.integers(1, nq + 1, npairs).astype(np.int32)
+    iwini = rng.integers(1, nwhis - 2, npairs).astype(np.int32)
+    nhw = rng.integers(1, 3, npairs).astype(np.int32)
+    iwend = (iwini + nhw - 1).astype(np.int32)
+    icouini = np.concatenate([[1], 1 + np.cumsum(nhw)[:-1]]).astype(np.int32)
+    whwc = rng.uniform(0.1, 1.0, int(nhw.sum()))
+    jpmc = np.ones(npairs, np.int32)
+    r0 = np.zeros((npr, npr, nwhis + 1), complex, order="F")
+    args = (npr, 1, npr, nwhis, 1, ns1, ns2, z, 1, npairs, iwini, iwend, itc, itpc, jpmc, icouini, whwc)
+    o = orc.x0gemm(r0.copy(order="F"), *args)
+    h = ecalj_amd.Handle(0, "bf16x3")
+    try:
+        one = h.x0gemm(r0.copy(order="F"), *args)
+        monkeypatch.setenv("ECALJ_X0_GROUP_ENTRIES", "64")
+        say("x0gemm bf16x3 in groups of 64 entries")
+        many = h.x0gemm(r0.copy(order="F"), *args)
+    finally:
+        h.close()
+    assert relerr(one, o) < TOL["bf16x3"]
+    assert relerr(many, o) < TOL["bf16x3"]
+    assert relerr(many, one) < 1e-5
+
+
 # (ngb, nm, ntq, niw): ragged sizes, tile-aligned sizes, and more than 32
 # frequencies (the bf16x3 kernel then accumulates over two launches)
 @pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10), (161, 13, 5, 40), (1, 3, 1, 2)])
