@@ -652,7 +652,12 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
         g.bB = {sm, 0}; g.bC = {nvmd * nm, 0};
         g.batch = bm; g.div = bm;
       }
-      ECALJ_TRY(gemm(h, g));
+      if (mode == ECALJ_PREC_BF16X3)
+        ECALJ_TRY(zmel_p_bf3(h, (int)nvmd, g.N, nv, h->d_ppb2 + h->ppb_off[a], nvmd,
+                             (const ecalj::cf*)cphim + ias, ndima, g.batch > 1 ? sm : 0, true, (ecalj::cf*)T, nvmd,
+                             g.batch > 1 ? nvmd * nm : 0, g.batch));
+      else
+        ECALJ_TRY(gemm(h, g));
       GemmCall z;  // per (n,k): zmel(ims+I, n, m) = phase * sum_b T[b + nv*I + nvmd*(n + nm*k)] cphiq_k(ias+b, m)
       z.mode = mode;
       z.ta = true;

@@ -91,6 +91,11 @@ ecalj_status x0_bf3(ecalj_ctx* h, const cf* z, int npr, int npr_col, int ipr_col
 ecalj_status zmel_z_bf3(ecalj_ctx* h, int M, int N, int K, const cf* A, int64_t lda, int64_t a_s0, int64_t a_s1,
                         const cf* B, int64_t ldb, int64_t b_s1, bool conjB, cd alpha, bool conjC, cf* C,
                         int64_t ldc, int64_t c_s0, int64_t c_s1, int batch, int div);
+// zmel's first contraction in the bf16x3 form: T_b = P opB(X_b), P (M x K,
+// real, lda) shared, X_b (K x N, K contiguous, ldb) at X + b x_s1, conjugated
+// when conjB; T_b (M x N, ldc) at C + b c_s1
+ecalj_status zmel_p_bf3(ecalj_ctx* h, int M, int N, int K, const double* P, int64_t lda, const cf* X,
+                        int64_t ldb, int64_t x_s1, bool conjB, cf* C, int64_t ldc, int64_t c_s1, int batch);
 // Batched complex GEMM on the bf16x3 core of the Sigma_c kernel:
 // C_b = alpha op(A_b) op(B_b) (m x ncol, leading dimension m, stride c_stride,
 // complex64).  An operand is complex128 (wide) or complex64 with leading
