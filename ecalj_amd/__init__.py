@@ -355,6 +355,12 @@ class Handle:
 
     # ---- state
     def set_precision(self, precision: str):
+        """"f64" (complex128, FP64 MFMA), "f32" (complex64 on FP32 MFMA),
+        "bf16x3" (complex64, hi/lo bf16 split products, FP32 accumulation) or
+        "bf16f8" (bf16x3, with the Sigma_c contraction's cross terms on the
+        scaled fp8 MFMA).  The epsilon inversion stays FP64 in every form."""
+        if precision not in PREC:
+            raise ValueError(f"precision {precision!r}: one of {sorted(PREC)}")
         self.precision = precision
         _chk(lib().ecalj_set_precision(self._h, PREC[precision]), "set_precision")
 
