@@ -553,7 +553,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "bf16f8", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM,ROT,F8,C3F8",
+    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM,ROT,F8,C3F8,C4F8",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -594,8 +594,8 @@ def main():
                 res[name] = bench_gamma(args.mixed, log_=log)
             elif name == "ROT":
                 res[name] = bench_rotwave(log_=log)
-            elif name == "C4":
-                res[name] = bench_c4(args.mixed, log_=log)
+            elif name in ("C4", "C4F8"):
+                res[name] = bench_c4(args.mixed if name == "C4" else "bf16f8", log_=log)
             elif name == "C5":
                 res[name] = bench_c5(log_=log)
             elif name.endswith("R") and name[:-1] in synth.CONFIGS:
@@ -659,8 +659,11 @@ def main():
         out["also"] = {}
     fp64_runs = {}  # config -> (inputs, FP64-form run) kept for a later F8 leg of the same config
     for name in also:
-        if name == "C4":
-            out["also"]["C4"] = bench_c4(args.mixed, log_=log)
+        if name in ("C4", "C4F8"):
+            try:
+                out["also"][name] = bench_c4(args.mixed if name == "C4" else "bf16f8", log_=log)
+            except Exception as e:  # report, never hide; the headline line still prints
+                out["also"][name] = {"error": repr(e)}
             continue
         if name == "C5":
             out["also"]["C5"] = bench_c5(log_=log)
