@@ -691,7 +691,11 @@ def main():
                 log(f"bench: generating {base} inputs")
                 q2 = synth.make_qpoint(synth.CONFIGS[base])
                 a64, st2 = run_form(q2, "f64", 2, 1), 2
-            a8 = run_form(q2, "bf16f8", st2, 1)
+            try:
+                a8 = run_form(q2, "bf16f8", st2, 1)
+            except Exception as e:  # report, never hide; the headline line still prints
+                out["also"][name] = {"error": repr(e)}
+                continue
             f2 = flops_per_q(synth.CONFIGS[base], nwhis_pairs_bins=a8["npb"])
             out["also"][name] = {
                 "workload": f"{base} q-point, precision bf16f8: bf16x3 except Sigma_c (bf16 hi*hi + scaled fp8 "
