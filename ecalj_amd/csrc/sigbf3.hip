@@ -23,17 +23,10 @@
 #include <cstring>
 #include <vector>
 
-#include "ctx.hpp"
-#include "gemm.hpp"
-#include "kernels.hpp"
+#include "sigtile.hpp"
 
 namespace ecalj {
 
-constexpr int QBM = 128, QBK = 32;  // block rows, k per stage
-constexpr int QNW = 32;                                // frequencies per launch (weight table in LDS)
-constexpr int F8_ROWB = 260;  // fp8 cross-term form: bytes per operand row and 32-wide k slice
-
-__device__ __host__ inline int chunk_slot(int row, int chunk) { return chunk ^ (((row >> 1) ^ (row >> 2)) & 3); }
 
 // ---------------------------------------------------------------- pre-split
 // A[b][(((v*NS + ks)*4 + p)*MP + I)*32 + slot] from W (complex128, ldw, per-batch stride w_stride)
@@ -142,193 +135,6 @@ __global__ __launch_bounds__(256) void split_z_kernel(const cf* __restrict__ z, 
   }
 }
 
-// ---------------------------------------------------------------- main kernel
-typedef __attribute__((address_space(3))) void* lds_ptr_t;
-typedef __attribute__((address_space(1))) void* gbl_ptr_t;
-typedef unsigned short us8q __attribute__((ext_vector_type(8)));
-typedef unsigned int u4q __attribute__((ext_vector_type(4)));
-
-template <int BN, int NSTG>
-struct alignas(16) SigSmem {
-  unsigned short a[NSTG][4][QBM * QBK];  // 8 KB per plane
-  unsigned short b[NSTG][4][BN * QBK];   // BN/16 KB per plane
-  float wt[QNW][BN];                     // per-frequency weights of the block's columns
-};
-
-// s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt[3:0] + vmcnt[5:4] at [15:14],
-// expcnt [6:4] and lgkmcnt [11:8] left at their maximum)
-#define ECALJ_WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(((N) & 0xF) | (((N) >> 4) << 14) | (0x7 << 4) | (0xF << 8))
-
-// One block tile of czmelwc: rows I0..I0+QBM of one internal k, columns
-// c0..c0+BN; NW waves in 2 x NW/2, wave tile 64 x 2BN/NW; NSTG stages of QBK
-// in LDS, NSTG-1 of them in flight.  A and B point at the k's plane sets
-// (A: 4 planes of MP x QBK per (v, ks); B: 4 planes of NCP x QBK per ks).
-// NW = 8 puts two waves on every SIMD: one wave alone issues a 16x16x32
-// bf16 MFMA only every ~30 cycles, two interleave to ~20.
-// AREAL: the A operand is real (its imaginary planes are zero and are not
-// multiplied): 6 MFMAs per 16x16 tile and k slice instead of 12.
-template <int BN, int NSTG, int NW, bool AREAL = false>
-__device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned short* __restrict__ Ak,
-                                         const unsigned short* __restrict__ Bk, const double* __restrict__ g,
-                                         int n, int nw1, int v0, int nwg, int ncol, int MP, int NS, int NCP,
-                                         int I0, int c0, int accumulate, cf* __restrict__ C, int mirror = 0,
-                                         int64_t ldc = 0) {
-  static_assert(NSTG == 2 || NSTG == 3, "two or three stages");
-  static_assert(NW == 4 || NW == 8, "four or eight waves");
-  constexpr int NT = NW * 64, WN = NW / 2;
-  constexpr int TM = 4, TN = BN / (16 * WN);          // 16x16 tiles per wave
-  constexpr int ARW = 4 * QBM * QBK * 2 / 1024 / NW;  // 1 KB A runs per wave and stage
-  constexpr int BRW = 4 * BN * QBK * 2 / 1024 / NW;   // 1 KB B runs per wave and stage
-  static_assert(TN >= 1 && ARW * NW == 4 * QBM * QBK * 2 / 1024 && BRW * NW == 4 * BN * QBK * 2 / 1024,
-                "runs split evenly over the waves");
-  constexpr int APP = QBM * QBK / 512, BPP = BN * QBK / 512;  // runs per plane
-  const int t = threadIdx.x, lane = t & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave / WN, wn = wave % WN;
-  const int64_t apst = (int64_t)MP * QBK, bpst = (int64_t)NCP * QBK;
-  const unsigned short* A = Ak + (int64_t)I0 * QBK + lane * 8;
-  const unsigned short* B = Bk + (int64_t)c0 * QBK + lane * 8;
-  const int nst = nwg * NS;
-
-  // weight table: wt[v][c] = wgt(v0+v, c0+c) (0 beyond ncol)
-  for (int e = t; e < nwg * BN; e += NT) {
-    const int v = e / BN, c = c0 + e % BN;
-    s.wt[v][e % BN] = (c < ncol) ? (float)g[(v0 + v) + (int64_t)nw1 * c] : 0.f;
-  }
-  __syncthreads();
-
-  auto issue = [&](int st, int buf) {
-    const int v = v0 + st / NS, ks = st % NS;
-    const unsigned short* a = A + ((int64_t)v * NS + ks) * 4 * apst;
-    const unsigned short* bb = B + (int64_t)ks * 4 * bpst;
-#pragma unroll
-    for (int q = 0; q < ARW; ++q) {
-      const int run = ARW * wave + q, p = run / APP, sub = run % APP;
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + p * apst + sub * 512), (lds_ptr_t)(&s.a[buf][p][sub * 512]),
-                                       16, 0, 0);
-    }
-#pragma unroll
-    for (int q = 0; q < BRW; ++q) {
-      const int run = BRW * wave + q, p = run / BPP, sub = run % BPP;
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(bb + p * bpst + sub * 512), (lds_ptr_t)(&s.b[buf][p][sub * 512]),
-                                       16, 0, 0);
-    }
-  };
-
-  f4 cr[TM][TN], ci[TM][TN], tr[TM][TN], ti[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      cr[i][j] = ci[i][j] = tr[i][j] = ti[i][j] = f4{0, 0, 0, 0};
-    }
-  // operand offsets in a plane: row r, logical chunk lane>>4, stored at chunk_slot(r, lane>>4);
-  // rows are 16-aligned tile rows + (lane & 15), so the slot depends on the lane only
-  const int slot = chunk_slot(lane & 15, lane >> 4);
-  const int aoff = (wm * TM * 16 + (lane & 15)) * QBK + 8 * slot;
-  const int boff = (wn * TN * 16 + (lane & 15)) * QBK + 8 * slot;
-
-  for (int q = 0; q < NSTG - 1; ++q)
-    if (q < nst) issue(q, q);
-  // frequency loop outside, k-slice loop inside: the inner loop touches only the
-  // per-frequency accumulators (MFMA-resident), the weighted totals only at
-  // the end of each frequency
-  int st = 0;
-  for (int vl = 0; vl < nwg; ++vl) {
-    for (int ks = 0; ks < NS; ++ks, ++st) {
-      const int buf = st % NSTG;
-      // stage st landed: the loads of later stages already issued may still be in flight
-      if (NSTG == 3 && st + 1 < nst) {
-        ECALJ_WAIT_VMCNT(ARW + BRW);
-      } else {
-        ECALJ_WAIT_VMCNT(0);
-      }
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      __builtin_amdgcn_s_barrier();
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      // the buffer of stage st+NSTG-1 was last read in iteration st-1, which every wave has left
-      if (st + NSTG - 1 < nst) issue(st + NSTG - 1, (st + NSTG - 1) % NSTG);
-
-      us8q brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const int o = boff + j * 16 * QBK;
-        brh[j] = *reinterpret_cast<const us8q*>(&s.b[buf][0][o]);
-        brl[j] = *reinterpret_cast<const us8q*>(&s.b[buf][1][o]);
-        bih[j] = *reinterpret_cast<const us8q*>(&s.b[buf][2][o]);
-        bil[j] = *reinterpret_cast<const us8q*>(&s.b[buf][3][o]);
-        // -b: flip the sign bit of every bf16 (two per 32-bit word)
-        nbh[j] = __builtin_bit_cast(us8q, __builtin_bit_cast(u4q, bih[j]) ^ 0x80008000u);
-        nbl[j] = __builtin_bit_cast(us8q, __builtin_bit_cast(u4q, bil[j]) ^ 0x80008000u);
-      }
-      // one A row tile at a time: its 4 planes meet the TN column tiles
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int o = aoff + i * 16 * QBK;
-        const us8q arh = *reinterpret_cast<const us8q*>(&s.a[buf][0][o]);
-        const us8q arl = *reinterpret_cast<const us8q*>(&s.a[buf][1][o]);
-#define ECALJ_Q_TERM(ACC, AF, BF)                                                                     \
-  _Pragma("unroll") for (int j = 0; j < TN; ++j) ACC[i][j] =                                          \
-      __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, AF), __builtin_bit_cast(bf8, BF[j]), \
-                                              ACC[i][j], 0, 0, 0);
-        // re = ar br - ai bi, im = ar bi + ai br; hi*hi + hi*lo + lo*hi of each
-        ECALJ_Q_TERM(cr, arh, brh)
-        ECALJ_Q_TERM(ci, arh, bih)
-        ECALJ_Q_TERM(cr, arh, brl)
-        ECALJ_Q_TERM(ci, arh, bil)
-        ECALJ_Q_TERM(cr, arl, brh)
-        ECALJ_Q_TERM(ci, arl, bih)
-        if constexpr (!AREAL) {
-          const us8q aih = *reinterpret_cast<const us8q*>(&s.a[buf][2][o]);
-          const us8q ail = *reinterpret_cast<const us8q*>(&s.a[buf][3][o]);
-          ECALJ_Q_TERM(cr, aih, nbh)
-          ECALJ_Q_TERM(ci, aih, brh)
-          ECALJ_Q_TERM(cr, aih, nbl)
-          ECALJ_Q_TERM(ci, aih, brl)
-          ECALJ_Q_TERM(cr, ail, nbh)
-          ECALJ_Q_TERM(ci, ail, brh)
-        }
-#undef ECALJ_Q_TERM
-      }
-    }
-    // end of frequency: tot += wgt(v,c) * acc
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const float wv = s.wt[vl][wn * TN * 16 + j * 16 + (lane & 15)];
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        tr[i][j] += wv * cr[i][j];
-        ti[i][j] += wv * ci[i][j];
-        cr[i][j] = f4{0, 0, 0, 0};
-        ci[i][j] = f4{0, 0, 0, 0};
-      }
-    }
-  }
-  const int64_t ld = ldc ? ldc : n;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int I = I0 + wm * TM * 16 + i * 16 + acc_row_f32(lane, e);
-        const int c = c0 + wn * TN * 16 + j * 16 + (lane & 15);
-        if (I < n && c < ncol) {
-          cf* p = C + I + ld * c;
-          if (accumulate) {
-            const cf o = *p;
-            *p = cf{o.re + tr[i][j][e], o.im + ti[i][j][e]};
-          } else {
-            *p = cf{tr[i][j][e], ti[i][j][e]};
-          }
-          if (mirror) {  // Hermitian output, off-diagonal tile: C(c, I) += conj
-            cf* q = C + c + ld * I;
-            const cf o = *q;
-            *q = cf{o.re + tr[i][j][e], o.im - ti[i][j][e]};
-          }
-        }
-      }
-}
-
 // Uniform batch.  G == 0: block (x, y, z) = (row tile, column tile, internal k).
 // G > 0, XCD-aware: a 1-D grid whose blocks the dispatcher deals round-robin
 // to the 8 XCDs (block p on XCD p % 8, in increasing p).  XCD x's j-th block
@@ -395,8 +201,6 @@ __global__ __launch_bounds__(NW * 64, 1) void sig_bf3_grouped_kernel(const int* 
 }
 
 // ---------------------------------------------------------------- launcher
-static inline int pad_to(int x, int m) { return (x + m - 1) / m * m; }
-constexpr int QBN_MAX = 128;  // columns are padded for the widest variant
 
 // Kernel variants of the tuner: {column tile, stages, waves, XCD-aware order}
 struct SigVariant { int bn, nstg, nw, xcd; const char* name; };
@@ -1239,405 +1043,5 @@ ecalj_status sig_f8_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zme
   return ECALJ_OK;
 }
 
-
-// ================================================================ x0gemm on the bf16x3 core
-// rcxq(:,:,slice(bin)) += sum_{t in bin} conj(z_t(i)) w_t z_t(ipr_col-1+j)
-// (S/x0gemm.f90:51-97) as one grouped GEMM per histogram bin: the bin's
-// columns are gathered once into pre-split bf16 planes (A = conj(z_t), B =
-// w_t z_t, K = the bin's entries padded to 32 with zeros), and every (bin,
-// row tile, column tile) runs the Sigma_c tile code (one frequency, unit
-// weights, accumulating).  A square output (ipr_col = 1, npr_col = npr) is
-// Hermitian: only tiles ti <= tj run, off-diagonal ones also add the
-// conjugate transpose.
-
-// one thread per (global K slice s, row r, chunk ch), ch fastest: the four
-// chunks of a row are adjacent lanes, so a wave writes 16 whole 64-byte rows
-// of each plane
-__global__ __launch_bounds__(256) void x0_split_kernel(const cf* __restrict__ z, const int64_t* __restrict__ colp,
-                                                       const double* __restrict__ wp, int64_t nslice, int npr,
-                                                       int npr_col, int64_t boff, int MP, int NCP, int RP,
-                                                       unsigned short* __restrict__ as,
-                                                       unsigned short* __restrict__ bs) {
-  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
-  const int64_t tot = nslice * 4 * RP;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-    const int ch = (int)(e % 4);
-    const int r = (int)((e / 4) % RP);
-    const int64_t sl = e / (4 * (int64_t)RP);
-    us8v arh, arl, aih, ail, brh, brl, bih, bil;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int64_t t = sl * QBK + 8 * ch + q;
-      const int64_t c = colp[t];
-      float xr = 0.f, xi = 0.f, yr = 0.f, yi = 0.f;
-      if (c >= 0) {
-        if (r < npr) {
-          const cf x = z[c + r];
-          xr = x.re;
-          xi = -x.im;
-        }
-        if (r < npr_col) {
-          const float w = (float)wp[t];
-          const cf y = z[c + boff + r];
-          yr = w * y.re;
-          yi = w * y.im;
-        }
-      }
-      unsigned short h, l;
-      split_bf16(xr, h, l);
-      arh[q] = h;
-      arl[q] = l;
-      split_bf16(xi, h, l);
-      aih[q] = h;
-      ail[q] = l;
-      split_bf16(yr, h, l);
-      brh[q] = h;
-      brl[q] = l;
-      split_bf16(yi, h, l);
-      bih[q] = h;
-      bil[q] = l;
-    }
-    if (r < MP) {
-      const int64_t pst = (int64_t)MP * QBK;
-      unsigned short* A = as + sl * 4 * pst + (int64_t)r * QBK + 8 * chunk_slot(r, ch);
-      *reinterpret_cast<us8v*>(A) = arh;
-      *reinterpret_cast<us8v*>(A + pst) = arl;
-      *reinterpret_cast<us8v*>(A + 2 * pst) = aih;
-      *reinterpret_cast<us8v*>(A + 3 * pst) = ail;
-    }
-    if (r < NCP) {
-      const int64_t pst = (int64_t)NCP * QBK;
-      unsigned short* B = bs + sl * 4 * pst + (int64_t)r * QBK + 8 * chunk_slot(r, ch);
-      *reinterpret_cast<us8v*>(B) = brh;
-      *reinterpret_cast<us8v*>(B + pst) = brl;
-      *reinterpret_cast<us8v*>(B + 2 * pst) = bih;
-      *reinterpret_cast<us8v*>(B + 3 * pst) = bil;
-    }
-  }
-}
-
-struct X0Bin {
-  int64_t aoff, boff;  // plane offsets (elements) of the bin's first K slice
-  int ns, slice;       // K slices, output slice
-};
-
-__global__ __launch_bounds__(512, 1) void x0_bf3_kernel(const int* __restrict__ tiles, int ntiles,
-                                                        const X0Bin* __restrict__ bins,
-                                                        const unsigned short* __restrict__ as,
-                                                        const unsigned short* __restrict__ bs,
-                                                        const double* __restrict__ ones, int npr, int npr_col,
-                                                        int MP, int NCP, int mirror, cf* __restrict__ rcxq,
-                                                        int64_t slice_elems) {
-  __shared__ SigSmem<128, 2> s;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int b = tiles[3 * t], ti = tiles[3 * t + 1], tj = tiles[3 * t + 2];
-    const X0Bin bb = bins[b];
-    __syncthreads();  // the previous tile's LDS reads are done before this tile's loads land
-    sig_tile<128, 2, 8>(s, as + bb.aoff, bs + bb.boff, ones, npr, 1, 0, 1, npr_col, MP, bb.ns, NCP, ti * QBM,
-                        tj * 128, 1, rcxq + slice_elems * bb.slice, mirror && ti != tj);
-  }
-}
-
-// Scratch for the gathered planes of one group of bins (A and B planes):
-// larger entry lists run as several groups, each accumulating into rcxq.
-constexpr size_t X0_SCRATCH = (size_t)4 << 30;
-
-ecalj_status x0_bf3(ecalj_ctx* h, const cf* z, int npr, int npr_col, int ipr_col, int nbin, const int64_t* cnt,
-                    const int64_t* col, const double* wt, const int* slice, cf* rcxq, int64_t slice_elems,
-                    int mirror) {
-  const int MP = pad_to(std::max(npr, 1), QBM), NCP = pad_to(std::max(npr_col, 1), 128);
-  const int RP = std::max(MP, NCP);
-  const int R = MP / QBM, Ct = NCP / 128;
-  // entries per group (multiple of 32): the group's planes fit X0_SCRATCH;
-  // ECALJ_X0_GROUP_ENTRIES sets a smaller group (tests of the grouping)
-  int64_t kcap = std::max<int64_t>(QBK, (int64_t)(X0_SCRATCH / (8 * (size_t)(MP + NCP))) / QBK * QBK);
-  if (const char* e = std::getenv("ECALJ_X0_GROUP_ENTRIES")) {
-    const long long v = std::atoll(e);
-    if (v > 0) kcap = std::min<int64_t>(kcap, std::max<int64_t>(QBK, (int64_t)v / QBK * QBK));
-  }
-  // pieces: (bin, first entry, entries) with at most kcap entries; groups of consecutive pieces
-  struct Piece { int bin; int64_t t0, k; };
-  std::vector<std::vector<Piece>> groups(1);
-  int64_t gk = 0;
-  for (int b = 0; b < nbin; ++b) {
-    const int64_t K = cnt[b + 1] - cnt[b];
-    for (int64_t t0 = 0; t0 < K; t0 += kcap) {
-      const int64_t k = std::min(kcap, K - t0), k32 = (k + QBK - 1) / QBK * QBK;
-      if (gk + k32 > kcap) {
-        groups.emplace_back();
-        gk = 0;
-      }
-      groups.back().push_back({b, cnt[b] + t0, k});
-      gk += k32;
-    }
-  }
-  if (groups.back().empty()) groups.pop_back();
-  if (groups.empty()) return ECALJ_OK;
-  // blobs of every group first: the list slot and the planes are sized once (no growth between groups)
-  struct Blob { std::vector<unsigned char> v; size_t o_w, o_one, o_bin, o_til; int64_t kp; int ntiles; };
-  std::vector<Blob> blobs(groups.size());
-  size_t maxblob = 0;
-  int64_t maxkp = 0;
-  for (size_t gi = 0; gi < groups.size(); ++gi) {
-    const std::vector<Piece>& g = groups[gi];
-    std::vector<X0Bin> bins;
-    int64_t kp = 0;
-    for (const Piece& pc : g) {
-      const int64_t k32 = (pc.k + QBK - 1) / QBK * QBK;
-      if (k32 / QBK > 0x7fffffff) return ECALJ_ERR_DIM;
-      bins.push_back({kp / QBK * 4 * (int64_t)MP * QBK, kp / QBK * 4 * (int64_t)NCP * QBK, (int)(k32 / QBK),
-                      slice[pc.bin]});
-      kp += k32;
-    }
-    std::vector<int64_t> colp(kp, -1);
-    std::vector<double> wp(kp, 0.0);
-    int64_t o = 0;
-    for (const Piece& pc : g) {
-      for (int64_t t = 0; t < pc.k; ++t) {
-        colp[o + t] = col[pc.t0 + t];
-        wp[o + t] = wt[pc.t0 + t];
-      }
-      o += (pc.k + QBK - 1) / QBK * QBK;
-    }
-    // tiles: longest K first (persistent grid)
-    std::vector<int> order(bins.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bins[x].ns > bins[y].ns; });
-    std::vector<int> tiles;
-    for (int b : order)
-      for (int ti = 0; ti < R; ++ti)
-        for (int tj = mirror ? ti : 0; tj < Ct; ++tj) {
-          tiles.push_back(b);
-          tiles.push_back(ti);
-          tiles.push_back(tj);
-        }
-    // blob: [colp | wp | ones (NCP) | bins | tiles]
-    Blob& B = blobs[gi];
-    B.kp = kp;
-    B.ntiles = (int)(tiles.size() / 3);
-    B.o_w = sizeof(int64_t) * kp;
-    B.o_one = B.o_w + sizeof(double) * kp;
-    B.o_bin = B.o_one + sizeof(double) * NCP;
-    B.o_til = B.o_bin + sizeof(X0Bin) * bins.size();
-    B.v.resize(B.o_til + sizeof(int) * tiles.size());
-    memcpy(B.v.data(), colp.data(), B.o_w);
-    memcpy(B.v.data() + B.o_w, wp.data(), B.o_one - B.o_w);
-    for (int c = 0; c < NCP; ++c) {
-      const double one = 1.0;
-      memcpy(B.v.data() + B.o_one + sizeof(double) * c, &one, sizeof(double));
-    }
-    memcpy(B.v.data() + B.o_bin, bins.data(), B.o_til - B.o_bin);
-    memcpy(B.v.data() + B.o_til, tiles.data(), sizeof(int) * tiles.size());
-    maxblob = std::max(maxblob, B.v.size());
-    maxkp = std::max(maxkp, kp);
-  }
-  unsigned char* d;
-  unsigned short *as, *bs;
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_LIST, maxblob, (void**)&d));
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0A, sizeof(unsigned short) * 4 * (size_t)MP * maxkp, (void**)&as));
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_X0B, sizeof(unsigned short) * 4 * (size_t)NCP * maxkp, (void**)&bs));
-  int tev = -1;
-  ECALJ_TRY(timing_begin(h, 1, &tev));
-  for (const Blob& B : blobs) {
-    ECALJ_TRY(upload_blob(h, ecalj_ctx::S_LIST, B.v, &d));
-    const int64_t nslice = B.kp / QBK;
-    const int64_t tot = nslice * 4 * RP;
-    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
-    hipLaunchKernelGGL(x0_split_kernel, dim3(blocks), dim3(256), 0, h->stream, z, (const int64_t*)d,
-                       (const double*)(d + B.o_w), nslice, npr, npr_col, (int64_t)(ipr_col - 1), MP, NCP, RP, as,
-                       bs);
-    ECALJ_HIP(hipGetLastError());
-    const int grid = std::max(1, std::min(B.ntiles, h->ncu));
-    hipLaunchKernelGGL(x0_bf3_kernel, dim3(grid), dim3(512), 0, h->stream, (const int*)(d + B.o_til), B.ntiles,
-                       (const X0Bin*)(d + B.o_bin), as, bs, (const double*)(d + B.o_one), npr, npr_col, MP, NCP,
-                       mirror, rcxq, slice_elems);
-    ECALJ_HIP(hipGetLastError());
-  }
-  ECALJ_TRY(timing_end(h, 1, tev));
-  return ECALJ_OK;
-}
-
-
-// ================================================================ zmel contraction on the bf16x3 core
-// The second GEMM of zmel (S/m_zmel.f90:323-415 in the occupied- or end-side
-// order): per batch b, C_b(I, c) = s(alpha sum_k T_b(k, I) X_b(k, c)), k
-// contiguous in both operands (T stored T[k + lda I], X stored X[k + ldb c]),
-// s = conjugation when conjC.  Both operands are split once into the
-// pre-tiled planes (alpha and the conjugations folded in), then the Sigma_c
-// tile code runs per (row tile, column tile, batch) with unit weights.
-
-// rows of a K-contiguous complex64 source -> planes; one thread per (b, ks, row, chunk), chunk fastest
-__global__ __launch_bounds__(256) void zsplit_kc_kernel(const cf* __restrict__ src, int64_t ld, int64_t s0,
-                                                        int64_t s1, int div, int rows, int K, float ar, float ai,
-                                                        int conj, int RP, int NS, unsigned short* __restrict__ dst,
-                                                        int64_t dstride, int nb) {
-  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
-  const int64_t per = (int64_t)NS * RP * 4, tot = per * nb;
-  const int64_t pst = (int64_t)RP * QBK;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-    const int ch = (int)(e % 4);
-    const int row = (int)((e / 4) % RP);
-    const int ks = (int)((e / (4 * (int64_t)RP)) % NS);
-    const int b = (int)(e / per);
-    const cf* base = src + (int64_t)(b % div) * s0 + (int64_t)(b / div) * s1 + ld * row;
-    us8v rh, rl, ih, il;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kk = ks * QBK + 8 * ch + q;
-      float xr = 0.f, xi = 0.f;
-      if (row < rows && kk < K) {
-        const cf x = base[kk];
-        xr = ar * x.re - ai * x.im;
-        xi = ar * x.im + ai * x.re;
-        if (conj) xi = -xi;
-      }
-      unsigned short h, l;
-      split_bf16(xr, h, l);
-      rh[q] = h;
-      rl[q] = l;
-      split_bf16(xi, h, l);
-      ih[q] = h;
-      il[q] = l;
-    }
-    unsigned short* D = dst + dstride * b + (int64_t)ks * 4 * pst + (int64_t)row * QBK + 8 * chunk_slot(row, ch);
-    *reinterpret_cast<us8v*>(D) = rh;
-    *reinterpret_cast<us8v*>(D + pst) = rl;
-    *reinterpret_cast<us8v*>(D + 2 * pst) = ih;
-    *reinterpret_cast<us8v*>(D + 3 * pst) = il;
-  }
-}
-
-__global__ __launch_bounds__(256) void ones_kernel(double* __restrict__ p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 1.0;
-}
-
-__global__ __launch_bounds__(512, 1) void zmel_bf3_kernel(const unsigned short* __restrict__ as, int64_t ape,
-                                                          const unsigned short* __restrict__ bs, int64_t bpe,
-                                                          int bdiv, const double* __restrict__ ones, int M, int N,
-                                                          int MP, int NS, int NCP, cf* __restrict__ C, int64_t ldc,
-                                                          int64_t c_s0, int64_t c_s1, int div) {
-  __shared__ SigSmem<128, 2> s;
-  const int b = blockIdx.z, r = blockIdx.x, c = blockIdx.y;
-  const unsigned short* Bk = bs + (bdiv ? bpe * (b / bdiv) : 0);
-  cf* Cb = C + (int64_t)(b % div) * c_s0 + (int64_t)(b / div) * c_s1;
-  sig_tile<128, 2, 8>(s, as + ape * b, Bk, ones, M, 1, 0, 1, N, MP, NS, NCP, r * QBM, c * 128, 0, Cb, 0, ldc);
-}
-
-ecalj_status zmel_z_bf3(ecalj_ctx* h, int M, int N, int K, const cf* A, int64_t lda, int64_t a_s0, int64_t a_s1,
-                        const cf* B, int64_t ldb, int64_t b_s1, bool conjB, cd alpha, bool conjC, cf* C,
-                        int64_t ldc, int64_t c_s0, int64_t c_s1, int batch, int div) {
-  if (M < 1 || N < 1 || batch < 1) return ECALJ_OK;
-  if (K < 1 || div < 1 || batch > 65535) return ECALJ_ERR_DIM;
-  const int MP = pad_to(M, QBM), NCP = pad_to(N, 128), NS = pad_to(K, QBK) / QBK;
-  const int nbb = b_s1 ? (batch + div - 1) / div : 1;  // distinct B operands
-  const int64_t ape = 4LL * NS * MP * QBK, bpe = 4LL * NS * NCP * QBK;
-  unsigned short *as, *bs;
-  double* ones;
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_ZA, sizeof(unsigned short) * (size_t)ape * batch, (void**)&as));
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_ZB, sizeof(unsigned short) * (size_t)bpe * nbb + sizeof(double) * NCP,
-                        (void**)&bs));
-  ones = reinterpret_cast<double*>(bs + (size_t)bpe * nbb);
-  hipLaunchKernelGGL(ones_kernel, dim3((NCP + 255) / 256), dim3(256), 0, h->stream, ones, NCP);
-  ECALJ_HIP(hipGetLastError());
-  // conj(alpha T) on A; on B its own conjugation combined with the output's
-  {
-    const int64_t tot = (int64_t)NS * MP * 4 * batch;
-    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
-    hipLaunchKernelGGL(zsplit_kc_kernel, dim3(blocks), dim3(256), 0, h->stream, A, lda, a_s0, a_s1, div, M, K,
-                       (float)alpha.re, (float)alpha.im, conjC ? 1 : 0, MP, NS, as, ape, batch);
-    ECALJ_HIP(hipGetLastError());
-  }
-  {
-    const int64_t tot = (int64_t)NS * NCP * 4 * nbb;
-    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
-    hipLaunchKernelGGL(zsplit_kc_kernel, dim3(blocks), dim3(256), 0, h->stream, B, ldb, (int64_t)0, b_s1, 1, N, K,
-                       1.f, 0.f, (conjB != conjC) ? 1 : 0, NCP, NS, bs, bpe, nbb);
-    ECALJ_HIP(hipGetLastError());
-  }
-  dim3 grid(MP / QBM, NCP / 128, batch);
-  hipLaunchKernelGGL(zmel_bf3_kernel, grid, dim3(512), 0, h->stream, as, ape, bs, bpe, b_s1 ? div : 0, ones, M, N,
-                     MP, NS, NCP, C, ldc, c_s0, c_s1, div);
-  ECALJ_HIP(hipGetLastError());
-  return ECALJ_OK;
-}
-
-
-// ================================================================ zmel first contraction (real ppb)
-// T(r, c) = sum_k P(r, k) opB(X)(k, c): P real (ppb blocks, P[r + lda k]),
-// X complex64 K-contiguous (X[k + ldb c], conjugated when conjB), per batch b
-// X_b = X + b x_s1, T_b = C + b c_s1; P shared by all batches.
-
-// real rows -> planes (imaginary planes zero); one thread per (ks, chunk, row), row fastest
-__global__ __launch_bounds__(256) void rsplit_kernel(const double* __restrict__ a, int64_t lda, int rows, int K,
-                                                     int RP, int NS, unsigned short* __restrict__ dst) {
-  typedef unsigned short us8v __attribute__((ext_vector_type(8)));
-  const int64_t tot = (int64_t)NS * 4 * RP;
-  const int64_t pst = (int64_t)RP * QBK;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-    const int row = (int)(e % RP);
-    const int ch = (int)((e / RP) % 4), ks = (int)(e / (4 * (int64_t)RP));
-    us8v rh, rl, z;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kk = ks * QBK + 8 * ch + q;
-      const float x = (row < rows && kk < K) ? (float)a[row + lda * kk] : 0.f;
-      unsigned short h, l;
-      split_bf16(x, h, l);
-      rh[q] = h;
-      rl[q] = l;
-      z[q] = 0;
-    }
-    unsigned short* D = dst + (int64_t)ks * 4 * pst + (int64_t)row * QBK + 8 * chunk_slot(row, ch);
-    *reinterpret_cast<us8v*>(D) = rh;
-    *reinterpret_cast<us8v*>(D + pst) = rl;
-    *reinterpret_cast<us8v*>(D + 2 * pst) = z;
-    *reinterpret_cast<us8v*>(D + 3 * pst) = z;
-  }
-}
-
-__global__ __launch_bounds__(512, 1) void zmel_p_kernel(const unsigned short* __restrict__ as,
-                                                        const unsigned short* __restrict__ bs, int64_t bpe,
-                                                        const double* __restrict__ ones, int M, int N, int MP, int NS,
-                                                        int NCP, cf* __restrict__ C, int64_t ldc, int64_t c_s1) {
-  __shared__ SigSmem<128, 2> s;
-  const int b = blockIdx.z, r = blockIdx.x, c = blockIdx.y;
-  sig_tile<128, 2, 8, true>(s, as, bs + bpe * b, ones, M, 1, 0, 1, N, MP, NS, NCP, r * QBM, c * 128, 0,
-                            C + c_s1 * b, 0, ldc);
-}
-
-ecalj_status zmel_p_bf3(ecalj_ctx* h, int M, int N, int K, const double* P, int64_t lda, const cf* X,
-                        int64_t ldb, int64_t x_s1, bool conjB, cf* C, int64_t ldc, int64_t c_s1, int batch) {
-  if (M < 1 || N < 1 || batch < 1) return ECALJ_OK;
-  if (K < 1 || batch > 65535) return ECALJ_ERR_DIM;
-  const int MP = pad_to(M, QBM), NCP = pad_to(N, 128), NS = pad_to(K, QBK) / QBK;
-  if ((int64_t)MP / QBM > 0x7fffffff) return ECALJ_ERR_DIM;
-  const int64_t ape = 4LL * NS * MP * QBK, bpe = 4LL * NS * NCP * QBK;
-  unsigned short *as, *bs;
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_ZPA, sizeof(unsigned short) * (size_t)ape, (void**)&as));
-  ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_ZPB, sizeof(unsigned short) * (size_t)bpe * batch + sizeof(double) * NCP,
-                        (void**)&bs));
-  double* ones = reinterpret_cast<double*>(bs + (size_t)bpe * batch);
-  hipLaunchKernelGGL(ones_kernel, dim3((NCP + 255) / 256), dim3(256), 0, h->stream, ones, NCP);
-  ECALJ_HIP(hipGetLastError());
-  {
-    const int64_t tot = (int64_t)NS * 4 * MP;
-    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
-    hipLaunchKernelGGL(rsplit_kernel, dim3(blocks), dim3(256), 0, h->stream, P, lda, M, K, MP, NS, as);
-    ECALJ_HIP(hipGetLastError());
-  }
-  {
-    const int64_t tot = (int64_t)NS * NCP * 4 * batch;
-    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65536 * 4);
-    hipLaunchKernelGGL(zsplit_kc_kernel, dim3(blocks), dim3(256), 0, h->stream, X, ldb, (int64_t)0, x_s1, 1, N, K,
-                       1.f, 0.f, conjB ? 1 : 0, NCP, NS, bs, bpe, batch);
-    ECALJ_HIP(hipGetLastError());
-  }
-  dim3 grid(MP / QBM, NCP / 128, batch);
-  hipLaunchKernelGGL(zmel_p_kernel, grid, dim3(512), 0, h->stream, as, bs, bpe, ones, M, N, MP, NS, NCP, C, ldc,
-                     c_s1);
-  ECALJ_HIP(hipGetLastError());
-  return ECALJ_OK;
-}
 
 }  // namespace ecalj
