@@ -564,8 +564,8 @@ def main():
                     help="profile one form only (prints a short line instead of the contract JSON)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one q-point per GPU; "
-              "see DESIGN.md for how the path shards). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the q-points of a QSGW "
+              "iteration are independent and would shard one per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from ecalj_amd import synth
