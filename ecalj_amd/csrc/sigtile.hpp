@@ -126,17 +126,18 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
       // the buffer of stage st+NSTG-1 was last read in iteration st-1, which every wave has left
       if (st + NSTG - 1 < nst) issue(st + NSTG - 1, (st + NSTG - 1) % NSTG);
 
-      us8q brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
+      // B fragments as 32-bit words: the sign flips are then one XOR per word
+      u4q brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         const int o = boff + j * 16 * QBK;
-        brh[j] = *reinterpret_cast<const us8q*>(&s.b[buf][0][o]);
-        brl[j] = *reinterpret_cast<const us8q*>(&s.b[buf][1][o]);
-        bih[j] = *reinterpret_cast<const us8q*>(&s.b[buf][2][o]);
-        bil[j] = *reinterpret_cast<const us8q*>(&s.b[buf][3][o]);
+        brh[j] = *reinterpret_cast<const u4q*>(&s.b[buf][0][o]);
+        brl[j] = *reinterpret_cast<const u4q*>(&s.b[buf][1][o]);
+        bih[j] = *reinterpret_cast<const u4q*>(&s.b[buf][2][o]);
+        bil[j] = *reinterpret_cast<const u4q*>(&s.b[buf][3][o]);
         // -b: flip the sign bit of every bf16 (two per 32-bit word)
-        nbh[j] = __builtin_bit_cast(us8q, __builtin_bit_cast(u4q, bih[j]) ^ 0x80008000u);
-        nbl[j] = __builtin_bit_cast(us8q, __builtin_bit_cast(u4q, bil[j]) ^ 0x80008000u);
+        nbh[j] = bih[j] ^ 0x80008000u;
+        nbl[j] = bil[j] ^ 0x80008000u;
       }
       // one A row tile at a time: its 4 planes meet the TN column tiles
 #pragma unroll
