@@ -90,7 +90,8 @@ def test_sigc_real_pole_outside_mesh():
         h.close()
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("bf16x3", 1e-4)])
+# bf16f8: the imaginary-axis part with fp8 cross terms (bar of tests/test_gpu_sigc_fp8.py)
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("bf16x3", 1e-4), ("bf16f8", 1e-3)])
 def test_sigc_full_dev_vs_oracle(orc, prec, tol):
     """Imaginary + real axis in one czmelwc, k by k with the clip after each k,
     against orc_sigc_full called per k (the reference's order).  The real-axis
