@@ -1,21 +1,24 @@
 """Benchmark: chi0 + W_c + Sigma_c per q-point on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2] [--mixed bf16x3|bf16f8|f32]
-                    [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--f64-steps K64] [--config C3]
+                    [--mixed bf16x3|bf16f8|f32] [--no-cpu-baseline] [--also LIST] [--c4-eager]
 
 A step is one q-point of the path (chi0(q) over all k and band pairs, W_c(q)
 at nf+1 frequencies, Sigma_c(q) over all internal k) on synthetic inputs
-already resident in device memory.  The same q-point is timed in the plain
-FP64 GPU form and in the mixed form; `value` is the mixed form's seconds per
-q-point, with its speedup over the FP64 form and the largest QP-energy
-difference between the two (meV).  Prints one JSON line.  The kernel roofline
-uses HIP events recorded by the library on its own stream around every launch
-of the dominant kernel (the Sigma_c frequency contraction) during the timed
-steps.  Timed regions are bracketed by device synchronisation through the
-library (ecalj_sync), which owns the process's only HIP runtime.
-cpu_baseline times the FP64 C oracle on a bounded sample of the same workload
-(one k-point of chi0 and of Sigma_c, one frequency of the W build), scaled to
-a whole q-point.
+already resident in device memory.  The headline workload is the
+mixed-precision q-point (C3: nk=64, nb=320, 18+300 bands, nI=1200, nf=20),
+timed in the mixed form (K steps after W warm-up steps) and in the plain
+FP64 GPU form (K64 steps); `value` is the mixed form's seconds per q-point,
+with its speedup over the FP64 form and the largest QP-energy difference
+between the two (meV).  Prints one JSON line.  The kernel roofline uses HIP
+events recorded by the library on its own stream around every launch of the
+dominant kernel (the Sigma_c frequency contraction) during the timed steps.
+Timed regions are bracketed by device synchronisation through the library
+(ecalj_sync), which owns the process's only HIP runtime.  cpu_baseline times
+the FP64 C oracle on a bounded sample of the same q-point (one k of chi0 and
+of Sigma_c, one frequency of the W build, ...; 1 warm-up + median of 5 per
+unit), scaled to a whole q-point.  Other workloads (C2, C4, C5, the real axis,
+...) are reported under `also`.
 """
 from __future__ import annotations
 
@@ -59,9 +62,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(qp, nks=8, nws=3):
-    """FP64 oracle on host cores: a bounded sample of the q-point (chi0 and
-    Sigma_c on nks of the nk k-points, the W build at nws frequencies), scaled."""
+def cpu_baseline(qp, reps=5):
+    """FP64 oracle on the host cores, timed on a bounded sample of the q-point
+    and scaled to one q-point.  Each unit of the sample runs once as a warm-up
+    and then `reps` times; the median counts.  The oracle's GEMMs and LU
+    inverses go through the LP64 zgemm / zgetrf / zgetri scipy ships
+    (OpenBLAS), where the reference calls MKL's zgemm3m / zgetrf / zgetri
+    (S/m_blas.f90:225, S/m_lapack.f90:21-47); the loops around them are the
+    oracle's own (OpenMP).  Units: zmel + x0gemm for one k of chi0 (x nk);
+    the Hilbert transform on an eighth of rcxq's columns (x 8); the W build
+    at one frequency (x nf+1); Sigma_c's zmel + weights for one internal k
+    (x nk); the Sigma_c contraction of one k at one and at two frequencies
+    (the difference is one frequency: x nk (nf+1); the rest, the final
+    product, x nk)."""
     import oracle as orc
     from ecalj_amd import synth
     from oracle import pipeline
@@ -69,39 +82,77 @@ def cpu_baseline(qp, nks=8, nws=3):
     orc.build()
     cfg = qp.cfg
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t = {}
-    ks = list(range(min(nks, cfg.nk)))
-    nw = min(nws, cfg.nf + 1)
-    # chi0 for nks k-points (zmel + x0gemm) -> scaled by nk; Hilbert once
-    t0 = time.perf_counter()
-    chi, _ = pipeline.chi0(qp, k_subset=ks)
-    t["chi0_k_plus_hilbert"] = time.perf_counter() - t0
-    # Hilbert alone, to separate it from the per-k part
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(limits=threads, user_api="blas")
+    except ImportError:
+        limiter = None
+    blas = orc.use_blas(True)
+
+    def med(f):
+        f()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    ni, nk, nw1 = cfg.ni, cfg.nk, cfg.nf + 1
     fr = synth.frhis(qp)
     nwhis = len(fr) - 1
-    rc = np.zeros((cfg.ni, cfg.ni, nwhis + 1), np.complex128, order="F")
-    _, _, _, _, nu = pipeline.frequencies(cfg)
-    im = orc.imatt(nu, fr)
-    t0 = time.perf_counter()
-    orc.dpsion_imag(rc, fr, im)
-    t_hil = time.perf_counter() - t0
+    pw = synth.pair_weights(qp, fr)
+    ix = orc.x0kf_init([(pw.n1b, pw.n2b, pw.ihw, pw.nhw, pw.jhw)], pw.whw, nband=cfg.nbt, nwhis=nwhis)
+    no, nu = cfg.no, cfg.nu
+    one = [1.0] * cfg.natom
+    t = {}
+    z = orc.zmel(cfg.nv_list, cfg.mdim_list, one, qp.ppb, qp.c_k[:, :no, 0], qp.c_kq[:, no:no + nu, 0], conjg=True)
+    t["zmel_chi0_k"] = med(lambda: orc.zmel(cfg.nv_list, cfg.mdim_list, one, qp.ppb, qp.c_k[:, :no, 0],
+                                            qp.c_kq[:, no:no + nu, 0], conjg=True))
+    rc = np.zeros((ni, ni, nwhis + 1), np.complex128, order="F")
+    t["x0gemm_k"] = med(lambda: orc.x0gemm(rc, ni, 1, ni, nwhis, 1, int(ix.nkmin[0]), int(ix.nkmax[0]), z,
+                                           int(ix.icounkmin[0]), int(ix.icounkmax[0]), ix.iwini, ix.iwend, ix.itc,
+                                           ix.itpc, ix.jpmc, ix.icouini, ix.whwc))
+    ncol = max(1, ni // 8)
+    slab = np.asfortranarray(rc[:, :ncol, :])
     del rc
-    # W build for nw frequencies -> scaled by nf+1
-    t0 = time.perf_counter()
-    wcs = orc.wc_build(np.asfortranarray(chi[:, :, :nw]), qp.vcousq, qp.zcousq, nspin=cfg.nspin)
-    t["w_freqs"] = time.perf_counter() - t0
-    # Sigma_c for nks internal k -> scaled by nk (W_c(k) = the sampled W_c(q) frequencies, repeated)
-    reps = [wcs[:, :, i % nw] for i in range(cfg.nf + 1)]
-    wck = [np.asfortranarray(np.stack(reps, axis=2))] * len(ks)
-    t0 = time.perf_counter()
-    pipeline.sigc(qp, wck, k_subset=ks)
-    t["sigc_k"] = time.perf_counter() - t0
-    per_q = ((t["chi0_k_plus_hilbert"] - t_hil) * cfg.nk / len(ks) + t_hil + t["w_freqs"] * (cfg.nf + 1) / nw
-             + t["sigc_k"] * cfg.nk / len(ks))
-    sample = (f"{cfg.name}: chi0 on {len(ks)} of {cfg.nk} k-points + full Hilbert transform, W build at {nw} of "
-              f"{cfg.nf + 1} frequencies, Sigma_c on {len(ks)} of {cfg.nk} internal k-points "
-              f"({sum(t.values()):.1f} s measured); scaled to one q-point")
-    return {"value": per_q, "unit": "s/q-point", "cores": threads, "kind": "port", "sample": sample}
+    _, _, _, _, nu_ = pipeline.frequencies(cfg)
+    im = orc.imatt(nu_, fr)
+    t["hilbert_eighth"] = med(lambda: orc.dpsion_imag(slab, fr, im))
+    del slab
+    # the W build's cost does not depend on the values: one Hermitian negative definite chi
+    xr = np.random.default_rng(0).standard_normal((ni, 64)) * (0.1 / ni)
+    chi1 = np.asfortranarray(-(xr @ xr.T)[:, :, None].astype(np.complex128))
+    t["w_freq"] = med(lambda: orc.wc_build(chi1, qp.vcousq, qp.zcousq, nspin=cfg.nspin))
+    wc1 = orc.wc_build(chi1, qp.vcousq, qp.zcousq, nspin=cfg.nspin)
+    freqx, _, wx, expa, _ = pipeline.frequencies(cfg)
+    omega = qp.e_q[:cfg.nq] / synth.RYDBERG_EV
+    nbt = cfg.nbt
+
+    def sig_prep():
+        zs = orc.zmel(cfg.nv_list, cfg.mdim_list, one, qp.ppb, qp.c_qmk[:, :nbt, 0], qp.c_q[:, :cfg.nq])
+        g = orc.wgtim(omega, qp.e_qmk[:nbt, 0] / synth.RYDBERG_EV, cfg.esmr, qp.wk[0], freqx, wx, expa, ua=cfg.ua)
+        return zs, g
+
+    t["zmel_wgt_sigma_k"] = med(sig_prep)
+    zs, g = sig_prep()
+    w2 = np.asfortranarray(np.concatenate([wc1, wc1], axis=2))
+    t["sigc_k_1freq"] = med(lambda: orc.sigc_imag(zs, wc1, g[:1]))
+    t["sigc_k_2freq"] = med(lambda: orc.sigc_imag(zs, w2, g[:2]))
+    tf = max(t["sigc_k_2freq"] - t["sigc_k_1freq"], 0.0)
+    tfin = max(t["sigc_k_1freq"] - tf, 0.0)
+    per_q = ((t["zmel_chi0_k"] + t["x0gemm_k"]) * nk + t["hilbert_eighth"] * (ni / ncol) + t["w_freq"] * nw1
+             + t["zmel_wgt_sigma_k"] * nk + (tf * nw1 + tfin) * nk)
+    if limiter is not None:
+        limiter.restore_original_limits()
+    measured = sum(t.values()) * (reps + 1)
+    sample = (f"{cfg.name}: FP64 oracle ({'OpenBLAS zgemm/zgetrf/zgetri' if blas else 'own loops'} + OpenMP), "
+              f"units timed 1 warm-up + median of {reps}: zmel+x0gemm of 1 of {nk} k, Hilbert on {ncol} of {ni} "
+              f"columns, W build at 1 of {nw1} frequencies, Sigma_c zmel+weights of 1 of {nk} k, Sigma_c "
+              f"contraction of 1 k at 1 and 2 of {nw1} frequencies; scaled to one q-point "
+              f"({measured:.1f} s of CPU work)")
+    return {"value": round(per_q, 3), "unit": "s/q-point", "cores": threads, "kind": "port", "sample": sample,
+            "units_s": {k: round(v, 4) for k, v in t.items()}}
 
 
 def run_form(qp, precision, steps, warmup):
@@ -138,10 +189,11 @@ def run_form(qp, precision, steps, warmup):
     return info
 
 
-def bench_c4(mixed, nq=16, log_=None):
+def bench_c4(mixed, nq=16, log_=None, graph=True):
     """Batched uneven workload, one GPU's share: nq q-points of 27 k-points
     with per-k band windows and per-q nI; FP64 form eager, mixed form as a
-    captured HIP graph per q-point (replayed, as the QSGW iterations do)."""
+    captured HIP graph per q-point (replayed, as the QSGW iterations do), or
+    eagerly only (graph=False, for runs under a profiler)."""
     from ecalj_amd import synth
     from ecalj_amd.batched import UnevenQPointGPU, flops_uneven
 
@@ -162,7 +214,9 @@ def bench_c4(mixed, nq=16, log_=None):
             g.h.sync()
             te = time.perf_counter() - t0
             tg = None
-            if prec != "f64":
+            if prec != "f64" and not graph:
+                tg = te
+            elif prec != "f64":
                 gr = g.capture_step()
                 gr.launch()
                 g.h.sync()
@@ -190,6 +244,7 @@ def bench_c4(mixed, nq=16, log_=None):
     return {
         "workload": f"C4 one-GPU share: {nq} q-points, nI ~ U[800,1600] per q, 27 k-points with per-k windows "
                     f"U{{8..24}} occ x U{{150..400}} unocc, nf=20, ntq=16",
+        "mixed_timing": "HIP-graph replay" if graph else "eager (no graph capture)",
         "s_per_q": round(tmx / nq, 6), "fp64_form_s_per_q": round(t64 / nq, 6),
         "mixed_eager_s_per_q": round(tmx_eager / nq, 6),
         "speedup_vs_fp64_form": round(t64 / tmx, 3), "qp_err_meV_vs_fp64_form": round(err, 5),
@@ -548,12 +603,16 @@ def roofline(fl, info, precision, steps, config="C2"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="C2")
+    ap.add_argument("--f64-steps", type=int, default=3,
+                    help="timed steps of the FP64 GPU form the speedup is taken against (C3: ~4 s each)")
+    ap.add_argument("--config", default="C3")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "bf16f8", "f32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--also", default="C3,C4,C5,C2R,TW,IPW,GAM,ROT,F8,C3F8,C4F8",
+    ap.add_argument("--c4-eager", action="store_true",
+                    help="C4 legs: time the mixed form eagerly only (no HIP-graph capture/replay)")
+    ap.add_argument("--also", default="C2,F8,C2F8,C4,C4F8,C5,C2R,TW,IPW,GAM,ROT",
                     help="comma-separated further configs timed in both forms (2 steps each) and reported "
                          "under 'also'; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -595,7 +654,7 @@ def main():
             elif name == "ROT":
                 res[name] = bench_rotwave(log_=log)
             elif name in ("C4", "C4F8"):
-                res[name] = bench_c4(args.mixed if name == "C4" else "bf16f8", log_=log)
+                res[name] = bench_c4(args.mixed if name == "C4" else "bf16f8", log_=log, graph=not args.c4_eager)
             elif name == "C5":
                 res[name] = bench_c5(log_=log)
             elif name.endswith("R") and name[:-1] in synth.CONFIGS:
@@ -612,7 +671,7 @@ def main():
         print(json.dumps({"form": args.only, "s_per_q": info["s_per_q"],
                           "kernel_ms_per_step": {k: v[0] / args.steps for k, v in info["ktimes"].items()}}))
         return
-    f64 = run_form(qp, "f64", args.steps, args.warmup)
+    f64 = run_form(qp, "f64", args.f64_steps, min(args.warmup, 1))
     mix = run_form(qp, args.mixed, args.steps, args.warmup)
     fl = flops_per_q(cfg, nwhis_pairs_bins=mix["npb"])
     ftot = sum(fl.values())
@@ -639,8 +698,9 @@ def main():
                    "form": args.mixed},
         "tflops": round(ftot / s_per_q / 1e12, 3),
         "fp64_form": {"s_per_q": round(f64["s_per_q"], 6), "tflops": round(ftot / f64["s_per_q"] / 1e12, 3),
-                      "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in f64["ktimes"].items()},
-                      "roofline": roofline(fl, f64, "f64", args.steps, cfg.name)},
+                      "steps": args.f64_steps, "warmup": min(args.warmup, 1),
+                      "kernel_ms_per_step": {k: round(v[0] / args.f64_steps, 3) for k, v in f64["ktimes"].items()},
+                      "roofline": roofline(fl, f64, "f64", args.f64_steps, cfg.name)},
         "speedup_vs_fp64_form": round(f64["s_per_q"] / s_per_q, 3),
         "qp_err_meV_vs_fp64_form": round(qp_err_mev, 5),
         "flops_per_q": ftot,
@@ -661,7 +721,8 @@ def main():
     for name in also:
         if name in ("C4", "C4F8"):
             try:
-                out["also"][name] = bench_c4(args.mixed if name == "C4" else "bf16f8", log_=log)
+                out["also"][name] = bench_c4(args.mixed if name == "C4" else "bf16f8", log_=log,
+                                             graph=not args.c4_eager)
             except Exception as e:  # report, never hide; the headline line still prints
                 out["also"][name] = {"error": repr(e)}
             continue

@@ -27,18 +27,41 @@ def sources():
     return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
 
 
-def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-           *sources(), "-o", LIB + ".tmp"]
+def _compile(src: str, obj: str, verbose: bool):
+    cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], "-c", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "csrc"), src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     errs = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln]
+    return r.returncode, errs
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every source (one hipcc per file, in parallel) and links the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    odir = os.path.join(HERE, "_build")
+    os.makedirs(odir, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(odir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        res = list(ex.map(lambda so: _compile(so[0], so[1], verbose), zip(srcs, objs)))
+    failed = False
+    for src, (rc, errs) in zip(srcs, res):
+        if rc != 0:
+            failed = True
+            sys.stderr.write(f"--- {os.path.basename(src)}\n" + "\n".join(errs) + "\n")
+        elif verbose and errs:
+            sys.stderr.write("\n".join(errs) + "\n")
+    if failed:
+        raise RuntimeError("hipcc failed")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        sys.stderr.write("\n".join(errs) + "\n")
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
-    if verbose and errs:
-        sys.stderr.write("\n".join(errs) + "\n")
+        sys.stderr.write(r.stderr)
+        raise RuntimeError(f"link failed ({r.returncode})")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -1,5 +1,6 @@
 // api.hip -- the extern "C" entry points of include/ecalj_gpu.h.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -42,6 +43,7 @@ ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out) {
       ECALJ_HIP(hipFree(h->slot[slot]));
       h->slot[slot] = nullptr;
       h->slot_bytes[slot] = 0;
+      ++h->state_gen;  // graphs captured before this point referenced the old slot
     }
     size_t want = bytes + bytes / 8;
     if (hipMalloc(&h->slot[slot], want) != hipSuccess) {
@@ -261,8 +263,10 @@ ecalj_status ecalj_create(ecalj_handle* out, int device) {
   ECALJ_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return ECALJ_ERR_ARG;
   ECALJ_HIP(hipSetDevice(device));
+  static std::atomic<uint64_t> next_id{0};
   ecalj_ctx* h = new ecalj_ctx;
   h->device = device;
+  h->id = ++next_id;
   if (hipStreamCreate(&h->own_stream) != hipSuccess) {
     delete h;
     return ECALJ_ERR_HIP;
@@ -315,7 +319,10 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
 struct ecalj_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<void*> pinned;
+  std::vector<void*> pinned;   // staging the graph's copy nodes read at every replay
+  uint64_t owner = 0;          // id of the capturing handle
+  uint64_t gen = 0;            // its state generation at capture
+  hipEvent_t done = nullptr;   // recorded after each launch; destroy waits for it
 };
 
 ecalj_status ecalj_graph_begin(ecalj_handle h) {
@@ -345,7 +352,11 @@ ecalj_status ecalj_graph_end(ecalj_handle h, ecalj_graph_t* out) {
     return ECALJ_ERR_HIP;
   }
   gr->graph = g;
-  if (hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+  gr->owner = h->id;
+  gr->gen = h->state_gen;
+  if (hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0) != hipSuccess ||
+      hipEventCreateWithFlags(&gr->done, hipEventDisableTiming) != hipSuccess) {
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
     (void)hipGraphDestroy(g);
     for (void* p : gr->pinned) (void)hipHostFree(p);
     delete gr;
@@ -358,12 +369,20 @@ ecalj_status ecalj_graph_end(ecalj_handle h, ecalj_graph_t* out) {
 ecalj_status ecalj_graph_launch(ecalj_handle h, ecalj_graph_t g) {
   if (!h || !g) return ECALJ_ERR_ARG;
   if (h->capturing) return ECALJ_ERR_STATE;
+  if (g->owner != h->id) return ECALJ_ERR_ARG;          // the nodes reference the capturing handle's memory
+  if (g->gen != h->state_gen) return ECALJ_ERR_STATE;   // that memory was freed or moved since the capture
+  ECALJ_HIP(hipSetDevice(h->device));
   ECALJ_HIP(hipGraphLaunch(g->exec, h->stream));
+  ECALJ_HIP(hipEventRecord(g->done, h->stream));
   return ECALJ_OK;
 }
 
 ecalj_status ecalj_graph_destroy(ecalj_graph_t g) {
   if (!g) return ECALJ_ERR_ARG;
+  if (g->done) {  // the pinned staging may still be read by a launch in flight
+    (void)hipEventSynchronize(g->done);
+    (void)hipEventDestroy(g->done);
+  }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   for (void* p : g->pinned) (void)hipHostFree(p);
@@ -517,6 +536,7 @@ ecalj_status ecalj_set_ppb(ecalj_handle h, int natom, const int* nv, const int* 
   }
   h->natom = natom;
   ECALJ_HIP(hipStreamSynchronize(h->stream));
+  ++h->state_gen;
   if (h->d_ppb) (void)hipFree(h->d_ppb);
   if (h->d_ppb2) (void)hipFree(h->d_ppb2);
   h->d_ppb = h->d_ppb2 = nullptr;
@@ -538,6 +558,7 @@ ecalj_status ecalj_set_coulomb(ecalj_handle h, int ngb, const double* vcoud, con
     vs[i] = std::sqrt(vcoud[i]);  // S/m_readVcoud.f90:40
   }
   ECALJ_HIP(hipStreamSynchronize(h->stream));
+  ++h->state_gen;
   if (h->d_vcousq) (void)hipFree(h->d_vcousq);
   if (h->d_zcousq) (void)hipFree(h->d_zcousq);
   if (h->d_zcousq32) (void)hipFree(h->d_zcousq32);

@@ -59,6 +59,11 @@ struct ecalj_ctx {
   // slots may not grow, uploads stage through fresh pinned buffers that the
   // graph keeps, kernel timing is off, the tuner uses cached choices only
   bool capturing = false;
+  // bumped whenever the handle frees or moves device memory that captured
+  // kernels may reference (scratch regrowth, set_ppb, set_coulomb, ...);
+  // a graph captured at another generation is refused at launch
+  uint64_t state_gen = 0;
+  uint64_t id = 0;                // unique per handle of the process
   hipEvent_t ev_join = nullptr;   // ecalj_stream_wait: last point of this stream another waited for
   bool timing_before_capture = false;
   std::vector<void*> cap_pinned;

@@ -192,6 +192,7 @@ ecalj_status ecalj_set_hamindex(ecalj_handle h, int nqtt, const double* qtt, con
   r.qlat.assign(qlat, qlat + 9);
   const size_t ldd = 2 * (size_t)lmxax + 1, nd = ldd * ldd * (lmxax + 1) * ngrp;
   ECALJ_HIP(hipStreamSynchronize(h->stream));
+  ++h->state_gen;
   RotState* keep = h->rot ? h->rot : new RotState;
   if (keep->d_dlmm) (void)hipFree(keep->d_dlmm);
   keep->d_dlmm = nullptr;

@@ -214,6 +214,7 @@ extern "C" ecalj_status ecalj_set_ppovl(ecalj_handle h, int nggg, const double* 
     if (igcgp2i[i] < 0 || igcgp2i[i] > ngcgp) return ECALJ_ERR_ARG;
   ECALJ_HIP(hipSetDevice(h->device));
   ECALJ_HIP(hipStreamSynchronize(h->stream));
+  ++h->state_gen;
   for (void** p : {(void**)&h->d_ggg, (void**)&h->d_igggi, (void**)&h->d_nvgcgp2, (void**)&h->d_igcgp2i})
     if (*p) {
       (void)hipFree(*p);

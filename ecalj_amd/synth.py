@@ -525,55 +525,93 @@ class X0Index:
     nkqmax: np.ndarray
 
 
-def x0kf_index(pw: PairWeights, nctot: int = 0, npm: int = 1) -> X0Index:
-    """Restates x0kf_v4hz_init job=0 then job=1 (S/x0kf_v4h.f90:96-138) for
-    valence pairs and npm=1: it = nctot + n1b, itp = n2b - nkqmin(k) + 1."""
+def x0kf_index(pw: PairWeights, nctot: int = 0, npm: int = 1, nband: int = 0,
+               pw_minus: PairWeights = None) -> X0Index:
+    """Restates x0kf_v4hz_init job=0 then job=1 (S/x0kf_v4h.f90:96-138).
+
+    pw holds the jpm = 1 pair tables; with npm = 2, pw_minus holds jpm = 2.
+    Bands n > nband are core states (nband = 0: no core bands).  Job 0 takes
+    per k the smallest n1b / n2b over all pairs and the largest over valence
+    bands only; job 1 skips pairs with n1b > nkmax or n2b > nkqmax (which
+    drops every core pair), maps it = nctot + n1b (valence) or n1b - nband
+    (core), itp = ncc + n2b - nkqmin + 1 (valence; ncc = nctot when npm = 2)
+    or n2b - nkqmin + 1 - nband (core), and takes the weights of each pair
+    from whw(jhw + iw - ihw)."""
+    tables = [pw] if npm == 1 else [pw, pw_minus]
+    if npm == 2 and pw_minus is None:
+        raise ValueError("x0kf_index: npm=2 needs the jpm=2 pair tables")
+    nband = nband if nband > 0 else 1 << 30
+    ncc = 0 if npm == 1 else nctot
     nk = len(pw.n1b)
-    nkmin = np.array([p.min() for p in pw.n1b], np.int32)
-    nkmax = np.array([p.max() for p in pw.n1b], np.int32)
-    nkqmin = np.array([p.min() for p in pw.n2b], np.int32)
-    nkqmax = np.array([p.max() for p in pw.n2b], np.int32)
+    nkmin = np.full(nk, 999999, np.int32)
+    nkqmin = np.full(nk, 999999, np.int32)
+    nkmax = np.full(nk, -999999, np.int32)
+    nkqmax = np.full(nk, -999999, np.int32)
+    for t in tables:
+        for k in range(nk):
+            a, b = np.asarray(t.n1b[k]), np.asarray(t.n2b[k])
+            if len(a) == 0:
+                continue
+            nkmin[k] = min(nkmin[k], a.min())
+            nkqmin[k] = min(nkqmin[k], b.min())
+            if (a <= nband).any():
+                nkmax[k] = max(nkmax[k], a[a <= nband].max())
+            if (b <= nband).any():
+                nkqmax[k] = max(nkqmax[k], b[b <= nband].max())
     kc, itc, itpc, jpmc, iwini, iwend, icouini, whwc = ([] for _ in range(8))
     icounkmin = np.zeros(nk, np.int32)
     icounkmax = np.zeros(nk, np.int32)
     icoun = 0
     icount = 0
     for k in range(nk):
-        npair = len(pw.n1b[k])
         icounkmin[k] = icoun + 1
-        kc.append(np.full(npair, k + 1, np.int32))
-        itc.append(nctot + pw.n1b[k])
-        itpc.append(pw.n2b[k] - nkqmin[k] + 1)
-        jpmc.append(np.ones(npair, np.int32))
-        iwini.append(pw.ihw[k])
-        iwend.append(pw.ihw[k] + pw.nhw[k] - 1)
-        ici = np.empty(npair, np.int32)
-        ici[0] = icount + 1
-        ici[1:] = icount + 1 + np.cumsum(pw.nhw[k])[:-1]
-        icouini.append(ici)
-        tot = int(pw.nhw[k].sum())
-        whwc.append(pw.whw[pw.jhw[k][0] - 1: pw.jhw[k][0] - 1 + tot])
-        icoun += npair
-        icount += tot
+        if npm == 2 and nkqmin[k] != 1:
+            raise ValueError("x0kf_index: npm=2 needs nkqmin=1")
+        for jpm, t in enumerate(tables, start=1):
+            a, b = np.asarray(t.n1b[k], np.int64), np.asarray(t.n2b[k], np.int64)
+            ih, nh, jh = (np.asarray(x, np.int64) for x in (t.ihw[k], t.nhw[k], t.jhw[k]))
+            keep = (a <= nkmax[k]) & (b <= nkqmax[k])
+            a, b, ih, nh, jh = a[keep], b[keep], ih[keep], nh[keep], jh[keep]
+            npair = len(a)
+            if npair == 0:
+                continue
+            kc.append(np.full(npair, k + 1))
+            itc.append(np.where(a <= nband, nctot + a, a - nband))
+            itpc.append(np.where(b <= nband, ncc + b - nkqmin[k] + 1, b - nkqmin[k] + 1 - nband))
+            jpmc.append(np.full(npair, jpm))
+            iwini.append(ih)
+            iwend.append(ih + nh - 1)
+            icouini.append(icount + 1 + np.concatenate([[0], np.cumsum(nh)[:-1]]))
+            # bin iw of a pair has weight whw(jhw + iw - ihw): gather per pair
+            off = np.repeat(jh - 1, nh) + (np.arange(int(nh.sum())) - np.repeat(np.cumsum(nh) - nh, nh))
+            whwc.append(np.asarray(t.whw)[off])
+            icoun += npair
+            icount += int(nh.sum())
         icounkmax[k] = icoun
-    cat = lambda xs, dt: np.ascontiguousarray(np.concatenate(xs).astype(dt))
+    cat = lambda xs, dt: np.ascontiguousarray(np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt))
     return X0Index(cat(kc, np.int32), cat(itc, np.int32), cat(itpc, np.int32), cat(jpmc, np.int32),
                    cat(iwini, np.int32), cat(iwend, np.int32), cat(icouini, np.int32),
                    cat(whwc, np.float64), icounkmin, icounkmax, nkmin, nkmax, nkqmin, nkqmax)
 
 
-def make_uneven_qpoint(cfg: Config, no_range=(8, 24), nu_range=(150, 400)) -> QPoint:
+def make_uneven_qpoint(cfg: Config, no_range=(8, 24), nu_range=(150, 400), windows=None) -> QPoint:
     """A q-point whose k-points have their own band windows: occupied counts
     drawn from U{no_range}, unoccupied from U{nu_range} (per k, seeded), the
     uneven per-k ranges that ecalj's x0kf_v4hz_init and sxcf_scz_count
     handle (nkmin..nkqmax per k, S/x0kf_v4h.f90:91-137).  cfg.no / cfg.nu
     must be the range maxima; the k-th window uses the first no_k + nu_k
-    bands, occupied energies for the first no_k of them."""
+    bands, occupied energies for the first no_k of them.  windows = (no_k,
+    nu_k) fixes the per-k counts instead of drawing them."""
     assert cfg.no == no_range[1] and cfg.nu == nu_range[1]
     qp = make_qpoint(cfg)
     rng = np.random.Generator(np.random.PCG64(cfg.seed + 3000))
     no_k = rng.integers(no_range[0], no_range[1] + 1, cfg.nk)
     nu_k = rng.integers(nu_range[0], nu_range[1] + 1, cfg.nk)
+    if windows is not None:
+        no_k, nu_k = np.asarray(windows[0], int), np.asarray(windows[1], int)
+        if (len(no_k) != cfg.nk or len(nu_k) != cfg.nk or no_k.min() < no_range[0] or no_k.max() > no_range[1]
+                or nu_k.min() < nu_range[0] or nu_k.max() > nu_range[1]):
+            raise ValueError("make_uneven_qpoint: windows outside the ranges")
     for k in range(cfg.nk):  # energies of the k-th window: no_k occupied, then unoccupied
         for e in (qp.e_k, qp.e_kq, qp.e_qmk):
             e[:, k] = _energies(rng, int(no_k[k]), cfg.nbt - int(no_k[k]))

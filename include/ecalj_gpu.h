@@ -92,7 +92,14 @@ ecalj_status ecalj_set_tuning(ecalj_handle h, int mode);
  * Index lists built on the host at capture time (x0gemm bins, Hilbert
  * coefficients) are fixed in the graph: a replay repeats the captured
  * problem on whatever data the device buffers hold.  Kernel timing is off
- * while capturing.  Fill kernels, not memset nodes, clear memory. */
+ * while capturing.  Fill kernels, not memset nodes, clear memory.
+ * A graph keeps the device addresses of the handle's scratch and state as
+ * they were at capture.  Any later call that frees or moves them (scratch
+ * growth in an eager call, set_ppb, set_coulomb, set_ppovl, set_hamindex)
+ * starts a new state generation: ecalj_graph_launch then returns
+ * ECALJ_ERR_STATE for graphs of an older generation, and ECALJ_ERR_ARG when
+ * called with a handle other than the capturing one.  ecalj_graph_destroy
+ * waits for the graph's last launch. */
 typedef struct ecalj_graph_s* ecalj_graph_t;
 ecalj_status ecalj_graph_begin(ecalj_handle h);
 ecalj_status ecalj_graph_end(ecalj_handle h, ecalj_graph_t* graph);

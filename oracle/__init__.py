@@ -51,7 +51,37 @@ def lib():
         _LIB.orc_weavx2.restype = C.c_double
         for name in ("orc_wfacx2", "orc_weavx2"):
             getattr(_LIB, name).argtypes = [C.c_double] * 4
+        _LIB.orc_use_blas.argtypes = [C.c_void_p] * 3
+        _LIB.orc_blas_active.restype = C.c_int
+        use_blas(os.environ.get("ECALJ_ORACLE_LOOPS", "0") != "1")
     return _LIB
+
+
+def _capsule_ptr(mod, name):
+    cap = mod.__pyx_capi__[name]
+    get_name = C.pythonapi.PyCapsule_GetName
+    get_name.restype, get_name.argtypes = C.c_char_p, [C.py_object]
+    get_ptr = C.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype, get_ptr.argtypes = C.c_void_p, [C.py_object, C.c_char_p]
+    return get_ptr(cap, get_name(cap))
+
+
+def use_blas(on: bool = True) -> bool:
+    """Route the oracle's complex GEMMs and LU inverses through the LP64
+    zgemm / zgetrf / zgetri that scipy ships (the routines the reference calls,
+    S/m_blas.f90:225, S/m_lapack.f90:21-47), or (on=False) through the oracle's
+    own loops.  Returns whether the BLAS path is active."""
+    lib_ = lib()
+    if not on:
+        lib_.orc_use_blas(None, None, None)
+        return False
+    try:
+        import scipy.linalg.cython_blas as cb
+        import scipy.linalg.cython_lapack as cl
+        lib_.orc_use_blas(_capsule_ptr(cb, "zgemm"), _capsule_ptr(cl, "zgetrf"), _capsule_ptr(cl, "zgetri"))
+    except (ImportError, KeyError, AttributeError):
+        lib_.orc_use_blas(None, None, None)
+    return bool(lib_.orc_blas_active())
 
 
 def ref():
@@ -582,3 +612,70 @@ def rotipw(case, iq, geig_in):
     if rc != 0:
         raise ValueError("orc_rotipw: rotated G outside the target list")
     return out
+
+
+def x0kf_init(tables, whw, nband, nctot=0, nwhis=None):
+    """x0kf_v4hz_init (S/x0kf_v4h.f90:48-142) on tetwt's per-k pair tables.
+
+    tables: one entry per jpm (npm = len(tables)); each entry a tuple of
+    per-k lists (n1b, n2b, ihw, nhw, jhw) of 1-based int arrays, as gettetwt
+    returns them; whw: all weights (jhw points into it, 1-based).  Returns a
+    namespace with the fields of the index lists (kc, itc, itpc, jpmc, iwini,
+    iwend, icouini, whwc, icounkmin, icounkmax, nkmin, nkmax, nkqmin, nkqmax)."""
+    from types import SimpleNamespace
+
+    npm = len(tables)
+    nqbz = len(tables[0][0])
+    nbnb = np.zeros((nqbz, npm), np.int32, order="F")
+    for j in range(npm):
+        for k in range(nqbz):
+            nbnb[k, j] = len(tables[j][0][k])
+    nbnbx = max(1, int(nbnb.max()))
+    arrs = []
+    for f in range(5):
+        a = np.zeros((nbnbx, nqbz, npm), np.int32, order="F")
+        for j in range(npm):
+            for k in range(nqbz):
+                v = np.asarray(tables[j][f][k], np.int32)
+                a[:len(v), k, j] = v
+        arrs.append(a)
+    if nwhis is None:
+        nwhis = max(int((arrs[2] + arrs[3] - 1).max()), 1)
+    cap_coun = max(1, int(nbnb.sum()))
+    cap_count = max(1, int(sum(int(np.sum(tables[j][3][k])) for j in range(npm) for k in range(nqbz))))
+    o = {n: np.zeros(cap_coun, np.int32) for n in ("kc", "itc", "itpc", "jpmc", "iwini", "iwend", "icouini")}
+    whwc = np.zeros(cap_count)
+    per_k = {n: np.zeros(nqbz, np.int32) for n in ("icounkmin", "icounkmax", "nkmin", "nkmax", "nkqmin", "nkqmax")}
+    ncoun, ncount = C.c_long(0), C.c_long(0)
+    whw = np.ascontiguousarray(whw, np.float64)
+    r = lib().orc_x0kf_init(C.c_int(nqbz), C.c_int(npm), C.c_int(nbnbx), _p(nbnb), *[_p(a) for a in arrs], _p(whw),
+                            C.c_int(nband), C.c_int(nctot), C.c_int(nwhis), C.c_long(cap_coun), C.c_long(cap_count),
+                            *[_p(o[n]) for n in ("kc", "itc", "itpc", "jpmc", "iwini", "iwend", "icouini")],
+                            _p(whwc), *[_p(per_k[n]) for n in ("icounkmin", "icounkmax", "nkmin", "nkmax", "nkqmin",
+                                                               "nkqmax")],
+                            C.byref(ncoun), C.byref(ncount))
+    if r:
+        raise ValueError(f"orc_x0kf_init: status {r}")
+    n1, n2 = ncoun.value, ncount.value
+    out = {k: v[:n1].copy() for k, v in o.items()}
+    out["whwc"] = whwc[:n2].copy()
+    out.update(per_k)
+    return SimpleNamespace(**out)
+
+
+def real_mesh(eq_ry, ebz_ry, esmr, frhis, deltaw=0.02, ffac=10.0, ef=0.0):
+    """getwemax + getfreq's real mesh (S/m_freq.f90:197-268, 110-129): freq_r(0:nw), Hartree."""
+    eq = np.ascontiguousarray(eq_ry, np.float64)
+    ebz = np.ascontiguousarray(ebz_ry, np.float64).ravel()
+    fr = np.ascontiguousarray(frhis, np.float64)
+    nwhis = len(fr) - 1
+    out = np.zeros(nwhis + 1)
+    wemax = C.c_double(0.0)
+    L = lib()
+    L.orc_real_mesh.restype = C.c_int
+    nw = L.orc_real_mesh(C.c_int(len(eq)), _p(eq), C.c_long(len(ebz)), _p(ebz), C.c_double(ef), C.c_double(deltaw),
+                         C.c_double(esmr), C.c_double(ffac), C.c_int(nwhis), _p(fr), C.byref(wemax), _p(out),
+                         C.c_int(len(out)))
+    if nw < 0:
+        raise ValueError("real_mesh: histogram mesh does not reach wemax")
+    return out[:nw + 1].copy()

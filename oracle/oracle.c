@@ -24,9 +24,38 @@ static inline double complex opget(char op, const double complex* a, long lda, l
   return conj(a[l + lda * i]);
 }
 
+/* Optional LP64 Fortran-ABI BLAS/LAPACK (orc_use_blas).  The reference calls
+ * zgemm(3m) and zgetrf+zgetri from MKL (S/m_blas.f90:225, S/m_lapack.f90:21-47);
+ * with the hooks set the oracle calls the same routines of another BLAS, and
+ * without them it runs the loops below (same results up to rounding).     */
+typedef void (*zgemm_fn)(char*, char*, int*, int*, int*, double complex*, const double complex*, int*,
+                         const double complex*, int*, double complex*, double complex*, int*);
+typedef void (*zgetrf_fn)(int*, int*, double complex*, int*, int*, int*);
+typedef void (*zgetri_fn)(int*, double complex*, int*, int*, double complex*, int*, int*);
+static zgemm_fn blas_zgemm = NULL;
+static zgetrf_fn lapack_zgetrf = NULL;
+static zgetri_fn lapack_zgetri = NULL;
+
+void orc_use_blas(void* zgemm, void* zgetrf, void* zgetri) {
+  blas_zgemm = (zgemm_fn)zgemm;
+  lapack_zgetrf = (zgetrf_fn)zgetrf;
+  lapack_zgetri = (zgetri_fn)zgetri;
+}
+
+int orc_blas_active(void) { return blas_zgemm != NULL; }
+
+static int fits_int(long v) { return v >= 0 && v <= 2147483647L; }
+
 void orc_zgemm(char opa, char opb, int m, int n, int k, double complex alpha,
                const double complex* a, long lda, const double complex* b, long ldb,
                double complex beta, double complex* c, long ldc) {
+  if (m <= 0 || n <= 0) return;
+  if (blas_zgemm && fits_int(lda) && fits_int(ldb) && fits_int(ldc) && k > 0) {
+    int im = m, in = n, ik = k, ilda = lda > 0 ? (int)lda : 1, ildb = ldb > 0 ? (int)ldb : 1, ildc = (int)ldc;
+    char ta = opa, tb = opb;
+    blas_zgemm(&ta, &tb, &im, &in, &ik, &alpha, a, &ilda, b, &ildb, &beta, c, &ildc);
+    return;
+  }
 #pragma omp parallel for schedule(dynamic, 4)
   for (int j = 0; j < n; ++j) {
     double complex* cj = c + ldc * (long)j;
@@ -368,28 +397,22 @@ int orc_zmel(int natom, const int* nv, const int* mdim, const double complex* ph
   for (int a = 0; a < natom; ++a) {
     int v = nv[a], md = mdim[a];
     const double* pa = ppb + poff;
-    /* ppbvphiq(b, m, i) = sum_c ppb(c, b, i) cphiq(ias+c, m) */
-    double complex* pv = (double complex*)malloc(sizeof(double complex) * (size_t)v * nq * md);
-#pragma omp parallel for
-    for (int i = 0; i < md; ++i)
+    /* the reference's two batched GEMMs over i (:353-360):
+     *   ppbvphiq(b, m, i) = sum_c ppb(c, b, i) cphiq(ias+c, m)              (op T)
+     *   zmelt(ims+i, n, m) = phase * sum_b conj(cphim(ias+b, n)) ppbvphiq(b, m, i)  (op C) */
+    double complex* pc = (double complex*)malloc(sizeof(double complex) * (size_t)v * v);
+    double complex* pv = (double complex*)malloc(sizeof(double complex) * (size_t)v * (nq > 0 ? nq : 1));
+    double complex* zt = (double complex*)malloc(sizeof(double complex) * (size_t)(nm > 0 ? nm : 1) * (nq > 0 ? nq : 1));
+    for (int i = 0; i < md; ++i) {
+      for (long e = 0; e < (long)v * v; ++e) pc[e] = pa[e + (long)v * v * i];
+      orc_zgemm('T', 'N', v, nq, v, 1.0, pc, v, cphiq + ias, ndima, 0.0, pv, v);
+      orc_zgemm('C', 'N', nm, nq, v, phase[a], cphim + ias, ndima, pv, v, 0.0, zt, nm);
       for (int m = 0; m < nq; ++m)
-        for (int b = 0; b < v; ++b) {
-          double complex s = 0.0;
-          for (int c = 0; c < v; ++c)
-            s += pa[c + (long)v * b + (long)v * v * i] * cphiq[ias + c + (long)ndima * m];
-          pv[b + (long)v * m + (long)v * nq * i] = s;
-        }
-    /* zmelt(ims+i, n, m) = phase * sum_b conj(cphim(ias+b, n)) ppbvphiq(b, m, i) */
-#pragma omp parallel for
-    for (int i = 0; i < md; ++i)
-      for (int m = 0; m < nq; ++m)
-        for (int n = 0; n < nm; ++n) {
-          double complex s = 0.0;
-          for (int b = 0; b < v; ++b)
-            s += conj(cphim[ias + b + (long)ndima * n]) * pv[b + (long)v * m + (long)v * nq * i];
-          zmelt[ims + i + (long)ngb * (n + (long)nm * m)] = phase[a] * s;
-        }
+        for (int n = 0; n < nm; ++n) zmelt[ims + i + (long)ngb * (n + (long)nm * m)] = zt[n + (long)nm * m];
+    }
+    free(pc);
     free(pv);
+    free(zt);
     ias += v;
     ims += md;
     poff += (long)v * v * md;
@@ -444,20 +467,29 @@ void orc_x0gemm(double complex* rcxq, int npr, int ipr_col, int npr_col, int nwh
       wt[p] = whwc[iw - iwini[ic - 1] + icouini[ic - 1] - 1];
     }
   }
+  /* per bin: zw(t,I) and wzw(t,J) gathered, rcxq(:,:,iw) += zw^H wzw (:72-90) */
+  long nmax = 0;
+  for (int b = 0; b < nb; ++b)
+    if (start[b + 1] - start[b] > nmax) nmax = start[b + 1] - start[b];
+  double complex* zw = (double complex*)malloc(sizeof(double complex) * (size_t)(nmax > 0 ? nmax : 1) * npr);
+  double complex* wzw = (double complex*)malloc(sizeof(double complex) * (size_t)(nmax > 0 ? nmax : 1) * npr_col);
   for (int jpm = 1; jpm <= npm; ++jpm)
     for (int iw = 1; iw <= nwhis; ++iw) {
       int b = (iw - 1) + nwhis * (jpm - 1);
-      if (start[b + 1] == start[b]) continue;
+      long nt = start[b + 1] - start[b];
+      if (nt == 0) continue;
       int fidx = iw * (3 - 2 * jpm);
       double complex* r = rcxq + slice * (fidx - s0);
-#pragma omp parallel for
-      for (int j = 0; j < npr_col; ++j)
-        for (long p = start[b]; p < start[b + 1]; ++p) {
-          const double complex* z = zmel + col[p];
-          double complex wz = z[j + ipr_col - 1] * wt[p];
-          for (int i = 0; i < npr; ++i) r[i + (long)npr * j] += conj(z[i]) * wz;
-        }
+      for (long t = 0; t < nt; ++t) {
+        const double complex* z = zmel + col[start[b] + t];
+        double w = wt[start[b] + t];
+        for (int i = 0; i < npr; ++i) zw[t + nt * i] = z[i];
+        for (int j = 0; j < npr_col; ++j) wzw[t + nt * j] = z[j + ipr_col - 1] * w;
+      }
+      orc_zgemm('C', 'N', npr, npr_col, (int)nt, 1.0, zw, nt, wzw, nt, 1.0, r, npr);
     }
+  free(zw);
+  free(wzw);
   free(cnt);
   free(start);
   free(col);
@@ -475,13 +507,11 @@ void orc_dpsion_imag(int npr, int npr_col, int nwhis, int niw, const double* frh
     double complex* r = rcxq + slice * iw;
     for (long e = 0; e < slice; ++e) r[e] *= s;
   }
-#pragma omp parallel for
-  for (long e = 0; e < slice; ++e)
-    for (int it = 0; it < niw; ++it) {
-      double complex s = 0.0;
-      for (int iw = 1; iw <= nwhis; ++iw) s += rcxq[e + slice * iw] * imatt[it + (long)niw * (iw - 1)];
-      zxqi[e + slice * it] = s;
-    }
+  /* zxqi = rcxq(:, 1:nwhis) imatt^T, the reference's GEMM (:229) */
+  double complex* im = (double complex*)malloc(sizeof(double complex) * (size_t)niw * nwhis);
+  for (long e = 0; e < (long)niw * nwhis; ++e) im[e] = imatt[e];
+  orc_zgemm('N', 'T', (int)slice, niw, nwhis, 1.0, rcxq + slice, slice, im, niw, 0.0, zxqi, slice);
+  free(im);
 }
 
 /* Composed direct-frequency chi0 (associativity of the x0gemm bin sum and
@@ -506,6 +536,17 @@ void orc_chi0_freq(int ni, int npairs, int nf, const double complex* z, const do
 /* zminv_h (S/m_lapack.f90:21-47): zgetrf + zgetri.  Restated as LU with
  * partial pivoting followed by inversion; same inverse up to rounding.   */
 int orc_zinv(int n, double complex* a, long lda) {
+  if (n <= 0) return 0;
+  if (lapack_zgetrf && lapack_zgetri && fits_int(lda)) {
+    int in = n, ilda = (int)lda, info = 0, lwork = 64 * n;
+    int* ipiv = (int*)malloc(sizeof(int) * n);
+    double complex* work = (double complex*)malloc(sizeof(double complex) * (size_t)lwork);
+    lapack_zgetrf(&in, &in, a, &ilda, ipiv, &info);
+    if (info == 0) lapack_zgetri(&in, a, &ilda, ipiv, work, &lwork, &info);
+    free(ipiv);
+    free(work);
+    return info == 0 ? 0 : 2;
+  }
   int* piv = (int*)malloc(sizeof(int) * n);
   /* LU factorisation, right-looking, partial pivoting */
   for (int k = 0; k < n; ++k) {

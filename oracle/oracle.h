@@ -62,6 +62,24 @@ extern "C" {
 void orc_zgemm(char opa, char opb, int m, int n, int k, double complex alpha,
                const double complex* a, long lda, const double complex* b, long ldb,
                double complex beta, double complex* c, long ldc);
+/* x0kf_v4hz_init (S/x0kf_v4h.f90:48-142), jobs 0 and 1: the flattened pair
+ * and bin lists of x0gemm from tetwt's per-k pair tables.  Pair tables are
+ * Fortran arrays (nbnbx, nqbz, npm); nbnb is (nqbz, npm); indices 1-based. */
+int orc_x0kf_init(int nqbz, int npm, int nbnbx, const int* nbnb, const int* n1b, const int* n2b,
+                  const int* ihw, const int* nhw, const int* jhw, const double* whw, int nband, int nctot,
+                  int nwhis, long cap_coun, long cap_count, int* kc, int* itc, int* itpc, int* jpmc,
+                  int* iwini, int* iwend, int* icouini, double* whwc, int* icounkmin, int* icounkmax,
+                  int* nkmin, int* nkmax, int* nkqmin, int* nkqmax, long* ncoun_out, long* ncount_out);
+
+/* getwemax (S/m_freq.f90:197-268) + getfreq's real mesh (:110-129): nw and
+ * freq_r(0:nw) (Hartree) from the external and BZ band energies (Ry). */
+int orc_real_mesh(int neq, const double* eq, long nbz, const double* ebz, double ef, double deltaw, double esmr,
+                  double ffac, int nwhis, const double* frhis, double* wemax_out, double* freq_r, int cap);
+
+/* Hooks to an LP64 Fortran-ABI BLAS/LAPACK: zgemm for orc_zgemm, zgetrf +
+ * zgetri for orc_zinv (the routines the reference calls). NULL = loops. */
+void orc_use_blas(void* zgemm, void* zgetrf, void* zgetri);
+int orc_blas_active(void);
 
 /* gauss (S/mate.f90:1-206): n-point Gauss-Legendre on [a,b]. Returns 0 or -1
  * when n is not one of the reference's supported sizes. */

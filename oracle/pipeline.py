@@ -45,8 +45,8 @@ def chi0(qp: synth.QPoint, k_subset=None, real_axis=False):
     fr = synth.frhis(qp)
     nwhis = len(fr) - 1
     nuc = cfg.nu_chi0 if cfg.nu_chi0 else cfg.nu   # nband_chi0 truncation (S/m_readgwinput.f90:34-35)
-    pw = synth.pair_weights(qp, fr, nuc)
-    ix = synth.x0kf_index(pw)
+    pw = synth.pair_weights(qp, fr, nuc)   # the synthetic stand-in for tetwt's output (an input here)
+    ix = orc.x0kf_init([(pw.n1b, pw.n2b, pw.ihw, pw.nhw, pw.jhw)], pw.whw, nband=cfg.nbt, nwhis=nwhis)
     _, _, _, _, nu = frequencies(cfg)
     imatt = orc.imatt(nu, fr)
     ni, no = cfg.ni, cfg.no
@@ -62,7 +62,9 @@ def chi0(qp: synth.QPoint, k_subset=None, real_axis=False):
                    ix.jpmc, ix.icouini, ix.whwc)
     chi = orc.dpsion_imag(rcxq, fr, imatt)
     if real_axis:  # the same dpsion_chiq call's real branch, on the scaled rcxq (S/dpsion5.f90:243-256)
-        freqr = synth.real_mesh(qp, fr)
+        cfg = qp.cfg   # getwemax's inputs: external states at q, all bands at k and k+q (E_F = 0)
+        freqr = orc.real_mesh(qp.e_q[:cfg.nq] / synth.RYDBERG_EV,
+                              np.concatenate([qp.e_k.ravel(), qp.e_kq.ravel()]) / synth.RYDBERG_EV, cfg.esmr, fr)
         chi_r = orc.dpsion_real(rcxq, fr, orc.rmatt(freqr, fr), scaled=True)
         return chi, nwhis, chi_r, freqr
     return chi, nwhis

@@ -65,3 +65,53 @@ def test_capture_without_warmup_is_refused():
         assert np.isfinite(g.results()[3]).all()
     finally:
         g.h.close()
+
+
+def test_graph_refused_after_scratch_moves():
+    """A graph keeps the scratch addresses of its capture.  An eager call that
+    makes the same handle's scratch grow (here a larger zmel) frees the old
+    scratch, so a replay of the older graph is refused with a state error on
+    the host instead of running on freed memory; so is a replay after
+    set_coulomb, and a launch on a handle other than the capturing one.  A
+    graph recorded after the growth replays normally."""
+    import ctypes as C
+
+    import ecalj_amd
+    from ecalj_amd.qsgw import QPointGPU
+
+    qp = synth.make_qpoint(synth.CONFIGS["T3"])
+    g = QPointGPU(qp, "f64")
+    other = ecalj_amd.Handle(0, "f64")
+    try:
+        g.step()
+        g.make_wk()
+        g.step()
+        zsec0 = g.results()[2]
+        g1 = g.capture_step()
+        g1.launch()
+        assert np.array_equal(g.results()[2], zsec0)
+        st = ecalj_amd.lib().ecalj_graph_launch(other.ptr, C.c_void_p(g1.ptr))
+        assert st == 4  # ECALJ_ERR_ARG: the graph belongs to another handle
+        say("graph: growing the handle's scratch with a larger eager zmel")
+        rng = np.random.default_rng(1)
+        nb = qp.cfg.nb
+        big = rng.standard_normal((nb, 600)) + 1j * rng.standard_normal((nb, 600))
+        g.h.zmel(big[:, :5], big, zmelconjg=True)
+        with pytest.raises(ecalj_amd.EcaljError) as e:
+            g1.launch()
+        assert e.value.status == 6  # ECALJ_ERR_STATE
+        g.step()                    # eager work is unaffected
+        assert np.array_equal(g.results()[2], zsec0)
+        g2 = g.capture_step()       # recorded at the new generation: replays
+        g.zsec.zero()
+        g2.launch()
+        assert np.array_equal(g.results()[2], zsec0)
+        g.h.set_coulomb(qp.vcoud, qp.zcousq)
+        with pytest.raises(ecalj_amd.EcaljError) as e:
+            g2.launch()
+        assert e.value.status == 6
+        g1.close()
+        g2.close()
+    finally:
+        other.close()
+        g.h.close()
