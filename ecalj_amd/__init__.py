@@ -52,6 +52,8 @@ _SIGS = {
     "ecalj_tuning_report": [_p, C.c_char_p, _l],
     "ecalj_set_ppb": [_p, _i, _p, _p, _p],
     "ecalj_set_coulomb": [_p, _i, _p, _p],
+    "ecalj_zhgv": [_p, _p, _p, _i, _p, _i, _i, _i, _i],
+    "ecalj_zhgv_dev": [_p, _p, _p, _i, _p, _i, _i, _i, _i],
     "ecalj_zmel": [_p, _p, _i, _i, _p, _i, _p, _i, _i, _i, _p],
     "ecalj_zmel_dev": [_p, _p, _i, _i, _p, _i, _p, _i, _i, _i, _p],
     "ecalj_zmel_strided_dev": [_p, _p, _i, _i, _p, _l, _i, _p, _l, _i, _i, _i, _p],
@@ -447,6 +449,34 @@ class Handle:
         v = np.ascontiguousarray(np.asarray(vcoud, np.float64))
         z = _f(zcousq, np.complex128)
         _chk(lib().ecalj_set_coulomb(self._h, len(v), _np_ptr(v), _np_ptr(z)), "set_coulomb")
+
+    def zhgv(self, a, b, il=1, iu=None):
+        """zhgv_h (S/m_lapack.f90:64-104): eigenpairs il..iu (1-based, ascending)
+        of a z = lambda b z from the upper triangles; returns (evl, z)."""
+        a = _f(a, np.complex128).copy(order="F")
+        b = _f(b, np.complex128)
+        n = a.shape[0]
+        iu = n if iu is None else iu
+        evl = np.zeros(iu - il + 1)
+        _chk(lib().ecalj_zhgv(self._h, _np_ptr(a), _np_ptr(b), n, _np_ptr(evl), il, iu, a.shape[0], b.shape[0]),
+             "zhgv")
+        return evl, a[:, :iu - il + 1].copy(order="F")
+
+    def zhgv_dev(self, a, b, n, evl, il=1, iu=None, lda=None, ldb=None):
+        iu = n if iu is None else iu
+        _chk(lib().ecalj_zhgv_dev(self._h, _t_ptr(a), _t_ptr(b), n, _t_ptr(evl), il, iu, lda or n, ldb or n),
+             "zhgv_dev")
+
+    def coulomb_from_matrix(self, vcoul, oo):
+        """hvccfp0's Diagonalize_Coulomb_matrix (S/main_hvccfp0.f90:322-368,
+        396-401): zhgv on -vcoul (eigenvalues of vcoul in descending order),
+        vcoud = -eb, zcousq = z; then Readvcoud (set_coulomb).  Returns
+        (vcoud, zcousq).  The reference's KeepPositiveCou option (-eb < 0 -> 0)
+        is left to the caller: set_coulomb refuses negative vcoud."""
+        eb, z = self.zhgv(-np.asarray(vcoul, np.complex128), oo)
+        vcoud = -eb
+        self.set_coulomb(vcoud, z)
+        return vcoud, z
 
     # ---- zmel
     def _phase(self, phase):

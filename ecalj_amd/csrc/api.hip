@@ -252,6 +252,8 @@ const char* ecalj_status_string(int s) {
     case ECALJ_ERR_ARG: return "bad argument";
     case ECALJ_ERR_NOMEM: return "device allocation failed";
     case ECALJ_ERR_STATE: return "required set-up call missing";
+    case ECALJ_ERR_NOT_PD: return "matrix not positive definite";
+    case ECALJ_ERR_NOT_CONVERGED: return "eigensolver did not converge";
     default: return "unknown status";
   }
 }

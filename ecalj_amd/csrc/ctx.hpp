@@ -45,7 +45,8 @@ struct ecalj_ctx {
 
   // grow-only scratch slots
   enum { S_T = 0, S_WH, S_CZ, S_PART, S_LIST, S_GJP, S_GJC, S_EPS, S_TMP, S_TMP2, S_COEF, S_STAGE,
-         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_GAM, S_ROT1, S_ROT2, S_ROT3, S_QA, S_QB, S_QW, S_GJW, S_X0A, S_X0B, S_ZA, S_ZB, S_ZPA, S_ZPB, S_NSLOT };
+         S_CVT1, S_CVT2, S_MLP, S_TAIL, S_GTAB, S_RTAB, S_RSLOT, S_KST, S_GJPIV, S_GJD, S_GJOC, S_GJR, S_IPWT, S_IPW1, S_IPW2, S_IPW3, S_IPWB, S_GAM, S_ROT1, S_ROT2, S_ROT3, S_QA, S_QB, S_QW, S_GJW, S_X0A, S_X0B, S_ZA, S_ZB, S_ZPA, S_ZPB,
+         S_EIG1, S_EIG2, S_EIG3, S_EIG4, S_EIG5, S_EIG6, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag

@@ -14,6 +14,7 @@ module ecalj_gpu
   public :: ecalj_sigc_weights_dev, ecalj_sigc_dev, ecalj_sigc_grouped_dev
   public :: ecalj_set_tuning, ecalj_graph_begin, ecalj_graph_end, ecalj_graph_launch, ecalj_graph_destroy
   public :: ecalj_dpsion_real, ecalj_dpsion_real_dev, ecalj_wc_build_real, ecalj_wc_build_real_dev
+  public :: ecalj_zhgv, ecalj_zhgv_dev
   public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
   public :: ecalj_tetwt, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
   public :: ecalj_wv_write_dev, ecalj_wv_read_dev
@@ -170,6 +171,20 @@ module ecalj_gpu
       type(c_ptr), value :: h, a
       integer(c_int), value :: n, lda, nbatch
       integer(c_int64_t), value :: stride
+    end function
+    ! zhgv_h / zhgv_d (S/m_lapack.f90:64-104,203-229): a z = lambda b z, eigenpairs il..iu
+    integer(c_int) function ecalj_zhgv(h, a, b, n, evl, il, iu, lda, ldb) bind(C, name="ecalj_zhgv")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(inout) :: a(*)
+      complex(c_double_complex), intent(in) :: b(*)
+      real(c_double), intent(out) :: evl(*)
+      integer(c_int), value :: n, il, iu, lda, ldb
+    end function
+    integer(c_int) function ecalj_zhgv_dev(h, a, b, n, evl, il, iu, lda, ldb) bind(C, name="ecalj_zhgv_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, a, b, evl
+      integer(c_int), value :: n, il, iu, lda, ldb
     end function
     ! gettetwt + tetwt5x_dtet4, npm=1 (S/m_tetwt.f90:36-256, S/tetwt5.f90:9-486)
     integer(c_int) function ecalj_tetwt(h, nband, nqbz, ntetf, idtetf, nqbzw, qbzw, ib1bz, ekxx1, ekxx2, ef, ebmx, &

@@ -35,7 +35,9 @@ typedef enum {
   ECALJ_ERR_HIP = 3,         /* HIP runtime error                            */
   ECALJ_ERR_ARG = 4,         /* NULL pointer or bad enum                     */
   ECALJ_ERR_NOMEM = 5,       /* device allocation failed                     */
-  ECALJ_ERR_STATE = 6        /* required set-up call missing (set_ppb, ...)  */
+  ECALJ_ERR_STATE = 6,       /* required set-up call missing (set_ppb, ...)  */
+  ECALJ_ERR_NOT_PD = 7,      /* matrix not positive definite (zhgv's B)      */
+  ECALJ_ERR_NOT_CONVERGED = 8 /* iterative eigensolver did not converge      */
 } ecalj_status;
 
 /* Arithmetic of the contractions (zmel, chi0, M<->E transforms, Sigma_c).
@@ -141,6 +143,23 @@ ecalj_status ecalj_set_ppb(ecalj_handle h, int natom, const int* nv, const int* 
  * S/m_zmel.f90:46-68): eigenvalues vcoud(ngb) and eigenvectors
  * zcousq(ngb,ngb) of the Coulomb matrix; vcousq = sqrt(vcoud).  Host pointers. */
 ecalj_status ecalj_set_coulomb(ecalj_handle h, int ngb, const double* vcoud, const double* zcousq);
+/* zhgv_h / zhgv_d (S/m_lapack.f90:64-104, 203-229), the Coulomb-matrix
+ * diagonalisation of hvccfp0 (S/main_hvccfp0.f90:322-368): the
+ * Hermitian-definite generalized eigenproblem a z = lambda b z (itype 1,
+ * upper triangles of a and b read), FP64 throughout on the device.
+ * Eigenvalues il..iu (1-based, of the ascending list; the reference passes
+ * 1..n) go to evl(1:m), m = iu - il + 1, and their b-orthonormal
+ * eigenvectors (z^H b z = 1) overwrite a(:, 1:m); b is left unchanged.
+ * hvccfp0 passes -vcoul to get the Coulomb eigenvalues in descending order,
+ * then stores vcoud = -evl and zcousq = z (feed them to ecalj_set_coulomb).
+ * ECALJ_ERR_NOT_PD when b is not positive definite (zhegvx: info > n),
+ * ECALJ_ERR_NOT_CONVERGED if the Jacobi sweeps do not converge.  Waits for
+ * the stream; not under graph capture.  ecalj_zhgv: host pointers;
+ * ecalj_zhgv_dev: device pointers (evl too). */
+ecalj_status ecalj_zhgv(ecalj_handle h, double* a, const double* b, int n, double* evl, int il, int iu,
+                        int lda, int ldb);
+ecalj_status ecalj_zhgv_dev(ecalj_handle h, void* a, const void* b, int n, double* evl, int il, int iu,
+                            int lda, int ldb);
 
 /* ---- zmel ---------------------------------------------------------------
  * get_zmel_init_gemm (S/m_zmel.f90:146-597), MT valence part with identity
