@@ -57,6 +57,9 @@ _SIGS = {
     "ecalj_zmel": [_p, _p, _i, _i, _p, _i, _p, _i, _i, _i, _p],
     "ecalj_zmel_dev": [_p, _p, _i, _i, _p, _i, _p, _i, _i, _i, _p],
     "ecalj_zmel_strided_dev": [_p, _p, _i, _i, _p, _l, _i, _p, _l, _i, _i, _i, _p],
+    "ecalj_set_ppb_full": [_p, _i, _i, _i, _p, _p, _p, _p],
+    "ecalj_zmel_init": [_p, _p, _p, _i, _i, _i, _i, _p, _i, _p, _i, _i, _i, _p],
+    "ecalj_zmel_init_dev": [_p, _p, _p, _i, _i, _i, _i, _p, _i, _p, _i, _i, _i, _p],
     "ecalj_x0gemm": [_p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p],
     "ecalj_x0gemm_batched_dev": [_p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p, _l, _i, _i, _p, _p, _i, _p, _p,
                                  _p, _p, _p, _p, _i, _p],
@@ -449,6 +452,29 @@ class Handle:
         v = np.ascontiguousarray(np.asarray(vcoud, np.float64))
         z = _f(zcousq, np.complex128)
         _chk(lib().ecalj_set_coulomb(self._h, len(v), _np_ptr(v), _np_ptr(z)), "set_coulomb")
+
+    def set_ppb_full(self, nlnmc, nlnmv, nblocha, ppb_full):
+        """set_ppb from the reference's ppb(nlnmx, nlnmx, mdimx, natom) with core functions first."""
+        c, v, m = _i32(nlnmc), _i32(nlnmv), _i32(nblocha)
+        p = np.asfortranarray(ppb_full, np.float64)
+        nlnmx, _, mdimx, natom = p.shape
+        _chk(lib().ecalj_set_ppb_full(self._h, natom, nlnmx, mdimx, _np_ptr(c), _np_ptr(v), _np_ptr(m),
+                                      _np_ptr(p)), "set_ppb_full")
+        self.natom = natom
+        self.ngb = int(m.sum())
+
+    def zmel_init(self, ns1, ns2, nctot, cphim, cphiq, phase=None, iatomp=None, is_m_basis=True, zmelconjg=False):
+        """get_zmel_init_gemm's MT part over middle states ns1..ns2 (core first); host arrays."""
+        cq = _f(cphiq, np.complex128)
+        ndima, nq = cq.shape
+        cm = _f(cphim, np.complex128) if cphim is not None and np.size(cphim) else np.zeros((ndima, 0), np.complex128)
+        out = np.zeros((self.ngb, ns2 - ns1 + 1, nq), np.complex128, order="F")
+        ph = self._phase(phase)
+        ia = _i32(iatomp) if iatomp is not None else None
+        _chk(lib().ecalj_zmel_init(self._h, _np_ptr(ph), _np_ptr(ia), ndima, ns1, ns2, nctot,
+                                   _np_ptr(cm) if cm.size else None, nq, _np_ptr(cq), int(is_m_basis),
+                                   int(zmelconjg), 1, _np_ptr(out)), "zmel_init")
+        return out
 
     def zhgv(self, a, b, il=1, iu=None):
         """zhgv_h (S/m_lapack.f90:64-104): eigenpairs il..iu (1-based, ascending)

@@ -299,6 +299,7 @@ ecalj_status ecalj_destroy(ecalj_handle h) {
   }
   if (h->d_ppb) (void)hipFree(h->d_ppb);
   if (h->d_ppb2) (void)hipFree(h->d_ppb2);
+  if (h->d_ppbc) (void)hipFree(h->d_ppbc);
   if (h->d_vcousq) (void)hipFree(h->d_vcousq);
   if (h->d_zcousq) (void)hipFree(h->d_zcousq);
   if (h->d_zcousq32) (void)hipFree(h->d_zcousq32);
@@ -539,6 +540,10 @@ ecalj_status ecalj_set_ppb(ecalj_handle h, int natom, const int* nv, const int* 
   h->natom = natom;
   ECALJ_HIP(hipStreamSynchronize(h->stream));
   ++h->state_gen;
+  h->ncore.assign(natom, 0);  // core blocks come only with ecalj_set_ppb_full
+  h->ppbc_off.assign(natom, 0);
+  if (h->d_ppbc) (void)hipFree(h->d_ppbc);
+  h->d_ppbc = nullptr;
   if (h->d_ppb) (void)hipFree(h->d_ppb);
   if (h->d_ppb2) (void)hipFree(h->d_ppb2);
   h->d_ppb = h->d_ppb2 = nullptr;
@@ -549,6 +554,42 @@ ecalj_status ecalj_set_ppb(ecalj_handle h, int natom, const int* nv, const int* 
   for (int a = 0; a < natom; ++a)
     ECALJ_TRY(ppb_relayout(h, h->d_ppb + h->ppb_off[a], h->d_ppb2 + h->ppb_off[a], nv[a], mdim[a]));
   ECALJ_HIP(hipStreamSynchronize(h->stream));
+  return ECALJ_OK;
+}
+
+ecalj_status ecalj_set_ppb_full(ecalj_handle h, int natom, int nlnmx, int mdimx, const int* nlnmc, const int* nlnmv,
+                                const int* nblocha, const double* ppb) {
+  if (!h || natom < 1 || !nlnmc || !nlnmv || !nblocha || !ppb) return ECALJ_ERR_ARG;
+  for (int a = 0; a < natom; ++a)
+    if (nlnmc[a] < 0 || nlnmv[a] < 0 || nblocha[a] < 0 || nlnmc[a] + nlnmv[a] > nlnmx || nblocha[a] > mdimx)
+      return ECALJ_ERR_DIM;
+  // valence blocks ppb(nc+1:nc+nv, nc+1:nc+nv, 1:mdim, a) for set_ppb, core blocks
+  // ppb(nc+1:nc+nv, 1:nc, 1:mdim, a) (valence x core x product basis) for the CoreValence rows
+  const int64_t s1 = nlnmx, s2 = (int64_t)nlnmx * nlnmx, s3 = s2 * mdimx;
+  std::vector<double> pv, pc;
+  std::vector<int64_t> coff(natom);
+  for (int a = 0; a < natom; ++a) {
+    const int nc = nlnmc[a], nv = nlnmv[a], md = nblocha[a];
+    const double* P = ppb + s3 * a;
+    for (int i = 0; i < md; ++i)
+      for (int b = 0; b < nv; ++b)
+        for (int c = 0; c < nv; ++c) pv.push_back(P[(nc + c) + s1 * (nc + b) + s2 * i]);
+    coff[a] = (int64_t)pc.size();
+    for (int i = 0; i < md; ++i)
+      for (int c = 0; c < nc; ++c)
+        for (int b = 0; b < nv; ++b) pc.push_back(P[(nc + b) + s1 * c + s2 * i]);
+  }
+  ECALJ_TRY(ecalj_set_ppb(h, natom, nlnmv, nblocha, pv.data()));
+  h->ncore.assign(nlnmc, nlnmc + natom);
+  h->ppbc_off = coff;
+  if (!pc.empty()) {
+    if (hipMalloc(&h->d_ppbc, sizeof(double) * pc.size()) != hipSuccess) {
+      h->d_ppbc = nullptr;
+      return ECALJ_ERR_NOMEM;
+    }
+    ECALJ_HIP(hipMemcpyAsync(h->d_ppbc, pc.data(), sizeof(double) * pc.size(), hipMemcpyHostToDevice, h->stream));
+    ECALJ_HIP(hipStreamSynchronize(h->stream));
+  }
   return ECALJ_OK;
 }
 
@@ -616,23 +657,54 @@ struct DevC {
 };
 
 // ------------------------------------------------------------------ zmel
-// get_zmel_init_gemm (S/m_zmel.f90:146-597), MT valence part.  For each atom
+// get_zmel_init_gemm (S/m_zmel.f90:146-597), muffin-tin part.  For each atom
 // block the smaller side is contracted with ppb first:
 //   m-first: T(b,I,n) = sum_a ppb(b,a,I) conj(cphim(a,n));  zmel(I,n,m) = phase sum_b T(b,I,n) cphiq(b,m)
 //   q-first: T(a,I,m) = sum_b ppb(b,a,I) cphiq(b,m);        zmel(I,n,m) = phase sum_a conj(cphim(a,n)) T(a,I,m)
 // cphim / cphiq are complex128; zmel is in the handle's storage type.
+// Optional parts of the middle-state range (ZmelRange): the valence columns
+// of cphim land at column voff of nmt middle states; core states (columns
+// core_c0.. in the range) come from the core blocks of ecalj_set_ppb_full
+// (S/m_zmel.f90:374-390); iatomp maps atom ia to the atom whose product
+// basis, ppb block and row offset it uses (S/m_zmel.f90:295-305, 334).
+struct ZmelRange {
+  int nmt = -1;            // middle states in zmel (-1: the valence count nm)
+  int voff = 0;            // column of the first valence state
+  const int* iatomp = nullptr;  // host, 0-based; nullptr = identity
+  int nm1c = 0, nm2c = -1; // core states in the range (1-based, the reference's nm1c:nm2c)
+  int nm1 = 1;             // first middle state of the range (ns1)
+  int nctot = 0;
+};
+
 static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int nm, const cd* cphim64,
                               int64_t sm, int nq, const cd* cphiq64, int64_t sq, int is_m_basis, int conjg,
-                              int nbatch, void* zmel) {
+                              int nbatch, void* zmel, const ZmelRange& rg = ZmelRange()) {
   if (!h->d_ppb) return ECALJ_ERR_STATE;
   if (nm < 0 || nq < 0 || nbatch < 1 || ndima < h->ndima_used) return ECALJ_ERR_DIM;
   const int ngb = h->ngb_ppb;
   if (!is_m_basis && (!h->d_zcousq || h->ngb != ngb)) return ECALJ_ERR_STATE;
-  if (nm == 0 || nq == 0) return ECALJ_OK;
+  const int nmt = rg.nmt < 0 ? nm : rg.nmt;
+  const bool any_core = rg.nm2c >= rg.nm1c && rg.nctot > 0;
+  if (any_core && !h->d_ppbc) return ECALJ_ERR_STATE;
+  if (nmt == 0 || nq == 0) return ECALJ_OK;
   const int mode = h->precision;
   const size_t es = celt(h);
   const int bm = (sm == 0) ? 1 : nbatch;
   const int bq = (sq == 0) ? 1 : nbatch;
+  // per-atom offsets: eigenvector rows (own atom), product-basis rows and ppb (mapped atom)
+  const int na = h->natom;
+  std::vector<int64_t> ias_v(na + 1, 0), ims_v(na + 1, 0), ics_v(na + 1, 0);
+  for (int a = 0; a < na; ++a) {
+    ias_v[a + 1] = ias_v[a] + h->nv[a];
+    ims_v[a + 1] = ims_v[a] + h->mdim[a];
+    ics_v[a + 1] = ics_v[a] + (a < (int)h->ncore.size() ? h->ncore[a] : 0);
+  }
+  if (rg.iatomp)
+    for (int a = 0; a < na; ++a) {
+      const int ap = rg.iatomp[a];
+      if (ap < 0 || ap >= na || h->nv[ap] != h->nv[a]) return ECALJ_ERR_DIM;
+      if (any_core && h->ncore[ap] != h->ncore[a]) return ECALJ_ERR_DIM;
+    }
   // eigenvectors in the storage type
   const void* cphim = cphim64;
   const void* cphiq = cphiq64;
@@ -640,103 +712,125 @@ static ecalj_status zmel_impl(ecalj_ctx* h, const double* phase, int ndima, int 
     void *m32, *q32;
     const int64_t nme = (bm == 1) ? (int64_t)ndima * nm : sm * (nbatch - 1) + (int64_t)ndima * nm;
     const int64_t nqe = (bq == 1) ? (int64_t)ndima * nq : sq * (nbatch - 1) + (int64_t)ndima * nq;
-    ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CVT1, sizeof(ecalj::cf) * nme, &m32));
+    ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CVT1, sizeof(ecalj::cf) * std::max<int64_t>(nme, 1), &m32));
     ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_CVT2, sizeof(ecalj::cf) * nqe, &q32));
-    ECALJ_TRY(convert(h, cphim64, true, m32, false, nme));
+    if (nme > 0) ECALJ_TRY(convert(h, cphim64, true, m32, false, nme));
     ECALJ_TRY(convert(h, cphiq64, true, q32, false, nqe));
     cphim = m32;
     cphiq = q32;
   }
-  const int64_t zsz = (int64_t)ngb * nm * nq;  // per batch
+  const int64_t zsz = (int64_t)ngb * nmt * nq;  // per batch
   char* zM = (char*)zmel;
   if (!is_m_basis) ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_TMP2, es * zsz * nbatch, (void**)&zM));
+  if (nmt != nm) ECALJ_TRY(dev_zero(h, zM, es * zsz * nbatch));  // core columns: rows of other atoms stay 0
   const bool conj_now = conjg && is_m_basis;
+  const int64_t vcol = (int64_t)ngb * rg.voff;  // element offset of the first valence column
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 2, &tev));
-  int64_t ias = 0, ims = 0;
-  for (int a = 0; a < h->natom; ++a) {
-    const int nv = h->nv[a], md = h->mdim[a];
+  for (int a = 0; a < na; ++a) {
+    const int ap = rg.iatomp ? rg.iatomp[a] : a;
+    const int nv = h->nv[a], md = h->mdim[ap];
+    const int64_t ias = ias_v[a], ims = ims_v[ap];
     const cd ph{phase ? phase[2 * a] : 1.0, phase ? phase[2 * a + 1] : 0.0};
     const int64_t nvmd = (int64_t)nv * md;
-    char* T;
-    if ((int64_t)nm * bm <= (int64_t)nq * bq) {
-      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, es * nvmd * nm * bm, (void**)&T));
-      GemmCall g;  // T[(b+nv*I), (n,k)] = P2 * conj(cphim)
-      g.mode = mode;
-      g.areal = true; g.cb = true;
-      g.M = (int)nvmd; g.K = nv;
-      g.A = h->d_ppb2 + h->ppb_off[a]; g.lda = nvmd;
-      g.B = (const char*)cphim + es * ias; g.ldb = ndima;
-      g.C = T; g.ldc = nvmd;
-      if (bm == 1 || sm == (int64_t)ndima * nm) {
-        g.N = nm * bm;
+    if (nm > 0) {
+      char* T;
+      if ((int64_t)nm * bm <= (int64_t)nq * bq) {
+        ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, es * nvmd * nm * bm, (void**)&T));
+        GemmCall g;  // T[(b+nv*I), (n,k)] = P2 * conj(cphim)
+        g.mode = mode;
+        g.areal = true; g.cb = true;
+        g.M = (int)nvmd; g.K = nv;
+        g.A = h->d_ppb2 + h->ppb_off[ap]; g.lda = nvmd;
+        g.B = (const char*)cphim + es * ias; g.ldb = ndima;
+        g.C = T; g.ldc = nvmd;
+        if (bm == 1 || sm == (int64_t)ndima * nm) {
+          g.N = nm * bm;
+        } else {
+          g.N = nm;
+          g.bB = {sm, 0}; g.bC = {nvmd * nm, 0};
+          g.batch = bm; g.div = bm;
+        }
+        if (mode == ECALJ_PREC_BF16X3)
+          ECALJ_TRY(zmel_p_bf3(h, (int)nvmd, g.N, nv, h->d_ppb2 + h->ppb_off[ap], nvmd,
+                               (const ecalj::cf*)cphim + ias, ndima, g.batch > 1 ? sm : 0, true, (ecalj::cf*)T, nvmd,
+                               g.batch > 1 ? nvmd * nm : 0, g.batch));
+        else
+          ECALJ_TRY(gemm(h, g));
+        GemmCall z;  // per (n,k): zmel(ims+I, voff+n, m) = phase * sum_b T[b + nv*I + nvmd*(n + nm*k)] cphiq_k(ias+b, m)
+        z.mode = mode;
+        z.ta = true;
+        z.M = md; z.N = nq; z.K = nv;
+        z.A = T; z.lda = nv; z.bA = {nvmd, (bm == 1) ? 0 : nvmd * nm};
+        z.B = (const char*)cphiq + es * ias; z.ldb = ndima; z.bB = {0, sq};
+        z.C = zM + es * (ims + vcol); z.ldc = (int64_t)ngb * nmt; z.bC = {ngb, zsz};
+        z.batch = nm * nbatch; z.div = nm;
+        z.alpha = ph; z.conjC = conj_now;
+        if (mode == ECALJ_PREC_BF16X3)
+          ECALJ_TRY(zmel_z_bf3(h, md, nq, nv, (const ecalj::cf*)T, nv, nvmd, (bm == 1) ? 0 : nvmd * nm,
+                               (const ecalj::cf*)cphiq + ias, ndima, sq, false, ph, conj_now,
+                               (ecalj::cf*)zM + ims + vcol, (int64_t)ngb * nmt, ngb, zsz, nm * nbatch, nm));
+        else
+          ECALJ_TRY(gemm(h, z));
       } else {
-        g.N = nm;
-        g.bB = {sm, 0}; g.bC = {nvmd * nm, 0};
-        g.batch = bm; g.div = bm;
-      }
-      if (mode == ECALJ_PREC_BF16X3)
-        ECALJ_TRY(zmel_p_bf3(h, (int)nvmd, g.N, nv, h->d_ppb2 + h->ppb_off[a], nvmd,
-                             (const ecalj::cf*)cphim + ias, ndima, g.batch > 1 ? sm : 0, true, (ecalj::cf*)T, nvmd,
-                             g.batch > 1 ? nvmd * nm : 0, g.batch));
-      else
+        ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, es * nvmd * nq * bq, (void**)&T));
+        GemmCall g;  // T[(a+nv*I), (m,k)] = ppb^T * cphiq, A(r,b) = ppb[b + nv*r]
+        g.mode = mode;
+        g.areal = true; g.ta = true;
+        g.M = (int)nvmd; g.K = nv;
+        g.A = h->d_ppb + h->ppb_off[ap]; g.lda = nv;
+        g.B = (const char*)cphiq + es * ias; g.ldb = ndima;
+        g.C = T; g.ldc = nvmd;
+        if (bq == 1 || sq == (int64_t)ndima * nq) {
+          g.N = nq * bq;
+        } else {
+          g.N = nq;
+          g.bB = {sq, 0}; g.bC = {nvmd * nq, 0};
+          g.batch = bq; g.div = bq;
+        }
         ECALJ_TRY(gemm(h, g));
-      GemmCall z;  // per (n,k): zmel(ims+I, n, m) = phase * sum_b T[b + nv*I + nvmd*(n + nm*k)] cphiq_k(ias+b, m)
-      z.mode = mode;
-      z.ta = true;
-      z.M = md; z.N = nq; z.K = nv;
-      z.A = T; z.lda = nv; z.bA = {nvmd, (bm == 1) ? 0 : nvmd * nm};
-      z.B = (const char*)cphiq + es * ias; z.ldb = ndima; z.bB = {0, sq};
-      z.C = zM + es * ims; z.ldc = (int64_t)ngb * nm; z.bC = {ngb, zsz};
-      z.batch = nm * nbatch; z.div = nm;
-      z.alpha = ph; z.conjC = conj_now;
-      if (mode == ECALJ_PREC_BF16X3)
-        ECALJ_TRY(zmel_z_bf3(h, md, nq, nv, (const ecalj::cf*)T, nv, nvmd, (bm == 1) ? 0 : nvmd * nm,
-                             (const ecalj::cf*)cphiq + ias, ndima, sq, false, ph, conj_now,
-                             (ecalj::cf*)zM + ims, (int64_t)ngb * nm, ngb, zsz, nm * nbatch, nm));
-      else
-        ECALJ_TRY(gemm(h, z));
-    } else {
-      ECALJ_TRY(slot_ensure(h, ecalj_ctx::S_T, es * nvmd * nq * bq, (void**)&T));
-      GemmCall g;  // T[(a+nv*I), (m,k)] = ppb^T * cphiq, A(r,b) = ppb[b + nv*r]
-      g.mode = mode;
-      g.areal = true; g.ta = true;
-      g.M = (int)nvmd; g.K = nv;
-      g.A = h->d_ppb + h->ppb_off[a]; g.lda = nv;
-      g.B = (const char*)cphiq + es * ias; g.ldb = ndima;
-      g.C = T; g.ldc = nvmd;
-      if (bq == 1 || sq == (int64_t)ndima * nq) {
-        g.N = nq * bq;
-      } else {
-        g.N = nq;
-        g.bB = {sq, 0}; g.bC = {nvmd * nq, 0};
-        g.batch = bq; g.div = bq;
+        GemmCall z;  // per (m,k): zmel(ims+I, voff+n, m) = phase * sum_a T[a + nv*I + nvmd*(m + nq*k)] conj(cphim_k(ias+a, n))
+        z.mode = mode;
+        z.ta = true; z.cb = true;
+        z.M = md; z.N = nm; z.K = nv;
+        z.A = T; z.lda = nv; z.bA = {nvmd, (bq == 1) ? 0 : nvmd * nq};
+        z.B = (const char*)cphim + es * ias; z.ldb = ndima; z.bB = {0, sm};
+        z.C = zM + es * (ims + vcol); z.ldc = ngb; z.bC = {(int64_t)ngb * nmt, zsz};
+        z.batch = nq * nbatch; z.div = nq;
+        z.alpha = ph; z.conjC = conj_now;
+        if (mode == ECALJ_PREC_BF16X3)
+          ECALJ_TRY(zmel_z_bf3(h, md, nm, nv, (const ecalj::cf*)T, nv, nvmd, (bq == 1) ? 0 : nvmd * nq,
+                               (const ecalj::cf*)cphim + ias, ndima, sm, true, ph, conj_now,
+                               (ecalj::cf*)zM + ims + vcol, ngb, (int64_t)ngb * nmt, zsz, nq * nbatch, nq));
+        else
+          ECALJ_TRY(gemm(h, z));
       }
-      ECALJ_TRY(gemm(h, g));
-      GemmCall z;  // per (m,k): zmel(ims+I, n, m) = phase * sum_a T[a + nv*I + nvmd*(m + nq*k)] conj(cphim_k(ias+a, n))
-      z.mode = mode;
-      z.ta = true; z.cb = true;
-      z.M = md; z.N = nm; z.K = nv;
-      z.A = T; z.lda = nv; z.bA = {nvmd, (bq == 1) ? 0 : nvmd * nq};
-      z.B = (const char*)cphim + es * ias; z.ldb = ndima; z.bB = {0, sm};
-      z.C = zM + es * ims; z.ldc = ngb; z.bC = {(int64_t)ngb * nm, zsz};
-      z.batch = nq * nbatch; z.div = nq;
-      z.alpha = ph; z.conjC = conj_now;
-      if (mode == ECALJ_PREC_BF16X3)
-        ECALJ_TRY(zmel_z_bf3(h, md, nm, nv, (const ecalj::cf*)T, nv, nvmd, (bq == 1) ? 0 : nvmd * nq,
-                             (const ecalj::cf*)cphim + ias, ndima, sm, true, ph, conj_now, (ecalj::cf*)zM + ims,
-                             ngb, (int64_t)ngb * nm, zsz, nq * nbatch, nq));
-      else
-        ECALJ_TRY(gemm(h, z));
     }
-    ias += nv;
-    ims += md;
+    if (any_core) {  // CoreValence (S/m_zmel.f90:374-390): zmel(ims+I, it, m) = phase sum_b ppb(nc+b, it-ics, I) cphiq(ias+b, m)
+      const int nc = h->ncore[a];
+      const int64_t ics = ics_v[a];
+      const int lo = std::max<int64_t>(rg.nm1c, ics + 1), hi = std::min<int64_t>(rg.nm2c, ics + nc);
+      if (hi >= lo) {
+        const int ncc = hi - lo + 1;
+        GemmCall c;  // A(I, b) = ppbc_ap[b + nv*(it-ics-1) + nv*nc*I]
+        c.mode = mode;
+        c.areal = true; c.ta = true;
+        c.M = md; c.N = nq; c.K = nv;
+        c.A = h->d_ppbc + h->ppbc_off[ap] + (int64_t)nv * (lo - ics - 1); c.lda = (int64_t)nv * nc;
+        c.bA = {nv, 0};
+        c.B = (const char*)cphiq + es * ias; c.ldb = ndima; c.bB = {0, sq};
+        c.C = zM + es * (ims + (int64_t)ngb * (lo - rg.nm1)); c.ldc = (int64_t)ngb * nmt; c.bC = {ngb, zsz};
+        c.batch = ncc * nbatch; c.div = ncc;
+        c.alpha = ph; c.conjC = conj_now;
+        ECALJ_TRY(gemm(h, c));
+      }
+    }
   }
   if (!is_m_basis) {  // zmel = zcousq^H zmel_M  (S/m_zmel.f90:559-560), then conj
     GemmCall e;
     e.mode = mode;
     e.ta = true; e.ca = true;
-    e.M = ngb; e.N = (int)((int64_t)nm * nq * nbatch); e.K = ngb;
+    e.M = ngb; e.N = (int)((int64_t)nmt * nq * nbatch); e.K = ngb;
     e.A = wide(h) ? (const void*)h->d_zcousq : (const void*)h->d_zcousq32; e.lda = ngb;
     e.B = zM; e.ldb = ngb;
     e.C = zmel; e.ldc = ngb;
@@ -761,6 +855,68 @@ ecalj_status ecalj_zmel_strided_dev(ecalj_handle h, const double* phase, int ndi
   if (cphim_stride < 0 || cphiq_stride < 0) return ECALJ_ERR_DIM;
   return zmel_impl(h, phase, ndima, nm, (const cd*)cphim, cphim_stride, nq, (const cd*)cphiq, cphiq_stride,
                    is_m_basis, zmelconjg, nbatch, zmel);
+}
+
+// Middle states ns1..ns2 over core states 1..nctot then valence bands
+// (S/m_zmel.f90:199-214): the reference's nm1c:nm2c and nm1v:nm2v.
+static void zmel_ranges(int ns1, int ns2, int nctot, int* nm1c, int* nm2c, int* nm1v, int* nm2v) {
+  if (ns1 > nctot) { *nm1c = 0; *nm2c = -1; *nm1v = ns1; *nm2v = ns2; }
+  else if (ns2 > nctot) { *nm1c = ns1; *nm2c = nctot; *nm1v = nctot + 1; *nm2v = ns2; }
+  else { *nm1c = ns1; *nm2c = ns2; *nm1v = 0; *nm2v = -1; }
+}
+
+ecalj_status ecalj_zmel_init_dev(ecalj_handle h, const double* phase, const int* iatomp, int ndima, int ns1,
+                                 int ns2, int nctot, const void* cphim, int nq, const void* cphiq, int is_m_basis,
+                                 int zmelconjg, int nbatch, void* zmel) {
+  if (!h || !cphiq || !zmel) return ECALJ_ERR_ARG;
+  if (ns1 < 1 || ns2 < ns1 - 1 || nctot < 0 || nbatch < 1) return ECALJ_ERR_DIM;
+  int nm1c, nm2c, nm1v, nm2v;
+  zmel_ranges(ns1, ns2, nctot, &nm1c, &nm2c, &nm1v, &nm2v);
+  const int nmv = nm2v >= nm1v ? nm2v - nm1v + 1 : 0;
+  if (nmv > 0 && !cphim) return ECALJ_ERR_ARG;
+  if (nm2c >= nm1c) {  // core states must be those of ecalj_set_ppb_full
+    int nct = 0;
+    for (int c : h->ncore) nct += c;
+    if (nctot != nct || !h->d_ppbc) return ECALJ_ERR_STATE;
+  }
+  std::vector<int> map;
+  ZmelRange rg;
+  if (iatomp) {
+    map.resize(h->natom);
+    for (int a = 0; a < h->natom; ++a) {
+      if (iatomp[a] < 1 || iatomp[a] > h->natom) return ECALJ_ERR_ARG;
+      map[a] = iatomp[a] - 1;
+    }
+    rg.iatomp = map.data();
+  }
+  rg.nmt = ns2 - ns1 + 1;
+  rg.voff = nmv > 0 ? nm1v - ns1 : 0;
+  rg.nm1c = nm1c;
+  rg.nm2c = nm2c;
+  rg.nm1 = ns1;
+  rg.nctot = nctot;
+  return zmel_impl(h, phase, ndima, nmv, (const cd*)cphim, (int64_t)ndima * nmv, nq, (const cd*)cphiq,
+                   (int64_t)ndima * nq, is_m_basis, zmelconjg, nbatch, zmel, rg);
+}
+
+ecalj_status ecalj_zmel_init(ecalj_handle h, const double* phase, const int* iatomp, int ndima, int ns1, int ns2,
+                             int nctot, const double* cphim, int nq, const double* cphiq, int is_m_basis,
+                             int zmelconjg, int nbatch, double* zmel) {
+  if (!h || !cphiq || !zmel || nbatch < 1 || ns2 < ns1 - 1 || ns1 < 1) return ECALJ_ERR_ARG;
+  int nm1c, nm2c, nm1v, nm2v;
+  zmel_ranges(ns1, ns2, nctot, &nm1c, &nm2c, &nm1v, &nm2v);
+  const int nmv = nm2v >= nm1v ? nm2v - nm1v + 1 : 0;
+  const int nmt = ns2 - ns1 + 1;
+  const size_t bm = sizeof(cd) * (size_t)ndima * nmv * nbatch, bq = sizeof(cd) * (size_t)ndima * nq * nbatch;
+  const int64_t nz = (int64_t)h->ngb_ppb * nmt * nq * nbatch;
+  DevBuf dm, dq;
+  DevC dz;
+  ECALJ_TRY(dm.put(h, nmv > 0 ? cphim : nullptr, bm));
+  ECALJ_TRY(dq.put(h, cphiq, bq));
+  ECALJ_TRY(dz.put(h, nullptr, nz));
+  ECALJ_TRY(ecalj_zmel_init_dev(h, phase, iatomp, ndima, ns1, ns2, nctot, dm.p, nq, dq.p, is_m_basis, zmelconjg,
+                                nbatch, dz.buf.p));
+  return dz.get(h, zmel, nz);
 }
 
 ecalj_status ecalj_zmel(ecalj_handle h, const double* phase, int ndima, int nm, const double* cphim, int nq,

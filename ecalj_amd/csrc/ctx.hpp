@@ -24,6 +24,10 @@ struct ecalj_ctx {
   int ndima_used = 0, ngb_ppb = 0;
   double* d_ppb = nullptr;        // original layout ppb(i1,i2,I) per atom
   double* d_ppb2 = nullptr;       // relaid: P2[(i1 + nv*I) + nv*mdim*i2]
+  // core blocks of ecalj_set_ppb_full: per atom (nv, ncore, mdim) = ppb(nc+b, c, I)
+  double* d_ppbc = nullptr;
+  std::vector<int64_t> ppbc_off;
+  std::vector<int> ncore;         // core functions (nlnmc) per atom; empty or zeros: none
   // Coulomb state
   int ngb = 0;
   double* d_vcousq = nullptr;     // sqrt(vcoud)

@@ -15,6 +15,7 @@ module ecalj_gpu
   public :: ecalj_set_tuning, ecalj_graph_begin, ecalj_graph_end, ecalj_graph_launch, ecalj_graph_destroy
   public :: ecalj_dpsion_real, ecalj_dpsion_real_dev, ecalj_wc_build_real, ecalj_wc_build_real_dev
   public :: ecalj_zhgv, ecalj_zhgv_dev
+  public :: ecalj_set_ppb_full, ecalj_zmel_init, ecalj_zmel_init_dev
   public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
   public :: ecalj_tetwt, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
   public :: ecalj_wv_write_dev, ecalj_wv_read_dev
@@ -171,6 +172,30 @@ module ecalj_gpu
       type(c_ptr), value :: h, a
       integer(c_int), value :: n, lda, nbatch
       integer(c_int64_t), value :: stride
+    end function
+    ! set_ppb from the reference's ppb(nlnmx,nlnmx,mdimx,natom) (S/m_zmel.f90:38,110-145)
+    integer(c_int) function ecalj_set_ppb_full(h, natom, nlnmx, mdimx, nlnmc, nlnmv, nblocha, ppb) &
+        bind(C, name="ecalj_set_ppb_full")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: natom, nlnmx, mdimx
+      integer(c_int), intent(in) :: nlnmc(*), nlnmv(*), nblocha(*)
+      real(c_double), intent(in) :: ppb(*)
+    end function
+    ! get_zmel_init_gemm, MT part with core states and iatomp (S/m_zmel.f90:146-147,323-391)
+    integer(c_int) function ecalj_zmel_init(h, phase, iatomp, ndima, ns1, ns2, nctot, cphim, nq, cphiq, &
+        is_m_basis, zmelconjg, nbatch, zmel) bind(C, name="ecalj_zmel_init")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h, phase, iatomp
+      integer(c_int), value :: ndima, ns1, ns2, nctot, nq, is_m_basis, zmelconjg, nbatch
+      complex(c_double_complex), intent(in) :: cphim(*), cphiq(*)
+      complex(c_double_complex), intent(out) :: zmel(*)
+    end function
+    integer(c_int) function ecalj_zmel_init_dev(h, phase, iatomp, ndima, ns1, ns2, nctot, cphim, nq, cphiq, &
+        is_m_basis, zmelconjg, nbatch, zmel) bind(C, name="ecalj_zmel_init_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, phase, iatomp, cphim, cphiq, zmel
+      integer(c_int), value :: ndima, ns1, ns2, nctot, nq, is_m_basis, zmelconjg, nbatch
     end function
     ! zhgv_h / zhgv_d (S/m_lapack.f90:64-104,203-229): a z = lambda b z, eigenpairs il..iu
     integer(c_int) function ecalj_zhgv(h, a, b, n, evl, il, iu, lda, ldb) bind(C, name="ecalj_zhgv")

@@ -183,6 +183,34 @@ ecalj_status ecalj_zmel_strided_dev(ecalj_handle h, const double* phase, int ndi
                                     const void* cphim, int64_t cphim_stride, int nq,
                                     const void* cphiq, int64_t cphiq_stride, int is_m_basis,
                                     int zmelconjg, int nbatch, void* zmel);
+/* set_ppb (S/m_zmel.f90:110-145) from the reference's own array
+ * ppb(nlnmx, nlnmx, mdimx, natom): atom a's functions are nlnmc[a] core
+ * then nlnmv[a] valence, with nblocha[a] product-basis functions.  Keeps
+ * the valence blocks (as ecalj_set_ppb) and the core-valence blocks
+ * ppb(nc+1:nc+nv, 1:nc, :, a) for the CoreValence rows of zmel. */
+ecalj_status ecalj_set_ppb_full(ecalj_handle h, int natom, int nlnmx, int mdimx, const int* nlnmc,
+                                const int* nlnmv, const int* nblocha, const double* ppb);
+/* get_zmel_init_gemm's muffin-tin part over a middle-state range with core
+ * states (S/m_zmel.f90:146-147, 199-214, 323-391, 559-560, 590-594):
+ * middle states ns1..ns2 (1-based) run over the nctot core states first
+ * (atom by atom, nlnmc of ecalj_set_ppb_full each) and then the valence
+ * bands; cphim holds only the valence bands of the range (nm1v..nm2v of the
+ * reference, ndima x nmv), cphiq the end states (ndima x nq).  Core rows:
+ * zmel(ims+I, it, m) = phase(ia) sum_b ppb(nc+b, it-ics, I) cphiq(ias+b, m).
+ * iatomp (natom, 1-based, NULL = identity; miat(:, invr) of the reference)
+ * maps atom ia to the atom whose product basis, ppb block and row offset it
+ * uses; ia's own eigenvector rows and core states stay.  zmel: ngb x
+ * (ns2-ns1+1) x nq (x nbatch; cphim and cphiq advance by their sizes).
+ * ECALJ_ERR_STATE when the range holds core states but nctot differs from
+ * the core count of ecalj_set_ppb_full. */
+ecalj_status ecalj_zmel_init(ecalj_handle h, const double* phase, const int* iatomp, int ndima,
+                             int ns1, int ns2, int nctot, const double* cphim, int nq,
+                             const double* cphiq, int is_m_basis, int zmelconjg, int nbatch,
+                             double* zmel);
+ecalj_status ecalj_zmel_init_dev(ecalj_handle h, const double* phase, const int* iatomp, int ndima,
+                                 int ns1, int ns2, int nctot, const void* cphim, int nq,
+                                 const void* cphiq, int is_m_basis, int zmelconjg, int nbatch,
+                                 void* zmel);
 
 /* ---- zmel: interstitial plane-wave rows ---------------------------------
  * m_read_ppovl tables of the current q (S/m_zmel.f90:419): ggg (nggg complex,

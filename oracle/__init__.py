@@ -679,3 +679,28 @@ def real_mesh(eq_ry, ebz_ry, esmr, frhis, deltaw=0.02, ffac=10.0, ef=0.0):
     if nw < 0:
         raise ValueError("real_mesh: histogram mesh does not reach wemax")
     return out[:nw + 1].copy()
+
+
+def zmel_init(nlnmc, nlnmv, nblocha, ppb_full, ns1, ns2, nctot, cphim, cphiq, phase=None, iatomp=None, m2e=None,
+              conjg=False):
+    """get_zmel_init_gemm's MT part with core states and iatomp (orc_zmel_init).
+    ppb_full: (nlnmx, nlnmx, mdimx, natom) real; cphim: ndima x (valence bands in ns1..ns2)."""
+    nlnmc, nlnmv, nblocha = _ci(nlnmc), _ci(nlnmv), _ci(nblocha)
+    natom = len(nlnmc)
+    ppb_full = np.asfortranarray(ppb_full, np.float64)
+    nlnmx, _, mdimx, _ = ppb_full.shape
+    phase = np.ascontiguousarray(np.ones(natom, np.complex128) if phase is None else np.asarray(phase, np.complex128))
+    cphim = _cz(cphim)
+    cphiq = _cz(cphiq)
+    ndima, nq = cphiq.shape
+    ngb = int(nblocha.sum())
+    nbb = m2e.shape[1] if m2e is not None else ngb
+    out = np.zeros((nbb, ns2 - ns1 + 1, nq), np.complex128, order="F")
+    ia = _ci(iatomp) if iatomp is not None else None
+    rc = lib().orc_zmel_init(natom, nlnmx, mdimx, _ip_(nlnmc), _ip_(nlnmv), _ip_(nblocha),
+                             _ip_(ia) if ia is not None else None, _p(phase), _p(ppb_full), ndima, ns1, ns2, nctot,
+                             _p(cphim) if cphim.size else _p(np.zeros(1, np.complex128)), nq, _p(cphiq),
+                             _p(_cz(m2e)) if m2e is not None else None, nbb, int(bool(conjg)), _p(out))
+    if rc != 0:
+        raise ValueError(f"orc_zmel_init rc={rc}")
+    return out

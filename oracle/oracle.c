@@ -932,3 +932,80 @@ void orc_sigma_herm(int ntq, const double complex* sex2, const double complex* s
       se[ij] = sex2[ij] + sexcore2[ij] + 0.5 * (sec[ij] + conj(sec[ji])) - vxcmat[ij];
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* get_zmel_init_gemm's muffin-tin part with core states and the rotated-atom
+ * map (S/m_zmel.f90:199-214, 295-305, 323-391), then M->E and conj
+ * (:559-560, 590-594).  ppb is the reference's array ppb(nlnmx, nlnmx,
+ * mdimx, natom): for atom a, functions 1..nlnmc(a) are core, then
+ * nlnmv(a) valence.  Atom ia uses its own eigenvector rows (iasx: the
+ * valence counts before it), its own core states (icsx: the core counts
+ * before it), and the product basis, ppb block and row offset of atom
+ * iap = iatomp(ia) (imdim(iap): the nblocha before iap).  Middle states
+ * ns1..ns2 run over core states 1..nctot, then valence bands (band nctot+j
+ * is column j - (nm1v - nctot) of cphim, which holds bands nm1v..nm2v).
+ *   valence (:349-363): zmelt(ims+i, it, m) = phase(ia) sum_{b,c}
+ *       conj(cphim(ias+b, it)) ppb(nc+c, nc+b, i, iap) cphiq(ias+c, m)
+ *   core (:374-390):    zmelt(ims+i, it, m) = phase(ia) sum_b
+ *       ppb(nc+b, it-ics, i, iap) cphiq(ias+b, m), it in
+ *       [max(nm1c, ics+1), min(nm2c, ics+ncorec)], ncorec = nlnmc(ia) if nctot > 0.
+ * zmel out: (m2e ? nbb : ngb) x (ns2-ns1+1) x nq.                          */
+int orc_zmel_init(int natom, int nlnmx, int mdimx, const int* nlnmc, const int* nlnmv, const int* nblocha,
+                  const int* iatomp, const double complex* phase, const double* ppb, int ndima, int ns1, int ns2,
+                  int nctot, const double complex* cphim, int nq, const double complex* cphiq,
+                  const double complex* m2e, int nbb, int conjg, double complex* zmel) {
+  int ngb = 0;
+  for (int a = 0; a < natom; ++a) ngb += nblocha[a];
+  const int nm1 = ns1, nm2 = ns2, nmt = nm2 - nm1 + 1;
+  int nm1c, nm2c, nm1v, nm2v; /* S/m_zmel.f90:199-214 */
+  if (nm1 > nctot) { nm1c = 0; nm2c = -1; nm1v = nm1; nm2v = nm2; }
+  else if (nm2 > nctot) { nm1c = nm1; nm2c = nctot; nm1v = nctot + 1; nm2v = nm2; }
+  else { nm1c = nm1; nm2c = nm2; nm1v = 0; nm2v = -1; }
+  const int nmv = nm2v >= nm1v ? nm2v - nm1v + 1 : 0;
+  double complex* zmelt = (double complex*)calloc((size_t)ngb * nmt * (nq > 0 ? nq : 1), sizeof(double complex));
+  const long pstride1 = nlnmx, pstride2 = (long)nlnmx * nlnmx, pstride3 = (long)nlnmx * nlnmx * mdimx;
+  int ias = 1, ics = 0;
+  for (int ia = 0; ia < natom; ++ia) {
+    const int iap = iatomp ? iatomp[ia] - 1 : ia;
+    if (iap < 0 || iap >= natom) { free(zmelt); return 2; }
+    const int nc = nlnmc[ia], nv = nlnmv[ia], mdim = nblocha[iap];
+    int ims = 0;
+    for (int a = 0; a < iap; ++a) ims += nblocha[a];
+    const double* P = ppb + pstride3 * iap;
+    const int ncorec = nctot > 0 ? nc : 0;
+    for (int i = 0; i < mdim; ++i)
+      for (int m = 0; m < nq; ++m) {
+        /* valence middle states */
+        for (int j = 0; j < nmv; ++j) {
+          double complex s = 0.0;
+          for (int b = 0; b < nv; ++b) {
+            double complex t = 0.0;
+            for (int c = 0; c < nv; ++c)
+              t += P[(nc + c) + pstride1 * (nc + b) + pstride2 * i] * cphiq[(ias - 1 + c) + (long)ndima * m];
+            s += conj(cphim[(ias - 1 + b) + (long)ndima * j]) * t;
+          }
+          const int col = (nm1v + j) - nm1;
+          zmelt[ims + i + (long)ngb * (col + (long)nmt * m)] = phase[ia] * s;
+        }
+        /* core middle states of this atom */
+        const int lo = nm1c > ics + 1 ? nm1c : ics + 1, hi = nm2c < ics + ncorec ? nm2c : ics + ncorec;
+        for (int it = lo; it <= hi; ++it) {
+          double complex s = 0.0;
+          for (int b = 0; b < nv; ++b)
+            s += P[(nc + b) + pstride1 * (it - ics - 1) + pstride2 * i] * cphiq[(ias - 1 + b) + (long)ndima * m];
+          zmelt[ims + i + (long)ngb * ((it - nm1) + (long)nmt * m)] = phase[ia] * s;
+        }
+      }
+    ias += nv;
+    ics += nc;
+  }
+  long nrow = m2e ? nbb : ngb;
+  if (m2e)
+    orc_zgemm('C', 'N', nbb, nmt * nq, ngb, 1.0, m2e, ngb, zmelt, ngb, 0.0, zmel, nbb);
+  else
+    memcpy(zmel, zmelt, sizeof(double complex) * (size_t)ngb * nmt * nq);
+  if (conjg)
+    for (long e = 0; e < nrow * nmt * nq; ++e) zmel[e] = conj(zmel[e]);
+  free(zmelt);
+  return 0;
+}

@@ -113,6 +113,17 @@ int orc_zmel(int natom, const int* nv, const int* mdim, const double complex* ph
              const double complex* cphiq, const double complex* m2e, int nbb, int conjg,
              double complex* zmel);
 
+/* get_zmel_init_gemm's MT part with core states and the rotated-atom map
+ * iatomp (1-based; NULL = identity) on the reference's ppb(nlnmx, nlnmx,
+ * mdimx, natom) (S/m_zmel.f90:199-214, 323-391, 559-560, 590-594).  Middle
+ * states ns1..ns2 (core 1..nctot first); cphim holds the valence bands of
+ * that range.  Pinned by the valence formula on core-extended vectors and by
+ * the row permutation that iatomp stands for (tests/test_zmel_core_oracle.py). */
+int orc_zmel_init(int natom, int nlnmx, int mdimx, const int* nlnmc, const int* nlnmv, const int* nblocha,
+                  const int* iatomp, const double complex* phase, const double* ppb, int ndima, int ns1, int ns2,
+                  int nctot, const double complex* cphim, int nq, const double complex* cphiq,
+                  const double complex* m2e, int nbb, int conjg, double complex* zmel);
+
 /* x0gemm (S/x0gemm.f90:2-97) with its module inputs explicit.
  * rcxq: npr x npr_col x nslice, slice s <-> Fortran index s + (1-npm)*nwhis
  *   (npm=1: slices 0..nwhis; npm=2: -nwhis..nwhis).

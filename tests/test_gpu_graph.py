@@ -95,8 +95,9 @@ def test_graph_refused_after_scratch_moves():
         say("graph: growing the handle's scratch with a larger eager zmel")
         rng = np.random.default_rng(1)
         nb = qp.cfg.nb
-        big = rng.standard_normal((nb, 600)) + 1j * rng.standard_normal((nb, 600))
-        g.h.zmel(big[:, :5], big, zmelconjg=True)
+        # both sides wide, so the contraction's intermediate (ndima x nI x 300) outgrows the step's scratch
+        big = rng.standard_normal((nb, 300)) + 1j * rng.standard_normal((nb, 300))
+        g.h.zmel(big, big, zmelconjg=True)
         with pytest.raises(ecalj_amd.EcaljError) as e:
             g1.launch()
         assert e.value.status == 6  # ECALJ_ERR_STATE
