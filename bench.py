@@ -189,6 +189,12 @@ def run_form(qp, precision, steps, warmup):
     return info
 
 
+def under_profiler() -> bool:
+    """rocprofv3 exports its options as ROCPROF_* to the program it runs; the
+    library refuses HIP-graph capture there, so the C4 legs run eagerly."""
+    return any(k in os.environ for k in ("ROCPROF_OUTPUT_PATH", "ROCPROF_KERNEL_TRACE", "ROCPROF_STATS"))
+
+
 def bench_c4(mixed, nq=16, log_=None, graph=True):
     """Batched uneven workload, one GPU's share: nq q-points of 27 k-points
     with per-k band windows and per-q nI; FP64 form eager, mixed form as a
@@ -197,6 +203,7 @@ def bench_c4(mixed, nq=16, log_=None, graph=True):
     from ecalj_amd import synth
     from ecalj_amd.batched import UnevenQPointGPU, flops_uneven
 
+    graph = graph and not under_profiler()
     rows = []
     t64 = tmx = tmx_eager = fl = 0.0
     err = 0.0

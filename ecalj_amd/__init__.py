@@ -82,6 +82,8 @@ _SIGS = {
     "ecalj_wv_d_read": [C.c_char_p, _p],
     "ecalj_wv_write_dev": [_p, C.c_char_p, _i, _i, _i, _p],
     "ecalj_wv_read_dev": [_p, C.c_char_p, _i, _i, _i, _p],
+    "ecalj_wv4_write_dev": [_p, C.c_char_p, _i, _i, _i, _p],
+    "ecalj_wv4_read_dev": [_p, C.c_char_p, _i, _i, _i, _p],
     "ecalj_set_ppovl": [_p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p],
     "ecalj_zmel_ipw": [_p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _i, _p, _p, _i, _p, _i, _i],
     "ecalj_zmel_ipw_dev": [_p, _i, _p, _p, _p, _i, _i, _p, _p, _i, _i, _p, _p, _i, _p, _i, _i],
@@ -673,6 +675,16 @@ class Handle:
     def wv_read_dev(self, path, rec0, w, ld):
         nrec = w.nbytes // (16 * ld * ld)
         _chk(lib().ecalj_wv_read_dev(self._h, str(path).encode(), rec0, nrec, ld, _t_ptr(w)), "wv_read_dev")
+
+    def wv4_write_dev(self, path, rec0, w, ld):
+        """Records rec0.. of a complex(4) __WVI/__WVR file (the reference's
+        mixed-precision build) from device W (ld x ld x nrec, complex128)."""
+        nrec = w.nbytes // (16 * ld * ld)
+        _chk(lib().ecalj_wv4_write_dev(self._h, str(path).encode(), rec0, nrec, ld, _t_ptr(w)), "wv4_write_dev")
+
+    def wv4_read_dev(self, path, rec0, w, ld):
+        nrec = w.nbytes // (16 * ld * ld)
+        _chk(lib().ecalj_wv4_read_dev(self._h, str(path).encode(), rec0, nrec, ld, _t_ptr(w)), "wv4_read_dev")
 
     def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10):
         """gettetwt (S/m_tetwt.f90:36-256), npm=1: pair lists and histogram weights.

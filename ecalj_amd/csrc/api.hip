@@ -4,6 +4,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -328,9 +329,20 @@ struct ecalj_graph_s {
   hipEvent_t done = nullptr;   // recorded after each launch; destroy waits for it
 };
 
+// True when the process runs under rocprofv3 (it exports its options as
+// ROCPROF_* variables to the program it starts).  A kernel-trace run of HIP
+// graph replays here ended in a segmentation fault inside hipGraphLaunch, in
+// a host memcpy issued by the tracer's library at a page-aligned address; the
+// graphs' own host memory (pinned staging, kept until destroy) was not it.
+// Graph capture is refused under the profiler: profile the eager calls.
+static bool under_profiler() {
+  return getenv("ROCPROF_OUTPUT_PATH") || getenv("ROCPROF_KERNEL_TRACE") || getenv("ROCPROF_STATS");
+}
+
 ecalj_status ecalj_graph_begin(ecalj_handle h) {
   if (!h) return ECALJ_ERR_ARG;
   if (h->capturing) return ECALJ_ERR_STATE;
+  if (under_profiler()) return ECALJ_ERR_STATE;
   ECALJ_HIP(hipSetDevice(h->device));
   ECALJ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));  // pinned staging is allocated while capturing
   h->capturing = true;

@@ -8,7 +8,9 @@
 //                complex(8), record rec at byte (rec-1)*mrecl with mrecl =
 //                16*nblochpmx**2 (MPI-IO, S/m_mpiio.f90:12-74; mrecl
 //                S/rdpp.f90:43; records iw (WVI) and iw-nw_i+1 (WVR),
-//                S/m_llw.f90:177,373)
+//                S/m_llw.f90:177,373); in the reference's mixed-precision
+//                build (__MP, S/m_kind.f90:3-6) the records are complex(4)
+//                with mrecl = 8*nblochpmx**2 (kp = 4 in S/rdpp.f90:43)
 //   __WV.d       one formatted line: nprecx mrecl nblochpmx nw+1 niw nq nw_i
 //                (S/main_hrcxq.f90:134-136)
 //   SEC2U        sequential unformatted: nspin nq ntq nqbz nqibz n1 n2 n3; per
@@ -188,6 +190,63 @@ extern "C" ecalj_status ecalj_wv_read_dev(ecalj_handle h, const char* path, int 
       break;
     }
     if (hipMemcpyAsync((char*)w + rl * r, pin, rl, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      st = ECALJ_ERR_HIP;
+  }
+  (void)hipHostFree(pin);
+  return st;
+}
+
+// The same records in the complex(4) form of the mixed-precision build
+// (record length 8*ld^2): device W stays complex128; each element is rounded
+// to complex(4) on the way out (round to nearest, as cmplx(w, kind=4)) and
+// widened exactly on the way in.  The conversion runs on the pinned staging.
+extern "C" ecalj_status ecalj_wv4_write_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld,
+                                            const void* w) {
+  if (!h || !path || !w || rec0 < 1 || nrec < 0 || ld < 1) return ECALJ_ERR_ARG;
+  if (h->capturing) return ECALJ_ERR_STATE;
+  const size_t ne = (size_t)ld * ld, rl8 = 16 * ne, rl4 = 8 * ne;
+  FILE* f = fopen(path, "r+b");
+  if (!f) f = fopen(path, "w+b");
+  if (!f) return ECALJ_ERR_ARG;
+  void* pin = nullptr;
+  ecalj_status st = ECALJ_OK;
+  if (hipHostMalloc(&pin, rl8) != hipSuccess) st = ECALJ_ERR_NOMEM;
+  std::vector<float> out(2 * ne);
+  for (int r = 0; r < nrec && st == ECALJ_OK; ++r) {
+    if (hipMemcpyAsync(pin, (const char*)w + rl8 * r, rl8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      st = ECALJ_ERR_HIP;
+      break;
+    }
+    const double* d = static_cast<const double*>(pin);
+    for (size_t i = 0; i < 2 * ne; ++i) out[i] = (float)d[i];
+    if (fseeko(f, (off_t)rl4 * (rec0 - 1 + r), SEEK_SET) != 0 || fwrite(out.data(), 1, rl4, f) != rl4)
+      st = ECALJ_ERR_ARG;
+  }
+  if (pin) (void)hipHostFree(pin);
+  fclose(f);
+  return st;
+}
+
+extern "C" ecalj_status ecalj_wv4_read_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld, void* w) {
+  if (!h || !path || !w || rec0 < 1 || nrec < 0 || ld < 1) return ECALJ_ERR_ARG;
+  if (h->capturing) return ECALJ_ERR_STATE;
+  const size_t ne = (size_t)ld * ld, rl8 = 16 * ne, rl4 = 8 * ne;
+  File fl(path, "rb");
+  if (!fl.f) return ECALJ_ERR_ARG;
+  void* pin = nullptr;
+  if (hipHostMalloc(&pin, rl8) != hipSuccess) return ECALJ_ERR_NOMEM;
+  std::vector<float> in(2 * ne);
+  ecalj_status st = ECALJ_OK;
+  for (int r = 0; r < nrec && st == ECALJ_OK; ++r) {
+    if (fseeko(fl.f, (off_t)rl4 * (rec0 - 1 + r), SEEK_SET) != 0 || fread(in.data(), 1, rl4, fl.f) != rl4) {
+      st = ECALJ_ERR_DIM;  // short file
+      break;
+    }
+    double* d = static_cast<double*>(pin);
+    for (size_t i = 0; i < 2 * ne; ++i) d[i] = in[i];
+    if (hipMemcpyAsync((char*)w + rl8 * r, pin, rl8, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
       st = ECALJ_ERR_HIP;
   }

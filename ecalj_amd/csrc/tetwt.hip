@@ -10,12 +10,13 @@
 //          windows of degenerate bands; tw_count_kernel / tw_lists_kernel
 //          build the per-k pair lists (ib-major, rsvwwk00_4's order), the
 //          bin windows (hisrange) and the whw pointers (block scans).
-//   job 1  tw_job1_kernel: one thread per (tetrahedron, pair) cuts the
-//          tetrahedron at E_F for the occupied band, each piece at E_F for
-//          the unoccupied band (Rath & Freeman), integrates the distribution
-//          of the excitation energy over every bin, and adds it to the pair's
-//          histogram at the four corners (FP64 atomics); tw_sym1_kernel
-//          averages degenerate bands.
+//   job 1  tw_job1_kernel: one thread per (tetrahedron, pair) integrates
+//          the excitation energy over every bin of the region where the
+//          occupied band is below and the unoccupied band above E_F
+//          (tetcut.hpp: two plane clips into at most nine tetrahedra, each
+//          binned through its cumulative volume), and adds the bins to the
+//          pair's histogram at the four corners (FP64 atomics);
+//          tw_sym1_kernel averages degenerate bands.
 // The host keeps what the reference computes once per call outside the
 // loops: per-tetrahedron band counts and the degenerate groups (chkdgn).
 #include <algorithm>
@@ -25,6 +26,7 @@
 
 #include "ctx.hpp"
 #include "ecalj_gpu.h"
+#include "tetcut.hpp"
 
 namespace ecalj {
 namespace {
@@ -68,203 +70,19 @@ __device__ inline void atomic_max_d(double* a, double v) {
   }
 }
 
-// ---------------------------------------------------------------- geometry
-__device__ inline void sort4(const double* e, int* ord) {  // stable insertion sort, ascending (sortea)
-  for (int i = 0; i < 4; ++i) ord[i] = i;
-  for (int ix = 1; ix < 4; ++ix)
-    for (int i = ix; i >= 1; --i) {
-      if (e[ord[i - 1]] > e[ord[i]]) {
-        const int t = ord[i - 1];
-        ord[i - 1] = ord[i];
-        ord[i] = t;
-        continue;
-      }
-      break;
-    }
-}
-
-__device__ inline double det3(const double a[3][3]) {  // a[col][row]
-  return a[0][0] * a[1][1] * a[2][2] - a[0][0] * a[1][2] * a[2][1] - a[0][1] * a[1][0] * a[2][2] +
-         a[0][1] * a[1][2] * a[2][0] + a[0][2] * a[1][0] * a[2][1] - a[0][2] * a[1][1] * a[2][0];
-}
-
-// integtetn (S/m_tetrakbt.f90:441-466); e ascending.  false on ee > e4.
-__device__ inline bool integtetn(const double* e, double ee, double& out) {
-  const double e1 = e[0] - 3e-8, e2 = e[1] - 2e-8, e3 = e[2] - 1e-8, e4 = e[3];
-  const double V1 = ee - e1, V2 = ee - e2, V4 = ee - e4;
-  if (ee < e1) {
-    out = 0.0;
-  } else if (e1 <= ee && ee < e2) {
-    out = V1 * V1 * V1 / ((e4 - e1) * (e3 - e1) * (e2 - e1));
-  } else if (e2 <= ee && ee < e3) {
-    const double a = V1 / pow((e4 - e1) * (e3 - e1) * (e2 - e1), 1.0 / 3.0);
-    const double b = V2 / pow(-(e4 - e2) * (e3 - e2) * (e1 - e2), 1.0 / 3.0);
-    out = (a - b) * (a * a + a * b + b * b);
-  } else if (e3 <= ee && ee < e4) {
-    out = 1.0 - V4 * V4 * V4 / ((e1 - e4) * (e2 - e4) * (e3 - e4));
-  } else if (ee == e4) {
-    out = 1.0;
-  } else {
-    return false;
-  }
-  return true;
-}
-
-// The four corners' histogram windows of this pair; weights go straight to whw.
-struct Sink {
+// The four corners' histogram windows of one pair: a bin of the
+// tetrahedron's histogram goes to every corner whose window holds it.
+struct CornerBins {
   int ihw[4], nhw[4], jhw[4];
   double scale;  // pi / (4 vol_BZ) * 4 * (1/4)
   double* whw;
-  int* err;
-  __device__ void add(int ihis, double w) const {
+  __device__ void operator()(int ihis, double w) const {
     const double v = w * scale;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       if (ihis >= ihw[c] && ihis < ihw[c] + nhw[c]) atomicAdd(whw + (jhw[c] - 1) + (ihis - ihw[c]), v);
   }
 };
-
-// intttvc6 with matrix_linear = F (S/tetwt5.f90:714-787)
-__device__ void intttvc6(const double v[4], double voltet, const double* frhis, int nwhis, const Sink& s) {
-  double mv[4], WW[4];
-  int ord[4];
-  for (int i = 0; i < 4; ++i) mv[i] = -v[i];
-  sort4(mv, ord);
-  for (int i = 0; i < 4; ++i) WW[i] = -v[ord[i]];
-  int inihis = -999, iedhis = -999, ix = 1;
-  for (int ihis = 1; ihis <= nwhis; ++ihis) {
-    if (ix == 1 && WW[0] < frhis[ihis]) {
-      inihis = ihis;
-      ix = 4;
-    }
-    if (ix == 4 && WW[3] < frhis[ihis]) {
-      iedhis = ihis;
-      break;
-    }
-  }
-  if (iedhis == -999 || inihis == -999) {
-    *s.err = 3;
-    return;
-  }
-  double norm;
-  if (!integtetn(WW, WW[3], norm)) {
-    *s.err = 4;
-    return;
-  }
-  const double pvn = voltet / norm;
-  double intega = 0.0, integb;
-  for (int ihis = inihis; ihis <= iedhis; ++ihis) {
-    if (frhis[ihis] > WW[3]) {
-      integb = norm;
-    } else if (!integtetn(WW, frhis[ihis], integb)) {
-      *s.err = 4;
-      return;
-    }
-    s.add(ihis, pvn * (integb - intega));
-    intega = integb;
-  }
-}
-
-__device__ inline void inttetrac6(const double kk[8][3], const double xx[8], int t0, int t1, int t2, int t3,
-                                  const double* frhis, int nwhis, const Sink& s) {
-  const int it[4] = {t0 - 1, t1 - 1, t2 - 1, t3 - 1};
-  double xin[4], am[3][3];
-  for (int i = 0; i < 4; ++i) xin[i] = xx[it[i]];
-  for (int i = 0; i < 3; ++i)
-    for (int d = 0; d < 3; ++d) am[i][d] = kk[it[i]][d] - kk[it[3]][d];
-  intttvc6(xin, fabs(det3(am) / 6.0), frhis, nwhis, s);
-}
-
-__device__ inline void midk(double kk[8][3], const double* ee, double* xx, int i, int j, int out) {
-  const double ratio = ee[i - 1] / (ee[i - 1] - ee[j - 1]);
-  xx[out - 1] = xx[i - 1] + ratio * (xx[j - 1] - xx[i - 1]);
-  for (int d = 0; d < 3; ++d) kk[out - 1][d] = kk[i - 1][d] + ratio * (kk[j - 1][d] - kk[i - 1][d]);
-}
-
-__device__ inline void midk3(double kk[8][3], const double* ee, double* xx, double* yy, int i, int j, int out) {
-  const double ratio = ee[i - 1] / (ee[i - 1] - ee[j - 1]);
-  xx[out - 1] = xx[i - 1] + ratio * (xx[j - 1] - xx[i - 1]);
-  yy[out - 1] = yy[i - 1] + ratio * (yy[j - 1] - yy[i - 1]);
-  for (int d = 0; d < 3; ++d) kk[out - 1][d] = kk[i - 1][d] + ratio * (kk[j - 1][d] - kk[i - 1][d]);
-}
-
-// inttetra6 (S/tetwt5.f90:601-672): the unoccupied cut of one occupied piece
-__device__ void inttetra6(const double kk_[8][3], const double xx_[8], const double ebf[8], int t0, int t1, int t2,
-                          int t3, const double* frhis, int nwhis, const Sink& s) {
-  const int it[4] = {t0 - 1, t1 - 1, t2 - 1, t3 - 1};
-  double ebfin[4], kk[8][3], xx[8], ee[4];
-  int ord[4];
-  for (int i = 0; i < 4; ++i) ebfin[i] = ebf[it[i]];
-  sort4(ebfin, ord);
-  for (int i = 0; i < 4; ++i) {
-    ee[i] = ebfin[ord[i]];
-    xx[i] = xx_[it[ord[i]]];
-    for (int d = 0; d < 3; ++d) kk[i][d] = kk_[it[ord[i]]][d];
-  }
-  if (0.0 >= ee[3]) return;
-  if (ee[3] > 0.0 && 0.0 >= ee[2]) {
-    midk(kk, ee, xx, 4, 2, 5);
-    midk(kk, ee, xx, 4, 1, 6);
-    midk(kk, ee, xx, 4, 3, 7);
-    inttetrac6(kk, xx, 4, 5, 6, 7, frhis, nwhis, s);
-  } else if (ee[2] > 0.0 && 0.0 >= ee[1]) {
-    midk(kk, ee, xx, 4, 2, 5);
-    midk(kk, ee, xx, 4, 1, 6);
-    midk(kk, ee, xx, 3, 1, 7);
-    midk(kk, ee, xx, 3, 2, 8);
-    inttetrac6(kk, xx, 4, 3, 5, 6, frhis, nwhis, s);
-    inttetrac6(kk, xx, 3, 6, 7, 5, frhis, nwhis, s);
-    inttetrac6(kk, xx, 3, 5, 7, 8, frhis, nwhis, s);
-  } else if (ee[1] > 0.0 && 0.0 >= ee[0]) {
-    midk(kk, ee, xx, 1, 4, 5);
-    midk(kk, ee, xx, 1, 2, 6);
-    midk(kk, ee, xx, 1, 3, 7);
-    inttetrac6(kk, xx, 3, 4, 7, 2, frhis, nwhis, s);
-    inttetrac6(kk, xx, 4, 5, 6, 7, frhis, nwhis, s);
-    inttetrac6(kk, xx, 4, 2, 6, 7, frhis, nwhis, s);
-  } else {
-    inttetrac6(kk, xx, 1, 2, 3, 4, frhis, nwhis, s);
-  }
-}
-
-// lindtet6 (S/tetwt5.f90:514-600): the occupied cut (ea < E_F)
-__device__ void lindtet6(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
-                         double ef, const double* frhis, int nwhis, const Sink& s) {
-  int ord[4];
-  sort4(ea, ord);
-  const int r[4] = {ord[3], ord[2], ord[1], ord[0]};
-  double kk[8][3], xx[8], ee[4], ebf[8];
-  for (int i = 0; i < 4; ++i) {
-    for (int d = 0; d < 3; ++d) kk[i][d] = kvec[r[i]][d];
-    ee[i] = ea[r[i]] - ef;
-    xx[i] = x[r[i]];
-    ebf[i] = eb[r[i]] - ef;
-  }
-  if (0.0 <= ee[3]) return;
-  if (ee[3] < 0.0 && 0.0 <= ee[2]) {
-    midk3(kk, ee, xx, ebf, 4, 2, 5);
-    midk3(kk, ee, xx, ebf, 4, 1, 6);
-    midk3(kk, ee, xx, ebf, 4, 3, 7);
-    inttetra6(kk, xx, ebf, 4, 5, 6, 7, frhis, nwhis, s);
-  } else if (ee[2] < 0.0 && 0.0 <= ee[1]) {
-    midk3(kk, ee, xx, ebf, 4, 2, 5);
-    midk3(kk, ee, xx, ebf, 4, 1, 6);
-    midk3(kk, ee, xx, ebf, 3, 1, 7);
-    midk3(kk, ee, xx, ebf, 3, 2, 8);
-    inttetra6(kk, xx, ebf, 4, 3, 5, 6, frhis, nwhis, s);
-    inttetra6(kk, xx, ebf, 3, 6, 7, 5, frhis, nwhis, s);
-    inttetra6(kk, xx, ebf, 3, 5, 7, 8, frhis, nwhis, s);
-  } else if (ee[1] < 0.0 && 0.0 <= ee[0]) {
-    midk3(kk, ee, xx, ebf, 1, 4, 5);
-    midk3(kk, ee, xx, ebf, 1, 2, 6);
-    midk3(kk, ee, xx, ebf, 1, 3, 7);
-    inttetra6(kk, xx, ebf, 3, 4, 7, 2, frhis, nwhis, s);
-    inttetra6(kk, xx, ebf, 4, 5, 6, 7, frhis, nwhis, s);
-    inttetra6(kk, xx, ebf, 4, 2, 6, 7, frhis, nwhis, s);
-  } else {
-    inttetra6(kk, xx, ebf, 1, 2, 3, 4, frhis, nwhis, s);
-  }
-}
 
 // ---------------------------------------------------------------- kernels
 struct TetRow {
@@ -407,24 +225,14 @@ __global__ __launch_bounds__(1024) void tw_lists_kernel(const unsigned char* __r
       jb = q % nband;
       const size_t e = idx3(ib, jb, k, nband);
       f = iwgt[e];
-      if (f) {  // hisrange (S/tetwt5.f90:487-513)
-        const double lo = demin[e], hi = demax[e];
-        ih = -9999;
-        for (int i = 1; i <= nwhis; ++i)
-          if (lo < frhis[i]) {
-            ih = i;
-            break;
-          }
-        nh = -9999;
-        if (ih > 0)
-          for (int i = ih; i <= nwhis; ++i)
-            if (hi < frhis[i]) {
-              nh = i - ih + 1;
-              break;
-            }
-        if (ih < 0 || nh < 0) {
+      if (f) {  // the bins [lo, hi] touches (hisrange, S/tetwt5.f90:487-513)
+        const int i0 = tet::first_edge_above(frhis, nwhis, demin[e]);
+        const int i1 = max(i0, tet::first_edge_above(frhis, nwhis, demax[e]));
+        if (i1 > nwhis) {
           *err = 2;
-          nh = 0;
+        } else {
+          ih = i0;
+          nh = i1 - i0 + 1;
         }
       }
     }
@@ -467,20 +275,18 @@ __global__ __launch_bounds__(256) void tw_job1_kernel(const TetRow* __restrict__
   int ib, jb;
   double eo[4], eu[4];
   if (!tw_pair(t, p, ek1, ek2, nband, ef, ib, jb, eo, eu)) return;
-  Sink s;
+  CornerBins s;
   s.scale = scale;
   s.whw = whw;
-  s.err = err;
-  double kv[4][3], x[4];
+  double kv[4][3];
   for (int c = 0; c < 4; ++c) {
     const size_t o = (size_t)(ibjb[idx3(ib, jb, t.k[c], nband)] - 1) + (size_t)ld * t.k[c];
     s.ihw[c] = ihw[o];
     s.nhw[c] = nhw[o];
     s.jhw[c] = jhw[o];
     for (int d = 0; d < 3; ++d) kv[c][d] = qbzw[d + 3 * (size_t)t.w[c]];
-    x[c] = 0.5 * (eo[c] - eu[c]);
   }
-  lindtet6(kv, eo, eu, x, ef, frhis, nwhis, s);
+  if (!tet::pair_histogram(kv, eo, eu, ef, frhis, nwhis, s)) *err = 3;  // outside the histogram mesh
 }
 
 // degenerate bands: every pair of a block takes the block's average histogram

@@ -18,7 +18,7 @@ module ecalj_gpu
   public :: ecalj_set_ppb_full, ecalj_zmel_init, ecalj_zmel_init_dev
   public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
   public :: ecalj_tetwt, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
-  public :: ecalj_wv_write_dev, ecalj_wv_read_dev
+  public :: ecalj_wv_write_dev, ecalj_wv_read_dev, ecalj_wv4_write_dev, ecalj_wv4_read_dev
   public :: ecalj_wc_build_gamma, ecalj_wc_build_gamma_dev, ecalj_eps_head, ecalj_eps_head_dev, ecalj_getw0
   public :: ecalj_wv0_patch, ecalj_wv0_patch_dev
   public :: ecalj_set_hamindex, ecalj_set_mnla, ecalj_readcphi_dev, ecalj_readgeig_dev, ecalj_mnla_read
@@ -267,6 +267,19 @@ module ecalj_gpu
       integer(c_int), value :: rec0, nrec, ld
     end function
     integer(c_int) function ecalj_wv_read_dev(h, path, rec0, nrec, ld, w) bind(C, name="ecalj_wv_read_dev")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, w
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: rec0, nrec, ld
+    end function
+    ! the complex(4) records of the mixed-precision build (__MP, S/m_kind.f90:3-6)
+    integer(c_int) function ecalj_wv4_write_dev(h, path, rec0, nrec, ld, w) bind(C, name="ecalj_wv4_write_dev")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, w
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: rec0, nrec, ld
+    end function
+    integer(c_int) function ecalj_wv4_read_dev(h, path, rec0, nrec, ld, w) bind(C, name="ecalj_wv4_read_dev")
       import :: c_int, c_ptr, c_char
       type(c_ptr), value :: h, w
       character(kind=c_char), intent(in) :: path(*)

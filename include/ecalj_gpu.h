@@ -101,7 +101,9 @@ ecalj_status ecalj_set_tuning(ecalj_handle h, int mode);
  * starts a new state generation: ecalj_graph_launch then returns
  * ECALJ_ERR_STATE for graphs of an older generation, and ECALJ_ERR_ARG when
  * called with a handle other than the capturing one.  ecalj_graph_destroy
- * waits for the graph's last launch. */
+ * waits for the graph's last launch.  ecalj_graph_begin returns
+ * ECALJ_ERR_STATE in a process run under the rocprofv3 profiler (a traced
+ * graph replay crashed inside the tracer): profile the eager calls. */
 typedef struct ecalj_graph_s* ecalj_graph_t;
 ecalj_status ecalj_graph_begin(ecalj_handle h);
 ecalj_status ecalj_graph_end(ecalj_handle h, ecalj_graph_t* graph);
@@ -488,6 +490,14 @@ ecalj_status ecalj_wv_d_read(const char* path, int64_t* v7);
 ecalj_status ecalj_wv_write_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld,
                                 const void* w);
 ecalj_status ecalj_wv_read_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld, void* w);
+/* The complex(4) records of the reference's mixed-precision build (__MP:
+ * kindgw = kindrcxq = 4, S/m_kind.f90:3-6; mrecl = 8*nblochpmx**2 bytes,
+ * S/rdpp.f90:43; read by S/m_sxcf_gemm.f90:317-318 with that recl).  Device W
+ * is complex128 (ld x ld x nrec); elements are rounded to complex(4) on write
+ * and widened exactly on read.  Synchronous. */
+ecalj_status ecalj_wv4_write_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld,
+                                 const void* w);
+ecalj_status ecalj_wv4_read_dev(ecalj_handle h, const char* path, int rec0, int nrec, int ld, void* w);
 
 /* ---- Sigma_c ------------------------------------------------------------
  * Imaginary-axis weights (S/m_sxcf_gemm.f90:432-458), FP64:

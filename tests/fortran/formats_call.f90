@@ -5,6 +5,7 @@
 !   vc_w / vc_r   __Vcoud: ngb; q(3); vcoud(ngb); zcousq(ngb,ngb)
 !   s2_w / s2_r   SEC2U: nspin nq ntq nqbz nqibz n1 n2 n3; per q: is, q(3), zsec(ntq,ntq)
 !   wv_w / wv_r   direct access, recl = 16*ld*ld bytes, records 1..nrec
+!   w4_w / w4_r   the same in complex(4) (the __MP build), recl = 8*ld*ld bytes
 !   qg_w          __QGpsi (mkqg): header; per q (q, ngp, irr), (ngvecp, ngvecprev)
 !   ev_w          __EValue (sugw): (nband, nqi, nsp, nspc), qirr(3,nqi), evl(nband,nqi,nspx)
 !   cp_w          __CPHI-like direct access: recl bytes, nrec records of n complex(8)
@@ -23,6 +24,7 @@ program formats_call
   real(8), allocatable :: v(:), qq(:,:)
   integer, allocatable :: isp(:)
   complex(8), allocatable :: z(:,:), zs(:,:,:), w(:,:,:)
+  complex(4), allocatable :: w4(:,:,:)
   call get_command_argument(1, mode)
   call get_command_argument(2, fname)
   call get_command_argument(3, sname)
@@ -98,6 +100,30 @@ program formats_call
     close(u)
     open(newunit=s, file=trim(sname), access='stream', form='unformatted', status='replace')
     write(s) w
+    close(s)
+  case ('w4_w')
+    open(newunit=s, file=trim(sname), access='stream', form='unformatted', status='old')
+    read(s) ld, nrec
+    allocate(w4(ld, ld, nrec))
+    read(s) w4
+    close(s)
+    open(newunit=u, file=trim(fname), form='unformatted', access='direct', recl=8*ld*ld)
+    do r = 1, nrec
+      write(u, rec=r) w4(:, :, r)
+    enddo
+    close(u)
+  case ('w4_r')
+    open(newunit=s, file=trim(sname), access='stream', form='unformatted', status='old')
+    read(s) ld, nrec
+    close(s)
+    allocate(w4(ld, ld, nrec))
+    open(newunit=u, file=trim(fname), form='unformatted', access='direct', recl=8*ld*ld, status='old')
+    do r = 1, nrec
+      read(u, rec=r) w4(:, :, r)
+    enddo
+    close(u)
+    open(newunit=s, file=trim(sname), access='stream', form='unformatted', status='replace')
+    write(s) w4
     close(s)
   case ('qg_w')
     open(newunit=s, file=trim(sname), access='stream', form='unformatted', status='old')

@@ -116,3 +116,26 @@ def test_graph_refused_after_scratch_moves():
     finally:
         other.close()
         g.h.close()
+
+
+def test_capture_refused_under_the_profiler(monkeypatch):
+    """Under rocprofv3 (its ROCPROF_* variables set) graph capture is refused
+    with a state error before anything is recorded, and eager calls run."""
+    import ecalj_amd
+    from ecalj_amd.qsgw import QPointGPU
+
+    qp = synth.make_qpoint(synth.CONFIGS["T1"])
+    g = QPointGPU(qp, "f64")
+    try:
+        monkeypatch.setenv("ROCPROF_KERNEL_TRACE", "1")
+        with pytest.raises(ecalj_amd.EcaljError) as e:
+            g.h.graph_begin()
+        assert e.value.status == 6  # ECALJ_ERR_STATE
+        g.step()
+        assert np.isfinite(g.results()[3]).all()
+        monkeypatch.delenv("ROCPROF_KERNEL_TRACE")
+        g.h.graph_begin()
+        g.step()
+        g.h.graph_end().close()
+    finally:
+        g.h.close()
