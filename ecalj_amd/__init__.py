@@ -91,6 +91,7 @@ _SIGS = {
     "ecalj_tetwt": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _l, _p, _p, _p, _p, _p, _p,
                     _p, _p, _p],
     "ecalj_zminv_dev": [_p, _p, _i, _i, _i, C.c_int64],
+    "ecalj_zminv_noalloc_dev": [_p, _p, _i, _i, _i, C.c_int64],
     "ecalj_wc_build_gamma": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
     "ecalj_wc_build_gamma_dev": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
     "ecalj_eps_head": [_p, _p, _i, _i, _i, _i, _d, _i, _p, _i, _p],
@@ -724,6 +725,13 @@ class Handle:
         lda = n if lda is None else lda
         stride = lda * n if stride is None else stride
         _chk(lib().ecalj_zminv_dev(self._h, _t_ptr(a), n, lda, nbatch, stride), "zminv_dev")
+
+    def zminv_noalloc_dev(self, a, n, nbatch=1, lda=None, stride=None):
+        """zminv_dev on the device's fixed scratch array (no allocation; calls
+        that overlap on one device race)."""
+        lda = n if lda is None else lda
+        stride = lda * n if stride is None else stride
+        _chk(lib().ecalj_zminv_noalloc_dev(self._h, _t_ptr(a), n, lda, nbatch, stride), "zminv_noalloc_dev")
 
     # ---- the Gamma point of the W build
     def wc_build_gamma(self, chi, nspin=1, is_x0_m_basis=True, real_axis=False):

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -37,6 +38,17 @@ void tune_store(ecalj_ctx* h, const TuneKey& k, int var, const char* op, const s
 
 ecalj_status slot_ensure(ecalj_ctx* h, int slot, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
+  if (h->arena) {  // fixed arena: no allocation, no synchronisation
+    if (h->arena_len[slot] < bytes) {  // first use of the slot in this call, or a larger one: a fresh region
+      const size_t off = (h->arena_used + 255) & ~(size_t)255;
+      if (off > h->arena_cap || bytes > h->arena_cap - off) return ECALJ_ERR_DIM;
+      h->arena_off[slot] = off;
+      h->arena_len[slot] = bytes;
+      h->arena_used = off + bytes;
+    }
+    *out = h->arena + h->arena_off[slot];
+    return ECALJ_OK;
+  }
   if (h->slot_bytes[slot] < bytes && h->capturing) return ECALJ_ERR_STATE;  // warm up before capture
   if (h->slot_bytes[slot] < bytes) {
     if (h->slot[slot]) {
@@ -255,9 +267,41 @@ const char* ecalj_status_string(int s) {
     case ECALJ_ERR_STATE: return "required set-up call missing";
     case ECALJ_ERR_NOT_PD: return "matrix not positive definite";
     case ECALJ_ERR_NOT_CONVERGED: return "eigensolver did not converge";
+    case ECALJ_ERR_SCRATCH: return "request exceeds the fixed scratch of the entry point that allocates nothing";
     default: return "unknown status";
   }
 }
+
+// Scratch of the entry points that allocate nothing (ecalj_zminv_noalloc_dev):
+// one fixed array per device, held until the process ends.  In host code the
+// name `scratch` is only a host-side stand-in; kernels get the device address
+// taken by hipGetSymbolAddress (per device, at the first ecalj_create on it).
+static __device__ __align__(256) unsigned char scratch[ECALJ_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_tried[kMaxDevices] = {};
+
+// With `device` current: the device address of its copy of `scratch`
+// (nullptr when the lookup failed; ecalj_zminv_noalloc_dev then refuses).
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_tried[device]) return;
+  g_scratch_tried[device] = true;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p) g_scratch_ptr[device] = p;
+  else (void)hipGetLastError();  // leave no sticky error for the next call
+}
+
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 ecalj_status ecalj_create(ecalj_handle* out, int device) {
   if (!out) return ECALJ_ERR_ARG;
@@ -266,6 +310,7 @@ ecalj_status ecalj_create(ecalj_handle* out, int device) {
   ECALJ_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return ECALJ_ERR_ARG;
   ECALJ_HIP(hipSetDevice(device));
+  scratch_setup(device);
   static std::atomic<uint64_t> next_id{0};
   ecalj_ctx* h = new ecalj_ctx;
   h->device = device;
@@ -1241,6 +1286,30 @@ ecalj_status ecalj_zminv_dev(ecalj_handle h, void* a, int n, int lda, int nbatch
   ECALJ_TRY(gj_inverse(h, (cd*)a, n, lda, stride, nbatch, &perm));
   ECALJ_TRY(gj_unscramble(h, (cd*)a, n, lda, stride, nbatch, perm));
   return ECALJ_OK;
+}
+
+// ecalj_zminv_dev on the device's fixed scratch array: the same kernels, no
+// allocation, no host synchronisation.  Calls that overlap on one device race
+// on the array.
+ecalj_status ecalj_zminv_noalloc_dev(ecalj_handle h, void* a, int n, int lda, int nbatch, int64_t stride) {
+  if (!h || !a) return ECALJ_ERR_ARG;
+  if (n < 0 || lda < std::max(n, 1) || nbatch < 0 || (nbatch > 1 && stride < (int64_t)lda * n))
+    return ECALJ_ERR_DIM;
+  if (n == 0 || nbatch == 0) return ECALJ_OK;
+  if (n > 8192 || nbatch > 65535) return ECALJ_ERR_DIM;  // limits of gj_inverse
+  if (zminv_scratch_bytes(n, nbatch) > (size_t)ECALJ_NOALLOC_SCRATCH_BYTES) return ECALJ_ERR_SCRATCH;
+  void* base = scratch_ptr(h->device);
+  if (!base || h->arena) return ECALJ_ERR_STATE;
+  h->arena = (unsigned char*)base;
+  h->arena_cap = ECALJ_NOALLOC_SCRATCH_BYTES;
+  h->arena_used = 0;
+  std::fill(h->arena_len, h->arena_len + ecalj_ctx::S_NSLOT, (size_t)0);
+  const int* perm = nullptr;
+  ecalj_status st = gj_inverse(h, (cd*)a, n, lda, stride, nbatch, &perm);
+  if (st == ECALJ_OK) st = gj_unscramble(h, (cd*)a, n, lda, stride, nbatch, perm);
+  h->arena = nullptr;
+  h->arena_cap = h->arena_used = 0;
+  return st;
 }
 
 ecalj_status ecalj_zminv(ecalj_handle h, double* a, int n, int lda) {

@@ -53,6 +53,12 @@ struct ecalj_ctx {
          S_EIG1, S_EIG2, S_EIG3, S_EIG4, S_EIG5, S_EIG6, S_NSLOT };
   void* slot[S_NSLOT] = {};
   size_t slot_bytes[S_NSLOT] = {};
+  // fixed-arena mode (the entry points that allocate nothing): while arena is
+  // set, slot_ensure carves each slot out of [arena, arena + arena_cap) by a
+  // bump pointer instead of allocating; the slot table above is untouched
+  unsigned char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  size_t arena_off[S_NSLOT] = {}, arena_len[S_NSLOT] = {};
   int* d_flag = nullptr;          // singular-pivot flag
   // pinned host staging per slot for asynchronous uploads; the event marks
   // the end of the last copy out of it

@@ -924,6 +924,21 @@ __global__ void gj_outer_copy_kernel(const cd* __restrict__ a, int n, int64_t ld
   }
 }
 
+static size_t round256(size_t b) { return ((b ? b : 256) + 255) & ~(size_t)255; }
+
+// The slot requests of gj_inverse (pivot) + gj_unscramble, in their order.
+size_t zminv_scratch_bytes(int64_t n, int64_t nmat) {
+  const size_t z = sizeof(cd), m = (size_t)nmat, N = (size_t)n;
+  size_t t = round256(sizeof(int) * 2 * N * m);                             // S_GJPIV
+  if (n <= GJO) {
+    t += round256(z * GJB * GJB * m) + round256(z * N * GJB * m);           // S_GJP, S_GJC
+  } else {
+    t += round256(z * GJO * GJO * m) + 3 * round256(z * N * GJO * m);       // S_GJD, S_GJOC, S_GJR, S_GJW
+    t += round256(z * GJB * GJB * m) + round256(z * (size_t)GJO * GJB * m); // gj_inverse_1 of the diagonal blocks
+  }
+  return t + round256(z * N * N * m);                                       // S_TMP2 of gj_unscramble
+}
+
 ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, const int** perm) {
   if (n == 0 || nmat == 0) return ECALJ_OK;
   if (nmat > 65535) return ECALJ_ERR_DIM;

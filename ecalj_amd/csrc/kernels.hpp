@@ -31,6 +31,9 @@ ecalj_status gj_inverse(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride
                         const int** perm = nullptr);
 // a(:, j) <- a(:, perm[j]) for each matrix (column unscramble after gj_inverse)
 ecalj_status gj_unscramble(ecalj_ctx* h, cd* a, int n, int64_t lda, int64_t astride, int nmat, const int* perm);
+// Scratch bytes that gj_inverse with pivoting followed by gj_unscramble take
+// from a fixed arena (slots rounded up to 256 B) for nmat matrices of order n.
+size_t zminv_scratch_bytes(int64_t n, int64_t nmat);
 struct RealSpec;
 // rs (optional): the real-axis pole term, added into czmelwc before the final product
 ecalj_status sig_contract(ecalj_ctx* h, int mode, int nbatch, const void* zmel, int n, int nm, int ntq, int niw,

@@ -37,8 +37,13 @@ typedef enum {
   ECALJ_ERR_NOMEM = 5,       /* device allocation failed                     */
   ECALJ_ERR_STATE = 6,       /* required set-up call missing (set_ppb, ...)  */
   ECALJ_ERR_NOT_PD = 7,      /* matrix not positive definite (zhgv's B)      */
-  ECALJ_ERR_NOT_CONVERGED = 8 /* iterative eigensolver did not converge      */
+  ECALJ_ERR_NOT_CONVERGED = 8, /* iterative eigensolver did not converge     */
+  ECALJ_ERR_SCRATCH = 9       /* request exceeds ECALJ_NOALLOC_SCRATCH_BYTES  */
 } ecalj_status;
+
+/* Fixed device scratch of the entry point that allocates nothing
+ * (ecalj_zminv_noalloc_dev), one array per device held until process exit. */
+#define ECALJ_NOALLOC_SCRATCH_BYTES (64u << 20)
 
 /* Arithmetic of the contractions (zmel, chi0, M<->E transforms, Sigma_c).
  * The epsilon inversion, the weights and the QP update are always FP64. */
@@ -377,6 +382,21 @@ ecalj_status ecalj_wc_build_real_dev(ecalj_handle h, const void* chi, int ngb, i
 ecalj_status ecalj_zminv(ecalj_handle h, double* a, int n, int lda);
 ecalj_status ecalj_zminv_dev(ecalj_handle h, void* a, int n, int lda, int nbatch,
                              int64_t stride);
+/* ecalj_zminv_dev without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (HIP graph capture with
+ * no warm-up call, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as ecalj_zminv_dev; its work space comes from
+ * the device's fixed array of ECALJ_NOALLOC_SCRATCH_BYTES (64 MiB): about
+ * 16 n^2 + 6.3 kB n + 0.3 MB per matrix above n = 128, so one matrix up to
+ * n = 1859, 6 of n = 600 or 2 of n = 1200 per call.  Larger requests return
+ * ECALJ_ERR_SCRATCH before any work is queued.  All calls on one device share
+ * the array: calls that overlap in time on one device (two streams, two
+ * handles) race and must be serialised by the caller.  Returns
+ * ECALJ_ERR_STATE if the array's device address could not be taken when the
+ * first handle on the device was created.  Asynchronous; a zero pivot is
+ * reported as for ecalj_zminv_dev. */
+ecalj_status ecalj_zminv_noalloc_dev(ecalj_handle h, void* a, int n, int lda, int nbatch,
+                                     int64_t stride);
 
 /* ---- the Gamma point of the W build ------------------------------------
  * q = 0 body (iq = 1) of WVIllwI / WVRllwR (S/m_llw.f90:98-178, 299-373 with
