@@ -585,11 +585,18 @@ def roofline(fl, info, precision, steps, config="C2"):
     avg_s = ms / 1e3 / max(nl, 1)
     achieved = flop_launch / avg_s / 1e12
     peak = PEAK[precision]
+    var = chosen_variant(info.get("tuning"))
     kname = {"bf16x3": "sig_bf3_kernel (bf16x3 Sigma_c: per-frequency W_c(iv)^H zmel, weighted sum, all internal k)",
              "bf16f8": "sig_f8_kernel (Sigma_c: bf16 hi*hi + scaled-fp8 cross terms, per-frequency W_c(iv)^H zmel, "
                        "weighted sum, all internal k)"}.get(
         precision, "sig_freq_gemm_kernel (Sigma_c: W_c(iv)^H x frequency-weighted zmel, all internal k)")
-    tr = MEASURED_TRAFFIC.get((config, precision, chosen_variant(info.get("tuning"))))
+    if precision == "bf16x3" and var and var.startswith("3m"):
+        kname = ("sig3m_kernel (bf16x3 Sigma_c, frequency-inner, three real products per complex product: "
+                 "W_c(iv)^H x frequency-weighted zmel, all internal k)")
+    elif precision == "bf16x3" and var and var.startswith("vin4"):
+        kname = ("sigv4_kernel (bf16x3 Sigma_c, frequency-inner, four real products per complex product: "
+                 "W_c(iv)^H x frequency-weighted zmel, all internal k)")
+    tr = MEASURED_TRAFFIC.get((config, precision, var))
     r = {"kernel": kname,
          "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
          "frac": round(achieved / peak, 4), "traffic": tr[0] if tr else None,
@@ -598,8 +605,14 @@ def roofline(fl, info, precision, steps, config="C2"):
         r["traffic_unit"] = "bytes per launch (HBM/fabric reads + writes, calibrated counters)"
         r["traffic_source"] = tr[1]
     if precision == "bf16x3":
-        r["note"] = ("achieved counts algorithmic complex flops (8 per complex MAC); each costs 3 bf16 MFMA "
-                     "products, so the attainable algorithmic rate is peak/3")
+        r["variant"] = var
+        if var and var.startswith("3m"):
+            r["note"] = ("achieved counts algorithmic complex flops (8 per complex MAC); the three-product form "
+                         "costs 9 bf16 MFMA products per 4 real products (3 complex products x 3 split terms), "
+                         "2.25 MFMA flops per algorithmic flop, so the attainable algorithmic rate is peak/2.25")
+        else:
+            r["note"] = ("achieved counts algorithmic complex flops (8 per complex MAC); each costs 3 bf16 MFMA "
+                         "products, so the attainable algorithmic rate is peak/3")
     if precision == "bf16f8":
         r["note"] = ("achieved counts algorithmic complex flops (8 per complex MAC); each costs one bf16 MFMA "
                      "product and two fp8 products at twice the bf16 rate, so the attainable algorithmic rate "

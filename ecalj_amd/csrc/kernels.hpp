@@ -81,6 +81,22 @@ ecalj_status real_plan_into_cz(ecalj_ctx* h, const RealPlanHandle* r, const Real
 ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm, const void* zmel, int n, int ntq,
                                   int niw, const cd* wc, int64_t ldw, const double* wgt, cd* zsec, int64_t ldz);
 size_t sig_bf3_a_elems_per_k(int n, int nw1);
+// The frequency-inner three-product form of the bf16x3 Sigma_c contraction
+// (sig3m.hip): operand preparation (six W planes into as, padded complex64
+// zmel into zp) and the contraction itself.
+size_t sig3m_a_elems_per_k(int n, int nw1);
+ecalj_status sig3m_prep(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1,
+                        const cd* wc, int64_t ldw, int64_t w_stride, unsigned short* as, cf* zp);
+ecalj_status sig3m_run(ecalj_ctx* h, bool xcd, int nb, int n, int ncol, int nw1, const unsigned short* as,
+                       const cf* zp, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride);
+ecalj_status sigv4_run(ecalj_ctx* h, bool xcd, int nb, int n, int ncol, int nw1, const unsigned short* as,
+                       int64_t as_stride, const cf* zp, const double* wgt, int64_t g_stride, cf* cz,
+                       int64_t cz_stride);
+ecalj_status sig_pad_z(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, cf* zp);
+ecalj_status sig3m_grouped_prep(ecalj_ctx* h, int nk, const int* ncol, const cf* zmel, int n, int nw1, const cd* wc,
+                                int64_t ldw, int64_t w_stride, unsigned short* as, cf* zp);
+ecalj_status sig3m_grouped_run(ecalj_ctx* h, int nk, const int* ncol, int n, int nw1, const unsigned short* as,
+                               const cf* zp, const double* wgt, cf* cz);
 // x0gemm in the bf16x3 form: per-bin entry lists on the host (bin b: entries
 // cnt[b]..cnt[b+1]-1 with zmel column offsets col[] and weights wt[]), output
 // slice slice[b] of rcxq (npr x npr_col, complex64, slice stride slice_elems)

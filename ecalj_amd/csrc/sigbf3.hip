@@ -204,12 +204,27 @@ __global__ __launch_bounds__(NW * 64, 1) void sig_bf3_grouped_kernel(const int* 
 
 // Kernel variants of the tuner: {column tile, stages, waves, XCD-aware order}
 struct SigVariant { int bn, nstg, nw, xcd; const char* name; };
+// The last two are the frequency-inner three-product form (sig3m.hip), whose
+// operands are prepared differently and whose sums round differently.
 static const SigVariant kSigVariants[] = {{64, 3, 4, 0, "128x64x3"},
                                           {128, 2, 4, 0, "128x128x2"},
                                           {128, 2, 8, 0, "128x128x2/8w"},
-                                          {128, 2, 8, 1, "128x128x2/8w/xcd"}};
-constexpr int kNumSigVariants = 4;
-constexpr int kNumSigGroupedVariants = 3;  // the first three; the grouped kernel has its own tile order
+                                          {128, 2, 8, 1, "128x128x2/8w/xcd"},
+                                          {128, 3, 8, 0, "3m/128x128"},
+                                          {128, 3, 8, 1, "3m/128x128/xcd"},
+                                          {128, 4, 8, 0, "vin4/128x128"},
+                                          {128, 4, 8, 1, "vin4/128x128/xcd"}};
+constexpr int kNumSigVariants = 8;
+constexpr int kFirst3m = 4;
+constexpr int kFirstVin4 = 6;
+// operand preparation of a variant: 0 = W and zmel split into bf16 planes;
+// 1 = six W planes + padded complex64 zmel (3m); 2 = four W planes + padded zmel
+static int sig_form(int var) { return var >= kFirstVin4 ? 2 : var >= kFirst3m ? 1 : 0; }
+// Grouped kernel: the first three tile shapes, then the three-product form
+// (each with its own tile order)
+constexpr int kNumSigGroupedVariants = 4;
+static const char* const kSigGroupedNames[kNumSigGroupedVariants] = {"128x64x3", "128x128x2", "128x128x2/8w",
+                                                                      "3m/128x128/xcd"};
 
 // Column tiles per super-tile of the XCD-aware order: the super-tile fills
 // about 32 CUs (one XCD) and its zmel panels (G x BN columns x NP x 8 bytes)
@@ -256,18 +271,32 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
   const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NCP = pad_to(ncol, QBN_MAX), NS = NP / QBK;
   const int64_t as_stride = 4LL * nw1 * MP * NP;
   const int64_t bs_stride = 4LL * NCP * NP;
-  // W planes: the split grid covers all MP x NP, so padding comes out zero
-  dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nb);
-  if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
-  hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
-  ECALJ_HIP(hipGetLastError());
-  const int64_t totz = (int64_t)NS * NCP * 4 * nb;
-  const int bz = (int)std::min<int64_t>((totz + 255) / 256, 65536);
-  hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel, z_stride, n, ncol, NCP, NS, bs,
-                     bs_stride, nb);
-  ECALJ_HIP(hipGetLastError());
+  // operands of a variant's form: four W planes and four zmel planes, or
+  // (three-product form) six W planes and the padded complex64 zmel
+  auto prep = [&](int form) -> ecalj_status {
+    if (form == 1)
+      return sig3m_prep(h, nb, zmel, z_stride, n, ncol, nw1, wc, ldw, w_stride, as, reinterpret_cast<cf*>(bs));
+    // W planes: the split grid covers all MP x NP, so padding comes out zero
+    dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nb);
+    if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
+    hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as,
+                       as_stride);
+    ECALJ_HIP(hipGetLastError());
+    if (form == 2) return sig_pad_z(h, nb, zmel, z_stride, n, ncol, reinterpret_cast<cf*>(bs));
+    const int64_t totz = (int64_t)NS * NCP * 4 * nb;
+    const int bz = (int)std::min<int64_t>((totz + 255) / 256, 65536);
+    hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel, z_stride, n, ncol, NCP, NS, bs,
+                       bs_stride, nb);
+    ECALJ_HIP(hipGetLastError());
+    return ECALJ_OK;
+  };
   auto run = [&](int var) -> ecalj_status {
     const bool xcd = kSigVariants[var].xcd != 0;
+    if (var >= kFirstVin4)
+      return sigv4_run(h, xcd, nb, n, ncol, nw1, as, as_stride, reinterpret_cast<const cf*>(bs), wgt, g_stride, cz,
+                       cz_stride);
+    if (var >= kFirst3m)
+      return sig3m_run(h, xcd, nb, n, ncol, nw1, as, reinterpret_cast<const cf*>(bs), wgt, g_stride, cz, cz_stride);
     if (kSigVariants[var].bn == 64)
       return sig_launch_t<64, 3, 4>(h, nb, n, ncol, nw1, MP, NS, NCP, as, as_stride, bs, bs_stride, wgt, g_stride, cz,
                                     cz_stride, xcd);
@@ -278,7 +307,8 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
                                    cz_stride, xcd);
   };
   // variant: forced by the handle, cached for this shape, or measured now
-  // (every variant computes the same sums in the same order, so whichever ran last leaves the result)
+  // (the variants of one form compute the same sums in the same order; after
+  // measuring, the chosen variant runs once more, so the result is always its own)
   int var = -1;
   const TuneKey key{TUNE_SIGC_BF3, n, ncol, nw1, nb};
   if (h->tune_mode > 0) {
@@ -287,9 +317,16 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
   } else if (h->capturing) {  // no timing under capture: first variant
     var = 0;
   } else {
+    // each variant is charged with its own operand preparation
     std::vector<double> ms(kNumSigVariants);
-    for (int v = 0; v < kNumSigVariants; ++v) {
-      ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
+    for (int f = 0; f < 3; ++f) {
+      double pm = 0.0;
+      ECALJ_TRY(tune_time(h, [&] { return prep(f); }, &pm));
+      for (int v = 0; v < kNumSigVariants; ++v)
+        if (sig_form(v) == f) {
+          ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
+          ms[v] += pm;
+        }
     }
     var = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
     // One launch per variant shows the first, cool launch.  Over consecutive
@@ -299,11 +336,19 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
     // kept when within 3% of the fastest single launch.
     for (int v = 0; v < kNumSigVariants; ++v)
       if (kSigVariants[v].xcd && ms[v] <= 1.03 * ms[var]) var = v;
+    // The three-product form issues a quarter fewer MFMAs per flop; under the
+    // power limit of consecutive q-points it keeps a higher clock, and its XCD
+    // variant ran the C3 step in 456 ms against 474-483 ms for the four-product
+    // variants that matched it in this single cold launch.  It is kept when
+    // within 5% of the fastest single launch.
+    if (ms[kFirst3m + 1] <= 1.05 * ms[var]) var = kFirst3m + 1;
     std::vector<const char*> names;
     for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
     tune_store(h, key, var, "sigc_bf3", names, ms);
-    return ECALJ_OK;
+    ECALJ_TRY(prep(sig_form(var)));
+    return run(var);
   }
+  ECALJ_TRY(prep(sig_form(var)));
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 0, &tev));
   ECALJ_TRY(run(var));
@@ -333,21 +378,29 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
     co += (int64_t)n * ncol[k];
     cols += ncol[k];
   }
-  dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nk);
-  if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
-  hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
-  ECALJ_HIP(hipGetLastError());
-  for (int k = 0; k < nk; ++k) {
-    if (ncol[k] == 0) continue;
-    const int64_t totz = (int64_t)NS * grp[k].ncp * 4;
-    const int bz = (int)std::min<int64_t>((totz + 255) / 256, 65536);
-    hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel + grp[k].czoff, (int64_t)0, n,
-                       ncol[k], grp[k].ncp, NS, bs + grp[k].boff, (int64_t)0, 1);
-    ECALJ_HIP(hipGetLastError());
-  }
   if (cols == 0) return ECALJ_OK;
+  constexpr int k3m = kNumSigGroupedVariants - 1;
+  auto prep = [&](bool is3m) -> ecalj_status {
+    if (is3m)
+      return sig3m_grouped_prep(h, nk, ncol, zmel, n, nw1, wc, ldw, w_stride, as, reinterpret_cast<cf*>(bs));
+    dim3 gw((NS + SPLIT_KS - 1) / SPLIT_KS, MP / SPT, nw1 * nk);
+    if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
+    hipLaunchKernelGGL(split_w_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as,
+                       as_stride);
+    ECALJ_HIP(hipGetLastError());
+    for (int k = 0; k < nk; ++k) {
+      if (ncol[k] == 0) continue;
+      const int64_t totz = (int64_t)NS * grp[k].ncp * 4;
+      const int bz = (int)std::min<int64_t>((totz + 255) / 256, 65536);
+      hipLaunchKernelGGL(split_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel + grp[k].czoff, (int64_t)0, n,
+                         ncol[k], grp[k].ncp, NS, bs + grp[k].boff, (int64_t)0, 1);
+      ECALJ_HIP(hipGetLastError());
+    }
+    return ECALJ_OK;
+  };
   // table: [SigGroup x nk][tiles: (k, row tile, column tile) x ntiles]
   auto run = [&](int var) -> ecalj_status {
+    if (var == k3m) return sig3m_grouped_run(h, nk, ncol, n, nw1, as, reinterpret_cast<const cf*>(bs), wgt, cz);
     const int bn = kSigVariants[var].bn;
     std::vector<int> tiles;
     for (int k = 0; k < nk; ++k)
@@ -390,13 +443,22 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
     var = 0;
   } else {
     std::vector<double> ms(kNumSigGroupedVariants);
-    for (int v = 0; v < kNumSigGroupedVariants; ++v) ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
+    for (int f = 0; f < 2; ++f) {
+      double pm = 0.0;
+      ECALJ_TRY(tune_time(h, [&] { return prep(f == 1); }, &pm));
+      for (int v = 0; v < kNumSigGroupedVariants; ++v)
+        if ((v == k3m) == (f == 1)) {
+          ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
+          ms[v] += pm;
+        }
+    }
     var = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
-    std::vector<const char*> names;
-    for (int v = 0; v < kNumSigGroupedVariants; ++v) names.push_back(kSigVariants[v].name);
+    std::vector<const char*> names(kSigGroupedNames, kSigGroupedNames + kNumSigGroupedVariants);
     tune_store(h, key, var, "sigc_bf3_grouped", names, ms);
-    return ECALJ_OK;
+    ECALJ_TRY(prep(var == k3m));
+    return run(var);
   }
+  ECALJ_TRY(prep(var == k3m));
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 0, &tev));
   ECALJ_TRY(run(var));
@@ -404,8 +466,10 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
   return ECALJ_OK;
 }
 
+// W plane elements per k of the largest form (the three-product form's six planes)
 size_t sig_bf3_a_elems_per_k(int n, int nw1) {
-  return (size_t)F8_ROWB / 2 * nw1 * pad_to(n, QBM) * (pad_to(n, QBK) / QBK);
+  return std::max((size_t)F8_ROWB / 2 * nw1 * pad_to(n, QBM) * (pad_to(n, QBK) / QBK),
+                  sig3m_a_elems_per_k(n, nw1));
 }
 
 // ---------------------------------------------------------------- plain batched GEMM on the same core

@@ -47,12 +47,13 @@ def _oracle(qp):
     return ref
 
 
-def _run_uniform(qp, prec, tail_model=None):
+def _run_uniform(qp, prec, tail_model=None, mode=0):
     from ecalj_amd.qsgw import QPointGPU
 
-    say(f"gpu: {qp.cfg.name} {prec}")
+    say(f"gpu: {qp.cfg.name} {prec} tuner mode {mode}")
     g = QPointGPU(qp, prec, tail_model=tail_model)
     try:
+        g.h.set_tuning(mode)
         g.chi0()
         g.wbuild()
         g.make_wk()
@@ -93,10 +94,13 @@ def test_c2_f64(c2):
     _check_f64(_run_uniform(qp, "f64"), ref)
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "f32"])
-def test_c2_mixed(c2, prec):
+@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 4), ("bf16x3", 6), ("bf16x3", 8), ("f32", 0)])
+def test_c2_mixed(c2, prec, mode):
+    """Mixed forms; bf16x3 with the tuner's choice and with the frequency-outer
+    (4), three-product (6) and four-product frequency-inner (8) Sigma_c
+    kernels forced."""
     qp, ref = c2
-    _check_mixed(_run_uniform(qp, prec), ref, prec)
+    _check_mixed(_run_uniform(qp, prec, mode=mode), ref, prec)
 
 
 # ------------------------------------------------------------------ C3
@@ -111,10 +115,11 @@ def test_c3_f64(c3):
     _check_f64(_run_uniform(qp, "f64"), ref)
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "f32", "bf16f8"])
-def test_c3_mixed(c3, prec):
+@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 4), ("bf16x3", 6), ("bf16x3", 8), ("f32", 0),
+                                       ("bf16f8", 0)])
+def test_c3_mixed(c3, prec, mode):
     qp, ref = c3
-    _check_mixed(_run_uniform(qp, prec), ref, prec)
+    _check_mixed(_run_uniform(qp, prec, mode=mode), ref, prec)
 
 
 # ------------------------------------------------------------------ C4
@@ -146,11 +151,11 @@ def test_c4_f64(c4):
     _check_f64(_run_uneven(qp, "f64"), ref)
 
 
-@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 1), ("bf16f8", 0)])
+@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 1), ("bf16x3", 4), ("bf16f8", 0)])
 def test_c4_grouped_mixed(c4, prec, mode):
     """The grouped persistent Sigma_c kernel over both k-points' tiles (the
-    tuner's choice and the 128x64 variant forced), at nI = 1600 and a
-    424-band window."""
+    tuner's choice, the 128x64 variant and the three-product form forced), at
+    nI = 1600 and a 424-band window."""
     qp, ref = c4
     _check_mixed(_run_uneven(qp, prec, mode), ref, prec)
 

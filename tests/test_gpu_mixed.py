@@ -150,9 +150,32 @@ def test_sigc_mixed(hp, orc, ngb, nm, ntq, niw):
     assert relerr(h.sigc(z, wc, g), orc.sigc_imag(z, wc, g)) < TOL[prec]
 
 
+@pytest.mark.parametrize("mode", [5, 6, 7, 8])
+@pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10), (161, 13, 5, 40), (1, 3, 1, 2),
+                                            (300, 40, 4, 0)])
+def test_sigc_bf3_frequency_inner_forms(orc, mode, ngb, nm, ntq, niw):
+    """The frequency-inner variants, forced (5, 6: three-product form; 7, 8:
+    four-product form): ragged and tile-aligned shapes, more than 24
+    frequencies (two launches that accumulate), one frequency, one row."""
+    import ecalj_amd
+
+    say("sigc bf16x3 3m variant", mode, ngb, nm, ntq, niw)
+    rng = np.random.default_rng(ngb + niw)
+    z = crand(rng, ngb, nm, ntq)
+    wc = crand(rng, ngb, ngb, niw + 1)
+    g = np.asfortranarray(rng.standard_normal((niw + 1, nm, ntq)))
+    h = ecalj_amd.Handle(0, "bf16x3")
+    try:
+        h.set_tuning(mode)
+        assert relerr(h.sigc(z, wc, g), orc.sigc_imag(z, wc, g)) < TOL["bf16x3"]
+    finally:
+        h.close()
+
+
 def test_sigc_bf3_tuner_variants(orc):
-    """Every kernel variant of the tuner gives the oracle's Sigma_c, bit-identical
-    to each other (same sums in the same order); auto mode records its choice."""
+    """Every kernel variant of the tuner gives the oracle's Sigma_c; the variants
+    of one form are bit-identical to each other (same sums in the same order);
+    auto mode records its choice and returns the chosen variant's result."""
     import ecalj_amd
 
     rng = np.random.default_rng(7)
@@ -164,20 +187,25 @@ def test_sigc_bf3_tuner_variants(orc):
     h = ecalj_amd.Handle(0, "bf16x3")
     try:
         outs = []
-        for mode in (1, 2, 3, 4):
+        for mode in (1, 2, 3, 4, 5, 6, 7, 8):
             say("sigc bf16x3 tuner variant", mode)
             h.set_tuning(mode)
             outs.append(h.sigc(z, wc, g))
             assert relerr(outs[-1], ref) < TOL["bf16x3"]
-        for o in outs[1:]:
+        for o in outs[1:4]:
             assert np.array_equal(outs[0], o)
+        assert np.array_equal(outs[4], outs[5])
+        assert np.array_equal(outs[6], outs[7])
         h.set_tuning(0)
         say("sigc bf16x3 tuner auto")
         a = h.sigc(z, wc, g)
         b = h.sigc(z, wc, g)  # cached choice
-        assert np.array_equal(a, outs[0]) and np.array_equal(b, outs[0])
         rep = h.tuning_report()
         assert rep.count("sigc_bf3") == 1 and "*" in rep
+        chosen = [ln for ln in rep.splitlines() if "sigc_bf3" in ln][0]
+        star = chosen[:chosen.index("*")].split()[-3]  # the name before "<ms> ms *"
+        expect = outs[4] if star.startswith("3m") else outs[6] if star.startswith("vin4") else outs[0]
+        assert np.array_equal(a, expect) and np.array_equal(b, expect)
     finally:
         h.close()
 
