@@ -44,6 +44,7 @@ MEASURED_TRAFFIC = {
     ("C2", "bf16x3", "128x128x2/8w/xcd"): (49.7e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
     ("C2", "bf16x3", "128x128x2/8w"): (124.0e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
     ("C2", "bf16f8", None): (48.8e9, "profiles/sigc_f8_c2_hbm_traffic.md"),  # one kernel, XCD-aware order
+    ("C3", "bf16x3", "3m/128x128/xcd"): (72.3e9, "profiles/sigc_3m_c3_hbm_traffic.md"),
 }
 
 
@@ -155,11 +156,14 @@ def cpu_baseline(qp, reps=5):
             "units_s": {k: round(v, 4) for k, v in t.items()}}
 
 
-def run_form(qp, precision, steps, warmup):
-    """Time `steps` q-point steps of one precision form; returns timings and QP energies."""
+def run_form(qp, precision, steps, warmup, tuning=0):
+    """Time `steps` q-point steps of one precision form; returns timings and QP energies.
+    tuning > 0 forces that Sigma_c kernel variant (profiling one variant)."""
     from ecalj_amd.qsgw import QPointGPU
 
     g = QPointGPU(qp, precision)
+    if tuning:
+        g.h.set_tuning(tuning)
     log(f"bench[{precision}]: setup run (W_c at internal k from the first W_c(q))")
     g.step()
     g.make_wk()
@@ -273,6 +277,7 @@ def bench_c5(log_=None):
     j = lt.judge(m, held, tol_mev=1.0, log=log_)
     qp = held[0]
     times = {}
+    chi0_t = {}
     for name, kw, q in (("full_fp64", {}, synth.dataclasses.replace(qp, cfg=cfg.scaled(nu_chi0=0))),
                         ("full_mixed", {}, synth.dataclasses.replace(qp, cfg=cfg.scaled(nu_chi0=0))),
                         ("learned_mixed", {"tail_model": m}, qp)):
@@ -285,7 +290,12 @@ def bench_c5(log_=None):
         g.step()
         g.h.sync()
         times[name] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g.chi0()  # chi0 alone (the learned form: truncated sum + tail)
+        g.h.sync()
+        chi0_t[name] = time.perf_counter() - t0
         g.h.close()
+    bound = lt.tail_bound(qp, cfg.nu_chi0, ndraw=7, log=log_)
     r = j["rows"][0]
     return {
         "workload": f"C5: nI={cfg.ni}, nf={cfg.nf}, nk={cfg.nk}, {cfg.no} occ, chi0 truncated at {cfg.nu_chi0} of "
@@ -296,8 +306,11 @@ def bench_c5(log_=None):
         "chi_diag_err_truncated": round(r["chi_diag_err_truncated"], 5),
         "chi_diag_err_learned": round(r["chi_diag_err_learned"], 5),
         "judged_error_meV": round(j["judged_error_meV"], 3), "accepted": j["accepted"],
+        "tail_bound": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in bound.items()},
         "form_used": "learned tail" if j["accepted"] else "full band sum (fallback)",
         "s_per_q": {k: round(v, 5) for k, v in times.items()},
+        "chi0_s": {k: round(v, 5) for k, v in chi0_t.items()},
+        "chi0_share_of_step_full_mixed": round(chi0_t["full_mixed"] / times["full_mixed"], 3),
     }
 
 
@@ -641,6 +654,8 @@ def main():
                     help="run only the --also legs (prints their JSON; for profiling them)")
     ap.add_argument("--only", choices=["f64", "mixed"], default=None,
                     help="profile one form only (prints a short line instead of the contract JSON)")
+    ap.add_argument("--tuning", type=int, default=0, metavar="K",
+                    help="with --only: force Sigma_c kernel variant K (1-based, as ecalj_set_tuning)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the q-points of a QSGW "
@@ -687,7 +702,7 @@ def main():
     log(f"bench: generating {cfg.name} inputs")
     qp = synth.make_qpoint(cfg)
     if args.only:
-        info = run_form(qp, "f64" if args.only == "f64" else args.mixed, args.steps, args.warmup)
+        info = run_form(qp, "f64" if args.only == "f64" else args.mixed, args.steps, args.warmup, args.tuning)
         print(json.dumps({"form": args.only, "s_per_q": info["s_per_q"],
                           "kernel_ms_per_step": {k: v[0] / args.steps for k, v in info["ktimes"].items()}}))
         return

@@ -255,6 +255,36 @@ def judge(m: TailModel, qps, tol_mev=1.0, log=None):
     return {"rows": rows, "judged_error_meV": worst, "accepted": worst <= tol_mev, "tol_meV": tol_mev}
 
 
+def tail_bound(qp: synth.QPoint, nu_t: int, ndraw: int = 4, precision: str = "bf16x3", log=None):
+    """A lower bound on the QP-energy error of any chi0 tail predicted from the
+    truncated sum: the unoccupied eigenvectors above nu_t are drawn again
+    (synth.redraw_high_bands; chi_T, the energies and all other inputs stay
+    exactly the same) and the full band sum is taken for each draw.  A
+    predictor sees the same chi_T for every draw, so its error on at least
+    one of two draws is at least half their difference; the spread over
+    draws is reported (max and RMS over bands and pairs of draws, meV)."""
+    full = qp.cfg.scaled(nu_chi0=0)
+    es = []
+    for d in range(ndraw + 1):
+        q = qp if d == 0 else synth.redraw_high_bands(qp, nu_t, seed=qp.cfg.seed * 7 + d)
+        g = QPointGPU(dataclasses.replace(q, cfg=full), precision)
+        try:
+            g.chi0()
+            g.wbuild()
+            g.make_wk()
+            g.sigc()
+            es.append(g.results()[3])
+        finally:
+            g.h.close()
+        if log:
+            log(f"tail bound: draw {d} done")
+    es = np.array(es)
+    diffs = np.array([es[i] - es[j] for i in range(len(es)) for j in range(i)])
+    return {"draws": ndraw + 1, "spread_max_meV": float(np.abs(diffs).max() * 1e3),
+            "spread_rms_meV": float(np.sqrt((diffs ** 2).mean()) * 1e3),
+            "min_achievable_max_err_meV": float(np.abs(diffs).max() * 1e3 / 2)}
+
+
 def seeded_qpoints(cfg: synth.Config, n: int, first: int = 0):
     """n q-points of the family cfg (seeds cfg.seed + 100 (first + i))."""
     return [synth.make_qpoint(cfg.scaled(seed=cfg.seed + 100 * (first + i))) for i in range(n)]

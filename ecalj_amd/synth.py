@@ -55,6 +55,9 @@ class Config:
     window: float = 0.02
     natom: int = 1   # atoms of the MT product basis: ppb in natom blocks of nb/natom x nb/natom x ni/natom
     nu_chi0: int = 0  # unoccupied bands in the chi0 band sum (0 = all); nband_chi0, S/m_readgwinput.f90:34-35
+    # ppb variance: "cell" natom^2/(ni*Omega) (cell-normalised product basis, the
+    # default), or "spec" 1/nb (the workloads' literal N(0, 1/nb) statement)
+    ppb_var: str = "cell"
     note: str = ""
 
     @property
@@ -175,11 +178,15 @@ def make_qpoint(cfg: Config) -> QPoint:
     # ppb: real Gaussian with variance natom^2/(ni*Omega): an eigenvector keeps
     # 1/natom of its norm on each atom, so sum_I |zmel|^2 ~ 1/Omega for any natom
     omega_cell = cfg.alat ** 3 / 4.0
+    if cfg.ppb_var not in ("cell", "spec"):
+        raise ValueError(f"ppb_var {cfg.ppb_var!r}: 'cell' or 'spec'")
     if cfg.natom == 1:
-        ppb = np.asfortranarray(rng.standard_normal((nb, nb, ni)) / math.sqrt(ni * omega_cell))
+        sd = 1.0 / math.sqrt(nb) if cfg.ppb_var == "spec" else 1.0 / math.sqrt(ni * omega_cell)
+        ppb = np.asfortranarray(rng.standard_normal((nb, nb, ni)) * sd)
     else:
         nv, md = nb // cfg.natom, ni // cfg.natom
-        ppb = rng.standard_normal(cfg.natom * nv * nv * md) * (cfg.natom / math.sqrt(ni * omega_cell))
+        sd = 1.0 / math.sqrt(nb) if cfg.ppb_var == "spec" else cfg.natom / math.sqrt(ni * omega_cell)
+        ppb = rng.standard_normal(cfg.natom * nv * nv * md) * sd
     # v(q)
     q = np.array([0.1, 0.2, 0.3])
     g = gvectors(ni)
@@ -191,6 +198,31 @@ def make_qpoint(cfg: Config) -> QPoint:
     return QPoint(cfg, np.asfortranarray(e_k), np.asfortranarray(e_kq), np.asfortranarray(c_k),
                   np.asfortranarray(c_kq), e_q, np.asfortranarray(e_qmk), c_q, np.asfortranarray(c_qmk),
                   ppb, vcoud, zcousq, wk)
+
+
+def redraw_high_bands(qp: QPoint, nu_t: int, seed: int) -> QPoint:
+    """The same q-point with the unoccupied k+q eigenvectors above band
+    no + nu_t drawn again (Haar-random in the orthogonal complement of the
+    kept ones); energies, the kept bands and everything else unchanged.  The
+    truncated chi0 (bands up to nu_t) is then identical, and the spread of
+    results from the full band sum over such draws is what no function of
+    the truncated sum can predict."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    no, nk = qp.cfg.no, qp.cfg.nk
+    keep = no + nu_t
+    c = np.array(qp.c_kq, copy=True)
+    nb, nbt = c.shape[0], c.shape[1]
+    if not 0 <= nu_t <= nbt - no:
+        raise ValueError(f"nu_t={nu_t} outside 0..{nbt - no}")
+    for k in range(nk):
+        low = c[:, :keep, k]
+        x = rng.standard_normal((nb, nbt - keep)) + 1j * rng.standard_normal((nb, nbt - keep))
+        for _ in range(2):  # project out the kept bands (twice, for orthogonality to rounding)
+            x -= low @ (low.conj().T @ x)
+        q, r = np.linalg.qr(x)
+        d = np.diag(r)
+        c[:, keep:, k] = q * (d / np.abs(d))[None, :]
+    return dataclasses.replace(qp, c_kq=np.asfortranarray(c))
 
 
 # --------------------------------------------------------------------------
