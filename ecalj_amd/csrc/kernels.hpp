@@ -87,7 +87,7 @@ size_t sig_bf3_a_elems_per_k(int n, int nw1);
 size_t sig3m_a_elems_per_k(int n, int nw1);
 ecalj_status sig3m_prep(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1,
                         const cd* wc, int64_t ldw, int64_t w_stride, unsigned short* as, cf* zp);
-ecalj_status sig3m_run(ecalj_ctx* h, bool xcd, int nb, int n, int ncol, int nw1, const unsigned short* as,
+ecalj_status sig3m_run(ecalj_ctx* h, bool xcd, int wm, int nb, int n, int ncol, int nw1, const unsigned short* as,
                        const cf* zp, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride);
 ecalj_status sigv4_run(ecalj_ctx* h, bool xcd, int nb, int n, int ncol, int nw1, const unsigned short* as,
                        int64_t as_stride, const cf* zp, const double* wgt, int64_t g_stride, cf* cz,

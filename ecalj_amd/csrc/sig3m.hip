@@ -124,16 +124,21 @@ __global__ __launch_bounds__(256) void pad_z_kernel(const cf* __restrict__ z, in
 // wave tile 64 x 32 (4 x 2 MFMA tiles).  Ak: the k's W planes; zp: its padded
 // zmel (column c at zp + c*NP).  Stage st = ks*nwg + vl carries the W tile of
 // frequency v0+vl and k slice ks.
+// WM: wave rows; the 8 waves form WM x 8/WM, each (128/WM) rows x 16*WM columns
+// (WM = 2: 4 x 2 MFMA tiles per wave; WM = 1: 8 x 1, so one split of the
+// weighted columns feeds twice the MFMAs and each wave reads all 128 rows).
+template <int WM>
 __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __restrict__ Ak,
                                            const cf* __restrict__ zp, const double* __restrict__ g, int n, int nw1,
                                            int v0, int nwg, int ncol, int MP, int NS, int NP, int I0, int c0,
                                            int accumulate, cf* __restrict__ C, int64_t ldc) {
-  constexpr int NT = 512, TM = 4, TN = 2;
+  static_assert(WM == 1 || WM == 2, "one or two wave rows");
+  constexpr int NT = 512, WN = 8 / WM, TM = 8 / WM, TN = WM;
   constexpr int APP = QBM * QBK / 512;        // 1 KB runs per plane
   constexpr int ARW = Q3PL * APP / 8;         // runs per wave and stage
   static_assert(ARW * 8 == Q3PL * APP, "runs split evenly over the waves");
   const int t = threadIdx.x, lane = t & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave >> 2, wn = wave & 3;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave / WN, wn = wave % WN;
   const int64_t apst = (int64_t)MP * QBK;
   const unsigned short* A = Ak + (int64_t)I0 * QBK + lane * 8;
   const int nst = NS * nwg;
@@ -160,7 +165,8 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
   f4 zv[TN][4];
   const cf* zcol[TN];
 #pragma unroll
-  for (int j = 0; j < TN; ++j) zcol[j] = zp + (int64_t)(c0 + wn * 32 + j * 16 + (lane & 15)) * NP + 8 * (lane >> 4);
+  for (int j = 0; j < TN; ++j)
+    zcol[j] = zp + (int64_t)(c0 + wn * 16 * TN + j * 16 + (lane & 15)) * NP + 8 * (lane >> 4);
   auto zload = [&](int ks) {
 #pragma unroll
     for (int j = 0; j < TN; ++j)
@@ -190,7 +196,7 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
   auto make_b = [&](int vl) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      const float w = s.wt[vl][wn * 32 + j * 16 + (lane & 15)];
+      const float w = s.wt[vl][wn * 16 * TN + j * 16 + (lane & 15)];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const f4 z = zv[j][q];
@@ -313,6 +319,7 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
 // p % 8).  XCD x's j-th block belongs to super-tile (j / G)*8 + x of G column
 // tiles of one (k, row tile), so the G blocks that read the same W stages run
 // together on one XCD and share them through its L2.
+template <int WM>
 __global__ __launch_bounds__(512, 1) void sig3m_kernel(const unsigned short* __restrict__ as, int64_t as_stride,
                                                        const cf* __restrict__ zp, int64_t zp_stride,
                                                        const double* __restrict__ wgt, int64_t g_stride, int n,
@@ -335,7 +342,7 @@ __global__ __launch_bounds__(512, 1) void sig3m_kernel(const unsigned short* __r
     c = (rem % NG) * G + loc;
     if (b >= nk || c >= Ct) return;
   }
-  sig3m_tile(s, as + as_stride * b, zp + zp_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS, NP,
+  sig3m_tile<WM>(s, as + as_stride * b, zp + zp_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS, NP,
              r * QBM, c * Q3BN, accumulate, cz + cz_stride * b, (int64_t)n);
 }
 
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(512, 1) void sig3m_grouped_kernel(const int* __rest
     const int it = tiles[3 * t + 1], jt = tiles[3 * t + 2];
     const Sig3Group gk = grp[k];
     __syncthreads();  // the previous tile's LDS reads are done before this tile's loads land
-    sig3m_tile(s, as + as_stride * k, zp + gk.zoff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, NP, it * QBM,
+    sig3m_tile<2>(s, as + as_stride * k, zp + gk.zoff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, NP, it * QBM,
                jt * Q3BN, accumulate, cz + gk.czoff, (int64_t)n);
   }
 }
@@ -614,7 +621,7 @@ static int sig3m_group(int Ct) {
   return (Ct + ng - 1) / ng;
 }
 
-ecalj_status sig3m_run(ecalj_ctx* h, bool xcd, int nb, int n, int ncol, int nw1, const unsigned short* as,
+ecalj_status sig3m_run(ecalj_ctx* h, bool xcd, int wm, int nb, int n, int ncol, int nw1, const unsigned short* as,
                        const cf* zp, const double* wgt, int64_t g_stride, cf* cz, int64_t cz_stride) {
   const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NCP = pad_to(ncol, QBN_MAX), NS = NP / QBK;
   const int R = MP / QBM, Ct = NCP / Q3BN;
@@ -632,8 +639,12 @@ ecalj_status sig3m_run(ecalj_ctx* h, bool xcd, int nb, int n, int ncol, int nw1,
   }
   for (int v0 = 0; v0 < nw1; v0 += Q3NW) {
     const int nwg = std::min(Q3NW, nw1 - v0);
-    hipLaunchKernelGGL(sig3m_kernel, grid, dim3(512), 0, h->stream, as, as_stride, zp, (int64_t)NCP * NP, wgt,
-                       g_stride, n, nw1, v0, nwg, ncol, MP, NS, NP, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G, nb);
+    if (wm == 1)
+      hipLaunchKernelGGL(sig3m_kernel<1>, grid, dim3(512), 0, h->stream, as, as_stride, zp, (int64_t)NCP * NP, wgt,
+                         g_stride, n, nw1, v0, nwg, ncol, MP, NS, NP, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G, nb);
+    else
+      hipLaunchKernelGGL(sig3m_kernel<2>, grid, dim3(512), 0, h->stream, as, as_stride, zp, (int64_t)NCP * NP, wgt,
+                         g_stride, n, nw1, v0, nwg, ncol, MP, NS, NP, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G, nb);
     ECALJ_HIP(hipGetLastError());
   }
   return ECALJ_OK;

@@ -94,7 +94,8 @@ def test_c2_f64(c2):
     _check_f64(_run_uniform(qp, "f64"), ref)
 
 
-@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 4), ("bf16x3", 6), ("bf16x3", 8), ("f32", 0)])
+@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 4), ("bf16x3", 6), ("bf16x3", 8), ("bf16x3", 10),
+                                       ("f32", 0)])
 def test_c2_mixed(c2, prec, mode):
     """Mixed forms; bf16x3 with the tuner's choice and with the frequency-outer
     (4), three-product (6) and four-product frequency-inner (8) Sigma_c

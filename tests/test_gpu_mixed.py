@@ -150,12 +150,12 @@ def test_sigc_mixed(hp, orc, ngb, nm, ntq, niw):
     assert relerr(h.sigc(z, wc, g), orc.sigc_imag(z, wc, g)) < TOL[prec]
 
 
-@pytest.mark.parametrize("mode", [5, 6, 7, 8])
+@pytest.mark.parametrize("mode", [5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10), (161, 13, 5, 40), (1, 3, 1, 2),
                                             (300, 40, 4, 0)])
 def test_sigc_bf3_frequency_inner_forms(orc, mode, ngb, nm, ntq, niw):
     """The frequency-inner variants, forced (5, 6: three-product form; 7, 8:
-    four-product form): ragged and tile-aligned shapes, more than 24
+    four-product form; 9, 10: three-product form, 8 waves side by side): ragged and tile-aligned shapes, more than 24
     frequencies (two launches that accumulate), one frequency, one row."""
     import ecalj_amd
 
@@ -187,7 +187,7 @@ def test_sigc_bf3_tuner_variants(orc):
     h = ecalj_amd.Handle(0, "bf16x3")
     try:
         outs = []
-        for mode in (1, 2, 3, 4, 5, 6, 7, 8):
+        for mode in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
             say("sigc bf16x3 tuner variant", mode)
             h.set_tuning(mode)
             outs.append(h.sigc(z, wc, g))
@@ -196,6 +196,7 @@ def test_sigc_bf3_tuner_variants(orc):
             assert np.array_equal(outs[0], o)
         assert np.array_equal(outs[4], outs[5])
         assert np.array_equal(outs[6], outs[7])
+        assert np.array_equal(outs[8], outs[9])
         h.set_tuning(0)
         say("sigc bf16x3 tuner auto")
         a = h.sigc(z, wc, g)
@@ -204,7 +205,8 @@ def test_sigc_bf3_tuner_variants(orc):
         assert rep.count("sigc_bf3") == 1 and "*" in rep
         chosen = [ln for ln in rep.splitlines() if "sigc_bf3" in ln][0]
         star = chosen[:chosen.index("*")].split()[-3]  # the name before "<ms> ms *"
-        expect = outs[4] if star.startswith("3m") else outs[6] if star.startswith("vin4") else outs[0]
+        expect = (outs[8] if "/w8" in star else outs[4] if star.startswith("3m") else
+                  outs[6] if star.startswith("vin4") else outs[0])
         assert np.array_equal(a, expect) and np.array_equal(b, expect)
     finally:
         h.close()
