@@ -90,6 +90,8 @@ _SIGS = {
     "ecalj_zmel_assemble_dev": [_p, _p, _i, _p, _i, _l, _i, _i, _p],
     "ecalj_tetwt": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _l, _p, _p, _p, _p, _p, _p,
                     _p, _p, _p],
+    "ecalj_tetwt_full": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _i, _p, _i, _l, _p, _p,
+                         _p, _p, _p, _p, _p, _p, _p],
     "ecalj_zminv_dev": [_p, _p, _i, _i, _i, C.c_int64],
     "ecalj_zminv_noalloc_dev": [_p, _p, _i, _i, _i, C.c_int64],
     "ecalj_wc_build_gamma": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
@@ -687,12 +689,17 @@ class Handle:
         nrec = w.nbytes // (16 * ld * ld)
         _chk(lib().ecalj_wv4_read_dev(self._h, str(path).encode(), rec0, nrec, ld, _t_ptr(w)), "wv4_read_dev")
 
-    def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10):
-        """gettetwt (S/m_tetwt.f90:36-256), npm=1: pair lists and histogram weights.
-        mesh: synth.TetraMesh (or any object with idtetf, qbzw, ib1bz, qlat)."""
+    def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10, npm=1, ecore=None):
+        """gettetwt (S/m_tetwt.f90:36-256): pair lists and histogram weights,
+        npm = 1 or 2, with the core states of energies ecore (Ry) appended after
+        the valence bands.  mesh: synth.TetraMesh (or any object with idtetf,
+        qbzw, ib1bz, qlat).  With npm = 1 the lists are (nbnbx, nqbz) and nbnb
+        (nqbz,); with npm = 2 they keep a trailing jpm axis."""
         e1 = np.asfortranarray(ek1, np.float64)
         e2 = np.asfortranarray(ek2, np.float64)
         nband, nqbz = e1.shape
+        ec = np.ascontiguousarray(np.zeros(0) if ecore is None else ecore, np.float64)
+        nctot = len(ec)
         idt = np.asfortranarray(mesh.idtetf, np.int32)
         qw = np.asfortranarray(mesh.qbzw, np.float64)
         ib1 = np.ascontiguousarray(mesh.ib1bz, np.int32)
@@ -700,16 +707,20 @@ class Handle:
         fr = np.ascontiguousarray(frhis, np.float64)
         nbx, nht = C.c_int(0), C.c_int64(0)
         args = (self._h, nband, nqbz, idt.shape[1], _np_ptr(idt), qw.shape[1], _np_ptr(qw), _np_ptr(ib1),
-                _np_ptr(e1), _np_ptr(e2), ef, ebmx, _np_ptr(ql), _np_ptr(fr), len(fr) - 1)
-        rc = lib().ecalj_tetwt(*args, 0, 0, None, None, None, None, None, None, None, C.byref(nbx), C.byref(nht))
+                _np_ptr(e1), _np_ptr(e2), ef, ebmx, _np_ptr(ql), _np_ptr(fr), len(fr) - 1, npm, nctot,
+                _np_ptr(ec) if nctot else None)
+        rc = lib().ecalj_tetwt_full(*args, 0, 0, None, None, None, None, None, None, None, C.byref(nbx),
+                                    C.byref(nht))
         if rc not in (0, 1):
             _chk(rc, "tetwt")
         cap = max(nbx.value, 1)
-        nbnb = np.zeros(nqbz, np.int32)
-        arrs = [np.zeros((cap, nqbz), np.int32, order="F") for _ in range(5)]
+        nbnb = np.zeros((nqbz, npm), np.int32, order="F")
+        arrs = [np.zeros((cap, nqbz, npm), np.int32, order="F") for _ in range(5)]
         whw = np.zeros(max(nht.value, 1))
-        _chk(lib().ecalj_tetwt(*args, cap, len(whw), _np_ptr(nbnb), *[_np_ptr(a) for a in arrs], _np_ptr(whw),
-                               C.byref(nbx), C.byref(nht)), "tetwt")
+        _chk(lib().ecalj_tetwt_full(*args, cap, len(whw), _np_ptr(nbnb), *[_np_ptr(a) for a in arrs],
+                                    _np_ptr(whw), C.byref(nbx), C.byref(nht)), "tetwt")
+        if npm == 1:
+            nbnb, arrs = nbnb[:, 0], [a[:, :, 0] for a in arrs]
         n1b, n2b, ihw, nhw, jhw = arrs
         return dict(nbnb=nbnb, n1b=n1b, n2b=n2b, ihw=ihw, nhw=nhw, jhw=jhw, whw=whw[:nht.value],
                     nbnbx=nbx.value, nhwtot=nht.value)

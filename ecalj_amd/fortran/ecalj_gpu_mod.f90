@@ -17,7 +17,7 @@ module ecalj_gpu
   public :: ecalj_zhgv, ecalj_zhgv_dev
   public :: ecalj_set_ppb_full, ecalj_zmel_init, ecalj_zmel_init_dev
   public :: ecalj_zminv, ecalj_zminv_dev, ecalj_sigc_real, ecalj_sigc_real_dev, ecalj_sigc_full_dev
-  public :: ecalj_tetwt, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
+  public :: ecalj_tetwt, ecalj_tetwt_full, ecalj_set_ppovl, ecalj_zmel_ipw, ecalj_zmel_ipw_dev, ecalj_zmel_assemble_dev
   public :: ecalj_wv_write_dev, ecalj_wv_read_dev, ecalj_wv4_write_dev, ecalj_wv4_read_dev
   public :: ecalj_wc_build_gamma, ecalj_wc_build_gamma_dev, ecalj_eps_head, ecalj_eps_head_dev, ecalj_getw0
   public :: ecalj_wv0_patch, ecalj_wv0_patch_dev
@@ -221,6 +221,22 @@ module ecalj_gpu
       integer(c_int64_t), value :: nhw_cap
       integer(c_int), intent(in) :: idtetf(*), ib1bz(*)
       real(c_double), intent(in) :: qbzw(*), ekxx1(*), ekxx2(*), qlat(*), frhis(*)
+      real(c_double), value :: ef, ebmx
+      integer(c_int), intent(out) :: nbnb(*), n1b(*), n2b(*), ihw(*), nhw(*), jhw(*), nbnbx_out
+      real(c_double), intent(out) :: whw(*)
+      integer(c_int64_t), intent(out) :: nhwtot_out
+    end function
+    ! gettetwt with npm = 1 or 2 and nctot core states ecore(1:nctot, is) (S/m_tetwt.f90:36-256,
+    ! S/tetwt5.f90:179-185, 252-264, 349-427); lists (nbnbx_cap, nqbz, npm), nbnb(nqbz, npm)
+    integer(c_int) function ecalj_tetwt_full(h, nband, nqbz, ntetf, idtetf, nqbzw, qbzw, ib1bz, ekxx1, ekxx2, ef, &
+        ebmx, qlat, frhis, nwhis, npm, nctot, ecore, nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, jhw, whw, &
+        nbnbx_out, nhwtot_out) bind(C, name="ecalj_tetwt_full")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: nband, nqbz, ntetf, nqbzw, nwhis, npm, nctot, nbnbx_cap
+      integer(c_int64_t), value :: nhw_cap
+      integer(c_int), intent(in) :: idtetf(*), ib1bz(*)
+      real(c_double), intent(in) :: qbzw(*), ekxx1(*), ekxx2(*), qlat(*), frhis(*), ecore(*)
       real(c_double), value :: ef, ebmx
       integer(c_int), intent(out) :: nbnb(*), n1b(*), n2b(*), ihw(*), nhw(*), jhw(*), nbnbx_out
       real(c_double), intent(out) :: whw(*)

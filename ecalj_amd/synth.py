@@ -528,11 +528,15 @@ def pair_weights(qp: QPoint, fr: np.ndarray, nu_max: int = 0) -> PairWeights:
     return PairWeights(n1b, n2b, ihw, nhw, jhw, np.concatenate(whw_parts))
 
 
-def pair_weights_from_tetwt(r) -> PairWeights:
-    """The tetrahedron weights (gettetwt's n1b/n2b/ihw/nhw/jhw/whw, per k) as
-    PairWeights for x0kf_index."""
-    nq = len(r["nbnb"])
-    sl = lambda a, k: np.ascontiguousarray(a[:r["nbnb"][k], k], np.int32)
+def pair_weights_from_tetwt(r, jpm=0) -> PairWeights:
+    """The tetrahedron weights (gettetwt's n1b/n2b/ihw/nhw/jhw/whw, per k) of
+    one jpm as PairWeights for x0kf_index (jhw keeps pointing into the whole
+    whw)."""
+    nbnb = np.asarray(r["nbnb"])
+    pick = (lambda a: a) if nbnb.ndim == 1 else (lambda a: a[..., jpm])
+    nb = pick(nbnb)
+    nq = len(nb)
+    sl = lambda a, k: np.ascontiguousarray(pick(a)[:nb[k], k], np.int32)
     return PairWeights([sl(r["n1b"], k) for k in range(nq)], [sl(r["n2b"], k) for k in range(nq)],
                        [sl(r["ihw"], k) for k in range(nq)], [sl(r["nhw"], k) for k in range(nq)],
                        [sl(r["jhw"], k) for k in range(nq)], np.asarray(r["whw"], np.float64))

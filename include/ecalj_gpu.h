@@ -306,6 +306,23 @@ ecalj_status ecalj_tetwt(ecalj_handle h, int nband, int nqbz, int ntetf, const i
                          const double* frhis, int nwhis, int nbnbx_cap, int64_t nhw_cap,
                          int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw,
                          double* whw, int* nbnbx_out, int64_t* nhwtot_out);
+/* gettetwt in full (S/m_tetwt.f90:36-256 with its module inputs npm from
+ * m_freq and nctot, ecore(:,is) from m_genallcf_v3; S/tetwt5.f90:179-185,
+ * 252-264 for the core states, 349-427 for their degenerate groups):
+ * npm = 2 adds the jpm = 2 pairs (a state at k, an occupied one at k+q) of
+ * the time-reversal-off chi0; nctot core states of energies ecore (Ry) are
+ * appended after the valence bands, flat in k, on the occupied side
+ * (n1b up to nband + nctot; with npm = 2, n2b up to nband + nctot for
+ * jpm = 2).  Outputs gain a trailing jpm axis: nbnb(nqbz, npm); n1b, n2b,
+ * ihw, nhw, jhw (nbnbx_cap, nqbz, npm); whw pointers run over jpm, then k,
+ * then the pairs.  ecalj_tetwt is this call with npm = 1, nctot = 0. */
+ecalj_status ecalj_tetwt_full(ecalj_handle h, int nband, int nqbz, int ntetf, const int* idtetf,
+                              int nqbzw, const double* qbzw, const int* ib1bz,
+                              const double* ekxx1, const double* ekxx2, double ef, double ebmx,
+                              const double* qlat, const double* frhis, int nwhis, int npm,
+                              int nctot, const double* ecore, int nbnbx_cap, int64_t nhw_cap,
+                              int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw,
+                              double* whw, int* nbnbx_out, int64_t* nhwtot_out);
 
 /* ---- chi0: Hilbert transform to the imaginary axis ---------------------
  * dpsion_chiq, imaginary branch with npm=1 (S/dpsion5.f90:211-232) with

@@ -215,36 +215,43 @@ def chkdgn(ene):
     return a[:n], b[:n]
 
 
-def tetwt(mesh, ek1, ek2, ef, frhis_, ebmx=1e10):
-    """gettetwt for npm=1: returns a PairWeights-like dict of the reference's
-    index sets (1-based values, per k) and whw.  mesh: synth.TetraMesh."""
+def tetwt(mesh, ek1, ek2, ef, frhis_, ebmx=1e10, npm=1, ecore=None):
+    """gettetwt (npm = 1 or 2, core states of energies ecore appended after
+    the nband valence bands): the reference's index sets (1-based values) and
+    whw.  mesh: synth.TetraMesh.  With npm = 1 the arrays are (nbnbx, nqbz)
+    and nbnb (nqbz,); with npm = 2 a trailing jpm axis is kept."""
     ek1 = _cd(ek1)
     ek2 = _cd(ek2)
     nband, nqbz = ek1.shape
     fr = _cd(frhis_)
     nwhis = len(fr) - 1
+    ec = np.ascontiguousarray(np.zeros(0) if ecore is None else ecore, np.float64)
+    nctot = len(ec)
     idt = np.asfortranarray(mesh.idtetf, np.int32)
     qw = np.asfortranarray(mesh.qbzw, np.float64)
     ib1 = np.ascontiguousarray(mesh.ib1bz, np.int32)
     ql = np.asfortranarray(mesh.qlat, np.float64)
     cap_b, cap_h = 1, 1
     for _ in range(3):  # sizes: nbnbx first, then nhwtot
-        nbnb = np.zeros(nqbz, np.int32)
-        arrs = [np.zeros((cap_b, nqbz), np.int32, order="F") for _ in range(5)]
+        nbnb = np.zeros((nqbz, npm), np.int32, order="F")
+        arrs = [np.zeros((cap_b, nqbz, npm), np.int32, order="F") for _ in range(5)]
         whw = np.zeros(cap_h)
         nbx, nht = C.c_int(0), C.c_int(0)
-        rc = lib().orc_tetwt(nband, nqbz, idt.shape[1], _ip_(idt), _p(qw), _ip_(ib1), _p(ek1), _p(ek2),
-                             C.c_double(ef), C.c_double(ebmx), _p(ql), _p(fr), nwhis, cap_b, cap_h, _ip_(nbnb),
-                             *[_ip_(a) for a in arrs], _p(whw), C.byref(nbx), C.byref(nht))
+        rc = lib().orc_tetwt_full(nband, nqbz, idt.shape[1], _ip_(idt), _p(qw), _ip_(ib1), _p(ek1), _p(ek2),
+                                  C.c_double(ef), C.c_double(ebmx), _p(ql), _p(fr), nwhis, npm, nctot,
+                                  _p(ec) if nctot else None, cap_b, cap_h, _ip_(nbnb), *[_ip_(a) for a in arrs],
+                                  _p(whw), C.byref(nbx), C.byref(nht))
         if rc == -1:
             cap_b, cap_h = max(nbx.value, 1), max(nht.value, 1)
             continue
         if rc != 0:
-            raise ValueError(f"orc_tetwt rc={rc}")
+            raise ValueError(f"orc_tetwt_full rc={rc}")
+        if npm == 1:
+            nbnb, arrs = nbnb[:, 0], [a[:, :, 0] for a in arrs]
         n1b, n2b, ihw, nhw, jhw = arrs
         return dict(nbnb=nbnb, n1b=n1b, n2b=n2b, ihw=ihw, nhw=nhw, jhw=jhw, whw=whw[:nht.value],
                     nbnbx=nbx.value, nhwtot=nht.value)
-    raise RuntimeError("orc_tetwt: capacity loop did not converge")
+    raise RuntimeError("orc_tetwt_full: capacity loop did not converge")
 
 
 def zmel_ipw(case, branch=0):

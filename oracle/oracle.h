@@ -230,6 +230,18 @@ int orc_tetwt(int nband, int nqbz, int ntetf, const int* idtetf, const double* q
               const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
               const double* frhis, int nwhis, int nbnbx_cap, int nhw_cap, int* nbnb, int* n1b, int* n2b,
               int* ihw, int* nhw, int* jhw, double* whw, int* nbnbx_out, int* nhwtot_out);
+/* gettetwt with npm = 1 or 2 (time reversal off: jpm = 2 pairs are unoccupied
+ * at k and occupied at k+q) and nctot core states of energies ecore (Ry),
+ * appended after the valence bands at every k (S/tetwt5.f90:179-185,
+ * 252-264, 349-427).  Per (k, jpm) outputs: nbnb(nqbz, npm); n1b, n2b, ihw,
+ * nhw, jhw (nbnbx_cap, nqbz, npm); whw with its pointers running over jpm,
+ * then k, then the pairs.  n1b ranges over nband + nctot, n2b over
+ * nband + (npm == 2 ? nctot : 0).  orc_tetwt is npm = 1 without core. */
+int orc_tetwt_full(int nband, int nqbz, int ntetf, const int* idtetf, const double* qbzw, const int* ib1bz,
+                   const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
+                   const double* frhis, int nwhis, int npm, int nctot, const double* ecore, int nbnbx_cap,
+                   int nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw, double* whw,
+                   int* nbnbx_out, int* nhwtot_out);
 /* lindtet6 (S/tetwt5.f90:514-600): histogram of one tetrahedron; kvec[corner][xyz] */
 int orc_lindtet6(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
                  double efermia, double efermib, const double* frhis, int nwhis, double* wtthis);

@@ -4,8 +4,8 @@
  * checker of the HIP implementation, loaded only by tests/, smoke() and the
  * cpu_baseline leg of bench.py.
  *
- * Scope (the reference's default path): npm = 1 (time reversal), no core
- * states (nctot = 0), nmtet = 1, tetrahedron_matrix_linear = F, no tetrakbt,
+ * Scope: npm = 1 (time reversal) or 2, nctot core states, nmtet = 1,
+ * tetrahedron_matrix_linear = F, no tetrakbt,
  * no --interbandonly/--intrabandonly filters, no Fermi-level shift.
  *
  * References (SRC/subroutines = S/):
@@ -263,18 +263,45 @@ int orc_chkdgn(const double* ene, int ndat, int* ixini, int* ixend) {
   return i;
 }
 
-#define IDX3(ib, jb, k) ((size_t)(ib) + (size_t)nband * ((size_t)(jb) + (size_t)nband * (size_t)(k)))
+/* Band index of pair coordinate (ibx, jbx) for jpm (0: occupied at k, unoccupied at k+q; 1: the reverse),
+ * S/tetwt5.f90:252-264: beyond the valence bands below E_F come the core states (index nband + c). */
+static void pair_bands(int jpm, int ibx, int jbx, int nocc_k, int nocc_kq, int nband, int* ib, int* jb) {
+  if (jpm == 0) {
+    *ib = ibx < nocc_k ? ibx : ibx - nocc_k + nband;
+    *jb = jbx;
+  } else {
+    *ib = ibx;
+    *jb = jbx < nocc_kq ? jbx : jbx - nocc_kq + nband;
+  }
+}
 
-int orc_tetwt(int nband, int nqbz, int ntetf, const int* idtetf, const double* qbzw, const int* ib1bz,
-              const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
-              const double* frhis, int nwhis, int nbnbx_cap, int nhw_cap, int* nbnb, int* n1b, int* n2b,
-              int* ihw, int* nhw, int* jhw, double* whw, int* nbnbx_out, int* nhwtot_out) {
+/* IDX4: ib over nband + nctot, jb over nband + ncc, then k, then jpm */
+#define IDX4(ib, jb, k, jpm) \
+  ((size_t)(ib) + (size_t)n1 * ((size_t)(jb) + (size_t)n2 * ((size_t)(k) + (size_t)nqbz * (size_t)(jpm))))
+
+int orc_tetwt_full(int nband, int nqbz, int ntetf, const int* idtetf, const double* qbzw, const int* ib1bz,
+                   const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
+                   const double* frhis, int nwhis, int npm, int nctot, const double* ecore, int nbnbx_cap,
+                   int nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw, double* whw,
+                   int* nbnbx_out, int* nhwtot_out) {
+  if (npm < 1 || npm > 2 || nctot < 0 || (nctot > 0 && !ecore)) return -5;
   double qb[3][3];
   for (int c = 0; c < 3; ++c)
     for (int r = 0; r < 3; ++r) qb[c][r] = qlat[r + 3 * c];
   const double voltot = fabs(det33(qb));
   const double piofvoltot = PI / 4.0 / voltot;
-  const size_t nbb = (size_t)nband * nband * nqbz;
+  const int ncc = npm == 2 ? nctot : 0, n1 = nband + nctot, n2 = nband + ncc;
+  /* energies with the core states appended at every k (S/tetwt5.f90:179-185) */
+  double* e1 = (double*)malloc(sizeof(double) * (size_t)n1 * nqbz);
+  double* e2 = (double*)malloc(sizeof(double) * (size_t)n1 * nqbz);
+  for (int k = 0; k < nqbz; ++k) {
+    for (int b = 0; b < nband; ++b) {
+      e1[b + (size_t)n1 * k] = ek1[b + (size_t)nband * k];
+      e2[b + (size_t)n1 * k] = ek2[b + (size_t)nband * k];
+    }
+    for (int c = 0; c < nctot; ++c) e1[nband + c + (size_t)n1 * k] = e2[nband + c + (size_t)n1 * k] = ecore[c];
+  }
+  const size_t nbb = (size_t)n1 * n2 * nqbz * npm;
   unsigned char* iwgt = (unsigned char*)calloc(nbb, 1);
   double* demin = (double*)malloc(sizeof(double) * nbb);
   double* demax = (double*)malloc(sizeof(double) * nbb);
@@ -293,154 +320,190 @@ int orc_tetwt(int nband, int nqbz, int ntetf, const int* idtetf, const double* q
         kk[c] = ib1bz[w] - 1;
         for (int d = 0; d < 3; ++d) kvec[c][d] = qbzw[d + 3 * w];
       }
-      int noccx_k = 0, nbmx_kq = nband;
+      /* valence bands below E_F (max over corners) and below the cutoff (min over corners) */
+      int nocc_k = 0, nocc_kq = 0, nbmx_k = nband, nbmx_kq = nband;
       for (int c = 0; c < 4; ++c) {
-        int n1 = 0, n2 = 0;
+        int o1 = 0, o2 = 0, m1 = 0, m2 = 0;
         for (int b = 0; b < nband; ++b) {
-          n1 += ek1[b + (size_t)nband * kk[c]] < ef;
-          n2 += ek2[b + (size_t)nband * kk[c]] < fmin(1e10, ebmx);
+          o1 += ek1[b + (size_t)nband * kk[c]] < ef;
+          o2 += ek2[b + (size_t)nband * kk[c]] < ef;
+          m1 += ek1[b + (size_t)nband * kk[c]] < fmin(1e10, ebmx);
+          m2 += ek2[b + (size_t)nband * kk[c]] < fmin(1e10, ebmx);
         }
-        if (n1 > noccx_k) noccx_k = n1;
-        if (n2 < nbmx_kq) nbmx_kq = n2;
+        if (o1 > nocc_k) nocc_k = o1;
+        if (o2 > nocc_kq) nocc_kq = o2;
+        if (m1 < nbmx_k) nbmx_k = m1;
+        if (m2 < nbmx_kq) nbmx_kq = m2;
       }
-      for (int ib = 0; ib < noccx_k; ++ib)
-        for (int jb = 0; jb < nbmx_kq; ++jb) {
-          double eocc[4], eun[4], x[4];
-          double mino = 1e300, maxu = -1e300, maxd = -1e300;
-          for (int c = 0; c < 4; ++c) {
-            eocc[c] = ek1[ib + (size_t)nband * kk[c]];
-            eun[c] = ek2[jb + (size_t)nband * kk[c]];
-            mino = fmin(mino, eocc[c]);
-            maxu = fmax(maxu, eun[c]);
-            maxd = fmax(maxd, eun[c] - eocc[c]);
-          }
-          if (!(mino <= ef && maxu >= ef)) continue;
-          if (maxd < 0.0) continue;
-          for (int c = 0; c < 4; ++c) x[c] = 0.5 * (eocc[c] - eun[c]);
-          if (job == 0) {
-            double dmx = -x[0], dmn = -x[0];
-            for (int c = 1; c < 4; ++c) {
-              dmx = fmax(dmx, -x[c]);
-              dmn = fmin(dmn, -x[c]);
-            }
+      for (int jpm = 0; jpm < npm && rc == 0; ++jpm) {
+        const int ni = jpm == 0 ? nocc_k + nctot : nbmx_k, nj = jpm == 0 ? nbmx_kq : nocc_kq + nctot;
+        for (int ibx = 0; ibx < ni && rc == 0; ++ibx)
+          for (int jbx = 0; jbx < nj; ++jbx) {
+            int ib, jb;
+            pair_bands(jpm, ibx, jbx, nocc_k, nocc_kq, nband, &ib, &jb);
+            double eocc[4], eun[4], x[4];
+            double mino = 1e300, maxu = -1e300, maxd = -1e300;
             for (int c = 0; c < 4; ++c) {
-              const size_t e = IDX3(ib, jb, kk[c]);
-              iwgt[e] = 1;
-              if (dmx > demax[e]) demax[e] = dmx;
-              if (dmn < demin[e]) demin[e] = dmn;
+              const double a = e1[ib + (size_t)n1 * kk[c]], b = e2[jb + (size_t)n1 * kk[c]];
+              eocc[c] = jpm == 0 ? a : b;
+              eun[c] = jpm == 0 ? b : a;
+              mino = fmin(mino, eocc[c]);
+              maxu = fmax(maxu, eun[c]);
+              maxd = fmax(maxd, eun[c] - eocc[c]);
             }
-            continue;
-          }
-          double* wt = (double*)calloc(nwhis, sizeof(double));
-          if (orc_lindtet6((const double(*)[3])kvec, eocc, eun, x, ef, ef, frhis, nwhis, wt)) {
-            rc = -3;
-            free(wt);
-            break;
-          }
-          for (int c = 0; c < 4; ++c) {
-            const int ibib = ibjb[IDX3(ib, jb, kk[c])] - 1;
-            const size_t o = (size_t)ibib + (size_t)nbnbx_cap * kk[c];
-            const int jini = jhw[o] - 1, iini = ihw[o] - 1, nnn = nhw[o];
-            for (int i = 0; i < nnn; ++i) whw[jini + i] += wt[iini + i] * piofvoltot * 4.0 * 0.25;
-          }
-          free(wt);
-        }
-    }
-    if (rc) break;
-    /* symmetrisation over degenerate bands */
-    int* i1 = (int*)malloc(sizeof(int) * 4 * nband);
-    int* e1 = i1 + nband;
-    int* i2 = e1 + nband;
-    int* e2 = i2 + nband;
-    for (int kx = 0; kx < nqbz && rc == 0; ++kx) {
-      const int nr1 = orc_chkdgn(ek1 + (size_t)nband * kx, nband, i1, e1);
-      const int nr2 = orc_chkdgn(ek2 + (size_t)nband * kx, nband, i2, e2);
-      for (int r1 = 0; r1 < nr1 && rc == 0; ++r1)
-        for (int r2 = 0; r2 < nr2; ++r2) {
-          const int a0 = i1[r1] - 1, a1 = e1[r1] - 1, b0 = i2[r2] - 1, b1 = e2[r2] - 1;
-          if (job == 0) {
-            unsigned char w = 0;
-            double mx = -1e300, mn = 1e300;
-            for (int a = a0; a <= a1; ++a)
-              for (int b = b0; b <= b1; ++b) {
-                w |= iwgt[IDX3(a, b, kx)];
-                mx = fmax(mx, demax[IDX3(a, b, kx)]);
-                mn = fmin(mn, demin[IDX3(a, b, kx)]);
+            if (!(mino <= ef && maxu >= ef)) continue;
+            if (maxd < 0.0) continue;
+            for (int c = 0; c < 4; ++c) x[c] = 0.5 * (eocc[c] - eun[c]);
+            if (job == 0) {
+              double dmx = -x[0], dmn = -x[0];
+              for (int c = 1; c < 4; ++c) {
+                dmx = fmax(dmx, -x[c]);
+                dmn = fmin(dmn, -x[c]);
               }
-            for (int a = a0; a <= a1; ++a)
-              for (int b = b0; b <= b1; ++b) {
-                iwgt[IDX3(a, b, kx)] = w;
-                demax[IDX3(a, b, kx)] = mx;
-                demin[IDX3(a, b, kx)] = mn;
+              for (int c = 0; c < 4; ++c) {
+                const size_t e = IDX4(ib, jb, kk[c], jpm);
+                iwgt[e] = 1;
+                if (dmx > demax[e]) demax[e] = dmx;
+                if (dmn < demin[e]) demin[e] = dmn;
               }
-          } else {
-            int isum = 0;
-            for (int a = a0; a <= a1; ++a)
-              for (int b = b0; b <= b1; ++b) isum += ibjb[IDX3(a, b, kx)] != 0;
-            const int bs = (a1 - a0 + 1) * (b1 - b0 + 1);
-            if (isum != 0 && isum != bs) {
-              rc = -4;
+              continue;
+            }
+            double* wt = (double*)calloc(nwhis, sizeof(double));
+            if (orc_lindtet6((const double(*)[3])kvec, eocc, eun, x, ef, ef, frhis, nwhis, wt)) {
+              rc = -3;
+              free(wt);
               break;
             }
-            if (isum == 0 || bs == 1) continue;
-            double* wt = (double*)calloc(nwhis, sizeof(double));
-            for (int a = a0; a <= a1; ++a)
-              for (int b = b0; b <= b1; ++b) {
-                const size_t o = (size_t)(ibjb[IDX3(a, b, kx)] - 1) + (size_t)nbnbx_cap * kx;
-                for (int i = 0; i < nhw[o]; ++i) wt[ihw[o] - 1 + i] += whw[jhw[o] - 1 + i];
-              }
-            for (int i = 0; i < nwhis; ++i) wt[i] /= bs;
-            for (int a = a0; a <= a1; ++a)
-              for (int b = b0; b <= b1; ++b) {
-                const size_t o = (size_t)(ibjb[IDX3(a, b, kx)] - 1) + (size_t)nbnbx_cap * kx;
-                for (int i = 0; i < nhw[o]; ++i) whw[jhw[o] - 1 + i] = wt[ihw[o] - 1 + i];
-              }
+            for (int c = 0; c < 4; ++c) {
+              const int ibib = ibjb[IDX4(ib, jb, kk[c], jpm)] - 1;
+              const size_t o = (size_t)ibib + (size_t)nbnbx_cap * (kk[c] + (size_t)nqbz * jpm);
+              const int jini = jhw[o] - 1, iini = ihw[o] - 1, nnn = nhw[o];
+              for (int i = 0; i < nnn; ++i) whw[jini + i] += wt[iini + i] * piofvoltot * 4.0 * 0.25;
+            }
             free(wt);
           }
-        }
+      }
+    }
+    if (rc) break;
+    /* symmetrisation over degenerate bands (S/tetwt5.f90:349-427): valence groups of each side,
+     * plus the core groups on the occupied side (jpm = 0: k side; jpm = 1: k+q side) */
+    int* i1 = (int*)malloc(sizeof(int) * 4 * (size_t)n1);
+    int* e1g = i1 + n1;
+    int* i2 = e1g + n1;
+    int* e2g = i2 + n1;
+    int ncr = 0;
+    int* ic = (int*)malloc(sizeof(int) * 2 * (size_t)(nctot > 0 ? nctot : 1));
+    int* ec = ic + (nctot > 0 ? nctot : 1);
+    if (nctot) ncr = orc_chkdgn(ecore, nctot, ic, ec);
+    for (int kx = 0; kx < nqbz && rc == 0; ++kx) {
+      const int nr1 = orc_chkdgn(ek1 + (size_t)nband * kx, nband, i1, e1g);
+      const int nr2 = orc_chkdgn(ek2 + (size_t)nband * kx, nband, i2, e2g);
+      for (int jpm = 0; jpm < npm && rc == 0; ++jpm) {
+        const int g1 = nr1 + (jpm == 0 ? ncr : 0), g2 = nr2 + (jpm == 1 ? ncr : 0);
+        for (int r1 = 0; r1 < g1 && rc == 0; ++r1)
+          for (int r2 = 0; r2 < g2; ++r2) {
+            const int a0 = r1 < nr1 ? i1[r1] - 1 : nband + ic[r1 - nr1] - 1;
+            const int a1 = r1 < nr1 ? e1g[r1] - 1 : nband + ec[r1 - nr1] - 1;
+            const int b0 = r2 < nr2 ? i2[r2] - 1 : nband + ic[r2 - nr2] - 1;
+            const int b1 = r2 < nr2 ? e2g[r2] - 1 : nband + ec[r2 - nr2] - 1;
+            if (job == 0) {
+              unsigned char w = 0;
+              double mx = -1e300, mn = 1e300;
+              for (int a = a0; a <= a1; ++a)
+                for (int b = b0; b <= b1; ++b) {
+                  w |= iwgt[IDX4(a, b, kx, jpm)];
+                  mx = fmax(mx, demax[IDX4(a, b, kx, jpm)]);
+                  mn = fmin(mn, demin[IDX4(a, b, kx, jpm)]);
+                }
+              for (int a = a0; a <= a1; ++a)
+                for (int b = b0; b <= b1; ++b) {
+                  iwgt[IDX4(a, b, kx, jpm)] = w;
+                  demax[IDX4(a, b, kx, jpm)] = mx;
+                  demin[IDX4(a, b, kx, jpm)] = mn;
+                }
+            } else {
+              int isum = 0;
+              for (int a = a0; a <= a1; ++a)
+                for (int b = b0; b <= b1; ++b) isum += ibjb[IDX4(a, b, kx, jpm)] != 0;
+              const int bs = (a1 - a0 + 1) * (b1 - b0 + 1);
+              if (isum != 0 && isum != bs) {
+                rc = -4;
+                break;
+              }
+              if (isum == 0 || bs == 1) continue;
+              double* wt = (double*)calloc(nwhis, sizeof(double));
+              for (int a = a0; a <= a1; ++a)
+                for (int b = b0; b <= b1; ++b) {
+                  const size_t o = (size_t)(ibjb[IDX4(a, b, kx, jpm)] - 1) + (size_t)nbnbx_cap * (kx + (size_t)nqbz * jpm);
+                  for (int i = 0; i < nhw[o]; ++i) wt[ihw[o] - 1 + i] += whw[jhw[o] - 1 + i];
+                }
+              for (int i = 0; i < nwhis; ++i) wt[i] /= bs;
+              for (int a = a0; a <= a1; ++a)
+                for (int b = b0; b <= b1; ++b) {
+                  const size_t o = (size_t)(ibjb[IDX4(a, b, kx, jpm)] - 1) + (size_t)nbnbx_cap * (kx + (size_t)nqbz * jpm);
+                  for (int i = 0; i < nhw[o]; ++i) whw[jhw[o] - 1 + i] = wt[ihw[o] - 1 + i];
+                }
+              free(wt);
+            }
+          }
+      }
     }
     free(i1);
+    free(ic);
     if (rc || job == 1) break;
-    /* index sets: nbnb, n1b/n2b (rsvwwk00_4), ihw/nhw (hisrange), jhw, ibjb */
+    /* index sets per (k, jpm): nbnb, n1b/n2b (rsvwwk00_4: ib-major), ihw/nhw (hisrange), jhw, ibjb */
     int nbnbx = 1;
-    for (int kx = 0; kx < nqbz; ++kx) {
-      int c = 0;
-      for (int ib = 0; ib < nband; ++ib)
-        for (int jb = 0; jb < nband; ++jb) c += iwgt[IDX3(ib, jb, kx)];
-      nbnb[kx] = c;
-      if (c > nbnbx) nbnbx = c;
-    }
+    for (int jpm = 0; jpm < npm; ++jpm)
+      for (int kx = 0; kx < nqbz; ++kx) {
+        int c = 0;
+        for (int ib = 0; ib < n1; ++ib)
+          for (int jb = 0; jb < n2; ++jb) c += iwgt[IDX4(ib, jb, kx, jpm)];
+        nbnb[kx + (size_t)nqbz * jpm] = c;
+        if (c > nbnbx) nbnbx = c;
+      }
     *nbnbx_out = nbnbx;
     if (nbnbx > nbnbx_cap) {
       rc = -1;
       break;
     }
-    int jtot = 1;
-    for (int kx = 0; kx < nqbz && rc == 0; ++kx) {
-      int ix = 0;
-      for (int ib = 0; ib < nband; ++ib)
-        for (int jb = 0; jb < nband; ++jb)
-          if (iwgt[IDX3(ib, jb, kx)]) {
-            const size_t o = (size_t)ix + (size_t)nbnbx_cap * kx;
-            n1b[o] = ib + 1;
-            n2b[o] = jb + 1;
-            if (orc_hisrange(frhis, nwhis, demin[IDX3(ib, jb, kx)], demax[IDX3(ib, jb, kx)], &ihw[o], &nhw[o])) {
-              rc = -2;
-              break;
+    int jtot = 1; /* whw pointers run over jpm, then k, then the pairs (S/m_tetwt.f90:182-196) */
+    for (int jpm = 0; jpm < npm && rc == 0; ++jpm)
+      for (int kx = 0; kx < nqbz && rc == 0; ++kx) {
+        int ix = 0;
+        for (int ib = 0; ib < n1 && rc == 0; ++ib)
+          for (int jb = 0; jb < n2; ++jb)
+            if (iwgt[IDX4(ib, jb, kx, jpm)]) {
+              const size_t o = (size_t)ix + (size_t)nbnbx_cap * (kx + (size_t)nqbz * jpm);
+              n1b[o] = ib + 1;
+              n2b[o] = jb + 1;
+              if (orc_hisrange(frhis, nwhis, demin[IDX4(ib, jb, kx, jpm)], demax[IDX4(ib, jb, kx, jpm)], &ihw[o],
+                               &nhw[o])) {
+                rc = -2;
+                break;
+              }
+              jhw[o] = jtot;
+              jtot += nhw[o];
+              ibjb[IDX4(ib, jb, kx, jpm)] = ix + 1;
+              ++ix;
             }
-            jhw[o] = jtot;
-            jtot += nhw[o];
-            ibjb[IDX3(ib, jb, kx)] = ix + 1;
-            ++ix;
-          }
-    }
+      }
     *nhwtot_out = jtot - 1;
     if (rc == 0 && jtot - 1 > nhw_cap) rc = -1;
     if (rc == 0) memset(whw, 0, sizeof(double) * (size_t)(jtot - 1));
   }
+  free(e1);
+  free(e2);
   free(iwgt);
   free(demin);
   free(demax);
   free(ibjb);
   return rc;
+}
+
+int orc_tetwt(int nband, int nqbz, int ntetf, const int* idtetf, const double* qbzw, const int* ib1bz,
+              const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
+              const double* frhis, int nwhis, int nbnbx_cap, int nhw_cap, int* nbnb, int* n1b, int* n2b,
+              int* ihw, int* nhw, int* jhw, double* whw, int* nbnbx_out, int* nhwtot_out) {
+  return orc_tetwt_full(nband, nqbz, ntetf, idtetf, qbzw, ib1bz, ek1, ek2, ef, ebmx, qlat, frhis, nwhis, 1, 0, NULL,
+                        nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, jhw, whw, nbnbx_out, nhwtot_out);
 }

@@ -1,5 +1,6 @@
-"""Tetrahedron histogram weights on the GPU (ecalj_tetwt) against the oracle
-(orc_tetwt, pinned in test_tetwt_oracle.py): index maps bit-exact (n1b, n2b,
+"""Tetrahedron histogram weights on the GPU (ecalj_tetwt, ecalj_tetwt_full
+with npm = 2 and core states) against the oracle (orc_tetwt_full, pinned in
+test_tetwt_oracle.py): index maps bit-exact (n1b, n2b,
 ihw, nhw, jhw, nbnb).  Weights: each (k, pair) total within 1e-12 (the bin
 sums of a piece telescope to its volume), single bins within 1e-8 of the
 largest weight.  The per-bin bar is wide because the reference's integtetn
@@ -49,11 +50,40 @@ def test_tetwt_vs_oracle(h, orc, n, nocc, nun, seed, degenerate):
 
 def _check_weights(g, o):
     assert np.abs(g["whw"] - o["whw"]).max() <= 1e-8 * np.abs(o["whw"]).max()
-    for k in range(len(o["nbnb"])):
-        for i in range(o["nbnb"][k]):
-            j0, nh = o["jhw"][i, k] - 1, o["nhw"][i, k]
-            tg, to = g["whw"][j0:j0 + nh].sum(), o["whw"][j0:j0 + nh].sum()
-            assert abs(tg - to) <= 1e-12 * max(abs(to), 1e-3)
+    nbnb = o["nbnb"].reshape(len(o["nbnb"]), -1)
+    jhw = o["jhw"].reshape(o["jhw"].shape[0], o["jhw"].shape[1], -1)
+    nhw = o["nhw"].reshape(jhw.shape)
+    for j in range(nbnb.shape[1]):
+        for k in range(nbnb.shape[0]):
+            for i in range(nbnb[k, j]):
+                j0, nh = jhw[i, k, j] - 1, nhw[i, k, j]
+                tg, to = g["whw"][j0:j0 + nh].sum(), o["whw"][j0:j0 + nh].sum()
+                assert abs(tg - to) <= 1e-12 * max(abs(to), 1e-3)
+
+
+@pytest.mark.parametrize("n,nocc,nun,seed,npm,ncore", [((2, 2, 2), 4, 5, 11, 2, 0), ((3, 3, 3), 6, 20, 12, 1, 3),
+                                                       ((3, 3, 3), 6, 20, 13, 2, 3), ((4, 3, 2), 7, 14, 14, 2, 1)])
+def test_tetwt_npm_core_vs_oracle(h, orc, n, nocc, nun, seed, npm, ncore):
+    """npm = 2 (time reversal off) and core states (a degenerate pair among
+    them): index maps bit-exact, weights as above."""
+    say("tetwt_full", n, nocc, nun, seed, "npm", npm, "core", ncore)
+    m = synth.tetra_mesh(*n)
+    ek1, ek2, ef = synth.tetra_bands(m, nocc, nun, seed=seed)
+    ecore = np.array([-32.0, -32.0, -19.5, -25.0][:ncore]) / synth.RYDBERG_EV
+    ecore = np.sort(ecore) if ncore else None
+    full = ek1 if ecore is None else np.vstack([np.repeat(ecore[:, None], ek1.shape[1], 1), ek1, ek2])
+    fr = synth.tetra_frhis(full, ek2)
+    g = h.tetwt(m, ek1, ek2, ef, fr, npm=npm, ecore=ecore)
+    o = orc.tetwt(m, ek1, ek2, ef, fr, npm=npm, ecore=ecore)
+    assert g["nbnbx"] == o["nbnbx"] and g["nhwtot"] == o["nhwtot"]
+    np.testing.assert_array_equal(g["nbnb"], o["nbnb"])
+    for key in ("n1b", "n2b", "ihw", "nhw", "jhw"):
+        np.testing.assert_array_equal(g[key], o[key], err_msg=key)
+    if ncore:
+        assert o["n1b"].max() > ek1.shape[0]          # core pairs present
+        if npm == 2:
+            assert o["n2b"][..., 1].max() > ek1.shape[0]
+    _check_weights(g, o)
 
 
 def test_tetwt_band_cutoff(h, orc):
@@ -86,4 +116,24 @@ def test_tetwt_feeds_x0gemm(h, orc):
     for a, b in zip((ixg.itc, ixg.itpc, ixg.iwini, ixg.iwend, ixg.icouini), (ixo.itc, ixo.itpc, ixo.iwini,
                                                                              ixo.iwend, ixo.icouini)):
         np.testing.assert_array_equal(a, b)
+    assert np.abs(ixg.whwc - ixo.whwc).max() <= 1e-8 * np.abs(ixo.whwc).max()
+
+
+def test_tetwt_npm2_core_feeds_x0kf(h, orc):
+    """npm = 2 with core states: the x0kf_v4hz_init lists built from the
+    device weights equal those from the oracle weights (core pairs dropped,
+    jpm = 2 pairs indexed after the core block)."""
+    say("tetwt npm=2 core -> x0kf lists")
+    m = synth.tetra_mesh(2, 2, 2)
+    ek1, ek2, ef = synth.tetra_bands(m, 4, 6, seed=15)
+    ecore = np.array([-32.0, -19.5]) / synth.RYDBERG_EV
+    fr = synth.tetra_frhis(np.vstack([np.repeat(ecore[:, None], ek1.shape[1], 1), ek1]), ek2)
+    lists = []
+    for r in (h.tetwt(m, ek1, ek2, ef, fr, npm=2, ecore=ecore), orc.tetwt(m, ek1, ek2, ef, fr, npm=2, ecore=ecore)):
+        pw = [synth.pair_weights_from_tetwt(r, j) for j in (0, 1)]
+        lists.append(synth.x0kf_index(pw[0], nctot=2, npm=2, nband=ek1.shape[0], pw_minus=pw[1]))
+    ixg, ixo = lists
+    for f in ("kc", "itc", "itpc", "jpmc", "iwini", "iwend", "icouini"):
+        np.testing.assert_array_equal(getattr(ixg, f), getattr(ixo, f), err_msg=f)
+    assert (ixo.jpmc == 2).any()
     assert np.abs(ixg.whwc - ixo.whwc).max() <= 1e-8 * np.abs(ixo.whwc).max()

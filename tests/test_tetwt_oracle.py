@@ -183,3 +183,100 @@ def test_hisrange_and_chkdgn(orc):
     a, b = orc.chkdgn(np.array([0.0, 0.00005, 0.3, 0.3, 0.7]))
     np.testing.assert_array_equal(a, [1, 3, 5])
     np.testing.assert_array_equal(b, [2, 4, 5])
+
+
+# ---------------------------------------------------------------- npm = 2 and core states
+def _segments(r, jpm=None):
+    """{(n1b, n2b): (ihw, whw segment)} of every k (1-based bands)."""
+    pick = (lambda a: a) if jpm is None else (lambda a: a[..., jpm])
+    out = {}
+    nbnb = pick(r["nbnb"])
+    for k in range(len(nbnb)):
+        for i in range(nbnb[k]):
+            j0, nh = pick(r["jhw"])[i, k] - 1, pick(r["nhw"])[i, k]
+            out[(k, pick(r["n1b"])[i, k], pick(r["n2b"])[i, k])] = (pick(r["ihw"])[i, k], r["whw"][j0:j0 + nh])
+    return out
+
+
+def _assert_same_segments(a, b, rtol=0.0):
+    assert a.keys() == b.keys()
+    for key, (ih, w) in a.items():
+        ih2, w2 = b[key]
+        assert ih == ih2 and len(w) == len(w2), key
+        if rtol == 0.0:
+            np.testing.assert_array_equal(w, w2, err_msg=str(key))
+        else:
+            np.testing.assert_allclose(w, w2, rtol=rtol, atol=0.0, err_msg=str(key))
+
+
+@pytest.mark.parametrize("ebmx", [1e10, 2.5])
+def test_tetwt_npm2_first_half_is_npm1(orc, ebmx):
+    """npm = 2: the jpm = 1 lists, pointers and weights are those of npm = 1
+    (the whw pointers run over jpm first)."""
+    m, ek1, ek2, ef, fr = _case(seed=4)
+    r1 = orc.tetwt(m, ek1, ek2, ef, fr, ebmx=ebmx)
+    r2 = orc.tetwt(m, ek1, ek2, ef, fr, ebmx=ebmx, npm=2)
+    assert r2["nbnb"].shape == (ek1.shape[1], 2)
+    np.testing.assert_array_equal(r2["nbnb"][:, 0], r1["nbnb"])
+    n = r1["nbnbx"]
+    for key in ("n1b", "n2b", "ihw", "nhw", "jhw"):
+        np.testing.assert_array_equal(r2[key][:n, :, 0], r1[key][:n], err_msg=key)
+    np.testing.assert_array_equal(r2["whw"][:r1["nhwtot"]], r1["whw"])
+    assert r2["nhwtot"] > r1["nhwtot"]
+    # the jpm = 2 pointers continue after the jpm = 1 weights
+    assert r2["jhw"][0, 0, 1] == r1["nhwtot"] + 1
+
+
+@pytest.mark.parametrize("ebmx", [1e10, 2.5])
+def test_tetwt_npm2_second_half_is_swapped_npm1(orc, ebmx):
+    """jpm = 2 pairs a state n at k with an occupied m at k+q; it is npm = 1
+    with the k and k+q energies exchanged and each pair (n, m) read as (m, n):
+    the same pairs, windows and histograms (ib-major order on each side)."""
+    m, ek1, ek2, ef, fr = _case(seed=5)
+    r2 = orc.tetwt(m, ek1, ek2, ef, fr, ebmx=ebmx, npm=2)
+    rs = orc.tetwt(m, ek2, ek1, ef, fr, ebmx=ebmx)
+    a = _segments(r2, 1)
+    b = {(k, jb, ib): v for (k, ib, jb), v in _segments(rs).items()}
+    assert len(a) > 0
+    _assert_same_segments(a, b, rtol=1e-15)
+    for k in range(ek1.shape[1]):          # ib-major: sorted by (n1b, n2b)
+        n = r2["nbnb"][k, 1]
+        pairs = list(zip(r2["n1b"][:n, k, 1], r2["n2b"][:n, k, 1]))
+        assert pairs == sorted(pairs)
+
+
+def _with_flat_bands(ek, ecore):
+    nq = ek.shape[1]
+    return np.vstack([np.repeat(np.asarray(ecore)[:, None], nq, axis=1), ek])
+
+
+@pytest.mark.parametrize("npm", [1, 2])
+def test_tetwt_core_is_a_flat_band(orc, npm):
+    """Core states (appended after the valence bands, flat in k) give the
+    weights of flat valence bands of the same energies placed below the
+    valence bands; a degenerate core pair is averaged like a degenerate band."""
+    m, ek1, ek2, ef, _ = _case(seed=6)
+    ecore = np.array([-32.0, -32.0, -19.5]) / synth.RYDBERG_EV
+    nband, nc = ek1.shape[0], len(ecore)
+    f1, f2 = _with_flat_bands(ek1, ecore), _with_flat_bands(ek2, ecore)
+    fr = synth.tetra_frhis(f1, f2)
+    rc = orc.tetwt(m, ek1, ek2, ef, fr, npm=npm, ecore=ecore)
+    rv = orc.tetwt(m, f1, f2, ef, fr, npm=npm)
+
+    def band(b):  # core-form band (1-based) -> flat-band form
+        return b - nband if b > nband else b + nc
+
+    for j in range(npm):
+        jj = None if npm == 1 else j
+        sc = {(k, band(a), band(b)): v for (k, a, b), v in _segments(rc, jj).items()}
+        sv = _segments(rv, jj)
+        _assert_same_segments(sc, sv)
+        core_side = [a for (_, a, b) in _segments(rc, jj)] if j == 0 else [b for (_, a, b) in _segments(rc, jj)]
+        assert max(core_side) > nband      # core pairs are present
+    np.testing.assert_array_equal(np.sort(rc["whw"]), np.sort(rv["whw"]))
+    # the degenerate core pair carries equal histograms
+    seg = _segments(rc, None if npm == 1 else 0)
+    both = [(k, b) for (k, a, b) in seg if a == nband + 1]
+    assert both
+    for k, b in both:
+        np.testing.assert_array_equal(seg[(k, nband + 1, b)][1], seg[(k, nband + 2, b)][1])
