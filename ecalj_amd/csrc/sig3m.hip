@@ -25,6 +25,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "bf16split.hpp"
 #include "sigtile.hpp"
 
 namespace ecalj {
@@ -140,7 +141,10 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave / WN, wn = wave % WN;
   const int64_t apst = (int64_t)MP * QBK;
-  const unsigned short* A = Ak + (int64_t)I0 * QBK + lane * 8;
+  // uniform base (scalar registers) + the lane's 16 bytes of each 1 KB run, so
+  // the stage loads take the scalar-base form with no per-load address VALU
+  const unsigned short* A = Ak + (int64_t)I0 * QBK;
+  const int lane8 = lane * 8;
   const int nst = NS * nwg;
 
   for (int e = t; e < nwg * Q3BN; e += NT) {
@@ -155,8 +159,8 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
 #pragma unroll
     for (int q = 0; q < ARW; ++q) {
       const int run = ARW * wave + q, p = run / APP, sub = run % APP;
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + p * apst + sub * 512), (lds_ptr_t)(&s.a[buf][p][sub * 512]),
-                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + (p * apst + sub * 512) + lane8),
+                                       (lds_ptr_t)(&s.a[buf][p][sub * 512]), 16, 0, 0);
     }
   };
 
@@ -193,6 +197,7 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
   for (int q = 0; q < Q3R - 1; ++q) issue(min(q, nst - 1), q);
   // weighted zmel columns of frequency vl, split into bf16 hi/lo: re, im, re + im
   u4q brh[TN], brl[TN], bih[TN], bil[TN], bsh[TN], bsl[TN];
+  const SplitMasks mk = split_masks();
   auto make_b = [&](int vl) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
@@ -202,19 +207,21 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
         const f4 z = zv[j][q];
         const float r0 = w * z[0], i0 = w * z[1], r1 = w * z[2], i1 = w * z[3];
         const float s0 = r0 + i0, s1 = r1 + i1;
-        typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-        typedef float fl2 __attribute__((ext_vector_type(2)));
-#define ECALJ_SPLIT2(X0, X1, H, L)                                                                       \
-  {                                                                                                      \
-    const unsigned hw = __builtin_bit_cast(unsigned, __builtin_convertvector((fl2){X0, X1}, bf2));      \
-    const float e0 = (X0) - __uint_as_float(hw << 16), e1 = (X1) - __uint_as_float(hw & 0xffff0000u); \
-    H[j][q] = hw;                                                                                        \
-    L[j][q] = __builtin_bit_cast(unsigned, __builtin_convertvector((fl2){e0, e1}, bf2));                \
-  }
-        ECALJ_SPLIT2(r0, r1, brh, brl)
-        ECALJ_SPLIT2(i0, i1, bih, bil)
-        ECALJ_SPLIT2(s0, s1, bsh, bsl)
-#undef ECALJ_SPLIT2
+        {
+          const HiLo2 t = split2_bf16(r0, r1, mk);
+          brh[j][q] = t.hi;
+          brl[j][q] = t.lo;
+        }
+        {
+          const HiLo2 t = split2_bf16(i0, i1, mk);
+          bih[j][q] = t.hi;
+          bil[j][q] = t.lo;
+        }
+        {
+          const HiLo2 t = split2_bf16(s0, s1, mk);
+          bsh[j][q] = t.hi;
+          bsl[j][q] = t.lo;
+        }
       }
     }
   };
@@ -236,7 +243,29 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_barrier();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // W fragments of row tile i (planes: re hi, re lo, im hi, im lo,
+    // (re + im) hi, (re + im) lo).  The LDS stage loads in flight make the
+    // compiler drain every outstanding LDS read (lgkmcnt(0)) before the first
+    // use of any of them, so each group of reads is issued just after the
+    // MFMA that waits for the previous group, and the rest of the group's
+    // MFMAs cover it.  Tile 0's first reads go before the stage's global loads.
+    //  WM = 1 (TN = 1, registers to spare): two register sets; tile i+1's six
+    //    planes are read after tile i's first MFMA.
+    //  WM = 2: one set; the lo planes of tile i after its first MFMA, the hi
+    //    planes of tile i+1 (in the registers just released) after its first
+    //    lo MFMA.
+    us8q fa[WM == 1 ? 2 : 1][Q3PL];
+    auto read = [&](int i, int set, int p0, int dp) {
+      const int o = aoff + i * 16 * QBK;
+#pragma unroll
+      for (int p = p0; p < Q3PL; p += dp) fa[set][p] = *reinterpret_cast<const us8q*>(&s.a[buf][p][o]);
+    };
     if constexpr (TOP) make_b(vl);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (WM == 1)
+      read(0, 0, 0, 1);
+    else
+      read(0, 0, 0, 2);
     if constexpr (ZL) {
       __builtin_amdgcn_sched_barrier(0);
       zload(min(ks + 1, NS - 1));
@@ -245,29 +274,62 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
     // the buffer of stage st+Q3R-1 was last read in iteration st-1, which every wave has left
     issue(min(st + Q3R - 1, nst - 1), (st + Q3R - 1) % Q3R);
 
+    auto mma = [&](f4& acc, const us8q& a, const u4q& b) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), acc, 0,
+                                                    0, 0);
+    };
+    if constexpr (WM == 1) {
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int o = aoff + i * 16 * QBK;
-      const us8q arh = *reinterpret_cast<const us8q*>(&s.a[buf][0][o]);
-      const us8q aih = *reinterpret_cast<const us8q*>(&s.a[buf][2][o]);
-      const us8q ash = *reinterpret_cast<const us8q*>(&s.a[buf][4][o]);
-      const us8q arl = *reinterpret_cast<const us8q*>(&s.a[buf][1][o]);
-      const us8q ail = *reinterpret_cast<const us8q*>(&s.a[buf][3][o]);
-      const us8q asl = *reinterpret_cast<const us8q*>(&s.a[buf][5][o]);
-#define ECALJ_3M_TERM(ACC, AF, BF)                                                                      \
-  _Pragma("unroll") for (int j = 0; j < TN; ++j) ACC[i][j] =                                            \
-      __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, AF), __builtin_bit_cast(bf8, BF[j]), \
-                                              ACC[i][j], 0, 0, 0);
-      ECALJ_3M_TERM(p1, arh, brh)
-      ECALJ_3M_TERM(p2, aih, bih)
-      ECALJ_3M_TERM(p3, ash, bsh)
-      ECALJ_3M_TERM(p1, arh, brl)
-      ECALJ_3M_TERM(p2, aih, bil)
-      ECALJ_3M_TERM(p3, ash, bsl)
-      ECALJ_3M_TERM(p1, arl, brh)
-      ECALJ_3M_TERM(p2, ail, bih)
-      ECALJ_3M_TERM(p3, asl, bsh)
-#undef ECALJ_3M_TERM
+      for (int i = 0; i < TM; ++i) {
+        const int c = i & 1;
+        const us8q* a = fa[c];
+        __builtin_amdgcn_sched_barrier(0);
+        mma(p1[i][0], a[0], brh[0]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (i + 1 < TM) read(i + 1, c ^ 1, 0, 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(p2[i][0], a[2], bih[0]);
+        mma(p3[i][0], a[4], bsh[0]);
+        mma(p1[i][0], a[0], brl[0]);
+        mma(p2[i][0], a[2], bil[0]);
+        mma(p3[i][0], a[4], bsl[0]);
+        mma(p1[i][0], a[1], brh[0]);
+        mma(p2[i][0], a[3], bih[0]);
+        mma(p3[i][0], a[5], bsh[0]);
+      }
+    } else {
+      us8q* a = fa[0];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        __builtin_amdgcn_sched_barrier(0);
+        mma(p1[i][0], a[0], brh[0]);
+        __builtin_amdgcn_sched_barrier(0);
+        read(i, 0, 1, 2);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 1; j < TN; ++j) mma(p1[i][j], a[0], brh[j]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mma(p2[i][j], a[2], bih[j]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mma(p3[i][j], a[4], bsh[j]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mma(p1[i][j], a[0], brl[j]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mma(p2[i][j], a[2], bil[j]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mma(p3[i][j], a[4], bsl[j]);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(p1[i][0], a[1], brh[0]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (i + 1 < TM) read(i + 1, 0, 0, 2);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 1; j < TN; ++j) mma(p1[i][j], a[1], brh[j]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mma(p2[i][j], a[3], bih[j]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mma(p3[i][j], a[5], bsh[j]);
+      }
     }
     if constexpr (END) {  // after the MFMAs in program order too: one set of operand registers
       __builtin_amdgcn_sched_barrier(0);
@@ -405,7 +467,10 @@ __device__ __forceinline__ void sigv4_tile(SigV4Smem& s, const unsigned short* _
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t apst = (int64_t)MP * QBK;
-  const unsigned short* A = Ak + (int64_t)I0 * QBK + lane * 8;
+  // uniform base (scalar registers) + the lane's 16 bytes of each 1 KB run, so
+  // the stage loads take the scalar-base form with no per-load address VALU
+  const unsigned short* A = Ak + (int64_t)I0 * QBK;
+  const int lane8 = lane * 8;
   const int nst = NS * nwg;
 
   for (int e = t; e < nwg * QV4BN; e += NT) {
@@ -420,8 +485,8 @@ __device__ __forceinline__ void sigv4_tile(SigV4Smem& s, const unsigned short* _
 #pragma unroll
     for (int q = 0; q < ARW; ++q) {
       const int run = ARW * wave + q, p = run / APP, sub = run % APP;
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + p * apst + sub * 512), (lds_ptr_t)(&s.a[buf][p][sub * 512]),
-                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + (p * apst + sub * 512) + lane8),
+                                       (lds_ptr_t)(&s.a[buf][p][sub * 512]), 16, 0, 0);
     }
   };
 
@@ -449,6 +514,7 @@ __device__ __forceinline__ void sigv4_tile(SigV4Smem& s, const unsigned short* _
 
   // weighted zmel columns (re, im; -im for the real part), split into bf16 hi/lo
   u4q brh[TN], brl[TN], bih[TN], bil[TN], nbh[TN], nbl[TN];
+  const SplitMasks mk = split_masks();
   auto make_b = [&](int vl) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
@@ -457,18 +523,16 @@ __device__ __forceinline__ void sigv4_tile(SigV4Smem& s, const unsigned short* _
       for (int q = 0; q < 4; ++q) {
         const f4 z = zv[j][q];
         const float r0 = w * z[0], i0 = w * z[1], r1 = w * z[2], i1 = w * z[3];
-        typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-        typedef float fl2 __attribute__((ext_vector_type(2)));
-#define ECALJ_SPLIT2(X0, X1, H, L)                                                                       \
-  {                                                                                                      \
-    const unsigned hw = __builtin_bit_cast(unsigned, __builtin_convertvector((fl2){X0, X1}, bf2));      \
-    const float e0 = (X0) - __uint_as_float(hw << 16), e1 = (X1) - __uint_as_float(hw & 0xffff0000u); \
-    H[j][q] = hw;                                                                                        \
-    L[j][q] = __builtin_bit_cast(unsigned, __builtin_convertvector((fl2){e0, e1}, bf2));                \
-  }
-        ECALJ_SPLIT2(r0, r1, brh, brl)
-        ECALJ_SPLIT2(i0, i1, bih, bil)
-#undef ECALJ_SPLIT2
+        {
+          const HiLo2 t = split2_bf16(r0, r1, mk);
+          brh[j][q] = t.hi;
+          brl[j][q] = t.lo;
+        }
+        {
+          const HiLo2 t = split2_bf16(i0, i1, mk);
+          bih[j][q] = t.hi;
+          bil[j][q] = t.lo;
+        }
       }
       nbh[j] = bih[j] ^ 0x80008000u;
       nbl[j] = bil[j] ^ 0x80008000u;

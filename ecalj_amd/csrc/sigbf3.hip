@@ -345,13 +345,6 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
     // kept when within 3% of the fastest single launch.
     for (int v = 0; v < kNumSigVariants; ++v)
       if (kSigVariants[v].xcd && ms[v] <= 1.03 * ms[var]) var = v;
-    // The three-product form issues a quarter fewer MFMAs per flop; under the
-    // power limit of consecutive q-points it keeps a higher clock, and its XCD
-    // variant ran the C3 step in 456 ms against 474-483 ms for the four-product
-    // variants that matched it in this single cold launch (and 470 ms for its
-    // 8-wave layout, which wins the cold launch by 1-2%).  It is kept when
-    // within 5% of the fastest single launch.
-    if (ms[kFirst3m + 1] <= 1.05 * ms[var]) var = kFirst3m + 1;
     std::vector<const char*> names;
     for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
     tune_store(h, key, var, "sigc_bf3", names, ms);

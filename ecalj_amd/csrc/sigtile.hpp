@@ -63,8 +63,11 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6), wm = wave / WN, wn = wave % WN;
   const int64_t apst = (int64_t)MP * QBK, bpst = (int64_t)NCP * QBK;
-  const unsigned short* A = Ak + (int64_t)I0 * QBK + lane * 8;
-  const unsigned short* B = Bk + (int64_t)c0 * QBK + lane * 8;
+  // uniform bases (scalar registers) + the lane's 16 bytes of each 1 KB run, so
+  // the stage loads take the scalar-base form with no per-load address VALU
+  const unsigned short* A = Ak + (int64_t)I0 * QBK;
+  const unsigned short* B = Bk + (int64_t)c0 * QBK;
+  const int lane8 = lane * 8;
   const int nst = nwg * NS;
 
   // weight table: wt[v][c] = wgt(v0+v, c0+c) (0 beyond ncol)
@@ -81,14 +84,14 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
 #pragma unroll
     for (int q = 0; q < ARW; ++q) {
       const int run = ARW * wave + q, p = run / APP, sub = run % APP;
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + p * apst + sub * 512), (lds_ptr_t)(&s.a[buf][p][sub * 512]),
-                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + (p * apst + sub * 512) + lane8),
+                                       (lds_ptr_t)(&s.a[buf][p][sub * 512]), 16, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < BRW; ++q) {
       const int run = BRW * wave + q, p = run / BPP, sub = run % BPP;
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(bb + p * bpst + sub * 512), (lds_ptr_t)(&s.b[buf][p][sub * 512]),
-                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(bb + (p * bpst + sub * 512) + lane8),
+                                       (lds_ptr_t)(&s.b[buf][p][sub * 512]), 16, 0, 0);
     }
   };
 
@@ -135,6 +138,25 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
         brl[j] = *reinterpret_cast<const u4q*>(&s.b[buf][1][o]);
         bih[j] = *reinterpret_cast<const u4q*>(&s.b[buf][2][o]);
         bil[j] = *reinterpret_cast<const u4q*>(&s.b[buf][3][o]);
+      }
+      // A fragments of row tile i (planes re hi, re lo, im hi, im lo) in two
+      // register sets.  The LDS stage loads in flight make the compiler drain
+      // every outstanding LDS read (lgkmcnt(0)) before the first use of any of
+      // them, so tile i+1's reads are issued just after tile i's first MFMA
+      // (which waits for tile i's) and the tile's other MFMAs cover them.
+      us8q fa[2][4];
+      auto read_a = [&](int i, int set) {
+        const int o = aoff + i * 16 * QBK;
+        fa[set][0] = *reinterpret_cast<const us8q*>(&s.a[buf][0][o]);
+        fa[set][1] = *reinterpret_cast<const us8q*>(&s.a[buf][1][o]);
+        if constexpr (!AREAL) {
+          fa[set][2] = *reinterpret_cast<const us8q*>(&s.a[buf][2][o]);
+          fa[set][3] = *reinterpret_cast<const us8q*>(&s.a[buf][3][o]);
+        }
+      };
+      read_a(0, 0);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
         // -b: flip the sign bit of every bf16 (two per 32-bit word)
         nbh[j] = bih[j] ^ 0x80008000u;
         nbl[j] = bil[j] ^ 0x80008000u;
@@ -142,29 +164,33 @@ __device__ __forceinline__ void sig_tile(SigSmem<BN, NSTG>& s, const unsigned sh
       // one A row tile at a time: its 4 planes meet the TN column tiles
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
-        const int o = aoff + i * 16 * QBK;
-        const us8q arh = *reinterpret_cast<const us8q*>(&s.a[buf][0][o]);
-        const us8q arl = *reinterpret_cast<const us8q*>(&s.a[buf][1][o]);
-#define ECALJ_Q_TERM(ACC, AF, BF)                                                                     \
-  _Pragma("unroll") for (int j = 0; j < TN; ++j) ACC[i][j] =                                          \
+        const us8q* a = fa[i & 1];
+        const us8q arh = a[0], arl = a[1];
+#define ECALJ_Q_TERM(ACC, AF, BF, J0)                                                                 \
+  _Pragma("unroll") for (int j = J0; j < TN; ++j) ACC[i][j] =                                         \
       __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, AF), __builtin_bit_cast(bf8, BF[j]), \
                                               ACC[i][j], 0, 0, 0);
         // re = ar br - ai bi, im = ar bi + ai br; hi*hi + hi*lo + lo*hi of each
-        ECALJ_Q_TERM(cr, arh, brh)
-        ECALJ_Q_TERM(ci, arh, bih)
-        ECALJ_Q_TERM(cr, arh, brl)
-        ECALJ_Q_TERM(ci, arh, bil)
-        ECALJ_Q_TERM(cr, arl, brh)
-        ECALJ_Q_TERM(ci, arl, bih)
+        __builtin_amdgcn_sched_barrier(0);
+        cr[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, arh), __builtin_bit_cast(bf8, brh[0]),
+                                                           cr[i][0], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (i + 1 < TM) read_a(i + 1, (i + 1) & 1);
+        __builtin_amdgcn_sched_barrier(0);
+        ECALJ_Q_TERM(cr, arh, brh, 1)
+        ECALJ_Q_TERM(ci, arh, bih, 0)
+        ECALJ_Q_TERM(cr, arh, brl, 0)
+        ECALJ_Q_TERM(ci, arh, bil, 0)
+        ECALJ_Q_TERM(cr, arl, brh, 0)
+        ECALJ_Q_TERM(ci, arl, bih, 0)
         if constexpr (!AREAL) {
-          const us8q aih = *reinterpret_cast<const us8q*>(&s.a[buf][2][o]);
-          const us8q ail = *reinterpret_cast<const us8q*>(&s.a[buf][3][o]);
-          ECALJ_Q_TERM(cr, aih, nbh)
-          ECALJ_Q_TERM(ci, aih, brh)
-          ECALJ_Q_TERM(cr, aih, nbl)
-          ECALJ_Q_TERM(ci, aih, brl)
-          ECALJ_Q_TERM(cr, ail, nbh)
-          ECALJ_Q_TERM(ci, ail, brh)
+          const us8q aih = a[2], ail = a[3];
+          ECALJ_Q_TERM(cr, aih, nbh, 0)
+          ECALJ_Q_TERM(ci, aih, brh, 0)
+          ECALJ_Q_TERM(cr, aih, nbl, 0)
+          ECALJ_Q_TERM(ci, aih, brl, 0)
+          ECALJ_Q_TERM(cr, ail, nbh, 0)
+          ECALJ_Q_TERM(ci, ail, brh, 0)
         }
 #undef ECALJ_Q_TERM
       }

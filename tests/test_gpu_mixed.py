@@ -174,7 +174,9 @@ def test_sigc_bf3_frequency_inner_forms(orc, mode, ngb, nm, ntq, niw):
 
 def test_sigc_bf3_tuner_variants(orc):
     """Every kernel variant of the tuner gives the oracle's Sigma_c; the variants
-    of one form are bit-identical to each other (same sums in the same order);
+    of one form (four products frequency-outer; three products in either wave
+    layout; four products frequency-inner) are bit-identical to each other
+    (same sums in the same order);
     auto mode records its choice and returns the chosen variant's result."""
     import ecalj_amd
 
@@ -197,6 +199,7 @@ def test_sigc_bf3_tuner_variants(orc):
         assert np.array_equal(outs[4], outs[5])
         assert np.array_equal(outs[6], outs[7])
         assert np.array_equal(outs[8], outs[9])
+        assert np.array_equal(outs[4], outs[8])  # three products: 2 x 4 and 1 x 8 wave layouts
         h.set_tuning(0)
         say("sigc bf16x3 tuner auto")
         a = h.sigc(z, wc, g)
