@@ -603,7 +603,10 @@ def roofline(fl, info, precision, steps, config="C2"):
              "bf16f8": "sig_f8_kernel (Sigma_c: bf16 hi*hi + scaled-fp8 cross terms, per-frequency W_c(iv)^H zmel, "
                        "weighted sum, all internal k)"}.get(
         precision, "sig_freq_gemm_kernel (Sigma_c: W_c(iv)^H x frequency-weighted zmel, all internal k)")
-    if precision == "bf16x3" and var and var.startswith("3m"):
+    if precision == "bf16x3" and var and var.startswith("3m32"):
+        kname = ("sig3m32_kernel (bf16x3 Sigma_c, frequency-inner, three real products per complex product on the "
+                 "32x32x16 MFMA: W_c(iv)^H x frequency-weighted zmel, all internal k)")
+    elif precision == "bf16x3" and var and var.startswith("3m"):
         kname = ("sig3m_kernel (bf16x3 Sigma_c, frequency-inner, three real products per complex product: "
                  "W_c(iv)^H x frequency-weighted zmel, all internal k)")
     elif precision == "bf16x3" and var and var.startswith("vin4"):

@@ -83,7 +83,9 @@ ecalj_status sig_contract_grouped(ecalj_ctx* h, int mode, int nk, const int* nm,
 size_t sig_bf3_a_elems_per_k(int n, int nw1);
 // The frequency-inner three-product form of the bf16x3 Sigma_c contraction
 // (sig3m.hip): operand preparation (six W planes into as, padded complex64
-// zmel into zp) and the contraction itself.
+// zmel into zp) and the contraction itself: wm = 2 (8 waves in 2 x 4) or 1
+// (8 waves side by side) on the 16x16x32 MFMA, wm = 0 (4 waves side by side)
+// on the 32x32x16 MFMA.
 size_t sig3m_a_elems_per_k(int n, int nw1);
 ecalj_status sig3m_prep(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, int nw1,
                         const cd* wc, int64_t ldw, int64_t w_stride, unsigned short* as, cf* zp);
@@ -95,8 +97,8 @@ ecalj_status sigv4_run(ecalj_ctx* h, bool xcd, int nb, int n, int ncol, int nw1,
 ecalj_status sig_pad_z(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int n, int ncol, cf* zp);
 ecalj_status sig3m_grouped_prep(ecalj_ctx* h, int nk, const int* ncol, const cf* zmel, int n, int nw1, const cd* wc,
                                 int64_t ldw, int64_t w_stride, unsigned short* as, cf* zp);
-ecalj_status sig3m_grouped_run(ecalj_ctx* h, int nk, const int* ncol, int n, int nw1, const unsigned short* as,
-                               const cf* zp, const double* wgt, cf* cz);
+ecalj_status sig3m_grouped_run(ecalj_ctx* h, int wm, int nk, const int* ncol, int n, int nw1,
+                               const unsigned short* as, const cf* zp, const double* wgt, cf* cz);
 // x0gemm in the bf16x3 form: per-bin entry lists on the host (bin b: entries
 // cnt[b]..cnt[b+1]-1 with zmel column offsets col[] and weights wt[]), output
 // slice slice[b] of rcxq (npr x npr_col, complex64, slice stride slice_elems)

@@ -38,6 +38,7 @@ constexpr int Q3BN = 128;  // block columns
 struct alignas(16) Sig3Smem {
   unsigned short a[Q3R][Q3PL][QBM * QBK];  // 8 KB per plane, 48 KB per stage
   float wt[Q3NW][Q3BN];                    // per-frequency weights of the block's columns
+  unsigned pf[512];                        // landing area of the 32x32 form's L2 prefetch (never read)
 };
 
 // ---------------------------------------------------------------- operand preparation
@@ -419,6 +420,7 @@ struct Sig3Group {
   int64_t czoff;  // output of k (elements)
   int ncol, pad;
 };
+template <int WM>
 __global__ __launch_bounds__(512, 1) void sig3m_grouped_kernel(const int* __restrict__ tiles, int ntiles,
                                                                const Sig3Group* __restrict__ grp,
                                                                const unsigned short* __restrict__ as,
@@ -433,9 +435,253 @@ __global__ __launch_bounds__(512, 1) void sig3m_grouped_kernel(const int* __rest
     const int it = tiles[3 * t + 1], jt = tiles[3 * t + 2];
     const Sig3Group gk = grp[k];
     __syncthreads();  // the previous tile's LDS reads are done before this tile's loads land
-    sig3m_tile<2>(s, as + as_stride * k, zp + gk.zoff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, NP, it * QBM,
-               jt * Q3BN, accumulate, cz + gk.czoff, (int64_t)n);
+    sig3m_tile<WM>(s, as + as_stride * k, zp + gk.zoff, wgt + gk.goff, n, nw1, v0, nwg, gk.ncol, MP, NS, NP,
+                   it * QBM, jt * Q3BN, accumulate, cz + gk.czoff, (int64_t)n);
   }
+}
+
+// ---------------------------------------------------------------- three products, 32x32 MFMA, one wave per SIMD
+// The frequency-inner three-product contraction on v_mfma_f32_32x32x16_bf16:
+// a block of 128 rows x 128 columns, 4 waves side by side, each all 128 rows
+// x 32 columns (4 row tiles of 32).  A wave reads each W fragment once for
+// 32 columns (half the LDS reads of the 16-column layout) and splits its 32
+// columns once per block (half the splits of the 2 x 4 layout).  With one
+// wave per SIMD the VALU split of the next frequency's weighted columns runs
+// between this frequency's MFMAs, in the same wave, while each 32x32x16 MFMA
+// occupies the matrix core for 32 cycles.  Operand layout of the 32x32x16
+// instruction: lane l holds A(row l % 32, k 8 (l / 32) .. +7) and B(k the
+// same, column l % 32); accumulator element e of lane l is C(row 8 (e / 4) +
+// 4 (l / 32) + e % 4, column l % 32).
+constexpr int Q32NT = 256;  // threads of the 32x32 form
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ void sig3m32_tile(Sig3Smem& s, const unsigned short* __restrict__ Ak,
+                                             const cf* __restrict__ zp, const double* __restrict__ g, int n,
+                                             int nw1, int v0, int nwg, int ncol, int MP, int NS, int NP, int I0,
+                                             int c0, int accumulate, cf* __restrict__ C, int64_t ldc) {
+  constexpr int TM = 4;                       // row tiles of 32
+  constexpr int APP = QBM * QBK / 512;        // 1 KB runs per plane
+  constexpr int ARW = Q3PL * APP / 4;         // runs per wave and stage
+  static_assert(ARW * 4 == Q3PL * APP, "runs split evenly over the waves");
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t apst = (int64_t)MP * QBK;
+  const unsigned short* A = Ak + (int64_t)I0 * QBK;
+  const int lane8 = lane * 8;
+  const int nst = NS * nwg;
+
+  for (int e = t; e < nwg * Q3BN; e += Q32NT) {
+    const int v = e / Q3BN, c = c0 + e % Q3BN;
+    s.wt[v][e % Q3BN] = (c < ncol) ? (float)g[(v0 + v) + (int64_t)nw1 * c] : 0.f;
+  }
+  __syncthreads();
+
+  auto issue = [&](int st, int buf) {
+    const int ks = st / nwg, v = v0 + st % nwg;
+    const unsigned short* a = A + ((int64_t)v * NS + ks) * Q3PL * apst;
+#pragma unroll
+    for (int q = 0; q < ARW; ++q) {
+      const int run = ARW * wave + q, p = run / APP, sub = run % APP;
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + (p * apst + sub * 512) + lane8),
+                                       (lds_ptr_t)(&s.a[buf][p][sub * 512]), 16, 0, 0);
+    }
+  };
+
+  // the lane's zmel: column c0 + 32 wave + (lane & 31), k = 16 h + 8 (lane >> 5) + 2q, +1 of the slice
+  const int col = wave * 32 + (lane & 31);
+  const cf* zcol = zp + (int64_t)(c0 + col) * NP + 8 * (lane >> 5);
+  f4 zc[2][4], zn[2][4];  // this slice's, the next slice's
+  auto zload = [&](f4 (&z)[2][4], int ks) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) z[h][q] = *reinterpret_cast<const f4*>(zcol + (int64_t)ks * QBK + 16 * h + 2 * q);
+  };
+
+  // weighted zmel columns, split: [plane][half] with planes re hi, re lo, im hi, im lo, (re+im) hi, (re+im) lo
+  typedef u4q BSet[6][2];
+  BSet b0, b1;
+  // one eighth (k pair q of half h) of the split of frequency vl from zc
+  // (the empty asm statements pin each result where it is formed: the
+  // computation is otherwise sunk to its first use, the next step's MFMAs)
+  auto split_part = [&](BSet& b, float w, int h, int q) {
+    const f4 z = zc[h][q];
+    const float r0 = w * z[0], i0 = w * z[1], r1 = w * z[2], i1 = w * z[3];
+    const float s0 = r0 + i0, s1 = r1 + i1;
+    HiLo2 x[3] = {split2_bf16_alu(r0, r1), split2_bf16_alu(i0, i1), split2_bf16_alu(s0, s1)};
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      asm volatile("" : "+v"(x[p].hi), "+v"(x[p].lo));
+      b[2 * p][h][q] = x[p].hi;
+      b[2 * p + 1][h][q] = x[p].lo;
+    }
+  };
+
+  f16v p1[TM], p2[TM], p3[TM];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) p1[i][e] = p2[i][e] = p3[i][e] = 0.f;
+  // fragment offsets in a plane: row 32 i + (lane & 31), logical chunk 2 h + (lane >> 5)
+  int aoff[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) aoff[h] = (lane & 31) * QBK + 8 * chunk_slot(lane & 31, 2 * h + (lane >> 5));
+
+  // prologue: slice 0's zmel (and slice 1's, which one frequency per slice
+  // needs at the first step), the first stages, frequency 0's split
+  zload(zc, 0);
+  zload(zn, min(1, NS - 1));
+  __builtin_amdgcn_sched_barrier(0);
+  for (int q = 0; q < Q3R - 1; ++q) issue(min(q, nst - 1), q);
+  {
+    const float w = s.wt[0][col];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) split_part(b0, w, h, q);
+  }
+
+  auto mma = [&](f16v& acc, const us8q& a, const u4q& b) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), acc, 0, 0,
+                                                  0);
+  };
+  // One step: frequency vl of slice ks from stage st, with the weighted
+  // columns in bc; the next step's are formed into bn between the MFMAs.
+  // The LDS stage loads in flight make the compiler drain every outstanding
+  // LDS read (lgkmcnt(0)) before the first use of any, so each group of
+  // fragment reads is issued just after the MFMA that waits for the previous
+  // group: half 1 of row tile i after the first MFMA of half 0, half 0 of
+  // tile i+1 after the first MFMA of half 1.
+  // L2 prefetch of the stage one step beyond the LDS ring: the ring holds
+  // two stages in flight, which at this kernel's step time does not cover
+  // the fabric latency.  Wave w touches the 64 128-byte lines of planes w and
+  // w + 4 (the last plane again when there is no plane w + 4), one line per
+  // lane, with dword loads through the LDS-direct path into a 2 KB scratch
+  // area that nothing reads: no register waits for them, and the stage wait
+  // counts them with the stage loads.
+  constexpr int NPF = 2;
+  auto prefetch = [&](int stp) {
+    const int ks = stp / nwg, v = v0 + stp % nwg;
+    const unsigned short* a = A + ((int64_t)v * NS + ks) * Q3PL * apst;
+#pragma unroll
+    for (int q = 0; q < NPF; ++q) {
+      const int p = min(wave + 4 * q, Q3PL - 1);
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)(a + p * apst + lane * 64), (lds_ptr_t)(&s.pf[(q * 4 + wave) * 64]),
+                                       4, 0, 0);
+    }
+  };
+  prefetch(min(Q3R - 1, nst - 1));  // every step's stage wait counts ARW stage loads + NPF prefetches last
+  auto step = [&](int st, BSet& bc, BSet& bn) {
+    const int ks = st / nwg, vl = st - ks * nwg, buf = st % Q3R;
+    ECALJ_WAIT_VMCNT(ARW + NPF);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // The split at the last frequency of slice ks is that of slice ks+1's
+    // first, from slice ks+1's zmel: moved into zc here from zn, which the
+    // first frequency of the slice loaded (one frequency per slice: the
+    // slice after, as the move comes first) before the step's stage loads,
+    // so the next step's stage wait covers it.
+    if (vl == nwg - 1) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) zc[h][q] = zn[h][q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (vl == 0) zload(zn, min(ks + (nwg == 1 ? 2 : 1), NS - 1));
+    us8q fr[2][6];  // [half][plane]
+    auto read = [&](int i, int h) {
+      const int o = aoff[h] + i * 32 * QBK;
+#pragma unroll
+      for (int p = 0; p < 6; ++p) fr[h][p] = *reinterpret_cast<const us8q*>(&s.a[buf][p][o]);
+    };
+    __builtin_amdgcn_sched_barrier(0);
+    read(0, 0);
+    const float wn = s.wt[vl + 1 < nwg ? vl + 1 : 0][col];
+    __builtin_amdgcn_sched_barrier(0);
+    // the buffer of stage st+Q3R-1 was last read in step st-1, which every wave has left
+    issue(min(st + Q3R - 1, nst - 1), (st + Q3R - 1) % Q3R);
+    prefetch(min(st + Q3R, nst - 1));
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const us8q* a = fr[h];
+        __builtin_amdgcn_sched_barrier(0);
+        mma(p1[i], a[0], bc[0][h]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (h == 0)
+          read(i, 1);
+        else if (i + 1 < TM)
+          read(i + 1, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(p2[i], a[2], bc[2][h]);
+        mma(p3[i], a[4], bc[4][h]);
+        mma(p1[i], a[0], bc[1][h]);
+        mma(p2[i], a[2], bc[3][h]);
+        mma(p3[i], a[4], bc[5][h]);
+        mma(p1[i], a[1], bc[0][h]);
+        mma(p2[i], a[3], bc[2][h]);
+        mma(p3[i], a[5], bc[4][h]);
+        // one eighth of the next step's split between these MFMAs
+        split_part(bn, wn, (2 * i + h) >> 2, (2 * i + h) & 3);
+      }
+    }
+  };
+  int st = 0;
+  for (; st + 1 < nst; st += 2) {
+    step(st, b0, b1);
+    step(st + 1, b1, b0);
+  }
+  if (st < nst) step(st, b0, b1);
+  ECALJ_WAIT_VMCNT(0);  // the repeated last stages have landed before the block's LDS is released
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int I = I0 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+      const int c = c0 + col;
+      if (I < n && c < ncol) {
+        const float re = p1[i][e] - p2[i][e];
+        const float im = p3[i][e] - p1[i][e] - p2[i][e];
+        cf* p = C + I + ldc * c;
+        if (accumulate) {
+          const cf o = *p;
+          *p = cf{o.re + re, o.im + im};
+        } else {
+          *p = cf{re, im};
+        }
+      }
+    }
+}
+
+// Uniform batch, block order as sig3m_kernel.
+__global__ __launch_bounds__(Q32NT, 1) void sig3m32_kernel(const unsigned short* __restrict__ as, int64_t as_stride,
+                                                           const cf* __restrict__ zp, int64_t zp_stride,
+                                                           const double* __restrict__ wgt, int64_t g_stride, int n,
+                                                           int nw1, int v0, int nwg, int ncol, int MP, int NS,
+                                                           int NP, int NCP, int accumulate, cf* __restrict__ cz,
+                                                           int64_t cz_stride, int G, int nk) {
+  __shared__ Sig3Smem s;
+  int b, r, c;
+  if (G == 0) {
+    c = blockIdx.x;
+    r = blockIdx.y;
+    b = blockIdx.z;
+  } else {
+    const int R = MP / QBM, Ct = NCP / Q3BN, NG = (Ct + G - 1) / G;
+    const int p = blockIdx.x, x = p & 7, j = p >> 3;
+    const int sigma = (j / G) * 8 + x, loc = j % G;
+    b = sigma / (R * NG);
+    const int rem = sigma % (R * NG);
+    r = rem / NG;
+    c = (rem % NG) * G + loc;
+    if (b >= nk || c >= Ct) return;
+  }
+  sig3m32_tile(s, as + as_stride * b, zp + zp_stride * b, wgt + g_stride * b, n, nw1, v0, nwg, ncol, MP, NS, NP,
+               r * QBM, c * Q3BN, accumulate, cz + cz_stride * b, (int64_t)n);
 }
 
 // ---------------------------------------------------------------- four-product, waves side by side
@@ -703,7 +949,10 @@ ecalj_status sig3m_run(ecalj_ctx* h, bool xcd, int wm, int nb, int n, int ncol, 
   }
   for (int v0 = 0; v0 < nw1; v0 += Q3NW) {
     const int nwg = std::min(Q3NW, nw1 - v0);
-    if (wm == 1)
+    if (wm == 0)
+      hipLaunchKernelGGL(sig3m32_kernel, grid, dim3(Q32NT), 0, h->stream, as, as_stride, zp, (int64_t)NCP * NP, wgt,
+                         g_stride, n, nw1, v0, nwg, ncol, MP, NS, NP, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G, nb);
+    else if (wm == 1)
       hipLaunchKernelGGL(sig3m_kernel<1>, grid, dim3(512), 0, h->stream, as, as_stride, zp, (int64_t)NCP * NP, wgt,
                          g_stride, n, nw1, v0, nwg, ncol, MP, NS, NP, NCP, v0 > 0 ? 1 : 0, cz, cz_stride, G, nb);
     else
@@ -779,8 +1028,8 @@ ecalj_status sig3m_grouped_prep(ecalj_ctx* h, int nk, const int* ncol, const cf*
   return ECALJ_OK;
 }
 
-ecalj_status sig3m_grouped_run(ecalj_ctx* h, int nk, const int* ncol, int n, int nw1, const unsigned short* as,
-                               const cf* zp, const double* wgt, cf* cz) {
+ecalj_status sig3m_grouped_run(ecalj_ctx* h, int wm, int nk, const int* ncol, int n, int nw1,
+                               const unsigned short* as, const cf* zp, const double* wgt, cf* cz) {
   const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NS = NP / QBK, R = MP / QBM;
   const int64_t as_stride = (int64_t)sig3m_a_elems_per_k(n, nw1);
   std::vector<Sig3Group> grp(nk);
@@ -821,8 +1070,12 @@ ecalj_status sig3m_grouped_run(ecalj_ctx* h, int nk, const int* ncol, int n, int
   const int* dt = reinterpret_cast<const int*>(d + sizeof(Sig3Group) * nk);
   for (int v0 = 0; v0 < nw1; v0 += Q3NW) {
     const int nwg = std::min(Q3NW, nw1 - v0);
-    hipLaunchKernelGGL(sig3m_grouped_kernel, dim3(grid), dim3(512), 0, h->stream, dt, ntiles, dg, as, as_stride, zp,
-                       wgt, n, nw1, v0, nwg, MP, NS, NP, v0 > 0 ? 1 : 0, cz);
+    if (wm == 1)
+      hipLaunchKernelGGL(sig3m_grouped_kernel<1>, dim3(grid), dim3(512), 0, h->stream, dt, ntiles, dg, as, as_stride,
+                         zp, wgt, n, nw1, v0, nwg, MP, NS, NP, v0 > 0 ? 1 : 0, cz);
+    else
+      hipLaunchKernelGGL(sig3m_grouped_kernel<2>, dim3(grid), dim3(512), 0, h->stream, dt, ntiles, dg, as, as_stride,
+                         zp, wgt, n, nw1, v0, nwg, MP, NS, NP, v0 > 0 ? 1 : 0, cz);
     ECALJ_HIP(hipGetLastError());
   }
   return ECALJ_OK;

@@ -215,21 +215,24 @@ static const SigVariant kSigVariants[] = {{64, 3, 4, 0, "128x64x3"},
                                           {128, 4, 8, 0, "vin4/128x128"},
                                           {128, 4, 8, 1, "vin4/128x128/xcd"},
                                           {128, 3, 8, 0, "3m/128x128/w8"},
-                                          {128, 3, 8, 1, "3m/128x128/w8/xcd"}};
-constexpr int kNumSigVariants = 10;
+                                          {128, 3, 8, 1, "3m/128x128/w8/xcd"},
+                                          {128, 3, 4, 0, "3m32/128x128"},
+                                          {128, 3, 4, 1, "3m32/128x128/xcd"}};
+constexpr int kNumSigVariants = 12;
 constexpr int kFirst3m = 4;
 constexpr int kFirstVin4 = 6;
-constexpr int kFirst3mW8 = 8;  // three-product form, 8 waves side by side
+constexpr int kFirst3mW8 = 8;   // three-product form, 8 waves side by side
+constexpr int kFirst3m32 = 10;  // three-product form on the 32x32x16 MFMA, 4 waves
 // operand preparation of a variant: 0 = W and zmel split into bf16 planes;
 // 1 = six W planes + padded complex64 zmel (3m); 2 = four W planes + padded zmel
 static int sig_form(int var) {
   return var >= kFirst3mW8 ? 1 : var >= kFirstVin4 ? 2 : var >= kFirst3m ? 1 : 0;
 }
 // Grouped kernel: the first three tile shapes, then the three-product form
-// (each with its own tile order)
-constexpr int kNumSigGroupedVariants = 4;
+// in its two 8-wave layouts (each with its own tile order)
+constexpr int kNumSigGroupedVariants = 5;
 static const char* const kSigGroupedNames[kNumSigGroupedVariants] = {"128x64x3", "128x128x2", "128x128x2/8w",
-                                                                      "3m/128x128/xcd"};
+                                                                      "3m/128x128/xcd", "3m/128x128/w8/xcd"};
 
 // Column tiles per super-tile of the XCD-aware order: the super-tile fills
 // about 32 CUs (one XCD) and its zmel panels (G x BN columns x NP x 8 bytes)
@@ -297,6 +300,9 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
   };
   auto run = [&](int var) -> ecalj_status {
     const bool xcd = kSigVariants[var].xcd != 0;
+    if (var >= kFirst3m32)
+      return sig3m_run(h, xcd, 0, nb, n, ncol, nw1, as, reinterpret_cast<const cf*>(bs), wgt, g_stride, cz,
+                       cz_stride);
     if (var >= kFirst3mW8)
       return sig3m_run(h, xcd, 1, nb, n, ncol, nw1, as, reinterpret_cast<const cf*>(bs), wgt, g_stride, cz,
                        cz_stride);
@@ -382,7 +388,7 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
     cols += ncol[k];
   }
   if (cols == 0) return ECALJ_OK;
-  constexpr int k3m = kNumSigGroupedVariants - 1;
+  constexpr int k3m = 3;  // and k3m + 1: the three-product form, wave layouts 2 x 4 and 1 x 8
   auto prep = [&](bool is3m) -> ecalj_status {
     if (is3m)
       return sig3m_grouped_prep(h, nk, ncol, zmel, n, nw1, wc, ldw, w_stride, as, reinterpret_cast<cf*>(bs));
@@ -403,7 +409,9 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
   };
   // table: [SigGroup x nk][tiles: (k, row tile, column tile) x ntiles]
   auto run = [&](int var) -> ecalj_status {
-    if (var == k3m) return sig3m_grouped_run(h, nk, ncol, n, nw1, as, reinterpret_cast<const cf*>(bs), wgt, cz);
+    if (var >= k3m)
+      return sig3m_grouped_run(h, var == k3m ? 2 : 1, nk, ncol, n, nw1, as, reinterpret_cast<const cf*>(bs), wgt,
+                               cz);
     const int bn = kSigVariants[var].bn;
     std::vector<int> tiles;
     for (int k = 0; k < nk; ++k)
@@ -450,7 +458,7 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
       double pm = 0.0;
       ECALJ_TRY(tune_time(h, [&] { return prep(f == 1); }, &pm));
       for (int v = 0; v < kNumSigGroupedVariants; ++v)
-        if ((v == k3m) == (f == 1)) {
+        if ((v >= k3m) == (f == 1)) {
           ECALJ_TRY(tune_time(h, [&] { return run(v); }, &ms[v]));
           ms[v] += pm;
         }
@@ -458,10 +466,10 @@ ecalj_status sig_bf3_grouped(ecalj_ctx* h, int nk, const int* ncol, const cf* zm
     var = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
     std::vector<const char*> names(kSigGroupedNames, kSigGroupedNames + kNumSigGroupedVariants);
     tune_store(h, key, var, "sigc_bf3_grouped", names, ms);
-    ECALJ_TRY(prep(var == k3m));
+    ECALJ_TRY(prep(var >= k3m));
     return run(var);
   }
-  ECALJ_TRY(prep(var == k3m));
+  ECALJ_TRY(prep(var >= k3m));
   int tev = -1;
   ECALJ_TRY(timing_begin(h, 0, &tev));
   ECALJ_TRY(run(var));

@@ -95,11 +95,11 @@ def test_c2_f64(c2):
 
 
 @pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 4), ("bf16x3", 6), ("bf16x3", 8), ("bf16x3", 10),
-                                       ("f32", 0)])
+                                       ("bf16x3", 12), ("f32", 0)])
 def test_c2_mixed(c2, prec, mode):
     """Mixed forms; bf16x3 with the tuner's choice and with the frequency-outer
-    (4), three-product (6) and four-product frequency-inner (8) Sigma_c
-    kernels forced."""
+    (4), three-product (6), four-product frequency-inner (8), three-product
+    8-wave (10) and three-product 32x32x16 (12) Sigma_c kernels forced."""
     qp, ref = c2
     _check_mixed(_run_uniform(qp, prec, mode=mode), ref, prec)
 
@@ -116,8 +116,8 @@ def test_c3_f64(c3):
     _check_f64(_run_uniform(qp, "f64"), ref)
 
 
-@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 4), ("bf16x3", 6), ("bf16x3", 8), ("f32", 0),
-                                       ("bf16f8", 0)])
+@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 4), ("bf16x3", 6), ("bf16x3", 8), ("bf16x3", 12),
+                                       ("f32", 0), ("bf16f8", 0)])
 def test_c3_mixed(c3, prec, mode):
     qp, ref = c3
     _check_mixed(_run_uniform(qp, prec, mode=mode), ref, prec)
@@ -152,11 +152,11 @@ def test_c4_f64(c4):
     _check_f64(_run_uneven(qp, "f64"), ref)
 
 
-@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 1), ("bf16x3", 4), ("bf16f8", 0)])
+@pytest.mark.parametrize("prec,mode", [("bf16x3", 0), ("bf16x3", 1), ("bf16x3", 4), ("bf16x3", 5), ("bf16f8", 0)])
 def test_c4_grouped_mixed(c4, prec, mode):
     """The grouped persistent Sigma_c kernel over both k-points' tiles (the
-    tuner's choice, the 128x64 variant and the three-product form forced), at
-    nI = 1600 and a 424-band window."""
+    tuner's choice, the 128x64 variant and the three-product form in both
+    8-wave layouts forced), at nI = 1600 and a 424-band window."""
     qp, ref = c4
     _check_mixed(_run_uneven(qp, prec, mode), ref, prec)
 

@@ -150,13 +150,15 @@ def test_sigc_mixed(hp, orc, ngb, nm, ntq, niw):
     assert relerr(h.sigc(z, wc, g), orc.sigc_imag(z, wc, g)) < TOL[prec]
 
 
-@pytest.mark.parametrize("mode", [5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("mode", [5, 6, 7, 8, 9, 10, 11, 12])
 @pytest.mark.parametrize("ngb,nm,ntq,niw", [(130, 21, 8, 12), (64, 64, 2, 10), (161, 13, 5, 40), (1, 3, 1, 2),
                                             (300, 40, 4, 0)])
 def test_sigc_bf3_frequency_inner_forms(orc, mode, ngb, nm, ntq, niw):
     """The frequency-inner variants, forced (5, 6: three-product form; 7, 8:
-    four-product form; 9, 10: three-product form, 8 waves side by side): ragged and tile-aligned shapes, more than 24
-    frequencies (two launches that accumulate), one frequency, one row."""
+    four-product form; 9, 10: three-product form, 8 waves side by side; 11,
+    12: three-product form on the 32x32x16 MFMA, 4 waves): ragged and
+    tile-aligned shapes, more than 24 frequencies (two launches that
+    accumulate), one frequency, one row."""
     import ecalj_amd
 
     say("sigc bf16x3 3m variant", mode, ngb, nm, ntq, niw)
@@ -175,7 +177,8 @@ def test_sigc_bf3_frequency_inner_forms(orc, mode, ngb, nm, ntq, niw):
 def test_sigc_bf3_tuner_variants(orc):
     """Every kernel variant of the tuner gives the oracle's Sigma_c; the variants
     of one form (four products frequency-outer; three products in either wave
-    layout; four products frequency-inner) are bit-identical to each other
+    layout; three products on the 32x32x16 MFMA; four products
+    frequency-inner) are bit-identical to each other
     (same sums in the same order);
     auto mode records its choice and returns the chosen variant's result."""
     import ecalj_amd
@@ -189,7 +192,7 @@ def test_sigc_bf3_tuner_variants(orc):
     h = ecalj_amd.Handle(0, "bf16x3")
     try:
         outs = []
-        for mode in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10):
+        for mode in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12):
             say("sigc bf16x3 tuner variant", mode)
             h.set_tuning(mode)
             outs.append(h.sigc(z, wc, g))
@@ -200,6 +203,7 @@ def test_sigc_bf3_tuner_variants(orc):
         assert np.array_equal(outs[6], outs[7])
         assert np.array_equal(outs[8], outs[9])
         assert np.array_equal(outs[4], outs[8])  # three products: 2 x 4 and 1 x 8 wave layouts
+        assert np.array_equal(outs[10], outs[11])  # three products on the 32x32x16 MFMA
         h.set_tuning(0)
         say("sigc bf16x3 tuner auto")
         a = h.sigc(z, wc, g)
@@ -208,8 +212,8 @@ def test_sigc_bf3_tuner_variants(orc):
         assert rep.count("sigc_bf3") == 1 and "*" in rep
         chosen = [ln for ln in rep.splitlines() if "sigc_bf3" in ln][0]
         star = chosen[:chosen.index("*")].split()[-3]  # the name before "<ms> ms *"
-        expect = (outs[8] if "/w8" in star else outs[4] if star.startswith("3m") else
-                  outs[6] if star.startswith("vin4") else outs[0])
+        expect = (outs[10] if star.startswith("3m32") else outs[8] if "/w8" in star else
+                  outs[4] if star.startswith("3m") else outs[6] if star.startswith("vin4") else outs[0])
         assert np.array_equal(a, expect) and np.array_equal(b, expect)
     finally:
         h.close()

@@ -53,11 +53,11 @@ def test_uneven_f64_parity(case):
     assert np.abs(eqp - ref.eqp).max() < 1e-8
 
 
-@pytest.mark.parametrize("mode", [1, 2, 4])
+@pytest.mark.parametrize("mode", [1, 2, 4, 5])
 def test_uneven_bf16x3_grouped(case, mode):
     """Each tile variant of the grouped persistent kernel, forced by the tuner
-    (4: the three-product form, whose Sigma_c kernel is also the uniform one's
-    frequency-outer XCD-aware variant)."""
+    (4, 5: the three-product form in its 2 x 4 and 1 x 8 wave layouts, whose
+    tiles are also the uniform frequency-inner kernels')."""
     from ecalj_amd.batched import UnevenQPointGPU
 
     qp, ref = case
