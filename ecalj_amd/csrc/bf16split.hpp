@@ -43,7 +43,10 @@ __device__ __forceinline__ HiLo2 split2_bf16(float x0, float x1, const SplitMask
 // these instructions it costs less: v_dot2c does not overlap a running
 // 32x32x16 MFMA the way plain VALU instructions do (measured: the split of
 // 19 value pairs between 72 MFMAs added 27% with v_dot2c, 11% this way).
+// The subtraction is not contracted with the multiplication that forms x in
+// the caller, so both forms split the same rounded x bit for bit.
 __device__ __forceinline__ HiLo2 split2_bf16_alu(float x0, float x1) {
+#pragma clang fp contract(off)
   const unsigned h = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2_t){x0, x1}, bf16x2_t));
   const float e0 = x0 - __uint_as_float(h << 16), e1 = x1 - __uint_as_float(h & 0xffff0000u);
   return {h, __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2_t){e0, e1}, bf16x2_t))};

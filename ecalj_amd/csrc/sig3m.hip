@@ -198,7 +198,12 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
   for (int q = 0; q < Q3R - 1; ++q) issue(min(q, nst - 1), q);
   // weighted zmel columns of frequency vl, split into bf16 hi/lo: re, im, re + im
   u4q brh[TN], brl[TN], bih[TN], bil[TN], bsh[TN], bsl[TN];
+  // the residual of the split: v_dot2c in the 1 x 8 layout (fewest VALU
+  // instructions); unpack and subtract in the 2 x 4 layout, where the split of
+  // one wave then overlaps the MFMAs of the other wave on the SIMD, which
+  // v_dot2c does not (C3 step: 422 against 435 ms)
   const SplitMasks mk = split_masks();
+  auto split2 = [&](float x0, float x1) { return WM == 1 ? split2_bf16(x0, x1, mk) : split2_bf16_alu(x0, x1); };
   auto make_b = [&](int vl) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
@@ -209,17 +214,17 @@ __device__ __forceinline__ void sig3m_tile(Sig3Smem& s, const unsigned short* __
         const float r0 = w * z[0], i0 = w * z[1], r1 = w * z[2], i1 = w * z[3];
         const float s0 = r0 + i0, s1 = r1 + i1;
         {
-          const HiLo2 t = split2_bf16(r0, r1, mk);
+          const HiLo2 t = split2(r0, r1);
           brh[j][q] = t.hi;
           brl[j][q] = t.lo;
         }
         {
-          const HiLo2 t = split2_bf16(i0, i1, mk);
+          const HiLo2 t = split2(i0, i1);
           bih[j][q] = t.hi;
           bil[j][q] = t.lo;
         }
         {
-          const HiLo2 t = split2_bf16(s0, s1, mk);
+          const HiLo2 t = split2(s0, s1);
           bsh[j][q] = t.hi;
           bsl[j][q] = t.lo;
         }
