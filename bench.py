@@ -44,7 +44,7 @@ MEASURED_TRAFFIC = {
     ("C2", "bf16x3", "128x128x2/8w/xcd"): (49.7e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
     ("C2", "bf16x3", "128x128x2/8w"): (124.0e9, "profiles/sigc_bf3_c2_mixed_hbm_traffic.md"),
     ("C2", "bf16f8", None): (48.8e9, "profiles/sigc_f8_c2_hbm_traffic.md"),  # one kernel, XCD-aware order
-    ("C3", "bf16x3", "3m/128x128/xcd"): (72.3e9, "profiles/sigc_3m_c3_hbm_traffic.md"),
+    ("C3", "bf16x3", "3m/128x128/xcd"): (50.83e9, "profiles/sigc_3m_c3_hbm_traffic.md"),
     ("C3", "bf16x3", "3m/128x128/w8/xcd"): (56.37e9, "profiles/sigc_3m_w8_c3_hbm_traffic.md"),
 }
 
