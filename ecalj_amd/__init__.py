@@ -90,8 +90,8 @@ _SIGS = {
     "ecalj_zmel_assemble_dev": [_p, _p, _i, _p, _i, _l, _i, _i, _p],
     "ecalj_tetwt": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _l, _p, _p, _p, _p, _p, _p,
                     _p, _p, _p],
-    "ecalj_tetwt_full": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _i, _p, _i, _l, _p, _p,
-                         _p, _p, _p, _p, _p, _p, _p],
+    "ecalj_tetwt_full": [_p, _i, _i, _i, _p, _i, _p, _p, _p, _p, _d, _d, _p, _p, _i, _i, _i, _p, _i, _i, _l, _p,
+                         _p, _p, _p, _p, _p, _p, _p, _p],
     "ecalj_zminv_dev": [_p, _p, _i, _i, _i, C.c_int64],
     "ecalj_zminv_noalloc_dev": [_p, _p, _i, _i, _i, C.c_int64],
     "ecalj_wc_build_gamma": [_p, _p, _i, _i, _i, _i, _i, _p, _i],
@@ -689,12 +689,14 @@ class Handle:
         nrec = w.nbytes // (16 * ld * ld)
         _chk(lib().ecalj_wv4_read_dev(self._h, str(path).encode(), rec0, nrec, ld, _t_ptr(w)), "wv4_read_dev")
 
-    def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10, npm=1, ecore=None):
+    def tetwt(self, mesh, ek1, ek2, ef, frhis, ebmx=1e10, npm=1, ecore=None, matrix_linear=False):
         """gettetwt (S/m_tetwt.f90:36-256): pair lists and histogram weights,
         npm = 1 or 2, with the core states of energies ecore (Ry) appended after
-        the valence bands.  mesh: synth.TetraMesh (or any object with idtetf,
-        qbzw, ib1bz, qlat).  With npm = 1 the lists are (nbnbx, nqbz) and nbnb
-        (nqbz,); with npm = 2 they keep a trailing jpm axis."""
+        the valence bands; matrix_linear: the tetrahedron_matrix_linear split
+        of each bin over the corner k-points (S/tetwt5.f90:334-336, 694-759).
+        mesh: synth.TetraMesh (or any object with idtetf, qbzw, ib1bz, qlat).
+        With npm = 1 the lists are (nbnbx, nqbz) and nbnb (nqbz,); with npm = 2
+        they keep a trailing jpm axis."""
         e1 = np.asfortranarray(ek1, np.float64)
         e2 = np.asfortranarray(ek2, np.float64)
         nband, nqbz = e1.shape
@@ -708,7 +710,7 @@ class Handle:
         nbx, nht = C.c_int(0), C.c_int64(0)
         args = (self._h, nband, nqbz, idt.shape[1], _np_ptr(idt), qw.shape[1], _np_ptr(qw), _np_ptr(ib1),
                 _np_ptr(e1), _np_ptr(e2), ef, ebmx, _np_ptr(ql), _np_ptr(fr), len(fr) - 1, npm, nctot,
-                _np_ptr(ec) if nctot else None)
+                _np_ptr(ec) if nctot else None, int(bool(matrix_linear)))
         rc = lib().ecalj_tetwt_full(*args, 0, 0, None, None, None, None, None, None, None, C.byref(nbx),
                                     C.byref(nht))
         if rc not in (0, 1):

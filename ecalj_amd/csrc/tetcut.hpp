@@ -1,7 +1,7 @@
 // tetcut.hpp -- linear-tetrahedron integration of one band pair over one
 // tetrahedron: the per-tetrahedron part of the chi0 histogram weights
-// (tetwt5x_dtet4 with tetrahedron_matrix_linear = F, S/tetwt5.f90:514-787,
-// the distribution function of S/m_tetrakbt.f90:441-466).
+// (tetwt5x_dtet4, S/tetwt5.f90:514-787, with tetrahedron_matrix_linear = F
+// or T; the distribution function of S/m_tetrakbt.f90:441-466).
 //
 // For a tetrahedron T on which the occupied energy ea, the unoccupied energy
 // eb and the excitation w = (eb - ea)/2 are linear, bin i of the histogram
@@ -21,7 +21,19 @@
 //  * which tetrahedra a clipped figure is cut into, and the order of their
 //    vertices, including how equal energies are ranked.  Both matter because
 //    the offsets act per piece: another triangulation of the same figure
-//    moves weight across a bin edge by ~1e-8 / (spread of w in the piece).
+//    moves weight across a bin edge by ~1e-8 / (spread of w in the piece),
+//    and with matrix_linear it moves the points that carry the bins;
+//  * the corner each cut point is interpolated from (clip_vertex), so that a
+//    corner on E_F yields cut points equal to it and ties rank alike.
+//
+// tetrahedron_matrix_linear (S/tetwt5.f90:694-709, 751-759, 788-840) splits
+// 4 W_i of each piece over the corners of the whole tetrahedron T, as the
+// barycentric coordinates in T of one point of the piece: the vertex average
+// of the piece's cross-section w = (f_{i-1} + f_i)/2, taken at the piece's
+// lowest / highest corner below / above its range (corner_weights).  The
+// reference finds a piece corner's coordinates by inverting the corner
+// k-vectors of T; here each vertex carries them, cut along the edges with
+// the same interpolation as its position.
 // Everything is plain C++ (host and device), so the same code is checked on
 // the host against the oracle and runs in the tetrahedron kernel.
 #pragma once
@@ -35,15 +47,29 @@
 #define TET_HD
 #endif
 
+// No fused multiply-adds in the interpolations and the integration: the
+// cut points must round as the reference's (x86, unfused) do.  Where a
+// piece has equal corner values of w (the Fermi-surface corners of a pair
+// whose occupied band is flat, a core state: all at w = (E_F - e_core)/2),
+// their order follows that rounding, and with matrix_linear a bin below or
+// above such a piece goes wholly to its first or last corner in that order.
+#if defined(__clang__)
+#define TET_UNFUSED _Pragma("clang fp contract(off)")
+#else
+#define TET_UNFUSED
+#endif
+
 namespace ecalj {
 namespace tet {
 
-// A vertex of a piece: position in k-space, excitation w, and the value
-// that the next clip tests (eb - E_F for pieces of the occupied clip).
+// A vertex of a piece: position in k-space, excitation w, the value that
+// the next clip tests (eb - E_F for pieces of the occupied clip), and the
+// barycentric coordinates in the tetrahedron's four corners.
 struct Vert {
   double k[3];
   double w;
   double e2;
+  double b[4];
 };
 
 // Pieces of a tetrahedron clipped to {g > 0}.  The corners carry labels
@@ -82,20 +108,62 @@ TET_HD inline void rank4(const double key[4], bool later_first, int label[4]) {
 }
 
 // The point of vertex code v among the labelled corners; g is the clip value.
-TET_HD inline Vert clip_vertex(const Vert* lab, const double* g, unsigned v) {
-  const unsigned a = v & 3u, b = v >> 2;
+// The cut is interpolated from the inside corner, or from the outside one
+// when it is the only corner outside (from_outside), as the reference does;
+// with a corner on the clip plane (g = 0) this keeps the cut point equal to
+// that corner bit for bit, and so the ranking of ties in the next clip.
+TET_HD inline Vert clip_vertex(const Vert* lab, const double* g, unsigned v, bool from_outside) {
+  TET_UNFUSED
+  unsigned a = v & 3u, b = v >> 2;
   if (a == b) return lab[a];
-  const double t = g[a] / (g[a] - g[b]);  // g[a] > 0 >= g[b]
+  if (from_outside) {
+    const unsigned x = a;
+    a = b;
+    b = x;
+  }
+  const double t = g[a] / (g[a] - g[b]);  // zero of g on the edge, from corner a
   Vert p;
   for (int d = 0; d < 3; ++d) p.k[d] = lab[a].k[d] + t * (lab[b].k[d] - lab[a].k[d]);
   p.w = lab[a].w + t * (lab[b].w - lab[a].w);
   p.e2 = lab[a].e2 + t * (lab[b].e2 - lab[a].e2);
+  for (int c = 0; c < 4; ++c) p.b[c] = lab[a].b[c] + t * (lab[b].b[c] - lab[a].b[c]);
   return p;
+}
+
+// Weights of the ascending corner values e of a piece in the point that
+// stands for its cross-section at ee (sum 1): the corner itself below e[0]
+// or above e[3]; else the mean of the cross-section's vertices, each the
+// zero of w - ee on an edge (3 vertices with one corner below ee or one
+// above, 4 with two), with the corner values pulled apart as in
+// cum_fraction.  At ee == e[3] exactly the reference keeps the previous
+// bin's weights; the top corner is taken here.
+TET_HD inline void corner_weights(const double e[4], double ee, double wcg[4]) {
+  TET_UNFUSED
+  const double q[4] = {e[0] - 3e-8, e[1] - 2e-8, e[2] - 1e-8, e[3]};
+  for (int c = 0; c < 4; ++c) wcg[c] = 0.0;
+  if (ee <= e[0]) {
+    wcg[0] = 1.0;
+    return;
+  }
+  if (!(ee < q[3])) {
+    wcg[3] = 1.0;
+    return;
+  }
+  // corners below ee: 1, 2 or 3 (ee between the pulled-apart values)
+  const int nb = ee < q[1] ? 1 : ee < q[2] ? 2 : 3;
+  const double inv = 1.0 / (nb * (4 - nb));
+  for (int lo = 0; lo < nb; ++lo)
+    for (int hi = nb; hi < 4; ++hi) {
+      const double d = q[hi] - q[lo];
+      wcg[lo] += (q[hi] - ee) / d * inv;
+      wcg[hi] += (ee - q[lo]) / d * inv;
+    }
 }
 
 // Cumulative volume fraction F(t) of a piece whose corner values of w,
 // ascending and pulled apart, are s[0] < s[1] < s[2] < s[3].
 TET_HD inline double cum_fraction(const double s[4], double t) {
+  TET_UNFUSED
   if (t < s[0]) return 0.0;
   if (t >= s[3]) return 1.0;
   const double d10 = s[1] - s[0], d20 = s[2] - s[0], d30 = s[3] - s[0];
@@ -122,18 +190,29 @@ TET_HD inline int first_edge_above(const double* f, int n, double x) {
   return lo;
 }
 
-// Histogram of w over one piece: emit(i, W_i) for the bins it covers.
-// false when w leaves the histogram mesh.
-template <class Emit>
+// Histogram of w over one piece: emit(i, W_i) for the bins it covers, or
+// with Linear (matrix_linear) emit(i, 4 W_i, lam), lam the coordinates in
+// the tetrahedron's corners that carry the bin.  false when w leaves the
+// histogram mesh.
+template <bool Linear, class Emit>
 TET_HD inline bool bin_piece(const Vert* p, const double* frhis, int nwhis, Emit& emit) {
+  TET_UNFUSED
   double s[4];
-  for (int i = 0; i < 4; ++i) s[i] = p[i].w;
-  for (int i = 1; i < 4; ++i)  // ascending
+  int ord[4];
+  for (int i = 0; i < 4; ++i) {
+    s[i] = p[i].w;
+    ord[i] = i;
+  }
+  for (int i = 1; i < 4; ++i)  // ascending; equal values keep their order
     for (int j = i; j > 0 && s[j - 1] > s[j]; --j) {
       const double x = s[j - 1];
       s[j - 1] = s[j];
       s[j] = x;
+      const int o = ord[j - 1];
+      ord[j - 1] = ord[j];
+      ord[j] = o;
     }
+  const double sr[4] = {s[0], s[1], s[2], s[3]};
   const int i0 = first_edge_above(frhis, nwhis, s[0]);
   const int i1 = first_edge_above(frhis, nwhis, s[3]);
   if (i0 > nwhis || i1 > nwhis) return false;
@@ -149,7 +228,17 @@ TET_HD inline bool bin_piece(const Vert* p, const double* frhis, int nwhis, Emit
   double below = 0.0;
   for (int i = i0; i <= i1; ++i) {
     const double now = frhis[i] > s[3] ? 1.0 : cum_fraction(s, frhis[i]);
-    emit(i, vol * (now - below));
+    if constexpr (Linear) {
+      double wcg[4], lam[4];
+      corner_weights(sr, 0.5 * (frhis[i - 1] + frhis[i]), wcg);
+      for (int c = 0; c < 4; ++c) {
+        lam[c] = 0.0;
+        for (int j = 0; j < 4; ++j) lam[c] += p[ord[j]].b[c] * wcg[j];
+      }
+      emit(i, 4.0 * vol * (now - below), lam);
+    } else {
+      emit(i, vol * (now - below));
+    }
     below = now;
   }
   return true;
@@ -157,9 +246,10 @@ TET_HD inline bool bin_piece(const Vert* p, const double* frhis, int nwhis, Emit
 
 // All bins of one band pair over one tetrahedron (corner positions kc,
 // energies ea, eb at the corners).  false when w leaves the histogram mesh.
-template <class Emit>
+template <bool Linear = false, class Emit>
 TET_HD inline bool pair_histogram(const double kc[4][3], const double ea[4], const double eb[4], double ef,
                                   const double* frhis, int nwhis, Emit& emit) {
+  TET_UNFUSED
   // clip 1: the occupied part, g = E_F - ea > 0; ranked by ea descending,
   // equal energies later corner first
   Vert lab[4];
@@ -173,6 +263,7 @@ TET_HD inline bool pair_histogram(const double kc[4][3], const double ea[4], con
     for (int d = 0; d < 3; ++d) lab[r].k[d] = kc[c][d];
     lab[r].w = 0.5 * (eb[c] - ea[c]);
     lab[r].e2 = eb[c] - ef;
+    for (int d = 0; d < 4; ++d) lab[r].b[d] = d == c ? 1.0 : 0.0;
     g[r] = ef - ea[c];
     nin += g[r] > 0.0;
   }
@@ -184,7 +275,7 @@ TET_HD inline bool pair_histogram(const double kc[4][3], const double ea[4], con
     Vert v[4], lab2[4];
     double g2[4];
     int lb2[4];
-    for (int i = 0; i < 4; ++i) v[i] = clip_vertex(lab, g, (unsigned)(code1 >> (16 * p1 + 4 * i)) & 15u);
+    for (int i = 0; i < 4; ++i) v[i] = clip_vertex(lab, g, (unsigned)(code1 >> (16 * p1 + 4 * i)) & 15u, nin == 3);
     for (int i = 0; i < 4; ++i) g2[i] = v[i].e2;
     rank4(g2, false, lb2);
     int nin2 = 0;
@@ -196,8 +287,9 @@ TET_HD inline bool pair_histogram(const double kc[4][3], const double ea[4], con
     const uint64_t code2 = clip_code(nin2);
     for (int p2 = 0; p2 < clip_pieces(nin2); ++p2) {
       Vert q[4];
-      for (int i = 0; i < 4; ++i) q[i] = clip_vertex(lab2, g2, (unsigned)(code2 >> (16 * p2 + 4 * i)) & 15u);
-      ok = bin_piece(q, frhis, nwhis, emit) && ok;
+      for (int i = 0; i < 4; ++i)
+        q[i] = clip_vertex(lab2, g2, (unsigned)(code2 >> (16 * p2 + 4 * i)) & 15u, nin2 == 3);
+      ok = bin_piece<Linear>(q, frhis, nwhis, emit) && ok;
     }
   }
   return ok;

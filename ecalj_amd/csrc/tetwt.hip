@@ -82,16 +82,24 @@ __device__ inline void atomic_max_d(double* a, double v) {
 }
 
 // The four corners' histogram windows of one pair: a bin of the
-// tetrahedron's histogram goes to every corner whose window holds it.
+// tetrahedron's histogram goes to every corner whose window holds it, in
+// full (times 4 wtet = 1), or with matrix_linear the corner's share lam[c]
+// of 4 W (S/tetwt5.f90:329-340).
 struct CornerBins {
   int ihw[4], nhw[4], jhw[4];
-  double scale;  // pi / (4 vol_BZ) * 4 * (1/4)
+  double scale;  // pi / (4 vol_BZ)
   double* whw;
   __device__ void operator()(int ihis, double w) const {
     const double v = w * scale;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       if (ihis >= ihw[c] && ihis < ihw[c] + nhw[c]) atomicAdd(whw + (jhw[c] - 1) + (ihis - ihw[c]), v);
+  }
+  __device__ void operator()(int ihis, double w4, const double* lam) const {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (ihis >= ihw[c] && ihis < ihw[c] + nhw[c])
+        atomicAdd(whw + (jhw[c] - 1) + (ihis - ihw[c]), w4 * lam[c] * scale);
   }
 };
 
@@ -290,6 +298,7 @@ __global__ void tw_jhw_kernel(const int* __restrict__ nbnb, const long long* __r
 }
 
 // grid (pairs, tetrahedra, npm)
+template <bool Linear>
 __global__ __launch_bounds__(256) void tw_job1_kernel(const TetRow* __restrict__ tets, const double* __restrict__ qbzw,
                                                       const double* __restrict__ e1, const double* __restrict__ e2,
                                                       int nband, PairDims pd, double ef,
@@ -315,7 +324,7 @@ __global__ __launch_bounds__(256) void tw_job1_kernel(const TetRow* __restrict__
     s.jhw[c] = jhw[o];
     for (int d = 0; d < 3; ++d) kv[c][d] = qbzw[d + 3 * (size_t)t.w[c]];
   }
-  if (!tet::pair_histogram(kv, eo, eu, ef, frhis, nwhis, s)) *err = 3;  // outside the histogram mesh
+  if (!tet::pair_histogram<Linear>(kv, eo, eu, ef, frhis, nwhis, s)) *err = 3;  // outside the histogram mesh
 }
 
 // degenerate bands: every pair of a block takes the block's average histogram
@@ -392,7 +401,7 @@ extern "C" ecalj_status ecalj_tetwt_full(ecalj_handle h, int nband, int nqbz, in
                                          int nqbzw, const double* qbzw, const int* ib1bz, const double* ekxx1,
                                          const double* ekxx2, double ef, double ebmx, const double* qlat,
                                          const double* frhis, int nwhis, int npm, int nctot, const double* ecore,
-                                         int nbnbx_cap, int64_t nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw,
+                                         int matrix_linear, int nbnbx_cap, int64_t nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw,
                                          int* nhw, int* jhw, double* whw, int* nbnbx_out, int64_t* nhwtot_out) {
   if (!h || !idtetf || !qbzw || !ib1bz || !ekxx1 || !ekxx2 || !qlat || !frhis || !nbnbx_out || !nhwtot_out)
     return ECALJ_ERR_ARG;
@@ -562,9 +571,10 @@ extern "C" ecalj_status ecalj_tetwt_full(ecalj_handle h, int nband, int nqbz, in
                                   qb[0][1] * qb[1][0] * qb[2][2] + qb[0][1] * qb[1][2] * qb[2][0] +
                                   qb[0][2] * qb[1][0] * qb[2][1] - qb[0][2] * qb[1][1] * qb[2][0]);
   const double pi = 3.14159265358979323846;
-  const double scale = pi / 4.0 / voltot * 4.0 * 0.25;  // piofvoltot * 4 * wtet (S/tetwt5.f90:342-343)
+  const double scale = pi / 4.0 / voltot;  // piofvoltot (S/tetwt5.f90:335-338)
   if (maxp > 0) {
-    hipLaunchKernelGGL(tw_job1_kernel, dim3((maxp + 255) / 256, ntetf, npm), dim3(256), 0, st, d_rows.as<TetRow>(),
+    hipLaunchKernelGGL(matrix_linear ? tw_job1_kernel<true> : tw_job1_kernel<false>,
+                       dim3((maxp + 255) / 256, ntetf, npm), dim3(256), 0, st, d_rows.as<TetRow>(),
                        d_qw.as<double>(), d_ek1.as<double>(), d_ek2.as<double>(), nband, pd, ef, d_ibjb.as<int>(),
                        ld, d_ihw.as<int>(), d_nhw.as<int>(), d_jhw.as<int>(), d_fr.as<double>(), nwhis, scale,
                        d_whw.as<double>(), d_err.as<int>());
@@ -594,6 +604,6 @@ extern "C" ecalj_status ecalj_tetwt(ecalj_handle h, int nband, int nqbz, int nte
                                     int nbnbx_cap, int64_t nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw,
                                     int* nhw, int* jhw, double* whw, int* nbnbx_out, int64_t* nhwtot_out) {
   return ecalj_tetwt_full(h, nband, nqbz, ntetf, idtetf, nqbzw, qbzw, ib1bz, ekxx1, ekxx2, ef, ebmx, qlat, frhis,
-                          nwhis, 1, 0, nullptr, nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, jhw, whw, nbnbx_out,
+                          nwhis, 1, 0, nullptr, 0, nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, jhw, whw, nbnbx_out,
                           nhwtot_out);
 }

@@ -229,11 +229,11 @@ module ecalj_gpu
     ! gettetwt with npm = 1 or 2 and nctot core states ecore(1:nctot, is) (S/m_tetwt.f90:36-256,
     ! S/tetwt5.f90:179-185, 252-264, 349-427); lists (nbnbx_cap, nqbz, npm), nbnb(nqbz, npm)
     integer(c_int) function ecalj_tetwt_full(h, nband, nqbz, ntetf, idtetf, nqbzw, qbzw, ib1bz, ekxx1, ekxx2, ef, &
-        ebmx, qlat, frhis, nwhis, npm, nctot, ecore, nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, jhw, whw, &
-        nbnbx_out, nhwtot_out) bind(C, name="ecalj_tetwt_full")
+        ebmx, qlat, frhis, nwhis, npm, nctot, ecore, matrix_linear, nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, &
+        jhw, whw, nbnbx_out, nhwtot_out) bind(C, name="ecalj_tetwt_full")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      integer(c_int), value :: nband, nqbz, ntetf, nqbzw, nwhis, npm, nctot, nbnbx_cap
+      integer(c_int), value :: nband, nqbz, ntetf, nqbzw, nwhis, npm, nctot, matrix_linear, nbnbx_cap
       integer(c_int64_t), value :: nhw_cap
       integer(c_int), intent(in) :: idtetf(*), ib1bz(*)
       real(c_double), intent(in) :: qbzw(*), ekxx1(*), ekxx2(*), qlat(*), frhis(*), ecore(*)

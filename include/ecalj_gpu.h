@@ -315,12 +315,19 @@ ecalj_status ecalj_tetwt(ecalj_handle h, int nband, int nqbz, int ntetf, const i
  * (n1b up to nband + nctot; with npm = 2, n2b up to nband + nctot for
  * jpm = 2).  Outputs gain a trailing jpm axis: nbnb(nqbz, npm); n1b, n2b,
  * ihw, nhw, jhw (nbnbx_cap, nqbz, npm); whw pointers run over jpm, then k,
- * then the pairs.  ecalj_tetwt is this call with npm = 1, nctot = 0. */
+ * then the pairs.  matrix_linear != 0 is the GWinput switch
+ * tetrahedron_matrix_linear (S/switch.f90:253-263; S/tetwt5.f90:334-336,
+ * 694-709, 751-759, mkwcg :788-840): each bin's weight goes to the four
+ * corner k-points of a tetrahedron in the barycentric shares of the
+ * cross-section's vertex average at the bin centre instead of equally; the
+ * lists are unchanged.  ecalj_tetwt is this call with npm = 1, nctot = 0,
+ * matrix_linear = 0. */
 ecalj_status ecalj_tetwt_full(ecalj_handle h, int nband, int nqbz, int ntetf, const int* idtetf,
                               int nqbzw, const double* qbzw, const int* ib1bz,
                               const double* ekxx1, const double* ekxx2, double ef, double ebmx,
                               const double* qlat, const double* frhis, int nwhis, int npm,
-                              int nctot, const double* ecore, int nbnbx_cap, int64_t nhw_cap,
+                              int nctot, const double* ecore, int matrix_linear, int nbnbx_cap,
+                              int64_t nhw_cap,
                               int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw,
                               double* whw, int* nbnbx_out, int64_t* nhwtot_out);
 

@@ -207,6 +207,20 @@ def lindtet6(kvec, ea, eb, x, efa, efb, frhis_):
     return out
 
 
+def lindtet6_ml(kvec, ea, eb, x, efa, efb, frhis_):
+    """The same with tetrahedron_matrix_linear: nwhis x 4, column c the
+    weight carried by corner c of kvec."""
+    fr = _cd(frhis_)
+    nwhis = len(fr) - 1
+    out = np.zeros((nwhis, 4), order="F")
+    k = np.ascontiguousarray(np.asarray(kvec, np.float64))
+    rc = lib().orc_lindtet6_ml(k.ctypes.data_as(_dp), _p(_cd(ea)), _p(_cd(eb)), _p(_cd(x)), C.c_double(efa),
+                               C.c_double(efb), _p(fr), nwhis, _p(out))
+    if rc != 0:
+        raise ValueError(f"lindtet6_ml rc={rc}")
+    return np.ascontiguousarray(out)
+
+
 def chkdgn(ene):
     e = _cd(ene)
     a = np.zeros(len(e), np.int32)
@@ -215,10 +229,10 @@ def chkdgn(ene):
     return a[:n], b[:n]
 
 
-def tetwt(mesh, ek1, ek2, ef, frhis_, ebmx=1e10, npm=1, ecore=None):
+def tetwt(mesh, ek1, ek2, ef, frhis_, ebmx=1e10, npm=1, ecore=None, matrix_linear=False):
     """gettetwt (npm = 1 or 2, core states of energies ecore appended after
-    the nband valence bands): the reference's index sets (1-based values) and
-    whw.  mesh: synth.TetraMesh.  With npm = 1 the arrays are (nbnbx, nqbz)
+    the nband valence bands, tetrahedron_matrix_linear): the reference's
+    index sets (1-based values) and whw.  mesh: synth.TetraMesh.  With npm = 1 the arrays are (nbnbx, nqbz)
     and nbnb (nqbz,); with npm = 2 a trailing jpm axis is kept."""
     ek1 = _cd(ek1)
     ek2 = _cd(ek2)
@@ -239,7 +253,7 @@ def tetwt(mesh, ek1, ek2, ef, frhis_, ebmx=1e10, npm=1, ecore=None):
         nbx, nht = C.c_int(0), C.c_int(0)
         rc = lib().orc_tetwt_full(nband, nqbz, idt.shape[1], _ip_(idt), _p(qw), _ip_(ib1), _p(ek1), _p(ek2),
                                   C.c_double(ef), C.c_double(ebmx), _p(ql), _p(fr), nwhis, npm, nctot,
-                                  _p(ec) if nctot else None, cap_b, cap_h, _ip_(nbnb), *[_ip_(a) for a in arrs],
+                                  _p(ec) if nctot else None, int(bool(matrix_linear)), cap_b, cap_h, _ip_(nbnb), *[_ip_(a) for a in arrs],
                                   _p(whw), C.byref(nbx), C.byref(nht))
         if rc == -1:
             cap_b, cap_h = max(nbx.value, 1), max(nht.value, 1)

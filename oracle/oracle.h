@@ -33,6 +33,13 @@
  *   orc_qp_energy      closed form of S/main_hqpe.sc.f90:160-161
  *   orc_lindtet6       Monte Carlo of f(ea)(1-f(eb)) delta(omega+x) over the
  *                      tetrahedron, every occupied/unoccupied cut figure
+ *   orc_lindtet6_ml    corner sum = 4 x orc_lindtet6; on an uncut tetrahedron
+ *                      the vertex average of the cross-section polygon at
+ *                      the bin centre (brute force over the six edges);
+ *                      sum_c w_c wt_c within the half bin width of the bin
+ *                      centre; the point inside the occupied/unoccupied
+ *                      region; invariance under an affine map of k and
+ *                      covariance under relabelling the corners
  *   orc_tetwt          brute-force pair sets; degenerate-band equality;
  *                      Monte Carlo of the whole-mesh histogram
  *   orc_sylm/sylmnc    oracle/_ref (S/sylm.f90 compiled with flang)
@@ -239,12 +246,16 @@ int orc_tetwt(int nband, int nqbz, int ntetf, const int* idtetf, const double* q
  * nband + (npm == 2 ? nctot : 0).  orc_tetwt is npm = 1 without core. */
 int orc_tetwt_full(int nband, int nqbz, int ntetf, const int* idtetf, const double* qbzw, const int* ib1bz,
                    const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
-                   const double* frhis, int nwhis, int npm, int nctot, const double* ecore, int nbnbx_cap,
-                   int nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw, double* whw,
+                   const double* frhis, int nwhis, int npm, int nctot, const double* ecore, int matrix_linear,
+                   int nbnbx_cap, int nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw, double* whw,
                    int* nbnbx_out, int* nhwtot_out);
 /* lindtet6 (S/tetwt5.f90:514-600): histogram of one tetrahedron; kvec[corner][xyz] */
 int orc_lindtet6(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
                  double efermia, double efermib, const double* frhis, int nwhis, double* wtthis);
+/* the same with tetrahedron_matrix_linear (S/tetwt5.f90:694-709, 751-759): wtthis(nwhis, 4), column c
+ * the weight carried by corner c of kvec, summing over c to 4 times orc_lindtet6's */
+int orc_lindtet6_ml(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
+                    double efermia, double efermib, const double* frhis, int nwhis, double* wtthis);
 int orc_hisrange(const double* frhis, int nwhis, double demin, double demax, int* ihw, int* nhw);
 int orc_chkdgn(const double* ene, int ndat, int* ixini, int* ixend);
 

@@ -5,7 +5,7 @@
  * cpu_baseline leg of bench.py.
  *
  * Scope: npm = 1 (time reversal) or 2, nctot core states, nmtet = 1,
- * tetrahedron_matrix_linear = F, no tetrakbt,
+ * tetrahedron_matrix_linear = F or T (S/switch.f90:253-263), no tetrakbt,
  * no --interbandonly/--intrabandonly filters, no Fermi-level shift.
  *
  * References (SRC/subroutines = S/):
@@ -16,6 +16,8 @@
  *   inttetra6         S/tetwt5.f90:601-672
  *   inttetrac6        S/tetwt5.f90:673-713
  *   intttvc6          S/tetwt5.f90:714-787
+ *   mkwcg, wab        S/tetwt5.f90:788-840
+ *   minv33tp          S/util.f90:202-212
  *   midk3, midk       S/tetwt5.f90:848-857, 913-919
  *   chkdgn            S/tetwt5.f90:868-912
  *   rsvwwk00_4        S/tetwt5.f90:920-942
@@ -72,8 +74,66 @@ static int integtetn(const double e[4], double ee, double* out) {
   return 0;
 }
 
-/* intttvc6, matrix_linear = F: wtthis(ihis) += int_bin d omega int d^3k delta(omega + v(k)) */
-static int intttvc6(const double v[4], double voltet, const double* frhis, int nwhis, double* wtthis) {
+/* What the per-tetrahedron routines share: the main tetrahedron's corners kkv (in idtetf order) and
+ * whether the weights are split over its corners (tetrahedron_matrix_linear).  The histogram wtthis
+ * is wtthis(nwhis, 4) column-major; without matrix_linear only column 1 is used. */
+typedef struct {
+  int ml;
+  double kkv[4][3];
+} tetctx;
+
+/* mkwcg: weight of each corner (ascending energies e) in the point that represents the
+ * cross-section of the tetrahedron at energy ee; the weights sum to 1 (S/tetwt5.f90:788-833) */
+static void mkwcg(const double e[4], double ee, double wcg[4]) {
+  const double e1 = e[0] - 3e-8, e2 = e[1] - 2e-8, e3 = e[2] - 1e-8, e4 = e[3];
+#define WAB(ea, eb, wa, wb) \
+  do {                      \
+    const double t_ = (eb) - (ea); \
+    wa = ((eb) - ee) / t_;  \
+    wb = (ee - (ea)) / t_;  \
+  } while (0)
+  double w12_1, w12_2, w13_1, w13_3, w14_1, w14_4, w23_2, w23_3, w24_2, w24_4, w34_3, w34_4;
+  if (ee <= e[0]) {
+    wcg[0] = 1.0;
+    wcg[1] = wcg[2] = wcg[3] = 0.0;
+  } else if (e1 <= ee && ee < e2) {
+    WAB(e1, e2, w12_1, w12_2);
+    WAB(e1, e3, w13_1, w13_3);
+    WAB(e1, e4, w14_1, w14_4);
+    wcg[0] = (w12_1 + w13_1 + w14_1) / 3.0;
+    wcg[1] = w12_2 / 3.0;
+    wcg[2] = w13_3 / 3.0;
+    wcg[3] = w14_4 / 3.0;
+  } else if (e2 <= ee && ee < e3) {
+    WAB(e1, e3, w13_1, w13_3);
+    WAB(e1, e4, w14_1, w14_4);
+    WAB(e2, e3, w23_2, w23_3);
+    WAB(e2, e4, w24_2, w24_4);
+    wcg[0] = (w13_1 + w14_1) / 4.0;
+    wcg[1] = (w23_2 + w24_2) / 4.0;
+    wcg[2] = (w13_3 + w23_3) / 4.0;
+    wcg[3] = (w14_4 + w24_4) / 4.0;
+  } else if (e3 <= ee && ee < e4) {
+    WAB(e1, e4, w14_1, w14_4);
+    WAB(e2, e4, w24_2, w24_4);
+    WAB(e3, e4, w34_3, w34_4);
+    wcg[0] = w14_1 / 3.0;
+    wcg[1] = w24_2 / 3.0;
+    wcg[2] = w34_3 / 3.0;
+    wcg[3] = (w24_4 + w34_4 + w14_4) / 3.0;
+  } else {
+    /* ee > e(4); the reference leaves wcg as it was for ee == e(4) exactly, taken here as the top corner */
+    wcg[3] = 1.0;
+    wcg[0] = wcg[1] = wcg[2] = 0.0;
+  }
+#undef WAB
+}
+
+/* intttvc6: wtthis(ihis) += int_bin d omega int d^3k delta(omega + v(k)); with matrix_linear the
+ * bin's weight (times 4) is split over the main corners by kkvkin(ix, i), the coordinates of this
+ * tetrahedron's corner i in the main corners, applied to mkwcg at the bin centre */
+static int intttvc6(const tetctx* cx, const double kkvkin[4][4], const double v[4], double voltet,
+                    const double* frhis, int nwhis, double* wtthis) {
   double mv[4], WW[4];
   int ord[4];
   for (int i = 0; i < 4; ++i) mv[i] = -v[i];
@@ -102,16 +162,41 @@ static int intttvc6(const double v[4], double voltet, const double* frhis, int n
     } else if (integtetn(WW, frhis[ihis], &integb)) {
       return -4;
     }
-    wtthis[ihis - 1] += pvn * (integb - intega);
+    if (cx->ml) {
+      double wcg[4];
+      mkwcg(WW, 0.5 * (frhis[ihis - 1] + frhis[ihis]), wcg);
+      const double wttt = pvn * (integb - intega) * 4.0;
+      for (int c = 0; c < 4; ++c) {
+        double www = 0.0;
+        for (int j = 0; j < 4; ++j) www += kkvkin[c][ord[j]] * wcg[j];
+        wtthis[ihis - 1 + (size_t)nwhis * c] += wttt * www;
+      }
+    } else {
+      wtthis[ihis - 1] += pvn * (integb - intega);
+    }
     intega = integb;
   }
   return 0;
 }
 
+/* minv33tp: the reciprocal vectors of the columns of a (columns of b; b(:,j).a(:,i) = delta_ij) */
+static void minv33tp(const double a[3][3], double b[3][3]) {
+  for (int j = 0; j < 3; ++j) {
+    const double* u = a[(j + 1) % 3];
+    const double* w = a[(j + 2) % 3];
+    b[j][0] = u[1] * w[2] - u[2] * w[1];
+    b[j][1] = u[2] * w[0] - u[0] * w[2];
+    b[j][2] = u[0] * w[1] - u[1] * w[0];
+  }
+  const double det = a[0][0] * b[0][0] + a[0][1] * b[0][1] + a[0][2] * b[0][2];
+  for (int j = 0; j < 3; ++j)
+    for (int d = 0; d < 3; ++d) b[j][d] /= det;
+}
+
 /* corners: kk[8][3], xx[8]; itetx 1-based as in the reference */
-static int inttetrac6(double kk[8][3], const double xx[8], const int itetx[4], const double* frhis, int nwhis,
-                      double* wtthis) {
-  double xin[4], kin[4][3], am[3][3];
+static int inttetrac6(const tetctx* cx, double kk[8][3], const double xx[8], const int itetx[4],
+                      const double* frhis, int nwhis, double* wtthis) {
+  double xin[4], kin[4][3], am[3][3], kkvkin[4][4] = {{0}};
   for (int i = 0; i < 4; ++i) {
     xin[i] = xx[itetx[i] - 1];
     for (int d = 0; d < 3; ++d) kin[i][d] = kk[itetx[i] - 1][d];
@@ -119,7 +204,19 @@ static int inttetrac6(double kk[8][3], const double xx[8], const int itetx[4], c
   for (int i = 0; i < 3; ++i)
     for (int d = 0; d < 3; ++d) am[i][d] = kin[i][d] - kin[3][d];
   const double voltet = fabs(det33(am) / 6.0);
-  return intttvc6(xin, voltet, frhis, nwhis, wtthis);
+  if (cx->ml) { /* kin(:, i) = sum_c kkv(:, c) kkvkin(c, i) */
+    double bm[3][3], kkv4bm[3];
+    for (int i = 0; i < 3; ++i)
+      for (int d = 0; d < 3; ++d) am[i][d] = cx->kkv[i][d] - cx->kkv[3][d];
+    minv33tp(am, bm);
+    for (int j = 0; j < 3; ++j) kkv4bm[j] = cx->kkv[3][0] * bm[j][0] + cx->kkv[3][1] * bm[j][1] + cx->kkv[3][2] * bm[j][2];
+    for (int i = 0; i < 4; ++i) {
+      for (int j = 0; j < 3; ++j)
+        kkvkin[j][i] = kin[i][0] * bm[j][0] + kin[i][1] * bm[j][1] + kin[i][2] * bm[j][2] - kkv4bm[j];
+      kkvkin[3][i] = 1.0 - (kkvkin[0][i] + kkvkin[1][i] + kkvkin[2][i]);
+    }
+  }
+  return intttvc6(cx, (const double(*)[4])kkvkin, xin, voltet, frhis, nwhis, wtthis);
 }
 
 /* point on the edge k(i)--k(j) where ee crosses zero (i, j 1-based) */
@@ -137,8 +234,8 @@ static void midk3(double kk[8][3], const double* ee, double* xx, double* yy, int
 }
 
 /* the unoccupied cut (eb > E_F) of one occupied sub-tetrahedron */
-static int inttetra6(double kk_[8][3], const double xx_[8], const double ebf[8], const int itetx[4],
-                     const double* frhis, int nwhis, double* wtthis) {
+static int inttetra6(const tetctx* cx, double kk_[8][3], const double xx_[8], const double ebf[8],
+                     const int itetx[4], const double* frhis, int nwhis, double* wtthis) {
   double xin[4], kin[4][3], ebfin[4], kk[8][3], xx[8], ee[4];
   int ord[4];
   for (int i = 0; i < 4; ++i) {
@@ -159,32 +256,33 @@ static int inttetra6(double kk_[8][3], const double xx_[8], const double ebf[8],
     midk(kk, ee, xx, 4, 2, 5);
     midk(kk, ee, xx, 4, 1, 6);
     midk(kk, ee, xx, 4, 3, 7);
-    rc |= inttetrac6(kk, xx, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
   } else if (ee[2] > 0.0 && 0.0 >= ee[1]) { /* Fig 2 */
     midk(kk, ee, xx, 4, 2, 5);
     midk(kk, ee, xx, 4, 1, 6);
     midk(kk, ee, xx, 3, 1, 7);
     midk(kk, ee, xx, 3, 2, 8);
-    rc |= inttetrac6(kk, xx, (const int[]){4, 3, 5, 6}, frhis, nwhis, wtthis);
-    rc |= inttetrac6(kk, xx, (const int[]){3, 6, 7, 5}, frhis, nwhis, wtthis);
-    rc |= inttetrac6(kk, xx, (const int[]){3, 5, 7, 8}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){4, 3, 5, 6}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){3, 6, 7, 5}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){3, 5, 7, 8}, frhis, nwhis, wtthis);
   } else if (ee[1] > 0.0 && 0.0 >= ee[0]) { /* Fig 3 */
     midk(kk, ee, xx, 1, 4, 5);
     midk(kk, ee, xx, 1, 2, 6);
     midk(kk, ee, xx, 1, 3, 7);
-    rc |= inttetrac6(kk, xx, (const int[]){3, 4, 7, 2}, frhis, nwhis, wtthis);
-    rc |= inttetrac6(kk, xx, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
-    rc |= inttetrac6(kk, xx, (const int[]){4, 2, 6, 7}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){3, 4, 7, 2}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){4, 2, 6, 7}, frhis, nwhis, wtthis);
   } else { /* Fig 4 */
-    rc |= inttetrac6(kk, xx, (const int[]){1, 2, 3, 4}, frhis, nwhis, wtthis);
+    rc |= inttetrac6(cx, kk, xx, (const int[]){1, 2, 3, 4}, frhis, nwhis, wtthis);
   }
   return rc;
 }
 
 /* occupied cut (ea < E_F) of one tetrahedron, then the unoccupied cut of each piece:
  * wtthis(ihis) = int_bin d omega int d^3k f(ea)(1-f(eb)) delta(omega + x(k)) */
-int orc_lindtet6(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
-                 double efermia, double efermib, const double* frhis, int nwhis, double* wtthis) {
+static int lindtet6(const tetctx* cx, const double kvec[4][3], const double ea[4], const double eb[4],
+                    const double x[4], double efermia, double efermib, const double* frhis, int nwhis,
+                    double* wtthis) {
   int ord[4];
   sortea(ea, ord, 4);
   int r[4] = {ord[3], ord[2], ord[1], ord[0]}; /* descending ea */
@@ -202,26 +300,43 @@ int orc_lindtet6(const double kvec[4][3], const double ea[4], const double eb[4]
     midk3(kk, ee, xx, ebf, 4, 2, 5);
     midk3(kk, ee, xx, ebf, 4, 1, 6);
     midk3(kk, ee, xx, ebf, 4, 3, 7);
-    rc |= inttetra6(kk, xx, ebf, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
   } else if (ee[2] < 0.0 && 0.0 <= ee[1]) { /* Fig 2 */
     midk3(kk, ee, xx, ebf, 4, 2, 5);
     midk3(kk, ee, xx, ebf, 4, 1, 6);
     midk3(kk, ee, xx, ebf, 3, 1, 7);
     midk3(kk, ee, xx, ebf, 3, 2, 8);
-    rc |= inttetra6(kk, xx, ebf, (const int[]){4, 3, 5, 6}, frhis, nwhis, wtthis);
-    rc |= inttetra6(kk, xx, ebf, (const int[]){3, 6, 7, 5}, frhis, nwhis, wtthis);
-    rc |= inttetra6(kk, xx, ebf, (const int[]){3, 5, 7, 8}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){4, 3, 5, 6}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){3, 6, 7, 5}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){3, 5, 7, 8}, frhis, nwhis, wtthis);
   } else if (ee[1] < 0.0 && 0.0 <= ee[0]) { /* Fig 3 */
     midk3(kk, ee, xx, ebf, 1, 4, 5);
     midk3(kk, ee, xx, ebf, 1, 2, 6);
     midk3(kk, ee, xx, ebf, 1, 3, 7);
-    rc |= inttetra6(kk, xx, ebf, (const int[]){3, 4, 7, 2}, frhis, nwhis, wtthis);
-    rc |= inttetra6(kk, xx, ebf, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
-    rc |= inttetra6(kk, xx, ebf, (const int[]){4, 2, 6, 7}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){3, 4, 7, 2}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){4, 5, 6, 7}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){4, 2, 6, 7}, frhis, nwhis, wtthis);
   } else { /* Fig 4 */
-    rc |= inttetra6(kk, xx, ebf, (const int[]){1, 2, 3, 4}, frhis, nwhis, wtthis);
+    rc |= inttetra6(cx, kk, xx, ebf, (const int[]){1, 2, 3, 4}, frhis, nwhis, wtthis);
   }
   return rc;
+}
+
+int orc_lindtet6(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
+                 double efermia, double efermib, const double* frhis, int nwhis, double* wtthis) {
+  tetctx cx;
+  cx.ml = 0;
+  memcpy(cx.kkv, kvec, sizeof(cx.kkv));
+  return lindtet6(&cx, kvec, ea, eb, x, efermia, efermib, frhis, nwhis, wtthis);
+}
+
+/* matrix_linear: wtthis(nwhis, 4), column c for corner c of kvec (nmtet = 1: kkv = kvec) */
+int orc_lindtet6_ml(const double kvec[4][3], const double ea[4], const double eb[4], const double x[4],
+                    double efermia, double efermib, const double* frhis, int nwhis, double* wtthis) {
+  tetctx cx;
+  cx.ml = 1;
+  memcpy(cx.kkv, kvec, sizeof(cx.kkv));
+  return lindtet6(&cx, kvec, ea, eb, x, efermia, efermib, frhis, nwhis, wtthis);
 }
 
 int orc_hisrange(const double* frhis, int nwhis, double demin, double demax, int* ihw, int* nhw) {
@@ -281,8 +396,8 @@ static void pair_bands(int jpm, int ibx, int jbx, int nocc_k, int nocc_kq, int n
 
 int orc_tetwt_full(int nband, int nqbz, int ntetf, const int* idtetf, const double* qbzw, const int* ib1bz,
                    const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
-                   const double* frhis, int nwhis, int npm, int nctot, const double* ecore, int nbnbx_cap,
-                   int nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw, double* whw,
+                   const double* frhis, int nwhis, int npm, int nctot, const double* ecore, int matrix_linear,
+                   int nbnbx_cap, int nhw_cap, int* nbnb, int* n1b, int* n2b, int* ihw, int* nhw, int* jhw, double* whw,
                    int* nbnbx_out, int* nhwtot_out) {
   if (npm < 1 || npm > 2 || nctot < 0 || (nctot > 0 && !ecore)) return -5;
   double qb[3][3];
@@ -368,17 +483,25 @@ int orc_tetwt_full(int nband, int nqbz, int ntetf, const int* idtetf, const doub
               }
               continue;
             }
-            double* wt = (double*)calloc(nwhis, sizeof(double));
-            if (orc_lindtet6((const double(*)[3])kvec, eocc, eun, x, ef, ef, frhis, nwhis, wt)) {
+            double* wt = (double*)calloc(4 * (size_t)nwhis, sizeof(double));
+            tetctx cx;
+            cx.ml = matrix_linear != 0;
+            memcpy(cx.kkv, kvec, sizeof(cx.kkv));
+            if (lindtet6(&cx, (const double(*)[3])kvec, eocc, eun, x, ef, ef, frhis, nwhis, wt)) {
               rc = -3;
               free(wt);
               break;
             }
+            /* S/tetwt5.f90:329-340: corner c takes column c with matrix_linear, else column 1 times
+             * 4 wtet (wtet = 1/4, :180) */
             for (int c = 0; c < 4; ++c) {
               const int ibib = ibjb[IDX4(ib, jb, kk[c], jpm)] - 1;
               const size_t o = (size_t)ibib + (size_t)nbnbx_cap * (kk[c] + (size_t)nqbz * jpm);
               const int jini = jhw[o] - 1, iini = ihw[o] - 1, nnn = nhw[o];
-              for (int i = 0; i < nnn; ++i) whw[jini + i] += wt[iini + i] * piofvoltot * 4.0 * 0.25;
+              if (cx.ml)
+                for (int i = 0; i < nnn; ++i) whw[jini + i] += wt[iini + i + (size_t)nwhis * c] * piofvoltot;
+              else
+                for (int i = 0; i < nnn; ++i) whw[jini + i] += wt[iini + i] * piofvoltot * 4.0 * 0.25;
             }
             free(wt);
           }
@@ -504,6 +627,6 @@ int orc_tetwt(int nband, int nqbz, int ntetf, const int* idtetf, const double* q
               const double* ek1, const double* ek2, double ef, double ebmx, const double* qlat,
               const double* frhis, int nwhis, int nbnbx_cap, int nhw_cap, int* nbnb, int* n1b, int* n2b,
               int* ihw, int* nhw, int* jhw, double* whw, int* nbnbx_out, int* nhwtot_out) {
-  return orc_tetwt_full(nband, nqbz, ntetf, idtetf, qbzw, ib1bz, ek1, ek2, ef, ebmx, qlat, frhis, nwhis, 1, 0, NULL,
+  return orc_tetwt_full(nband, nqbz, ntetf, idtetf, qbzw, ib1bz, ek1, ek2, ef, ebmx, qlat, frhis, nwhis, 1, 0, NULL, 0,
                         nbnbx_cap, nhw_cap, nbnb, n1b, n2b, ihw, nhw, jhw, whw, nbnbx_out, nhwtot_out);
 }

@@ -1,5 +1,5 @@
 """Tetrahedron histogram weights on the GPU (ecalj_tetwt, ecalj_tetwt_full
-with npm = 2 and core states) against the oracle (orc_tetwt_full, pinned in
+with npm = 2, core states and tetrahedron_matrix_linear) against the oracle (orc_tetwt_full, pinned in
 test_tetwt_oracle.py): index maps bit-exact (n1b, n2b,
 ihw, nhw, jhw, nbnb).  Weights: each (k, pair) total within 1e-12 (the bin
 sums of a piece telescope to its volume), single bins within 1e-8 of the
@@ -84,6 +84,43 @@ def test_tetwt_npm_core_vs_oracle(h, orc, n, nocc, nun, seed, npm, ncore):
         if npm == 2:
             assert o["n2b"][..., 1].max() > ek1.shape[0]
     _check_weights(g, o)
+
+
+@pytest.mark.parametrize("n,nocc,nun,seed,npm,ncore,degenerate", [((2, 2, 2), 4, 5, 21, 1, 0, True),
+                                                                  ((3, 3, 3), 6, 20, 22, 1, 0, True),
+                                                                  ((3, 2, 4), 5, 12, 23, 2, 3, False)])
+def test_tetwt_matrix_linear_vs_oracle(h, orc, n, nocc, nun, seed, npm, ncore, degenerate):
+    """tetrahedron_matrix_linear: the same lists as without it, bit-exact
+    with the oracle; each bin's weight split over the corner k-points.  Bars:
+    single bins as above; a (k, pair) total no longer telescopes (a bin's
+    share depends on the bin), so it takes the single-bin bar; the sum of
+    all weights telescopes (1e-12)."""
+    say("tetwt matrix_linear", n, nocc, nun, seed, "npm", npm, "core", ncore)
+    m = synth.tetra_mesh(*n)
+    ek1, ek2, ef = synth.tetra_bands(m, nocc, nun, seed=seed, degenerate=degenerate)
+    ecore = np.sort(np.array([-32.0, -32.0, -19.5][:ncore]) / synth.RYDBERG_EV) if ncore else None
+    full = ek1 if ecore is None else np.vstack([np.repeat(ecore[:, None], ek1.shape[1], 1), ek1, ek2])
+    fr = synth.tetra_frhis(full, ek2)
+    g = h.tetwt(m, ek1, ek2, ef, fr, npm=npm, ecore=ecore, matrix_linear=True)
+    o = orc.tetwt(m, ek1, ek2, ef, fr, npm=npm, ecore=ecore, matrix_linear=True)
+    p = orc.tetwt(m, ek1, ek2, ef, fr, npm=npm, ecore=ecore)
+    assert g["nbnbx"] == o["nbnbx"] == p["nbnbx"] and g["nhwtot"] == o["nhwtot"] == p["nhwtot"]
+    for key in ("nbnb", "n1b", "n2b", "ihw", "nhw", "jhw"):
+        np.testing.assert_array_equal(g[key], o[key], err_msg=key)
+        np.testing.assert_array_equal(o[key], p[key], err_msg=key)
+    top = np.abs(o["whw"]).max()
+    assert np.abs(g["whw"] - o["whw"]).max() <= 1e-8 * top
+    assert abs(g["whw"].sum() - o["whw"].sum()) <= 1e-12 * abs(o["whw"].sum())
+    nbnb = o["nbnb"].reshape(len(o["nbnb"]), -1)
+    jhw = o["jhw"].reshape(o["jhw"].shape[0], o["jhw"].shape[1], -1)
+    nhw = o["nhw"].reshape(jhw.shape)
+    for j in range(nbnb.shape[1]):
+        for k in range(nbnb.shape[0]):
+            for i in range(nbnb[k, j]):
+                j0, nh = jhw[i, k, j] - 1, nhw[i, k, j]
+                assert abs(g["whw"][j0:j0 + nh].sum() - o["whw"][j0:j0 + nh].sum()) <= 1e-8 * top
+    # the split differs from the plain weights (it is not a no-op)
+    assert np.abs(o["whw"] - p["whw"]).max() > 1e-3 * top
 
 
 def test_tetwt_band_cutoff(h, orc):

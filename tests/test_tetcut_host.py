@@ -2,7 +2,8 @@
 (ecalj_amd/csrc/tetcut.hpp, compiled here for the host with g++) against the
 oracle's restatement of lindtet6 (pinned by Monte Carlo in
 test_tetwt_oracle.py): every cut figure, equal corner energies, corners on
-E_F, and every (tetrahedron, pair) of the meshes of the GPU test.
+E_F, and every (tetrahedron, pair) of the meshes of the GPU test; the same
+with tetrahedron_matrix_linear's per-corner shares.
 
 Bars.  Against the same pieces evaluated in 50-digit arithmetic (mpmath),
 every bin within 1e-13 of the tetrahedron's volume.  Against the oracle,
@@ -25,26 +26,40 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.fixture(scope="module")
-def cut(tmp_path_factory):
+def _cutlib(tmp_path_factory):
     so = str(tmp_path_factory.mktemp("tetcut") / "libtetcut.so")
     subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-shared", "-fPIC",
                     "-I", os.path.join(ROOT, "ecalj_amd", "csrc"),
                     os.path.join(ROOT, "tests", "host", "tetcut_host.cpp"), "-o", so], check=True)
     lib = C.CDLL(so)
     dp = C.POINTER(C.c_double)
-    lib.tetcut_pair_histogram.argtypes = [dp, dp, dp, C.c_double, dp, C.c_int, dp]
-    lib.tetcut_pair_histogram.restype = C.c_int
+    for f in (lib.tetcut_pair_histogram, lib.tetcut_pair_histogram_ml):
+        f.argtypes = [dp, dp, dp, C.c_double, dp, C.c_int, dp]
+        f.restype = C.c_int
 
-    def run(kvec, ea, eb, ef, fr):
-        k = np.ascontiguousarray(kvec, np.float64)
-        ea, eb, fr = (np.ascontiguousarray(a, np.float64) for a in (ea, eb, fr))
-        out = np.zeros(len(fr) - 1)
-        rc = lib.tetcut_pair_histogram(k.ctypes.data_as(dp), ea.ctypes.data_as(dp), eb.ctypes.data_as(dp), ef,
-                                       fr.ctypes.data_as(dp), len(fr) - 1, out.ctypes.data_as(dp))
-        if rc:
-            raise ValueError(rc)
-        return out
-    return run
+    def run(f, ncol):
+        def call(kvec, ea, eb, ef, fr):
+            k = np.ascontiguousarray(kvec, np.float64)
+            ea, eb, fr = (np.ascontiguousarray(a, np.float64) for a in (ea, eb, fr))
+            out = np.zeros((len(fr) - 1, ncol))
+            rc = f(k.ctypes.data_as(dp), ea.ctypes.data_as(dp), eb.ctypes.data_as(dp), ef, fr.ctypes.data_as(dp),
+                   len(fr) - 1, out.ctypes.data_as(dp))
+            if rc:
+                raise ValueError(rc)
+            return out[:, 0] if ncol == 1 else out
+        return call
+    return run(lib.tetcut_pair_histogram, 1), run(lib.tetcut_pair_histogram_ml, 4)
+
+
+@pytest.fixture(scope="module")
+def cut(_cutlib):
+    return _cutlib[0]
+
+
+@pytest.fixture(scope="module")
+def cut_ml(_cutlib):
+    """tetrahedron_matrix_linear: nwhis x 4, column c carried by corner c."""
+    return _cutlib[1]
 
 
 def _vol(kvec):
@@ -216,5 +231,58 @@ def test_mesh_pairs(cut, orc, n, nocc, nun, seed, degenerate):
                 if eb.max() < ef or (eb - ea).max() < 0:
                     continue
                 _same(cut(kvec, ea, eb, ef, fr), _oracle(orc, kvec, ea, eb, ef, fr), kvec)
+                count += 1
+    assert count > 100
+
+
+def _same_ml(a, b, kvec):
+    """matrix_linear: the bins carry 4 W, so the bars are 4 x those of the
+    plain histogram; a corner share moves with its bin's weight."""
+    assert np.abs(a - b).max() <= 2e-7 * _vol(kvec)
+    assert abs(a.sum() - b.sum()) <= 4e-12 * _vol(kvec)
+
+
+def test_matrix_linear_every_figure_and_ties(cut, cut_ml, orc):
+    """Per-corner shares against the oracle (which finds each piece corner's
+    coordinates by inverting the tetrahedron's k-vectors, as the reference
+    does): random figures, equal corner energies, corners on E_F; and the
+    corner sum is 4 x the plain histogram of the same code."""
+    rng = np.random.default_rng(7)
+    lv = np.array([-0.4, -0.1, 0.0, 0.2])
+    cases = list(_figures(np.random.default_rng(8), 1500))
+    for _ in range(800):
+        kvec = rng.uniform(-1, 1, (4, 3))
+        ea, eb = rng.choice(lv, 4), rng.choice(lv + 0.3, 4)
+        if rng.random() < 0.3:
+            eb = ea + 0.25
+        cases.append((kvec, ea, eb))
+    for kvec, ea, eb in cases:
+        got = cut_ml(kvec, ea, eb, 0.0, FR)
+        ref = orc.lindtet6_ml(kvec, ea, eb, 0.5 * (np.asarray(ea) - np.asarray(eb)), 0.0, 0.0, FR)
+        _same_ml(got, ref, kvec)
+        np.testing.assert_allclose(got.sum(axis=1), 4.0 * cut(kvec, ea, eb, 0.0, FR), rtol=1e-13,
+                                   atol=1e-15 * _vol(kvec))
+
+
+def test_matrix_linear_mesh_pairs(cut_ml, orc):
+    m = synth.tetra_mesh(3, 2, 4)
+    ek1, ek2, ef = synth.tetra_bands(m, 5, 12, seed=4, degenerate=True)
+    fr = synth.tetra_frhis(ek1, ek2)
+    nb = ek1.shape[0]
+    count = 0
+    for t in range(m.idtetf.shape[1]):
+        w = m.idtetf[:, t] - 1
+        kk = m.ib1bz[w] - 1
+        kvec = m.qbzw[:, w].T
+        for ib in range(nb):
+            ea = ek1[ib, kk]
+            if ea.min() > ef:
+                continue
+            for jb in range(nb):
+                eb = ek2[jb, kk]
+                if eb.max() < ef or (eb - ea).max() < 0:
+                    continue
+                ref = orc.lindtet6_ml(kvec, ea, eb, 0.5 * (ea - eb), ef, ef, fr)
+                _same_ml(cut_ml(kvec, ea, eb, ef, fr), ref, kvec)
                 count += 1
     assert count > 100

@@ -280,3 +280,156 @@ def test_tetwt_core_is_a_flat_band(orc, npm):
     assert both
     for k, b in both:
         np.testing.assert_array_equal(seg[(k, nband + 1, b)][1], seg[(k, nband + 2, b)][1])
+
+
+# ---------------------------------------------------------------- tetrahedron_matrix_linear
+# With matrix_linear each bin's weight (times 4) is split over the four main
+# corners as the barycentric coordinates of one point per integration
+# tetrahedron: the vertex average of its cross-section polygon at the bin
+# centre (mkwcg), clamped to the lowest / highest corner outside its range.
+
+def _ml_tet(rng, no, nu, ef=0.0):
+    kv = rng.normal(size=(4, 3))
+    ea = rng.permutation(np.concatenate([rng.uniform(-1.0, -0.05, no), rng.uniform(0.05, 0.7, 4 - no)]))
+    eb = rng.permutation(np.concatenate([rng.uniform(0.05, 1.0, nu), rng.uniform(-0.7, -0.05, 4 - nu)]))
+    return kv, ea + ef, eb + ef
+
+
+def _all_figures(seed, reps=3):
+    rng = np.random.default_rng(seed)
+    for t in range(25 * reps):
+        yield (t,) + _ml_tet(rng, t % 5, (t // 5) % 5)
+
+
+def test_lindtet6_ml_corner_sum_is_four_times_plain(orc):
+    fr = orc.frhis(1.6)[1]
+    for t, kv, ea, eb in _all_figures(11):
+        x = 0.5 * (ea - eb)
+        plain = orc.lindtet6(kv, ea, eb, x, 0.0, 0.0, fr)
+        ml = orc.lindtet6_ml(kv, ea, eb, x, 0.0, 0.0, fr)
+        assert ml.shape == (len(fr) - 1, 4)
+        np.testing.assert_allclose(ml.sum(axis=1), 4.0 * plain, rtol=1e-12, atol=1e-15 * plain.sum(), err_msg=t)
+
+
+def _cross_section_average(w, ec):
+    """Barycentric coordinates of the vertex average of {w = ec} over a
+    tetrahedron (w at its corners): brute force over the six edges."""
+    lo, hi = int(np.argmin(w)), int(np.argmax(w))
+    if ec <= w[lo]:
+        return np.eye(4)[lo]
+    if ec >= w[hi]:
+        return np.eye(4)[hi]
+    pts = []
+    for i in range(4):
+        for j in range(i + 1, 4):
+            if (w[i] - ec) * (w[j] - ec) < 0.0:
+                t = (ec - w[i]) / (w[j] - w[i])
+                p = np.zeros(4)
+                p[i], p[j] = 1.0 - t, t
+                pts.append(p)
+    assert len(pts) in (3, 4)
+    return np.mean(pts, axis=0)
+
+
+def test_lindtet6_ml_uncut_tetrahedron_is_cross_section_average(orc):
+    """ea < E_F < eb at every corner: one integration tetrahedron, the main
+    one, so corner c of bin i carries 4 W_i times the c-th coordinate of the
+    cross-section's vertex average at the bin centre."""
+    rng = np.random.default_rng(12)
+    fr = np.linspace(0.0, 1.5, 61)
+    for t in range(40):
+        kv = rng.normal(size=(4, 3))
+        ea = rng.uniform(-1.0, -0.05, 4)
+        eb = rng.uniform(0.05, 1.0, 4)
+        x = 0.5 * (ea - eb)
+        w = -x
+        plain = orc.lindtet6(kv, ea, eb, x, 0.0, 0.0, fr)
+        ml = orc.lindtet6_ml(kv, ea, eb, x, 0.0, 0.0, fr)
+        gap = np.diff(np.sort(w)).min()
+        for i in np.nonzero(plain)[0]:
+            lam = _cross_section_average(w, 0.5 * (fr[i] + fr[i + 1]))
+            # the reference's corner offsets (up to 3e-8) move a coordinate by <= 3e-8 / (closest corner gap)
+            np.testing.assert_allclose(ml[i], 4.0 * plain[i] * lam, rtol=0.0,
+                                       atol=4.0 * plain[i] * 6e-8 / gap + 1e-14, err_msg=(t, i))
+
+
+def test_lindtet6_ml_points_sit_on_the_bin_centre_inside_the_region(orc):
+    """Every cut figure: the point that carries a bin lies in the tetrahedron
+    (coordinates >= 0), in the occupied / unoccupied region (ea <= E_F <= eb
+    there, both linear), and on the excitation plane w = bin centre, up to
+    half a bin where a piece's range starts or ends inside the bin."""
+    fr = np.linspace(0.0, 1.2, 2401)
+    h = fr[1] - fr[0]
+    for t, kv, ea, eb in _all_figures(13):
+        x = 0.5 * (ea - eb)
+        plain = orc.lindtet6(kv, ea, eb, x, 0.0, 0.0, fr)
+        ml = orc.lindtet6_ml(kv, ea, eb, x, 0.0, 0.0, fr)
+        tot = plain.sum()
+        if tot == 0.0:
+            assert not ml.any()
+            continue
+        m = 4.0 * plain
+        assert ml.min() >= -1e-12 * tot, t
+        ec = 0.5 * (fr[1:] + fr[:-1])
+        assert np.all(ml @ (-x) - m * ec <= m * (0.5 * h + 1e-7) + 1e-14 * tot), t
+        assert np.all(ml @ (-x) - m * ec >= -m * (0.5 * h + 1e-7) - 1e-14 * tot), t
+        assert np.all(ml @ ea <= 1e-9 * m + 1e-14 * tot), t
+        assert np.all(ml @ eb >= -1e-9 * m - 1e-14 * tot), t
+
+
+def test_lindtet6_ml_affine_invariant_and_corner_covariant(orc):
+    """The corner weights depend on k only through barycentric coordinates:
+    unchanged under k -> A k + b; relabelling the corners relabels the
+    columns (the integration tetrahedra follow the energies, not the labels)."""
+    fr = orc.frhis(1.6)[1]
+    rng = np.random.default_rng(14)
+    for t, kv, ea, eb in _all_figures(15, reps=2):
+        x = 0.5 * (ea - eb)
+        ml = orc.lindtet6_ml(kv, ea, eb, x, 0.0, 0.0, fr)
+        scale = np.abs(ml).max() + 1e-300
+        a = rng.normal(size=(3, 3)) + 2.0 * np.eye(3)
+        kv2 = kv @ a.T + rng.normal(size=3)
+        det = abs(np.linalg.det(a))
+        np.testing.assert_allclose(orc.lindtet6_ml(kv2, ea, eb, x, 0.0, 0.0, fr), det * ml,
+                                   atol=1e-10 * det * scale, err_msg=t)
+        p = rng.permutation(4)
+        got = orc.lindtet6_ml(kv[p], ea[p], eb[p], x[p], 0.0, 0.0, fr)
+        np.testing.assert_allclose(got, ml[:, p], atol=1e-10 * scale, err_msg=t)
+
+
+def test_tetwt_ml_same_lists_and_bin_totals(orc):
+    """tetrahedron_matrix_linear changes only whw: the same pair lists,
+    windows and pointers, and per histogram bin the same total over all
+    (k, pair) (each tetrahedron's 4 x W splits over its corners)."""
+    m, ek1, ek2, ef, fr = _case(n=(3, 2, 2), seed=7)
+    for npm in (1, 2):
+        r0 = orc.tetwt(m, ek1, ek2, ef, fr, npm=npm)
+        r1 = orc.tetwt(m, ek1, ek2, ef, fr, npm=npm, matrix_linear=True)
+        for key in ("nbnb", "n1b", "n2b", "ihw", "nhw", "jhw"):
+            np.testing.assert_array_equal(r1[key], r0[key], err_msg=key)
+        nw = len(fr) - 1
+
+        def bins(r):
+            out = np.zeros(nw)
+            seg = _segments(r, None) if npm == 1 else {**{(0,) + k: v for k, v in _segments(r, 0).items()},
+                                                        **{(1,) + k: v for k, v in _segments(r, 1).items()}}
+            for ih, w in seg.values():
+                out[ih - 1:ih - 1 + len(w)] += w
+            return out
+
+        b0, b1 = bins(r0), bins(r1)
+        np.testing.assert_allclose(b1, b0, rtol=1e-12, atol=1e-14 * b0.sum())
+        assert not np.allclose(r1["whw"], r0["whw"], rtol=1e-3)   # the split is not the plain weight
+
+
+def test_tetwt_ml_degenerate_bands_equal(orc):
+    m = synth.tetra_mesh(2, 2, 2)
+    ek1, ek2, ef = synth.tetra_bands(m, 4, 5, seed=3)
+    ek1[2] = ek1[1]                 # a degenerate occupied pair at every k
+    fr = synth.tetra_frhis(ek1, ek2)
+    r = orc.tetwt(m, ek1, ek2, ef, fr, matrix_linear=True)
+    seg = _segments(r)
+    pairs = [(k, b) for (k, a, b) in seg if a == 2]
+    assert pairs
+    for k, b in pairs:
+        np.testing.assert_array_equal(seg[(k, 2, b)][1], seg[(k, 3, b)][1])
