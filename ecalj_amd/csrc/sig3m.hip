@@ -45,68 +45,91 @@ struct alignas(16) Sig3Smem {
 // A[b][(((v*NS + ks)*6 + p)*MP + I)*32 + slot] from W (complex128, ldw, per-batch stride w_stride),
 // keepwv convention Wt(I,J) = (I <= J) ? W(I,J) : conj(W(J,I)); zero beyond n.
 constexpr int S3T = 32;
-constexpr int S3KS = 4;  // k slices per block
+// One 32 x 32 tile pair per block: tile (a, b), a <= b, of the upper
+// triangle of W is read once and gives two output tiles, (row tile a, k
+// slice b) from W itself and (row tile b, k slice a) from its conjugate
+// transpose, so every element of W is read from HBM once per frequency
+// (half of reading both triangles) and each output tile written once.
 __global__ __launch_bounds__(256) void split_w6_kernel(const cd* __restrict__ w, int64_t ldw, int64_t w_stride, int n,
                                                        int nw1, int MP, int NS, unsigned short* __restrict__ as,
                                                        int64_t as_stride) {
   __shared__ cd up[S3T][S3T + 1];  // up[jj][ii] = W(I0+ii, J0+jj)
-  __shared__ cd lo[S3T][S3T + 1];  // lo[ii][jj] = W(J0+jj, I0+ii)
-  const int I0 = blockIdx.y * S3T;
-  const int v = blockIdx.z % nw1, b = blockIdx.z / nw1;
-  const cd* W = w + w_stride * b + (int64_t)ldw * n * v;
+  const int64_t tri = blockIdx.x;
+  int b = (int)((sqrt(8.0 * (double)tri + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(b + 1) * (b + 2) / 2 <= tri) ++b;
+  while ((int64_t)b * (b + 1) / 2 > tri) --b;
+  const int a = (int)(tri - (int64_t)b * (b + 1) / 2);
+  const int I0 = a * S3T, J0 = b * S3T;
+  const int v = blockIdx.y % nw1, bk = blockIdx.y / nw1;
+  const cd* W = w + w_stride * bk + (int64_t)ldw * n * v;
   const int tx = threadIdx.x % S3T, ty = threadIdx.x / S3T;
+  for (int r = ty; r < S3T; r += 8) {
+    const int I = I0 + tx, J = J0 + r;
+    up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
+  }
+  __syncthreads();
   typedef unsigned short us4v __attribute__((ext_vector_type(4)));
-  for (int kq = 0; kq < S3KS; ++kq) {
-    const int ks = blockIdx.x * S3KS + kq;
-    if (ks >= NS) break;
-    const int J0 = ks * S3T;
-    if (kq) __syncthreads();
-    const bool need_up = I0 <= J0 + S3T - 1, need_lo = I0 + S3T - 1 > J0;
-    for (int r = ty; r < S3T; r += 8) {
-      if (need_up) {
-        const int I = I0 + tx, J = J0 + r;
-        up[r][tx] = (I < n && J < n) ? W[I + ldw * J] : cd{0.0, 0.0};
-      }
-      if (need_lo) {
-        const int J2 = J0 + tx, I2 = I0 + r;
-        lo[r][tx] = (I2 < n && J2 < n) ? W[J2 + ldw * I2] : cd{0.0, 0.0};
-      }
-    }
-    __syncthreads();
-    // thread: row r, 4 k values (half hf of chunk ch)
-    const int r = threadIdx.x >> 3, ch = (threadIdx.x >> 1) & 3, hf = threadIdx.x & 1;
-    const int I = I0 + r;
+  // thread: row r of the output tile, 4 k values (half hf of chunk ch)
+  const int r = threadIdx.x >> 3, ch = (threadIdx.x >> 1) & 3, hf = threadIdx.x & 1;
+  const int64_t pst = (int64_t)MP * QBK;
+  unsigned short* Ab = as + as_stride * bk + (int64_t)v * NS * Q3PL * pst;
+  auto put = [&](int I, int ks, const double (&xr)[4], const double (&xi)[4]) {
     us4v p[Q3PL];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int jj = 8 * ch + 4 * hf + e, J = J0 + jj;
-      double xr = 0.0, xi = 0.0;
-      if (I < n && J < n) {
-        if (I <= J) {
-          xr = up[jj][r].re;
-          xi = up[jj][r].im;
-        } else {
-          xr = lo[r][jj].re;
-          xi = -lo[r][jj].im;
-        }
-      }
       unsigned short h, l;
-      split_bf16((float)xr, h, l);
+      split_bf16((float)xr[e], h, l);
       p[0][e] = h;
       p[1][e] = l;
-      split_bf16((float)xi, h, l);
+      split_bf16((float)xi[e], h, l);
       p[2][e] = h;
       p[3][e] = l;
-      split_bf16((float)(xr + xi), h, l);
+      split_bf16((float)(xr[e] + xi[e]), h, l);
       p[4][e] = h;
       p[5][e] = l;
     }
-    const int64_t pst = (int64_t)MP * QBK;
-    unsigned short* A = as + as_stride * b + ((int64_t)v * NS + ks) * Q3PL * pst + (int64_t)I * QBK +
-                        8 * chunk_slot(I, ch) + 4 * hf;
+    unsigned short* A = Ab + (int64_t)ks * Q3PL * pst + (int64_t)I * QBK + 8 * chunk_slot(I, ch) + 4 * hf;
 #pragma unroll
     for (int q = 0; q < Q3PL; ++q) *reinterpret_cast<us4v*>(A + q * pst) = p[q];
+  };
+  double xr[4], xi[4];
+  // (row tile a, k slice b): Wt(I, J) = W(I, J) for I <= J, conj(W(J, I)) below the diagonal
+  if (a < MP / S3T && b < NS) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int jj = 8 * ch + 4 * hf + e;
+      if (a == b && r > jj) {
+        xr[e] = up[r][jj].re;
+        xi[e] = -up[r][jj].im;
+      } else {
+        xr[e] = up[jj][r].re;
+        xi[e] = up[jj][r].im;
+      }
+    }
+    put(I0 + r, b, xr, xi);
   }
+  // (row tile b, k slice a), a < b: every element below the diagonal, conj(W(K, I))
+  if (a < b && b < MP / S3T && a < NS) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int jj = 8 * ch + 4 * hf + e;
+      xr[e] = up[r][jj].re;
+      xi[e] = -up[r][jj].im;
+    }
+    put(J0 + r, a, xr, xi);
+  }
+}
+
+// Launch of split_w6_kernel over nb k-points: the tile pairs of the larger
+// of the row-tile and k-slice counts.
+static ecalj_status split_w6_launch(ecalj_ctx* h, const cd* wc, int64_t ldw, int64_t w_stride, int n, int nw1, int nb,
+                                    int MP, int NS, unsigned short* as, int64_t as_stride) {
+  const int64_t T = std::max(MP / S3T, NS), npair = T * (T + 1) / 2;
+  if (npair > 0x7fffffff || (int64_t)nw1 * nb > 65535) return ECALJ_ERR_DIM;
+  hipLaunchKernelGGL(split_w6_kernel, dim3((unsigned)npair, nw1 * nb), dim3(256), 0, h->stream, wc, ldw, w_stride, n,
+                     nw1, MP, NS, as, as_stride);
+  ECALJ_HIP(hipGetLastError());
+  return ECALJ_OK;
 }
 
 // zp[b][c*NP + J] = zmel_b(J, c) (complex64), zero for J >= n or c >= ncol; c < NCP, J < NP
@@ -917,10 +940,7 @@ ecalj_status sig3m_prep(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, 
                         const cd* wc, int64_t ldw, int64_t w_stride, unsigned short* as, cf* zp) {
   const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NCP = pad_to(ncol, QBN_MAX), NS = NP / QBK;
   const int64_t as_stride = (int64_t)sig3m_a_elems_per_k(n, nw1);
-  dim3 gw((NS + S3KS - 1) / S3KS, MP / S3T, nw1 * nb);
-  if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
-  hipLaunchKernelGGL(split_w6_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
-  ECALJ_HIP(hipGetLastError());
+  ECALJ_TRY(split_w6_launch(h, wc, ldw, w_stride, n, nw1, nb, MP, NS, as, as_stride));
   const int64_t totz = (int64_t)NCP * NP * nb;
   const int bz = (int)std::min<int64_t>((totz + 255) / 256, 65536);
   hipLaunchKernelGGL(pad_z_kernel, dim3(bz), dim3(256), 0, h->stream, zmel, z_stride, n, ncol, NP, NCP, zp,
@@ -1013,10 +1033,7 @@ ecalj_status sig3m_grouped_prep(ecalj_ctx* h, int nk, const int* ncol, const cf*
                                 int64_t ldw, int64_t w_stride, unsigned short* as, cf* zp) {
   const int MP = pad_to(n, QBM), NP = pad_to(n, QBK), NS = NP / QBK;
   const int64_t as_stride = (int64_t)sig3m_a_elems_per_k(n, nw1);
-  dim3 gw((NS + S3KS - 1) / S3KS, MP / S3T, nw1 * nk);
-  if (gw.z > 65535 || gw.y > 65535) return ECALJ_ERR_DIM;
-  hipLaunchKernelGGL(split_w6_kernel, gw, dim3(256), 0, h->stream, wc, ldw, w_stride, n, nw1, MP, NS, as, as_stride);
-  ECALJ_HIP(hipGetLastError());
+  ECALJ_TRY(split_w6_launch(h, wc, ldw, w_stride, n, nw1, nk, MP, NS, as, as_stride));
   int64_t zo = 0, co = 0;
   for (int k = 0; k < nk; ++k) {
     const int ncp = pad_to(std::max(ncol[k], 1), QBN_MAX);
