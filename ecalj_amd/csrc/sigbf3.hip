@@ -349,8 +349,16 @@ ecalj_status sig_bf3(ecalj_ctx* h, int nb, const cf* zmel, int64_t z_stride, int
     // fabric, runs faster than a variant that beat it by a few percent once
     // (C2: 22.6 against 23.2 ms per launch over 8 q-point steps), so it is
     // kept when within 3% of the fastest single launch.
-    for (int v = 0; v < kNumSigVariants; ++v)
-      if (kSigVariants[v].xcd && ms[v] <= 1.03 * ms[var]) var = v;
+    // (measured against the fastest launch itself, and the fastest XCD-aware
+    // variant of those within 3%, so that the choice cannot drift along a
+    // chain of variants each within 3% of the previous one)
+    if (!kSigVariants[var].xcd) {
+      const double lim = 1.03 * ms[var];
+      int pick = -1;
+      for (int v = 0; v < kNumSigVariants; ++v)
+        if (kSigVariants[v].xcd && ms[v] <= lim && (pick < 0 || ms[v] < ms[pick])) pick = v;
+      if (pick >= 0) var = pick;
+    }
     std::vector<const char*> names;
     for (int v = 0; v < kNumSigVariants; ++v) names.push_back(kSigVariants[v].name);
     tune_store(h, key, var, "sigc_bf3", names, ms);
