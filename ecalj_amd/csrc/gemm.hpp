@@ -476,6 +476,11 @@ struct GemmArgs {
   int epi;
   const double* svec;  // EPI_EPS: eps = delta - alpha * s_i * acc * s_j
   int gj_k0, gj_k1;    // EPI_GJ: rows [k0,k1) skipped; cols [k0,k1) start from 0
+  // XCD order (xt > 0): a 1-D grid whose block p the dispatcher deals to XCD
+  // p % 8; XCD x's j-th block takes tile j % xt (xg row tiles per column
+  // tile) of matrix (j / xt) * 8 + x, so every tile of one matrix runs on one
+  // XCD and its A and B panels are fetched into one L2 instead of all eight
+  int xg, xt;
 };
 
 template <int MODE, class T, int BM, int BN, bool AT, bool AREAL, bool BT>
@@ -484,9 +489,20 @@ __global__ __launch_bounds__(kThreads) void zgemm_kernel(GemmArgs g) {
   typedef typename StageOf<MODE, BM, BN, BK>::type Stage;
   typedef typename ElemT<T>::V V;
   __shared__ Stage s;
-  const int b = blockIdx.z;
-  const int m0 = blockIdx.x * BM;
-  const int n0 = blockIdx.y * BN;
+  int b, mt, nt;
+  if (g.xt == 0) {
+    b = blockIdx.z;
+    mt = blockIdx.x;
+    nt = blockIdx.y;
+  } else {
+    const int p = blockIdx.x, j = p >> 3, t = j % g.xt;
+    b = (j / g.xt) * 8 + (p & 7);
+    if (b >= g.batch) return;
+    mt = t % g.xg;
+    nt = t / g.xg;
+  }
+  const int m0 = mt * BM;
+  const int n0 = nt * BN;
   const char* Ab = static_cast<const char*>(g.A) + g.bA.off(b, g.div) * (AREAL ? 8 : (int)sizeof(T));
   const T* Bb = static_cast<const T*>(g.B) + g.bB.off(b, g.div);
   T* Cb = static_cast<T*>(g.C) + g.bC.off(b, g.div);

@@ -16,8 +16,21 @@ namespace ecalj {
 // Generic GEMM dispatcher.  Complex operands are complex128 in mode M64 and
 // complex64 in M32 / MBF3.
 template <int MODE, class T, int BM, int BN>
-static ecalj_status launch_gemm(ecalj_ctx* h, const GemmArgs& g, bool at, bool areal, bool bt) {
+static ecalj_status launch_gemm(ecalj_ctx* h, const GemmArgs& g0, bool at, bool areal, bool bt) {
+  GemmArgs g = g0;
   dim3 grid((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, g.batch);
+  g.xg = g.xt = 0;
+  // Many matrices: the XCD order (GemmArgs), when the matrices deal out
+  // evenly over the 8 XCDs (a multiple of 8, or enough that the last round
+  // is a small share)
+  if (g.batch >= 8 && (g.batch % 8 == 0 || g.batch >= 128)) {
+    const int64_t xt = (int64_t)grid.x * grid.y, p = 8 * xt * ((g.batch + 7) / 8);
+    if (p <= 0x7fffffff) {
+      g.xg = (int)grid.x;
+      g.xt = (int)xt;
+      grid = dim3((unsigned)p, 1, 1);
+    }
+  }
   if (grid.y > 65535 || grid.z > 65535) return ECALJ_ERR_DIM;
   const int key = (at ? 4 : 0) | (areal ? 2 : 0) | (bt ? 1 : 0);
   switch (key) {
