@@ -141,7 +141,7 @@ template <int BM, int BN, int BK> struct StageOf<M64, BM, BN, BK> { typedef Stag
 template <int BM, int BN, int BK> struct StageOf<M32, BM, BN, BK> { typedef StagePlanar<BM, BN, BK, float> type; };
 template <int BM, int BN, int BK> struct StageOf<MBF3, BM, BN, BK> { typedef StageBF3<BM, BN, BK> type; };
 
-template <int MODE> struct ModeBK { static constexpr int value = (MODE == MBF3) ? 32 : 16; };
+template <int MODE> struct ModeBK { static constexpr int value = (MODE == MBF3 || MODE == M64) ? 32 : 16; };
 
 // --------------------------------------------------------------------------
 // Operand loaders.  An accessor gives element (x, k) of an operand -- x is
